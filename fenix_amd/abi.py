@@ -1,0 +1,227 @@
+"""ctypes mirror of include/fenix.h and the loader of libfenix.so.
+
+The structs below are field-for-field copies of the C declarations; every
+function prototype is declared once in `FUNCTIONS` so the ABI test can check
+that the shared library exports each symbol the header declares.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfenix.so")
+
+# ---- constants (include/fenix.h) -------------------------------------------
+FX_OK = 0
+FX_ERR_INVALID_ARG = -1
+FX_ERR_OUT_OF_MEMORY = -2
+FX_ERR_DEVICE = -3
+FX_ERR_UNSUPPORTED = -4
+FX_ERR_STATE = -5
+
+FX_ADVECT_MACCORMACK = 0
+FX_ADVECT_SEMI_LAGRANGIAN = 1
+FX_DOMAIN_DENSE = 0
+FX_DOMAIN_SPARSE = 1
+FX_ACTIVATION_REFERENCE = 0
+FX_ACTIVATION_THRESHOLD = 1
+FX_SOLVER_REFERENCE = 0
+FX_SOLVER_VCYCLE = 1
+FX_PRECISION_FP32 = 0
+FX_PRECISION_BF16 = 1
+FX_PRECISION_FP16 = 2
+
+FIELDS = {
+    "density": 0, "temperature": 1, "fuel": 2, "reaction": 3,
+    "velocity_x": 4, "velocity_y": 5, "velocity_z": 6,
+    "scratch_x": 7, "scratch_y": 8, "scratch_z": 9,
+    "vorticity_x": 10, "vorticity_y": 11, "vorticity_z": 12,
+    "vorticity_magnitude": 13, "confinement_magnitude": 14, "velocity_magnitude": 15,
+    "divergence": 16, "pressure": 17,
+    "adv_density": 18, "adv_temperature": 19, "adv_fuel": 20, "adv_reaction": 21,
+}
+FX_FIELD_COUNT = 22
+
+FX_LEVEL_DIVERGENCE = 0
+FX_LEVEL_PRESSURE_A = 1
+FX_LEVEL_PRESSURE_B = 2
+FX_FLAGS_RENDER = 0
+FX_FLAGS_INJECTED = 1
+
+PASSES = {
+    "inject_scalar": 1, "inject_velocity": 2, "inject_igniter": 3,
+    "activate_render": 4, "activate_advected": 5,
+    "advect_velocity": 6, "advect_scalars": 7, "combustion": 8, "buoyancy": 9,
+    "vorticity": 10, "confinement": 11, "divergence": 12, "pressure": 13,
+    "projection": 14, "debug_fields": 15,
+}
+
+FX_DISPLAY_TEMPERATURE = 0
+FX_DISPLAY_DENSITY = 1
+FX_DISPLAY_FUEL = 2
+FX_FLOATS_PER_EMITTER = 32
+FX_MAX_RENDER_LIGHTS = 3
+
+FX_RP_WIND = 1 << 0
+FX_RP_WIND_STRENGTH = 1 << 1
+FX_RP_GRAVITY = 1 << 2
+FX_RP_GRAVITY_STRENGTH = 1 << 3
+FX_RP_BUOYANCY = 1 << 4
+FX_RP_VORTICITY_STRENGTH = 1 << 5
+FX_RP_VORTICITY_CONSTANT_MASK = 1 << 6
+FX_RP_VORTICITY_VELOCITY_MASK = 1 << 7
+FX_RP_VORTICITY_HEAT_MASK = 1 << 8
+FX_RP_VORTICITY_DENSITY_MASK = 1 << 9
+FX_RP_WORLD_SIZE = 1 << 10
+FX_RP_ALL = 0x7FF
+
+
+# ---- structs ---------------------------------------------------------------
+class Quality(C.Structure):
+    _fields_ = [
+        ("pressure_interval", C.c_int32), ("vorticity_interval", C.c_int32),
+        ("fine_pre_iterations", C.c_int32), ("fine_post_iterations", C.c_int32),
+        ("mid_pre_iterations", C.c_int32), ("mid_post_iterations", C.c_int32),
+        ("coarse_iterations", C.c_int32),
+    ]
+
+
+class RuntimeParams(C.Structure):
+    _fields_ = [
+        ("wind", C.c_float * 3), ("wind_strength", C.c_float),
+        ("gravity", C.c_float * 3), ("gravity_strength", C.c_float),
+        ("buoyancy", C.c_float), ("vorticity_strength", C.c_float),
+        ("vorticity_constant_mask", C.c_float), ("vorticity_velocity_mask", C.c_float),
+        ("vorticity_heat_mask", C.c_float), ("vorticity_density_mask", C.c_float),
+        ("world_size", C.c_float),
+    ]
+
+
+class SimParams(C.Structure):
+    _fields_ = [
+        ("time", C.c_float), ("delta_time", C.c_float), ("previous_time", C.c_float),
+        ("scalar_set_index", C.c_float), ("wind", C.c_float * 4), ("gravity", C.c_float * 4),
+        ("buoyancy", C.c_float), ("vorticity_strength", C.c_float), ("dx", C.c_float),
+        ("world_size", C.c_float), ("vorticity_mask", C.c_float * 4),
+    ]
+
+
+class VCycle(C.Structure):
+    _fields_ = [
+        ("levels", C.c_int32), ("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32),
+        ("cycles", C.c_int32), ("coarse_sweeps", C.c_int32), ("omega", C.c_float),
+    ]
+
+
+class CreateDesc(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32), ("height", C.c_uint32), ("depth", C.c_uint32),
+        ("advection_mode", C.c_int32),
+        ("runtime", RuntimeParams),
+        ("quality", Quality),
+        ("domain_mode", C.c_int32), ("activation_rule", C.c_int32),
+        ("solver", C.c_int32), ("precision", C.c_int32),
+        ("vcycle", VCycle),
+        ("ignition_lo", C.c_float), ("ignition_hi", C.c_float), ("burn_rate", C.c_float),
+        ("activation_density", C.c_float), ("activation_speed", C.c_float),
+        ("debug_fields", C.c_int32), ("device", C.c_int32),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("position", C.c_float * 3), ("target", C.c_float * 3), ("forward", C.c_float * 3),
+        ("right", C.c_float * 3), ("up", C.c_float * 3), ("aspect", C.c_float),
+        ("tan_half_fov_y", C.c_float),
+    ]
+
+
+class Light(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32), ("direction", C.c_float * 3), ("position", C.c_float * 3),
+        ("color", C.c_float * 3), ("intensity", C.c_float),
+    ]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [
+        ("step_count", C.c_int32), ("light_count", C.c_int32),
+        ("lights", Light * FX_MAX_RENDER_LIGHTS),
+        ("scattering_forward", C.c_float), ("scattering_back", C.c_float),
+        ("display_mode", C.c_int32), ("elapsed_s", C.c_float),
+    ]
+
+
+P = C.c_void_p
+i32, u32 = C.c_int32, C.c_uint32
+FUNCTIONS = {
+    "fx_default_create_desc": (None, [C.POINTER(CreateDesc)]),
+    "fx_create": (C.c_int, [C.POINTER(CreateDesc), C.POINTER(P)]),
+    "fx_destroy": (None, [P]),
+    "fx_reset": (C.c_int, [P]),
+    "fx_step": (C.c_int, [P, C.c_double, C.c_double]),
+    "fx_synchronize": (C.c_int, [P]),
+    "fx_get_runtime_params": (C.c_int, [P, C.POINTER(RuntimeParams)]),
+    "fx_set_runtime_params": (C.c_int, [P, C.POINTER(RuntimeParams), u32]),
+    "fx_get_quality": (C.c_int, [P, C.POINTER(Quality)]),
+    "fx_set_quality": (C.c_int, [P, C.POINTER(Quality)]),
+    "fx_get_advection_mode": (C.c_int, [P, C.POINTER(i32)]),
+    "fx_set_advection_mode": (C.c_int, [P, i32]),
+    "fx_update_sources": (C.c_int, [P, C.POINTER(C.c_float), u32]),
+    "fx_read_field": (C.c_int, [P, i32, C.POINTER(C.c_float), C.c_size_t]),
+    "fx_write_field": (C.c_int, [P, i32, C.POINTER(C.c_float), C.c_size_t]),
+    "fx_read_flags": (C.c_int, [P, i32, C.POINTER(u32), C.c_size_t]),
+    "fx_read_active_list": (C.c_int, [P, C.POINTER(u32), C.c_size_t, C.POINTER(u32)]),
+    "fx_read_page_table": (C.c_int, [P, C.POINTER(i32), C.c_size_t]),
+    "fx_field_device_ptr": (C.c_void_p, [P, i32]),
+    "fx_read_level": (C.c_int, [P, i32, i32, C.POINTER(C.c_float), C.c_size_t]),
+    "fx_level_dims": (C.c_int, [P, i32, C.POINTER(u32)]),
+    "fx_solver_residuals": (C.c_int, [P, C.POINTER(C.c_double), C.c_size_t, C.POINTER(u32)]),
+    "fx_active_brick_count": (C.c_int, [P, C.POINTER(u32)]),
+    "fx_raymarch": (C.c_int, [P, C.POINTER(Camera), C.POINTER(RenderParams), u32, u32, C.POINTER(C.c_uint16)]),
+    "fx_raymarch_device": (C.c_int, [P, C.POINTER(Camera), C.POINTER(RenderParams), u32, u32, C.c_void_p]),
+    "fx_run_pass": (C.c_int, [P, i32, C.POINTER(SimParams)]),
+    "fx_last_error": (C.c_char_p, [P]),
+}
+
+_LIB = None
+
+
+class FenixError(RuntimeError):
+    """Raised when a libfenix call returns a non-zero status."""
+
+    def __init__(self, code, message):
+        super().__init__(f"libfenix error {code}: {message}")
+        self.code = code
+
+
+def load(path=None):
+    """Loads libfenix.so; raises if it has not been built (no fallback path exists)."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(
+            f"{p} is missing: build it with `python -m fenix_amd.build` (libfenix has no CPU fallback)")
+    lib = C.CDLL(p)
+    for name, (res, args) in FUNCTIONS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def check(lib, sim, rc):
+    if rc != FX_OK:
+        msg = lib.fx_last_error(sim)
+        raise FenixError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def default_desc(lib=None):
+    lib = lib or load()
+    d = CreateDesc()
+    lib.fx_default_create_desc(C.byref(d))
+    return d

@@ -1,0 +1,959 @@
+// fenix_api.cpp — host side of libfenix: the C ABI declared in include/fenix.h.
+//
+// Owns every device buffer of one simulation, records the reference's pass
+// sequence (src/engine/simulation/runtime/createCombustionVolumeSimulation.ts:230-305,
+// "ORCH" below) on the simulation's HIP stream and converts between the
+// reference's dense row-major layout and the brick-major device layout.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "fenix_internal.h"
+
+using namespace fx;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+constexpr int kStageSlots = 8;
+
+struct Level {
+  Grid g;
+  size_t nb = 0, nv = 0;
+  float* div = nullptr;
+  float* p[2] = {nullptr, nullptr};  // pressureA, pressureB
+  uint32_t* flags = nullptr;         // sparse: level activity flags
+  BrickList L{};                     // sparse: active list; dense: identity
+};
+
+struct StageSlot {
+  StepConst* c = nullptr;      // pinned
+  uint32_t* cand = nullptr;    // pinned, capacity nb
+  hipEvent_t done = nullptr;
+  bool used = false;
+};
+
+}  // namespace
+
+struct fx_sim {
+  fx_create_desc desc{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Grid g{};
+  size_t nb = 0, nv = 0;
+  int blocks = 1;          // persistent grid for brick kernels
+  bool sparse = false;
+
+  float* S[2][4] = {};
+  float* u[3] = {};
+  float* us[3] = {};
+  float* w[3] = {};
+  float* wmag = nullptr;
+  float* cmag = nullptr;
+  float* umag = nullptr;
+  uint32_t* flagsInj = nullptr;
+  uint32_t* flagsRender = nullptr;  // aliases flagsInj when sparse
+  uint32_t* touched = nullptr;
+  std::vector<Level> lv;
+  uint32_t* identity = nullptr;     // 0..nb-1
+  uint32_t* identity_count = nullptr;
+
+  StepConst* d_const = nullptr;     // device copy of the step constants
+  uint32_t* d_cand = nullptr;       // device candidate list (injection + host-written bricks)
+  float* d_src = nullptr;
+  size_t src_cap = 0;               // floats
+  std::vector<float> h_src;         // packed, >= 1 source
+  uint32_t nsrc = 0;
+  StageSlot slots[kStageSlots];
+  int slot_next = 0;
+  double* d_norms = nullptr;
+  int norm_cap = 0;
+  int norm_count = 0;
+
+  fx_runtime_params rt{};
+  fx_quality q{};
+  int mode = 0;
+  bool initialized = false;
+  double startS = 0, lastS = 0;
+  uint64_t stepIndex = 0;
+  int lastPressure = 0;
+  int curSet = 0;
+  std::vector<uint32_t> pending;    // sparse: bricks written by fx_write_field since the last step
+  std::string err;
+};
+
+namespace {
+
+int fail(fx_sim* s, int code, const std::string& msg) {
+  if (s) s->err = msg; else g_create_error = msg;
+  return code;
+}
+
+#define HIP_OK(sim, expr)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(sim, FX_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+int clamp_int(double v, int lo, int hi) {  // CVS/types.ts:26-28
+  double r = std::floor(v + 0.5);
+  return int(std::max(double(lo), std::min(double(hi), r)));
+}
+fx_quality clamp_quality(const fx_quality& q) {  // CVS/types.ts:30-44
+  fx_quality o;
+  o.pressure_interval = clamp_int(q.pressure_interval, 1, 16);
+  o.vorticity_interval = clamp_int(q.vorticity_interval, 1, 16);
+  o.fine_pre_iterations = clamp_int(q.fine_pre_iterations, 0, 32);
+  o.fine_post_iterations = clamp_int(q.fine_post_iterations, 0, 32);
+  o.mid_pre_iterations = clamp_int(q.mid_pre_iterations, 0, 32);
+  o.mid_post_iterations = clamp_int(q.mid_post_iterations, 0, 32);
+  o.coarse_iterations = clamp_int(q.coarse_iterations, 0, 64);
+  return o;
+}
+uint32_t halve(uint32_t n) { return std::max<uint32_t>(4u, n / 2u); }  // CVS/buffers.ts:10-16
+
+template <class T>
+int dalloc(fx_sim* s, T** p, size_t count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+  if (e != hipSuccess) return fail(s, FX_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+  *p = static_cast<T*>(q);
+  return FX_OK;
+}
+
+Scalars set_of(fx_sim* s, int k) { return Scalars{s->S[k][0], s->S[k][1], s->S[k][2], s->S[k][3]}; }
+Vec3 vec_of(float* const* v) { return Vec3{v[0], v[1], v[2]}; }
+
+int grid_for(const fx_sim* s, size_t bricks) { return int(std::max<size_t>(1, std::min<size_t>(size_t(s->blocks), bricks))); }
+
+// Fine-level brick list used by every stage.
+const uint32_t* fine_list(const fx_sim* s) { return s->sparse ? s->lv[0].L.list : s->identity; }
+const uint32_t* fine_count(const fx_sim* s) { return s->sparse ? s->lv[0].L.count : s->identity_count; }
+const uint32_t* level_list(const fx_sim* s, int l) { return s->lv[l].L.list; }
+const uint32_t* level_count(const fx_sim* s, int l) { return s->lv[l].L.count; }
+
+std::vector<float*> fine_buffers(fx_sim* s) {
+  std::vector<float*> b;
+  for (int k = 0; k < 2; ++k) for (int f = 0; f < 4; ++f) b.push_back(s->S[k][f]);
+  for (int c = 0; c < 3; ++c) { b.push_back(s->u[c]); b.push_back(s->us[c]); b.push_back(s->w[c]); }
+  b.push_back(s->wmag); b.push_back(s->cmag); b.push_back(s->umag);
+  b.push_back(s->lv[0].div); b.push_back(s->lv[0].p[0]); b.push_back(s->lv[0].p[1]);
+  return b;
+}
+
+// ORCH:236-266 — computed in double, stored as f32.
+fx_sim_params make_params(const fx_sim* s, double elapsed, double stepS) {
+  fx_sim_params p;
+  double eff = std::min(stepS, 1.0 / 15.0);
+  p.time = float(elapsed - s->startS);
+  p.delta_time = float(eff);
+  p.previous_time = float(s->lastS - s->startS);
+  p.scalar_set_index = float(s->curSet);
+  p.wind[0] = s->rt.wind[0]; p.wind[1] = s->rt.wind[1]; p.wind[2] = s->rt.wind[2]; p.wind[3] = s->rt.wind_strength;
+  p.gravity[0] = s->rt.gravity[0]; p.gravity[1] = s->rt.gravity[1]; p.gravity[2] = s->rt.gravity[2];
+  p.gravity[3] = s->rt.gravity_strength;
+  p.buoyancy = s->rt.buoyancy;
+  p.vorticity_strength = s->rt.vorticity_strength;
+  p.dx = float(double(s->rt.world_size) / double(std::max(s->g.W, s->g.D)));
+  p.world_size = s->rt.world_size;
+  p.vorticity_mask[0] = s->rt.vorticity_constant_mask; p.vorticity_mask[1] = s->rt.vorticity_velocity_mask;
+  p.vorticity_mask[2] = s->rt.vorticity_heat_mask; p.vorticity_mask[3] = s->rt.vorticity_density_mask;
+  return p;
+}
+
+// Sparse injection bricks: every brick overlapping the voxel bounding box (+1
+// voxel) of a gate-active source's influence sphere (burst: outer limit,
+// inject-source-common.wgsl.ts:70-77).
+void injection_bricks(const fx_sim* s, const fx_sim_params& p, std::vector<uint8_t>& mark, std::vector<uint32_t>& out) {
+  uint32_t ns = std::max<uint32_t>(s->nsrc, 1);
+  for (uint32_t i = 0; i < ns; ++i) {
+    const float* e = &s->h_src[size_t(i) * FX_FLOATS_PER_EMITTER];
+    float age = p.time - e[4];
+    if (!(age >= 0.f && e[5] >= age)) continue;
+    float r = e[3];
+    if (e[28] == 3.f) {
+      float vw = p.dx / std::max(p.world_size, 0.0001f);
+      r = r + std::max(r * 0.24f, vw * 3.f);
+    }
+    if (!(r > 0.f)) continue;
+    const uint32_t n[3] = {s->g.W, s->g.H, s->g.D};
+    int lo[3], hi[3];
+    bool empty = false;
+    for (int a = 0; a < 3; ++a) {
+      double l = std::floor((double(e[a]) - r) * n[a] - 0.5) - 1.0;
+      double h = std::ceil((double(e[a]) + r) * n[a] - 0.5) + 1.0;
+      l = std::max(l, 0.0); h = std::min(h, double(n[a]) - 1.0);
+      if (l > h) { empty = true; break; }
+      lo[a] = int(l) >> 3; hi[a] = int(h) >> 3;
+    }
+    if (empty) continue;
+    for (int z = lo[2]; z <= hi[2]; ++z)
+      for (int y = lo[1]; y <= hi[1]; ++y)
+        for (int x = lo[0]; x <= hi[0]; ++x) {
+          uint32_t b = uint32_t(x) + s->g.BX * (uint32_t(y) + s->g.BY * uint32_t(z));
+          if (!mark[b]) { mark[b] = 1; out.push_back(b); }
+        }
+  }
+}
+
+// Uploads the step constants (and, sparse, the candidate list) through a pinned
+// staging slot; the slot is reused only after its previous copy completed.
+int upload_step(fx_sim* s, const fx_sim_params& p, const std::vector<uint32_t>* cand, uint32_t inj_count) {
+  StageSlot& slot = s->slots[s->slot_next];
+  s->slot_next = (s->slot_next + 1) % kStageSlots;
+  if (slot.used) HIP_OK(s, hipEventSynchronize(slot.done));
+  StepConst& c = *slot.c;
+  c.p = p;
+  c.nsrc = std::max<uint32_t>(s->nsrc, 1);
+  c.ignition_lo = s->desc.ignition_lo; c.ignition_hi = s->desc.ignition_hi; c.burn_rate = s->desc.burn_rate;
+  c.act_density = s->desc.activation_density; c.act_speed = s->desc.activation_speed;
+  c.omega = s->desc.vcycle.omega;
+  c.inj_count = inj_count;
+  c.cand_count = cand ? uint32_t(cand->size()) : 0u;
+  c.pad[0] = c.pad[1] = 0;
+  HIP_OK(s, hipMemcpyAsync(s->d_const, slot.c, sizeof(StepConst), hipMemcpyHostToDevice, s->stream));
+  if (cand && !cand->empty()) {
+    std::memcpy(slot.cand, cand->data(), cand->size() * 4);
+    HIP_OK(s, hipMemcpyAsync(s->d_cand, slot.cand, cand->size() * 4, hipMemcpyHostToDevice, s->stream));
+  }
+  HIP_OK(s, hipEventRecord(slot.done, s->stream));
+  slot.used = true;
+  return FX_OK;
+}
+
+bool debug_each_frame(const fx_sim* s) {
+  if (s->desc.debug_fields == 0) return false;
+  if (s->desc.debug_fields == 1) return true;
+  return s->nv < 1800000ull;  // ORCH:193 (voxelCount)
+}
+
+// ------------------------------------------------------------- passes --
+void pass_activate_render(fx_sim* s) {
+  if (s->sparse) {
+    // Candidates: last step's active list + injection/host-written bricks.
+    launch_clear_u32(s->stream, s->flagsInj, s->nb, 0u);
+    launch_activate(s->stream, grid_for(s, s->nb), s->lv[0].L.list, s->lv[0].L.count, s->d_cand, &s->d_const->cand_count,
+                    s->g, s->d_const, s->desc.activation_rule, set_of(s, 0), vec_of(s->u), s->flagsInj);
+    std::vector<Grid> grids;
+    std::vector<uint32_t*> flags;
+    std::vector<BrickList> lists;
+    for (auto& L : s->lv) { grids.push_back(L.g); flags.push_back(L.flags); lists.push_back(L.L); }
+    launch_compact(s->stream, int(s->lv.size()), grids.data(), flags.data(), lists.data(), s->touched);
+    std::vector<float*> fb = fine_buffers(s);
+    launch_zero_bricks(s->stream, grid_for(s, s->nb), s->lv[0].L.retire, s->lv[0].L.retire_count, fb.data(), int(fb.size()));
+    for (size_t l = 1; l < s->lv.size(); ++l) {
+      float* cb[3] = {s->lv[l].div, s->lv[l].p[0], s->lv[l].p[1]};
+      launch_zero_bricks(s->stream, grid_for(s, s->lv[l].nb), s->lv[l].L.retire, s->lv[l].L.retire_count, cb, 3);
+    }
+  } else {
+    launch_clear_u32(s->stream, s->flagsInj, s->nb, 0u);  // ORCH:271
+    launch_activate(s->stream, grid_for(s, s->nb), s->identity, s->identity_count, nullptr, nullptr, s->g, s->d_const,
+                    s->desc.activation_rule, set_of(s, 0), vec_of(s->u), s->flagsInj);
+  }
+}
+void pass_activate_advected(fx_sim* s) {  // ORCH:280-281 (dense only)
+  launch_clear_u32(s->stream, s->flagsRender, s->nb, 0u);
+  launch_activate(s->stream, grid_for(s, s->nb), s->identity, s->identity_count, nullptr, nullptr, s->g, s->d_const,
+                  s->desc.activation_rule, set_of(s, 1), vec_of(s->us), s->flagsRender);
+}
+void pass_inject(fx_sim* s, int parts) {
+  if (s->sparse)
+    launch_inject(s->stream, grid_for(s, s->nb), s->d_cand, &s->d_const->inj_count, s->g, s->d_const, s->d_src, set_of(s, 0),
+                  vec_of(s->u), s->touched, parts);
+  else
+    launch_inject(s->stream, grid_for(s, s->nb), s->identity, s->identity_count, s->g, s->d_const, s->d_src, set_of(s, 0),
+                  vec_of(s->u), nullptr, parts);
+}
+void zero_level_p(fx_sim* s, int l, int which) {
+  float* b[1] = {s->lv[l].p[which]};
+  launch_zero_bricks(s->stream, grid_for(s, s->lv[l].nb), level_list(s, l), level_count(s, l), b, 1);
+}
+void clear_pressure(fx_sim* s) {  // PressureSolvePass.ts:62-66
+  for (size_t l = 0; l < s->lv.size(); ++l) zero_level_p(s, int(l), 0);
+}
+int smooth_ref(fx_sim* s, int l, int iters, int start) {  // PressureSolvePass.ts:129-143
+  int a = start;
+  Level& L = s->lv[l];
+  for (int k = 0; k < iters; ++k) {
+    launch_jacobi(s->stream, grid_for(s, L.nb), level_list(s, l), level_count(s, l), L.g, s->d_const, L.div, L.p[a], L.p[1 - a], 0);
+    a = 1 - a;
+  }
+  return a;
+}
+int solve_reference(fx_sim* s) {  // PressureSolvePass.ts:68-84 with its buffer aliasing
+  Level& F = s->lv[0]; Level& M = s->lv[1]; Level& C = s->lv[2];
+  int finePre = smooth_ref(s, 0, s->q.fine_pre_iterations, 0);
+  launch_residual(s->stream, grid_for(s, F.nb), level_list(s, 0), level_count(s, 0), F.g, F.div, F.p[finePre], F.p[1 - finePre], nullptr);
+  launch_restrict(s->stream, grid_for(s, M.nb), level_list(s, 1), level_count(s, 1), F.g, M.g, F.div, M.div, 0);
+  int midPre = smooth_ref(s, 1, s->q.mid_pre_iterations, 0);
+  launch_residual(s->stream, grid_for(s, M.nb), level_list(s, 1), level_count(s, 1), M.g, M.div, M.p[midPre], M.p[1 - midPre], nullptr);
+  launch_restrict(s->stream, grid_for(s, C.nb), level_list(s, 2), level_count(s, 2), M.g, C.g, M.div, C.div, 0);
+  int co = smooth_ref(s, 2, s->q.coarse_iterations, 0);
+  launch_prolong(s->stream, grid_for(s, M.nb), level_list(s, 1), level_count(s, 1), C.g, M.g, C.p[co], M.p[0]);
+  int mid = smooth_ref(s, 1, s->q.mid_post_iterations, midPre);
+  launch_prolong(s->stream, grid_for(s, F.nb), level_list(s, 0), level_count(s, 0), M.g, F.g, M.p[mid], F.p[0]);
+  return smooth_ref(s, 0, s->q.fine_post_iterations, finePre);
+}
+// Weighted-Jacobi V-cycle; cur[l] = buffer holding the current iterate.
+int solve_vcycle(fx_sim* s) {
+  const int L = int(s->lv.size());
+  const fx_vcycle& v = s->desc.vcycle;
+  std::vector<int> cur(L, 0);
+  auto smoothW = [&](int l, int n) {
+    Level& V = s->lv[l];
+    for (int k = 0; k < n; ++k) {
+      launch_jacobi(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, s->d_const, V.div, V.p[cur[l]],
+                    V.p[1 - cur[l]], 1);
+      cur[l] = 1 - cur[l];
+    }
+  };
+  int cycles = std::min(v.cycles, s->norm_cap);
+  if (cycles > 0) launch_clear_u32(s->stream, reinterpret_cast<uint32_t*>(s->d_norms), size_t(cycles) * 2, 0u);
+  s->norm_count = cycles;
+  for (int c = 0; c < v.cycles; ++c) {
+    for (int l = 0; l < L - 1; ++l) {
+      Level& V = s->lv[l];
+      if (l > 0) { cur[l] = 0; zero_level_p(s, l, 0); }
+      smoothW(l, v.pre_sweeps);
+      launch_residual(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, V.div, V.p[cur[l]], V.p[1 - cur[l]],
+                      (l == 0 && c < s->norm_cap) ? s->d_norms + c : nullptr);
+      launch_restrict(s->stream, grid_for(s, s->lv[l + 1].nb), level_list(s, l + 1), level_count(s, l + 1), V.g, s->lv[l + 1].g,
+                      V.p[1 - cur[l]], s->lv[l + 1].div, 1);
+    }
+    cur[L - 1] = 0;
+    zero_level_p(s, L - 1, 0);
+    smoothW(L - 1, v.coarse_sweeps);
+    for (int l = L - 2; l >= 0; --l) {
+      Level& V = s->lv[l];
+      launch_prolong(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), s->lv[l + 1].g, V.g, s->lv[l + 1].p[cur[l + 1]],
+                     V.p[cur[l]]);
+      smoothW(l, v.post_sweeps);
+    }
+  }
+  return cur[0];
+}
+int pressure_solve(fx_sim* s) {
+  clear_pressure(s);
+  if (s->desc.solver == FX_SOLVER_VCYCLE) return solve_vcycle(s);
+  return solve_reference(s);
+}
+
+void run_stage(fx_sim* s, int pass) {
+  const uint32_t* list = fine_list(s);
+  const uint32_t* count = fine_count(s);
+  const int blocks = grid_for(s, s->nb);
+  switch (pass) {
+    case FX_PASS_ADVECT_VELOCITY:
+      launch_advect_velocity(s->stream, blocks, list, count, s->g, s->d_const, vec_of(s->u), vec_of(s->us));
+      break;
+    case FX_PASS_ADVECT_SCALARS:
+      launch_advect_scalars(s->stream, blocks, list, count, s->g, s->d_const, s->mode, set_of(s, 0), vec_of(s->us), set_of(s, 1),
+                            s->flagsInj);
+      break;
+    case FX_PASS_COMBUSTION:
+      launch_combustion(s->stream, blocks, list, count, s->g, s->d_const, set_of(s, 1), set_of(s, 0), s->flagsRender);
+      break;
+    case FX_PASS_BUOYANCY:
+      launch_buoyancy(s->stream, blocks, list, count, s->g, s->d_const, set_of(s, 0), vec_of(s->us), s->flagsRender);
+      break;
+    case FX_PASS_VORTICITY:
+      launch_vorticity(s->stream, blocks, list, count, s->g, vec_of(s->us), vec_of(s->w), s->wmag);
+      break;
+    case FX_PASS_CONFINEMENT:
+      launch_confinement(s->stream, blocks, list, count, s->g, s->d_const, vec_of(s->w), s->wmag, vec_of(s->us), s->cmag, set_of(s, 0));
+      break;
+    case FX_PASS_DIVERGENCE:
+      launch_divergence(s->stream, blocks, list, count, s->g, vec_of(s->us), s->lv[0].div);
+      break;
+    case FX_PASS_PROJECTION:
+      launch_projection(s->stream, blocks, list, count, s->g, vec_of(s->us), s->lv[0].p[s->lastPressure], vec_of(s->u));
+      break;
+    case FX_PASS_DEBUG_FIELDS:
+      launch_debug(s->stream, blocks, list, count, s->g, vec_of(s->u), s->umag);
+      break;
+    default:
+      break;
+  }
+}
+
+// Candidate list for the sparse activation: injection bricks then host-written bricks.
+void build_candidates(fx_sim* s, const fx_sim_params& p, std::vector<uint32_t>& cand, uint32_t& inj_count) {
+  std::vector<uint8_t> mark(s->nb, 0);
+  injection_bricks(s, p, mark, cand);
+  inj_count = uint32_t(cand.size());
+  for (uint32_t b : s->pending)
+    if (!mark[b]) { mark[b] = 1; cand.push_back(b); }
+  s->pending.clear();
+}
+
+float* field_ptr(fx_sim* s, int id) {
+  switch (id) {
+    case FX_FIELD_DENSITY: case FX_FIELD_TEMPERATURE: case FX_FIELD_FUEL: case FX_FIELD_REACTION: return s->S[0][id];
+    case FX_FIELD_VELOCITY_X: case FX_FIELD_VELOCITY_Y: case FX_FIELD_VELOCITY_Z: return s->u[id - FX_FIELD_VELOCITY_X];
+    case FX_FIELD_SCRATCH_X: case FX_FIELD_SCRATCH_Y: case FX_FIELD_SCRATCH_Z: return s->us[id - FX_FIELD_SCRATCH_X];
+    case FX_FIELD_VORTICITY_X: case FX_FIELD_VORTICITY_Y: case FX_FIELD_VORTICITY_Z: return s->w[id - FX_FIELD_VORTICITY_X];
+    case FX_FIELD_VORTICITY_MAG: return s->wmag;
+    case FX_FIELD_CONFINEMENT_MAG: return s->cmag;
+    case FX_FIELD_VELOCITY_MAG: return s->umag;
+    case FX_FIELD_DIVERGENCE: return s->lv[0].div;
+    case FX_FIELD_PRESSURE: return s->lv[0].p[s->lastPressure];
+    case FX_FIELD_ADV_DENSITY: case FX_FIELD_ADV_TEMPERATURE: case FX_FIELD_ADV_FUEL: case FX_FIELD_ADV_REACTION:
+      return s->S[1][id - FX_FIELD_ADV_DENSITY];
+    default: return nullptr;
+  }
+}
+
+// brick-major <-> row-major
+void gather_rowmajor(const Grid& g, const float* bm, float* rm) {
+  for (uint32_t z = 0; z < g.D; ++z)
+    for (uint32_t y = 0; y < g.H; ++y) {
+      float* row = rm + (size_t(z) * g.H + y) * g.W;
+      for (uint32_t x = 0; x < g.W; ++x) {
+        size_t b = (x >> 3) + size_t(g.BX) * ((y >> 3) + size_t(g.BY) * (z >> 3));
+        row[x] = bm[(b << 9) | ((z & 7u) << 6) | ((y & 7u) << 3) | (x & 7u)];
+      }
+    }
+}
+void scatter_brickmajor(const Grid& g, const float* rm, float* bm) {
+  for (uint32_t z = 0; z < g.D; ++z)
+    for (uint32_t y = 0; y < g.H; ++y) {
+      const float* row = rm + (size_t(z) * g.H + y) * g.W;
+      for (uint32_t x = 0; x < g.W; ++x) {
+        size_t b = (x >> 3) + size_t(g.BX) * ((y >> 3) + size_t(g.BY) * (z >> 3));
+        bm[(b << 9) | ((z & 7u) << 6) | ((y & 7u) << 3) | (x & 7u)] = row[x];
+      }
+    }
+}
+
+int read_brickmajor(fx_sim* s, const Grid& g, const float* dev, float* host, size_t bytes) {
+  size_t nv = size_t(g.W) * g.H * g.D;
+  if (bytes != nv * 4) return fail(s, FX_ERR_INVALID_ARG, "buffer size mismatch");
+  std::vector<float> tmp(storage_count(g));
+  HIP_OK(s, hipStreamSynchronize(s->stream));
+  HIP_OK(s, hipMemcpy(tmp.data(), dev, tmp.size() * 4, hipMemcpyDeviceToHost));
+  gather_rowmajor(g, tmp.data(), host);
+  return FX_OK;
+}
+
+}  // namespace
+
+// ====================================================================== ABI ==
+extern "C" {
+
+void fx_default_create_desc(fx_create_desc* d) {
+  if (!d) return;
+  std::memset(d, 0, sizeof(*d));
+  d->width = 128; d->height = 256; d->depth = 128;           // ORCH:54
+  d->advection_mode = FX_ADVECT_MACCORMACK;                  // ORCH:181
+  d->runtime.gravity[1] = -1.f;                              // ORCH:194-204
+  d->runtime.gravity_strength = 0.45f;
+  d->runtime.buoyancy = 3.6f;
+  d->runtime.vorticity_strength = 2.15f;
+  d->runtime.vorticity_constant_mask = 1.0f;
+  d->runtime.vorticity_velocity_mask = 0.4f;
+  d->runtime.vorticity_heat_mask = 0.8f;
+  d->runtime.vorticity_density_mask = 0.35f;
+  d->runtime.world_size = 10.f;
+  d->quality = fx_quality{1, 1, 2, 3, 2, 3, 10};             // CVS/types.ts:16-24
+  d->domain_mode = FX_DOMAIN_DENSE;
+  d->activation_rule = FX_ACTIVATION_REFERENCE;
+  d->solver = FX_SOLVER_REFERENCE;
+  d->precision = FX_PRECISION_FP32;
+  d->vcycle = fx_vcycle{4, 2, 2, 4, 16, 2.f / 3.f};
+  d->ignition_lo = 0.1f; d->ignition_hi = 0.55f; d->burn_rate = 1.f;  // SH/combustion-update.wgsl.ts:73-74
+  d->activation_density = 1e-4f; d->activation_speed = 1e-3f;
+  d->debug_fields = -1;
+  d->device = 0;
+}
+
+int fx_create(const fx_create_desc* desc, fx_sim** out) {
+  if (!desc || !out) return fail(nullptr, FX_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (desc->width < 4 || desc->height < 4 || desc->depth < 4)
+    return fail(nullptr, FX_ERR_INVALID_ARG, "resolution must be at least 4 in every axis");
+  if (uint64_t(desc->width) * desc->height * desc->depth > (1ull << 31))
+    return fail(nullptr, FX_ERR_INVALID_ARG, "resolution exceeds 2^31 voxels");
+  if (desc->precision != FX_PRECISION_FP32) return fail(nullptr, FX_ERR_UNSUPPORTED, "precision not supported");
+  if (desc->solver == FX_SOLVER_VCYCLE && (desc->vcycle.levels < 2 || desc->vcycle.levels > FX_MAX_LEVELS_DEV))
+    return fail(nullptr, FX_ERR_INVALID_ARG, "vcycle.levels must be in [2, 8]");
+  if (desc->advection_mode != FX_ADVECT_MACCORMACK && desc->advection_mode != FX_ADVECT_SEMI_LAGRANGIAN)
+    return fail(nullptr, FX_ERR_INVALID_ARG, "unknown advection mode");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, FX_ERR_DEVICE, "no HIP device");
+  if (desc->device < 0 || desc->device >= ndev) return fail(nullptr, FX_ERR_INVALID_ARG, "device ordinal out of range");
+
+  fx_sim* s = new (std::nothrow) fx_sim();
+  if (!s) return fail(nullptr, FX_ERR_OUT_OF_MEMORY, "host allocation failed");
+  s->desc = *desc;
+  s->device = desc->device;
+  s->sparse = desc->domain_mode == FX_DOMAIN_SPARSE;
+  s->rt = desc->runtime;
+  s->q = clamp_quality(desc->quality);
+  s->mode = desc->advection_mode;
+  s->g = make_grid(desc->width, desc->height, desc->depth);
+  s->nb = brick_count(s->g);
+  s->nv = size_t(desc->width) * desc->height * desc->depth;
+  s->h_src.assign(FX_FLOATS_PER_EMITTER, 0.f);
+  auto bail = [&](int code) { std::string e = s->err; fx_destroy(s); g_create_error = e; return code; };
+
+  if (hipSetDevice(s->device) != hipSuccess) return bail(fail(s, FX_ERR_DEVICE, "hipSetDevice failed"));
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, s->device) != hipSuccess) return bail(fail(s, FX_ERR_DEVICE, "hipGetDeviceProperties failed"));
+  s->blocks = std::max(1, prop.multiProcessorCount * 4);
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(s, FX_ERR_DEVICE, "stream creation failed"));
+
+  const size_t nsv = storage_count(s->g);
+  int rc = FX_OK;
+  for (int k = 0; k < 2 && rc == FX_OK; ++k) for (int f = 0; f < 4 && rc == FX_OK; ++f) rc = dalloc(s, &s->S[k][f], nsv);
+  for (int c = 0; c < 3 && rc == FX_OK; ++c) {
+    rc = dalloc(s, &s->u[c], nsv);
+    if (rc == FX_OK) rc = dalloc(s, &s->us[c], nsv);
+    if (rc == FX_OK) rc = dalloc(s, &s->w[c], nsv);
+  }
+  if (rc == FX_OK) rc = dalloc(s, &s->wmag, nsv);
+  if (rc == FX_OK) rc = dalloc(s, &s->cmag, nsv);
+  if (rc == FX_OK) rc = dalloc(s, &s->umag, nsv);
+  if (rc == FX_OK) rc = dalloc(s, &s->flagsInj, s->nb);
+  if (rc == FX_OK) rc = s->sparse ? FX_OK : dalloc(s, &s->flagsRender, s->nb);
+  if (rc == FX_OK) rc = dalloc(s, &s->touched, s->nb);
+  if (rc == FX_OK) rc = dalloc(s, &s->identity, s->nb);
+  if (rc == FX_OK) rc = dalloc(s, &s->identity_count, 1);
+  if (rc == FX_OK) rc = dalloc(s, &s->d_const, 1);
+  if (rc == FX_OK) rc = dalloc(s, &s->d_cand, s->nb);
+  if (rc != FX_OK) return bail(rc);
+  if (s->sparse) s->flagsRender = s->flagsInj;
+
+  int nlev = desc->solver == FX_SOLVER_VCYCLE ? desc->vcycle.levels : 3;  // PressureSolvePass.ts:32-35
+  Grid lg = s->g;
+  for (int l = 0; l < nlev && rc == FX_OK; ++l) {
+    Level L;
+    L.g = lg;
+    L.nb = brick_count(lg);
+    L.nv = storage_count(lg);
+    rc = dalloc(s, &L.div, L.nv);
+    if (rc == FX_OK) rc = dalloc(s, &L.p[0], L.nv);
+    if (rc == FX_OK) rc = dalloc(s, &L.p[1], L.nv);
+    if (rc == FX_OK) rc = dalloc(s, &L.L.list, L.nb);
+    if (rc == FX_OK) rc = dalloc(s, &L.L.count, 1);
+    if (rc == FX_OK) rc = dalloc(s, &L.L.page, L.nb);
+    if (rc == FX_OK) rc = dalloc(s, &L.L.retire, L.nb);
+    if (rc == FX_OK) rc = dalloc(s, &L.L.retire_count, 1);
+    if (rc == FX_OK) rc = l == 0 ? FX_OK : dalloc(s, &L.flags, L.nb);
+    if (l == 0) L.flags = s->flagsInj;
+    s->lv.push_back(L);
+    lg = make_grid(halve(lg.W), halve(lg.H), halve(lg.D));
+  }
+  if (rc != FX_OK) return bail(rc);
+  s->norm_cap = std::max(1, desc->vcycle.cycles);
+  if (dalloc(s, &s->d_norms, size_t(s->norm_cap)) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
+  s->src_cap = FX_FLOATS_PER_EMITTER;
+  if (dalloc(s, &s->d_src, s->src_cap) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
+  for (auto& slot : s->slots) {
+    if (hipHostMalloc(reinterpret_cast<void**>(&slot.c), sizeof(StepConst), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&slot.cand), std::max<size_t>(s->nb, 1) * 4, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&slot.done, hipEventDisableTiming) != hipSuccess)
+      return bail(fail(s, FX_ERR_OUT_OF_MEMORY, "pinned staging allocation failed"));
+  }
+
+  // Identity lists (dense) and zeroed state.
+  std::vector<uint32_t> ident(s->nb);
+  for (size_t b = 0; b < s->nb; ++b) ident[b] = uint32_t(b);
+  uint32_t nbc = uint32_t(s->nb);
+  if (hipMemcpy(s->identity, ident.data(), s->nb * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->identity_count, &nbc, 4, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(s, FX_ERR_DEVICE, "identity upload failed"));
+  for (auto& L : s->lv) {
+    if (!s->sparse) {
+      std::vector<uint32_t> id(L.nb);
+      for (size_t b = 0; b < L.nb; ++b) id[b] = uint32_t(b);
+      uint32_t n = uint32_t(L.nb);
+      if (hipMemcpy(L.L.list, id.data(), L.nb * 4, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(L.L.count, &n, 4, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(s, FX_ERR_DEVICE, "level list upload failed"));
+    }
+  }
+  s->h_src.assign(FX_FLOATS_PER_EMITTER, 0.f);
+  if (hipMemcpy(s->d_src, s->h_src.data(), FX_FLOATS_PER_EMITTER * 4, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(s, FX_ERR_DEVICE, "source upload failed"));
+  int r = fx_reset(s);
+  if (r != FX_OK) return bail(r);
+  if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(s, FX_ERR_DEVICE, "initial clear failed"));
+  *out = s;
+  return FX_OK;
+}
+
+void fx_destroy(fx_sim* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  auto F = [](void* p) { if (p) (void)hipFree(p); };
+  for (int k = 0; k < 2; ++k) for (int f = 0; f < 4; ++f) F(s->S[k][f]);
+  for (int c = 0; c < 3; ++c) { F(s->u[c]); F(s->us[c]); F(s->w[c]); }
+  F(s->wmag); F(s->cmag); F(s->umag);
+  F(s->flagsInj);
+  if (!s->sparse) F(s->flagsRender);
+  F(s->touched); F(s->identity); F(s->identity_count); F(s->d_const); F(s->d_cand); F(s->d_src); F(s->d_norms);
+  for (size_t l = 0; l < s->lv.size(); ++l) {
+    Level& L = s->lv[l];
+    F(L.div); F(L.p[0]); F(L.p[1]); F(L.L.list); F(L.L.count); F(L.L.page); F(L.L.retire); F(L.L.retire_count);
+    if (l > 0) F(L.flags);
+  }
+  for (auto& slot : s->slots) {
+    if (slot.c) (void)hipHostFree(slot.c);
+    if (slot.cand) (void)hipHostFree(slot.cand);
+    if (slot.done) (void)hipEventDestroy(slot.done);
+  }
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+int fx_reset(fx_sim* s) {  // ORCH:206-226
+  if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
+  HIP_OK(s, hipSetDevice(s->device));
+  s->curSet = 0; s->stepIndex = 0; s->lastPressure = 0; s->initialized = false;
+  s->startS = s->lastS;
+  s->pending.clear();
+  const size_t nsv = storage_count(s->g);
+  for (float* b : fine_buffers(s)) launch_clear_f32(s->stream, b, nsv);
+  launch_clear_u32(s->stream, s->flagsInj, s->nb, 0u);
+  if (!s->sparse) launch_clear_u32(s->stream, s->flagsRender, s->nb, 0u);
+  launch_clear_u32(s->stream, s->touched, s->nb, 0u);
+  for (size_t l = 0; l < s->lv.size(); ++l) {
+    Level& L = s->lv[l];
+    if (l > 0) {
+      launch_clear_f32(s->stream, L.div, L.nv);
+      launch_clear_f32(s->stream, L.p[0], L.nv);
+      launch_clear_f32(s->stream, L.p[1], L.nv);
+      launch_clear_u32(s->stream, L.flags, L.nb, 0u);
+    }
+    if (s->sparse) {
+      launch_clear_u32(s->stream, reinterpret_cast<uint32_t*>(L.L.page), L.nb, 0xffffffffu);
+      launch_clear_u32(s->stream, L.L.count, 1, 0u);
+      launch_clear_u32(s->stream, L.L.retire_count, 1, 0u);
+    }
+  }
+  HIP_OK(s, hipGetLastError());
+  return FX_OK;
+}
+
+int fx_step(fx_sim* s, double elapsed_s, double step_s) {  // ORCH:230-305
+  if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
+  if (!std::isfinite(elapsed_s) || !std::isfinite(step_s)) return fail(s, FX_ERR_INVALID_ARG, "non-finite time");
+  HIP_OK(s, hipSetDevice(s->device));
+  if (!s->initialized) { s->initialized = true; s->startS = elapsed_s; s->lastS = elapsed_s; }
+  fx_sim_params p = make_params(s, elapsed_s, step_s);
+  std::vector<uint32_t> cand;
+  uint32_t inj = 0;
+  if (s->sparse) build_candidates(s, p, cand, inj);
+  int rc = upload_step(s, p, s->sparse ? &cand : nullptr, inj);
+  if (rc != FX_OK) return rc;
+
+  pass_inject(s, 7);                                   // ORCH:270
+  pass_activate_render(s);                             // ORCH:271-272
+  run_stage(s, FX_PASS_ADVECT_VELOCITY);               // ORCH:273
+  run_stage(s, FX_PASS_ADVECT_SCALARS);                // ORCH:274-279
+  if (!s->sparse) pass_activate_advected(s);           // ORCH:280-281
+  run_stage(s, FX_PASS_COMBUSTION);                    // ORCH:282
+  run_stage(s, FX_PASS_BUOYANCY);                      // ORCH:283
+  if (s->stepIndex % uint64_t(s->q.vorticity_interval) == 0) {  // ORCH:285-288
+    run_stage(s, FX_PASS_VORTICITY);
+    run_stage(s, FX_PASS_CONFINEMENT);
+  }
+  if (s->stepIndex % uint64_t(s->q.pressure_interval) == 0) {    // ORCH:290-294
+    clear_pressure(s);
+    run_stage(s, FX_PASS_DIVERGENCE);
+    s->lastPressure = pressure_solve(s);
+  }
+  run_stage(s, FX_PASS_PROJECTION);                    // ORCH:296
+  if (debug_each_frame(s)) run_stage(s, FX_PASS_DEBUG_FIELDS);  // ORCH:298-301
+  s->curSet = s->curSet == 0 ? 1 : 0;
+  s->lastS = elapsed_s;
+  s->stepIndex += 1;
+  HIP_OK(s, hipGetLastError());
+  return FX_OK;
+}
+
+int fx_run_pass(fx_sim* s, int32_t pass, const fx_sim_params* params) {
+  if (!s || !params) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  HIP_OK(s, hipSetDevice(s->device));
+  std::vector<uint32_t> cand;
+  uint32_t inj = 0;
+  if (s->sparse) build_candidates(s, *params, cand, inj);
+  int rc = upload_step(s, *params, s->sparse ? &cand : nullptr, inj);
+  if (rc != FX_OK) return rc;
+  switch (pass) {
+    case FX_PASS_INJECT_SCALAR: pass_inject(s, 1); break;
+    case FX_PASS_INJECT_VELOCITY: pass_inject(s, 2); break;
+    case FX_PASS_INJECT_IGNITER: pass_inject(s, 4); break;
+    case FX_PASS_ACTIVATE_RENDER: pass_activate_render(s); break;
+    case FX_PASS_ACTIVATE_ADVECTED:
+      if (s->sparse) return fail(s, FX_ERR_UNSUPPORTED, "sparse domains activate once per step");
+      pass_activate_advected(s);
+      break;
+    case FX_PASS_PRESSURE: s->lastPressure = pressure_solve(s); break;
+    case FX_PASS_ADVECT_VELOCITY: case FX_PASS_ADVECT_SCALARS: case FX_PASS_COMBUSTION: case FX_PASS_BUOYANCY:
+    case FX_PASS_VORTICITY: case FX_PASS_CONFINEMENT: case FX_PASS_DIVERGENCE: case FX_PASS_PROJECTION:
+    case FX_PASS_DEBUG_FIELDS:
+      run_stage(s, pass);
+      break;
+    default: return fail(s, FX_ERR_INVALID_ARG, "unknown pass id");
+  }
+  HIP_OK(s, hipGetLastError());
+  return FX_OK;
+}
+
+int fx_synchronize(fx_sim* s) {
+  if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
+  HIP_OK(s, hipStreamSynchronize(s->stream));
+  return FX_OK;
+}
+
+int fx_get_runtime_params(fx_sim* s, fx_runtime_params* out) {
+  if (!s || !out) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  *out = s->rt;
+  return FX_OK;
+}
+int fx_set_runtime_params(fx_sim* s, const fx_runtime_params* p, uint32_t m) {  // ORCH:353-365
+  if (!s || !p) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  fx_runtime_params& r = s->rt;
+  if (m & FX_RP_WIND) { r.wind[0] = p->wind[0]; r.wind[1] = p->wind[1]; r.wind[2] = p->wind[2]; }
+  if (m & FX_RP_WIND_STRENGTH) r.wind_strength = p->wind_strength;
+  if (m & FX_RP_GRAVITY) { r.gravity[0] = p->gravity[0]; r.gravity[1] = p->gravity[1]; r.gravity[2] = p->gravity[2]; }
+  if (m & FX_RP_GRAVITY_STRENGTH) r.gravity_strength = p->gravity_strength;
+  if (m & FX_RP_BUOYANCY) r.buoyancy = p->buoyancy;
+  if (m & FX_RP_VORTICITY_STRENGTH) r.vorticity_strength = p->vorticity_strength;
+  if (m & FX_RP_VORTICITY_CONSTANT_MASK) r.vorticity_constant_mask = p->vorticity_constant_mask;
+  if (m & FX_RP_VORTICITY_VELOCITY_MASK) r.vorticity_velocity_mask = p->vorticity_velocity_mask;
+  if (m & FX_RP_VORTICITY_HEAT_MASK) r.vorticity_heat_mask = p->vorticity_heat_mask;
+  if (m & FX_RP_VORTICITY_DENSITY_MASK) r.vorticity_density_mask = p->vorticity_density_mask;
+  if (m & FX_RP_WORLD_SIZE) r.world_size = p->world_size;
+  return FX_OK;
+}
+int fx_get_quality(fx_sim* s, fx_quality* out) {
+  if (!s || !out) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  *out = s->q;
+  return FX_OK;
+}
+int fx_set_quality(fx_sim* s, const fx_quality* q) {
+  if (!s || !q) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  s->q = clamp_quality(*q);
+  return FX_OK;
+}
+int fx_get_advection_mode(fx_sim* s, int32_t* out) {
+  if (!s || !out) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  *out = s->mode;
+  return FX_OK;
+}
+int fx_set_advection_mode(fx_sim* s, int32_t m) {
+  if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
+  if (m != FX_ADVECT_MACCORMACK && m != FX_ADVECT_SEMI_LAGRANGIAN) return fail(s, FX_ERR_INVALID_ARG, "unknown advection mode");
+  s->mode = m;
+  return FX_OK;
+}
+
+int fx_update_sources(fx_sim* s, const float* packed, uint32_t count) {  // SourceInjectionPass.ts:71-83
+  if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
+  if (count && !packed) return fail(s, FX_ERR_INVALID_ARG, "null source data");
+  HIP_OK(s, hipSetDevice(s->device));
+  const size_t nf = size_t(std::max<uint32_t>(count, 1)) * FX_FLOATS_PER_EMITTER;
+  std::vector<float> h(nf, 0.f);
+  if (count) std::memcpy(h.data(), packed, size_t(count) * FX_FLOATS_PER_EMITTER * 4);
+  HIP_OK(s, hipStreamSynchronize(s->stream));
+  if (nf > s->src_cap) {
+    HIP_OK(s, hipFree(s->d_src));
+    s->d_src = nullptr;
+    if (dalloc(s, &s->d_src, nf) != FX_OK) return FX_ERR_OUT_OF_MEMORY;
+    s->src_cap = nf;
+  }
+  HIP_OK(s, hipMemcpy(s->d_src, h.data(), nf * 4, hipMemcpyHostToDevice));
+  s->h_src = std::move(h);
+  s->nsrc = count;
+  return FX_OK;
+}
+
+int fx_read_field(fx_sim* s, int32_t id, float* host, size_t bytes) {
+  if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  float* f = field_ptr(s, id);
+  if (!f) return fail(s, FX_ERR_INVALID_ARG, "unknown field id");
+  HIP_OK(s, hipSetDevice(s->device));
+  return read_brickmajor(s, s->g, f, host, bytes);
+}
+
+int fx_write_field(fx_sim* s, int32_t id, const float* host, size_t bytes) {
+  if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  float* f = field_ptr(s, id);
+  if (!f) return fail(s, FX_ERR_INVALID_ARG, "unknown field id");
+  if (bytes != s->nv * 4) return fail(s, FX_ERR_INVALID_ARG, "buffer size mismatch");
+  HIP_OK(s, hipSetDevice(s->device));
+  std::vector<float> tmp(storage_count(s->g), 0.f);
+  scatter_brickmajor(s->g, host, tmp.data());
+  HIP_OK(s, hipStreamSynchronize(s->stream));
+  HIP_OK(s, hipMemcpy(f, tmp.data(), tmp.size() * 4, hipMemcpyHostToDevice));
+  if (s->sparse) {
+    // Bricks holding non-zero data join the next activation's candidates.
+    std::vector<uint32_t> nz;
+    for (size_t b = 0; b < s->nb; ++b) {
+      const float* p = &tmp[b << 9];
+      for (int k = 0; k < 512; ++k)
+        if (p[k] != 0.f) { nz.push_back(uint32_t(b)); break; }
+    }
+    if (!nz.empty()) {
+      HIP_OK(s, hipMemcpy(s->d_cand, nz.data(), nz.size() * 4, hipMemcpyHostToDevice));
+      launch_mark(s->stream, s->d_cand, uint32_t(nz.size()), s->touched);
+      HIP_OK(s, hipStreamSynchronize(s->stream));
+      std::vector<uint8_t> m(s->nb, 0);
+      for (uint32_t b : s->pending) m[b] = 1;
+      for (uint32_t b : nz) if (!m[b]) s->pending.push_back(b);
+    }
+  }
+  return FX_OK;
+}
+
+int fx_read_flags(fx_sim* s, int32_t which, uint32_t* host, size_t count) {
+  if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  if (count != s->nb) return fail(s, FX_ERR_INVALID_ARG, "count must equal the brick count");
+  HIP_OK(s, hipSetDevice(s->device));
+  HIP_OK(s, hipStreamSynchronize(s->stream));
+  const uint32_t* f = which == FX_FLAGS_INJECTED ? s->flagsInj : s->flagsRender;
+  HIP_OK(s, hipMemcpy(host, f, count * 4, hipMemcpyDeviceToHost));
+  return FX_OK;
+}
+
+int fx_read_active_list(fx_sim* s, uint32_t* host, size_t capacity, uint32_t* count) {
+  if (!s || !count) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  HIP_OK(s, hipSetDevice(s->device));
+  HIP_OK(s, hipStreamSynchronize(s->stream));
+  if (s->sparse) {
+    uint32_t n = 0;
+    HIP_OK(s, hipMemcpy(&n, s->lv[0].L.count, 4, hipMemcpyDeviceToHost));
+    *count = n;
+    if (host && capacity) HIP_OK(s, hipMemcpy(host, s->lv[0].L.list, std::min<size_t>(n, capacity) * 4, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<uint32_t> f(s->nb);
+    HIP_OK(s, hipMemcpy(f.data(), s->flagsRender, s->nb * 4, hipMemcpyDeviceToHost));
+    uint32_t n = 0;
+    for (size_t b = 0; b < s->nb; ++b)
+      if (f[b]) { if (host && n < capacity) host[n] = uint32_t(b); ++n; }
+    *count = n;
+  }
+  return FX_OK;
+}
+
+int fx_read_page_table(fx_sim* s, int32_t* host, size_t count) {
+  if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  if (count != s->nb) return fail(s, FX_ERR_INVALID_ARG, "count must equal the brick count");
+  HIP_OK(s, hipSetDevice(s->device));
+  HIP_OK(s, hipStreamSynchronize(s->stream));
+  if (s->sparse) {
+    HIP_OK(s, hipMemcpy(host, s->lv[0].L.page, count * 4, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<uint32_t> f(s->nb);
+    HIP_OK(s, hipMemcpy(f.data(), s->flagsRender, s->nb * 4, hipMemcpyDeviceToHost));
+    int32_t n = 0;
+    for (size_t b = 0; b < s->nb; ++b) host[b] = f[b] ? n++ : -1;
+  }
+  return FX_OK;
+}
+
+void* fx_field_device_ptr(fx_sim* s, int32_t id) { return s ? static_cast<void*>(field_ptr(s, id)) : nullptr; }
+
+int fx_read_level(fx_sim* s, int32_t level, int32_t which, float* host, size_t bytes) {
+  if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  if (level < 0 || level >= int(s->lv.size())) return fail(s, FX_ERR_INVALID_ARG, "level out of range");
+  Level& L = s->lv[level];
+  const float* f = which == FX_LEVEL_DIVERGENCE ? L.div : which == FX_LEVEL_PRESSURE_A ? L.p[0] : L.p[1];
+  HIP_OK(s, hipSetDevice(s->device));
+  return read_brickmajor(s, L.g, f, host, bytes);
+}
+int fx_level_dims(fx_sim* s, int32_t level, uint32_t dims[3]) {
+  if (!s || !dims) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  if (level < 0 || level >= int(s->lv.size())) return fail(s, FX_ERR_INVALID_ARG, "level out of range");
+  dims[0] = s->lv[level].g.W; dims[1] = s->lv[level].g.H; dims[2] = s->lv[level].g.D;
+  return FX_OK;
+}
+int fx_solver_residuals(fx_sim* s, double* host, size_t capacity, uint32_t* count) {
+  if (!s || !count) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  HIP_OK(s, hipSetDevice(s->device));
+  HIP_OK(s, hipStreamSynchronize(s->stream));
+  uint32_t n = s->desc.solver == FX_SOLVER_VCYCLE ? uint32_t(s->norm_count) : 0u;
+  *count = n;
+  if (n && host && capacity) {
+    std::vector<double> t(n);
+    HIP_OK(s, hipMemcpy(t.data(), s->d_norms, n * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < std::min<size_t>(n, capacity); ++i) host[i] = std::sqrt(t[i]);
+  }
+  return FX_OK;
+}
+int fx_active_brick_count(fx_sim* s, uint32_t* out) {
+  if (!s || !out) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  HIP_OK(s, hipSetDevice(s->device));
+  HIP_OK(s, hipStreamSynchronize(s->stream));
+  if (s->sparse) HIP_OK(s, hipMemcpy(out, s->lv[0].L.count, 4, hipMemcpyDeviceToHost));
+  else *out = uint32_t(s->nb);
+  return FX_OK;
+}
+
+static int build_ray_const(fx_sim* s, const fx_camera* c, const fx_render_params* p, uint32_t w, uint32_t h, RayConst& R) {
+  if (!c || !p || w == 0 || h == 0) return fail(s, FX_ERR_INVALID_ARG, "invalid raymarch arguments");
+  // createVolumeRaymarchPass.ts:83-88, 196-262; volumeWorldBounds.ts:14-31
+  double maxH = double(std::max(s->g.W, s->g.D));
+  float hx = float(2.05 * 13.5 * (s->g.W / maxH)), hy = float(2.05 * 13.5 * (s->g.H / maxH)), hz = float(2.05 * 13.5 * (s->g.D / maxH));
+  float cy = float(double(hy) - 0.25);
+  std::memset(&R, 0, sizeof(R));
+  for (int k = 0; k < 3; ++k) { R.cam_pos[k] = c->position[k]; R.right[k] = c->right[k]; R.up[k] = c->up[k]; R.fwd[k] = c->forward[k]; }
+  R.aspect = c->aspect; R.tan_half = c->tan_half_fov_y;
+  const float center[3] = {0.f, cy, 0.f}, half[3] = {hx, hy, hz};
+  for (int k = 0; k < 3; ++k) { R.box_min[k] = center[k] - half[k]; R.box_max[k] = center[k] + half[k]; R.half_ext[k] = half[k]; }
+  R.mode = float(p->display_mode);
+  R.step_count = uint32_t(float(p->step_count));
+  R.light_count = float(p->light_count);
+  R.scat_f = p->scattering_forward; R.scat_b = p->scattering_back;
+  for (int i = 0; i < 3; ++i) {
+    if (i < p->light_count) {
+      const fx_light& l = p->lights[i];
+      for (int k = 0; k < 3; ++k) { R.light_dir[i][k] = l.direction[k]; R.light_pos[i][k] = l.position[k]; R.light_color[i][k] = l.color[k]; }
+      R.light_kind[i] = l.type == 1 ? 1.f : 0.f;
+      R.light_intensity[i] = l.intensity;
+    }
+  }
+  R.width = w; R.height = h;
+  return FX_OK;
+}
+
+int fx_raymarch_device(fx_sim* s, const fx_camera* c, const fx_render_params* p, uint32_t w, uint32_t h, void* dev) {
+  if (!s || !dev) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  RayConst R;
+  int rc = build_ray_const(s, c, p, w, h, R);
+  if (rc != FX_OK) return rc;
+  HIP_OK(s, hipSetDevice(s->device));
+  launch_raymarch(s->stream, s->g, R, set_of(s, 0), s->flagsRender, static_cast<uint16_t*>(dev));
+  HIP_OK(s, hipGetLastError());
+  return FX_OK;
+}
+
+int fx_raymarch(fx_sim* s, const fx_camera* c, const fx_render_params* p, uint32_t w, uint32_t h, uint16_t* host) {
+  if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  HIP_OK(s, hipSetDevice(s->device));
+  void* dev = nullptr;
+  HIP_OK(s, hipMalloc(&dev, size_t(w) * h * 8));
+  int rc = fx_raymarch_device(s, c, p, w, h, dev);
+  if (rc == FX_OK) {
+    hipError_t e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) e = hipMemcpy(host, dev, size_t(w) * h * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(s, FX_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
+  }
+  (void)hipFree(dev);
+  return rc;
+}
+
+const char* fx_last_error(fx_sim* s) { return s ? s->err.c_str() : g_create_error.c_str(); }
+
+}  // extern "C"

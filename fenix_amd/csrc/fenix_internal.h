@@ -1,0 +1,124 @@
+// fenix_internal.h — shared declarations between the HIP kernels and the host
+// side of libfenix. Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fenix.h"
+
+#define FX_MAX_LEVELS_DEV 8
+#define FX_MAX_ZERO_BUFS 24
+
+namespace fx {
+
+// Dimensions of one grid level and its 8^3 brick tiling. Storage is
+// brick-major: brick b = bx + BX*(by + BY*bz) owns elements [b*512, b*512+512),
+// voxel (x,y,z) sits at b*512 + (z&7)*64 + (y&7)*8 + (x&7).
+struct Grid {
+  uint32_t W, H, D;
+  uint32_t BX, BY, BZ;
+};
+
+inline Grid make_grid(uint32_t W, uint32_t H, uint32_t D) {
+  return Grid{W, H, D, (W + 7) / 8, (H + 7) / 8, (D + 7) / 8};
+}
+inline size_t brick_count(const Grid& g) { return size_t(g.BX) * g.BY * g.BZ; }
+inline size_t storage_count(const Grid& g) { return brick_count(g) * 512; }
+
+// Per-step constants read by every kernel from device memory (uploaded once per
+// step; scalar loads).
+struct StepConst {
+  fx_sim_params p;        // the reference's 80-byte uniform
+  uint32_t nsrc;          // emitters iterated (>= 1, the reference's 128-byte minimum buffer)
+  float ignition_lo, ignition_hi, burn_rate;
+  float act_density, act_speed;
+  float omega;            // V-cycle weighted-Jacobi factor
+  uint32_t inj_count;     // sparse: bricks visited by injection (head of the candidate list)
+  uint32_t cand_count;    // sparse: injection bricks + bricks written by the host
+  uint32_t pad[2];
+};
+
+// Brick list of one level: ascending brick indices, device count, page table.
+struct BrickList {
+  uint32_t* list;    // capacity = brick count
+  uint32_t* count;   // device scalar
+  int32_t* page;     // brick -> list position or -1
+  uint32_t* retire;  // bricks to zero (sparse)
+  uint32_t* retire_count;
+};
+
+// Launch helpers implemented in fenix_kernels.hip. All are asynchronous on `s`.
+// `list`/`count` select the bricks visited; `grid_blocks` is the persistent grid.
+struct Fields;  // host-side view, see fenix_api.cpp
+
+struct Scalars {
+  float* d;
+  float* t;
+  float* f;
+  float* r;
+};
+struct Vec3 {
+  float* x;
+  float* y;
+  float* z;
+};
+
+void launch_inject(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
+                   const StepConst* c, const float* src, Scalars set0, Vec3 u, uint32_t* touched, int parts);
+void launch_clear_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t value);
+void launch_mark(hipStream_t s, const uint32_t* list, uint32_t n, uint32_t* touched);
+void launch_clear_f32(hipStream_t s, float* p, size_t n);
+void launch_activate(hipStream_t s, int blocks, const uint32_t* listA, const uint32_t* countA,
+                     const uint32_t* listB, const uint32_t* countB, Grid g, const StepConst* c, int rule,
+                     Scalars set, Vec3 vel, uint32_t* flags);
+void launch_compact(hipStream_t s, int nlevels, const Grid* grids, uint32_t* const* flags_lv,
+                    BrickList* lists, uint32_t* touched);
+void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
+                            const StepConst* c, Vec3 u, Vec3 us);
+void launch_advect_scalars(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
+                           const StepConst* c, int mode, Scalars src, Vec3 us, Scalars dst, const uint32_t* flags);
+void launch_combustion(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
+                       const StepConst* c, Scalars src, Scalars dst, const uint32_t* flags);
+void launch_buoyancy(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
+                     const StepConst* c, Scalars set0, Vec3 us, const uint32_t* flags);
+void launch_vorticity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
+                      Vec3 us, Vec3 w, float* wmag);
+void launch_confinement(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
+                        const StepConst* c, Vec3 w, const float* wmag, Vec3 us, float* cmag, Scalars set0);
+void launch_divergence(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
+                       Vec3 us, float* div);
+void launch_zero_bricks(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count,
+                        float* const* bufs, int nbufs);
+void launch_jacobi(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
+                   const StepConst* c, const float* rhs, const float* src, float* dst, int weighted);
+void launch_residual(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
+                     const float* rhs, const float* p, float* r, double* norm2);
+void launch_restrict(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid fine,
+                     Grid coarse, const float* src, float* dst, int full_weight);
+void launch_prolong(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid coarse,
+                    Grid fine, const float* src, float* dst);
+void launch_projection(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
+                       Vec3 us, const float* p, Vec3 u);
+void launch_debug(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 u,
+                  float* umag);
+
+struct RayConst {
+  float cam_pos[3], right[3], up[3], fwd[3];
+  float aspect, tan_half;
+  float box_min[3], box_max[3], half_ext[3];
+  float mode;
+  uint32_t step_count;
+  float light_count;
+  float scat_f, scat_b;
+  float light_dir[3][3];
+  float light_kind[3];
+  float light_pos[3][3];
+  float light_intensity[3];
+  float light_color[3][3];
+  uint32_t width, height;
+};
+void launch_raymarch(hipStream_t s, Grid g, const RayConst& rc, Scalars set0, const uint32_t* flags,
+                     uint16_t* out);
+
+}  // namespace fx

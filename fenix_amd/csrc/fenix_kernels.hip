@@ -1,0 +1,1051 @@
+// fenix_kernels.hip — gfx950 kernels of the per-step combustion-volume pipeline.
+//
+// Every kernel walks a device-resident brick list with a persistent grid-stride
+// loop (one 512-thread workgroup = one 8^3 brick = eight wave64 z-slices of 8x8
+// voxels; each wave reads/writes 256 contiguous bytes per field). The same
+// kernels serve the dense domain (list = every brick) and the sparse domain
+// (list = active bricks; every buffer holds zeros outside the active bricks, so
+// stencils and trilinear gathers read memory unmasked).
+//
+// Arithmetic follows the reference WGSL term by term (paths relative to the
+// reference root, SH = src/engine/simulation/shaders/passes); the library is
+// built with -ffp-contract=off so no FMA contraction changes the rounding.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "fenix_internal.h"
+
+namespace fx {
+
+// --------------------------------------------------------------- helpers --
+struct F3 { float x, y, z; };
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return F3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return F3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 operator-(F3 a) { return F3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ F3 operator*(F3 a, float s) { return F3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ F3 operator/(F3 a, float s) { return F3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ F3 mul3(F3 a, F3 b) { return F3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float len3(F3 a) { return sqrtf(dot3(a, a)); }
+__device__ __forceinline__ F3 norm3(F3 a) { float l = len3(a); return F3{a.x / l, a.y / l, a.z / l}; }
+__device__ __forceinline__ F3 cross3(F3 a, F3 b) {
+  return F3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+__device__ __forceinline__ float clamp01(float v) { return clampf(v, 0.f, 1.f); }
+// WGSL mix(e1,e2,e3) = e1*(1-e3) + e2*e3
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.f - t) + b * t; }
+__device__ __forceinline__ F3 mix3(F3 a, F3 b, float t) { return F3{mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t)}; }
+__device__ __forceinline__ float smoothf(float e0, float e1, float x) {
+  float t = clampf((x - e0) / (e1 - e0), 0.f, 1.f);
+  return t * t * (3.f - 2.f * t);
+}
+__device__ __forceinline__ float stepf(float edge, float x) { return edge <= x ? 1.f : 0.f; }
+__device__ __forceinline__ float fractf(float x) { return x - floorf(x); }
+__device__ __forceinline__ uint32_t decu(uint32_t v) { return v == 0u ? 0u : v - 1u; }
+__device__ __forceinline__ uint32_t incu(uint32_t v, uint32_t m) { return min(v + 1u, m); }
+
+__device__ __forceinline__ uint32_t vidx(const Grid& g, uint32_t x, uint32_t y, uint32_t z) {
+  return ((((z >> 3) * g.BY + (y >> 3)) * g.BX + (x >> 3)) << 9) | ((z & 7u) << 6) | ((y & 7u) << 3) | (x & 7u);
+}
+__device__ __forceinline__ uint32_t cidx(const Grid& g, uint32_t x, uint32_t y, uint32_t z) {
+  return vidx(g, min(x, g.W - 1), min(y, g.H - 1), min(z, g.D - 1));
+}
+__device__ __forceinline__ bool on_boundary(const Grid& g, uint32_t x, uint32_t y, uint32_t z) {
+  return x == 0 || y == 0 || z == 0 || x == g.W - 1 || y == g.H - 1 || z == g.D - 1;
+}
+// isActiveCoord, src/engine/simulation/shaders/common/active-brick-mask.wgsl.ts:13-18
+__device__ __forceinline__ bool flag_on(const Grid& g, const uint32_t* flags, uint32_t x, uint32_t y, uint32_t z) {
+  uint32_t bx = min(x >> 3, g.BX - 1), by = min(y >> 3, g.BY - 1), bz = min(z >> 3, g.BZ - 1);
+  return flags[bx + g.BX * (by + g.BY * bz)] != 0u;
+}
+
+// Voxel owned by this thread inside brick b.
+struct Vox {
+  uint32_t x, y, z, i;
+  bool in;
+};
+__device__ __forceinline__ Vox vox_of(const Grid& g, uint32_t b) {
+  uint32_t t = threadIdx.x;
+  uint32_t bx = b % g.BX, by = (b / g.BX) % g.BY, bz = b / (g.BX * g.BY);
+  Vox v;
+  v.x = bx * 8 + (t & 7u);
+  v.y = by * 8 + ((t >> 3) & 7u);
+  v.z = bz * 8 + (t >> 6);
+  v.i = (b << 9) | t;
+  v.in = v.x < g.W && v.y < g.H && v.z < g.D;
+  return v;
+}
+
+// Trilinear sampler, src/engine/simulation/shaders/common/sampling-scalar.wgsl.ts:11-34.
+struct Tri {
+  uint32_t i[8];
+  float fx, fy, fz;
+};
+__device__ __forceinline__ Tri tri(const Grid& g, F3 p) {
+  float mx = float(g.W - 1), my = float(g.H - 1), mz = float(g.D - 1);
+  float cx = clampf(p.x, 0.f, mx), cy = clampf(p.y, 0.f, my), cz = clampf(p.z, 0.f, mz);
+  uint32_t bx = uint32_t(floorf(cx)), by = uint32_t(floorf(cy)), bz = uint32_t(floorf(cz));
+  uint32_t ux = min(bx + 1u, g.W - 1), uy = min(by + 1u, g.H - 1), uz = min(bz + 1u, g.D - 1);
+  Tri t;
+  t.fx = fractf(cx); t.fy = fractf(cy); t.fz = fractf(cz);
+  t.i[0] = vidx(g, bx, by, bz); t.i[1] = vidx(g, ux, by, bz); t.i[2] = vidx(g, bx, uy, bz); t.i[3] = vidx(g, ux, uy, bz);
+  t.i[4] = vidx(g, bx, by, uz); t.i[5] = vidx(g, ux, by, uz); t.i[6] = vidx(g, bx, uy, uz); t.i[7] = vidx(g, ux, uy, uz);
+  return t;
+}
+__device__ __forceinline__ float tri_eval(const Tri& t, const float* __restrict__ f) {
+  float x00 = mixf(f[t.i[0]], f[t.i[1]], t.fx);
+  float x10 = mixf(f[t.i[2]], f[t.i[3]], t.fx);
+  float x01 = mixf(f[t.i[4]], f[t.i[5]], t.fx);
+  float x11 = mixf(f[t.i[6]], f[t.i[7]], t.fx);
+  return mixf(mixf(x00, x10, t.fy), mixf(x01, x11, t.fy), t.fz);
+}
+__device__ __forceinline__ F3 sample_u(const Grid& g, const Vec3& u, F3 p) {
+  Tri t = tri(g, p);
+  return F3{tri_eval(t, u.x), tri_eval(t, u.y), tri_eval(t, u.z)};
+}
+
+#define FX_BRICK_LOOP(list, count)                                       \
+  const uint32_t fx_n = *(count);                                        \
+  for (uint32_t fx_k = blockIdx.x; fx_k < fx_n; fx_k += gridDim.x)
+
+// ------------------------------------------------------------ emitters --
+// EmitterSource (SH/inject-source-common.wgsl.ts:3-12), 32 floats per source.
+struct Src {
+  const float* s;
+  __device__ float px() const { return s[0]; } __device__ float py() const { return s[1]; }
+  __device__ float pz() const { return s[2]; } __device__ float radius() const { return s[3]; }
+  __device__ float start() const { return s[4]; } __device__ float duration() const { return s[5]; }
+  __device__ float yd() const { return s[8]; } __device__ float yt() const { return s[9]; } __device__ float yf() const { return s[10]; }
+  __device__ float speed() const { return s[12]; } __device__ float mode() const { return s[13]; }
+  __device__ float falloff() const { return s[14]; } __device__ float vw() const { return s[15]; }
+  __device__ float dx_() const { return s[16]; } __device__ float dy_() const { return s[17]; } __device__ float dz_() const { return s[18]; }
+  __device__ float nscale() const { return s[20]; } __device__ float nmix() const { return s[21]; }
+  __device__ float intensity() const { return s[22]; }
+  __device__ float kind() const { return s[28]; } __device__ float seed() const { return s[29]; }
+};
+
+// SH/inject-source-common.wgsl.ts:102-121
+__device__ __forceinline__ float hash31(F3 p) { return fractf(sinf(dot3(p, f3(127.1f, 311.7f, 74.7f))) * 43758.5453123f); }
+__device__ float value_noise3(F3 p) {
+  F3 c = f3(floorf(p.x), floorf(p.y), floorf(p.z));
+  F3 f = f3(fractf(p.x), fractf(p.y), fractf(p.z));
+  F3 u = f3(f.x * f.x * (3.f - 2.f * f.x), f.y * f.y * (3.f - 2.f * f.y), f.z * f.z * (3.f - 2.f * f.z));
+  float h000 = hash31(c), h100 = hash31(c + f3(1, 0, 0)), h010 = hash31(c + f3(0, 1, 0)), h110 = hash31(c + f3(1, 1, 0));
+  float h001 = hash31(c + f3(0, 0, 1)), h101 = hash31(c + f3(1, 0, 1)), h011 = hash31(c + f3(0, 1, 1)), h111 = hash31(c + f3(1, 1, 1));
+  return mixf(mixf(mixf(h000, h100, u.x), mixf(h010, h110, u.x), u.y), mixf(mixf(h001, h101, u.x), mixf(h011, h111, u.x), u.y), u.z);
+}
+__device__ __forceinline__ float emitter_noise(F3 pos, float scale, float seed) {
+  return value_noise3(pos * fmaxf(scale, 0.1f) + f3(seed * 0.137f, seed * 0.271f, -seed * 0.193f));
+}
+__device__ __forceinline__ float active_gate(const fx_sim_params& P, Src s) {
+  float age = P.time - s.start();
+  return stepf(0.f, age) * stepf(age, s.duration());
+}
+__device__ __forceinline__ float source_duration(Src s) { return fmaxf(s.duration(), 0.0001f); }
+__device__ __forceinline__ float sphere_falloff(float dist, float radius, float falloff) {
+  float t = clampf(1.f - dist / fmaxf(radius, 0.0001f), 0.f, 1.f);
+  return mixf(stepf(0.f, t - 0.0001f), t * t, falloff);
+}
+__device__ __forceinline__ float source_feather(const fx_sim_params& P, float r) {
+  float vw = P.dx / fmaxf(P.world_size, 0.0001f);
+  return fmaxf(r * 0.24f, vw * 3.f);
+}
+__device__ __forceinline__ float outer_limit(const fx_sim_params& P, float r) { return r + source_feather(P, r); }
+__device__ float eruptive_falloff(const fx_sim_params& P, F3 pos, float dist, Src s, float falloff) {
+  float targetRadius = s.radius();
+  float growthSeconds = clampf(targetRadius * 5.5f, 0.1f, 0.75f);
+  float age = fmaxf(P.time - s.start(), 0.f);
+  float growth = smoothf(0.f, 1.f, clampf(age / growthSeconds, 0.f, 1.f));
+  float feather = source_feather(P, targetRadius);
+  float seed = s.seed();
+  float coarse = emitter_noise(pos + f3(age * 0.17f, -age * 0.11f, age * 0.07f), 6.f, seed);
+  float fine = emitter_noise(pos + f3(-age * 0.33f, age * 0.21f, -age * 0.15f), 18.f, seed + 41.f);
+  float lobe = clampf(coarse * 0.72f + fine * 0.28f, 0.f, 1.f);
+  float reach = targetRadius * growth * mixf(0.62f, 1.f, smoothf(0.2f, 0.92f, lobe));
+  float front = 1.f - smoothf(reach - feather, reach + feather, dist);
+  float coreRadius = fmaxf(targetRadius * (0.18f + growth * 0.38f), feather);
+  float core = sphere_falloff(dist, coreRadius, falloff) * (1.f - growth * 0.35f);
+  float body = sphere_falloff(dist, fmaxf(reach + feather, feather), falloff) * front;
+  return clampf(fmaxf(core, body) * mixf(0.72f, 1.2f, lobe), 0.f, 1.f);
+}
+__device__ F3 turbulent_dir(F3 pos, float seed) {  // SH/inject-velocity.wgsl.ts:39-44
+  float nx = emitter_noise(pos, 4.f, seed);
+  float ny = emitter_noise(pos, 4.f, seed + 17.3f);
+  float nz = emitter_noise(pos, 4.f, seed + 31.7f);
+  return norm3(f3(nx, ny, nz) - f3(0.5f, 0.5f, 0.5f));
+}
+
+// SH/inject-scalar.wgsl.ts:32-70, SH/inject-velocity.wgsl.ts:46-96 and
+// SH/inject-igniter.wgsl.ts:32-66 fused per voxel in dispatch order (each pass
+// touches only its own voxel, so fusing keeps the sequential result).
+__global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                const StepConst* __restrict__ C, const float* __restrict__ srcs,
+                                                Scalars S, Vec3 u, uint32_t* touched, int parts) {
+  const fx_sim_params& P = C->p;
+  const uint32_t ns = C->nsrc;
+  FX_BRICK_LOOP(list, count) {
+    const uint32_t b = list[fx_k];
+    if (touched && threadIdx.x == 0) touched[b] = 1u;
+    Vox v = vox_of(g, b);
+    if (!v.in) continue;
+    const uint32_t i = v.i;
+    F3 pos = f3((float(v.x) + 0.5f) / float(g.W), (float(v.y) + 0.5f) / float(g.H), (float(v.z) + 0.5f) / float(g.D));
+    if (parts & 1) {
+      float den = S.d[i], tem = S.t[i], fu = S.f[i];
+      for (uint32_t k = 0; k < ns; ++k) {
+        Src s{srcs + size_t(k) * FX_FLOATS_PER_EMITTER};
+        bool burst = s.kind() == 3.f;
+        if (!(s.kind() == 0.f || burst)) continue;
+        if (active_gate(P, s) < 0.0001f) continue;
+        F3 off = pos - f3(s.px(), s.py(), s.pz());
+        float dist = len3(off);
+        float radius = s.radius();
+        if (dist > (burst ? outer_limit(P, radius) : radius)) continue;
+        float weight = burst ? eruptive_falloff(P, pos, dist, s, 0.5f) : sphere_falloff(dist, radius, 0.5f);
+        float nm = clampf(s.nmix(), 0.f, 1.f);
+        float modulation = nm == 0.f ? 1.f : mixf(1.f, emitter_noise(pos, s.nscale(), s.seed()), nm);
+        float release = burst ? 1.f / source_duration(s) : 1.f;
+        float contribution = weight * modulation * release * P.delta_time;
+        den = clamp01(den + s.yd() * contribution);
+        tem = clamp01(tem + s.yt() * contribution);
+        fu = clamp01(fu + s.yf() * contribution);
+      }
+      S.d[i] = den; S.t[i] = tem; S.f[i] = fu;
+    }
+    if (parts & 2) {
+      F3 vel = f3(u.x[i], u.y[i], u.z[i]);
+      for (uint32_t k = 0; k < ns; ++k) {
+        Src s{srcs + size_t(k) * FX_FLOATS_PER_EMITTER};
+        bool burst = s.kind() == 3.f;
+        if (!(s.kind() == 1.f || burst)) continue;
+        if (active_gate(P, s) < 0.0001f) continue;
+        F3 off = pos - f3(s.px(), s.py(), s.pz());
+        float dist = len3(off);
+        float radius = s.radius();
+        if (dist > (burst ? outer_limit(P, radius) : radius)) continue;
+        float weight = burst ? eruptive_falloff(P, pos, dist, s, s.falloff()) : sphere_falloff(dist, radius, s.falloff());
+        float speed = s.speed();
+        float mode = s.mode();
+        F3 dir = f3(0, 0, 0);
+        F3 up = f3(0.f, 1.f, 0.f);
+        if (burst) {
+          F3 radial = dist > 0.0001f ? off / fmaxf(dist, 0.0001f) : up;
+          F3 turb = turbulent_dir(pos + f3(P.time * 0.19f, -P.time * 0.13f, P.time * 0.07f), s.seed() + 83.f);
+          F3 bv = radial * speed + f3(0.f, s.mode(), 0.f) + turb * s.vw();
+          F3 impulse = bv * weight * P.delta_time / source_duration(s);
+          vel = vel + impulse;
+          continue;
+        } else if (mode == 0.f) {
+          dir = dist > 0.0001f ? off / fmaxf(dist, 0.0001f) : up;
+        } else if (mode == 1.f) {
+          F3 dd = f3(s.dx_(), s.dy_(), s.dz_());
+          dir = dot3(dd, dd) > 0.0001f ? norm3(dd) : up;
+        } else {
+          dir = turbulent_dir(pos, s.seed());
+        }
+        F3 force = dir * speed * weight * P.delta_time;
+        float tight = clampf(s.vw(), 0.f, 1.f) * weight;
+        vel = mix3(vel + force, dir * speed, tight);
+      }
+      float lim = 80.f / fmaxf(P.dx, 0.001f);
+      u.x[i] = clampf(vel.x, -lim, lim); u.y[i] = clampf(vel.y, -lim, lim); u.z[i] = clampf(vel.z, -lim, lim);
+    }
+    if (parts & 4) {
+      float rea = S.r[i], tem = S.t[i];
+      const float fuel = S.f[i];
+      for (uint32_t k = 0; k < ns; ++k) {
+        Src s{srcs + size_t(k) * FX_FLOATS_PER_EMITTER};
+        bool burst = s.kind() == 3.f;
+        if (!(s.kind() == 2.f || burst)) continue;
+        if (active_gate(P, s) < 0.0001f) continue;
+        float dist = len3(pos - f3(s.px(), s.py(), s.pz()));
+        float radius = s.radius();
+        if (dist > (burst ? outer_limit(P, radius) : radius)) continue;
+        float weight = burst ? eruptive_falloff(P, pos, dist, s, 0.6f) : sphere_falloff(dist, radius, 0.6f);
+        float intensity = s.intensity();
+        float avail = burst ? 1.f : smoothf(0.02f, 0.22f, fuel);
+        float release = burst ? 1.f / source_duration(s) : 1.6f;
+        float seedv = intensity * weight * avail * P.delta_time * release;
+        rea = clamp01(rea + seedv);
+        tem = clamp01(tem + intensity * weight * P.delta_time * 4.f);
+      }
+      S.r[i] = rea; S.t[i] = tem;
+    }
+  }
+}
+
+// ----------------------------------------------------------- clearing --
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t value) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = value;
+}
+__global__ void k_mark(const uint32_t* __restrict__ list, uint32_t n, uint32_t* touched) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) touched[list[k]] = 1u;
+}
+__global__ void k_fill_f32(float* p, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = 0.f;
+}
+
+// ---------------------------------------------------------- activation --
+// REFERENCE: SH/compute-active-bricks.wgsl.ts:24-79 (one thread per brick).
+// THRESHOLD: density > a or |u| > b anywhere in the brick, dilation 1 (one
+// workgroup per brick). Candidates = listA (+ listB).
+__device__ __forceinline__ void dilate(const Grid& g, uint32_t b, int r, uint32_t* flags) {
+  int bx = int(b % g.BX), by = int((b / g.BX) % g.BY), bz = int(b / (g.BX * g.BY));
+  for (int z = -r; z <= r; ++z)
+    for (int y = -r; y <= r; ++y)
+      for (int x = -r; x <= r; ++x) {
+        int nx = min(max(bx + x, 0), int(g.BX) - 1);
+        int ny = min(max(by + y, 0), int(g.BY) - 1);
+        int nz = min(max(bz + z, 0), int(g.BZ) - 1);
+        flags[nx + g.BX * (ny + g.BY * nz)] = 1u;
+      }
+}
+__global__ __launch_bounds__(256) void k_activate_reference(const uint32_t* __restrict__ listA, const uint32_t* __restrict__ countA,
+                                                            const uint32_t* __restrict__ listB, const uint32_t* __restrict__ countB,
+                                                            Grid g, Scalars S, uint32_t* flags) {
+  const uint32_t na = *countA, nb = listB ? *countB : 0u;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na + nb; k += gridDim.x * blockDim.x) {
+    const uint32_t b = k < na ? listA[k] : listB[k - na];
+    uint32_t bx = b % g.BX, by = (b / g.BX) % g.BY, bz = b / (g.BX * g.BY);
+    uint32_t base[3] = {bx * 8, by * 8, bz * 8};
+    bool active = false;
+    float peak = 0.f;
+    for (uint32_t z = 0; z < 8; z += 2)
+      for (uint32_t y = 0; y < 8; y += 2)
+        for (uint32_t x = 0; x < 8; x += 2) {
+          uint32_t cx = base[0] + x, cy = base[1] + y, cz = base[2] + z;
+          if (cx < g.W && cy < g.H && cz < g.D) {
+            uint32_t i = vidx(g, cx, cy, cz);
+            float e = S.d[i] + S.t[i] * 0.7f + S.r[i] * 0.9f;
+            peak = fmaxf(peak, e);
+            if (e > 0.012f) active = true;
+          }
+        }
+    uint32_t ci = vidx(g, min(base[0] + 4, g.W - 1), min(base[1] + 4, g.H - 1), min(base[2] + 4, g.D - 1));
+    float ce = S.d[ci] + S.t[ci] * 0.7f + S.r[ci] * 0.9f;
+    peak = fmaxf(peak, ce);
+    if (ce > 0.012f) active = true;
+    if (!active) continue;
+    dilate(g, b, (peak > 0.05f || ce > 0.035f) ? 1 : 0, flags);
+  }
+}
+__global__ __launch_bounds__(512) void k_activate_threshold(const uint32_t* __restrict__ listA, const uint32_t* __restrict__ countA,
+                                                            const uint32_t* __restrict__ listB, const uint32_t* __restrict__ countB,
+                                                            Grid g, const StepConst* __restrict__ C, const float* __restrict__ den,
+                                                            Vec3 vel, uint32_t* flags) {
+  const uint32_t na = *countA, nb = listB ? *countB : 0u;
+  const float dth = C->act_density, sth = C->act_speed;
+  for (uint32_t k = blockIdx.x; k < na + nb; k += gridDim.x) {
+    const uint32_t b = k < na ? listA[k] : listB[k - na];
+    Vox v = vox_of(g, b);
+    int pred = 0;
+    if (v.in) {
+      F3 w = f3(vel.x[v.i], vel.y[v.i], vel.z[v.i]);
+      pred = (den[v.i] > dth || len3(w) > sth) ? 1 : 0;
+    }
+    if (__syncthreads_or(pred) && threadIdx.x == 0) dilate(g, b, 1, flags);
+  }
+}
+
+// ---------------------------------------------------------- compaction --
+// One workgroup of 1024 threads. For every level: flags (level 0 given; coarse
+// levels = OR of the 2x2x2 child bricks) -> ascending active list, page table
+// and the list of bricks that leave the set (to be zeroed). Each thread owns a
+// contiguous chunk of bricks, so the old page entry is read before it is
+// overwritten by the same thread.
+struct CompactArgs {
+  int nlevels;
+  Grid g[FX_MAX_LEVELS_DEV];
+  uint32_t* flags[FX_MAX_LEVELS_DEV];
+  BrickList L[FX_MAX_LEVELS_DEV];
+  uint32_t* touched;
+};
+
+__device__ void block_scan2(uint32_t a, uint32_t r, uint32_t* ea, uint32_t* er, uint32_t* ta, uint32_t* tr) {
+  __shared__ uint32_t sa[1024], sr[1024];
+  const uint32_t t = threadIdx.x;
+  sa[t] = a; sr[t] = r;
+  __syncthreads();
+  for (uint32_t off = 1; off < 1024; off <<= 1) {
+    uint32_t va = t >= off ? sa[t - off] : 0u, vr = t >= off ? sr[t - off] : 0u;
+    __syncthreads();
+    sa[t] += va; sr[t] += vr;
+    __syncthreads();
+  }
+  *ea = sa[t] - a; *er = sr[t] - r;
+  *ta = sa[1023]; *tr = sr[1023];
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(1024) void k_compact(CompactArgs A) {
+  const uint32_t t = threadIdx.x;
+  for (int l = 0; l < A.nlevels; ++l) {
+    const Grid g = A.g[l];
+    const uint32_t nb = g.BX * g.BY * g.BZ;
+    uint32_t* flags = A.flags[l];
+    if (l > 0) {
+      const Grid f = A.g[l - 1];
+      const uint32_t* ff = A.flags[l - 1];
+      for (uint32_t b = t; b < nb; b += 1024) {
+        uint32_t x = b % g.BX, y = (b / g.BX) % g.BY, z = b / (g.BX * g.BY);
+        uint32_t a = 0;
+        for (uint32_t k = 0; k < 2; ++k)
+          for (uint32_t j = 0; j < 2; ++j)
+            for (uint32_t i = 0; i < 2; ++i) {
+              uint32_t fx = 2 * x + i, fy = 2 * y + j, fz = 2 * z + k;
+              if (fx < f.BX && fy < f.BY && fz < f.BZ) a |= ff[fx + f.BX * (fy + f.BY * fz)];
+            }
+        flags[b] = a ? 1u : 0u;
+      }
+      __syncthreads();
+    }
+    const uint32_t chunk = (nb + 1023) / 1024;
+    const uint32_t lo = min(t * chunk, nb), hi = min(lo + chunk, nb);
+    BrickList L = A.L[l];
+    uint32_t ca = 0, cr = 0;
+    for (uint32_t b = lo; b < hi; ++b) {
+      bool on = flags[b] != 0u;
+      bool was = L.page[b] >= 0 || (l == 0 && A.touched[b] != 0u);
+      ca += on ? 1u : 0u;
+      cr += (was && !on) ? 1u : 0u;
+    }
+    uint32_t ea, er, ta, tr;
+    block_scan2(ca, cr, &ea, &er, &ta, &tr);
+    for (uint32_t b = lo; b < hi; ++b) {
+      bool on = flags[b] != 0u;
+      bool was = L.page[b] >= 0 || (l == 0 && A.touched[b] != 0u);
+      if (on) { L.list[ea] = b; L.page[b] = int32_t(ea); ++ea; } else { L.page[b] = -1; }
+      if (was && !on) L.retire[er++] = b;
+      if (l == 0) A.touched[b] = on ? 1u : 0u;
+    }
+    if (t == 0) { *L.count = ta; *L.retire_count = tr; }
+    __syncthreads();
+  }
+}
+
+// Zero `nbufs` buffers in every brick of the list.
+struct ZeroArgs { float* b[FX_MAX_ZERO_BUFS]; int n; };
+__global__ __launch_bounds__(512) void k_zero_bricks(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, ZeroArgs Z) {
+  FX_BRICK_LOOP(list, count) {
+    const uint32_t i = (list[fx_k] << 9) | threadIdx.x;
+    for (int k = 0; k < Z.n; ++k) Z.b[k][i] = 0.f;
+  }
+}
+
+// ----------------------------------------------------------- advection --
+// SH/advect-velocity-semilagrangian.wgsl.ts:23-41
+__global__ __launch_bounds__(512) void k_advect_velocity(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                         const StepConst* __restrict__ C, Vec3 u, Vec3 us) {
+  const float dt = C->p.delta_time;
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    if (on_boundary(g, v.x, v.y, v.z)) { us.x[v.i] = 0.f; us.y[v.i] = 0.f; us.z[v.i] = 0.f; continue; }
+    F3 coord = f3(float(v.x) + 0.5f, float(v.y) + 0.5f, float(v.z) + 0.5f);
+    F3 half = f3(0.5f, 0.5f, 0.5f);
+    F3 back = coord - sample_u(g, u, coord - half) * dt;
+    F3 adv = sample_u(g, u, back - half) * (1.f - dt * 0.025f);
+    us.x[v.i] = adv.x; us.y[v.i] = adv.y; us.z[v.i] = adv.z;
+  }
+}
+
+// SH/advect-scalar-maccormack.wgsl.ts:52-57 (and semilagrangian :69-74)
+__device__ __forceinline__ float thermal_noise(F3 pos, float time) {
+  F3 p = pos + f3(time * 1.7f, -time * 3.1f, time * 1.1f);
+  float a = 0.5f + 0.5f * sinf(dot3(p, f3(0.097f, 0.061f, -0.083f)));
+  float b = 0.5f + 0.5f * sinf(dot3(p, f3(-0.043f, 0.137f, 0.071f)) + sinf(p.y * 0.079f) * 1.6f);
+  return clampf(a * 0.58f + b * 0.42f, 0.f, 1.f);
+}
+// SH/advect-scalar-maccormack.wgsl.ts:75-100
+__device__ __forceinline__ void decay_write(const Grid& g, uint32_t i, uint32_t y, float den, float tem, float fu, float rea,
+                                            float tn, float dt, Scalars D) {
+  float ny = (float(y) + 0.5f) / float(g.H);
+  float hot = smoothf(0.22f, 0.78f, tem + fu * 0.2f + rea * 0.26f);
+  float coolingPocket = smoothf(0.44f, 0.9f, 1.f - tn) * smoothf(0.08f, 0.74f, den) * (1.f - hot * 0.72f);
+  float cooling = dt * (0.016f + ny * 0.052f) * mixf(1.28f, 0.58f, hot) * mixf(0.86f, 1.9f, coolingPocket);
+  float smokeLoss = dt * (0.008f + ny * 0.026f) * mixf(0.75f, 1.38f, tn);
+  float coldBreakup = dt * den * (1.f - smoothf(0.2f, 0.86f, hot)) * smoothf(0.18f, 0.72f, den) * smoothf(0.34f, 0.92f, tn) * 0.34f;
+  float clearing = dt * den * hot * mixf(0.1f, 0.72f, tn);
+  D.d[i] = clamp01(fmaxf(0.f, den * (1.f - smokeLoss) - clearing - coldBreakup));
+  D.t[i] = clamp01(fmaxf(0.f, tem - cooling));
+  D.f[i] = clamp01(fmaxf(0.f, fu - dt * mixf(0.012f, 0.034f, tn)));
+  D.r[i] = clamp01(fmaxf(0.f, rea - dt * mixf(0.46f, 0.82f, tn)));
+}
+// SH/advect-scalar-maccormack.wgsl.ts:33-101 / SH/advect-scalar-semilagrangian.wgsl.ts:26-44.
+// The four advect<Name> calls share every position; sampleVelocity(coord-0.5)
+// and the `original` sample hit the voxel exactly (fraction 0).
+__global__ __launch_bounds__(512) void k_advect_scalars(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                        const StepConst* __restrict__ C, int mode, Scalars S, Vec3 us, Scalars D,
+                                                        const uint32_t* __restrict__ flags) {
+  const float dt = C->p.delta_time, time = C->p.time;
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    const uint32_t i = v.i;
+    if (!flag_on(g, flags, v.x, v.y, v.z)) { D.d[i] = 0.f; D.t[i] = 0.f; D.f[i] = 0.f; D.r[i] = 0.f; continue; }
+    F3 coord = f3(float(v.x) + 0.5f, float(v.y) + 0.5f, float(v.z) + 0.5f);
+    F3 half = f3(0.5f, 0.5f, 0.5f);
+    F3 velocity = f3(us.x[i], us.y[i], us.z[i]);
+    F3 back = coord - velocity * dt;
+    F3 sp = back - half;
+    Tri tp = tri(g, sp);
+    if (mode == FX_ADVECT_SEMI_LAGRANGIAN) {
+      decay_write(g, i, v.y, tri_eval(tp, S.d), tri_eval(tp, S.t), tri_eval(tp, S.f), tri_eval(tp, S.r), thermal_noise(sp, time), dt, D);
+      continue;
+    }
+    F3 fwd = sample_u(g, us, sp);
+    F3 rec = back + fwd * dt;
+    Tri tr = tri(g, rec - half);
+    float out[4];
+    const float* fs[4] = {S.d, S.t, S.f, S.r};
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      float predicted = tri_eval(tp, fs[f]);
+      float recovered = tri_eval(tr, fs[f]);
+      float original = fs[f][i];
+      float corrected = predicted + 0.5f * (original - recovered);
+      out[f] = clampf(corrected, fminf(original, predicted), fmaxf(original, predicted));
+    }
+    decay_write(g, i, v.y, out[0], out[1], out[2], out[3], thermal_noise(coord, time), dt, D);
+  }
+}
+
+// ---------------------------------------------------------- combustion --
+// SH/combustion-update.wgsl.ts:27-93 with read-old semantics: advected set ->
+// render set; voxels outside active bricks are copied.
+__device__ __forceinline__ float thermal_cluster(F3 c, float time) {
+  F3 p = c + f3(time * 2.2f, time * 5.2f, -time * 1.7f);
+  float body = 0.5f + 0.5f * sinf(dot3(p, f3(0.071f, 0.093f, -0.064f)) + sinf(p.y * 0.057f) * 1.8f);
+  float pocket = 0.5f + 0.5f * sinf(dot3(p, f3(0.143f, -0.081f, 0.119f)) + sinf(p.x * 0.11f + p.z * 0.07f) * 1.2f);
+  float rising = 0.5f + 0.5f * sinf(p.x * 0.18f + p.y * 0.13f - p.z * 0.16f + time * 1.4f);
+  return clampf(body * 0.46f + pocket * 0.34f + rising * 0.2f, 0.f, 1.f);
+}
+__global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                    const StepConst* __restrict__ C, Scalars S, Scalars D,
+                                                    const uint32_t* __restrict__ flags) {
+  const float dt = C->p.delta_time, time = C->p.time;
+  const float ilo = C->ignition_lo, ihi = C->ignition_hi, rate = C->burn_rate;
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
+    if (!flag_on(g, flags, x, y, z)) { D.d[i] = S.d[i]; D.t[i] = S.t[i]; D.f[i] = S.f[i]; D.r[i] = S.r[i]; continue; }
+    auto hotgas = [&](uint32_t a, uint32_t b2, uint32_t c2) {
+      uint32_t k = cidx(g, a, b2, c2);
+      return clampf(S.t[k] * 1.55f + S.f[k] * 0.26f + S.r[k] * 0.44f, 0.f, 1.f);
+    };
+    float ny = (float(y) + 0.5f) / float(g.H);
+    float cluster = thermal_cluster(f3(float(x), float(y), float(z)), time);
+    float nh = (hotgas(incu(x, g.W - 1), y, z) + hotgas(decu(x), y, z) + hotgas(x, incu(y, g.H - 1), z) + hotgas(x, decu(y), z) +
+                hotgas(x, y, incu(z, g.D - 1)) + hotgas(x, y, decu(z))) * 0.16666667f;
+    float localBefore = hotgas(x, y, z);
+    float den = S.d[i], tem = S.t[i], fu = S.f[i], rea = S.r[i];
+    float radiative = fmaxf(nh - localBefore * 0.55f, 0.f) * smoothf(0.04f, 0.76f, den) * (0.45f + (1.f - cluster) * 0.55f) * dt * 2.6f;
+    float hotPocket = smoothf(0.58f, 0.92f, cluster + rea * 0.22f);
+    float coolingPocket = smoothf(0.42f, 0.82f, 1.f - cluster) * smoothf(0.08f, 0.68f, den) * (1.f - smoothf(0.54f, 0.94f, rea));
+    float ignition = smoothf(ilo, ihi, tem);
+    float burn = fminf(fu, fu * ignition * dt * mixf(4.2f, 8.6f, hotPocket) * rate);
+    float hotSoot = smoothf(0.24f, 0.88f, tem + rea * 0.2f) * smoothf(0.36f, 0.9f, 1.f - cluster + den * 0.18f);
+    float smokeYield = burn * mixf(0.16f, 0.58f, coolingPocket + (1.f - ignition) * 0.5f + hotSoot * 0.45f);
+    float hotCooling = mixf(1.3f, 0.62f, hotPocket);
+    float heatLoss = dt * (0.018f + ny * 0.052f) * hotCooling * mixf(1.f, 2.25f, coolingPocket);
+    float densityLoss = dt * den * (0.02f + ny * 0.07f);
+    float clearing = den * smoothf(0.18f, 0.82f, tem + rea * 0.22f + fu * 0.1f) * dt * mixf(0.55f, 2.45f, hotPocket);
+    float coolingSmoke = den * tem * (coolingPocket + hotSoot * 0.32f) * dt * 0.16f;
+    float reactionDecay = rea * (1.f - dt * mixf(0.68f, 0.22f, hotPocket));
+    D.f[i] = clamp01(fu - burn);
+    D.t[i] = clamp01(tem + burn * mixf(2.1f, 3.25f, hotPocket) + radiative - heatLoss);
+    D.d[i] = clamp01(fmaxf(0.f, den + smokeYield + coolingSmoke - densityLoss - clearing * 0.72f));
+    D.r[i] = clamp01(fmaxf(reactionDecay, burn * mixf(4.f, 7.5f, hotPocket) + radiative * 1.8f));
+  }
+}
+
+// ------------------------------------------------------------ buoyancy --
+// SH/apply-buoyancy.wgsl.ts:27-110 (render set, velocityScratch in place).
+__global__ __launch_bounds__(512) void k_buoyancy(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                  const StepConst* __restrict__ C, Scalars S, Vec3 us, const uint32_t* __restrict__ flags) {
+  const fx_sim_params& P = C->p;
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
+    if (!flag_on(g, flags, x, y, z)) continue;
+    auto hot = [&](uint32_t a, uint32_t b2, uint32_t c2) {
+      uint32_t k = cidx(g, a, b2, c2);
+      return clampf(S.t[k] * 1.f + S.r[k] * 0.85f, 0.f, 1.f);
+    };
+    auto rd = [&](uint32_t a, uint32_t b2, uint32_t c2) { return S.d[cidx(g, a, b2, c2)]; };
+    const float dt = P.delta_time;
+    const float den = S.d[i];
+    float response = P.buoyancy * (1.f + smoothf(6.f, 24.f, fabsf(P.buoyancy)) * 0.38f);
+    float hotGas = hot(x, y, z);
+    float lift = hotGas * response;
+    F3 gdir = f3(P.gravity[0], P.gravity[1], P.gravity[2]);
+    F3 gravityForce = gdir * den * P.gravity[3];
+    float entrain = den * smoothf(0.04f, 0.3f, hotGas);
+    float ny = (float(y) + 0.5f) / float(g.H);
+    float shear = sinf(float(y) * 0.073f + P.time * 0.9f) * 0.35f + sinf(float(x) * 0.041f - float(z) * 0.052f + P.time * 0.43f) * 0.22f;
+    float windAmount = P.wind[3] * (0.18f + den * 0.55f + ny * 0.45f);
+    F3 wdir = f3(P.wind[0], P.wind[1], P.wind[2]);
+    F3 wind = wdir * windAmount + f3(-P.wind[2], 0.f, P.wind[0]) * windAmount * shear;
+    F3 tg = f3(hot(incu(x, g.W - 1), y, z) - hot(decu(x), y, z), hot(x, incu(y, g.H - 1), z) - hot(x, decu(y), z),
+               hot(x, y, incu(z, g.D - 1)) - hot(x, y, decu(z))) * 0.5f;
+    F3 dg = f3(rd(incu(x, g.W - 1), y, z) - rd(decu(x), y, z), rd(x, incu(y, g.H - 1), z) - rd(x, decu(y), z),
+               rd(x, y, incu(z, g.D - 1)) - rd(x, y, decu(z))) * 0.5f;
+    float dgl = fmaxf(len3(dg), 0.0001f);
+    float coldSmoke = den * (1.f - smoothf(0.12f, 0.72f, hotGas));
+    float surfaceBreakup = smoothf(0.008f, 0.06f, dgl) * coldSmoke;
+    F3 shred = norm3(f3(sinf(float(y) * 0.117f + P.time * 1.9f), sinf(float(x) * 0.071f - float(z) * 0.083f + P.time * 1.35f) * 0.35f,
+                        cosf(float(x) * 0.061f + float(y) * 0.049f - P.time * 1.6f)));
+    float collapse = den * (1.f - smoothf(0.16f, 0.64f, hotGas));
+    float smokeDrag = den * smoothf(0.03f, 0.45f, hotGas);
+    F3 vel = f3(us.x[i], us.y[i], us.z[i]) * (1.f - dt * 0.08f);
+    vel = vel + wind * dt;
+    vel = vel + gravityForce * dt;
+    vel = vel + tg * collapse * dt * (3.2f + fabsf(P.buoyancy) * 0.08f);
+    vel = vel + (-dg / dgl) * surfaceBreakup * dt * 2.8f;
+    vel = vel + shred * coldSmoke * dt * (0.72f + surfaceBreakup * 2.4f);
+    vel = vel + wdir * coldSmoke * P.wind[3] * dt * 0.9f;
+    vel = vel + f3(0.f, smokeDrag * response * 0.55f, 0.f) * dt;
+    float lim = 80.f / fmaxf(P.dx, 0.001f);
+    vel.y = clampf(vel.y + (lift + entrain * response * 0.24f) * dt, -lim, lim);
+    if (x == 0 || x == g.W - 1 || z == 0 || z == g.D - 1) { vel.x = 0.f; vel.z = 0.f; }
+    if (y == 0) vel.y = fmaxf(vel.y, 0.f);
+    us.x[i] = vel.x; us.y[i] = vel.y; us.z[i] = vel.z;
+  }
+}
+
+// ------------------------------------------------ vorticity/confinement --
+// SH/compute-vorticity.wgsl.ts:23-50
+__global__ __launch_bounds__(512) void k_vorticity(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g, Vec3 us,
+                                                   Vec3 w, float* wmag) {
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
+    if (on_boundary(g, x, y, z)) { w.x[i] = 0.f; w.y[i] = 0.f; w.z[i] = 0.f; wmag[i] = 0.f; continue; }
+    auto rv = [&](uint32_t a, uint32_t b2, uint32_t c2) { uint32_t k = cidx(g, a, b2, c2); return f3(us.x[k], us.y[k], us.z[k]); };
+    F3 x0 = rv(decu(x), y, z), x1 = rv(incu(x, g.W - 1), y, z);
+    F3 y0 = rv(x, decu(y), z), y1 = rv(x, incu(y, g.H - 1), z);
+    F3 z0 = rv(x, y, decu(z)), z1 = rv(x, y, incu(z, g.D - 1));
+    F3 curl = f3((y1.z - y0.z) - (z1.y - z0.y), (z1.x - z0.x) - (x1.z - x0.z), (x1.y - x0.y) - (y1.x - y0.x)) * 0.5f;
+    w.x[i] = curl.x; w.y[i] = curl.y; w.z[i] = curl.z;
+    wmag[i] = len3(curl);
+  }
+}
+// SH/apply-vorticity-confinement.wgsl.ts:30-66 (render set)
+__global__ __launch_bounds__(512) void k_confinement(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                     const StepConst* __restrict__ C, Vec3 w, const float* __restrict__ wmag, Vec3 us,
+                                                     float* cmag, Scalars S) {
+  const fx_sim_params& P = C->p;
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
+    if (on_boundary(g, x, y, z)) { cmag[i] = 0.f; continue; }
+    auto m = [&](uint32_t a, uint32_t b2, uint32_t c2) { return wmag[cidx(g, a, b2, c2)]; };
+    F3 grad = f3(m(incu(x, g.W - 1), y, z) - m(decu(x), y, z), m(x, incu(y, g.H - 1), z) - m(x, decu(y), z),
+                 m(x, y, incu(z, g.D - 1)) - m(x, y, decu(z))) * 0.5f;
+    F3 dir = grad / fmaxf(len3(grad), 0.0001f);
+    float hotGas = clampf(S.t[i] + S.r[i] * 0.85f, 0.f, 1.f);
+    float mask = clampf(P.vorticity_mask[0] + smoothf(0.02f, 0.38f, wmag[i]) * P.vorticity_mask[1] +
+                            smoothf(0.04f, 0.7f, hotGas) * P.vorticity_mask[2] + smoothf(0.04f, 0.65f, S.d[i]) * P.vorticity_mask[3],
+                        0.f, 4.f);
+    F3 force = cross3(dir, f3(w.x[i], w.y[i], w.z[i])) * P.vorticity_strength * mask * 1.45f;
+    F3 vel = f3(us.x[i], us.y[i], us.z[i]) + force * P.delta_time;
+    float lim = 80.f / fmaxf(P.dx, 0.001f);
+    us.x[i] = clampf(vel.x, -lim, lim); us.y[i] = clampf(vel.y, -lim, lim); us.z[i] = clampf(vel.z, -lim, lim);
+    cmag[i] = len3(force);
+  }
+}
+
+// ------------------------------------------------ divergence / pressure --
+// SH/compute-divergence.wgsl.ts:22-42
+__global__ __launch_bounds__(512) void k_divergence(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g, Vec3 us,
+                                                    float* div) {
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
+    if (on_boundary(g, x, y, z)) { div[i] = 0.f; continue; }
+    float dx = us.x[cidx(g, incu(x, g.W - 1), y, z)] - us.x[cidx(g, decu(x), y, z)];
+    float dy = us.y[cidx(g, x, incu(y, g.H - 1), z)] - us.y[cidx(g, x, decu(y), z)];
+    float dz = us.z[cidx(g, x, y, incu(z, g.D - 1))] - us.z[cidx(g, x, y, decu(z))];
+    div[i] = (dx + dy + dz) * 0.5f;
+  }
+}
+__device__ __forceinline__ float nsum(const Grid& g, const float* __restrict__ s, uint32_t x, uint32_t y, uint32_t z) {
+  return s[cidx(g, incu(x, g.W - 1), y, z)] + s[cidx(g, decu(x), y, z)] + s[cidx(g, x, incu(y, g.H - 1), z)] +
+         s[cidx(g, x, decu(y), z)] + s[cidx(g, x, y, incu(z, g.D - 1))] + s[cidx(g, x, y, decu(z))];
+}
+// SH/pressure-jacobi.wgsl.ts:15-36 (weighted: mix(p, jacobi, omega))
+__global__ __launch_bounds__(512) void k_jacobi(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                const StepConst* __restrict__ C, const float* __restrict__ rhs,
+                                                const float* __restrict__ src, float* __restrict__ dst, int weighted) {
+  const float om = C->omega;
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    if (on_boundary(g, v.x, v.y, v.z)) { dst[v.i] = 0.f; continue; }
+    float pj = (nsum(g, src, v.x, v.y, v.z) - rhs[v.i]) / 6.f;
+    dst[v.i] = weighted ? mixf(src[v.i], pj, om) : pj;
+  }
+}
+// SH/pressure-residual.wgsl.ts:16-38 (+ FP64 sum of squares when norm2 != null)
+__global__ __launch_bounds__(512) void k_residual(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                  const float* __restrict__ rhs, const float* __restrict__ p, float* __restrict__ r,
+                                                  double* norm2) {
+  __shared__ double part[8];
+  double acc = 0.0;
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    if (on_boundary(g, v.x, v.y, v.z)) { r[v.i] = 0.f; continue; }
+    float lap = nsum(g, p, v.x, v.y, v.z) - p[v.i] * 6.f;
+    float res = rhs[v.i] - lap;
+    r[v.i] = res;
+    acc += double(res) * double(res);
+  }
+  if (!norm2) return;
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 0; k < 8; ++k) s += part[k];
+    atomicAdd(norm2, s);
+  }
+}
+// SH/pressure-restriction.wgsl.ts:27-50 (8-child mean) or (1,3,3,1)/8 x 4 (V-cycle).
+__global__ __launch_bounds__(512) void k_restrict(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid f, Grid c,
+                                                  const float* __restrict__ src, float* __restrict__ dst, int full) {
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(c, list[fx_k]);
+    if (!v.in) continue;
+    if (on_boundary(c, v.x, v.y, v.z)) { dst[v.i] = 0.f; continue; }
+    float sum = 0.f;
+    if (!full) {
+      for (uint32_t k = 0; k < 2; ++k)
+        for (uint32_t j = 0; j < 2; ++j)
+          for (uint32_t a = 0; a < 2; ++a) sum += src[cidx(f, 2 * v.x + a, 2 * v.y + j, 2 * v.z + k)];
+      dst[v.i] = sum * 0.125f;
+    } else {
+      const float wt[4] = {0.125f, 0.375f, 0.375f, 0.125f};
+      for (int k = 0; k < 4; ++k) {
+        uint32_t fz = uint32_t(min(max(int(2 * v.z) - 1 + k, 0), int(f.D) - 1));
+        for (int j = 0; j < 4; ++j) {
+          uint32_t fy = uint32_t(min(max(int(2 * v.y) - 1 + j, 0), int(f.H) - 1));
+          for (int a = 0; a < 4; ++a) {
+            uint32_t fx = uint32_t(min(max(int(2 * v.x) - 1 + a, 0), int(f.W) - 1));
+            sum += (wt[k] * wt[j] * wt[a]) * src[vidx(f, fx, fy, fz)];
+          }
+        }
+      }
+      dst[v.i] = sum * 4.f;
+    }
+  }
+}
+// SH/pressure-prolongation.wgsl.ts:29-44
+__global__ __launch_bounds__(512) void k_prolong(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid c, Grid f,
+                                                 const float* __restrict__ src, float* __restrict__ dst) {
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(f, list[fx_k]);
+    if (!v.in) continue;
+    if (on_boundary(f, v.x, v.y, v.z)) { dst[v.i] = 0.f; continue; }
+    F3 cp = f3((float(v.x) + 0.5f) * 0.5f - 0.5f, (float(v.y) + 0.5f) * 0.5f - 0.5f, (float(v.z) + 0.5f) * 0.5f - 0.5f);
+    Tri t = tri(c, cp);
+    dst[v.i] += tri_eval(t, src);
+  }
+}
+// SH/project-velocity.wgsl.ts:21-43
+__global__ __launch_bounds__(512) void k_projection(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g, Vec3 us,
+                                                    const float* __restrict__ p, Vec3 u) {
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
+    if (on_boundary(g, x, y, z)) { u.x[i] = 0.f; u.y[i] = 0.f; u.z[i] = 0.f; continue; }
+    float gx = (p[cidx(g, incu(x, g.W - 1), y, z)] - p[cidx(g, decu(x), y, z)]) * 0.5f;
+    float gy = (p[cidx(g, x, incu(y, g.H - 1), z)] - p[cidx(g, x, decu(y), z)]) * 0.5f;
+    float gz = (p[cidx(g, x, y, incu(z, g.D - 1))] - p[cidx(g, x, y, decu(z))]) * 0.5f;
+    u.x[i] = us.x[i] - gx; u.y[i] = us.y[i] - gy; u.z[i] = us.z[i] - gz;
+  }
+}
+// SH/compute-debug-fields.wgsl.ts:18-25
+__global__ __launch_bounds__(512) void k_debug(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g, Vec3 u,
+                                               float* umag) {
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    umag[v.i] = len3(f3(u.x[v.i], u.y[v.i], u.z[v.i]));
+  }
+}
+
+// ------------------------------------------------------------ raymarch --
+// src/engine/render/passes/createVolumeRaymarchPass.ts:361-724, one thread per
+// pixel, row 0 = top; discarded pixels are written as 0 (cleared target).
+__device__ __forceinline__ F3 fire_palette(float t) {
+  F3 ember = f3(0.18f, 0.018f, 0.f), red = f3(1.f, 0.075f, 0.012f), orange = f3(1.f, 0.42f, 0.035f), white = f3(1.f, 0.86f, 0.56f);
+  float rm = smoothf(0.01f, 0.11f, t), om = smoothf(0.09f, 0.38f, t), wm = smoothf(0.58f, 0.92f, t);
+  return mix3(mix3(mix3(ember, red, rm), orange, om), white, wm);
+}
+__device__ __forceinline__ F3 smoke_albedo(float den, float tem, float hot) {
+  float warm = clampf(tem * 0.38f, 0.f, 1.f);
+  float soot = smoothf(0.16f, 0.9f, den) * (1.f - smoothf(0.2f, 0.74f, hot));
+  return mix3(mix3(f3(0.26f, 0.275f, 0.295f), f3(0.04f, 0.041f, 0.045f), soot * 0.86f), f3(0.36f, 0.27f, 0.18f), warm * (0.35f + hot * 0.45f));
+}
+__device__ __forceinline__ float phase_hg(float c, float gg) {
+  float g2 = gg * gg;
+  float denom = powf(fmaxf(1.f + g2 - 2.f * gg * c, 0.001f), 1.5f);
+  return (1.f - g2) / (12.566370614359172f * denom);
+}
+
+__global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, Scalars S, const uint32_t* __restrict__ flags, uint16_t* out) {
+  const uint32_t px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (px >= R.width || py >= R.height) return;
+  __half* o = reinterpret_cast<__half*>(out) + (size_t(py) * R.width + px) * 4;
+  const F3 dims = f3(float(g.W) - 1.f, float(g.H) - 1.f, float(g.D) - 1.f);
+  const F3 camPos = f3(R.cam_pos[0], R.cam_pos[1], R.cam_pos[2]);
+  float uvx = (float(px) + 0.5f) / float(R.width), uvy = 1.f - (float(py) + 0.5f) / float(R.height);
+  float ndcx = uvx * 2.f - 1.f, ndcy = uvy * 2.f - 1.f;
+  F3 rayDir = norm3(f3(R.fwd[0], R.fwd[1], R.fwd[2]) + f3(R.right[0], R.right[1], R.right[2]) * ndcx * R.aspect * R.tan_half +
+                    f3(R.up[0], R.up[1], R.up[2]) * ndcy * R.tan_half);
+  F3 bmin = f3(R.box_min[0], R.box_min[1], R.box_min[2]), bmax = f3(R.box_max[0], R.box_max[1], R.box_max[2]);
+  F3 inv = f3(1.f / rayDir.x, 1.f / rayDir.y, 1.f / rayDir.z);
+  F3 t0 = mul3(bmin - camPos, inv), t1 = mul3(bmax - camPos, inv);
+  float bx = fmaxf(fminf(t0.x, t1.x), fmaxf(fminf(t0.y, t1.y), fminf(t0.z, t1.z)));
+  float by = fminf(fmaxf(t0.x, t1.x), fminf(fmaxf(t0.y, t1.y), fmaxf(t0.z, t1.z)));
+  const __half zero = __float2half(0.f);
+  if (bx > by) { o[0] = zero; o[1] = zero; o[2] = zero; o[3] = zero; return; }
+  bx = fmaxf(bx, 0.f);
+  F3 invExt = f3(1.f / (R.half_ext[0] * 2.f), 1.f / (R.half_ext[1] * 2.f), 1.f / (R.half_ext[2] * 2.f));
+  F3 vrd = mul3(mul3(rayDir, invExt), dims);
+  const uint32_t stepCount = R.step_count;
+  float voxDelta = len3(vrd) * (by - bx) / fmaxf(float(stepCount), 1.f);
+  float worldDelta = (by - bx) / fmaxf(float(stepCount), 1.f);
+  uint32_t nl = uint32_t(clampf(R.light_count, 0.f, 3.f));
+  auto lightDir = [&](int k, F3 wp) {
+    if (R.light_kind[k] > 0.5f) return norm3(f3(R.light_pos[k][0], R.light_pos[k][1], R.light_pos[k][2]) - wp);
+    F3 raw = f3(R.light_dir[k][0], R.light_dir[k][1], R.light_dir[k][2]);
+    return norm3(dot3(raw, raw) > 0.0001f ? raw : f3(-0.34f, 0.88f, 0.31f));
+  };
+  auto lightAtt = [&](int k, F3 wp) {
+    if (R.light_kind[k] <= 0.5f) return 1.f;
+    float dist = len3(f3(R.light_pos[k][0], R.light_pos[k][1], R.light_pos[k][2]) - wp);
+    return 1.f / (1.f + dist * 0.09f + dist * dist * 0.02f);
+  };
+  F3 primary = nl > 0 ? lightDir(0, camPos) : f3(0.f, 1.f, 0.f);
+  F3 lightStep = mul3(primary, dims) * 0.036f;
+  F3 vp = mul3(mul3(camPos + rayDir * bx - bmin, invExt), dims);
+  float travelled = 0.f;
+  float rayLen = len3(vrd) * (by - bx);
+  F3 acc = f3(0, 0, 0);
+  float alpha = 0.f;
+  F3 nrd = norm3(vrd);
+  const uint32_t mxu = uint32_t(dims.x), myu = uint32_t(dims.y), mzu = uint32_t(dims.z);
+  for (uint32_t k = 0; k < stepCount && travelled <= rayLen; ++k) {
+    if (alpha > 0.985f) break;
+    float vstep = voxDelta;
+    F3 sp = vp;
+    F3 uvw = f3(sp.x / dims.x, sp.y / dims.y, sp.z / dims.z);
+    uint32_t ax = uint32_t(floorf(clampf(sp.x, 0.f, dims.x))), ay = uint32_t(floorf(clampf(sp.y, 0.f, dims.y))),
+             az = uint32_t(floorf(clampf(sp.z, 0.f, dims.z)));
+    if (flag_on(g, flags, ax, ay, az)) {
+      Tri t = tri(g, sp);
+      float den = fmaxf(tri_eval(t, S.d) - 0.004f, 0.f);
+      float tem = tri_eval(t, S.t), fu = tri_eval(t, S.f), rea = tri_eval(t, S.r);
+      float hot = clampf(tem * 1.45f + fu * 0.22f + rea * 0.38f, 0.f, 1.f);
+      if (den > 0.001f || hot > 0.018f) {
+        if (den < 0.012f && hot < 0.04f) {
+          vstep = voxDelta * 1.8f;
+        } else {
+          uint32_t cx = min(uint32_t(fmaxf(rintf(sp.x), 0.f)), mxu), cy = min(uint32_t(fmaxf(rintf(sp.y), 0.f)), myu),
+                   cz = min(uint32_t(fmaxf(rintf(sp.z), 0.f)), mzu);
+          F3 dg = f3(S.d[vidx(g, min(cx + 1, mxu), cy, cz)] - S.d[vidx(g, cx == 0 ? 0 : cx - 1, cy, cz)],
+                     S.d[vidx(g, cx, min(cy + 1, myu), cz)] - S.d[vidx(g, cx, cy == 0 ? 0 : cy - 1, cz)],
+                     S.d[vidx(g, cx, cy, min(cz + 1, mzu))] - S.d[vidx(g, cx, cy, cz == 0 ? 0 : cz - 1)]);
+          float gl = fmaxf(len3(dg), 0.0001f);
+          F3 normal = gl > 0.00011f ? (-dg) / gl : f3(0.f, 1.f, 0.f);
+          float detail = clampf(gl * 5.2f + hot * 0.28f + rea * 0.18f, 0.f, 1.f);
+          float dd = fmaxf(powf(den, 0.96f) * mixf(0.82f, 1.42f, detail), 0.f);
+          F3 wp = f3(mixf(bmin.x, bmax.x, uvw.x), mixf(bmin.y, bmax.y, uvw.y), mixf(bmin.z, bmax.z, uvw.z));
+          float topFade = 1.f - smoothf(0.9f, 0.995f, uvw.y);
+          F3 total = f3(0, 0, 0);
+          float primT = 1.f;
+          if (nl > 0) {
+            auto nearest = [&](F3 p) {
+              F3 c = f3(clampf(p.x, 0.f, dims.x), clampf(p.y, 0.f, dims.y), clampf(p.z, 0.f, dims.z));
+              return S.d[vidx(g, min(uint32_t(floorf(c.x)), mxu), min(uint32_t(floorf(c.y)), myu), min(uint32_t(floorf(c.z)), mzu))];
+            };
+            float d1 = fmaxf(nearest(sp + lightStep * 0.9f) - 0.004f, 0.f);
+            float d2 = fmaxf(nearest(sp + lightStep * 2.f) - 0.004f, 0.f);
+            float d3 = fmaxf(nearest(sp + lightStep * 3.6f) - 0.004f, 0.f);
+            float od = dd * 0.34f + d1 * 1.15f + d2 * 0.72f + d3 * 0.4f - hot * 0.06f;
+            primT = expf(-fmaxf(od, 0.f) * 2.15f);
+          }
+          for (uint32_t li = 0; li < 3; ++li) {
+            if (li >= nl) break;
+            F3 dl = lightDir(int(li), wp);
+            float raw = fmaxf(R.light_intensity[li], 0.f) * lightAtt(int(li), wp);
+            float power = raw / (1.f + raw * 0.18f);
+            F3 color = f3(R.light_color[li][0], R.light_color[li][1], R.light_color[li][2]);
+            float lvc = clampf(dot3(dl, -rayDir), -1.f, 1.f);
+            float fs = phase_hg(lvc, R.scat_f) * 5.2f * power;
+            float bs = phase_hg(lvc, R.scat_b) * 1.55f * power;
+            float nf = clampf(dot3(normal, dl) * 0.5f + 0.5f, 0.f, 1.f);
+            float surface = powf(nf, 1.35f) * mixf(1.f, 0.56f, smoothf(0.16f, 0.9f, dd));
+            float rim = powf(1.f - fabsf(dot3(normal, -rayDir)), 2.4f) * powf(clampf(dot3(dl, -rayDir) * 0.5f + 0.5f, 0.f, 1.f), 1.2f);
+            float directional = (surface * 0.42f + rim * 0.22f) * power;
+            float scatter = (fs + bs) * mixf(0.36f, 0.74f, hot);
+            float trans = li > 0 ? 1.f : primT;
+            total = total + color * trans * (directional + scatter);
+          }
+          F3 tl = f3(1.f - expf(-total.x * 0.72f), 1.f - expf(-total.y * 0.72f), 1.f - expf(-total.z * 0.72f));
+          F3 smoke = smoke_albedo(dd, tem, hot);
+          float absorb = mixf(0.92f, 0.34f, smoothf(0.26f, 1.05f, dd));
+          float heat = clampf(tem * 1.25f + fu * 0.18f + rea * 0.34f, 0.f, 1.f);
+          float flame = clampf(tem * 0.9f + fu * 0.12f + rea * 0.18f, 0.f, 1.f);
+          F3 heatScatter = fire_palette(flame) * heat * heat * smoothf(0.012f, 0.38f, dd) * 0.18f;
+          F3 lit = mul3(smoke, f3(0.045f, 0.048f, 0.052f)) + mul3(smoke, tl) * absorb + tl * smoothf(0.018f, 0.22f, dd) * 0.18f + heatScatter;
+          float flameHeat = clampf(tem * 0.9f + fu * 0.14f + rea * 0.16f, 0.f, 1.f);
+          F3 emissive = fire_palette(flameHeat) * hot * (0.48f + fu * 0.16f + rea * 0.24f);
+          float contribution = clampf(dd * 1.35f + hot * 1.25f, 0.f, 1.f);
+          float stepDistance = worldDelta * mixf(1.02f, 0.76f, contribution);
+          vstep = voxDelta * mixf(1.02f, 0.76f, contribution);
+          F3 comp = lit + emissive;
+          float opacity = (1.f - expf(-(dd * 4.75f + hot * 0.42f) * stepDistance)) * topFade;
+          if (R.mode > 0.5f && R.mode < 1.5f) {
+            comp = mix3(f3(0.01f, 0.01f, 0.012f), f3(0.88f, 0.9f, 0.94f), clampf(dd * 1.4f, 0.f, 1.f));
+            opacity = 1.f - expf(-dd * stepDistance * 8.6f);
+          } else if (R.mode >= 1.5f) {
+            comp = mix3(f3(0.04f, 0.012f, 0.f), f3(1.f, 0.72f, 0.22f), clampf(fu * 1.3f, 0.f, 1.f));
+            opacity = 1.f - expf(-fu * stepDistance * 7.f);
+          }
+          acc = acc + comp * ((1.f - alpha) * opacity);
+          alpha = alpha + (1.f - alpha) * opacity;
+        }
+      } else {
+        vstep = voxDelta * 1.35f;
+      }
+    } else {
+      // voxelDistToNextBrick, RP:383-396
+      F3 bc = f3(floorf(sp.x / 8.f), floorf(sp.y / 8.f), floorf(sp.z / 8.f));
+      F3 bnd = f3(vrd.x > 0.f ? (bc.x + 1.f) * 8.f : bc.x * 8.f, vrd.y > 0.f ? (bc.y + 1.f) * 8.f : bc.y * 8.f,
+                  vrd.z > 0.f ? (bc.z + 1.f) * 8.f : bc.z * 8.f);
+      float tx = fabsf(vrd.x) > 0.00001f ? (bnd.x - sp.x) / vrd.x : 1e9f;
+      float ty = fabsf(vrd.y) > 0.00001f ? (bnd.y - sp.y) / vrd.y : 1e9f;
+      float tz = fabsf(vrd.z) > 0.00001f ? (bnd.z - sp.z) / vrd.z : 1e9f;
+      vstep = fmaxf(fminf(tx, fminf(ty, tz)) * len3(vrd) + 0.5f, voxDelta);
+    }
+    vp = vp + nrd * vstep;
+    travelled += vstep;
+  }
+  if (alpha < 0.01f) { o[0] = zero; o[1] = zero; o[2] = zero; o[3] = zero; return; }
+  o[0] = __float2half(acc.x); o[1] = __float2half(acc.y); o[2] = __float2half(acc.z); o[3] = __float2half(alpha);
+}
+
+// ------------------------------------------------------------ launchers --
+#define FX_LAUNCH(kernel, blocks, threads, stream, ...) \
+  hipLaunchKernelGGL(kernel, dim3(blocks), dim3(threads), 0, stream, __VA_ARGS__)
+
+void launch_inject(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
+                   const float* src, Scalars set0, Vec3 u, uint32_t* touched, int parts) {
+  FX_LAUNCH(k_inject, blocks, 512, s, list, count, g, c, src, set0, u, touched, parts);
+}
+void launch_clear_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t value) {
+  int blocks = int(std::min<size_t>((n + 255) / 256, 4096));
+  if (blocks < 1) blocks = 1;
+  FX_LAUNCH(k_fill_u32, blocks, 256, s, p, n, value);
+}
+void launch_mark(hipStream_t s, const uint32_t* list, uint32_t n, uint32_t* touched) {
+  if (n == 0) return;
+  FX_LAUNCH(k_mark, int(std::min<uint32_t>((n + 255) / 256, 1024)), 256, s, list, n, touched);
+}
+void launch_clear_f32(hipStream_t s, float* p, size_t n) {
+  int blocks = int(std::min<size_t>((n + 255) / 256, 8192));
+  if (blocks < 1) blocks = 1;
+  FX_LAUNCH(k_fill_f32, blocks, 256, s, p, n);
+}
+void launch_activate(hipStream_t s, int blocks, const uint32_t* listA, const uint32_t* countA, const uint32_t* listB,
+                     const uint32_t* countB, Grid g, const StepConst* c, int rule, Scalars set, Vec3 vel, uint32_t* flags) {
+  if (rule == FX_ACTIVATION_REFERENCE) {
+    int b = std::max(1, std::min(blocks, int((brick_count(g) + 255) / 256) + 1));
+    FX_LAUNCH(k_activate_reference, b, 256, s, listA, countA, listB, countB, g, set, flags);
+  } else {
+    FX_LAUNCH(k_activate_threshold, blocks, 512, s, listA, countA, listB, countB, g, c, set.d, vel, flags);
+  }
+}
+void launch_compact(hipStream_t s, int nlevels, const Grid* grids, uint32_t* const* flags_lv, BrickList* lists, uint32_t* touched) {
+  CompactArgs A;
+  A.nlevels = nlevels;
+  for (int l = 0; l < nlevels; ++l) { A.g[l] = grids[l]; A.flags[l] = flags_lv[l]; A.L[l] = lists[l]; }
+  A.touched = touched;
+  FX_LAUNCH(k_compact, 1, 1024, s, A);
+}
+void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
+                            Vec3 u, Vec3 us) {
+  FX_LAUNCH(k_advect_velocity, blocks, 512, s, list, count, g, c, u, us);
+}
+void launch_advect_scalars(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
+                           int mode, Scalars src, Vec3 us, Scalars dst, const uint32_t* flags) {
+  FX_LAUNCH(k_advect_scalars, blocks, 512, s, list, count, g, c, mode, src, us, dst, flags);
+}
+void launch_combustion(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
+                       Scalars src, Scalars dst, const uint32_t* flags) {
+  FX_LAUNCH(k_combustion, blocks, 512, s, list, count, g, c, src, dst, flags);
+}
+void launch_buoyancy(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
+                     Scalars set0, Vec3 us, const uint32_t* flags) {
+  FX_LAUNCH(k_buoyancy, blocks, 512, s, list, count, g, c, set0, us, flags);
+}
+void launch_vorticity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 us, Vec3 w, float* wmag) {
+  FX_LAUNCH(k_vorticity, blocks, 512, s, list, count, g, us, w, wmag);
+}
+void launch_confinement(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c, Vec3 w,
+                        const float* wmag, Vec3 us, float* cmag, Scalars set0) {
+  FX_LAUNCH(k_confinement, blocks, 512, s, list, count, g, c, w, wmag, us, cmag, set0);
+}
+void launch_divergence(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 us, float* div) {
+  FX_LAUNCH(k_divergence, blocks, 512, s, list, count, g, us, div);
+}
+void launch_zero_bricks(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, float* const* bufs, int nbufs) {
+  for (int base = 0; base < nbufs; base += FX_MAX_ZERO_BUFS) {
+    ZeroArgs Z;
+    Z.n = std::min(FX_MAX_ZERO_BUFS, nbufs - base);
+    for (int k = 0; k < Z.n; ++k) Z.b[k] = bufs[base + k];
+    FX_LAUNCH(k_zero_bricks, blocks, 512, s, list, count, Z);
+  }
+}
+void launch_jacobi(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
+                   const float* rhs, const float* src, float* dst, int weighted) {
+  FX_LAUNCH(k_jacobi, blocks, 512, s, list, count, g, c, rhs, src, dst, weighted);
+}
+void launch_residual(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const float* rhs,
+                     const float* p, float* r, double* norm2) {
+  FX_LAUNCH(k_residual, blocks, 512, s, list, count, g, rhs, p, r, norm2);
+}
+void launch_restrict(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid fine, Grid coarse,
+                     const float* src, float* dst, int full_weight) {
+  FX_LAUNCH(k_restrict, blocks, 512, s, list, count, fine, coarse, src, dst, full_weight);
+}
+void launch_prolong(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid coarse, Grid fine,
+                    const float* src, float* dst) {
+  FX_LAUNCH(k_prolong, blocks, 512, s, list, count, coarse, fine, src, dst);
+}
+void launch_projection(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 us, const float* p,
+                       Vec3 u) {
+  FX_LAUNCH(k_projection, blocks, 512, s, list, count, g, us, p, u);
+}
+void launch_debug(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 u, float* umag) {
+  FX_LAUNCH(k_debug, blocks, 512, s, list, count, g, u, umag);
+}
+void launch_raymarch(hipStream_t s, Grid g, const RayConst& rc, Scalars set0, const uint32_t* flags, uint16_t* out) {
+  dim3 grid((rc.width + 15) / 16, (rc.height + 15) / 16);
+  hipLaunchKernelGGL(k_raymarch, grid, dim3(256), 0, s, g, rc, set0, flags, out);
+}
+
+}  // namespace fx

@@ -1,0 +1,353 @@
+/*
+ * fenix.h — C ABI of libfenix, the MI355X (gfx950) implementation of fenix's
+ * per-step combustion-volume pipeline (simulation step + emission–absorption
+ * raymarch).
+ *
+ * Every entry point replaces one member of the reference's TypeScript API; the
+ * reference interface is cited next to each declaration (paths relative to the
+ * reference repository root). Plain C types only: pointers, sizes, PODs.
+ *
+ * Conventions
+ *   - Every call returns an int status: FX_OK (0) or a negative FX_ERR_* code;
+ *     fx_last_error() returns the message of the last failure on that handle.
+ *   - Field reads/writes use the reference's dense row-major FP32 layout
+ *     (index = x + W*(y + H*z), src/engine/simulation/shaders/common/indexing.wgsl.ts:2-4).
+ *     Device storage is brick-major; conversion happens inside the call.
+ *   - fx_step and fx_raymarch_device enqueue work on the simulation's HIP
+ *     stream and return without waiting; reads synchronise.
+ *   - The library owns all device memory. Inputs are copied during the call.
+ */
+#ifndef FENIX_H
+#define FENIX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define FX_OK 0
+#define FX_ERR_INVALID_ARG (-1)
+#define FX_ERR_OUT_OF_MEMORY (-2)
+#define FX_ERR_DEVICE (-3)
+#define FX_ERR_UNSUPPORTED (-4)
+#define FX_ERR_STATE (-5)
+
+/* ---- enums -------------------------------------------------------------- */
+/* ScalarAdvectionMode (src/engine/simulation/runtime/combustion-volume-simulation/types.ts:4) */
+#define FX_ADVECT_MACCORMACK 0
+#define FX_ADVECT_SEMI_LAGRANGIAN 1
+
+/* Domain: DENSE runs every stage over every brick with the reference's
+ * activity-flag semantics; SPARSE runs every stage over the active-brick list
+ * only, with all fields defined as zero outside the active bricks. */
+#define FX_DOMAIN_DENSE 0
+#define FX_DOMAIN_SPARSE 1
+
+/* Activation: REFERENCE = compute-active-bricks.wgsl.ts:24-79 (energy samples,
+ * conditional dilation); THRESHOLD = density > a or |u| > b anywhere in the
+ * brick, always dilated by one brick. */
+#define FX_ACTIVATION_REFERENCE 0
+#define FX_ACTIVATION_THRESHOLD 1
+
+/* Pressure: REFERENCE = the 3-level schedule of PressureSolvePass.ts:68-84;
+ * VCYCLE = L-level weighted-Jacobi V-cycle with h^2-scaled residual restriction. */
+#define FX_SOLVER_REFERENCE 0
+#define FX_SOLVER_VCYCLE 1
+
+/* Storage precision of density, temperature and fuel (reaction, velocity and
+ * pressure always FP32; all arithmetic FP32). */
+#define FX_PRECISION_FP32 0
+#define FX_PRECISION_BF16 1
+#define FX_PRECISION_FP16 2
+
+/* Field ids for fx_read_field / fx_write_field / fx_field_device_ptr. */
+#define FX_FIELD_DENSITY 0          /* render set (getRenderBuffers) */
+#define FX_FIELD_TEMPERATURE 1
+#define FX_FIELD_FUEL 2
+#define FX_FIELD_REACTION 3
+#define FX_FIELD_VELOCITY_X 4       /* velocityCurrent */
+#define FX_FIELD_VELOCITY_Y 5
+#define FX_FIELD_VELOCITY_Z 6
+#define FX_FIELD_SCRATCH_X 7        /* velocityScratch */
+#define FX_FIELD_SCRATCH_Y 8
+#define FX_FIELD_SCRATCH_Z 9
+#define FX_FIELD_VORTICITY_X 10
+#define FX_FIELD_VORTICITY_Y 11
+#define FX_FIELD_VORTICITY_Z 12
+#define FX_FIELD_VORTICITY_MAG 13
+#define FX_FIELD_CONFINEMENT_MAG 14
+#define FX_FIELD_VELOCITY_MAG 15
+#define FX_FIELD_DIVERGENCE 16      /* fine-level divergence */
+#define FX_FIELD_PRESSURE 17        /* fine-level pressure last used by projection */
+#define FX_FIELD_ADV_DENSITY 18     /* advected set (scalar advection output, combustion input) */
+#define FX_FIELD_ADV_TEMPERATURE 19
+#define FX_FIELD_ADV_FUEL 20
+#define FX_FIELD_ADV_REACTION 21
+#define FX_FIELD_COUNT 22
+
+/* Pressure-level buffers for fx_read_level / fx_write_level. */
+#define FX_LEVEL_DIVERGENCE 0
+#define FX_LEVEL_PRESSURE_A 1
+#define FX_LEVEL_PRESSURE_B 2
+
+/* Flag sets for fx_read_flags. */
+#define FX_FLAGS_RENDER 0           /* flags returned by getRenderBuffers */
+#define FX_FLAGS_INJECTED 1         /* flags computed right after injection */
+
+/* Single passes for fx_run_pass (test hook; the reference records these
+ * inside step(), ORCH = src/engine/simulation/runtime/createCombustionVolumeSimulation.ts:270-301). */
+#define FX_PASS_INJECT_SCALAR 1     /* inject-scalar.wgsl.ts:32-70 into the render set */
+#define FX_PASS_INJECT_VELOCITY 2   /* inject-velocity.wgsl.ts:46-96 */
+#define FX_PASS_INJECT_IGNITER 3    /* inject-igniter.wgsl.ts:32-66 */
+#define FX_PASS_ACTIVATE_RENDER 4   /* compute-active-bricks over the render set -> injected flags */
+#define FX_PASS_ACTIVATE_ADVECTED 5 /* compute-active-bricks over the advected set -> render flags */
+#define FX_PASS_ADVECT_VELOCITY 6   /* advect-velocity-semilagrangian.wgsl.ts:23-41 */
+#define FX_PASS_ADVECT_SCALARS 7    /* advect-scalar-{maccormack,semilagrangian}: render -> advected */
+#define FX_PASS_COMBUSTION 8        /* combustion-update.wgsl.ts:42-93: advected -> render (read-old) */
+#define FX_PASS_BUOYANCY 9          /* apply-buoyancy.wgsl.ts:37-110 */
+#define FX_PASS_VORTICITY 10        /* compute-vorticity.wgsl.ts:23-50 */
+#define FX_PASS_CONFINEMENT 11      /* apply-vorticity-confinement.wgsl.ts:30-66 */
+#define FX_PASS_DIVERGENCE 12       /* compute-divergence.wgsl.ts:22-42 */
+#define FX_PASS_PRESSURE 13         /* PressureSolvePass.clear + solve */
+#define FX_PASS_PROJECTION 14       /* project-velocity.wgsl.ts:21-43 */
+#define FX_PASS_DEBUG_FIELDS 15     /* compute-debug-fields.wgsl.ts:18-25 */
+
+/* Raymarch display modes (src/engine/render/volumetrics/volumeDisplayMode.ts:1). */
+#define FX_DISPLAY_TEMPERATURE 0
+#define FX_DISPLAY_DENSITY 1
+#define FX_DISPLAY_FUEL 2
+
+#define FX_FLOATS_PER_EMITTER 32    /* packEmitterSources.ts:55 */
+#define FX_MAX_RENDER_LIGHTS 3      /* renderLight.ts:10 */
+
+/* present_mask bits for fx_set_runtime_params (Partial<SimulationRuntimeParams>). */
+#define FX_RP_WIND (1u << 0)
+#define FX_RP_WIND_STRENGTH (1u << 1)
+#define FX_RP_GRAVITY (1u << 2)
+#define FX_RP_GRAVITY_STRENGTH (1u << 3)
+#define FX_RP_BUOYANCY (1u << 4)
+#define FX_RP_VORTICITY_STRENGTH (1u << 5)
+#define FX_RP_VORTICITY_CONSTANT_MASK (1u << 6)
+#define FX_RP_VORTICITY_VELOCITY_MASK (1u << 7)
+#define FX_RP_VORTICITY_HEAT_MASK (1u << 8)
+#define FX_RP_VORTICITY_DENSITY_MASK (1u << 9)
+#define FX_RP_WORLD_SIZE (1u << 10)
+#define FX_RP_ALL 0x7ffu
+
+/* ---- PODs --------------------------------------------------------------- */
+
+/* SimulationQualitySettings, CVS = src/engine/simulation/runtime/combustion-volume-simulation/;
+ * fields and clamps follow CVS/types.ts:6-44. */
+typedef struct fx_quality {
+  int32_t pressure_interval;    /* [1,16] */
+  int32_t vorticity_interval;   /* [1,16] */
+  int32_t fine_pre_iterations;  /* [0,32] */
+  int32_t fine_post_iterations; /* [0,32] */
+  int32_t mid_pre_iterations;   /* [0,32] */
+  int32_t mid_post_iterations;  /* [0,32] */
+  int32_t coarse_iterations;    /* [0,64] */
+} fx_quality;
+
+/* SimulationRuntimeParams (CVS/types.ts:46-58); defaults ORCH:194-204. */
+typedef struct fx_runtime_params {
+  float wind[3];
+  float wind_strength;
+  float gravity[3];
+  float gravity_strength;
+  float buoyancy;
+  float vorticity_strength;
+  float vorticity_constant_mask;
+  float vorticity_velocity_mask;
+  float vorticity_heat_mask;
+  float vorticity_density_mask;
+  float world_size;
+} fx_runtime_params;
+
+/* The 80-byte per-step uniform (src/engine/simulation/shaders/common/simulation-params.wgsl.ts:2-14,
+ * packed at ORCH:241-266). Used by fx_run_pass. */
+typedef struct fx_sim_params {
+  float time;
+  float delta_time;
+  float previous_time;
+  float scalar_set_index;
+  float wind[4];          /* xyz direction, w strength */
+  float gravity[4];       /* xyz direction, w strength */
+  float buoyancy;
+  float vorticity_strength;
+  float dx;
+  float world_size;
+  float vorticity_mask[4];
+} fx_sim_params;
+
+/* V-cycle configuration (FX_SOLVER_VCYCLE only). */
+typedef struct fx_vcycle {
+  int32_t levels;        /* >= 2 */
+  int32_t pre_sweeps;
+  int32_t post_sweeps;
+  int32_t cycles;
+  int32_t coarse_sweeps;
+  float omega;           /* weighted-Jacobi factor */
+} fx_vcycle;
+
+/* CombustionVolumeSimulationOptions (ORCH:39-48) plus this library's knobs. */
+typedef struct fx_create_desc {
+  uint32_t width, height, depth;  /* resolution; ORCH:54 default 128x256x128 */
+  int32_t advection_mode;         /* FX_ADVECT_* */
+  fx_runtime_params runtime;      /* wind/gravity/worldSize options + ORCH:198-203 defaults */
+  fx_quality quality;             /* qualitySettings (clamped like CVS/types.ts:30-44) */
+  int32_t domain_mode;            /* FX_DOMAIN_* */
+  int32_t activation_rule;        /* FX_ACTIVATION_* */
+  int32_t solver;                 /* FX_SOLVER_* */
+  int32_t precision;              /* FX_PRECISION_* */
+  fx_vcycle vcycle;
+  float ignition_lo;              /* combustion ignition window; reference 0.1 .. 0.55 */
+  float ignition_hi;
+  float burn_rate;                /* multiplier of the reference burn; reference 1 */
+  float activation_density;       /* THRESHOLD rule: density threshold */
+  float activation_speed;         /* THRESHOLD rule: |u| threshold */
+  int32_t debug_fields;           /* -1: reference rule (N < 1.8M, ORCH:193); 0 off; 1 on */
+  int32_t device;                 /* HIP device ordinal */
+} fx_create_desc;
+
+/* OrbitCameraSnapshot (src/engine/scene/camera/createOrbitCameraController.ts:11-19). */
+typedef struct fx_camera {
+  float position[3];
+  float target[3];
+  float forward[3];
+  float right[3];
+  float up[3];
+  float aspect;
+  float tan_half_fov_y;
+} fx_camera;
+
+/* RenderLight after sanitizeRenderLights (createVolumeRaymarchPass.ts:62-76). */
+typedef struct fx_light {
+  int32_t type;          /* 0 directional, 1 point */
+  float direction[3];
+  float position[3];
+  float color[3];
+  float intensity;
+} fx_light;
+
+/* RaymarchRenderParams after setRenderParams clamps (createVolumeRaymarchPass.ts:8-13, 284-297)
+ * plus the per-call render() arguments. */
+typedef struct fx_render_params {
+  int32_t step_count;
+  int32_t light_count;    /* 0..3 */
+  fx_light lights[FX_MAX_RENDER_LIGHTS];
+  float scattering_forward;
+  float scattering_back;
+  int32_t display_mode;   /* FX_DISPLAY_* */
+  float elapsed_s;
+} fx_render_params;
+
+typedef struct fx_sim fx_sim;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+
+/* Fills *desc with the reference defaults (ORCH:54, 194-204; CVS/types.ts:16-24). */
+void fx_default_create_desc(fx_create_desc* desc);
+
+/* createCombustionVolumeSimulation(device, options) — ORCH:50-226. */
+int fx_create(const fx_create_desc* desc, fx_sim** out);
+
+/* dispose() — ORCH:384-400. Accepts NULL. */
+void fx_destroy(fx_sim* sim);
+
+/* reset() — ORCH:206-226, 381-383. */
+int fx_reset(fx_sim* sim);
+
+/* ---- step --------------------------------------------------------------- */
+
+/* step(encoder, elapsedSeconds, stepSeconds) — CVS/types.ts:64, ORCH:230-305. */
+int fx_step(fx_sim* sim, double elapsed_s, double step_s);
+
+/* Waits for all work queued on the simulation's stream. */
+int fx_synchronize(fx_sim* sim);
+
+/* ---- parameters --------------------------------------------------------- */
+
+/* getRuntimeParams / setRuntimeParams(Partial<...>) — ORCH:338-365. */
+int fx_get_runtime_params(fx_sim* sim, fx_runtime_params* out);
+int fx_set_runtime_params(fx_sim* sim, const fx_runtime_params* params, uint32_t present_mask);
+
+/* getQualitySettings / setQualitySettings — ORCH:366-377 (clamps CVS/types.ts:30-44). */
+int fx_get_quality(fx_sim* sim, fx_quality* out);
+int fx_set_quality(fx_sim* sim, const fx_quality* quality);
+
+/* getScalarAdvectionMode / setScalarAdvectionMode — ORCH:327-337. */
+int fx_get_advection_mode(fx_sim* sim, int32_t* out);
+int fx_set_advection_mode(fx_sim* sim, int32_t mode);
+
+/* updateSources(sources) — ORCH:378-380, packed by packEmitterSources.ts:69-121
+ * (32 floats per source). count may be 0. */
+int fx_update_sources(fx_sim* sim, const float* packed, uint32_t count);
+
+/* ---- outputs ------------------------------------------------------------ */
+
+/* getRenderBuffers()/getDebugBuffers() readback — ORCH:306-326. `bytes` must be
+ * W*H*D*4. Synchronises the stream. */
+int fx_read_field(fx_sim* sim, int32_t field_id, float* host, size_t bytes);
+
+/* Uploads a dense row-major FP32 field (test and scene-initialisation hook; the
+ * reference writes buffers only through its passes). In FX_DOMAIN_SPARSE the
+ * bricks holding non-zero values join the active set. */
+int fx_write_field(fx_sim* sim, int32_t field_id, const float* host, size_t bytes);
+
+/* Brick flags (u32 per brick, ORCH:71-81): FX_FLAGS_RENDER is activeBrickFlags of
+ * getRenderBuffers (ORCH:309). `count` must equal the brick count. */
+int fx_read_flags(fx_sim* sim, int32_t which, uint32_t* host, size_t count);
+
+/* Active-brick list (ascending brick indices with flag != 0) of the render flags;
+ * *count receives the list length. No reference counterpart (the reference has
+ * flags only); pinned by the flags. */
+int fx_read_active_list(fx_sim* sim, uint32_t* host, size_t capacity, uint32_t* count);
+
+/* Page table: brick index -> position in the active list, or -1. */
+int fx_read_page_table(fx_sim* sim, int32_t* host, size_t count);
+
+/* Borrowed device pointer of a field's brick-major storage (valid until the next
+ * fx_step / fx_reset / fx_destroy). Element type follows the precision. */
+void* fx_field_device_ptr(fx_sim* sim, int32_t field_id);
+
+/* Pressure-hierarchy buffers (PressureSolvePass.ts:93-127): level 0 = fine. */
+int fx_read_level(fx_sim* sim, int32_t level, int32_t which, float* host, size_t bytes);
+int fx_level_dims(fx_sim* sim, int32_t level, uint32_t dims[3]);
+
+/* FP64 residual norms ||div - L p||_2 of the fine level, one per V-cycle of the
+ * last solve (FX_SOLVER_VCYCLE); *count receives how many were written. */
+int fx_solver_residuals(fx_sim* sim, double* host, size_t capacity, uint32_t* count);
+
+/* Number of bricks in the last computed active list (sparse) or all bricks (dense). */
+int fx_active_brick_count(fx_sim* sim, uint32_t* out);
+
+/* ---- raymarch ----------------------------------------------------------- */
+
+/* VolumeRaymarchPass.render(...) — createVolumeRaymarchPass.ts:183-283, shader :573-724,
+ * over the current render buffers. Writes width*height RGBA float16 pixels
+ * (row 0 = top, premultiplied, cleared to 0 where the reference discards). */
+int fx_raymarch(fx_sim* sim, const fx_camera* camera, const fx_render_params* params,
+                uint32_t width, uint32_t height, uint16_t* host_rgba_f16);
+
+/* Same, into a caller-owned device buffer of width*height*8 bytes; asynchronous. */
+int fx_raymarch_device(fx_sim* sim, const fx_camera* camera, const fx_render_params* params,
+                       uint32_t width, uint32_t height, void* device_rgba_f16);
+
+/* ---- test hook ---------------------------------------------------------- */
+
+/* Runs one pass of the step with explicit uniforms (no reference counterpart:
+ * the reference only records passes inside step()). For FX_PASS_PRESSURE the
+ * current quality/solver settings are used. */
+int fx_run_pass(fx_sim* sim, int32_t pass_id, const fx_sim_params* params);
+
+/* Message of the last failure on this handle (or of the last fx_create when sim is NULL). */
+const char* fx_last_error(fx_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FENIX_H */
