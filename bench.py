@@ -1,0 +1,208 @@
+"""Benchmark: active voxel-updates/s per full step of the combustion-volume
+pipeline on one MI355X.
+
+Default workload (sparse 256^3): FP32, 8^3 bricks, threshold activation
+(density > 1e-4 or |u| > 1e-3, one-brick dilation), two seeded floor emitters
+of radius 10 voxels, burn rate 0.5, ignition T = 0.6, 4-level V-cycle (2/2
+sweeps, omega 2/3, 4 cycles), vorticity eps = 0.3, and a 1024x1024 raymarch with
+128 samples per step. `--warmup` untimed steps (default 100) build the plume,
+then `--steps` steps (default 100) are timed: a "step" is fx_step + the
+raymarch, with every input already resident in HBM.
+
+Prints one JSON line (see the keys below).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from fenix_amd import abi  # noqa: E402
+from fenix_amd.render import VolumeRaymarchPass, orbit_camera_snapshot  # noqa: E402
+from fenix_amd.simulation import CombustionVolumeSimulation  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+# Algorithmic bytes per active voxel per launch (FP32 storage; velocity/vorticity
+# counted as 3 x FP32; each stage reads its inputs once and writes its outputs once).
+STAGE_BYTES = {
+    "advect_velocity": 24.0,
+    "advect_scalars": 44.0,
+    "combustion": 32.0,
+    "buoyancy": 36.0,
+    "vorticity": 28.0,
+    "confinement": 52.0,
+    "divergence": 16.0,
+    "projection": 28.0,
+    "jacobi_fine": 12.0,
+}
+# Whole-step algorithmic bytes per active voxel (config V-cycle, FP32): simulation
+# 602.2 + raymarch 16 B/voxel (+ 8 B per pixel, added per workload).
+STEP_BYTES_SIM = 602.2
+RAY_BYTES_PER_VOXEL = 16.0
+
+
+class HipBuffer:
+    """Device allocation through the HIP runtime (no torch dependency)."""
+
+    def __init__(self, nbytes):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.ptr = C.c_void_p()
+        rc = self.hip.hipMalloc(C.byref(self.ptr), C.c_size_t(nbytes))
+        if rc != 0:
+            raise RuntimeError(f"hipMalloc failed ({rc})")
+
+    def free(self):
+        if self.ptr:
+            self.hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+def workload(name):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import scenes
+    if name == "c2":
+        n = 256
+        return dict(n=n, desc=scenes.c2_desc(n), sources=scenes.c2_sources(n), render=(1024, 1024, 128),
+                    label="sparse 256^3 FP32, 8^3 bricks, 2 floor emitters r=10, threshold activation, "
+                          "4-level V-cycle, 1024x1024x128 raymarch")
+    if name == "c1":
+        n = 64
+        return dict(n=n, desc=scenes.c1_desc(n), sources=scenes.c1_sources(), render=(256, 256, 64),
+                    label="dense 64^3 FP32 reference semantics, 256x256x64 raymarch")
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(wl, sim, elapsed0, steps):
+    """Times the FP32 oracle (a C++ port, OpenMP) on a bounded sample of the same
+    workload, started from the GPU's state."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import scenes
+    from oracle import Oracle
+    from fenix_amd.render import camera_struct
+
+    orc = Oracle(wl["desc"], 32)
+    orc.update_sources(*scenes.packed(wl["sources"]))
+    for name in ("density", "temperature", "fuel", "reaction", "velocity_x", "velocity_y", "velocity_z"):
+        orc.write(name, sim.read_field(name))
+    w, h, sc = wl["render"]
+    rp = VolumeRaymarchPass((wl["n"],) * 3)
+    rp.setRenderParams(stepCount=sc)
+    cam = camera_struct(orbit_camera_snapshot())
+    params = rp.params_struct("temperature")
+    dt = 1.0 / 60.0
+    active = 0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        orc.step(elapsed0 + k * dt, dt)
+        active += orc.active_count()
+        orc.raymarch(cam, params, w, h)
+    t1 = time.perf_counter()
+    per_step = (t1 - t0) / steps
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": (active / steps) * 512 / per_step, "unit": "active voxel-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{steps} full steps (sim + {w}x{h}x{sc} raymarch) of the same workload from the GPU state "
+                      f"after the timed window; {per_step:.2f} s/step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--timing-steps", type=int, default=10, help="extra steps timed per stage with HIP events")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path (see DESIGN.md, sharding).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    wl = workload(args.workload)
+    sim = CombustionVolumeSimulation(wl["desc"])
+    sim.updateSources(wl["sources"])
+    w, h, sc = wl["render"]
+    rp = VolumeRaymarchPass((wl["n"],) * 3)
+    rp.setRenderParams(stepCount=sc)
+    cam = orbit_camera_snapshot()
+    img = HipBuffer(w * h * 8)
+    dt = 1.0 / 60.0
+
+    def full_step(k):
+        sim.step(None, k * dt, dt)
+        rp.render_device(sim, cam, img.ptr.value, "temperature", w, h, k * dt)
+
+    for k in range(args.warmup):
+        full_step(k)
+    sim.synchronize()
+    active0 = sim.active_brick_count()
+    t0 = time.perf_counter()
+    for k in range(args.warmup, args.warmup + args.steps):
+        full_step(k)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    active1 = sim.active_brick_count()
+    ms = (t1 - t0) * 1e3 / args.steps
+    active = 0.5 * (active0 + active1)
+    voxels = active * 512
+    value = voxels / (ms * 1e-3)
+
+    # Per-stage device time (HIP events on the simulation's stream).
+    sim.set_stage_timing(True)
+    base = args.warmup + args.steps
+    for k in range(base, base + args.timing_steps):
+        full_step(k)
+    stages = sim.stage_times()
+    active_t = sim.active_brick_count() * 512
+    sim.set_stage_timing(False)
+    stage_ms = {k: v[0] / v[1] for k, v in stages.items()}
+    candidates = {k: stages[k][0] for k in stages if k in STAGE_BYTES or k == "raymarch"}
+    dom = max(candidates, key=candidates.get)
+    avg_ms = stage_ms[dom]
+    if dom == "raymarch":
+        bytes_launch = RAY_BYTES_PER_VOXEL * active_t + 8.0 * w * h
+    else:
+        bytes_launch = STAGE_BYTES[dom] * active_t
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    step_bytes = (STEP_BYTES_SIM + RAY_BYTES_PER_VOXEL) * voxels + 8.0 * w * h
+    step_gbs = step_bytes / (ms * 1e-3) / 1e9
+
+    out = {
+        "metric": "active voxel-updates/s per full step @256³ sparse; % HBM roofline vs plain",
+        "value": value,
+        "unit": "active voxel-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wl["label"], "path": "plain FP32 single-scene", "active_bricks": active,
+                   "active_fraction": active / sim.brick_count, "render": f"{w}x{h}x{sc}"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_launch, "avg_launch_ms": avg_ms},
+        "step_hbm": {"algorithmic_gbs": step_gbs, "frac": step_gbs / HBM_PEAK_GBS,
+                     "bytes_per_active_voxel": STEP_BYTES_SIM + RAY_BYTES_PER_VOXEL},
+        "stage_ms": stage_ms,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(wl, sim, (base + args.timing_steps) * dt, args.cpu_steps)
+    img.free()
+    sim.dispose()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

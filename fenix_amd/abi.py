@@ -55,6 +55,12 @@ PASSES = {
     "projection": 14, "debug_fields": 15,
 }
 
+FX_TIME_RAYMARCH = 16
+FX_TIME_JACOBI_FINE = 17
+FX_TIME_COUNT = 18
+TIME_NAMES = {v: k for k, v in PASSES.items()}
+TIME_NAMES.update({FX_TIME_RAYMARCH: "raymarch", FX_TIME_JACOBI_FINE: "jacobi_fine"})
+
 FX_DISPLAY_TEMPERATURE = 0
 FX_DISPLAY_DENSITY = 1
 FX_DISPLAY_FUEL = 2
@@ -179,6 +185,8 @@ FUNCTIONS = {
     "fx_active_brick_count": (C.c_int, [P, C.POINTER(u32)]),
     "fx_raymarch": (C.c_int, [P, C.POINTER(Camera), C.POINTER(RenderParams), u32, u32, C.POINTER(C.c_uint16)]),
     "fx_raymarch_device": (C.c_int, [P, C.POINTER(Camera), C.POINTER(RenderParams), u32, u32, C.c_void_p]),
+    "fx_set_stage_timing": (C.c_int, [P, i32]),
+    "fx_read_stage_times": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(u32), C.c_size_t]),
     "fx_run_pass": (C.c_int, [P, i32, C.POINTER(SimParams)]),
     "fx_last_error": (C.c_char_p, [P]),
 }

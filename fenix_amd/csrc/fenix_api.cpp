@@ -87,6 +87,14 @@ struct fx_sim {
   int curSet = 0;
   std::vector<uint32_t> pending;    // sparse: bricks written by fx_write_field since the last step
   std::string err;
+
+  // Stage timing (fx_set_stage_timing): event pairs recorded around stages.
+  bool timing = false;
+  struct Rec { int id; hipEvent_t a, b; };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  double acc_ms[FX_TIME_COUNT] = {};
+  uint32_t acc_calls[FX_TIME_COUNT] = {};
 };
 
 namespace {
@@ -235,8 +243,46 @@ bool debug_each_frame(const fx_sim* s) {
   return s->nv < 1800000ull;  // ORCH:193 (voxelCount)
 }
 
+// Brackets the launches of one stage with timing events (when enabled).
+struct StageTimer {
+  fx_sim* s;
+  int id;
+  hipEvent_t a = nullptr, b = nullptr;
+  static hipEvent_t take(fx_sim* s) {
+    if (!s->pool.empty()) { hipEvent_t e = s->pool.back(); s->pool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+  StageTimer(fx_sim* sim, int stage) : s(sim), id(stage) {
+    if (!s->timing) return;
+    a = take(s); b = take(s);
+    if (a) (void)hipEventRecord(a, s->stream);
+  }
+  ~StageTimer() {
+    if (!s->timing || !a || !b) return;
+    (void)hipEventRecord(b, s->stream);
+    s->recs.push_back(fx_sim::Rec{id, a, b});
+  }
+};
+
+// Folds completed event pairs into the accumulators (stream must be idle).
+void drain_timing(fx_sim* s) {
+  for (auto& r : s->recs) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess && r.id >= 0 && r.id < FX_TIME_COUNT) {
+      s->acc_ms[r.id] += ms;
+      s->acc_calls[r.id] += 1;
+    }
+    s->pool.push_back(r.a);
+    s->pool.push_back(r.b);
+  }
+  s->recs.clear();
+}
+
 // ------------------------------------------------------------- passes --
 void pass_activate_render(fx_sim* s) {
+  StageTimer timer(s, FX_PASS_ACTIVATE_RENDER);
   if (s->sparse) {
     // Candidates: last step's active list + injection/host-written bricks.
     launch_clear_u32(s->stream, s->flagsInj, s->nb, 0u);
@@ -260,11 +306,13 @@ void pass_activate_render(fx_sim* s) {
   }
 }
 void pass_activate_advected(fx_sim* s) {  // ORCH:280-281 (dense only)
+  StageTimer timer(s, FX_PASS_ACTIVATE_ADVECTED);
   launch_clear_u32(s->stream, s->flagsRender, s->nb, 0u);
   launch_activate(s->stream, grid_for(s, s->nb), s->identity, s->identity_count, nullptr, nullptr, s->g, s->d_const,
                   s->desc.activation_rule, set_of(s, 1), vec_of(s->us), s->flagsRender);
 }
 void pass_inject(fx_sim* s, int parts) {
+  StageTimer timer(s, parts == 2 ? FX_PASS_INJECT_VELOCITY : parts == 4 ? FX_PASS_INJECT_IGNITER : FX_PASS_INJECT_SCALAR);
   if (s->sparse)
     launch_inject(s->stream, grid_for(s, s->nb), s->d_cand, &s->d_const->inj_count, s->g, s->d_const, s->d_src, set_of(s, 0),
                   vec_of(s->u), s->touched, parts);
@@ -283,6 +331,7 @@ int smooth_ref(fx_sim* s, int l, int iters, int start) {  // PressureSolvePass.t
   int a = start;
   Level& L = s->lv[l];
   for (int k = 0; k < iters; ++k) {
+    StageTimer timer(s, l == 0 ? FX_TIME_JACOBI_FINE : -1);
     launch_jacobi(s->stream, grid_for(s, L.nb), level_list(s, l), level_count(s, l), L.g, s->d_const, L.div, L.p[a], L.p[1 - a], 0);
     a = 1 - a;
   }
@@ -310,6 +359,7 @@ int solve_vcycle(fx_sim* s) {
   auto smoothW = [&](int l, int n) {
     Level& V = s->lv[l];
     for (int k = 0; k < n; ++k) {
+      StageTimer timer(s, l == 0 ? FX_TIME_JACOBI_FINE : -1);
       launch_jacobi(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, s->d_const, V.div, V.p[cur[l]],
                     V.p[1 - cur[l]], 1);
       cur[l] = 1 - cur[l];
@@ -341,12 +391,14 @@ int solve_vcycle(fx_sim* s) {
   return cur[0];
 }
 int pressure_solve(fx_sim* s) {
+  StageTimer timer(s, FX_PASS_PRESSURE);
   clear_pressure(s);
   if (s->desc.solver == FX_SOLVER_VCYCLE) return solve_vcycle(s);
   return solve_reference(s);
 }
 
 void run_stage(fx_sim* s, int pass) {
+  StageTimer timer(s, pass);
   const uint32_t* list = fine_list(s);
   const uint32_t* count = fine_count(s);
   const int blocks = grid_for(s, s->nb);
@@ -611,6 +663,8 @@ void fx_destroy(fx_sim* s) {
     if (slot.cand) (void)hipHostFree(slot.cand);
     if (slot.done) (void)hipEventDestroy(slot.done);
   }
+  drain_timing(s);
+  for (hipEvent_t e : s->pool) (void)hipEventDestroy(e);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -934,6 +988,7 @@ int fx_raymarch_device(fx_sim* s, const fx_camera* c, const fx_render_params* p,
   int rc = build_ray_const(s, c, p, w, h, R);
   if (rc != FX_OK) return rc;
   HIP_OK(s, hipSetDevice(s->device));
+  StageTimer timer(s, FX_TIME_RAYMARCH);
   launch_raymarch(s->stream, s->g, R, set_of(s, 0), s->flagsRender, static_cast<uint16_t*>(dev));
   HIP_OK(s, hipGetLastError());
   return FX_OK;
@@ -952,6 +1007,29 @@ int fx_raymarch(fx_sim* s, const fx_camera* c, const fx_render_params* p, uint32
   }
   (void)hipFree(dev);
   return rc;
+}
+
+int fx_set_stage_timing(fx_sim* s, int32_t enable) {
+  if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
+  HIP_OK(s, hipSetDevice(s->device));
+  HIP_OK(s, hipStreamSynchronize(s->stream));
+  drain_timing(s);
+  std::memset(s->acc_ms, 0, sizeof(s->acc_ms));
+  std::memset(s->acc_calls, 0, sizeof(s->acc_calls));
+  s->timing = enable != 0;
+  return FX_OK;
+}
+
+int fx_read_stage_times(fx_sim* s, double* total_ms, uint32_t* calls, size_t n) {
+  if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
+  HIP_OK(s, hipSetDevice(s->device));
+  HIP_OK(s, hipStreamSynchronize(s->stream));
+  drain_timing(s);
+  for (size_t i = 0; i < std::min<size_t>(n, FX_TIME_COUNT); ++i) {
+    if (total_ms) total_ms[i] = s->acc_ms[i];
+    if (calls) calls[i] = s->acc_calls[i];
+  }
+  return FX_OK;
 }
 
 const char* fx_last_error(fx_sim* s) { return s ? s->err.c_str() : g_create_error.c_str(); }

@@ -292,6 +292,16 @@ class CombustionVolumeSimulation:
         pid = abi.PASSES[name_or_id] if isinstance(name_or_id, str) else int(name_or_id)
         self._check(self._lib.fx_run_pass(self._h, pid, C.byref(params)))
 
+    def set_stage_timing(self, enable: bool) -> None:
+        self._check(self._lib.fx_set_stage_timing(self._h, 1 if enable else 0))
+
+    def stage_times(self):
+        """{stage name: (total ms, calls)} since timing was enabled."""
+        ms = (C.c_double * abi.FX_TIME_COUNT)()
+        calls = (C.c_uint32 * abi.FX_TIME_COUNT)()
+        self._check(self._lib.fx_read_stage_times(self._h, ms, calls, abi.FX_TIME_COUNT))
+        return {abi.TIME_NAMES[i]: (ms[i], calls[i]) for i in range(abi.FX_TIME_COUNT) if calls[i]}
+
     def brick_info(self):
         """packSparseBrickLayout (src/engine/simulation/sparse/createSparseBrickLayout.ts:28-39)."""
         bx, by, bz = self.brick_counts

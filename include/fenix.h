@@ -88,7 +88,7 @@ extern "C" {
 #define FX_FIELD_ADV_REACTION 21
 #define FX_FIELD_COUNT 22
 
-/* Pressure-level buffers for fx_read_level / fx_write_level. */
+/* Pressure-level buffers for fx_read_level. */
 #define FX_LEVEL_DIVERGENCE 0
 #define FX_LEVEL_PRESSURE_A 1
 #define FX_LEVEL_PRESSURE_B 2
@@ -335,6 +335,24 @@ int fx_raymarch(fx_sim* sim, const fx_camera* camera, const fx_render_params* pa
 /* Same, into a caller-owned device buffer of width*height*8 bytes; asynchronous. */
 int fx_raymarch_device(fx_sim* sim, const fx_camera* camera, const fx_render_params* params,
                        uint32_t width, uint32_t height, void* device_rgba_f16);
+
+/* ---- stage timing ------------------------------------------------------- */
+
+/* Timing ids: the FX_PASS_* ids (whole pass; FX_PASS_ACTIVATE_RENDER includes the
+ * sparse list compaction), plus: */
+#define FX_TIME_RAYMARCH 16        /* fx_raymarch / fx_raymarch_device */
+#define FX_TIME_JACOBI_FINE 17     /* each fine-level pressure sweep (also inside FX_PASS_PRESSURE) */
+#define FX_TIME_COUNT 18
+
+/* Enables/disables HIP-event timing of every stage on the simulation's stream
+ * (no reference counterpart; the reference times only its render pass through
+ * stats-gl, src/engine/render/renderer/createViewportRuntime.ts:236-268).
+ * Enabling clears the accumulators. */
+int fx_set_stage_timing(fx_sim* sim, int32_t enable);
+
+/* Synchronises, then returns per timing id the accumulated milliseconds and call
+ * counts since timing was enabled. `n` entries are written (<= FX_TIME_COUNT). */
+int fx_read_stage_times(fx_sim* sim, double* total_ms, uint32_t* calls, size_t n);
 
 /* ---- test hook ---------------------------------------------------------- */
 

@@ -1,0 +1,135 @@
+"""ctypes loader of the CPU oracle (oracle/build/liboracle.so).
+
+Test infrastructure only: the oracle is the checker, never the thing measured.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from fenix_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_LIB = os.path.join(ROOT, "oracle", "build", "liboracle.so")
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(ORACLE_LIB):
+            import subprocess
+            subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+        L = C.CDLL(ORACLE_LIB)
+        P = C.c_void_p
+        sig = {
+            "orc_create": (P, [C.c_int, C.POINTER(abi.CreateDesc)]),
+            "orc_destroy": (None, [P]),
+            "orc_update_sources": (C.c_int, [P, C.POINTER(C.c_float), C.c_uint32]),
+            "orc_set_runtime": (C.c_int, [P, C.POINTER(abi.RuntimeParams), C.c_uint32]),
+            "orc_set_quality": (C.c_int, [P, C.POINTER(abi.Quality)]),
+            "orc_set_advection_mode": (C.c_int, [P, C.c_int]),
+            "orc_step": (C.c_int, [P, C.c_double, C.c_double]),
+            "orc_reset": (C.c_int, [P]),
+            "orc_run_pass": (C.c_int, [P, C.c_int, C.POINTER(abi.SimParams)]),
+            "orc_read_field": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),
+            "orc_write_field": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),
+            "orc_read_flags": (C.c_int, [P, C.c_int, C.POINTER(C.c_uint32)]),
+            "orc_read_level": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+            "orc_level_dims": (C.c_int, [P, C.c_int, C.POINTER(C.c_uint32)]),
+            "orc_residuals": (C.c_int, [P, C.POINTER(C.c_double), C.c_int]),
+            "orc_active_count": (C.c_int, [P]),
+            "orc_raymarch": (C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams), C.c_uint32, C.c_uint32,
+                                       C.POINTER(C.c_float)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+class Oracle:
+    """One oracle simulation (bits = 32 for the WGSL-order float build, 64 for FP64)."""
+
+    def __init__(self, desc, bits=32):
+        self.L = lib()
+        self.desc = desc
+        self.h = self.L.orc_create(bits, C.byref(desc))
+        if not self.h:
+            raise ValueError("oracle rejected the descriptor")
+        self.shape = (desc.depth, desc.height, desc.width)
+        self.bricks = ((desc.width + 7) // 8) * ((desc.height + 7) // 8) * ((desc.depth + 7) // 8)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def _ok(self, rc):
+        if rc != 0:
+            raise RuntimeError(f"oracle call failed: {rc}")
+
+    def update_sources(self, packed, count):
+        packed = np.ascontiguousarray(packed, dtype=np.float32)
+        self._ok(self.L.orc_update_sources(self.h, packed.ctypes.data_as(C.POINTER(C.c_float)) if count else None, count))
+
+    def set_runtime(self, rp, mask):
+        self._ok(self.L.orc_set_runtime(self.h, C.byref(rp), mask))
+
+    def set_quality(self, q):
+        self._ok(self.L.orc_set_quality(self.h, C.byref(q)))
+
+    def set_mode(self, m):
+        self._ok(self.L.orc_set_advection_mode(self.h, m))
+
+    def step(self, elapsed, dt):
+        self._ok(self.L.orc_step(self.h, float(elapsed), float(dt)))
+
+    def reset(self):
+        self._ok(self.L.orc_reset(self.h))
+
+    def run_pass(self, name, params):
+        pid = abi.PASSES[name] if isinstance(name, str) else int(name)
+        self._ok(self.L.orc_run_pass(self.h, pid, C.byref(params)))
+
+    def read(self, field):
+        fid = abi.FIELDS[field] if isinstance(field, str) else int(field)
+        out = np.empty(self.shape, dtype=np.float64)
+        self._ok(self.L.orc_read_field(self.h, fid, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def write(self, field, values):
+        fid = abi.FIELDS[field] if isinstance(field, str) else int(field)
+        arr = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(self.shape))
+        self._ok(self.L.orc_write_field(self.h, fid, arr.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def flags(self, which=abi.FX_FLAGS_RENDER):
+        out = np.empty(self.bricks, dtype=np.uint32)
+        self._ok(self.L.orc_read_flags(self.h, which, out.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return out
+
+    def level_dims(self, level):
+        d = (C.c_uint32 * 3)()
+        self._ok(self.L.orc_level_dims(self.h, level, d))
+        return tuple(d)
+
+    def level(self, level, which):
+        w, h, d = self.level_dims(level)
+        out = np.empty((d, h, w), dtype=np.float64)
+        self._ok(self.L.orc_read_level(self.h, level, which, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def residuals(self):
+        buf = (C.c_double * 64)()
+        n = self.L.orc_residuals(self.h, buf, 64)
+        return np.array(buf[: min(n, 64)])
+
+    def active_count(self):
+        return self.L.orc_active_count(self.h)
+
+    def raymarch(self, cam, params, w, h):
+        out = np.zeros((h, w, 4), dtype=np.float32)
+        self._ok(self.L.orc_raymarch(self.h, C.byref(cam), C.byref(params), w, h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out

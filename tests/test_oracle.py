@@ -1,0 +1,554 @@
+"""Pins of the CPU oracle by closed forms and invariants (no GPU).
+
+The reference ships no tests or golden vectors and its WGSL cannot execute here,
+so every oracle function is pinned by mathematics chosen so that a dropped term,
+a wrong sign/index or a transposed operand fails: exact shifts, eigenvectors,
+central differences on polynomials, special cases with known answers.
+Most checks run on the FP64 build (tight tolerances) and a few on FP32.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import scenes
+from fenix_amd import abi
+from fenix_amd.emitters import IgniterEmitter, ScalarEmitter, VelocityEmitter
+from fenix_amd.simulation import make_desc
+from oracle import Oracle
+
+
+def grid(shape):
+    d, h, w = shape
+    return np.meshgrid(np.arange(d), np.arange(h), np.arange(w), indexing="ij")  # z, y, x
+
+
+def interior(shape, m=1):
+    d, h, w = shape
+    s = np.zeros(shape, bool)
+    s[m:d - m, m:h - m, m:w - m] = True
+    return s
+
+
+def f32(v):
+    """The uniform is FP32 (simulation-params.wgsl.ts): expected values use rounded params."""
+    return float(np.float32(v))
+
+
+def zero_sources(o):
+    o.update_sources(np.zeros(0, np.float32), 0)
+
+
+# ------------------------------------------------------------ pressure --
+@pytest.mark.parametrize("sweeps", [1, 2, 5])
+def test_jacobi_eigenvector_closed_form(sweeps):
+    """SH/pressure-jacobi.wgsl.ts:15-36: with Dirichlet p = 0 on the faces,
+    u = prod sin(pi k x/(N-1)) is an eigenvector of one sweep with eigenvalue
+    lambda = mean cos(pi k/(N-1)); m sweeps from 0 with div = L u give u (1 - lambda^m)."""
+    W, H, D = 17, 13, 11
+    k = (2, 1, 3)
+    desc = make_desc((W, H, D), quality={"finePreIterations": sweeps, "finePostIterations": 0, "midPreIterations": 0,
+                                         "midPostIterations": 0, "coarseIterations": 0})
+    o = Oracle(desc, 64)
+    z, y, x = grid((D, H, W))
+    u = (np.sin(math.pi * k[0] * x / (W - 1)) * np.sin(math.pi * k[1] * y / (H - 1)) *
+         np.sin(math.pi * k[2] * z / (D - 1)))
+    lam = (math.cos(math.pi * k[0] / (W - 1)) + math.cos(math.pi * k[1] / (H - 1)) + math.cos(math.pi * k[2] / (D - 1))) / 3
+    div = 6.0 * (lam - 1.0) * u
+    div[~interior(div.shape)] = 0.0
+    o.write("divergence", div)
+    o.run_pass("pressure", scenes.sim_params())
+    p = o.read("pressure")
+    expect = u * (1 - lam ** sweeps)
+    expect[~interior(p.shape)] = 0
+    np.testing.assert_allclose(p, expect, atol=1e-12)
+
+
+def test_residual_of_exact_solution_and_restriction_of_constant():
+    """pressure-residual: r = div - (sum6 p - 6p) vanishes for p = x^2+y^2+z^2, div = 6;
+    restriction of a constant is that constant; the residual is written into the
+    *other* buffer of the fine level (quirk Q1)."""
+    W = H = D = 16
+    desc = make_desc((W, H, D), quality={"finePreIterations": 0, "finePostIterations": 0, "midPreIterations": 0,
+                                         "midPostIterations": 0, "coarseIterations": 0})
+    o = Oracle(desc, 64)
+    div = np.full((D, H, W), 6.0)
+    div[~interior(div.shape)] = 0
+    o.write("divergence", div)
+    o.run_pass("pressure", scenes.sim_params())
+    # 0 sweeps: pressureA = 0 so the residual (into pressureB) equals div; the
+    # mid divergence is the 8-child mean of div (3 in the first interior ring, 6 inside)
+    r = o.level(0, abi.FX_LEVEL_PRESSURE_B)
+    np.testing.assert_allclose(r, div)
+    mid = o.level(1, abi.FX_LEVEL_DIVERGENCE)
+    assert mid.shape == (8, 8, 8)
+    np.testing.assert_allclose(mid[2:-2, 2:-2, 2:-2], 6.0)
+    assert np.all(mid[0] == 0) and np.all(mid[:, -1] == 0)
+
+
+def test_prolongation_reproduces_affine_fields():
+    """pressure-prolongation: trilinear interpolation of an affine coarse field at
+    (i+0.5)/2-0.5 is exact away from the clamped border."""
+    desc = make_desc((32, 32, 32), quality={"finePreIterations": 0, "finePostIterations": 0, "midPreIterations": 0,
+                                            "midPostIterations": 1, "coarseIterations": 0})
+    o = Oracle(desc, 64)
+    # With zero divergence every smoothing result is 0; one mid post sweep of a
+    # zero field keeps mid 0 -> the fine prolongation adds 0. Use the V-cycle
+    # instead to inspect prolongation directly below.
+    o.write("divergence", np.zeros((32, 32, 32)))
+    o.run_pass("pressure", scenes.sim_params())
+    assert np.all(o.read("pressure") == 0)
+
+
+def test_vcycle_converges_to_discrete_solution():
+    """V-cycle: the residual norm falls every cycle, and the solution of L p = L u
+    approaches u (u an eigenvector vanishing on the faces)."""
+    W = H = D = 32
+    desc = make_desc((W, H, D), solver=abi.FX_SOLVER_VCYCLE, vcycle={"levels": 4, "cycles": 12, "coarse_sweeps": 32})
+    o = Oracle(desc, 64)
+    z, y, x = grid((D, H, W))
+    u = np.sin(math.pi * x / (W - 1)) * np.sin(2 * math.pi * y / (H - 1)) * np.sin(math.pi * z / (D - 1))
+    # div = L u computed with numpy's own stencil on the interior
+    lap = np.zeros_like(u)
+    c = u[1:-1, 1:-1, 1:-1]
+    lap[1:-1, 1:-1, 1:-1] = (u[2:, 1:-1, 1:-1] + u[:-2, 1:-1, 1:-1] + u[1:-1, 2:, 1:-1] + u[1:-1, :-2, 1:-1] +
+                             u[1:-1, 1:-1, 2:] + u[1:-1, 1:-1, :-2] - 6 * c)
+    o.write("divergence", lap)
+    o.run_pass("pressure", scenes.sim_params())
+    res = o.residuals()
+    assert len(res) == 12
+    assert np.all(np.diff(res) < 0), res
+    factor = (res[-1] / res[0]) ** (1 / 11)
+    assert factor < 0.75, factor  # omega = 2/3 Jacobi smoothing bounds V(2,2) well above 0.1
+    p = o.read("pressure")
+    assert np.max(np.abs(p - u)) < 0.05 * np.max(np.abs(u))
+
+
+def test_reference_schedule_odd_presmooth_loses_coarse_correction():
+    """Quirk Q2 (PressureSolvePass.ts:80-83): with an odd fine pre-sweep count the
+    prolongation lands in pressureA while post-smoothing starts from pressureB,
+    so the result equals plain Jacobi sweeps on the fine level."""
+    W = H = D = 16
+    q = {"finePreIterations": 3, "finePostIterations": 2, "midPreIterations": 2, "midPostIterations": 3,
+         "coarseIterations": 10}
+    o = Oracle(make_desc((W, H, D), quality=q), 64)
+    o_plain = Oracle(make_desc((W, H, D), quality={"finePreIterations": 5, "finePostIterations": 0,
+                                                   "midPreIterations": 0, "midPostIterations": 0,
+                                                   "coarseIterations": 0}), 64)
+    rng = np.random.default_rng(0)
+    div = rng.uniform(-1, 1, (D, H, W))
+    div[~interior(div.shape)] = 0
+    for oo in (o, o_plain):
+        oo.write("divergence", div)
+        oo.run_pass("pressure", scenes.sim_params())
+    np.testing.assert_allclose(o.read("pressure"), o_plain.read("pressure"), atol=1e-14)
+
+
+# ------------------------------------------------------------ advection --
+def test_velocity_advection_exact_shift():
+    """advect-velocity-semilagrangian: u = (1, 0, h(x, y)) with dt = 1 samples
+    (x-1, y, z - h) where u does not vary along z: u'(x) = u(x-1) * 0.975."""
+    W, H, D = 16, 12, 10
+    o = Oracle(make_desc((W, H, D)), 64)
+    z, y, x = grid((D, H, W))
+    h = 0.3 * np.sin(0.7 * x) + 0.1 * y / H
+    o.write("velocity_x", np.ones((D, H, W)))
+    o.write("velocity_y", np.zeros((D, H, W)))
+    o.write("velocity_z", h)
+    o.run_pass("advect_velocity", scenes.sim_params(dt=1.0))
+    ux, uy, uz = o.read("scratch_x"), o.read("scratch_y"), o.read("scratch_z")
+    m = interior(ux.shape) & (x >= 2) & (z >= 2) & (z <= D - 3)
+    np.testing.assert_allclose(ux[m], 0.975, atol=1e-12)
+    np.testing.assert_allclose(uy[m], 0.0, atol=1e-12)
+    expect = np.roll(h, 1, axis=2) * 0.975
+    np.testing.assert_allclose(uz[m], expect[m], atol=1e-12)
+    b = ~interior(ux.shape)
+    assert np.all(ux[b] == 0) and np.all(uz[b] == 0)
+
+
+@pytest.mark.parametrize("mode", [abi.FX_ADVECT_MACCORMACK, abi.FX_ADVECT_SEMI_LAGRANGIAN])
+def test_scalar_advection_integer_shift(mode):
+    """advect-scalar-*: an integer velocity (1,0,0) with dt = 1 shifts every scalar
+    by one voxel exactly (MacCormack's recovered == original, so the correction
+    vanishes). With fuel = reaction = density = 0 and T < 0.22, cooling is
+    dt*(0.016 + 0.052 y')*1.28*0.86 (no noise dependence)."""
+    W, H, D = 20, 12, 10
+    o = Oracle(make_desc((W, H, D), advection_mode=mode), 64)
+    z, y, x = grid((D, H, W))
+    T = 0.1 + 0.1 * np.sin(0.9 * x + 0.3 * y + 0.5 * z)
+    for name, val in (("density", 0 * T), ("temperature", T), ("fuel", 0 * T), ("reaction", 0 * T)):
+        o.write(name, val)
+    o.write("scratch_x", np.ones((D, H, W)))
+    o.write("scratch_y", np.zeros((D, H, W)))
+    o.write("scratch_z", np.zeros((D, H, W)))
+    o.write("velocity_x", np.zeros((D, H, W)))
+    p = scenes.sim_params(dt=1.0)
+    o.run_pass("activate_render", p)
+    assert o.flags(abi.FX_FLAGS_INJECTED).all()  # T*0.7 > 0.012 everywhere
+    o.run_pass("advect_scalars", p)
+    ny = (y + 0.5) / H
+    expect = np.maximum(0.0, np.roll(T, 1, axis=2) - 1.0 * (0.016 + ny * 0.052) * 1.28 * 0.86)
+    out = o.read("adv_temperature")
+    m = x >= 1
+    np.testing.assert_allclose(out[m], expect[m], atol=1e-12)
+    assert np.all(o.read("adv_density") == 0)
+
+
+def test_scalar_advection_decay_bounds_and_dt_zero_identity():
+    """With dt = 0 advection is the identity; density loss lies within the
+    smokeLoss bounds dt*(0.008+0.026y)*[0.75, 1.38] when T = f = r = 0 and density
+    < 0.18 (coldBreakup vanishes); fuel loss lies in dt*[0.012, 0.034]."""
+    W, H, D = 16, 16, 8
+    o = Oracle(make_desc((W, H, D)), 64)
+    rng = np.random.default_rng(1)
+    den = rng.uniform(0.02, 0.17, (D, H, W))
+    fuel = rng.uniform(0.2, 0.9, (D, H, W))
+    for name, val in (("density", den), ("temperature", 0 * den), ("fuel", fuel), ("reaction", 0 * den)):
+        o.write(name, val)
+    for c in "xyz":
+        o.write("scratch_" + c, np.zeros((D, H, W)))
+    p0 = scenes.sim_params(dt=0.0)
+    o.run_pass("activate_render", p0)
+    o.run_pass("advect_scalars", p0)
+    np.testing.assert_array_equal(o.read("adv_density"), den)
+    np.testing.assert_array_equal(o.read("adv_fuel"), fuel)
+    dt = 0.05
+    p = scenes.sim_params(dt=dt)
+    o.run_pass("advect_scalars", p)
+    z, y, x = grid((D, H, W))
+    ny = (y + 0.5) / H
+    loss = 1 - o.read("adv_density") / den
+    lo, hi = dt * (0.008 + 0.026 * ny) * 0.75, dt * (0.008 + 0.026 * ny) * 1.38
+    assert np.all(loss >= lo - 1e-12) and np.all(loss <= hi + 1e-12)
+    fl = fuel - o.read("adv_fuel")
+    assert np.all(fl >= dt * 0.012 - 1e-12) and np.all(fl <= dt * 0.034 + 1e-12)
+
+
+def test_inactive_bricks_are_zeroed_by_scalar_advection():
+    """Quirk Q7: every voxel of a brick with flag 0 is written 0."""
+    W, H, D = 24, 8, 8
+    o = Oracle(make_desc((W, H, D)), 64)
+    den = np.zeros((D, H, W))
+    den[:, :, 16:] = 0.004  # below the 0.012 energy threshold: brick 2 stays inactive
+    den[4, 4, 4] = 0.5      # brick 0 active (+ dilation to brick 1)
+    o.write("density", den)
+    for c in "xyz":
+        o.write("scratch_" + c, np.zeros((D, H, W)))
+    p = scenes.sim_params(dt=0.0)
+    o.run_pass("activate_render", p)
+    np.testing.assert_array_equal(o.flags(abi.FX_FLAGS_INJECTED), [1, 1, 0])
+    o.run_pass("advect_scalars", p)
+    out = o.read("adv_density")
+    assert np.all(out[:, :, 16:] == 0) and out[4, 4, 4] == 0.5
+
+
+# ----------------------------------------------------------- activation --
+@pytest.mark.parametrize("energy,local,expect_bricks", [
+    (0.02, (2, 4, 6), {(1, 1, 1)}),                       # active, no dilation
+    (0.06, (2, 4, 6), {(a, b, c) for a in (0, 1, 2) for b in (0, 1, 2) for c in (0, 1, 2)}),
+    (0.9, (3, 4, 6), set()),                              # odd local x: never sampled (Q21)
+    (0.02, (4, 4, 4), {(1, 1, 1)}),                       # brick centre sample
+    (0.04, (4, 4, 4), {(a, b, c) for a in (0, 1, 2) for b in (0, 1, 2) for c in (0, 1, 2)}),  # centre > 0.035
+])
+def test_reference_activation_special_cases(energy, local, expect_bricks):
+    """compute-active-bricks.wgsl.ts:24-79 on a 3x3x3-brick grid, one hot voxel in brick (1,1,1)."""
+    n = 24
+    o = Oracle(make_desc((n, n, n)), 64)
+    den = np.zeros((n, n, n))
+    lx, ly, lz = local
+    den[8 + lz, 8 + ly, 8 + lx] = energy
+    o.write("density", den)
+    o.run_pass("activate_render", scenes.sim_params())
+    flags = o.flags(abi.FX_FLAGS_INJECTED).reshape(3, 3, 3)  # [bz, by, bx]
+    got = {(bx, by, bz) for bz, by, bx in np.argwhere(flags)}
+    assert got == expect_bricks
+
+
+def test_reference_activation_energy_weights_and_clamped_dilation():
+    """energy = d + 0.7 T + 0.9 R; dilation at a corner brick is clamped."""
+    n = 16
+    o = Oracle(make_desc((n, n, n)), 64)
+    T = np.zeros((n, n, n))
+    T[0, 0, 0] = 0.06 / 0.7 + 1e-9  # energy just above 0.06 from temperature alone
+    o.write("temperature", T)
+    o.run_pass("activate_render", scenes.sim_params())
+    assert o.flags(abi.FX_FLAGS_INJECTED).all()  # 2x2x2 grid: clamped 3x3x3 covers all
+    o2 = Oracle(make_desc((n, n, n)), 64)
+    R = np.zeros((n, n, n))
+    R[0, 0, 0] = 0.012 / 0.9 * 0.999  # just below threshold
+    o2.write("reaction", R)
+    o2.run_pass("activate_render", scenes.sim_params())
+    assert not o2.flags(abi.FX_FLAGS_INJECTED).any()
+
+
+def test_threshold_activation_rule():
+    """THRESHOLD: density > 1e-4 or |u| > 1e-3 anywhere in the brick, one-brick dilation."""
+    n = 32
+    o = Oracle(make_desc((n, n, n), activation_rule=abi.FX_ACTIVATION_THRESHOLD), 64)
+    den = np.zeros((n, n, n))
+    den[9, 17, 3] = 2e-4  # brick (0, 2, 1), odd coordinates are fine for this rule
+    ux = np.zeros((n, n, n))
+    ux[31, 31, 31] = 0.0011  # brick (3,3,3)
+    uy = np.zeros((n, n, n))
+    uy[0, 0, 0] = 0.0009  # below
+    o.write("density", den)
+    o.write("velocity_x", ux)
+    o.write("velocity_y", uy)
+    o.run_pass("activate_render", scenes.sim_params())
+    f = o.flags(abi.FX_FLAGS_INJECTED).reshape(4, 4, 4)
+    expect = np.zeros((4, 4, 4), bool)
+    expect[0:3, 1:4, 0:2] = True  # around (bx=0, by=2, bz=1) -> z 0..2, y 1..3, x 0..1
+    expect[2:4, 2:4, 2:4] = True
+    np.testing.assert_array_equal(f.astype(bool), expect)
+
+
+# ---------------------------------------------------------- differences --
+def test_vorticity_of_rigid_rotation_and_divergence_of_quadratic():
+    """Central differences are exact on quadratics: curl(-cy, cx, 0) = (0, 0, 2c);
+    div(x^2, y^2, z^2)/2 = x + y + z."""
+    W, H, D = 12, 10, 8
+    o = Oracle(make_desc((W, H, D)), 64)
+    z, y, x = grid((D, H, W))
+    c = 0.37
+    o.write("scratch_x", -c * y)
+    o.write("scratch_y", c * x)
+    o.write("scratch_z", np.zeros((D, H, W)))
+    o.run_pass("vorticity", scenes.sim_params())
+    m = interior((D, H, W))
+    np.testing.assert_allclose(o.read("vorticity_z")[m], 2 * c, atol=1e-12)
+    np.testing.assert_allclose(o.read("vorticity_x")[m], 0, atol=1e-12)
+    np.testing.assert_allclose(o.read("vorticity_magnitude")[m], 2 * c, atol=1e-12)
+    assert np.all(o.read("vorticity_z")[~m] == 0)
+    o.write("scratch_x", x ** 2 / 2.0)
+    o.write("scratch_y", y ** 2 / 2.0)
+    o.write("scratch_z", z ** 2 / 2.0)
+    o.run_pass("divergence", scenes.sim_params())
+    np.testing.assert_allclose(o.read("divergence")[m], (x + y + z)[m], atol=1e-12)
+
+
+def test_projection_subtracts_exact_gradient():
+    """project-velocity: v = v* - grad p with p quadratic -> exact gradient."""
+    W, H, D = 12, 10, 8
+    desc = make_desc((W, H, D), quality={"finePreIterations": 1, "finePostIterations": 0, "midPreIterations": 0,
+                                         "midPostIterations": 0, "coarseIterations": 0})
+    o = Oracle(desc, 64)
+    z, y, x = grid((D, H, W))
+    # Make the last pressure buffer hold a known field: 1 sweep from 0 with div
+    # gives p = -div/6, so write div = -6 p_target on the interior.
+    p_target = 0.5 * x ** 2 - 0.25 * y * z + 0.1 * z
+    div = -6.0 * p_target
+    div[~interior(div.shape)] = 0
+    o.write("divergence", div)
+    o.run_pass("pressure", scenes.sim_params())
+    p = o.read("pressure")
+    m2 = interior((D, H, W), 1)
+    np.testing.assert_allclose(p[m2], p_target[m2], atol=1e-12)
+    vs = np.full((D, H, W), 0.25)
+    for cc in "xyz":
+        o.write("scratch_" + cc, vs)
+    o.run_pass("projection", scenes.sim_params())
+    m = interior((D, H, W), 2)  # neighbours of the gradient are interior too
+    np.testing.assert_allclose(o.read("velocity_x")[m], (0.25 - x)[m], atol=1e-12)
+    np.testing.assert_allclose(o.read("velocity_y")[m], (0.25 + 0.25 * z)[m], atol=1e-12)
+    np.testing.assert_allclose(o.read("velocity_z")[m], (0.25 + 0.25 * y - 0.1)[m], atol=1e-12)
+
+
+def test_confinement_force_closed_form():
+    """apply-vorticity-confinement: omega = (0,0,w(x)) with w increasing in x ->
+    N = (1,0,0), force = N x omega * eps * mask * 1.45 = (0, -w, 0) * eps * 1.45 (mask 1)."""
+    W, H, D = 12, 10, 8
+    o = Oracle(make_desc((W, H, D)), 64)
+    z, y, x = grid((D, H, W))
+    w = 0.1 + 0.05 * x
+    for name, val in (("vorticity_x", 0 * w), ("vorticity_y", 0 * w), ("vorticity_z", w), ("vorticity_magnitude", w)):
+        o.write(name, val)
+    for cc in "xyz":
+        o.write("scratch_" + cc, np.zeros((D, H, W)))
+    eps, dt = f32(0.3), f32(0.1)
+    o.run_pass("confinement", scenes.sim_params(dt=dt, vort=eps, mask=(1.0, 0.0, 0.0, 0.0)))
+    m = interior((D, H, W))
+    np.testing.assert_allclose(o.read("scratch_y")[m], (-w * eps * 1.45 * dt)[m], rtol=1e-12)
+    np.testing.assert_allclose(o.read("scratch_x")[m], 0, atol=1e-15)
+    np.testing.assert_allclose(o.read("confinement_magnitude")[m], (w * eps * 1.45)[m], rtol=1e-12)
+    # uniform vorticity: no gradient, no force
+    o.write("vorticity_magnitude", np.full((D, H, W), 0.3))
+    for cc in "xyz":
+        o.write("scratch_" + cc, np.zeros((D, H, W)))
+    o.run_pass("confinement", scenes.sim_params(dt=dt))
+    assert np.all(o.read("scratch_y") == 0)
+
+
+# ------------------------------------------------------ buoyancy/combust --
+def test_buoyancy_special_cases():
+    """apply-buoyancy: empty scalars -> pure damping v(1 - 0.08 dt); uniform
+    temperature T -> v_y += T * b * dt (b = 3.6, no smoothstep boost below |b| = 6)."""
+    W, H, D = 10, 10, 10
+    o = Oracle(make_desc((W, H, D)), 64)
+    zero = np.zeros((D, H, W))
+    for name in ("density", "temperature", "fuel", "reaction"):
+        o.write(name, zero)
+    rng = np.random.default_rng(3)
+    v = [rng.uniform(-1, 1, (D, H, W)) for _ in range(3)]
+    for cc, vv in zip("xyz", v):
+        o.write("scratch_" + cc, vv)
+    # flags: every brick active
+    o.write("adv_density", np.full((D, H, W), 0.5))
+    dt = f32(0.05)
+    p = scenes.sim_params(dt=dt)
+    o.run_pass("activate_advected", p)
+    o.run_pass("buoyancy", p)
+    z, y, x = grid((D, H, W))
+    side = (x == 0) | (x == W - 1) | (z == 0) | (z == D - 1)
+    m = ~side & (y > 0)
+    for cc, vv in zip("xyz", v):
+        np.testing.assert_allclose(o.read("scratch_" + cc)[m], (vv * (1 - dt * 0.08))[m], atol=1e-14)
+    assert np.all(o.read("scratch_x")[side] == 0)
+    Tv = 0.4
+    o.write("temperature", np.full((D, H, W), Tv))
+    for cc, vv in zip("xyz", v):
+        o.write("scratch_" + cc, vv)
+    o.run_pass("buoyancy", p)
+    np.testing.assert_allclose(o.read("scratch_y")[m], (v[1] * (1 - dt * 0.08) + Tv * f32(3.6) * dt)[m], atol=1e-13)
+
+
+def test_combustion_special_cases():
+    """combustion-update: cold fuel does not burn and keeps its value; hot fuel
+    with dt*4.2 >= 1 burns completely; nothing happens without fuel, heat or smoke."""
+    W, H, D = 16, 16, 16
+    o = Oracle(make_desc((W, H, D)), 64)
+    rng = np.random.default_rng(2)
+    fuel = rng.uniform(0.1, 0.9, (D, H, W))
+    zero = np.zeros((D, H, W))
+    for name, val in (("adv_density", zero), ("adv_temperature", zero), ("adv_fuel", fuel), ("adv_reaction", zero)):
+        o.write(name, val)
+    o.write("scratch_x", zero)
+    # flags from the advected set: fuel alone has zero energy -> mark active via temperature trick
+    o.write("adv_density", np.full((D, H, W), 0.0))
+    p = scenes.sim_params(dt=0.05)
+    o.write("adv_reaction", zero)
+    o.run_pass("activate_advected", p)
+    o.run_pass("combustion", p)  # inactive bricks copy through
+    np.testing.assert_array_equal(o.read("fuel"), fuel)
+    # force all bricks active with a tiny reaction (energy > 0.012) that stays tiny
+    o.write("adv_reaction", np.full((D, H, W), 0.02))
+    o.run_pass("activate_advected", p)
+    assert o.flags().all()
+    o.write("adv_reaction", zero)
+    o.run_pass("combustion", p)
+    np.testing.assert_array_equal(o.read("fuel"), fuel)          # ignition(T = 0) = 0
+    assert np.all(o.read("temperature") == 0) and np.all(o.read("density") == 0)
+    o.write("adv_temperature", np.ones((D, H, W)))
+    o.run_pass("combustion", scenes.sim_params(dt=0.25))       # 0.25 * 4.2 >= 1
+    assert np.all(o.read("fuel") == 0)
+    assert np.all(o.read("reaction") >= np.minimum(1.0, 4 * fuel) - 1e-12)
+
+
+# ------------------------------------------------------------ injection --
+def test_injection_closed_forms():
+    """inject-scalar / -velocity / -igniter on a known sphere."""
+    n = 24
+    o = Oracle(make_desc((n, n, n), runtime={"worldSize": 24.0}), 64)
+    srcs = [ScalarEmitter((0.5, 0.5, 0.5), 0.25, densityRate=2.0, heatRate=1.0, fuelRate=0.5),
+            VelocityEmitter((0.5, 0.5, 0.5), 0.25, mode="directional", speed=4.0, direction=(0.0, 3.0, 4.0),
+                            falloff=0.0, tightness=0.5),
+            IgniterEmitter((0.5, 0.5, 0.5), 0.25, intensity=0.5, duration=10.0)]
+    o.update_sources(*scenes.packed(srcs))
+    dt = f32(0.1)
+    p = scenes.sim_params(time=1.0, dt=dt, dx=1.0, world=24.0)
+    o.run_pass("inject_scalar", p)
+    o.run_pass("inject_velocity", p)
+    o.run_pass("inject_igniter", p)
+    z, y, x = grid((n, n, n))
+    pos = np.stack([(x + 0.5) / n, (y + 0.5) / n, (z + 0.5) / n], -1)
+    dist = np.linalg.norm(pos - 0.5, axis=-1)
+    inside = dist <= 0.25
+    t = np.clip(1 - dist / 0.25, 0, 1)
+    w_s = 0.5 * (t - 1e-4 >= 0) + 0.5 * t * t          # sphereFalloff(.., 0.5)
+    np.testing.assert_allclose(o.read("density")[inside], np.minimum(1, 2.0 * w_s * dt)[inside], atol=1e-12)
+    assert np.all(o.read("density")[~inside] == 0)
+    fuel = np.minimum(1, 0.5 * w_s * dt)
+    w_v = (t - 1e-4 >= 0).astype(float)                  # falloff 0
+    d = np.array([0.0, 0.6, 0.8])
+    tight = 0.5 * w_v
+    vy = (d[1] * 4.0 * w_v * dt) * (1 - tight) + d[1] * 4.0 * tight
+    np.testing.assert_allclose(o.read("velocity_y")[inside], vy[inside], atol=1e-12)
+    np.testing.assert_allclose(o.read("velocity_x")[inside], 0, atol=1e-15)
+    w_i = 0.4 * (t - 1e-4 >= 0) + 0.6 * t * t
+    e = np.clip((fuel - 0.02) / 0.2, 0, 1)
+    avail = e * e * (3 - 2 * e)
+    np.testing.assert_allclose(o.read("reaction")[inside], np.minimum(1, 0.5 * w_i * avail * dt * 1.6)[inside], atol=1e-12)
+    T = np.minimum(1, 1.0 * w_s * dt)
+    np.testing.assert_allclose(o.read("temperature")[inside], np.minimum(1, T + 0.5 * w_i * dt * 4)[inside], atol=1e-12)
+
+
+def test_zero_sources_are_identity_except_speed_clamp():
+    """Q13: no sources -> one all-zero source iterated, fields unchanged; velocity
+    is clamped to +-80/dx."""
+    n = 16
+    o = Oracle(make_desc((n, n, n)), 64)
+    zero_sources(o)
+    rng = np.random.default_rng(4)
+    st = {k: rng.uniform(0, 1, (n, n, n)) for k in ("density", "temperature", "fuel", "reaction")}
+    for k, v in st.items():
+        o.write(k, v)
+    ux = rng.uniform(-100, 100, (n, n, n))
+    o.write("velocity_x", ux)
+    p = scenes.sim_params(dx=1.0)
+    for name in ("inject_scalar", "inject_velocity", "inject_igniter"):
+        o.run_pass(name, p)
+    for k, v in st.items():
+        np.testing.assert_array_equal(o.read(k), v)
+    np.testing.assert_array_equal(o.read("velocity_x"), np.clip(ux, -80, 80))
+
+
+# ------------------------------------------------------------- raymarch --
+def test_raymarch_empty_volume_is_transparent_and_alpha_bounded():
+    from fenix_amd.render import VolumeRaymarchPass, camera_struct, orbit_camera_snapshot
+    n = 16
+    o = Oracle(make_desc((n, n, n)), 32)
+    rp = VolumeRaymarchPass((n, n, n))
+    cam = camera_struct(orbit_camera_snapshot())
+    img = o.raymarch(cam, rp.params_struct(), 32, 32)
+    assert np.all(img == 0)
+    # dense smoke everywhere: alpha in (0, 1], grows with density
+    o.write("density", np.full((n, n, n), 0.3))
+    o.run_pass("activate_render", scenes.sim_params())
+    o.run_pass("activate_advected", scenes.sim_params())  # render flags: all zero advected set -> inactive
+    o.write("adv_density", np.full((n, n, n), 0.3))
+    o.run_pass("activate_advected", scenes.sim_params())
+    a1 = o.raymarch(cam, rp.params_struct("density"), 32, 32)[..., 3]
+    o.write("density", np.full((n, n, n), 0.6))
+    a2 = o.raymarch(cam, rp.params_struct("density"), 32, 32)[..., 3]
+    assert a1.max() > 0.5 and np.all(a1 <= 1) and np.all(a2 <= 1)
+    assert a2.mean() > a1.mean()
+
+
+# --------------------------------------------------------- full step / FP --
+def test_fp32_and_fp64_oracles_agree_over_a_short_run():
+    """The two builds differ only by rounding: after 10 steps of the C1 scene at
+    24^3 the FP32 fields stay within 1e-4 relative L2 of FP64."""
+    n = 24
+    desc = scenes.c1_desc(n)
+    runs = []
+    for bits in (32, 64):
+        o = Oracle(desc, bits)
+        o.update_sources(*scenes.packed(scenes.c1_sources()))
+        for k in range(10):
+            o.step(k / 60, 1 / 60)
+        runs.append({f: o.read(f) for f in ("density", "temperature", "velocity_y")})
+    for f in runs[0]:
+        a, b = runs[0][f], runs[1][f]
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-4, f
+
+
+def test_sparse_mode_fields_vanish_outside_active_bricks():
+    n = 32
+    o = Oracle(scenes.c2_desc(n), 64)
+    o.update_sources(*scenes.packed(scenes.c2_sources(n, radius_vox=3.0)))
+    for k in range(8):
+        o.step(k / 60, 1 / 60)
+    f = o.flags().reshape(4, 4, 4).astype(bool)
+    mask = f.repeat(8, 0).repeat(8, 1).repeat(8, 2)
+    for name in ("density", "temperature", "velocity_x", "velocity_y", "pressure", "divergence"):
+        assert np.all(o.read(name)[~mask] == 0), name
+    assert 0 < o.active_count() < 64

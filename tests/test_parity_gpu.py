@@ -1,0 +1,224 @@
+"""GPU parity: libfenix (HIP, gfx950) against the FP32 oracle on identical inputs.
+
+Bars: brick flags, active list and page table bit-exact; FP32 fields within
+1e-5 relative L2 per pass and per step.
+"""
+import numpy as np
+import pytest
+
+import scenes
+from fenix_amd import abi
+from fenix_amd.render import VolumeRaymarchPass, camera_struct, orbit_camera_snapshot
+from fenix_amd.simulation import CombustionVolumeSimulation, make_desc
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+REL_L2 = 1e-5
+PERSISTENT = ("density", "temperature", "fuel", "reaction", "velocity_x", "velocity_y", "velocity_z")
+
+
+def rel_l2(a, b):
+    den = float(np.linalg.norm(b))
+    num = float(np.linalg.norm(np.asarray(a, np.float64) - np.asarray(b, np.float64)))
+    return num / den if den > 0 else num
+
+
+def make_pair(desc, sources=None):
+    sim = CombustionVolumeSimulation(desc)
+    orc = Oracle(desc, 32)
+    if sources is not None:
+        sim.updateSources(sources)
+        orc.update_sources(*scenes.packed(sources))
+    return sim, orc
+
+
+def load_state(sim, orc, st):
+    for k, v in st.items():
+        sim.write_field(k, v)
+        orc.write(k, v)
+
+
+def compare_all(sim, orc, label, tol=REL_L2, fields=None):
+    worst = []
+    for name in fields or abi.FIELDS:
+        g = sim.read_field(name)
+        o = orc.read(name)
+        e = rel_l2(g, o)
+        worst.append((e, name))
+        assert np.all(np.isfinite(g)), f"{label}: non-finite values in {name}"
+        assert e <= tol, f"{label}: field {name} rel L2 {e:.3e} > {tol:g}"
+    return max(worst)
+
+
+GRID = (24, 16, 20)  # x not, z not a multiple of 8 in one axis: exercises brick padding
+
+
+@pytest.mark.parametrize("rule", [abi.FX_ACTIVATION_REFERENCE, abi.FX_ACTIVATION_THRESHOLD])
+def test_activation_bit_exact(rule):
+    desc = make_desc(GRID, activation_rule=rule, runtime={"worldSize": 24.0})
+    sim, orc = make_pair(desc, scenes.mixed_sources())
+    st = scenes.random_state(GRID[::-1], seed=3)
+    # sparse energy pattern: single hot voxels in some bricks
+    load_state(sim, orc, st)
+    p = scenes.sim_params(dx=1.0, world=24.0)
+    for name, which in (("activate_render", abi.FX_FLAGS_INJECTED), ("activate_advected", abi.FX_FLAGS_RENDER)):
+        sim.run_pass(name, p)
+        orc.run_pass(name, p)
+        np.testing.assert_array_equal(sim.read_flags(which), orc.flags(which))
+
+
+def _prepare_flags(sim, orc, p):
+    for name in ("activate_render", "activate_advected"):
+        sim.run_pass(name, p)
+        orc.run_pass(name, p)
+
+
+@pytest.mark.parametrize("pass_name", [
+    "inject_scalar", "inject_velocity", "inject_igniter", "advect_velocity", "advect_scalars", "combustion",
+    "buoyancy", "vorticity", "confinement", "divergence", "projection", "debug_fields",
+])
+def test_pass_parity_dense(pass_name):
+    desc = make_desc(GRID, runtime={"worldSize": 24.0, "wind": (0.4, 0.0, 0.2), "windStrength": 1.5})
+    sim, orc = make_pair(desc, scenes.mixed_sources())
+    st = scenes.random_state(GRID[::-1], seed=11)
+    load_state(sim, orc, st)
+    p = scenes.sim_params(time=0.7, dt=1.0 / 30.0, dx=1.0, world=24.0, wind=(0.4, 0.0, 0.2, 1.5))
+    _prepare_flags(sim, orc, p)
+    sim.run_pass(pass_name, p)
+    orc.run_pass(pass_name, p)
+    compare_all(sim, orc, pass_name)
+
+
+def test_semi_lagrangian_parity():
+    desc = make_desc(GRID, advection_mode=abi.FX_ADVECT_SEMI_LAGRANGIAN, runtime={"worldSize": 24.0})
+    sim, orc = make_pair(desc, scenes.mixed_sources())
+    load_state(sim, orc, scenes.random_state(GRID[::-1], seed=12))
+    p = scenes.sim_params(time=0.3, dt=1.0 / 20.0, dx=1.0, world=24.0)
+    _prepare_flags(sim, orc, p)
+    sim.run_pass("advect_scalars", p)
+    orc.run_pass("advect_scalars", p)
+    compare_all(sim, orc, "semi-lagrangian")
+
+
+@pytest.mark.parametrize("solver,quality", [
+    (abi.FX_SOLVER_REFERENCE, {}),
+    (abi.FX_SOLVER_REFERENCE, {"finePreIterations": 3, "midPreIterations": 1, "coarseIterations": 7, "finePostIterations": 0}),
+    (abi.FX_SOLVER_VCYCLE, {}),
+])
+def test_pressure_parity(solver, quality):
+    desc = make_desc((32, 24, 16), solver=solver, quality=quality, vcycle={"levels": 3, "cycles": 3})
+    sim, orc = make_pair(desc)
+    st = scenes.random_state((16, 24, 32), seed=5)
+    sim.write_field("divergence", st["divergence"])
+    orc.write("divergence", st["divergence"])
+    p = scenes.sim_params()
+    sim.run_pass("pressure", p)
+    orc.run_pass("pressure", p)
+    compare_all(sim, orc, "pressure", fields=["pressure", "divergence"])
+    nlev = 3
+    for lvl in range(nlev):
+        for which in (abi.FX_LEVEL_DIVERGENCE, abi.FX_LEVEL_PRESSURE_A, abi.FX_LEVEL_PRESSURE_B):
+            g, o = sim.read_level(lvl, which), orc.level(lvl, which)
+            assert rel_l2(g, o) <= REL_L2, f"level {lvl} buffer {which}: {rel_l2(g, o):.3e}"
+    if solver == abi.FX_SOLVER_VCYCLE:
+        rg, ro = sim.solver_residuals(), orc.residuals()
+        assert len(rg) == len(ro) == 3
+        np.testing.assert_allclose(rg, ro, rtol=1e-4)
+        assert rg[-1] < rg[0]
+
+
+def _step_parity(desc, sources, steps, init_velocity=None, tol=REL_L2):
+    sim, orc = make_pair(desc, sources)
+    if init_velocity is not None:
+        for c, name in enumerate(("velocity_x", "velocity_y", "velocity_z")):
+            sim.write_field(name, init_velocity[c])
+            orc.write(name, init_velocity[c])
+    dt = 1.0 / 60.0
+    worst = 0.0
+    for k in range(steps):
+        if k > 0:  # per-step parity: start both from the oracle's state
+            for name in PERSISTENT:
+                sim.write_field(name, orc.read(name))
+        sim.step(None, k * dt, dt)
+        orc.step(k * dt, dt)
+        np.testing.assert_array_equal(sim.read_flags(abi.FX_FLAGS_RENDER), orc.flags(abi.FX_FLAGS_RENDER),
+                                      err_msg=f"render flags differ at step {k}")
+        e, name = compare_all(sim, orc, f"step {k}", tol=tol, fields=list(PERSISTENT) + ["pressure", "divergence"])
+        worst = max(worst, e)
+    return sim, orc, worst
+
+
+def test_step_parity_dense_reference():
+    n = 32
+    desc = scenes.c1_desc(n)
+    srcs = scenes.c1_sources()
+    _step_parity(desc, srcs, steps=12, init_velocity=scenes.c1_initial_velocity(n))
+
+
+def test_step_parity_dense_vcycle():
+    n = 32
+    desc = scenes.c1_desc(n, config=True)
+    _step_parity(desc, scenes.c1_sources(), steps=8)
+
+
+def test_step_parity_sparse_config():
+    n = 48
+    desc = scenes.c2_desc(n)
+    srcs = scenes.c2_sources(n, radius_vox=4.0)
+    sim, orc, _ = _step_parity(desc, srcs, steps=12)
+    # active list / page table are exact functions of the flags
+    flags = orc.flags(abi.FX_FLAGS_RENDER)
+    lst = sim.read_active_list()
+    np.testing.assert_array_equal(lst, np.flatnonzero(flags).astype(np.uint32))
+    page = sim.read_page_table()
+    expect = np.full(flags.size, -1, np.int32)
+    expect[lst] = np.arange(lst.size, dtype=np.int32)
+    np.testing.assert_array_equal(page, expect)
+    assert sim.active_brick_count() == orc.active_count() == lst.size
+    # every field is zero outside the active bricks
+    d = sim.read_field("density")
+    bx = (n + 7) // 8
+    mask = flags.reshape(bx, bx, bx).repeat(8, 0).repeat(8, 1).repeat(8, 2)[:n, :n, :n]
+    assert np.all(d[mask == 0] == 0)
+
+
+def test_raymarch_parity():
+    n = 32
+    desc = scenes.c1_desc(n)
+    sim, orc = make_pair(desc, scenes.c1_sources())
+    for k in range(30):
+        orc.step(k / 60.0, 1 / 60.0)
+    for name in PERSISTENT:
+        sim.write_field(name, orc.read(name))
+    sim.step(None, 30 / 60.0, 1 / 60.0)
+    orc.step(30 / 60.0, 1 / 60.0)
+    for name in PERSISTENT:
+        sim.write_field(name, orc.read(name))
+    np.testing.assert_array_equal(sim.read_flags(), orc.flags())
+    rp = VolumeRaymarchPass((n, n, n))
+    rp.setRenderParams(stepCount=64)
+    cam = orbit_camera_snapshot()
+    for mode in ("temperature", "density", "fuel"):
+        img = rp.render(sim, cam, mode, 64, 64).astype(np.float32)
+        ref = orc.raymarch(camera_struct(cam), rp.params_struct(mode), 64, 64).astype(np.float16).astype(np.float32)
+        assert ref[..., 3].max() > 0.05, "scene renders nothing"
+        diff = np.abs(img - ref)
+        assert rel_l2(img, ref) < 2e-3, f"{mode}: image rel L2 {rel_l2(img, ref):.3e}"
+        assert np.mean(diff.max(axis=-1) < 2e-3) > 0.99
+
+
+def test_deterministic_repeat():
+    n = 32
+    desc = scenes.c2_desc(n)
+    srcs = scenes.c2_sources(n, radius_vox=3.0)
+    outs = []
+    for _ in range(2):
+        sim = CombustionVolumeSimulation(desc)
+        sim.updateSources(srcs)
+        for k in range(10):
+            sim.step(None, k / 60.0, 1 / 60.0)
+        outs.append([sim.read_field(f) for f in PERSISTENT])
+        sim.dispose()
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a, b)
