@@ -122,7 +122,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on exactly one GPU: this build has no multi-GPU path (see DESIGN.md, sharding).",
+        print("bench.py runs on exactly one GPU: there is no multi-GPU path (the sharding plan is design-only).",
               file=sys.stderr)
         sys.exit(2)
 
@@ -165,6 +165,8 @@ def main():
     sim.set_stage_timing(False)
     stage_ms = {k: v[0] / v[1] for k, v in stages.items()}
     candidates = {k: stages[k][0] for k in stages if k in STAGE_BYTES or k == "raymarch"}
+    if not candidates:
+        raise SystemExit("--timing-steps must be >= 1 to find the dominant kernel")
     dom = max(candidates, key=candidates.get)
     avg_ms = stage_ms[dom]
     if dom == "raymarch":

@@ -148,6 +148,11 @@ const uint32_t* fine_count(const fx_sim* s) { return s->sparse ? s->lv[0].L.coun
 const uint32_t* level_list(const fx_sim* s, int l) { return s->lv[l].L.list; }
 const uint32_t* level_count(const fx_sim* s, int l) { return s->lv[l].L.count; }
 
+LvMask mask_of(const fx_sim* s, int l) {
+  if (!s->sparse || l == 0) return LvMask{nullptr, 0, 0, 0, 0};
+  return LvMask{s->flagsInj, s->g.BX, s->g.BY, s->g.BZ, uint32_t(l)};
+}
+
 std::vector<float*> fine_buffers(fx_sim* s) {
   std::vector<float*> b;
   for (int k = 0; k < 2; ++k) for (int f = 0; f < 4; ++f) b.push_back(s->S[k][f]);
@@ -332,7 +337,7 @@ int smooth_ref(fx_sim* s, int l, int iters, int start) {  // PressureSolvePass.t
   Level& L = s->lv[l];
   for (int k = 0; k < iters; ++k) {
     StageTimer timer(s, l == 0 ? FX_TIME_JACOBI_FINE : -1);
-    launch_jacobi(s->stream, grid_for(s, L.nb), level_list(s, l), level_count(s, l), L.g, s->d_const, L.div, L.p[a], L.p[1 - a], 0);
+    launch_jacobi(s->stream, grid_for(s, L.nb), level_list(s, l), level_count(s, l), L.g, mask_of(s, l), s->d_const, L.div, L.p[a], L.p[1 - a], 0);
     a = 1 - a;
   }
   return a;
@@ -340,15 +345,15 @@ int smooth_ref(fx_sim* s, int l, int iters, int start) {  // PressureSolvePass.t
 int solve_reference(fx_sim* s) {  // PressureSolvePass.ts:68-84 with its buffer aliasing
   Level& F = s->lv[0]; Level& M = s->lv[1]; Level& C = s->lv[2];
   int finePre = smooth_ref(s, 0, s->q.fine_pre_iterations, 0);
-  launch_residual(s->stream, grid_for(s, F.nb), level_list(s, 0), level_count(s, 0), F.g, F.div, F.p[finePre], F.p[1 - finePre], nullptr);
-  launch_restrict(s->stream, grid_for(s, M.nb), level_list(s, 1), level_count(s, 1), F.g, M.g, F.div, M.div, 0);
+  launch_residual(s->stream, grid_for(s, F.nb), level_list(s, 0), level_count(s, 0), F.g, mask_of(s, 0), F.div, F.p[finePre], F.p[1 - finePre], nullptr);
+  launch_restrict(s->stream, grid_for(s, M.nb), level_list(s, 1), level_count(s, 1), F.g, M.g, mask_of(s, 1), F.div, M.div, 0);
   int midPre = smooth_ref(s, 1, s->q.mid_pre_iterations, 0);
-  launch_residual(s->stream, grid_for(s, M.nb), level_list(s, 1), level_count(s, 1), M.g, M.div, M.p[midPre], M.p[1 - midPre], nullptr);
-  launch_restrict(s->stream, grid_for(s, C.nb), level_list(s, 2), level_count(s, 2), M.g, C.g, M.div, C.div, 0);
+  launch_residual(s->stream, grid_for(s, M.nb), level_list(s, 1), level_count(s, 1), M.g, mask_of(s, 1), M.div, M.p[midPre], M.p[1 - midPre], nullptr);
+  launch_restrict(s->stream, grid_for(s, C.nb), level_list(s, 2), level_count(s, 2), M.g, C.g, mask_of(s, 2), M.div, C.div, 0);
   int co = smooth_ref(s, 2, s->q.coarse_iterations, 0);
-  launch_prolong(s->stream, grid_for(s, M.nb), level_list(s, 1), level_count(s, 1), C.g, M.g, C.p[co], M.p[0]);
+  launch_prolong(s->stream, grid_for(s, M.nb), level_list(s, 1), level_count(s, 1), C.g, M.g, mask_of(s, 1), C.p[co], M.p[0]);
   int mid = smooth_ref(s, 1, s->q.mid_post_iterations, midPre);
-  launch_prolong(s->stream, grid_for(s, F.nb), level_list(s, 0), level_count(s, 0), M.g, F.g, M.p[mid], F.p[0]);
+  launch_prolong(s->stream, grid_for(s, F.nb), level_list(s, 0), level_count(s, 0), M.g, F.g, mask_of(s, 0), M.p[mid], F.p[0]);
   return smooth_ref(s, 0, s->q.fine_post_iterations, finePre);
 }
 // Weighted-Jacobi V-cycle; cur[l] = buffer holding the current iterate.
@@ -360,7 +365,7 @@ int solve_vcycle(fx_sim* s) {
     Level& V = s->lv[l];
     for (int k = 0; k < n; ++k) {
       StageTimer timer(s, l == 0 ? FX_TIME_JACOBI_FINE : -1);
-      launch_jacobi(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, s->d_const, V.div, V.p[cur[l]],
+      launch_jacobi(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, mask_of(s, l), s->d_const, V.div, V.p[cur[l]],
                     V.p[1 - cur[l]], 1);
       cur[l] = 1 - cur[l];
     }
@@ -373,17 +378,18 @@ int solve_vcycle(fx_sim* s) {
       Level& V = s->lv[l];
       if (l > 0) { cur[l] = 0; zero_level_p(s, l, 0); }
       smoothW(l, v.pre_sweeps);
-      launch_residual(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, V.div, V.p[cur[l]], V.p[1 - cur[l]],
+      launch_residual(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, mask_of(s, l), V.div, V.p[cur[l]], V.p[1 - cur[l]],
                       (l == 0 && c < s->norm_cap) ? s->d_norms + c : nullptr);
       launch_restrict(s->stream, grid_for(s, s->lv[l + 1].nb), level_list(s, l + 1), level_count(s, l + 1), V.g, s->lv[l + 1].g,
-                      V.p[1 - cur[l]], s->lv[l + 1].div, 1);
+                      mask_of(s, l + 1), V.p[1 - cur[l]], s->lv[l + 1].div, 1);
     }
     cur[L - 1] = 0;
     zero_level_p(s, L - 1, 0);
     smoothW(L - 1, v.coarse_sweeps);
     for (int l = L - 2; l >= 0; --l) {
       Level& V = s->lv[l];
-      launch_prolong(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), s->lv[l + 1].g, V.g, s->lv[l + 1].p[cur[l + 1]],
+      launch_prolong(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), s->lv[l + 1].g, V.g, mask_of(s, l),
+                     s->lv[l + 1].p[cur[l + 1]],
                      V.p[cur[l]]);
       smoothW(l, v.post_sweeps);
     }

@@ -48,6 +48,15 @@ struct BrickList {
   uint32_t* retire_count;
 };
 
+// Activity of the voxels of one pressure level in a sparse domain: level-l
+// voxel J lies inside fine brick (J << l) >> 3 and is active iff that brick is.
+// flags == nullptr: every voxel of every listed brick is active (dense).
+struct LvMask {
+  const uint32_t* flags;  // fine-level brick flags
+  uint32_t FBX, FBY, FBZ;
+  uint32_t shift;         // level index
+};
+
 // Launch helpers implemented in fenix_kernels.hip. All are asynchronous on `s`.
 // `list`/`count` select the bricks visited; `grid_blocks` is the persistent grid.
 struct Fields;  // host-side view, see fenix_api.cpp
@@ -90,14 +99,14 @@ void launch_divergence(hipStream_t s, int blocks, const uint32_t* list, const ui
                        Vec3 us, float* div);
 void launch_zero_bricks(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count,
                         float* const* bufs, int nbufs);
-void launch_jacobi(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
+void launch_jacobi(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvMask m,
                    const StepConst* c, const float* rhs, const float* src, float* dst, int weighted);
-void launch_residual(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
+void launch_residual(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvMask m,
                      const float* rhs, const float* p, float* r, double* norm2);
 void launch_restrict(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid fine,
-                     Grid coarse, const float* src, float* dst, int full_weight);
+                     Grid coarse, LvMask mc, const float* src, float* dst, int full_weight);
 void launch_prolong(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid coarse,
-                    Grid fine, const float* src, float* dst);
+                    Grid fine, LvMask mf, const float* src, float* dst);
 void launch_projection(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
                        Vec3 us, const float* p, Vec3 u);
 void launch_debug(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 u,

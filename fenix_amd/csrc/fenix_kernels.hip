@@ -342,12 +342,20 @@ __global__ __launch_bounds__(512) void k_activate_threshold(const uint32_t* __re
   for (uint32_t k = blockIdx.x; k < na + nb; k += gridDim.x) {
     const uint32_t b = k < na ? listA[k] : listB[k - na];
     Vox v = vox_of(g, b);
-    int pred = 0;
+    int dens = 0, fast = 0;
     if (v.in) {
       F3 w = f3(vel.x[v.i], vel.y[v.i], vel.z[v.i]);
-      pred = (den[v.i] > dth || len3(w) > sth) ? 1 : 0;
+      dens = den[v.i] > dth ? 1 : 0;
+      fast = len3(w) > sth ? 1 : 0;
     }
-    if (__syncthreads_or(pred) && threadIdx.x == 0) dilate(g, b, 1, flags);
+    // Density-active bricks dilate by one brick; velocity-only bricks stay
+    // active without dilating (velocity lives inside the active set only).
+    const int any_dens = __syncthreads_or(dens);
+    const int any_fast = __syncthreads_or(fast);
+    if (threadIdx.x == 0) {
+      if (any_dens) dilate(g, b, 1, flags);
+      else if (any_fast) flags[b] = 1u;
+    }
   }
 }
 
@@ -678,33 +686,39 @@ __global__ __launch_bounds__(512) void k_divergence(const uint32_t* __restrict__
     div[i] = (dx + dy + dz) * 0.5f;
   }
 }
+__device__ __forceinline__ bool lv_active(const LvMask& m, uint32_t x, uint32_t y, uint32_t z) {
+  if (!m.flags) return true;
+  uint32_t fx = min((x << m.shift) >> 3, m.FBX - 1), fy = min((y << m.shift) >> 3, m.FBY - 1),
+           fz = min((z << m.shift) >> 3, m.FBZ - 1);
+  return m.flags[fx + m.FBX * (fy + m.FBY * fz)] != 0u;
+}
 __device__ __forceinline__ float nsum(const Grid& g, const float* __restrict__ s, uint32_t x, uint32_t y, uint32_t z) {
   return s[cidx(g, incu(x, g.W - 1), y, z)] + s[cidx(g, decu(x), y, z)] + s[cidx(g, x, incu(y, g.H - 1), z)] +
          s[cidx(g, x, decu(y), z)] + s[cidx(g, x, y, incu(z, g.D - 1))] + s[cidx(g, x, y, decu(z))];
 }
 // SH/pressure-jacobi.wgsl.ts:15-36 (weighted: mix(p, jacobi, omega))
 __global__ __launch_bounds__(512) void k_jacobi(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                const StepConst* __restrict__ C, const float* __restrict__ rhs,
+                                                LvMask m, const StepConst* __restrict__ C, const float* __restrict__ rhs,
                                                 const float* __restrict__ src, float* __restrict__ dst, int weighted) {
   const float om = C->omega;
   FX_BRICK_LOOP(list, count) {
     Vox v = vox_of(g, list[fx_k]);
     if (!v.in) continue;
-    if (on_boundary(g, v.x, v.y, v.z)) { dst[v.i] = 0.f; continue; }
+    if (on_boundary(g, v.x, v.y, v.z) || !lv_active(m, v.x, v.y, v.z)) { dst[v.i] = 0.f; continue; }
     float pj = (nsum(g, src, v.x, v.y, v.z) - rhs[v.i]) / 6.f;
     dst[v.i] = weighted ? mixf(src[v.i], pj, om) : pj;
   }
 }
 // SH/pressure-residual.wgsl.ts:16-38 (+ FP64 sum of squares when norm2 != null)
 __global__ __launch_bounds__(512) void k_residual(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                  const float* __restrict__ rhs, const float* __restrict__ p, float* __restrict__ r,
-                                                  double* norm2) {
+                                                  LvMask m, const float* __restrict__ rhs, const float* __restrict__ p,
+                                                  float* __restrict__ r, double* norm2) {
   __shared__ double part[8];
   double acc = 0.0;
   FX_BRICK_LOOP(list, count) {
     Vox v = vox_of(g, list[fx_k]);
     if (!v.in) continue;
-    if (on_boundary(g, v.x, v.y, v.z)) { r[v.i] = 0.f; continue; }
+    if (on_boundary(g, v.x, v.y, v.z) || !lv_active(m, v.x, v.y, v.z)) { r[v.i] = 0.f; continue; }
     float lap = nsum(g, p, v.x, v.y, v.z) - p[v.i] * 6.f;
     float res = rhs[v.i] - lap;
     r[v.i] = res;
@@ -722,11 +736,11 @@ __global__ __launch_bounds__(512) void k_residual(const uint32_t* __restrict__ l
 }
 // SH/pressure-restriction.wgsl.ts:27-50 (8-child mean) or (1,3,3,1)/8 x 4 (V-cycle).
 __global__ __launch_bounds__(512) void k_restrict(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid f, Grid c,
-                                                  const float* __restrict__ src, float* __restrict__ dst, int full) {
+                                                  LvMask mc, const float* __restrict__ src, float* __restrict__ dst, int full) {
   FX_BRICK_LOOP(list, count) {
     Vox v = vox_of(c, list[fx_k]);
     if (!v.in) continue;
-    if (on_boundary(c, v.x, v.y, v.z)) { dst[v.i] = 0.f; continue; }
+    if (on_boundary(c, v.x, v.y, v.z) || !lv_active(mc, v.x, v.y, v.z)) { dst[v.i] = 0.f; continue; }
     float sum = 0.f;
     if (!full) {
       for (uint32_t k = 0; k < 2; ++k)
@@ -751,11 +765,11 @@ __global__ __launch_bounds__(512) void k_restrict(const uint32_t* __restrict__ l
 }
 // SH/pressure-prolongation.wgsl.ts:29-44
 __global__ __launch_bounds__(512) void k_prolong(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid c, Grid f,
-                                                 const float* __restrict__ src, float* __restrict__ dst) {
+                                                 LvMask mf, const float* __restrict__ src, float* __restrict__ dst) {
   FX_BRICK_LOOP(list, count) {
     Vox v = vox_of(f, list[fx_k]);
     if (!v.in) continue;
-    if (on_boundary(f, v.x, v.y, v.z)) { dst[v.i] = 0.f; continue; }
+    if (on_boundary(f, v.x, v.y, v.z) || !lv_active(mf, v.x, v.y, v.z)) { dst[v.i] = 0.f; continue; }
     F3 cp = f3((float(v.x) + 0.5f) * 0.5f - 0.5f, (float(v.y) + 0.5f) * 0.5f - 0.5f, (float(v.z) + 0.5f) * 0.5f - 0.5f);
     Tri t = tri(c, cp);
     dst[v.i] += tri_eval(t, src);
@@ -1020,21 +1034,21 @@ void launch_zero_bricks(hipStream_t s, int blocks, const uint32_t* list, const u
     FX_LAUNCH(k_zero_bricks, blocks, 512, s, list, count, Z);
   }
 }
-void launch_jacobi(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
+void launch_jacobi(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const StepConst* c,
                    const float* rhs, const float* src, float* dst, int weighted) {
-  FX_LAUNCH(k_jacobi, blocks, 512, s, list, count, g, c, rhs, src, dst, weighted);
+  FX_LAUNCH(k_jacobi, blocks, 512, s, list, count, g, m, c, rhs, src, dst, weighted);
 }
-void launch_residual(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const float* rhs,
+void launch_residual(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const float* rhs,
                      const float* p, float* r, double* norm2) {
-  FX_LAUNCH(k_residual, blocks, 512, s, list, count, g, rhs, p, r, norm2);
+  FX_LAUNCH(k_residual, blocks, 512, s, list, count, g, m, rhs, p, r, norm2);
 }
-void launch_restrict(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid fine, Grid coarse,
+void launch_restrict(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid fine, Grid coarse, LvMask mc,
                      const float* src, float* dst, int full_weight) {
-  FX_LAUNCH(k_restrict, blocks, 512, s, list, count, fine, coarse, src, dst, full_weight);
+  FX_LAUNCH(k_restrict, blocks, 512, s, list, count, fine, coarse, mc, src, dst, full_weight);
 }
-void launch_prolong(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid coarse, Grid fine,
+void launch_prolong(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid coarse, Grid fine, LvMask mf,
                     const float* src, float* dst) {
-  FX_LAUNCH(k_prolong, blocks, 512, s, list, count, coarse, fine, src, dst);
+  FX_LAUNCH(k_prolong, blocks, 512, s, list, count, coarse, fine, mf, src, dst);
 }
 void launch_projection(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 us, const float* p,
                        Vec3 u) {

@@ -48,7 +48,7 @@ extern "C" {
 
 /* Activation: REFERENCE = compute-active-bricks.wgsl.ts:24-79 (energy samples,
  * conditional dilation); THRESHOLD = density > a or |u| > b anywhere in the
- * brick, always dilated by one brick. */
+ * brick; bricks with density > a also activate their 26 neighbours. */
 #define FX_ACTIVATION_REFERENCE 0
 #define FX_ACTIVATION_THRESHOLD 1
 

@@ -238,12 +238,23 @@ inline fx_quality clampQuality(const fx_quality& q) {
   return o;
 }
 
+// One pressure level. In a sparse domain a level-l voxel J lies inside exactly
+// one fine brick ((J << l) >> 3 per axis, l <= 3), and it is active iff that fine
+// brick is: every level solves on exactly the fine active region.
 template <class T> struct Level {
   Dims d;
   uint32_t BX, BY, BZ;
   std::vector<T> div, pA, pB;
-  std::vector<uint8_t> bact;  // per-brick activity (sparse); all 1 when dense
-  bool on(uint32_t x, uint32_t y, uint32_t z) const { return bact[(x >> 3) + BX * ((y >> 3) + BY * (z >> 3))] != 0; }
+  std::vector<uint8_t> bact;            // per-brick: brick holds an active voxel (sparse); all 1 when dense
+  int shift = 0;                        // level index l
+  const std::vector<uint8_t>* fine = nullptr;  // fine brick activity (sparse only)
+  uint32_t FBX = 0, FBY = 0, FBZ = 0;
+  bool on(uint32_t x, uint32_t y, uint32_t z) const {
+    if (!fine) return bact[(x >> 3) + BX * ((y >> 3) + BY * (z >> 3))] != 0;
+    uint32_t fx = std::min((x << shift) >> 3, FBX - 1), fy = std::min((y << shift) >> 3, FBY - 1),
+             fz = std::min((z << shift) >> 3, FBZ - 1);
+    return (*fine)[fx + FBX * (fy + size_t(FBY) * fz)] != 0;
+  }
 };
 
 template <class T> struct Sim : Base {
@@ -290,6 +301,7 @@ template <class T> struct Sim : Base {
       v.BX = (ld.W + 7) / 8; v.BY = (ld.H + 7) / 8; v.BZ = (ld.D + 7) / 8;
       v.div.assign(ld.N(), T(0)); v.pA.assign(ld.N(), T(0)); v.pB.assign(ld.N(), T(0));
       v.bact.assign(size_t(v.BX) * v.BY * v.BZ, sparse() ? 0 : 1);
+      v.shift = l;
       lv.push_back(std::move(v));
       ld = Dims{halve(ld.W), halve(ld.H), halve(ld.D)};
     }
@@ -512,18 +524,23 @@ template <class T> struct Sim : Base {
         if (ce > T(0.012)) active = true;
         if (active) dil[b] = (peak > T(0.05) || ce > T(0.035)) ? 1 : 0;
       } else {
-        bool active = false;
+        // Density-active bricks dilate by one brick; velocity-only bricks stay
+        // active without dilating (velocity exists only inside the active set,
+        // so dilating on it would grow the set by one ring every step).
+        bool dens = false, fast = false;
         T dth = T(desc.activation_density), sth = T(desc.activation_speed);
-        for (uint32_t z = 0; z < 8 && !active; ++z)
-          for (uint32_t y = 0; y < 8 && !active; ++y)
+        for (uint32_t z = 0; z < 8; ++z)
+          for (uint32_t y = 0; y < 8; ++y)
             for (uint32_t x = 0; x < 8; ++x) {
               uint32_t cx = base[0] + x, cy = base[1] + y, cz = base[2] + z;
               if (cx >= d.W || cy >= d.H || cz >= d.D) continue;
               size_t i = d.idx(cx, cy, cz);
               V3<T> v{vel[0][i], vel[1][i], vel[2][i]};
-              if (Dn[i] > dth || len3(v) > sth) { active = true; break; }
+              if (Dn[i] > dth) dens = true;
+              if (len3(v) > sth) fast = true;
             }
-        if (active) dil[b] = 1;
+        if (dens) dil[b] = 1;
+        else if (fast) dil[b] = 0;
       }
     }
     for (size_t b = 0; b < NB; ++b) {
@@ -1020,6 +1037,7 @@ template <class T> struct Sim : Base {
             C.bact[x + C.BX * (y + size_t(C.BY) * z)] = a;
           }
     }
+    for (auto& L : lv) { L.fine = &bact; L.FBX = BX; L.FBY = BY; L.FBZ = BZ; }
     auto zero_out = [&](std::vector<T>& f, const Dims& g, const std::vector<uint8_t>& m, uint32_t bx, uint32_t by) {
       for (uint32_t z = 0; z < g.D; ++z)
         for (uint32_t y = 0; y < g.H; ++y)
@@ -1029,9 +1047,11 @@ template <class T> struct Sim : Base {
     for (int s = 0; s < 2; ++s) for (int f = 0; f < 4; ++f) zero_out(S[s][f], d, bact, BX, BY);
     for (int c = 0; c < 3; ++c) { zero_out(u[c], d, bact, BX, BY); zero_out(us[c], d, bact, BX, BY); zero_out(w[c], d, bact, BX, BY); }
     zero_out(wmag, d, bact, BX, BY); zero_out(cmag, d, bact, BX, BY); zero_out(umag, d, bact, BX, BY);
-    for (auto& L : lv) {
-      zero_out(L.div, L.d, L.bact, L.BX, L.BY); zero_out(L.pA, L.d, L.bact, L.BX, L.BY); zero_out(L.pB, L.d, L.bact, L.BX, L.BY);
-    }
+    for (auto& L : lv)
+      for (uint32_t z = 0; z < L.d.D; ++z)
+        for (uint32_t y = 0; y < L.d.H; ++y)
+          for (uint32_t x = 0; x < L.d.W; ++x)
+            if (!L.on(x, y, z)) { size_t i = L.d.idx(x, y, z); L.div[i] = L.pA[i] = L.pB[i] = T(0); }
   }
 
   bool debug_each_frame() const {

@@ -282,7 +282,8 @@ def test_reference_activation_energy_weights_and_clamped_dilation():
 
 
 def test_threshold_activation_rule():
-    """THRESHOLD: density > 1e-4 or |u| > 1e-3 anywhere in the brick, one-brick dilation."""
+    """THRESHOLD: density > 1e-4 or |u| > 1e-3 anywhere in the brick; density
+    bricks dilate by one brick, velocity-only bricks do not."""
     n = 32
     o = Oracle(make_desc((n, n, n), activation_rule=abi.FX_ACTIVATION_THRESHOLD), 64)
     den = np.zeros((n, n, n))
@@ -298,7 +299,7 @@ def test_threshold_activation_rule():
     f = o.flags(abi.FX_FLAGS_INJECTED).reshape(4, 4, 4)
     expect = np.zeros((4, 4, 4), bool)
     expect[0:3, 1:4, 0:2] = True  # around (bx=0, by=2, bz=1) -> z 0..2, y 1..3, x 0..1
-    expect[2:4, 2:4, 2:4] = True
+    expect[3, 3, 3] = True        # velocity-only brick: active, not dilated
     np.testing.assert_array_equal(f.astype(bool), expect)
 
 
