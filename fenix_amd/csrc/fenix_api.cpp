@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -23,6 +24,7 @@ namespace {
 thread_local std::string g_create_error;
 
 constexpr int kStageSlots = 8;
+constexpr size_t kBlockSmoothVoxels = 65536;  // coarsest levels up to this size run in one workgroup
 
 struct Level {
   Grid g;
@@ -87,6 +89,16 @@ struct fx_sim {
   int curSet = 0;
   std::vector<uint32_t> pending;    // sparse: bricks written by fx_write_field since the last step
   std::string err;
+
+  CompactArgs cargs{};               // sparse: list compaction of every level
+  uint32_t* d_compact = nullptr;     // per-block counts and offsets
+
+  // hipGraph of the step body (rebuilt when the recorded sequence changes).
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<int64_t> graph_key;
+  int graph_last_pressure = 0;
 
   // Stage timing (fx_set_stage_timing): event pairs recorded around stages.
   bool timing = false;
@@ -293,11 +305,7 @@ void pass_activate_render(fx_sim* s) {
     launch_clear_u32(s->stream, s->flagsInj, s->nb, 0u);
     launch_activate(s->stream, grid_for(s, s->nb), s->lv[0].L.list, s->lv[0].L.count, s->d_cand, &s->d_const->cand_count,
                     s->g, s->d_const, s->desc.activation_rule, set_of(s, 0), vec_of(s->u), s->flagsInj);
-    std::vector<Grid> grids;
-    std::vector<uint32_t*> flags;
-    std::vector<BrickList> lists;
-    for (auto& L : s->lv) { grids.push_back(L.g); flags.push_back(L.flags); lists.push_back(L.L); }
-    launch_compact(s->stream, int(s->lv.size()), grids.data(), flags.data(), lists.data(), s->touched);
+    launch_compact(s->stream, s->cargs);
     std::vector<float*> fb = fine_buffers(s);
     launch_zero_bricks(s->stream, grid_for(s, s->nb), s->lv[0].L.retire, s->lv[0].L.retire_count, fb.data(), int(fb.size()));
     for (size_t l = 1; l < s->lv.size(); ++l) {
@@ -361,11 +369,25 @@ int solve_vcycle(fx_sim* s) {
   const int L = int(s->lv.size());
   const fx_vcycle& v = s->desc.vcycle;
   std::vector<int> cur(L, 0);
-  auto smoothW = [&](int l, int n) {
+  // n sweeps at level l; zero_start: the first sweep starts from p = 0 (a
+  // coarse level's correction), so neither a clear nor a read of p is needed.
+  auto smoothW = [&](int l, int n, bool zero_start) {
     Level& V = s->lv[l];
+    if (n == 0) {
+      if (zero_start) { cur[l] = 0; zero_level_p(s, l, 0); }
+      return;
+    }
+    if (zero_start) cur[l] = 0;
+    if (l > 0 && l == L - 1 && V.nv <= kBlockSmoothVoxels) {
+      launch_smooth_block(s->stream, level_list(s, l), level_count(s, l), V.g, mask_of(s, l), s->d_const, V.div, V.p[cur[l]],
+                          V.p[1 - cur[l]], n, 1, zero_start ? 1 : 0);
+      if (n & 1) cur[l] = 1 - cur[l];
+      return;
+    }
     for (int k = 0; k < n; ++k) {
       StageTimer timer(s, l == 0 ? FX_TIME_JACOBI_FINE : -1);
-      launch_jacobi(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, mask_of(s, l), s->d_const, V.div, V.p[cur[l]],
+      const float* src = (zero_start && k == 0) ? nullptr : V.p[cur[l]];
+      launch_jacobi(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, mask_of(s, l), s->d_const, V.div, src,
                     V.p[1 - cur[l]], 1);
       cur[l] = 1 - cur[l];
     }
@@ -376,29 +398,25 @@ int solve_vcycle(fx_sim* s) {
   for (int c = 0; c < v.cycles; ++c) {
     for (int l = 0; l < L - 1; ++l) {
       Level& V = s->lv[l];
-      if (l > 0) { cur[l] = 0; zero_level_p(s, l, 0); }
-      smoothW(l, v.pre_sweeps);
-      launch_residual(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, mask_of(s, l), V.div, V.p[cur[l]], V.p[1 - cur[l]],
-                      (l == 0 && c < s->norm_cap) ? s->d_norms + c : nullptr);
+      smoothW(l, v.pre_sweeps, l > 0);
+      launch_residual(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, mask_of(s, l), V.div, V.p[cur[l]],
+                      V.p[1 - cur[l]], (l == 0 && c < s->norm_cap) ? s->d_norms + c : nullptr);
       launch_restrict(s->stream, grid_for(s, s->lv[l + 1].nb), level_list(s, l + 1), level_count(s, l + 1), V.g, s->lv[l + 1].g,
                       mask_of(s, l + 1), V.p[1 - cur[l]], s->lv[l + 1].div, 1);
     }
-    cur[L - 1] = 0;
-    zero_level_p(s, L - 1, 0);
-    smoothW(L - 1, v.coarse_sweeps);
+    smoothW(L - 1, v.coarse_sweeps, true);
     for (int l = L - 2; l >= 0; --l) {
       Level& V = s->lv[l];
       launch_prolong(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), s->lv[l + 1].g, V.g, mask_of(s, l),
-                     s->lv[l + 1].p[cur[l + 1]],
-                     V.p[cur[l]]);
-      smoothW(l, v.post_sweeps);
+                     s->lv[l + 1].p[cur[l + 1]], V.p[cur[l]]);
+      smoothW(l, v.post_sweeps, false);
     }
   }
   return cur[0];
 }
+// PressureSolvePass.solve (the clear is recorded separately, ORCH:291).
 int pressure_solve(fx_sim* s) {
   StageTimer timer(s, FX_PASS_PRESSURE);
-  clear_pressure(s);
   if (s->desc.solver == FX_SOLVER_VCYCLE) return solve_vcycle(s);
   return solve_reference(s);
 }
@@ -498,6 +516,43 @@ int read_brickmajor(fx_sim* s, const Grid& g, const float* dev, float* host, siz
   HIP_OK(s, hipStreamSynchronize(s->stream));
   HIP_OK(s, hipMemcpy(tmp.data(), dev, tmp.size() * 4, hipMemcpyDeviceToHost));
   gather_rowmajor(g, tmp.data(), host);
+  return FX_OK;
+}
+
+// The pass sequence of one step (ORCH:270-301) on the simulation stream.
+void record_step(fx_sim* s, bool vort, bool pres, bool dbg) {
+  pass_inject(s, 7);                                   // ORCH:270
+  pass_activate_render(s);                             // ORCH:271-272
+  run_stage(s, FX_PASS_ADVECT_VELOCITY);               // ORCH:273
+  run_stage(s, FX_PASS_ADVECT_SCALARS);                // ORCH:274-279
+  if (!s->sparse) pass_activate_advected(s);           // ORCH:280-281
+  run_stage(s, FX_PASS_COMBUSTION);                    // ORCH:282
+  run_stage(s, FX_PASS_BUOYANCY);                      // ORCH:283
+  if (vort) {                                          // ORCH:285-288
+    run_stage(s, FX_PASS_VORTICITY);
+    run_stage(s, FX_PASS_CONFINEMENT);
+  }
+  if (pres) {                                          // ORCH:290-294
+    clear_pressure(s);
+    run_stage(s, FX_PASS_DIVERGENCE);
+    s->lastPressure = pressure_solve(s);
+  }
+  run_stage(s, FX_PASS_PROJECTION);                    // ORCH:296
+  if (dbg) run_stage(s, FX_PASS_DEBUG_FIELDS);         // ORCH:298-301
+}
+
+// Captures record_step into a hipGraph (kernel nodes only: every size the
+// kernels need is read from device memory, so one graph serves every step).
+int capture_step(fx_sim* s, bool vort, bool pres, bool dbg) {
+  if (s->graph_exec) { (void)hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
+  if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+  HIP_OK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  record_step(s, vort, pres, dbg);
+  hipGraph_t g = nullptr;
+  HIP_OK(s, hipStreamEndCapture(s->stream, &g));
+  s->graph = g;
+  HIP_OK(s, hipGraphInstantiateWithFlags(&s->graph_exec, s->graph, 0));
+  s->graph_last_pressure = s->lastPressure;
   return FX_OK;
 }
 
@@ -610,6 +665,27 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
     lg = make_grid(halve(lg.W), halve(lg.H), halve(lg.D));
   }
   if (rc != FX_OK) return bail(rc);
+  if (s->sparse) {
+    CompactArgs& A = s->cargs;
+    A.nlevels = int(s->lv.size());
+    uint32_t blocks = 0;
+    for (int l = 0; l < A.nlevels; ++l) {
+      A.g[l] = s->lv[l].g;
+      A.flags[l] = s->lv[l].flags;
+      A.L[l] = s->lv[l].L;
+      A.block_start[l] = blocks;
+      blocks += uint32_t((s->lv[l].nb + 1023) / 1024);
+    }
+    A.block_start[A.nlevels] = blocks;
+    A.touched = s->touched;
+    if (dalloc(s, &s->d_compact, size_t(blocks) * 4) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
+    A.counts = s->d_compact;
+    A.offsets = s->d_compact + size_t(blocks) * 2;
+  }
+  {
+    const char* g = std::getenv("FENIX_GRAPH");
+    s->use_graph = !(g && g[0] == '0');
+  }
   s->norm_cap = std::max(1, desc->vcycle.cycles);
   if (dalloc(s, &s->d_norms, size_t(s->norm_cap)) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
   s->src_cap = FX_FLOATS_PER_EMITTER;
@@ -669,6 +745,9 @@ void fx_destroy(fx_sim* s) {
     if (slot.cand) (void)hipHostFree(slot.cand);
     if (slot.done) (void)hipEventDestroy(slot.done);
   }
+  if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  F(s->d_compact);
   drain_timing(s);
   for (hipEvent_t e : s->pool) (void)hipEventDestroy(e);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -716,24 +795,24 @@ int fx_step(fx_sim* s, double elapsed_s, double step_s) {  // ORCH:230-305
   int rc = upload_step(s, p, s->sparse ? &cand : nullptr, inj);
   if (rc != FX_OK) return rc;
 
-  pass_inject(s, 7);                                   // ORCH:270
-  pass_activate_render(s);                             // ORCH:271-272
-  run_stage(s, FX_PASS_ADVECT_VELOCITY);               // ORCH:273
-  run_stage(s, FX_PASS_ADVECT_SCALARS);                // ORCH:274-279
-  if (!s->sparse) pass_activate_advected(s);           // ORCH:280-281
-  run_stage(s, FX_PASS_COMBUSTION);                    // ORCH:282
-  run_stage(s, FX_PASS_BUOYANCY);                      // ORCH:283
-  if (s->stepIndex % uint64_t(s->q.vorticity_interval) == 0) {  // ORCH:285-288
-    run_stage(s, FX_PASS_VORTICITY);
-    run_stage(s, FX_PASS_CONFINEMENT);
+  const bool vort = s->stepIndex % uint64_t(s->q.vorticity_interval) == 0;
+  const bool pres = s->stepIndex % uint64_t(s->q.pressure_interval) == 0;
+  const bool dbg = debug_each_frame(s);
+  if (s->use_graph && !s->timing) {
+    std::vector<int64_t> key = {vort, pres, dbg, s->mode, pres ? -1 : s->lastPressure, s->q.fine_pre_iterations,
+                                s->q.fine_post_iterations, s->q.mid_pre_iterations, s->q.mid_post_iterations,
+                                s->q.coarse_iterations};
+    if (!s->graph_exec || key != s->graph_key) {
+      rc = capture_step(s, vort, pres, dbg);
+      if (rc != FX_OK) return rc;
+      s->graph_key = key;
+    } else if (pres) {
+      s->lastPressure = s->graph_last_pressure;
+    }
+    HIP_OK(s, hipGraphLaunch(s->graph_exec, s->stream));
+  } else {
+    record_step(s, vort, pres, dbg);
   }
-  if (s->stepIndex % uint64_t(s->q.pressure_interval) == 0) {    // ORCH:290-294
-    clear_pressure(s);
-    run_stage(s, FX_PASS_DIVERGENCE);
-    s->lastPressure = pressure_solve(s);
-  }
-  run_stage(s, FX_PASS_PROJECTION);                    // ORCH:296
-  if (debug_each_frame(s)) run_stage(s, FX_PASS_DEBUG_FIELDS);  // ORCH:298-301
   s->curSet = s->curSet == 0 ? 1 : 0;
   s->lastS = elapsed_s;
   s->stepIndex += 1;
@@ -758,7 +837,7 @@ int fx_run_pass(fx_sim* s, int32_t pass, const fx_sim_params* params) {
       if (s->sparse) return fail(s, FX_ERR_UNSUPPORTED, "sparse domains activate once per step");
       pass_activate_advected(s);
       break;
-    case FX_PASS_PRESSURE: s->lastPressure = pressure_solve(s); break;
+    case FX_PASS_PRESSURE: clear_pressure(s); s->lastPressure = pressure_solve(s); break;
     case FX_PASS_ADVECT_VELOCITY: case FX_PASS_ADVECT_SCALARS: case FX_PASS_COMBUSTION: case FX_PASS_BUOYANCY:
     case FX_PASS_VORTICITY: case FX_PASS_CONFINEMENT: case FX_PASS_DIVERGENCE: case FX_PASS_PROJECTION:
     case FX_PASS_DEBUG_FIELDS:

@@ -81,8 +81,20 @@ void launch_clear_f32(hipStream_t s, float* p, size_t n);
 void launch_activate(hipStream_t s, int blocks, const uint32_t* listA, const uint32_t* countA,
                      const uint32_t* listB, const uint32_t* countB, Grid g, const StepConst* c, int rule,
                      Scalars set, Vec3 vel, uint32_t* flags);
-void launch_compact(hipStream_t s, int nlevels, const Grid* grids, uint32_t* const* flags_lv,
-                    BrickList* lists, uint32_t* touched);
+// Brick-list compaction of every level (see k_compact_* in fenix_kernels.hip).
+struct CompactArgs {
+  int nlevels;
+  Grid g[FX_MAX_LEVELS_DEV];
+  uint32_t* flags[FX_MAX_LEVELS_DEV];      // level 0: activation output; l > 0: written
+  BrickList L[FX_MAX_LEVELS_DEV];
+  uint32_t block_start[FX_MAX_LEVELS_DEV + 1];  // first block of each level (1024 bricks per block)
+  uint32_t* touched;
+  uint32_t* counts;   // 2 per block: active, retiring
+  uint32_t* offsets;  // 2 per block
+};
+void launch_compact(hipStream_t s, const CompactArgs& A);
+void launch_smooth_block(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const StepConst* c,
+                         const float* rhs, float* p0, float* p1, int sweeps, int weighted, int zero_start);
 void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
                             const StepConst* c, Vec3 u, Vec3 us);
 void launch_advect_scalars(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,

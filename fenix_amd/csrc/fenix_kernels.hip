@@ -360,78 +360,120 @@ __global__ __launch_bounds__(512) void k_activate_threshold(const uint32_t* __re
 }
 
 // ---------------------------------------------------------- compaction --
-// One workgroup of 1024 threads. For every level: flags (level 0 given; coarse
-// levels = OR of the 2x2x2 child bricks) -> ascending active list, page table
-// and the list of bricks that leave the set (to be zeroed). Each thread owns a
-// contiguous chunk of bricks, so the old page entry is read before it is
-// overwritten by the same thread.
-struct CompactArgs {
-  int nlevels;
-  Grid g[FX_MAX_LEVELS_DEV];
-  uint32_t* flags[FX_MAX_LEVELS_DEV];
-  BrickList L[FX_MAX_LEVELS_DEV];
-  uint32_t* touched;
-};
-
-__device__ void block_scan2(uint32_t a, uint32_t r, uint32_t* ea, uint32_t* er, uint32_t* ta, uint32_t* tr) {
-  __shared__ uint32_t sa[1024], sr[1024];
-  const uint32_t t = threadIdx.x;
-  sa[t] = a; sr[t] = r;
-  __syncthreads();
-  for (uint32_t off = 1; off < 1024; off <<= 1) {
-    uint32_t va = t >= off ? sa[t - off] : 0u, vr = t >= off ? sr[t - off] : 0u;
-    __syncthreads();
-    sa[t] += va; sr[t] += vr;
-    __syncthreads();
-  }
-  *ea = sa[t] - a; *er = sr[t] - r;
-  *ta = sa[1023]; *tr = sr[1023];
-  __syncthreads();
+// Three launches for every level at once. Each block owns 1024 consecutive
+// bricks of one level (256 threads x 4 bricks, 16-byte loads):
+//   count: level flags (level 0 given; level l > 0: OR over the 2^l x 2^l x 2^l
+//          fine bricks a level-l brick covers), per-block counts of active and
+//          retiring bricks (retiring = was listed or touched, now inactive);
+//   scan:  one block turns the per-block counts into offsets and list lengths;
+//   write: ascending active list, page table (brick -> position or -1), retire
+//          list; level 0 also resets `touched` to the new flags.
+__device__ __forceinline__ uint32_t level_flag(const CompactArgs& A, int l, uint32_t b) {
+  if (l == 0) return A.flags[0][b] != 0u ? 1u : 0u;
+  const Grid g = A.g[l], f = A.g[0];
+  uint32_t bx = b % g.BX, by = (b / g.BX) % g.BY, bz = b / (g.BX * g.BY);
+  uint32_t span = 1u << l;
+  uint32_t x0 = min(bx << l, f.BX - 1), y0 = min(by << l, f.BY - 1), z0 = min(bz << l, f.BZ - 1);
+  uint32_t x1 = min((bx << l) + span - 1, f.BX - 1), y1 = min((by << l) + span - 1, f.BY - 1), z1 = min((bz << l) + span - 1, f.BZ - 1);
+  const uint32_t* ff = A.flags[0];
+  for (uint32_t z = z0; z <= z1; ++z)
+    for (uint32_t y = y0; y <= y1; ++y)
+      for (uint32_t x = x0; x <= x1; ++x)
+        if (ff[x + f.BX * (y + f.BY * z)]) return 1u;
+  return 0u;
 }
-
-__global__ __launch_bounds__(1024) void k_compact(CompactArgs A) {
-  const uint32_t t = threadIdx.x;
+__device__ __forceinline__ void block_of(const CompactArgs& A, uint32_t blk, int* level, uint32_t* base) {
+  int l = 0;
+  while (l + 1 < A.nlevels && blk >= A.block_start[l + 1]) ++l;
+  *level = l;
+  *base = (blk - A.block_start[l]) * 1024u;
+}
+// Exclusive scan of one value per thread over a 256-thread block; returns the total.
+__device__ __forceinline__ uint32_t scan256(uint32_t v, uint32_t* excl, uint32_t* lds) {
+  const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+  uint32_t x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= uint32_t(off)) x += y;
+  }
+  if (lane == 63) lds[wv] = x;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+  for (uint32_t k = 0; k < 4; ++k) {
+    uint32_t w = lds[k];
+    if (k < wv) before += w;
+    total += w;
+  }
+  *excl = before + x - v;
+  __syncthreads();
+  return total;
+}
+__global__ __launch_bounds__(256) void k_compact_count(CompactArgs A) {
+  __shared__ uint32_t lds[8];
+  int l;
+  uint32_t base;
+  block_of(A, blockIdx.x, &l, &base);
+  const uint32_t nb = A.g[l].BX * A.g[l].BY * A.g[l].BZ;
+  const BrickList L = A.L[l];
+  uint32_t on_n = 0, ret_n = 0;
+  for (uint32_t k = 0; k < 4; ++k) {
+    uint32_t b = base + threadIdx.x * 4 + k;
+    if (b >= nb) break;
+    uint32_t on = level_flag(A, l, b);
+    if (l > 0) A.flags[l][b] = on;
+    bool was = L.page[b] >= 0 || (l == 0 && A.touched[b] != 0u);
+    on_n += on;
+    ret_n += (was && !on) ? 1u : 0u;
+  }
+  uint32_t e;
+  uint32_t ta = scan256(on_n, &e, lds);
+  uint32_t tr = scan256(ret_n, &e, lds + 4);
+  if (threadIdx.x == 0) { A.counts[2 * blockIdx.x] = ta; A.counts[2 * blockIdx.x + 1] = tr; }
+}
+__global__ __launch_bounds__(1024) void k_compact_scan(CompactArgs A) {
+  if (threadIdx.x != 0) return;
   for (int l = 0; l < A.nlevels; ++l) {
-    const Grid g = A.g[l];
-    const uint32_t nb = g.BX * g.BY * g.BZ;
-    uint32_t* flags = A.flags[l];
-    if (l > 0) {
-      const Grid f = A.g[l - 1];
-      const uint32_t* ff = A.flags[l - 1];
-      for (uint32_t b = t; b < nb; b += 1024) {
-        uint32_t x = b % g.BX, y = (b / g.BX) % g.BY, z = b / (g.BX * g.BY);
-        uint32_t a = 0;
-        for (uint32_t k = 0; k < 2; ++k)
-          for (uint32_t j = 0; j < 2; ++j)
-            for (uint32_t i = 0; i < 2; ++i) {
-              uint32_t fx = 2 * x + i, fy = 2 * y + j, fz = 2 * z + k;
-              if (fx < f.BX && fy < f.BY && fz < f.BZ) a |= ff[fx + f.BX * (fy + f.BY * fz)];
-            }
-        flags[b] = a ? 1u : 0u;
-      }
-      __syncthreads();
+    uint32_t a = 0, r = 0;
+    for (uint32_t j = A.block_start[l]; j < A.block_start[l + 1]; ++j) {
+      uint32_t ca = A.counts[2 * j], cr = A.counts[2 * j + 1];
+      A.offsets[2 * j] = a; A.offsets[2 * j + 1] = r;
+      a += ca; r += cr;
     }
-    const uint32_t chunk = (nb + 1023) / 1024;
-    const uint32_t lo = min(t * chunk, nb), hi = min(lo + chunk, nb);
-    BrickList L = A.L[l];
-    uint32_t ca = 0, cr = 0;
-    for (uint32_t b = lo; b < hi; ++b) {
-      bool on = flags[b] != 0u;
+    *A.L[l].count = a;
+    *A.L[l].retire_count = r;
+  }
+}
+__global__ __launch_bounds__(256) void k_compact_write(CompactArgs A) {
+  __shared__ uint32_t lds[8];
+  int l;
+  uint32_t base;
+  block_of(A, blockIdx.x, &l, &base);
+  const uint32_t nb = A.g[l].BX * A.g[l].BY * A.g[l].BZ;
+  const BrickList L = A.L[l];
+  uint32_t on[4], ret[4];
+  uint32_t on_n = 0, ret_n = 0;
+  for (uint32_t k = 0; k < 4; ++k) {
+    uint32_t b = base + threadIdx.x * 4 + k;
+    on[k] = ret[k] = 0;
+    if (b < nb) {
+      on[k] = l == 0 ? (A.flags[0][b] != 0u ? 1u : 0u) : A.flags[l][b];
       bool was = L.page[b] >= 0 || (l == 0 && A.touched[b] != 0u);
-      ca += on ? 1u : 0u;
-      cr += (was && !on) ? 1u : 0u;
+      ret[k] = (was && !on[k]) ? 1u : 0u;
     }
-    uint32_t ea, er, ta, tr;
-    block_scan2(ca, cr, &ea, &er, &ta, &tr);
-    for (uint32_t b = lo; b < hi; ++b) {
-      bool on = flags[b] != 0u;
-      bool was = L.page[b] >= 0 || (l == 0 && A.touched[b] != 0u);
-      if (on) { L.list[ea] = b; L.page[b] = int32_t(ea); ++ea; } else { L.page[b] = -1; }
-      if (was && !on) L.retire[er++] = b;
-      if (l == 0) A.touched[b] = on ? 1u : 0u;
-    }
-    if (t == 0) { *L.count = ta; *L.retire_count = tr; }
-    __syncthreads();
+    on_n += on[k];
+    ret_n += ret[k];
+  }
+  uint32_t ea, er;
+  scan256(on_n, &ea, lds);
+  scan256(ret_n, &er, lds + 4);
+  ea += A.offsets[2 * blockIdx.x];
+  er += A.offsets[2 * blockIdx.x + 1];
+  for (uint32_t k = 0; k < 4; ++k) {
+    uint32_t b = base + threadIdx.x * 4 + k;
+    if (b >= nb) break;
+    if (on[k]) { L.list[ea] = b; L.page[b] = int32_t(ea); ++ea; } else { L.page[b] = -1; }
+    if (ret[k]) L.retire[er++] = b;
+    if (l == 0) A.touched[b] = on[k];
   }
 }
 
@@ -705,8 +747,35 @@ __global__ __launch_bounds__(512) void k_jacobi(const uint32_t* __restrict__ lis
     Vox v = vox_of(g, list[fx_k]);
     if (!v.in) continue;
     if (on_boundary(g, v.x, v.y, v.z) || !lv_active(m, v.x, v.y, v.z)) { dst[v.i] = 0.f; continue; }
-    float pj = (nsum(g, src, v.x, v.y, v.z) - rhs[v.i]) / 6.f;
-    dst[v.i] = weighted ? mixf(src[v.i], pj, om) : pj;
+    // src == nullptr: the sweep starts from p = 0 (sum of six zeros, centre 0).
+    float pj = ((src ? nsum(g, src, v.x, v.y, v.z) : 0.f) - rhs[v.i]) / 6.f;
+    dst[v.i] = weighted ? mixf(src ? src[v.i] : 0.f, pj, om) : pj;
+  }
+}
+// All `sweeps` sweeps of one small level in a single workgroup (1024 threads,
+// __syncthreads between sweeps): p0 -> p1 -> p0 ... The first sweep starts from
+// zero when zero_start is set. Used for the coarsest level, whose dispatches
+// would otherwise be launch-bound.
+__global__ __launch_bounds__(1024) void k_smooth_block(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                       LvMask m, const StepConst* __restrict__ C, const float* __restrict__ rhs,
+                                                       float* p0, float* p1, int sweeps, int weighted, int zero_start) {
+  const float om = C->omega;
+  const uint32_t total = (*count) * 512u;
+  for (int sw = 0; sw < sweeps; ++sw) {
+    const float* src = (sw & 1) ? p1 : p0;
+    float* dst = (sw & 1) ? p0 : p1;
+    const bool zero = zero_start && sw == 0;
+    for (uint32_t q = threadIdx.x; q < total; q += blockDim.x) {
+      const uint32_t b = list[q >> 9], t = q & 511u;
+      uint32_t bx = b % g.BX, by = (b / g.BX) % g.BY, bz = b / (g.BX * g.BY);
+      uint32_t x = bx * 8 + (t & 7u), y = by * 8 + ((t >> 3) & 7u), z = bz * 8 + (t >> 6);
+      if (x >= g.W || y >= g.H || z >= g.D) continue;
+      const uint32_t i = (b << 9) | t;
+      if (on_boundary(g, x, y, z) || !lv_active(m, x, y, z)) { dst[i] = 0.f; continue; }
+      float pj = ((zero ? 0.f : nsum(g, src, x, y, z)) - rhs[i]) / 6.f;
+      dst[i] = weighted ? mixf(zero ? 0.f : src[i], pj, om) : pj;
+    }
+    __syncthreads();
   }
 }
 // SH/pressure-residual.wgsl.ts:16-38 (+ FP64 sum of squares when norm2 != null)
@@ -993,12 +1062,11 @@ void launch_activate(hipStream_t s, int blocks, const uint32_t* listA, const uin
     FX_LAUNCH(k_activate_threshold, blocks, 512, s, listA, countA, listB, countB, g, c, set.d, vel, flags);
   }
 }
-void launch_compact(hipStream_t s, int nlevels, const Grid* grids, uint32_t* const* flags_lv, BrickList* lists, uint32_t* touched) {
-  CompactArgs A;
-  A.nlevels = nlevels;
-  for (int l = 0; l < nlevels; ++l) { A.g[l] = grids[l]; A.flags[l] = flags_lv[l]; A.L[l] = lists[l]; }
-  A.touched = touched;
-  FX_LAUNCH(k_compact, 1, 1024, s, A);
+void launch_compact(hipStream_t s, const CompactArgs& A) {
+  const uint32_t blocks = A.block_start[A.nlevels];
+  FX_LAUNCH(k_compact_count, blocks, 256, s, A);
+  FX_LAUNCH(k_compact_scan, 1, 64, s, A);
+  FX_LAUNCH(k_compact_write, blocks, 256, s, A);
 }
 void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
                             Vec3 u, Vec3 us) {
@@ -1037,6 +1105,10 @@ void launch_zero_bricks(hipStream_t s, int blocks, const uint32_t* list, const u
 void launch_jacobi(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const StepConst* c,
                    const float* rhs, const float* src, float* dst, int weighted) {
   FX_LAUNCH(k_jacobi, blocks, 512, s, list, count, g, m, c, rhs, src, dst, weighted);
+}
+void launch_smooth_block(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const StepConst* c,
+                         const float* rhs, float* p0, float* p1, int sweeps, int weighted, int zero_start) {
+  FX_LAUNCH(k_smooth_block, 1, 1024, s, list, count, g, m, c, rhs, p0, p1, sweeps, weighted, zero_start);
 }
 void launch_residual(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const float* rhs,
                      const float* p, float* r, double* norm2) {

@@ -117,8 +117,13 @@ def test_pressure_parity(solver, quality):
     orc.run_pass("pressure", p)
     compare_all(sim, orc, "pressure", fields=["pressure", "divergence"])
     nlev = 3
+    # The reference schedule fixes which buffer holds what (quirks Q1-Q4); the
+    # V-cycle's ping-pong buffers are an implementation detail, only the
+    # per-level right-hand sides and the final pressure are defined.
+    buffers = (abi.FX_LEVEL_DIVERGENCE,) if solver == abi.FX_SOLVER_VCYCLE else (
+        abi.FX_LEVEL_DIVERGENCE, abi.FX_LEVEL_PRESSURE_A, abi.FX_LEVEL_PRESSURE_B)
     for lvl in range(nlev):
-        for which in (abi.FX_LEVEL_DIVERGENCE, abi.FX_LEVEL_PRESSURE_A, abi.FX_LEVEL_PRESSURE_B):
+        for which in buffers:
             g, o = sim.read_level(lvl, which), orc.level(lvl, which)
             assert rel_l2(g, o) <= REL_L2, f"level {lvl} buffer {which}: {rel_l2(g, o):.3e}"
     if solver == abi.FX_SOLVER_VCYCLE:
@@ -222,3 +227,25 @@ def test_deterministic_repeat():
         sim.dispose()
     for a, b in zip(*outs):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("sparse", [False, True])
+def test_graph_replay_matches_eager(sparse, monkeypatch):
+    """The captured step graph, replayed many times with eager readbacks in
+    between, gives bit-identical results to the eager launch sequence."""
+    n = 32
+    if sparse:
+        desc, srcs = scenes.c2_desc(n), scenes.c2_sources(n, radius_vox=3.0)
+    else:
+        desc, srcs = scenes.c1_desc(n, quality={"pressureInterval": 2, "vorticityInterval": 3}), scenes.c1_sources()
+    sims = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("FENIX_GRAPH", mode)
+        sims[mode] = CombustionVolumeSimulation(desc)
+        sims[mode].updateSources(srcs)
+    for k in range(14):
+        for sim in sims.values():
+            sim.step(None, k / 60.0, 1 / 60.0)
+        for name in PERSISTENT + ("pressure",):
+            np.testing.assert_array_equal(sims["1"].read_field(name), sims["0"].read_field(name), err_msg=f"{name} step {k}")
+        np.testing.assert_array_equal(sims["1"].read_flags(), sims["0"].read_flags())
