@@ -91,6 +91,7 @@ struct fx_sim {
   std::string err;
 
   CompactArgs cargs{};               // sparse: list compaction of every level
+  LevelFlags lflags{};               // sparse: coarse flags marked by activation (n = 0 when dense)
   uint32_t* d_compact = nullptr;     // per-block counts and offsets
 
   // hipGraph of the step body (rebuilt when the recorded sequence changes).
@@ -304,7 +305,7 @@ void pass_activate_render(fx_sim* s) {
     // Candidates: last step's active list + injection/host-written bricks.
     launch_clear_u32(s->stream, s->flagsInj, s->nb, 0u);
     launch_activate(s->stream, grid_for(s, s->nb), s->lv[0].L.list, s->lv[0].L.count, s->d_cand, &s->d_const->cand_count,
-                    s->g, s->d_const, s->desc.activation_rule, set_of(s, 0), vec_of(s->u), s->flagsInj);
+                    s->g, s->d_const, s->desc.activation_rule, set_of(s, 0), vec_of(s->u), s->flagsInj, s->lflags);
     launch_compact(s->stream, s->cargs);
     std::vector<float*> fb = fine_buffers(s);
     launch_zero_bricks(s->stream, grid_for(s, s->nb), s->lv[0].L.retire, s->lv[0].L.retire_count, fb.data(), int(fb.size()));
@@ -315,14 +316,14 @@ void pass_activate_render(fx_sim* s) {
   } else {
     launch_clear_u32(s->stream, s->flagsInj, s->nb, 0u);  // ORCH:271
     launch_activate(s->stream, grid_for(s, s->nb), s->identity, s->identity_count, nullptr, nullptr, s->g, s->d_const,
-                    s->desc.activation_rule, set_of(s, 0), vec_of(s->u), s->flagsInj);
+                    s->desc.activation_rule, set_of(s, 0), vec_of(s->u), s->flagsInj, s->lflags);
   }
 }
 void pass_activate_advected(fx_sim* s) {  // ORCH:280-281 (dense only)
   StageTimer timer(s, FX_PASS_ACTIVATE_ADVECTED);
   launch_clear_u32(s->stream, s->flagsRender, s->nb, 0u);
   launch_activate(s->stream, grid_for(s, s->nb), s->identity, s->identity_count, nullptr, nullptr, s->g, s->d_const,
-                  s->desc.activation_rule, set_of(s, 1), vec_of(s->us), s->flagsRender);
+                  s->desc.activation_rule, set_of(s, 1), vec_of(s->us), s->flagsRender, s->lflags);
 }
 void pass_inject(fx_sim* s, int parts) {
   StageTimer timer(s, parts == 2 ? FX_PASS_INJECT_VELOCITY : parts == 4 ? FX_PASS_INJECT_IGNITER : FX_PASS_INJECT_SCALAR);
@@ -398,15 +399,30 @@ int solve_vcycle(fx_sim* s) {
   for (int c = 0; c < v.cycles; ++c) {
     for (int l = 0; l < L - 1; ++l) {
       Level& V = s->lv[l];
-      smoothW(l, v.pre_sweeps, l > 0);
+      Level& W = s->lv[l + 1];
+      double* norm = (l == 0 && c < s->norm_cap) ? s->d_norms + c : nullptr;
+      // Level 0 starts each solve from p = 0; coarse levels start every cycle from 0.
+      const bool zero = l > 0 || c == 0;
+      if (zero) cur[l] = 0;
+      if (launch_vdown(s->stream, grid_for(s, V.nb), v.pre_sweeps, level_list(s, l), level_count(s, l), V.g, mask_of(s, l), W.g,
+                       mask_of(s, l + 1), s->d_const, V.div, zero ? nullptr : V.p[cur[l]], V.p[1 - cur[l]], W.div, norm)) {
+        cur[l] = 1 - cur[l];
+        continue;
+      }
+      smoothW(l, v.pre_sweeps, zero);
       launch_residual(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, mask_of(s, l), V.div, V.p[cur[l]],
-                      V.p[1 - cur[l]], (l == 0 && c < s->norm_cap) ? s->d_norms + c : nullptr);
-      launch_restrict(s->stream, grid_for(s, s->lv[l + 1].nb), level_list(s, l + 1), level_count(s, l + 1), V.g, s->lv[l + 1].g,
-                      mask_of(s, l + 1), V.p[1 - cur[l]], s->lv[l + 1].div, 1);
+                      V.p[1 - cur[l]], norm);
+      launch_restrict(s->stream, grid_for(s, W.nb), level_list(s, l + 1), level_count(s, l + 1), V.g, W.g, mask_of(s, l + 1),
+                      V.p[1 - cur[l]], W.div, 1);
     }
     smoothW(L - 1, v.coarse_sweeps, true);
     for (int l = L - 2; l >= 0; --l) {
       Level& V = s->lv[l];
+      if (launch_vup(s->stream, grid_for(s, V.nb), v.post_sweeps, level_list(s, l), level_count(s, l), s->lv[l + 1].g, V.g,
+                     mask_of(s, l), s->d_const, s->lv[l + 1].p[cur[l + 1]], V.div, V.p[cur[l]], V.p[1 - cur[l]])) {
+        cur[l] = 1 - cur[l];
+        continue;
+      }
       launch_prolong(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), s->lv[l + 1].g, V.g, mask_of(s, l),
                      s->lv[l + 1].p[cur[l + 1]], V.p[cur[l]]);
       smoothW(l, v.post_sweeps, false);
@@ -533,7 +549,9 @@ void record_step(fx_sim* s, bool vort, bool pres, bool dbg) {
     run_stage(s, FX_PASS_CONFINEMENT);
   }
   if (pres) {                                          // ORCH:290-294
-    clear_pressure(s);
+    // PressureSolvePass.clear (ORCH:291); the V-cycle starts from p = 0 in its
+    // first sweep and needs no clear.
+    if (s->desc.solver != FX_SOLVER_VCYCLE) clear_pressure(s);
     run_stage(s, FX_PASS_DIVERGENCE);
     s->lastPressure = pressure_solve(s);
   }
@@ -678,6 +696,11 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
     }
     A.block_start[A.nlevels] = blocks;
     A.touched = s->touched;
+    s->lflags.n = A.nlevels - 1;
+    for (int l = 1; l < A.nlevels; ++l) {
+      s->lflags.f[l] = s->lv[l].flags;
+      s->lflags.BX[l] = s->lv[l].g.BX; s->lflags.BY[l] = s->lv[l].g.BY; s->lflags.BZ[l] = s->lv[l].g.BZ;
+    }
     if (dalloc(s, &s->d_compact, size_t(blocks) * 4) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
     A.counts = s->d_compact;
     A.offsets = s->d_compact + size_t(blocks) * 2;

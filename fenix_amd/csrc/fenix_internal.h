@@ -78,9 +78,18 @@ void launch_inject(hipStream_t s, int blocks, const uint32_t* list, const uint32
 void launch_clear_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t value);
 void launch_mark(hipStream_t s, const uint32_t* list, uint32_t n, uint32_t* touched);
 void launch_clear_f32(hipStream_t s, float* p, size_t n);
+struct LevelFlags;
 void launch_activate(hipStream_t s, int blocks, const uint32_t* listA, const uint32_t* countA,
                      const uint32_t* listB, const uint32_t* countB, Grid g, const StepConst* c, int rule,
-                     Scalars set, Vec3 vel, uint32_t* flags);
+                     Scalars set, Vec3 vel, uint32_t* flags, const LevelFlags& lf);
+// Coarse-level brick flags written by the activation kernels in a sparse
+// domain: marking fine brick f also marks level-l brick (f >> l) per axis.
+struct LevelFlags {
+  int n;                                 // coarse levels (0 when dense)
+  uint32_t* f[FX_MAX_LEVELS_DEV];        // index 1..n
+  uint32_t BX[FX_MAX_LEVELS_DEV], BY[FX_MAX_LEVELS_DEV], BZ[FX_MAX_LEVELS_DEV];
+};
+
 // Brick-list compaction of every level (see k_compact_* in fenix_kernels.hip).
 struct CompactArgs {
   int nlevels;
@@ -93,6 +102,11 @@ struct CompactArgs {
   uint32_t* offsets;  // 2 per block
 };
 void launch_compact(hipStream_t s, const CompactArgs& A);
+// Fused V-cycle legs (LDS temporal blocking); return false for unsupported sweep counts.
+bool launch_vdown(hipStream_t s, int blocks, int pre, const uint32_t* list, const uint32_t* count, Grid f, LvMask mf, Grid c,
+                  LvMask mc, const StepConst* cst, const float* rhs, const float* p_in, float* p_out, float* crhs, double* norm2);
+bool launch_vup(hipStream_t s, int blocks, int post, const uint32_t* list, const uint32_t* count, Grid c, Grid f, LvMask mf,
+                const StepConst* cst, const float* e, const float* rhs, const float* p_in, float* p_out);
 void launch_smooth_block(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const StepConst* c,
                          const float* rhs, float* p0, float* p1, int sweeps, int weighted, int zero_start);
 void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,

@@ -293,7 +293,14 @@ __global__ void k_fill_f32(float* p, size_t n) {
 // REFERENCE: SH/compute-active-bricks.wgsl.ts:24-79 (one thread per brick).
 // THRESHOLD: density > a or |u| > b anywhere in the brick, dilation 1 (one
 // workgroup per brick). Candidates = listA (+ listB).
-__device__ __forceinline__ void dilate(const Grid& g, uint32_t b, int r, uint32_t* flags) {
+__device__ __forceinline__ void mark(const Grid& g, uint32_t* flags, const LevelFlags& lf, uint32_t x, uint32_t y, uint32_t z) {
+  flags[x + g.BX * (y + g.BY * z)] = 1u;
+  for (int l = 1; l <= lf.n; ++l) {
+    uint32_t cx = min(x >> l, lf.BX[l] - 1), cy = min(y >> l, lf.BY[l] - 1), cz = min(z >> l, lf.BZ[l] - 1);
+    lf.f[l][cx + lf.BX[l] * (cy + lf.BY[l] * cz)] = 1u;
+  }
+}
+__device__ __forceinline__ void dilate(const Grid& g, uint32_t b, int r, uint32_t* flags, const LevelFlags& lf) {
   int bx = int(b % g.BX), by = int((b / g.BX) % g.BY), bz = int(b / (g.BX * g.BY));
   for (int z = -r; z <= r; ++z)
     for (int y = -r; y <= r; ++y)
@@ -301,12 +308,12 @@ __device__ __forceinline__ void dilate(const Grid& g, uint32_t b, int r, uint32_
         int nx = min(max(bx + x, 0), int(g.BX) - 1);
         int ny = min(max(by + y, 0), int(g.BY) - 1);
         int nz = min(max(bz + z, 0), int(g.BZ) - 1);
-        flags[nx + g.BX * (ny + g.BY * nz)] = 1u;
+        mark(g, flags, lf, uint32_t(nx), uint32_t(ny), uint32_t(nz));
       }
 }
 __global__ __launch_bounds__(256) void k_activate_reference(const uint32_t* __restrict__ listA, const uint32_t* __restrict__ countA,
                                                             const uint32_t* __restrict__ listB, const uint32_t* __restrict__ countB,
-                                                            Grid g, Scalars S, uint32_t* flags) {
+                                                            Grid g, Scalars S, uint32_t* flags, LevelFlags lf) {
   const uint32_t na = *countA, nb = listB ? *countB : 0u;
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na + nb; k += gridDim.x * blockDim.x) {
     const uint32_t b = k < na ? listA[k] : listB[k - na];
@@ -330,13 +337,13 @@ __global__ __launch_bounds__(256) void k_activate_reference(const uint32_t* __re
     peak = fmaxf(peak, ce);
     if (ce > 0.012f) active = true;
     if (!active) continue;
-    dilate(g, b, (peak > 0.05f || ce > 0.035f) ? 1 : 0, flags);
+    dilate(g, b, (peak > 0.05f || ce > 0.035f) ? 1 : 0, flags, lf);
   }
 }
 __global__ __launch_bounds__(512) void k_activate_threshold(const uint32_t* __restrict__ listA, const uint32_t* __restrict__ countA,
                                                             const uint32_t* __restrict__ listB, const uint32_t* __restrict__ countB,
                                                             Grid g, const StepConst* __restrict__ C, const float* __restrict__ den,
-                                                            Vec3 vel, uint32_t* flags) {
+                                                            Vec3 vel, uint32_t* flags, LevelFlags lf) {
   const uint32_t na = *countA, nb = listB ? *countB : 0u;
   const float dth = C->act_density, sth = C->act_speed;
   for (uint32_t k = blockIdx.x; k < na + nb; k += gridDim.x) {
@@ -353,8 +360,8 @@ __global__ __launch_bounds__(512) void k_activate_threshold(const uint32_t* __re
     const int any_dens = __syncthreads_or(dens);
     const int any_fast = __syncthreads_or(fast);
     if (threadIdx.x == 0) {
-      if (any_dens) dilate(g, b, 1, flags);
-      else if (any_fast) flags[b] = 1u;
+      if (any_dens) dilate(g, b, 1, flags, lf);
+      else if (any_fast) mark(g, flags, lf, b % g.BX, (b / g.BX) % g.BY, b / (g.BX * g.BY));
     }
   }
 }
@@ -369,18 +376,7 @@ __global__ __launch_bounds__(512) void k_activate_threshold(const uint32_t* __re
 //   write: ascending active list, page table (brick -> position or -1), retire
 //          list; level 0 also resets `touched` to the new flags.
 __device__ __forceinline__ uint32_t level_flag(const CompactArgs& A, int l, uint32_t b) {
-  if (l == 0) return A.flags[0][b] != 0u ? 1u : 0u;
-  const Grid g = A.g[l], f = A.g[0];
-  uint32_t bx = b % g.BX, by = (b / g.BX) % g.BY, bz = b / (g.BX * g.BY);
-  uint32_t span = 1u << l;
-  uint32_t x0 = min(bx << l, f.BX - 1), y0 = min(by << l, f.BY - 1), z0 = min(bz << l, f.BZ - 1);
-  uint32_t x1 = min((bx << l) + span - 1, f.BX - 1), y1 = min((by << l) + span - 1, f.BY - 1), z1 = min((bz << l) + span - 1, f.BZ - 1);
-  const uint32_t* ff = A.flags[0];
-  for (uint32_t z = z0; z <= z1; ++z)
-    for (uint32_t y = y0; y <= y1; ++y)
-      for (uint32_t x = x0; x <= x1; ++x)
-        if (ff[x + f.BX * (y + f.BY * z)]) return 1u;
-  return 0u;
+  return A.flags[l][b] != 0u ? 1u : 0u;
 }
 __device__ __forceinline__ void block_of(const CompactArgs& A, uint32_t blk, int* level, uint32_t* base) {
   int l = 0;
@@ -420,7 +416,6 @@ __global__ __launch_bounds__(256) void k_compact_count(CompactArgs A) {
     uint32_t b = base + threadIdx.x * 4 + k;
     if (b >= nb) break;
     uint32_t on = level_flag(A, l, b);
-    if (l > 0) A.flags[l][b] = on;
     bool was = L.page[b] >= 0 || (l == 0 && A.touched[b] != 0u);
     on_n += on;
     ret_n += (was && !on) ? 1u : 0u;
@@ -456,7 +451,7 @@ __global__ __launch_bounds__(256) void k_compact_write(CompactArgs A) {
     uint32_t b = base + threadIdx.x * 4 + k;
     on[k] = ret[k] = 0;
     if (b < nb) {
-      on[k] = l == 0 ? (A.flags[0][b] != 0u ? 1u : 0u) : A.flags[l][b];
+      on[k] = level_flag(A, l, b);
       bool was = L.page[b] >= 0 || (l == 0 && A.touched[b] != 0u);
       ret[k] = (was && !on[k]) ? 1u : 0u;
     }
@@ -474,6 +469,7 @@ __global__ __launch_bounds__(256) void k_compact_write(CompactArgs A) {
     if (on[k]) { L.list[ea] = b; L.page[b] = int32_t(ea); ++ea; } else { L.page[b] = -1; }
     if (ret[k]) L.retire[er++] = b;
     if (l == 0) A.touched[b] = on[k];
+    else A.flags[l][b] = 0u;  // coarse flags start cleared for the next activation
   }
 }
 
@@ -844,6 +840,197 @@ __global__ __launch_bounds__(512) void k_prolong(const uint32_t* __restrict__ li
     dst[v.i] += tri_eval(t, src);
   }
 }
+// ------------------------------------------------ fused V-cycle kernels --
+// Temporal blocking in LDS: one 512-thread workgroup per brick of the level.
+// Tiles share one origin (brick origin - R0) and side S = 8 + 2*R0; stage s of
+// a chain computes on the brick +- (R0 - s). Halo voxels are recomputed with the
+// same arithmetic their own brick uses, so results equal the unfused sweeps.
+template <int R0>
+struct Tile {
+  static constexpr int S = 8 + 2 * R0;
+  static constexpr int N = S * S * S;
+};
+
+// Loads f (level grid g) on brick +- R0 into t; 0 outside the domain.
+template <int R0>
+__device__ __forceinline__ void tile_load(const Grid& g, const float* __restrict__ f, int ox, int oy, int oz, float* t) {
+  constexpr int S = Tile<R0>::S, N = Tile<R0>::N;
+  for (int q = threadIdx.x; q < N; q += 512) {
+    int x = ox + q % S, y = oy + (q / S) % S, z = oz + q / (S * S);
+    t[q] = (x >= 0 && y >= 0 && z >= 0 && x < int(g.W) && y < int(g.H) && z < int(g.D)) ? f[vidx(g, x, y, z)] : 0.f;
+  }
+}
+template <int R0>
+__device__ __forceinline__ void tile_zero(float* t) {
+  for (int q = threadIdx.x; q < Tile<R0>::N; q += 512) t[q] = 0.f;
+}
+// One weighted-Jacobi sweep A -> B on brick +- r (tiles of radius R0).
+template <int R0>
+__device__ __forceinline__ void tile_sweep(const Grid& g, const LvMask& m, int ox, int oy, int oz, int r, const float* A,
+                                           const float* rhs, float* B, float om) {
+  constexpr int S = Tile<R0>::S;
+  const int side = 8 + 2 * r, n = side * side * side, off = R0 - r;
+  for (int q = threadIdx.x; q < n; q += 512) {
+    int tx = off + q % side, ty = off + (q / side) % side, tz = off + q / (side * side);
+    int i = tx + S * (ty + S * tz);
+    int x = ox + tx, y = oy + ty, z = oz + tz;
+    if (x < 0 || y < 0 || z < 0 || x >= int(g.W) || y >= int(g.H) || z >= int(g.D) ||
+        on_boundary(g, x, y, z) || !lv_active(m, x, y, z)) { B[i] = 0.f; continue; }
+    float sum = A[i + 1] + A[i - 1] + A[i + S] + A[i - S] + A[i + S * S] + A[i - S * S];
+    float pj = (sum - rhs[i]) / 6.f;
+    B[i] = mixf(A[i], pj, om);
+  }
+}
+
+// Pre-smoothing (PRE sweeps), residual, FP64 norm and (1,3,3,1)/8 x 4 restriction.
+// p_in == nullptr: start from p = 0. Writes the smoothed p of the brick to p_out
+// and the coarse right-hand side of the 4^3 coarse voxels the brick covers.
+template <int PRE>
+__global__ __launch_bounds__(512) void k_vdown(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid f,
+                                               LvMask mf, Grid c, LvMask mc, const StepConst* __restrict__ C,
+                                               const float* __restrict__ rhs, const float* __restrict__ p_in,
+                                               float* __restrict__ p_out, float* __restrict__ crhs, double* norm2) {
+  constexpr int R0 = PRE + 2, S = Tile<R0>::S, N = Tile<R0>::N;
+  __shared__ float ta[N], tb[N], tr[N];
+  __shared__ double part[8];
+  const float om = C->omega;
+  double acc = 0.0;
+  FX_BRICK_LOOP(list, count) {
+    const uint32_t b = list[fx_k];
+    const int bx = int(b % f.BX), by = int((b / f.BX) % f.BY), bz = int(b / (f.BX * f.BY));
+    const int ox = bx * 8 - R0, oy = by * 8 - R0, oz = bz * 8 - R0;
+    __syncthreads();  // previous brick's tiles are no longer read
+    tile_load<R0>(f, rhs, ox, oy, oz, tr);
+    if (p_in) tile_load<R0>(f, p_in, ox, oy, oz, ta); else tile_zero<R0>(ta);
+    __syncthreads();
+    float* A = ta;
+    float* B = tb;
+    for (int sw = 1; sw <= PRE; ++sw) {
+      tile_sweep<R0>(f, mf, ox, oy, oz, R0 - sw, A, tr, B, om);
+      __syncthreads();
+      float* t = A; A = B; B = t;
+    }
+    // residual on brick +- 1 into B
+    {
+      const int side = 10, n = 1000, off = R0 - 1;
+      for (int q = threadIdx.x; q < n; q += 512) {
+        int tx = off + q % side, ty = off + (q / side) % side, tz = off + q / (side * side);
+        int i = tx + S * (ty + S * tz);
+        int x = ox + tx, y = oy + ty, z = oz + tz;
+        float res = 0.f;
+        if (!(x < 0 || y < 0 || z < 0 || x >= int(f.W) || y >= int(f.H) || z >= int(f.D) || on_boundary(f, x, y, z) ||
+              !lv_active(mf, x, y, z))) {
+          float sum = A[i + 1] + A[i - 1] + A[i + S] + A[i - S] + A[i + S * S] + A[i - S * S];
+          float lap = sum - A[i] * 6.f;
+          res = tr[i] - lap;
+        }
+        B[i] = res;
+        bool inside = tx >= R0 && ty >= R0 && tz >= R0 && tx < R0 + 8 && ty < R0 + 8 && tz < R0 + 8;
+        if (inside) acc += double(res) * double(res);
+      }
+    }
+    // smoothed p of the brick
+    {
+      const int t = threadIdx.x;
+      int tx = R0 + (t & 7), ty = R0 + ((t >> 3) & 7), tz = R0 + (t >> 6);
+      int x = ox + tx, y = oy + ty, z = oz + tz;
+      if (x < int(f.W) && y < int(f.H) && z < int(f.D)) p_out[(b << 9) | t] = A[tx + S * (ty + S * tz)];
+    }
+    __syncthreads();
+    // restriction of the brick's 4^3 coarse voxels
+    if (threadIdx.x < 64) {
+      const int t = threadIdx.x;
+      int cx = bx * 4 + (t & 3), cy = by * 4 + ((t >> 2) & 3), cz = bz * 4 + (t >> 4);
+      if (cx < int(c.W) && cy < int(c.H) && cz < int(c.D)) {
+        float out = 0.f;
+        if (!on_boundary(c, cx, cy, cz) && lv_active(mc, cx, cy, cz)) {
+          const float wt[4] = {0.125f, 0.375f, 0.375f, 0.125f};
+          float sum = 0.f;
+          for (int k = 0; k < 4; ++k) {
+            int fz = min(max(2 * cz - 1 + k, 0), int(f.D) - 1) - oz;
+            for (int j = 0; j < 4; ++j) {
+              int fy = min(max(2 * cy - 1 + j, 0), int(f.H) - 1) - oy;
+              for (int a = 0; a < 4; ++a) {
+                int fx = min(max(2 * cx - 1 + a, 0), int(f.W) - 1) - ox;
+                sum += (wt[k] * wt[j] * wt[a]) * B[fx + S * (fy + S * fz)];
+              }
+            }
+          }
+          out = sum * 4.f;
+        }
+        crhs[vidx(c, cx, cy, cz)] = out;
+      }
+    }
+  }
+  if (!norm2) return;
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sacc = 0.0;
+    for (int k = 0; k < 8; ++k) sacc += part[k];
+    atomicAdd(norm2, sacc);
+  }
+}
+
+// Prolongation of the coarse correction e (pressure-prolongation.wgsl.ts:29-44
+// sampler) added to p_in on brick +- POST, then POST sweeps; writes p_out.
+template <int POST>
+__global__ __launch_bounds__(512) void k_vup(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid c, Grid f,
+                                             LvMask mf, const StepConst* __restrict__ C, const float* __restrict__ e,
+                                             const float* __restrict__ rhs, const float* __restrict__ p_in,
+                                             float* __restrict__ p_out) {
+  constexpr int R0 = POST > 0 ? POST : 0, S = Tile<R0>::S, N = Tile<R0>::N;
+  constexpr int CR = (POST + 3) / 2, CS = 4 + 2 * CR, CN = CS * CS * CS;
+  __shared__ float ta[N], tb[N], tr[N], tc[CN];
+  const float om = C->omega;
+  const float mcx = float(c.W - 1), mcy = float(c.H - 1), mcz = float(c.D - 1);
+  FX_BRICK_LOOP(list, count) {
+    const uint32_t b = list[fx_k];
+    const int bx = int(b % f.BX), by = int((b / f.BX) % f.BY), bz = int(b / (f.BX * f.BY));
+    const int ox = bx * 8 - R0, oy = by * 8 - R0, oz = bz * 8 - R0;
+    const int cox = bx * 4 - CR, coy = by * 4 - CR, coz = bz * 4 - CR;
+    __syncthreads();
+    for (int q = threadIdx.x; q < CN; q += 512) {
+      int x = min(max(cox + q % CS, 0), int(c.W) - 1), y = min(max(coy + (q / CS) % CS, 0), int(c.H) - 1),
+          z = min(max(coz + q / (CS * CS), 0), int(c.D) - 1);
+      tc[q] = e[vidx(c, x, y, z)];
+    }
+    tile_load<R0>(f, p_in, ox, oy, oz, ta);
+    if (POST > 0) tile_load<R0>(f, rhs, ox, oy, oz, tr);
+    __syncthreads();
+    // p + P(e) on brick +- R0 into tb
+    for (int q = threadIdx.x; q < N; q += 512) {
+      int tx = q % S, ty = (q / S) % S, tz = q / (S * S);
+      int x = ox + tx, y = oy + ty, z = oz + tz;
+      if (x < 0 || y < 0 || z < 0 || x >= int(f.W) || y >= int(f.H) || z >= int(f.D) || on_boundary(f, x, y, z) ||
+          !lv_active(mf, x, y, z)) { tb[q] = 0.f; continue; }
+      float px = clampf((float(x) + 0.5f) * 0.5f - 0.5f, 0.f, mcx), py = clampf((float(y) + 0.5f) * 0.5f - 0.5f, 0.f, mcy),
+            pz = clampf((float(z) + 0.5f) * 0.5f - 0.5f, 0.f, mcz);
+      int x0 = int(floorf(px)), y0 = int(floorf(py)), z0 = int(floorf(pz));
+      int x1 = min(x0 + 1, int(c.W) - 1), y1 = min(y0 + 1, int(c.H) - 1), z1 = min(z0 + 1, int(c.D) - 1);
+      float fx = fractf(px), fy = fractf(py), fz = fractf(pz);
+      auto at = [&](int a, int bb, int cc) { return tc[(a - cox) + CS * ((bb - coy) + CS * (cc - coz))]; };
+      float x00 = mixf(at(x0, y0, z0), at(x1, y0, z0), fx);
+      float x10 = mixf(at(x0, y1, z0), at(x1, y1, z0), fx);
+      float x01 = mixf(at(x0, y0, z1), at(x1, y0, z1), fx);
+      float x11 = mixf(at(x0, y1, z1), at(x1, y1, z1), fx);
+      tb[q] = ta[q] + mixf(mixf(x00, x10, fy), mixf(x01, x11, fy), fz);
+    }
+    __syncthreads();
+    float* A = tb;
+    float* B = ta;
+    for (int sw = 1; sw <= POST; ++sw) {
+      tile_sweep<R0>(f, mf, ox, oy, oz, R0 - sw, A, tr, B, om);
+      __syncthreads();
+      float* t = A; A = B; B = t;
+    }
+    const int t = threadIdx.x;
+    int tx = R0 + (t & 7), ty = R0 + ((t >> 3) & 7), tz = R0 + (t >> 6);
+    if (ox + tx < int(f.W) && oy + ty < int(f.H) && oz + tz < int(f.D)) p_out[(b << 9) | t] = A[tx + S * (ty + S * tz)];
+  }
+}
+
 // SH/project-velocity.wgsl.ts:21-43
 __global__ __launch_bounds__(512) void k_projection(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g, Vec3 us,
                                                     const float* __restrict__ p, Vec3 u) {
@@ -1054,12 +1241,13 @@ void launch_clear_f32(hipStream_t s, float* p, size_t n) {
   FX_LAUNCH(k_fill_f32, blocks, 256, s, p, n);
 }
 void launch_activate(hipStream_t s, int blocks, const uint32_t* listA, const uint32_t* countA, const uint32_t* listB,
-                     const uint32_t* countB, Grid g, const StepConst* c, int rule, Scalars set, Vec3 vel, uint32_t* flags) {
+                     const uint32_t* countB, Grid g, const StepConst* c, int rule, Scalars set, Vec3 vel, uint32_t* flags,
+                     const LevelFlags& lf) {
   if (rule == FX_ACTIVATION_REFERENCE) {
     int b = std::max(1, std::min(blocks, int((brick_count(g) + 255) / 256) + 1));
-    FX_LAUNCH(k_activate_reference, b, 256, s, listA, countA, listB, countB, g, set, flags);
+    FX_LAUNCH(k_activate_reference, b, 256, s, listA, countA, listB, countB, g, set, flags, lf);
   } else {
-    FX_LAUNCH(k_activate_threshold, blocks, 512, s, listA, countA, listB, countB, g, c, set.d, vel, flags);
+    FX_LAUNCH(k_activate_threshold, blocks, 512, s, listA, countA, listB, countB, g, c, set.d, vel, flags, lf);
   }
 }
 void launch_compact(hipStream_t s, const CompactArgs& A) {
@@ -1109,6 +1297,25 @@ void launch_jacobi(hipStream_t s, int blocks, const uint32_t* list, const uint32
 void launch_smooth_block(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const StepConst* c,
                          const float* rhs, float* p0, float* p1, int sweeps, int weighted, int zero_start) {
   FX_LAUNCH(k_smooth_block, 1, 1024, s, list, count, g, m, c, rhs, p0, p1, sweeps, weighted, zero_start);
+}
+bool launch_vdown(hipStream_t s, int blocks, int pre, const uint32_t* list, const uint32_t* count, Grid f, LvMask mf, Grid c,
+                  LvMask mc, const StepConst* cst, const float* rhs, const float* p_in, float* p_out, float* crhs, double* norm2) {
+  switch (pre) {
+    case 0: FX_LAUNCH(k_vdown<0>, blocks, 512, s, list, count, f, mf, c, mc, cst, rhs, p_in, p_out, crhs, norm2); return true;
+    case 1: FX_LAUNCH(k_vdown<1>, blocks, 512, s, list, count, f, mf, c, mc, cst, rhs, p_in, p_out, crhs, norm2); return true;
+    case 2: FX_LAUNCH(k_vdown<2>, blocks, 512, s, list, count, f, mf, c, mc, cst, rhs, p_in, p_out, crhs, norm2); return true;
+    default: return false;  // larger halos exceed the LDS budget: unfused sweeps
+  }
+}
+bool launch_vup(hipStream_t s, int blocks, int post, const uint32_t* list, const uint32_t* count, Grid c, Grid f, LvMask mf,
+                const StepConst* cst, const float* e, const float* rhs, const float* p_in, float* p_out) {
+  switch (post) {
+    case 0: FX_LAUNCH(k_vup<0>, blocks, 512, s, list, count, c, f, mf, cst, e, rhs, p_in, p_out); return true;
+    case 1: FX_LAUNCH(k_vup<1>, blocks, 512, s, list, count, c, f, mf, cst, e, rhs, p_in, p_out); return true;
+    case 2: FX_LAUNCH(k_vup<2>, blocks, 512, s, list, count, c, f, mf, cst, e, rhs, p_in, p_out); return true;
+    case 3: FX_LAUNCH(k_vup<3>, blocks, 512, s, list, count, c, f, mf, cst, e, rhs, p_in, p_out); return true;
+    default: return false;
+  }
 }
 void launch_residual(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const float* rhs,
                      const float* p, float* r, double* norm2) {

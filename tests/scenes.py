@@ -46,6 +46,8 @@ def c1_desc(n=64, config=False, **kw):
                "vorticityVelocityMask": 0.0, "vorticityHeatMask": 0.0, "vorticityDensityMask": 0.0}
     quality = {"finePreIterations": 2, "finePostIterations": 2, "midPreIterations": 2, "midPostIterations": 2,
                "coarseIterations": 10}
+    quality.update(kw.pop("quality", {}))
+    runtime.update(kw.pop("runtime", {}))
     extra = {}
     if config:
         extra = dict(solver=abi.FX_SOLVER_VCYCLE)
@@ -93,6 +95,7 @@ def c2_desc(n=256, **kw):
     args = dict(domain_mode=abi.FX_DOMAIN_SPARSE, activation_rule=abi.FX_ACTIVATION_THRESHOLD,
                 solver=abi.FX_SOLVER_VCYCLE, burn_rate=0.5, ignition_lo=0.6 - 0.225, ignition_hi=0.6 + 0.225,
                 debug_fields=0)
+    runtime.update(kw.pop("runtime", {}))
     args.update(kw)
     return make_desc((n, n, n), runtime=runtime, **args)
 

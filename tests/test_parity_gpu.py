@@ -171,7 +171,7 @@ def test_step_parity_sparse_config():
     n = 48
     desc = scenes.c2_desc(n)
     srcs = scenes.c2_sources(n, radius_vox=4.0)
-    sim, orc, _ = _step_parity(desc, srcs, steps=12)
+    sim, orc, _ = _step_parity(desc, srcs, steps=40)
     # active list / page table are exact functions of the flags
     flags = orc.flags(abi.FX_FLAGS_RENDER)
     lst = sim.read_active_list()
