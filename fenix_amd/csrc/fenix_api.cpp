@@ -161,8 +161,10 @@ const uint32_t* fine_count(const fx_sim* s) { return s->sparse ? s->lv[0].L.coun
 const uint32_t* level_list(const fx_sim* s, int l) { return s->lv[l].L.list; }
 const uint32_t* level_count(const fx_sim* s, int l) { return s->lv[l].L.count; }
 
+// Voxel activity of level l (sparse). Level 0 needs it too: fused V-cycle
+// tiles compute halo voxels that may lie in inactive bricks.
 LvMask mask_of(const fx_sim* s, int l) {
-  if (!s->sparse || l == 0) return LvMask{nullptr, 0, 0, 0, 0};
+  if (!s->sparse) return LvMask{nullptr, 0, 0, 0, 0};
   return LvMask{s->flagsInj, s->g.BX, s->g.BY, s->g.BZ, uint32_t(l)};
 }
 
