@@ -93,6 +93,7 @@ struct fx_sim {
   CompactArgs cargs{};               // sparse: list compaction of every level
   LevelFlags lflags{};               // sparse: coarse flags marked by activation (n = 0 when dense)
   uint32_t* d_compact = nullptr;     // per-block counts and offsets
+  uint32_t* d_bits = nullptr;        // packed render flags for the raymarch
 
   // hipGraph of the step body (rebuilt when the recorded sequence changes).
   bool use_graph = true;
@@ -661,6 +662,7 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
   if (rc == FX_OK) rc = dalloc(s, &s->identity_count, 1);
   if (rc == FX_OK) rc = dalloc(s, &s->d_const, 1);
   if (rc == FX_OK) rc = dalloc(s, &s->d_cand, s->nb);
+  if (rc == FX_OK) rc = dalloc(s, &s->d_bits, (s->nb + 31) / 32);
   if (rc != FX_OK) return bail(rc);
   if (s->sparse) s->flagsRender = s->flagsInj;
 
@@ -759,7 +761,7 @@ void fx_destroy(fx_sim* s) {
   F(s->wmag); F(s->cmag); F(s->umag);
   F(s->flagsInj);
   if (!s->sparse) F(s->flagsRender);
-  F(s->touched); F(s->identity); F(s->identity_count); F(s->d_const); F(s->d_cand); F(s->d_src); F(s->d_norms);
+  F(s->touched); F(s->d_bits); F(s->identity); F(s->identity_count); F(s->d_const); F(s->d_cand); F(s->d_src); F(s->d_norms);
   for (size_t l = 0; l < s->lv.size(); ++l) {
     Level& L = s->lv[l];
     F(L.div); F(L.p[0]); F(L.p[1]); F(L.L.list); F(L.L.count); F(L.L.page); F(L.L.retire); F(L.L.retire_count);
@@ -1099,7 +1101,7 @@ int fx_raymarch_device(fx_sim* s, const fx_camera* c, const fx_render_params* p,
   if (rc != FX_OK) return rc;
   HIP_OK(s, hipSetDevice(s->device));
   StageTimer timer(s, FX_TIME_RAYMARCH);
-  launch_raymarch(s->stream, s->g, R, set_of(s, 0), s->flagsRender, static_cast<uint16_t*>(dev));
+  launch_raymarch(s->stream, s->g, R, set_of(s, 0), s->flagsRender, s->d_bits, static_cast<uint16_t*>(dev));
   HIP_OK(s, hipGetLastError());
   return FX_OK;
 }

@@ -153,7 +153,8 @@ struct RayConst {
   float light_color[3][3];
   uint32_t width, height;
 };
-void launch_raymarch(hipStream_t s, Grid g, const RayConst& rc, Scalars set0, const uint32_t* flags,
+// bits: scratch of ceil(bricks/32) words for the LDS copy of the flags (nullptr: read flags from memory).
+void launch_raymarch(hipStream_t s, Grid g, const RayConst& rc, Scalars set0, const uint32_t* flags, uint32_t* bits,
                      uint16_t* out);
 
 }  // namespace fx

@@ -752,11 +752,56 @@ __global__ __launch_bounds__(512) void k_jacobi(const uint32_t* __restrict__ lis
 // __syncthreads between sweeps): p0 -> p1 -> p0 ... The first sweep starts from
 // zero when zero_start is set. Used for the coarsest level, whose dispatches
 // would otherwise be launch-bound.
+constexpr uint32_t kSmoothSlots = 8;      // active bricks staged in LDS
+constexpr uint32_t kSmoothTable = 4096;   // level bricks addressable through the LDS slot table
 __global__ __launch_bounds__(1024) void k_smooth_block(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
                                                        LvMask m, const StepConst* __restrict__ C, const float* __restrict__ rhs,
                                                        float* p0, float* p1, int sweeps, int weighted, int zero_start) {
+  __shared__ float sp[2][kSmoothSlots * 512];
+  __shared__ float sr[kSmoothSlots * 512];
+  __shared__ int16_t slot[kSmoothTable];
   const float om = C->omega;
-  const uint32_t total = (*count) * 512u;
+  const uint32_t n = *count, nbl = g.BX * g.BY * g.BZ;
+  const uint32_t total = n * 512u;
+  if (n <= kSmoothSlots && nbl <= kSmoothTable) {
+    // Stage the active bricks: bricks outside the list hold zeros (sparse invariant).
+    for (uint32_t q = threadIdx.x; q < nbl; q += blockDim.x) slot[q] = -1;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) slot[list[k]] = int16_t(k);
+    for (uint32_t q = threadIdx.x; q < total; q += blockDim.x) {
+      const uint32_t gi = (list[q >> 9] << 9) | (q & 511u);
+      sr[q] = rhs[gi];
+      sp[0][q] = zero_start ? 0.f : p0[gi];
+    }
+    __syncthreads();
+    auto at = [&](const float* buf, uint32_t x, uint32_t y, uint32_t z) {
+      int sl = slot[(x >> 3) + g.BX * ((y >> 3) + g.BY * (z >> 3))];
+      return sl < 0 ? 0.f : buf[(uint32_t(sl) << 9) | ((z & 7u) << 6) | ((y & 7u) << 3) | (x & 7u)];
+    };
+    int cur = 0;
+    for (int sw = 0; sw < sweeps; ++sw) {
+      const float* A = sp[cur];
+      float* B = sp[1 - cur];
+      for (uint32_t q = threadIdx.x; q < total; q += blockDim.x) {
+        const uint32_t b = list[q >> 9], t = q & 511u;
+        uint32_t x = (b % g.BX) * 8 + (t & 7u), y = ((b / g.BX) % g.BY) * 8 + ((t >> 3) & 7u), z = (b / (g.BX * g.BY)) * 8 + (t >> 6);
+        if (x >= g.W || y >= g.H || z >= g.D || on_boundary(g, x, y, z) || !lv_active(m, x, y, z)) { B[q] = 0.f; continue; }
+        float sum = at(A, x + 1, y, z) + at(A, x - 1, y, z) + at(A, x, y + 1, z) + at(A, x, y - 1, z) + at(A, x, y, z + 1) +
+                    at(A, x, y, z - 1);
+        float pj = (sum - sr[q]) / 6.f;
+        B[q] = weighted ? mixf(A[q], pj, om) : pj;
+      }
+      __syncthreads();
+      cur = 1 - cur;
+    }
+    float* dst = (sweeps & 1) ? p1 : p0;
+    for (uint32_t q = threadIdx.x; q < total; q += blockDim.x) {
+      const uint32_t b = list[q >> 9], t = q & 511u;
+      uint32_t x = (b % g.BX) * 8 + (t & 7u), y = ((b / g.BX) % g.BY) * 8 + ((t >> 3) & 7u), z = (b / (g.BX * g.BY)) * 8 + (t >> 6);
+      if (x < g.W && y < g.H && z < g.D) dst[(b << 9) | t] = sp[cur][q];
+    }
+    return;
+  }
   for (int sw = 0; sw < sweeps; ++sw) {
     const float* src = (sw & 1) ? p1 : p0;
     float* dst = (sw & 1) ? p0 : p1;
@@ -1074,7 +1119,29 @@ __device__ __forceinline__ float phase_hg(float c, float gg) {
   return (1.f - g2) / (12.566370614359172f * denom);
 }
 
-__global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, Scalars S, const uint32_t* __restrict__ flags, uint16_t* out) {
+// Brick flags packed 32 per word (the raymarch keeps them in LDS).
+__global__ void k_pack_flags(const uint32_t* __restrict__ flags, uint32_t nb, uint32_t* __restrict__ bits) {
+  uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w * 32 >= nb) return;
+  uint32_t v = 0;
+  for (uint32_t k = 0; k < 32 && w * 32 + k < nb; ++k) v |= (flags[w * 32 + k] != 0u ? 1u : 0u) << k;
+  bits[w] = v;
+}
+
+__global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, Scalars S, const uint32_t* __restrict__ flags,
+                                                  const uint32_t* __restrict__ bits, uint32_t nwords, uint16_t* out) {
+  extern __shared__ uint32_t sbits[];
+  // isActiveSample (RP:378-381) reads the brick flags from LDS when they fit.
+  const bool lds = bits != nullptr;
+  if (lds) {
+    for (uint32_t q = threadIdx.x; q < nwords; q += 256) sbits[q] = bits[q];
+    __syncthreads();
+  }
+  auto active_brick = [&](uint32_t x, uint32_t y, uint32_t z) {
+    uint32_t bx = min(x >> 3, g.BX - 1), by = min(y >> 3, g.BY - 1), bz = min(z >> 3, g.BZ - 1);
+    uint32_t b = bx + g.BX * (by + g.BY * bz);
+    return lds ? ((sbits[b >> 5] >> (b & 31u)) & 1u) != 0u : flags[b] != 0u;
+  };
   const uint32_t px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (px >= R.width || py >= R.height) return;
   __half* o = reinterpret_cast<__half*>(out) + (size_t(py) * R.width + px) * 4;
@@ -1116,6 +1183,7 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, Scalars S,
   F3 acc = f3(0, 0, 0);
   float alpha = 0.f;
   F3 nrd = norm3(vrd);
+  const float vrdLen = len3(vrd);
   const uint32_t mxu = uint32_t(dims.x), myu = uint32_t(dims.y), mzu = uint32_t(dims.z);
   for (uint32_t k = 0; k < stepCount && travelled <= rayLen; ++k) {
     if (alpha > 0.985f) break;
@@ -1124,7 +1192,7 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, Scalars S,
     F3 uvw = f3(sp.x / dims.x, sp.y / dims.y, sp.z / dims.z);
     uint32_t ax = uint32_t(floorf(clampf(sp.x, 0.f, dims.x))), ay = uint32_t(floorf(clampf(sp.y, 0.f, dims.y))),
              az = uint32_t(floorf(clampf(sp.z, 0.f, dims.z)));
-    if (flag_on(g, flags, ax, ay, az)) {
+    if (active_brick(ax, ay, az)) {
       Tri t = tri(g, sp);
       float den = fmaxf(tri_eval(t, S.d) - 0.004f, 0.f);
       float tem = tri_eval(t, S.t), fu = tri_eval(t, S.f), rea = tri_eval(t, S.r);
@@ -1209,7 +1277,7 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, Scalars S,
       float tx = fabsf(vrd.x) > 0.00001f ? (bnd.x - sp.x) / vrd.x : 1e9f;
       float ty = fabsf(vrd.y) > 0.00001f ? (bnd.y - sp.y) / vrd.y : 1e9f;
       float tz = fabsf(vrd.z) > 0.00001f ? (bnd.z - sp.z) / vrd.z : 1e9f;
-      vstep = fmaxf(fminf(tx, fminf(ty, tz)) * len3(vrd) + 0.5f, voxDelta);
+      vstep = fmaxf(fminf(tx, fminf(ty, tz)) * vrdLen + 0.5f, voxDelta);
     }
     vp = vp + nrd * vstep;
     travelled += vstep;
@@ -1336,9 +1404,14 @@ void launch_projection(hipStream_t s, int blocks, const uint32_t* list, const ui
 void launch_debug(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 u, float* umag) {
   FX_LAUNCH(k_debug, blocks, 512, s, list, count, g, u, umag);
 }
-void launch_raymarch(hipStream_t s, Grid g, const RayConst& rc, Scalars set0, const uint32_t* flags, uint16_t* out) {
+void launch_raymarch(hipStream_t s, Grid g, const RayConst& rc, Scalars set0, const uint32_t* flags, uint32_t* bits,
+                     uint16_t* out) {
+  const uint32_t nb = uint32_t(brick_count(g)), nwords = (nb + 31) / 32;
+  const bool lds = bits && nwords <= 8192;  // up to 32 KB of flag bits in LDS
+  if (lds) FX_LAUNCH(k_pack_flags, (nwords + 255) / 256, 256, s, flags, nb, bits);
   dim3 grid((rc.width + 15) / 16, (rc.height + 15) / 16);
-  hipLaunchKernelGGL(k_raymarch, grid, dim3(256), 0, s, g, rc, set0, flags, out);
+  hipLaunchKernelGGL(k_raymarch, grid, dim3(256), lds ? nwords * 4 : 0, s, g, rc, set0, flags, lds ? bits : nullptr,
+                     nwords, out);
 }
 
 }  // namespace fx
