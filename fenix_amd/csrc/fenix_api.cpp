@@ -310,12 +310,21 @@ void pass_activate_render(fx_sim* s) {
     launch_activate(s->stream, grid_for(s, s->nb), s->lv[0].L.list, s->lv[0].L.count, s->d_cand, &s->d_const->cand_count,
                     s->g, s->d_const, s->desc.activation_rule, set_of(s, 0), vec_of(s->u), s->flagsInj, s->lflags);
     launch_compact(s->stream, s->cargs);
-    std::vector<float*> fb = fine_buffers(s);
-    launch_zero_bricks(s->stream, grid_for(s, s->nb), s->lv[0].L.retire, s->lv[0].L.retire_count, fb.data(), int(fb.size()));
-    for (size_t l = 1; l < s->lv.size(); ++l) {
-      float* cb[3] = {s->lv[l].div, s->lv[l].p[0], s->lv[l].p[1]};
-      launch_zero_bricks(s->stream, grid_for(s, s->lv[l].nb), s->lv[l].L.retire, s->lv[l].L.retire_count, cb, 3);
+    RetireArgs R{};
+    R.nlevels = int(s->lv.size());
+    int nb = 0;
+    for (int l = 0; l < R.nlevels; ++l) {
+      R.list[l] = s->lv[l].L.retire;
+      R.count[l] = s->lv[l].L.retire_count;
+      R.first[l] = nb;
+      if (l == 0) {
+        for (float* b : fine_buffers(s)) R.b[nb++] = b;
+      } else {
+        R.b[nb++] = s->lv[l].div; R.b[nb++] = s->lv[l].p[0]; R.b[nb++] = s->lv[l].p[1];
+      }
     }
+    R.first[R.nlevels] = nb;
+    launch_retire(s->stream, grid_for(s, s->nb), R);
   } else {
     launch_clear_u32(s->stream, s->flagsInj, s->nb, 0u);  // ORCH:271
     launch_activate(s->stream, grid_for(s, s->nb), s->identity, s->identity_count, nullptr, nullptr, s->g, s->d_const,

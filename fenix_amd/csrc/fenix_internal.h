@@ -125,6 +125,16 @@ void launch_divergence(hipStream_t s, int blocks, const uint32_t* list, const ui
                        Vec3 us, float* div);
 void launch_zero_bricks(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count,
                         float* const* bufs, int nbufs);
+// Retiring bricks of every level, zeroed in one launch: level l's buffers are
+// b[first[l] .. first[l+1]) and its brick list is list[l] / count[l].
+struct RetireArgs {
+  int nlevels;
+  const uint32_t* list[FX_MAX_LEVELS_DEV];
+  const uint32_t* count[FX_MAX_LEVELS_DEV];
+  int first[FX_MAX_LEVELS_DEV + 1];
+  float* b[FX_MAX_ZERO_BUFS + 3 * FX_MAX_LEVELS_DEV];
+};
+void launch_retire(hipStream_t s, int blocks, const RetireArgs& R);
 void launch_jacobi(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvMask m,
                    const StepConst* c, const float* rhs, const float* src, float* dst, int weighted);
 void launch_residual(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvMask m,

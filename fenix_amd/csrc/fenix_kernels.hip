@@ -473,6 +473,18 @@ __global__ __launch_bounds__(256) void k_compact_write(CompactArgs A) {
   }
 }
 
+// Zero the retiring bricks of every level in one launch: level l's buffers are
+// Z.b[Z.first[l] .. Z.first[l+1]) and its list is retire[l] / count[l].
+__global__ __launch_bounds__(512) void k_retire(RetireArgs R) {
+  for (int l = 0; l < R.nlevels; ++l) {
+    const uint32_t n = *R.count[l];
+    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
+      const uint32_t i = (R.list[l][k] << 9) | threadIdx.x;
+      for (int q = R.first[l]; q < R.first[l + 1]; ++q) R.b[q][i] = 0.f;
+    }
+  }
+}
+
 // Zero `nbufs` buffers in every brick of the list.
 struct ZeroArgs { float* b[FX_MAX_ZERO_BUFS]; int n; };
 __global__ __launch_bounds__(512) void k_zero_bricks(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, ZeroArgs Z) {
@@ -774,20 +786,37 @@ __global__ __launch_bounds__(1024) void k_smooth_block(const uint32_t* __restric
       sp[0][q] = zero_start ? 0.f : p0[gi];
     }
     __syncthreads();
-    auto at = [&](const float* buf, uint32_t x, uint32_t y, uint32_t z) {
-      int sl = slot[(x >> 3) + g.BX * ((y >> 3) + g.BY * (z >> 3))];
-      return sl < 0 ? 0.f : buf[(uint32_t(sl) << 9) | ((z & 7u) << 6) | ((y & 7u) << 3) | (x & 7u)];
-    };
+    // Per-thread voxels (<= 4 with 1024 threads and 8 slots): neighbour offsets
+    // into the staged slots are resolved once; -1 marks a zero neighbour, -2 a
+    // voxel that is always 0 (boundary, inactive or padding).
+    constexpr int kPer = kSmoothSlots * 512 / 1024;
+    int nbr[kPer][6];
+    int self[kPer];
+    for (int j = 0; j < kPer; ++j) {
+      const uint32_t q = threadIdx.x + uint32_t(j) * 1024u;
+      self[j] = -2;
+      if (q >= total) continue;
+      const uint32_t b = list[q >> 9], t = q & 511u;
+      uint32_t x = (b % g.BX) * 8 + (t & 7u), y = ((b / g.BX) % g.BY) * 8 + ((t >> 3) & 7u), z = (b / (g.BX * g.BY)) * 8 + (t >> 6);
+      if (x >= g.W || y >= g.H || z >= g.D || on_boundary(g, x, y, z) || !lv_active(m, x, y, z)) continue;
+      self[j] = int(q);
+      const uint32_t nx[6] = {x + 1, x - 1, x, x, x, x}, ny[6] = {y, y, y + 1, y - 1, y, y}, nz[6] = {z, z, z, z, z + 1, z - 1};
+      for (int k = 0; k < 6; ++k) {
+        int sl = slot[(nx[k] >> 3) + g.BX * ((ny[k] >> 3) + g.BY * (nz[k] >> 3))];
+        nbr[j][k] = sl < 0 ? -1 : int((uint32_t(sl) << 9) | ((nz[k] & 7u) << 6) | ((ny[k] & 7u) << 3) | (nx[k] & 7u));
+      }
+    }
     int cur = 0;
     for (int sw = 0; sw < sweeps; ++sw) {
       const float* A = sp[cur];
       float* B = sp[1 - cur];
-      for (uint32_t q = threadIdx.x; q < total; q += blockDim.x) {
-        const uint32_t b = list[q >> 9], t = q & 511u;
-        uint32_t x = (b % g.BX) * 8 + (t & 7u), y = ((b / g.BX) % g.BY) * 8 + ((t >> 3) & 7u), z = (b / (g.BX * g.BY)) * 8 + (t >> 6);
-        if (x >= g.W || y >= g.H || z >= g.D || on_boundary(g, x, y, z) || !lv_active(m, x, y, z)) { B[q] = 0.f; continue; }
-        float sum = at(A, x + 1, y, z) + at(A, x - 1, y, z) + at(A, x, y + 1, z) + at(A, x, y - 1, z) + at(A, x, y, z + 1) +
-                    at(A, x, y, z - 1);
+      for (int j = 0; j < kPer; ++j) {
+        const uint32_t q = threadIdx.x + uint32_t(j) * 1024u;
+        if (q >= total) break;
+        if (self[j] < 0) { B[q] = 0.f; continue; }
+        float v[6];
+        for (int k = 0; k < 6; ++k) v[k] = nbr[j][k] < 0 ? 0.f : A[nbr[j][k]];
+        float sum = v[0] + v[1] + v[2] + v[3] + v[4] + v[5];
         float pj = (sum - sr[q]) / 6.f;
         B[q] = weighted ? mixf(A[q], pj, om) : pj;
       }
@@ -1350,6 +1379,7 @@ void launch_confinement(hipStream_t s, int blocks, const uint32_t* list, const u
 void launch_divergence(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 us, float* div) {
   FX_LAUNCH(k_divergence, blocks, 512, s, list, count, g, us, div);
 }
+void launch_retire(hipStream_t s, int blocks, const RetireArgs& R) { FX_LAUNCH(k_retire, blocks, 512, s, R); }
 void launch_zero_bricks(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, float* const* bufs, int nbufs) {
   for (int base = 0; base < nbufs; base += FX_MAX_ZERO_BUFS) {
     ZeroArgs Z;
