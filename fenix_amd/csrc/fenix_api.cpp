@@ -53,7 +53,7 @@ struct fx_sim {
   int blocks = 1;          // persistent grid for brick kernels
   bool sparse = false;
 
-  float* S[2][4] = {};
+  void* S[2][4] = {};               // d, t, f in the storage precision; r FP32
   float* u[3] = {};
   float* us[3] = {};
   float* w[3] = {};
@@ -151,7 +151,10 @@ int dalloc(fx_sim* s, T** p, size_t count) {
   return FX_OK;
 }
 
-Scalars set_of(fx_sim* s, int k) { return Scalars{s->S[k][0], s->S[k][1], s->S[k][2], s->S[k][3]}; }
+Scalars set_of(fx_sim* s, int k) {
+  return Scalars{s->S[k][0], s->S[k][1], s->S[k][2], static_cast<float*>(s->S[k][3]), s->desc.precision};
+}
+int field_esz(const fx_sim* s, int f) { return f < 3 ? int(precision_bytes(s->desc.precision)) : 4; }
 Vec3 vec_of(float* const* v) { return Vec3{v[0], v[1], v[2]}; }
 
 int grid_for(const fx_sim* s, size_t bricks) { return int(std::max<size_t>(1, std::min<size_t>(size_t(s->blocks), bricks))); }
@@ -169,12 +172,12 @@ LvMask mask_of(const fx_sim* s, int l) {
   return LvMask{s->flagsInj, s->g.BX, s->g.BY, s->g.BZ, uint32_t(l)};
 }
 
-std::vector<float*> fine_buffers(fx_sim* s) {
-  std::vector<float*> b;
-  for (int k = 0; k < 2; ++k) for (int f = 0; f < 4; ++f) b.push_back(s->S[k][f]);
-  for (int c = 0; c < 3; ++c) { b.push_back(s->u[c]); b.push_back(s->us[c]); b.push_back(s->w[c]); }
-  b.push_back(s->wmag); b.push_back(s->cmag); b.push_back(s->umag);
-  b.push_back(s->lv[0].div); b.push_back(s->lv[0].p[0]); b.push_back(s->lv[0].p[1]);
+struct Buf { void* p; int esz; };
+std::vector<Buf> fine_buffers(fx_sim* s) {
+  std::vector<Buf> b;
+  for (int k = 0; k < 2; ++k) for (int f = 0; f < 4; ++f) b.push_back(Buf{s->S[k][f], field_esz(s, f)});
+  for (int c = 0; c < 3; ++c) { b.push_back(Buf{s->u[c], 4}); b.push_back(Buf{s->us[c], 4}); b.push_back(Buf{s->w[c], 4}); }
+  for (float* p : {s->wmag, s->cmag, s->umag, s->lv[0].div, s->lv[0].p[0], s->lv[0].p[1]}) b.push_back(Buf{p, 4});
   return b;
 }
 
@@ -318,9 +321,9 @@ void pass_activate_render(fx_sim* s) {
       R.count[l] = s->lv[l].L.retire_count;
       R.first[l] = nb;
       if (l == 0) {
-        for (float* b : fine_buffers(s)) R.b[nb++] = b;
+        for (const Buf& b : fine_buffers(s)) { R.esz[nb] = uint8_t(b.esz); R.b[nb++] = b.p; }
       } else {
-        R.b[nb++] = s->lv[l].div; R.b[nb++] = s->lv[l].p[0]; R.b[nb++] = s->lv[l].p[1];
+        for (float* p : {s->lv[l].div, s->lv[l].p[0], s->lv[l].p[1]}) { R.esz[nb] = 4; R.b[nb++] = p; }
       }
     }
     R.first[R.nlevels] = nb;
@@ -498,9 +501,13 @@ void build_candidates(fx_sim* s, const fx_sim_params& p, std::vector<uint32_t>& 
   s->pending.clear();
 }
 
-float* field_ptr(fx_sim* s, int id) {
+// Device buffer of a field and its element size in bytes (4, or 2 for d/t/f in 16-bit storage).
+void* field_ptr(fx_sim* s, int id, int* esz = nullptr) {
+  if (esz) *esz = 4;
   switch (id) {
-    case FX_FIELD_DENSITY: case FX_FIELD_TEMPERATURE: case FX_FIELD_FUEL: case FX_FIELD_REACTION: return s->S[0][id];
+    case FX_FIELD_DENSITY: case FX_FIELD_TEMPERATURE: case FX_FIELD_FUEL: case FX_FIELD_REACTION:
+      if (esz) *esz = field_esz(s, id);
+      return s->S[0][id];
     case FX_FIELD_VELOCITY_X: case FX_FIELD_VELOCITY_Y: case FX_FIELD_VELOCITY_Z: return s->u[id - FX_FIELD_VELOCITY_X];
     case FX_FIELD_SCRATCH_X: case FX_FIELD_SCRATCH_Y: case FX_FIELD_SCRATCH_Z: return s->us[id - FX_FIELD_SCRATCH_X];
     case FX_FIELD_VORTICITY_X: case FX_FIELD_VORTICITY_Y: case FX_FIELD_VORTICITY_Z: return s->w[id - FX_FIELD_VORTICITY_X];
@@ -510,6 +517,7 @@ float* field_ptr(fx_sim* s, int id) {
     case FX_FIELD_DIVERGENCE: return s->lv[0].div;
     case FX_FIELD_PRESSURE: return s->lv[0].p[s->lastPressure];
     case FX_FIELD_ADV_DENSITY: case FX_FIELD_ADV_TEMPERATURE: case FX_FIELD_ADV_FUEL: case FX_FIELD_ADV_REACTION:
+      if (esz) *esz = field_esz(s, id - FX_FIELD_ADV_DENSITY);
       return s->S[1][id - FX_FIELD_ADV_DENSITY];
     default: return nullptr;
   }
@@ -535,6 +543,59 @@ void scatter_brickmajor(const Grid& g, const float* rm, float* bm) {
         bm[(b << 9) | ((z & 7u) << 6) | ((y & 7u) << 3) | (x & 7u)] = row[x];
       }
     }
+}
+
+// Host conversions of the 16-bit storage formats (round to nearest even).
+float bf16_to_f32(uint16_t h) { uint32_t u = uint32_t(h) << 16; float f; std::memcpy(&f, &u, 4); return f; }
+uint16_t f32_to_bf16(float f) {
+  uint32_t u; std::memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40);  // quiet NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+float f16_to_f32(uint16_t h) {
+  uint32_t sign = uint32_t(h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu, u;
+  if (e == 0) {
+    if (m == 0) u = sign;
+    else { e = 127 - 14; while (!(m & 0x400u)) { m <<= 1; --e; } m &= 0x3ffu; u = sign | (e << 23) | (m << 13); }
+  } else if (e == 31) u = sign | 0x7f800000u | (m << 13);
+  else u = sign | ((e + 112) << 23) | (m << 13);
+  float f; std::memcpy(&f, &u, 4); return f;
+}
+uint16_t f32_to_f16(float f) {
+  uint32_t u; std::memcpy(&u, &f, 4);
+  uint32_t sign = (u >> 16) & 0x8000u, a = u & 0x7fffffffu;
+  if (a > 0x7f800000u) return uint16_t(sign | 0x7e00u);
+  if (a >= 0x477ff000u) return uint16_t(sign | 0x7c00u);         // rounds to infinity
+  if (a < 0x38800000u) {                                          // subnormal or zero half
+    float af; std::memcpy(&af, &a, 4);
+    float scaled = af * 16777216.f;                               // 2^24: half subnormal units
+    uint32_t q = uint32_t(std::nearbyint(scaled));                // round half to even
+    return uint16_t(sign | q);
+  }
+  uint32_t mant = a & 0x7fffffu, exp = (a >> 23) - 112;
+  uint32_t half = (exp << 10) | (mant >> 13);
+  uint32_t rest = mant & 0x1fffu;
+  if (rest > 0x1000u || (rest == 0x1000u && (half & 1u))) ++half;
+  return uint16_t(sign | half);
+}
+
+int read_brickmajor_typed(fx_sim* s, const Grid& g, const void* dev, int esz, float* host, size_t bytes) {
+  size_t nv = size_t(g.W) * g.H * g.D;
+  if (bytes != nv * 4) return fail(s, FX_ERR_INVALID_ARG, "buffer size mismatch");
+  const size_t n = storage_count(g);
+  std::vector<float> tmp(n);
+  HIP_OK(s, hipStreamSynchronize(s->stream));
+  if (esz == 4) {
+    HIP_OK(s, hipMemcpy(tmp.data(), dev, n * 4, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<uint16_t> h(n);
+    HIP_OK(s, hipMemcpy(h.data(), dev, n * 2, hipMemcpyDeviceToHost));
+    const bool bf = s->desc.precision == FX_PRECISION_BF16;
+    for (size_t i = 0; i < n; ++i) tmp[i] = bf ? bf16_to_f32(h[i]) : f16_to_f32(h[i]);
+  }
+  gather_rowmajor(g, tmp.data(), host);
+  return FX_OK;
 }
 
 int read_brickmajor(fx_sim* s, const Grid& g, const float* dev, float* host, size_t bytes) {
@@ -624,7 +685,8 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
     return fail(nullptr, FX_ERR_INVALID_ARG, "resolution must be at least 4 in every axis");
   if (uint64_t(desc->width) * desc->height * desc->depth > (1ull << 31))
     return fail(nullptr, FX_ERR_INVALID_ARG, "resolution exceeds 2^31 voxels");
-  if (desc->precision != FX_PRECISION_FP32) return fail(nullptr, FX_ERR_UNSUPPORTED, "precision not supported");
+  if (desc->precision != FX_PRECISION_FP32 && desc->precision != FX_PRECISION_BF16 && desc->precision != FX_PRECISION_FP16)
+    return fail(nullptr, FX_ERR_INVALID_ARG, "unknown precision");
   if (desc->solver == FX_SOLVER_VCYCLE && (desc->vcycle.levels < 2 || desc->vcycle.levels > FX_MAX_LEVELS_DEV))
     return fail(nullptr, FX_ERR_INVALID_ARG, "vcycle.levels must be in [2, 8]");
   if (desc->advection_mode != FX_ADVECT_MACCORMACK && desc->advection_mode != FX_ADVECT_SEMI_LAGRANGIAN)
@@ -655,7 +717,12 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
 
   const size_t nsv = storage_count(s->g);
   int rc = FX_OK;
-  for (int k = 0; k < 2 && rc == FX_OK; ++k) for (int f = 0; f < 4 && rc == FX_OK; ++f) rc = dalloc(s, &s->S[k][f], nsv);
+  for (int k = 0; k < 2 && rc == FX_OK; ++k)
+    for (int f = 0; f < 4 && rc == FX_OK; ++f) {
+      uint8_t* p = nullptr;
+      rc = dalloc(s, &p, nsv * size_t(field_esz(s, f)));
+      s->S[k][f] = p;
+    }
   for (int c = 0; c < 3 && rc == FX_OK; ++c) {
     rc = dalloc(s, &s->u[c], nsv);
     if (rc == FX_OK) rc = dalloc(s, &s->us[c], nsv);
@@ -797,7 +864,7 @@ int fx_reset(fx_sim* s) {  // ORCH:206-226
   s->startS = s->lastS;
   s->pending.clear();
   const size_t nsv = storage_count(s->g);
-  for (float* b : fine_buffers(s)) launch_clear_f32(s->stream, b, nsv);
+  for (const Buf& b : fine_buffers(s)) launch_clear_u32(s->stream, static_cast<uint32_t*>(b.p), nsv * size_t(b.esz) / 4, 0u);
   launch_clear_u32(s->stream, s->flagsInj, s->nb, 0u);
   if (!s->sparse) launch_clear_u32(s->stream, s->flagsRender, s->nb, 0u);
   launch_clear_u32(s->stream, s->touched, s->nb, 0u);
@@ -956,22 +1023,31 @@ int fx_update_sources(fx_sim* s, const float* packed, uint32_t count) {  // Sour
 
 int fx_read_field(fx_sim* s, int32_t id, float* host, size_t bytes) {
   if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
-  float* f = field_ptr(s, id);
+  int esz = 4;
+  void* f = field_ptr(s, id, &esz);
   if (!f) return fail(s, FX_ERR_INVALID_ARG, "unknown field id");
   HIP_OK(s, hipSetDevice(s->device));
-  return read_brickmajor(s, s->g, f, host, bytes);
+  return read_brickmajor_typed(s, s->g, f, esz, host, bytes);
 }
 
 int fx_write_field(fx_sim* s, int32_t id, const float* host, size_t bytes) {
   if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
-  float* f = field_ptr(s, id);
+  int esz = 4;
+  void* f = field_ptr(s, id, &esz);
   if (!f) return fail(s, FX_ERR_INVALID_ARG, "unknown field id");
   if (bytes != s->nv * 4) return fail(s, FX_ERR_INVALID_ARG, "buffer size mismatch");
   HIP_OK(s, hipSetDevice(s->device));
   std::vector<float> tmp(storage_count(s->g), 0.f);
   scatter_brickmajor(s->g, host, tmp.data());
   HIP_OK(s, hipStreamSynchronize(s->stream));
-  HIP_OK(s, hipMemcpy(f, tmp.data(), tmp.size() * 4, hipMemcpyHostToDevice));
+  if (esz == 4) {
+    HIP_OK(s, hipMemcpy(f, tmp.data(), tmp.size() * 4, hipMemcpyHostToDevice));
+  } else {
+    const bool bf = s->desc.precision == FX_PRECISION_BF16;
+    std::vector<uint16_t> h(tmp.size());
+    for (size_t i = 0; i < tmp.size(); ++i) h[i] = bf ? f32_to_bf16(tmp[i]) : f32_to_f16(tmp[i]);
+    HIP_OK(s, hipMemcpy(f, h.data(), h.size() * 2, hipMemcpyHostToDevice));
+  }
   if (s->sparse) {
     // Bricks holding non-zero data join the next activation's candidates.
     std::vector<uint32_t> nz;
@@ -1038,7 +1114,7 @@ int fx_read_page_table(fx_sim* s, int32_t* host, size_t count) {
   return FX_OK;
 }
 
-void* fx_field_device_ptr(fx_sim* s, int32_t id) { return s ? static_cast<void*>(field_ptr(s, id)) : nullptr; }
+void* fx_field_device_ptr(fx_sim* s, int32_t id) { return s ? field_ptr(s, id) : nullptr; }
 
 int fx_read_level(fx_sim* s, int32_t level, int32_t which, float* host, size_t bytes) {
   if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");

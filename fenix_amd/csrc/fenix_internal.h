@@ -61,12 +61,16 @@ struct LvMask {
 // `list`/`count` select the bricks visited; `grid_blocks` is the persistent grid.
 struct Fields;  // host-side view, see fenix_api.cpp
 
+// One scalar set. Density, temperature and fuel use the simulation's storage
+// precision (FX_PRECISION_*: float, bf16 bits or half); reaction is FP32.
 struct Scalars {
-  float* d;
-  float* t;
-  float* f;
+  void* d;
+  void* t;
+  void* f;
   float* r;
+  int prec;
 };
+inline size_t precision_bytes(int prec) { return prec == FX_PRECISION_FP32 ? 4 : 2; }
 struct Vec3 {
   float* x;
   float* y;
@@ -132,7 +136,8 @@ struct RetireArgs {
   const uint32_t* list[FX_MAX_LEVELS_DEV];
   const uint32_t* count[FX_MAX_LEVELS_DEV];
   int first[FX_MAX_LEVELS_DEV + 1];
-  float* b[FX_MAX_ZERO_BUFS + 3 * FX_MAX_LEVELS_DEV];
+  void* b[FX_MAX_ZERO_BUFS + 3 * FX_MAX_LEVELS_DEV];
+  uint8_t esz[FX_MAX_ZERO_BUFS + 3 * FX_MAX_LEVELS_DEV];  // element bytes (4 or 2)
 };
 void launch_retire(hipStream_t s, int blocks, const RetireArgs& R);
 void launch_jacobi(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvMask m,

@@ -103,6 +103,47 @@ __device__ __forceinline__ float tri_eval(const Tri& t, const float* __restrict_
   float x11 = mixf(f[t.i[6]], f[t.i[7]], t.fx);
   return mixf(mixf(x00, x10, t.fy), mixf(x01, x11, t.fy), t.fz);
 }
+// Storage of density, temperature and fuel: FP32, BF16 (round to nearest even)
+// or FP16. Every load widens to FP32; all arithmetic is FP32.
+struct StF32 {
+  using T = float;
+  static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return p[i]; }
+  static __device__ __forceinline__ void st(T* p, uint32_t i, float v) { p[i] = v; }
+};
+struct StBF16 {
+  using T = uint16_t;
+  static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return __uint_as_float(uint32_t(p[i]) << 16); }
+  static __device__ __forceinline__ void st(T* p, uint32_t i, float v) {
+    uint32_t u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite values)
+    p[i] = uint16_t(u >> 16);
+  }
+};
+struct StF16 {
+  using T = __half;
+  static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return __half2float(p[i]); }
+  static __device__ __forceinline__ void st(T* p, uint32_t i, float v) { p[i] = __float2half(v); }
+};
+template <class ST>
+struct SS {
+  typename ST::T* d;
+  typename ST::T* t;
+  typename ST::T* f;
+  float* r;
+};
+template <class ST>
+__host__ __device__ inline SS<ST> typed(const Scalars& s) {
+  return SS<ST>{static_cast<typename ST::T*>(s.d), static_cast<typename ST::T*>(s.t), static_cast<typename ST::T*>(s.f), s.r};
+}
+template <class ST>
+__device__ __forceinline__ float tri_evalP(const Tri& t, const typename ST::T* __restrict__ f) {
+  float x00 = mixf(ST::ld(f, t.i[0]), ST::ld(f, t.i[1]), t.fx);
+  float x10 = mixf(ST::ld(f, t.i[2]), ST::ld(f, t.i[3]), t.fx);
+  float x01 = mixf(ST::ld(f, t.i[4]), ST::ld(f, t.i[5]), t.fx);
+  float x11 = mixf(ST::ld(f, t.i[6]), ST::ld(f, t.i[7]), t.fx);
+  return mixf(mixf(x00, x10, t.fy), mixf(x01, x11, t.fy), t.fz);
+}
+
 __device__ __forceinline__ F3 sample_u(const Grid& g, const Vec3& u, F3 p) {
   Tri t = tri(g, p);
   return F3{tri_eval(t, u.x), tri_eval(t, u.y), tri_eval(t, u.z)};
@@ -182,9 +223,10 @@ __device__ F3 turbulent_dir(F3 pos, float seed) {  // SH/inject-velocity.wgsl.ts
 // SH/inject-scalar.wgsl.ts:32-70, SH/inject-velocity.wgsl.ts:46-96 and
 // SH/inject-igniter.wgsl.ts:32-66 fused per voxel in dispatch order (each pass
 // touches only its own voxel, so fusing keeps the sequential result).
+template <class ST>
 __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
                                                 const StepConst* __restrict__ C, const float* __restrict__ srcs,
-                                                Scalars S, Vec3 u, uint32_t* touched, int parts) {
+                                                SS<ST> S, Vec3 u, uint32_t* touched, int parts) {
   const fx_sim_params& P = C->p;
   const uint32_t ns = C->nsrc;
   FX_BRICK_LOOP(list, count) {
@@ -195,7 +237,7 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
     const uint32_t i = v.i;
     F3 pos = f3((float(v.x) + 0.5f) / float(g.W), (float(v.y) + 0.5f) / float(g.H), (float(v.z) + 0.5f) / float(g.D));
     if (parts & 1) {
-      float den = S.d[i], tem = S.t[i], fu = S.f[i];
+      float den = ST::ld(S.d, i), tem = ST::ld(S.t, i), fu = ST::ld(S.f, i);
       for (uint32_t k = 0; k < ns; ++k) {
         Src s{srcs + size_t(k) * FX_FLOATS_PER_EMITTER};
         bool burst = s.kind() == 3.f;
@@ -214,7 +256,7 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
         tem = clamp01(tem + s.yt() * contribution);
         fu = clamp01(fu + s.yf() * contribution);
       }
-      S.d[i] = den; S.t[i] = tem; S.f[i] = fu;
+      ST::st(S.d, i, den); ST::st(S.t, i, tem); ST::st(S.f, i, fu);
     }
     if (parts & 2) {
       F3 vel = f3(u.x[i], u.y[i], u.z[i]);
@@ -255,8 +297,8 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
       u.x[i] = clampf(vel.x, -lim, lim); u.y[i] = clampf(vel.y, -lim, lim); u.z[i] = clampf(vel.z, -lim, lim);
     }
     if (parts & 4) {
-      float rea = S.r[i], tem = S.t[i];
-      const float fuel = S.f[i];
+      float rea = S.r[i], tem = ST::ld(S.t, i);
+      const float fuel = ST::ld(S.f, i);
       for (uint32_t k = 0; k < ns; ++k) {
         Src s{srcs + size_t(k) * FX_FLOATS_PER_EMITTER};
         bool burst = s.kind() == 3.f;
@@ -273,7 +315,7 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
         rea = clamp01(rea + seedv);
         tem = clamp01(tem + intensity * weight * P.delta_time * 4.f);
       }
-      S.r[i] = rea; S.t[i] = tem;
+      S.r[i] = rea; ST::st(S.t, i, tem);
     }
   }
 }
@@ -311,9 +353,10 @@ __device__ __forceinline__ void dilate(const Grid& g, uint32_t b, int r, uint32_
         mark(g, flags, lf, uint32_t(nx), uint32_t(ny), uint32_t(nz));
       }
 }
+template <class ST>
 __global__ __launch_bounds__(256) void k_activate_reference(const uint32_t* __restrict__ listA, const uint32_t* __restrict__ countA,
                                                             const uint32_t* __restrict__ listB, const uint32_t* __restrict__ countB,
-                                                            Grid g, Scalars S, uint32_t* flags, LevelFlags lf) {
+                                                            Grid g, SS<ST> S, uint32_t* flags, LevelFlags lf) {
   const uint32_t na = *countA, nb = listB ? *countB : 0u;
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na + nb; k += gridDim.x * blockDim.x) {
     const uint32_t b = k < na ? listA[k] : listB[k - na];
@@ -327,22 +370,23 @@ __global__ __launch_bounds__(256) void k_activate_reference(const uint32_t* __re
           uint32_t cx = base[0] + x, cy = base[1] + y, cz = base[2] + z;
           if (cx < g.W && cy < g.H && cz < g.D) {
             uint32_t i = vidx(g, cx, cy, cz);
-            float e = S.d[i] + S.t[i] * 0.7f + S.r[i] * 0.9f;
+            float e = ST::ld(S.d, i) + ST::ld(S.t, i) * 0.7f + S.r[i] * 0.9f;
             peak = fmaxf(peak, e);
             if (e > 0.012f) active = true;
           }
         }
     uint32_t ci = vidx(g, min(base[0] + 4, g.W - 1), min(base[1] + 4, g.H - 1), min(base[2] + 4, g.D - 1));
-    float ce = S.d[ci] + S.t[ci] * 0.7f + S.r[ci] * 0.9f;
+    float ce = ST::ld(S.d, ci) + ST::ld(S.t, ci) * 0.7f + S.r[ci] * 0.9f;
     peak = fmaxf(peak, ce);
     if (ce > 0.012f) active = true;
     if (!active) continue;
     dilate(g, b, (peak > 0.05f || ce > 0.035f) ? 1 : 0, flags, lf);
   }
 }
+template <class ST>
 __global__ __launch_bounds__(512) void k_activate_threshold(const uint32_t* __restrict__ listA, const uint32_t* __restrict__ countA,
                                                             const uint32_t* __restrict__ listB, const uint32_t* __restrict__ countB,
-                                                            Grid g, const StepConst* __restrict__ C, const float* __restrict__ den,
+                                                            Grid g, const StepConst* __restrict__ C, const typename ST::T* __restrict__ den,
                                                             Vec3 vel, uint32_t* flags, LevelFlags lf) {
   const uint32_t na = *countA, nb = listB ? *countB : 0u;
   const float dth = C->act_density, sth = C->act_speed;
@@ -352,7 +396,7 @@ __global__ __launch_bounds__(512) void k_activate_threshold(const uint32_t* __re
     int dens = 0, fast = 0;
     if (v.in) {
       F3 w = f3(vel.x[v.i], vel.y[v.i], vel.z[v.i]);
-      dens = den[v.i] > dth ? 1 : 0;
+      dens = ST::ld(den, v.i) > dth ? 1 : 0;
       fast = len3(w) > sth ? 1 : 0;
     }
     // Density-active bricks dilate by one brick; velocity-only bricks stay
@@ -480,7 +524,10 @@ __global__ __launch_bounds__(512) void k_retire(RetireArgs R) {
     const uint32_t n = *R.count[l];
     for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
       const uint32_t i = (R.list[l][k] << 9) | threadIdx.x;
-      for (int q = R.first[l]; q < R.first[l + 1]; ++q) R.b[q][i] = 0.f;
+      for (int q = R.first[l]; q < R.first[l + 1]; ++q) {
+        if (R.esz[q] == 4) static_cast<float*>(R.b[q])[i] = 0.f;
+        else static_cast<uint16_t*>(R.b[q])[i] = 0;
+      }
     }
   }
 }
@@ -519,8 +566,9 @@ __device__ __forceinline__ float thermal_noise(F3 pos, float time) {
   return clampf(a * 0.58f + b * 0.42f, 0.f, 1.f);
 }
 // SH/advect-scalar-maccormack.wgsl.ts:75-100
+template <class ST>
 __device__ __forceinline__ void decay_write(const Grid& g, uint32_t i, uint32_t y, float den, float tem, float fu, float rea,
-                                            float tn, float dt, Scalars D) {
+                                            float tn, float dt, SS<ST> D) {
   float ny = (float(y) + 0.5f) / float(g.H);
   float hot = smoothf(0.22f, 0.78f, tem + fu * 0.2f + rea * 0.26f);
   float coolingPocket = smoothf(0.44f, 0.9f, 1.f - tn) * smoothf(0.08f, 0.74f, den) * (1.f - hot * 0.72f);
@@ -528,23 +576,24 @@ __device__ __forceinline__ void decay_write(const Grid& g, uint32_t i, uint32_t 
   float smokeLoss = dt * (0.008f + ny * 0.026f) * mixf(0.75f, 1.38f, tn);
   float coldBreakup = dt * den * (1.f - smoothf(0.2f, 0.86f, hot)) * smoothf(0.18f, 0.72f, den) * smoothf(0.34f, 0.92f, tn) * 0.34f;
   float clearing = dt * den * hot * mixf(0.1f, 0.72f, tn);
-  D.d[i] = clamp01(fmaxf(0.f, den * (1.f - smokeLoss) - clearing - coldBreakup));
-  D.t[i] = clamp01(fmaxf(0.f, tem - cooling));
-  D.f[i] = clamp01(fmaxf(0.f, fu - dt * mixf(0.012f, 0.034f, tn)));
+  ST::st(D.d, i, clamp01(fmaxf(0.f, den * (1.f - smokeLoss) - clearing - coldBreakup)));
+  ST::st(D.t, i, clamp01(fmaxf(0.f, tem - cooling)));
+  ST::st(D.f, i, clamp01(fmaxf(0.f, fu - dt * mixf(0.012f, 0.034f, tn))));
   D.r[i] = clamp01(fmaxf(0.f, rea - dt * mixf(0.46f, 0.82f, tn)));
 }
 // SH/advect-scalar-maccormack.wgsl.ts:33-101 / SH/advect-scalar-semilagrangian.wgsl.ts:26-44.
 // The four advect<Name> calls share every position; sampleVelocity(coord-0.5)
 // and the `original` sample hit the voxel exactly (fraction 0).
+template <class ST>
 __global__ __launch_bounds__(512) void k_advect_scalars(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                        const StepConst* __restrict__ C, int mode, Scalars S, Vec3 us, Scalars D,
+                                                        const StepConst* __restrict__ C, int mode, SS<ST> S, Vec3 us, SS<ST> D,
                                                         const uint32_t* __restrict__ flags) {
   const float dt = C->p.delta_time, time = C->p.time;
   FX_BRICK_LOOP(list, count) {
     Vox v = vox_of(g, list[fx_k]);
     if (!v.in) continue;
     const uint32_t i = v.i;
-    if (!flag_on(g, flags, v.x, v.y, v.z)) { D.d[i] = 0.f; D.t[i] = 0.f; D.f[i] = 0.f; D.r[i] = 0.f; continue; }
+    if (!flag_on(g, flags, v.x, v.y, v.z)) { ST::st(D.d, i, 0.f); ST::st(D.t, i, 0.f); ST::st(D.f, i, 0.f); D.r[i] = 0.f; continue; }
     F3 coord = f3(float(v.x) + 0.5f, float(v.y) + 0.5f, float(v.z) + 0.5f);
     F3 half = f3(0.5f, 0.5f, 0.5f);
     F3 velocity = f3(us.x[i], us.y[i], us.z[i]);
@@ -552,23 +601,22 @@ __global__ __launch_bounds__(512) void k_advect_scalars(const uint32_t* __restri
     F3 sp = back - half;
     Tri tp = tri(g, sp);
     if (mode == FX_ADVECT_SEMI_LAGRANGIAN) {
-      decay_write(g, i, v.y, tri_eval(tp, S.d), tri_eval(tp, S.t), tri_eval(tp, S.f), tri_eval(tp, S.r), thermal_noise(sp, time), dt, D);
+      decay_write<ST>(g, i, v.y, tri_evalP<ST>(tp, S.d), tri_evalP<ST>(tp, S.t), tri_evalP<ST>(tp, S.f), tri_eval(tp, S.r), thermal_noise(sp, time), dt, D);
       continue;
     }
     F3 fwd = sample_u(g, us, sp);
     F3 rec = back + fwd * dt;
     Tri tr = tri(g, rec - half);
-    float out[4];
-    const float* fs[4] = {S.d, S.t, S.f, S.r};
-#pragma unroll
-    for (int f = 0; f < 4; ++f) {
-      float predicted = tri_eval(tp, fs[f]);
-      float recovered = tri_eval(tr, fs[f]);
-      float original = fs[f][i];
+    auto mac = [&](float predicted, float recovered, float original) {
       float corrected = predicted + 0.5f * (original - recovered);
-      out[f] = clampf(corrected, fminf(original, predicted), fmaxf(original, predicted));
-    }
-    decay_write(g, i, v.y, out[0], out[1], out[2], out[3], thermal_noise(coord, time), dt, D);
+      return clampf(corrected, fminf(original, predicted), fmaxf(original, predicted));
+    };
+    float out[4];
+    out[0] = mac(tri_evalP<ST>(tp, S.d), tri_evalP<ST>(tr, S.d), ST::ld(S.d, i));
+    out[1] = mac(tri_evalP<ST>(tp, S.t), tri_evalP<ST>(tr, S.t), ST::ld(S.t, i));
+    out[2] = mac(tri_evalP<ST>(tp, S.f), tri_evalP<ST>(tr, S.f), ST::ld(S.f, i));
+    out[3] = mac(tri_eval(tp, S.r), tri_eval(tr, S.r), S.r[i]);
+    decay_write<ST>(g, i, v.y, out[0], out[1], out[2], out[3], thermal_noise(coord, time), dt, D);
   }
 }
 
@@ -582,8 +630,9 @@ __device__ __forceinline__ float thermal_cluster(F3 c, float time) {
   float rising = 0.5f + 0.5f * sinf(p.x * 0.18f + p.y * 0.13f - p.z * 0.16f + time * 1.4f);
   return clampf(body * 0.46f + pocket * 0.34f + rising * 0.2f, 0.f, 1.f);
 }
+template <class ST>
 __global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                    const StepConst* __restrict__ C, Scalars S, Scalars D,
+                                                    const StepConst* __restrict__ C, SS<ST> S, SS<ST> D,
                                                     const uint32_t* __restrict__ flags) {
   const float dt = C->p.delta_time, time = C->p.time;
   const float ilo = C->ignition_lo, ihi = C->ignition_hi, rate = C->burn_rate;
@@ -591,17 +640,17 @@ __global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__
     Vox v = vox_of(g, list[fx_k]);
     if (!v.in) continue;
     const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
-    if (!flag_on(g, flags, x, y, z)) { D.d[i] = S.d[i]; D.t[i] = S.t[i]; D.f[i] = S.f[i]; D.r[i] = S.r[i]; continue; }
+    if (!flag_on(g, flags, x, y, z)) { ST::st(D.d, i, ST::ld(S.d, i)); ST::st(D.t, i, ST::ld(S.t, i)); ST::st(D.f, i, ST::ld(S.f, i)); D.r[i] = S.r[i]; continue; }
     auto hotgas = [&](uint32_t a, uint32_t b2, uint32_t c2) {
       uint32_t k = cidx(g, a, b2, c2);
-      return clampf(S.t[k] * 1.55f + S.f[k] * 0.26f + S.r[k] * 0.44f, 0.f, 1.f);
+      return clampf(ST::ld(S.t, k) * 1.55f + ST::ld(S.f, k) * 0.26f + S.r[k] * 0.44f, 0.f, 1.f);
     };
     float ny = (float(y) + 0.5f) / float(g.H);
     float cluster = thermal_cluster(f3(float(x), float(y), float(z)), time);
     float nh = (hotgas(incu(x, g.W - 1), y, z) + hotgas(decu(x), y, z) + hotgas(x, incu(y, g.H - 1), z) + hotgas(x, decu(y), z) +
                 hotgas(x, y, incu(z, g.D - 1)) + hotgas(x, y, decu(z))) * 0.16666667f;
     float localBefore = hotgas(x, y, z);
-    float den = S.d[i], tem = S.t[i], fu = S.f[i], rea = S.r[i];
+    float den = ST::ld(S.d, i), tem = ST::ld(S.t, i), fu = ST::ld(S.f, i), rea = S.r[i];
     float radiative = fmaxf(nh - localBefore * 0.55f, 0.f) * smoothf(0.04f, 0.76f, den) * (0.45f + (1.f - cluster) * 0.55f) * dt * 2.6f;
     float hotPocket = smoothf(0.58f, 0.92f, cluster + rea * 0.22f);
     float coolingPocket = smoothf(0.42f, 0.82f, 1.f - cluster) * smoothf(0.08f, 0.68f, den) * (1.f - smoothf(0.54f, 0.94f, rea));
@@ -615,17 +664,18 @@ __global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__
     float clearing = den * smoothf(0.18f, 0.82f, tem + rea * 0.22f + fu * 0.1f) * dt * mixf(0.55f, 2.45f, hotPocket);
     float coolingSmoke = den * tem * (coolingPocket + hotSoot * 0.32f) * dt * 0.16f;
     float reactionDecay = rea * (1.f - dt * mixf(0.68f, 0.22f, hotPocket));
-    D.f[i] = clamp01(fu - burn);
-    D.t[i] = clamp01(tem + burn * mixf(2.1f, 3.25f, hotPocket) + radiative - heatLoss);
-    D.d[i] = clamp01(fmaxf(0.f, den + smokeYield + coolingSmoke - densityLoss - clearing * 0.72f));
+    ST::st(D.f, i, clamp01(fu - burn));
+    ST::st(D.t, i, clamp01(tem + burn * mixf(2.1f, 3.25f, hotPocket) + radiative - heatLoss));
+    ST::st(D.d, i, clamp01(fmaxf(0.f, den + smokeYield + coolingSmoke - densityLoss - clearing * 0.72f)));
     D.r[i] = clamp01(fmaxf(reactionDecay, burn * mixf(4.f, 7.5f, hotPocket) + radiative * 1.8f));
   }
 }
 
 // ------------------------------------------------------------ buoyancy --
 // SH/apply-buoyancy.wgsl.ts:27-110 (render set, velocityScratch in place).
+template <class ST>
 __global__ __launch_bounds__(512) void k_buoyancy(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                  const StepConst* __restrict__ C, Scalars S, Vec3 us, const uint32_t* __restrict__ flags) {
+                                                  const StepConst* __restrict__ C, SS<ST> S, Vec3 us, const uint32_t* __restrict__ flags) {
   const fx_sim_params& P = C->p;
   FX_BRICK_LOOP(list, count) {
     Vox v = vox_of(g, list[fx_k]);
@@ -634,11 +684,11 @@ __global__ __launch_bounds__(512) void k_buoyancy(const uint32_t* __restrict__ l
     if (!flag_on(g, flags, x, y, z)) continue;
     auto hot = [&](uint32_t a, uint32_t b2, uint32_t c2) {
       uint32_t k = cidx(g, a, b2, c2);
-      return clampf(S.t[k] * 1.f + S.r[k] * 0.85f, 0.f, 1.f);
+      return clampf(ST::ld(S.t, k) * 1.f + S.r[k] * 0.85f, 0.f, 1.f);
     };
-    auto rd = [&](uint32_t a, uint32_t b2, uint32_t c2) { return S.d[cidx(g, a, b2, c2)]; };
+    auto rd = [&](uint32_t a, uint32_t b2, uint32_t c2) { return ST::ld(S.d, cidx(g, a, b2, c2)); };
     const float dt = P.delta_time;
-    const float den = S.d[i];
+    const float den = ST::ld(S.d, i);
     float response = P.buoyancy * (1.f + smoothf(6.f, 24.f, fabsf(P.buoyancy)) * 0.38f);
     float hotGas = hot(x, y, z);
     float lift = hotGas * response;
@@ -696,9 +746,10 @@ __global__ __launch_bounds__(512) void k_vorticity(const uint32_t* __restrict__ 
   }
 }
 // SH/apply-vorticity-confinement.wgsl.ts:30-66 (render set)
+template <class ST>
 __global__ __launch_bounds__(512) void k_confinement(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
                                                      const StepConst* __restrict__ C, Vec3 w, const float* __restrict__ wmag, Vec3 us,
-                                                     float* cmag, Scalars S) {
+                                                     float* cmag, SS<ST> S) {
   const fx_sim_params& P = C->p;
   FX_BRICK_LOOP(list, count) {
     Vox v = vox_of(g, list[fx_k]);
@@ -709,9 +760,9 @@ __global__ __launch_bounds__(512) void k_confinement(const uint32_t* __restrict_
     F3 grad = f3(m(incu(x, g.W - 1), y, z) - m(decu(x), y, z), m(x, incu(y, g.H - 1), z) - m(x, decu(y), z),
                  m(x, y, incu(z, g.D - 1)) - m(x, y, decu(z))) * 0.5f;
     F3 dir = grad / fmaxf(len3(grad), 0.0001f);
-    float hotGas = clampf(S.t[i] + S.r[i] * 0.85f, 0.f, 1.f);
+    float hotGas = clampf(ST::ld(S.t, i) + S.r[i] * 0.85f, 0.f, 1.f);
     float mask = clampf(P.vorticity_mask[0] + smoothf(0.02f, 0.38f, wmag[i]) * P.vorticity_mask[1] +
-                            smoothf(0.04f, 0.7f, hotGas) * P.vorticity_mask[2] + smoothf(0.04f, 0.65f, S.d[i]) * P.vorticity_mask[3],
+                            smoothf(0.04f, 0.7f, hotGas) * P.vorticity_mask[2] + smoothf(0.04f, 0.65f, ST::ld(S.d, i)) * P.vorticity_mask[3],
                         0.f, 4.f);
     F3 force = cross3(dir, f3(w.x[i], w.y[i], w.z[i])) * P.vorticity_strength * mask * 1.45f;
     F3 vel = f3(us.x[i], us.y[i], us.z[i]) + force * P.delta_time;
@@ -1157,7 +1208,8 @@ __global__ void k_pack_flags(const uint32_t* __restrict__ flags, uint32_t nb, ui
   bits[w] = v;
 }
 
-__global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, Scalars S, const uint32_t* __restrict__ flags,
+template <class ST>
+__global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, SS<ST> S, const uint32_t* __restrict__ flags,
                                                   const uint32_t* __restrict__ bits, uint32_t nwords, uint16_t* out) {
   extern __shared__ uint32_t sbits[];
   // isActiveSample (RP:378-381) reads the brick flags from LDS when they fit.
@@ -1223,8 +1275,8 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, Scalars S,
              az = uint32_t(floorf(clampf(sp.z, 0.f, dims.z)));
     if (active_brick(ax, ay, az)) {
       Tri t = tri(g, sp);
-      float den = fmaxf(tri_eval(t, S.d) - 0.004f, 0.f);
-      float tem = tri_eval(t, S.t), fu = tri_eval(t, S.f), rea = tri_eval(t, S.r);
+      float den = fmaxf(tri_evalP<ST>(t, S.d) - 0.004f, 0.f);
+      float tem = tri_evalP<ST>(t, S.t), fu = tri_evalP<ST>(t, S.f), rea = tri_eval(t, S.r);
       float hot = clampf(tem * 1.45f + fu * 0.22f + rea * 0.38f, 0.f, 1.f);
       if (den > 0.001f || hot > 0.018f) {
         if (den < 0.012f && hot < 0.04f) {
@@ -1232,9 +1284,9 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, Scalars S,
         } else {
           uint32_t cx = min(uint32_t(fmaxf(rintf(sp.x), 0.f)), mxu), cy = min(uint32_t(fmaxf(rintf(sp.y), 0.f)), myu),
                    cz = min(uint32_t(fmaxf(rintf(sp.z), 0.f)), mzu);
-          F3 dg = f3(S.d[vidx(g, min(cx + 1, mxu), cy, cz)] - S.d[vidx(g, cx == 0 ? 0 : cx - 1, cy, cz)],
-                     S.d[vidx(g, cx, min(cy + 1, myu), cz)] - S.d[vidx(g, cx, cy == 0 ? 0 : cy - 1, cz)],
-                     S.d[vidx(g, cx, cy, min(cz + 1, mzu))] - S.d[vidx(g, cx, cy, cz == 0 ? 0 : cz - 1)]);
+          F3 dg = f3(ST::ld(S.d, vidx(g, min(cx + 1, mxu), cy, cz)) - ST::ld(S.d, vidx(g, cx == 0 ? 0 : cx - 1, cy, cz)),
+                     ST::ld(S.d, vidx(g, cx, min(cy + 1, myu), cz)) - ST::ld(S.d, vidx(g, cx, cy == 0 ? 0 : cy - 1, cz)),
+                     ST::ld(S.d, vidx(g, cx, cy, min(cz + 1, mzu))) - ST::ld(S.d, vidx(g, cx, cy, cz == 0 ? 0 : cz - 1)));
           float gl = fmaxf(len3(dg), 0.0001f);
           F3 normal = gl > 0.00011f ? (-dg) / gl : f3(0.f, 1.f, 0.f);
           float detail = clampf(gl * 5.2f + hot * 0.28f + rea * 0.18f, 0.f, 1.f);
@@ -1246,7 +1298,7 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, Scalars S,
           if (nl > 0) {
             auto nearest = [&](F3 p) {
               F3 c = f3(clampf(p.x, 0.f, dims.x), clampf(p.y, 0.f, dims.y), clampf(p.z, 0.f, dims.z));
-              return S.d[vidx(g, min(uint32_t(floorf(c.x)), mxu), min(uint32_t(floorf(c.y)), myu), min(uint32_t(floorf(c.z)), mzu))];
+              return ST::ld(S.d, vidx(g, min(uint32_t(floorf(c.x)), mxu), min(uint32_t(floorf(c.y)), myu), min(uint32_t(floorf(c.z)), mzu)));
             };
             float d1 = fmaxf(nearest(sp + lightStep * 0.9f) - 0.004f, 0.f);
             float d2 = fmaxf(nearest(sp + lightStep * 2.f) - 0.004f, 0.f);
@@ -1321,7 +1373,11 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, Scalars S,
 
 void launch_inject(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
                    const float* src, Scalars set0, Vec3 u, uint32_t* touched, int parts) {
-  FX_LAUNCH(k_inject, blocks, 512, s, list, count, g, c, src, set0, u, touched, parts);
+  switch (set0.prec) {
+    case FX_PRECISION_BF16: FX_LAUNCH(k_inject<StBF16>, blocks, 512, s, list, count, g, c, src, typed<StBF16>(set0), u, touched, parts); break;
+    case FX_PRECISION_FP16: FX_LAUNCH(k_inject<StF16>, blocks, 512, s, list, count, g, c, src, typed<StF16>(set0), u, touched, parts); break;
+    default: FX_LAUNCH(k_inject<StF32>, blocks, 512, s, list, count, g, c, src, typed<StF32>(set0), u, touched, parts);
+  }
 }
 void launch_clear_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t value) {
   int blocks = int(std::min<size_t>((n + 255) / 256, 4096));
@@ -1342,9 +1398,17 @@ void launch_activate(hipStream_t s, int blocks, const uint32_t* listA, const uin
                      const LevelFlags& lf) {
   if (rule == FX_ACTIVATION_REFERENCE) {
     int b = std::max(1, std::min(blocks, int((brick_count(g) + 255) / 256) + 1));
-    FX_LAUNCH(k_activate_reference, b, 256, s, listA, countA, listB, countB, g, set, flags, lf);
+    switch (set.prec) {
+      case FX_PRECISION_BF16: FX_LAUNCH(k_activate_reference<StBF16>, b, 256, s, listA, countA, listB, countB, g, typed<StBF16>(set), flags, lf); break;
+      case FX_PRECISION_FP16: FX_LAUNCH(k_activate_reference<StF16>, b, 256, s, listA, countA, listB, countB, g, typed<StF16>(set), flags, lf); break;
+      default: FX_LAUNCH(k_activate_reference<StF32>, b, 256, s, listA, countA, listB, countB, g, typed<StF32>(set), flags, lf);
+    }
   } else {
-    FX_LAUNCH(k_activate_threshold, blocks, 512, s, listA, countA, listB, countB, g, c, set.d, vel, flags, lf);
+    switch (set.prec) {
+      case FX_PRECISION_BF16: FX_LAUNCH(k_activate_threshold<StBF16>, blocks, 512, s, listA, countA, listB, countB, g, c, typed<StBF16>(set).d, vel, flags, lf); break;
+      case FX_PRECISION_FP16: FX_LAUNCH(k_activate_threshold<StF16>, blocks, 512, s, listA, countA, listB, countB, g, c, typed<StF16>(set).d, vel, flags, lf); break;
+      default: FX_LAUNCH(k_activate_threshold<StF32>, blocks, 512, s, listA, countA, listB, countB, g, c, typed<StF32>(set).d, vel, flags, lf);
+    }
   }
 }
 void launch_compact(hipStream_t s, const CompactArgs& A) {
@@ -1359,22 +1423,38 @@ void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, con
 }
 void launch_advect_scalars(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
                            int mode, Scalars src, Vec3 us, Scalars dst, const uint32_t* flags) {
-  FX_LAUNCH(k_advect_scalars, blocks, 512, s, list, count, g, c, mode, src, us, dst, flags);
+  switch (src.prec) {
+    case FX_PRECISION_BF16: FX_LAUNCH(k_advect_scalars<StBF16>, blocks, 512, s, list, count, g, c, mode, typed<StBF16>(src), us, typed<StBF16>(dst), flags); break;
+    case FX_PRECISION_FP16: FX_LAUNCH(k_advect_scalars<StF16>, blocks, 512, s, list, count, g, c, mode, typed<StF16>(src), us, typed<StF16>(dst), flags); break;
+    default: FX_LAUNCH(k_advect_scalars<StF32>, blocks, 512, s, list, count, g, c, mode, typed<StF32>(src), us, typed<StF32>(dst), flags);
+  }
 }
 void launch_combustion(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
                        Scalars src, Scalars dst, const uint32_t* flags) {
-  FX_LAUNCH(k_combustion, blocks, 512, s, list, count, g, c, src, dst, flags);
+  switch (src.prec) {
+    case FX_PRECISION_BF16: FX_LAUNCH(k_combustion<StBF16>, blocks, 512, s, list, count, g, c, typed<StBF16>(src), typed<StBF16>(dst), flags); break;
+    case FX_PRECISION_FP16: FX_LAUNCH(k_combustion<StF16>, blocks, 512, s, list, count, g, c, typed<StF16>(src), typed<StF16>(dst), flags); break;
+    default: FX_LAUNCH(k_combustion<StF32>, blocks, 512, s, list, count, g, c, typed<StF32>(src), typed<StF32>(dst), flags);
+  }
 }
 void launch_buoyancy(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
                      Scalars set0, Vec3 us, const uint32_t* flags) {
-  FX_LAUNCH(k_buoyancy, blocks, 512, s, list, count, g, c, set0, us, flags);
+  switch (set0.prec) {
+    case FX_PRECISION_BF16: FX_LAUNCH(k_buoyancy<StBF16>, blocks, 512, s, list, count, g, c, typed<StBF16>(set0), us, flags); break;
+    case FX_PRECISION_FP16: FX_LAUNCH(k_buoyancy<StF16>, blocks, 512, s, list, count, g, c, typed<StF16>(set0), us, flags); break;
+    default: FX_LAUNCH(k_buoyancy<StF32>, blocks, 512, s, list, count, g, c, typed<StF32>(set0), us, flags);
+  }
 }
 void launch_vorticity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 us, Vec3 w, float* wmag) {
   FX_LAUNCH(k_vorticity, blocks, 512, s, list, count, g, us, w, wmag);
 }
 void launch_confinement(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c, Vec3 w,
                         const float* wmag, Vec3 us, float* cmag, Scalars set0) {
-  FX_LAUNCH(k_confinement, blocks, 512, s, list, count, g, c, w, wmag, us, cmag, set0);
+  switch (set0.prec) {
+    case FX_PRECISION_BF16: FX_LAUNCH(k_confinement<StBF16>, blocks, 512, s, list, count, g, c, w, wmag, us, cmag, typed<StBF16>(set0)); break;
+    case FX_PRECISION_FP16: FX_LAUNCH(k_confinement<StF16>, blocks, 512, s, list, count, g, c, w, wmag, us, cmag, typed<StF16>(set0)); break;
+    default: FX_LAUNCH(k_confinement<StF32>, blocks, 512, s, list, count, g, c, w, wmag, us, cmag, typed<StF32>(set0));
+  }
 }
 void launch_divergence(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 us, float* div) {
   FX_LAUNCH(k_divergence, blocks, 512, s, list, count, g, us, div);
@@ -1440,8 +1520,13 @@ void launch_raymarch(hipStream_t s, Grid g, const RayConst& rc, Scalars set0, co
   const bool lds = bits && nwords <= 8192;  // up to 32 KB of flag bits in LDS
   if (lds) FX_LAUNCH(k_pack_flags, (nwords + 255) / 256, 256, s, flags, nb, bits);
   dim3 grid((rc.width + 15) / 16, (rc.height + 15) / 16);
-  hipLaunchKernelGGL(k_raymarch, grid, dim3(256), lds ? nwords * 4 : 0, s, g, rc, set0, flags, lds ? bits : nullptr,
-                     nwords, out);
+  const uint32_t* b = lds ? bits : nullptr;
+  const size_t sh = lds ? nwords * 4 : 0;
+  switch (set0.prec) {
+    case FX_PRECISION_BF16: hipLaunchKernelGGL(k_raymarch<StBF16>, grid, dim3(256), sh, s, g, rc, typed<StBF16>(set0), flags, b, nwords, out); break;
+    case FX_PRECISION_FP16: hipLaunchKernelGGL(k_raymarch<StF16>, grid, dim3(256), sh, s, g, rc, typed<StF16>(set0), flags, b, nwords, out); break;
+    default: hipLaunchKernelGGL(k_raymarch<StF32>, grid, dim3(256), sh, s, g, rc, typed<StF32>(set0), flags, b, nwords, out);
+  }
 }
 
 }  // namespace fx

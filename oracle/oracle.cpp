@@ -76,6 +76,28 @@ template <class T> inline T smooth(T e0, T e1, T x) {
 template <class T> inline T stepT(T edge, T x) { return edge <= x ? T(1) : T(0); }
 template <class T> inline T fractT(T x) { return x - std::floor(x); }
 
+// 16-bit storage of density, temperature and fuel (FX_PRECISION_BF16/FP16):
+// values are rounded to nearest even on every store; FP32 arithmetic.
+inline float round_bf16(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  float r;
+  std::memcpy(&r, &u, 4);
+  return r;
+}
+inline float round_f16(float f) {
+  // IEEE binary16, round to nearest even, including subnormals (unit 2^-24).
+  float a = std::fabs(f);
+  if (!(a < 65520.f)) return std::copysign(INFINITY, f);
+  if (a < 6.103515625e-05f) return std::copysign(std::nearbyint(a * 16777216.f) / 16777216.f, f);
+  int e;
+  std::frexp(a, &e);                       // a in [2^(e-1), 2^e)
+  float ulp = std::ldexp(1.f, e - 11);     // 10 fraction bits
+  return std::copysign(std::nearbyint(a / ulp) * ulp, f);
+}
+
 struct Dims {
   uint32_t W, H, D;
   size_t N() const { return size_t(W) * H * D; }
@@ -280,6 +302,17 @@ template <class T> struct Sim : Base {
   std::vector<double> resid;
 
   bool sparse() const { return desc.domain_mode == FX_DOMAIN_SPARSE; }
+  // Rounds a scalar array to the storage precision (FP32 build only; the FP64
+  // build is the unrounded reference).
+  void round_store(std::vector<T>& a) const {
+    if (sizeof(T) != sizeof(float) || desc.precision == FX_PRECISION_FP32) return;
+    const bool bf = desc.precision == FX_PRECISION_BF16;
+    for (auto& v : a) v = T(bf ? round_bf16(float(v)) : round_f16(float(v)));
+  }
+  void round_set(int set, bool with_fuel = true) {
+    round_store(S[set][0]); round_store(S[set][1]);
+    if (with_fuel) round_store(S[set][2]);
+  }
   size_t brickOf(uint32_t x, uint32_t y, uint32_t z) const { return (x >> 3) + BX * ((y >> 3) + size_t(BY) * (z >> 3)); }
   bool on(uint32_t x, uint32_t y, uint32_t z) const { return bact[brickOf(x, y, z)] != 0; }
 
@@ -406,6 +439,7 @@ template <class T> struct Sim : Base {
           }
           D_[i] = den; Tm[i] = tem; F[i] = fu;
         }
+    round_set(0);
   }
 
   // SH/inject-velocity.wgsl.ts:46-96
@@ -491,6 +525,7 @@ template <class T> struct Sim : Base {
           }
           R[i] = rea; Tm[i] = tem;
         }
+    round_set(0, false);
   }
 
   // ------------------------------------------------------ activation --
@@ -656,6 +691,7 @@ template <class T> struct Sim : Base {
             decay_write(i, y, vals[0], vals[1], vals[2], vals[3], thermalNoise(coord, P.time), P);
           }
         }
+    round_set(1);
   }
 
   // ------------------------------------------------------ combustion --
@@ -712,6 +748,7 @@ template <class T> struct Sim : Base {
           S[0][0][i] = clamp01(std::max(T(0), den + smokeYield + coolingSmoke - densityLoss - clearing * T(0.72)));
           S[0][3][i] = clamp01(std::max(reactionDecay, burn * mixT(T(4), T(7.5), hotPocket) + radiative * T(1.8)));
         }
+    round_set(0);
   }
 
   // -------------------------------------------------------- buoyancy --
@@ -1186,6 +1223,8 @@ template <class T> struct Sim : Base {
     auto* f = field(id);
     if (!f) return FX_ERR_INVALID_ARG;
     for (size_t i = 0; i < f->size(); ++i) (*f)[i] = T(in[i]);
+    if ((id >= FX_FIELD_DENSITY && id <= FX_FIELD_FUEL) || (id >= FX_FIELD_ADV_DENSITY && id <= FX_FIELD_ADV_FUEL))
+      round_store(*f);
     return 0;
   }
   int read_flags(int which, uint32_t* out) override {

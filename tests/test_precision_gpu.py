@@ -1,0 +1,128 @@
+"""Mixed precision: density, temperature and fuel stored in BF16 or FP16 (FP32
+arithmetic, FP32 reaction/velocity/pressure).
+
+Parity is checked against the FP32 oracle with the same round-to-nearest-even
+on store; accuracy is reported against the FP64 oracle (C4-style scene at a
+size the FP64 oracle finishes in seconds).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import scenes
+from fenix_amd import abi
+from fenix_amd.emitters import ScalarEmitter
+from fenix_amd.simulation import CombustionVolumeSimulation, make_desc
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+PERSISTENT = ("density", "temperature", "fuel", "reaction", "velocity_x", "velocity_y", "velocity_z")
+# One 16-bit ulp flip in a handful of voxels moves rel L2 by ~1e-5 (bf16: 2^-8 relative).
+TOL = {abi.FX_PRECISION_BF16: 2e-3, abi.FX_PRECISION_FP16: 3e-4}
+
+
+def rel_l2(a, b):
+    den = float(np.linalg.norm(b))
+    num = float(np.linalg.norm(np.asarray(a, np.float64) - np.asarray(b, np.float64)))
+    return num / den if den > 0 else num
+
+
+def c4_sources(n, seed=4, count=4, radius_vox=None):
+    rng = scenes.SplitMix64(seed)
+    r = (radius_vox or max(3.0, n / 25.6)) / n
+    out = []
+    for _ in range(count):
+        x = r + (1 - 2 * r) * rng.uniform()
+        z = r + (1 - 2 * r) * rng.uniform()
+        out.append(ScalarEmitter(position=(x, r, z), radius=r, densityRate=1.0, heatRate=2.0, fuelRate=1.5))
+    return out
+
+
+@pytest.mark.parametrize("prec", [abi.FX_PRECISION_BF16, abi.FX_PRECISION_FP16])
+@pytest.mark.parametrize("pass_name", ["inject_scalar", "inject_igniter", "advect_scalars", "combustion", "buoyancy",
+                                       "confinement"])
+def test_pass_parity_16bit(prec, pass_name):
+    grid = (24, 16, 20)
+    desc = make_desc(grid, precision=prec, runtime={"worldSize": 24.0})
+    sim = CombustionVolumeSimulation(desc)
+    orc = Oracle(desc, 32)
+    srcs = scenes.mixed_sources()
+    sim.updateSources(srcs)
+    orc.update_sources(*scenes.packed(srcs))
+    st = scenes.random_state(grid[::-1], seed=21)
+    for k, v in st.items():
+        sim.write_field(k, v)
+        orc.write(k, v)
+    # the written 16-bit fields are identical on both sides
+    np.testing.assert_array_equal(sim.read_field("density"), orc.read("density").astype(np.float32))
+    p = scenes.sim_params(time=0.4, dt=1.0 / 30.0, dx=1.0, world=24.0)
+    for name in ("activate_render", "activate_advected"):
+        sim.run_pass(name, p)
+        orc.run_pass(name, p)
+    sim.run_pass(pass_name, p)
+    orc.run_pass(pass_name, p)
+    for name in abi.FIELDS:
+        e = rel_l2(sim.read_field(name), orc.read(name))
+        assert e <= TOL[prec], f"{pass_name}/{prec}: {name} rel L2 {e:.3e}"
+
+
+@pytest.mark.parametrize("prec", [abi.FX_PRECISION_BF16, abi.FX_PRECISION_FP16])
+def test_step_parity_16bit_sparse(prec):
+    n = 48
+    desc = scenes.c2_desc(n, precision=prec)
+    srcs = scenes.c2_sources(n, radius_vox=4.0)
+    sim = CombustionVolumeSimulation(desc)
+    orc = Oracle(desc, 32)
+    sim.updateSources(srcs)
+    orc.update_sources(*scenes.packed(srcs))
+    for k in range(20):
+        if k > 0:
+            for name in PERSISTENT:
+                sim.write_field(name, orc.read(name))
+        sim.step(None, k / 60.0, 1 / 60.0)
+        orc.step(k / 60.0, 1 / 60.0)
+        np.testing.assert_array_equal(sim.read_flags(), orc.flags())
+        for name in PERSISTENT:
+            e = rel_l2(sim.read_field(name), orc.read(name))
+            assert e <= TOL[prec], f"step {k}: {name} rel L2 {e:.3e}"
+
+
+def test_precision_report_against_fp64():
+    """C4-style sparse scene (4 seeded emitters, config mode) for 100 steps:
+    per-field relative L2 and max error of FP32 / BF16 / FP16 storage against
+    the FP64 oracle. FP32 must stay within 1e-3 relative L2."""
+    n, steps = 64, 100
+    srcs = c4_sources(n)
+    ref = Oracle(scenes.c2_desc(n), 64)
+    ref.update_sources(*scenes.packed(srcs))
+    for k in range(steps):
+        ref.step(k / 60.0, 1 / 60.0)
+    truth = {f: ref.read(f) for f in PERSISTENT}
+    report = {"scene": f"sparse {n}^3, 4 floor emitters (seed 4), config mode, {steps} steps",
+              "active_bricks_fp64": ref.active_count(), "fields": {}}
+    for label, prec in (("fp32", abi.FX_PRECISION_FP32), ("bf16", abi.FX_PRECISION_BF16), ("fp16", abi.FX_PRECISION_FP16)):
+        sim = CombustionVolumeSimulation(scenes.c2_desc(n, precision=prec))
+        sim.updateSources(srcs)
+        for k in range(steps):
+            sim.step(None, k / 60.0, 1 / 60.0)
+        row = {}
+        for f in PERSISTENT:
+            g = sim.read_field(f).astype(np.float64)
+            row[f] = {"rel_l2": rel_l2(g, truth[f]), "max_abs": float(np.max(np.abs(g - truth[f]))),
+                      "ref_max": float(np.max(np.abs(truth[f])))}
+        report["fields"][label] = row
+        report.setdefault("active_bricks", {})[label] = sim.active_brick_count()
+        sim.dispose()
+    out = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
+    os.makedirs(out, exist_ok=True)
+    with open(os.path.join(out, "precision_report.json"), "w") as fh:
+        json.dump(report, fh, indent=1)
+    for f in ("density", "temperature", "fuel", "velocity_y"):
+        assert report["fields"]["fp32"][f]["rel_l2"] <= 1e-3, (f, report["fields"]["fp32"][f])
+    for label in ("bf16", "fp16"):
+        for f in PERSISTENT:
+            assert np.isfinite(report["fields"][label][f]["rel_l2"])
+            assert report["fields"][label][f]["rel_l2"] < 0.2, (label, f, report["fields"][label][f])
