@@ -5,9 +5,11 @@ Default workload (sparse 256^3): FP32, 8^3 bricks, threshold activation
 (density > 1e-4 or |u| > 1e-3, one-brick dilation), two seeded floor emitters
 of radius 10 voxels, burn rate 0.5, ignition T = 0.6, 4-level V-cycle (2/2
 sweeps, omega 2/3, 4 cycles), vorticity eps = 0.3, and a 1024x1024 raymarch with
-128 samples per step. `--warmup` untimed steps (default 100) build the plume,
-then `--steps` steps (default 100) are timed: a "step" is fx_step + the
-raymarch, with every input already resident in HBM.
+128 samples per step. The scene starts from a seeded, developed rising plume
+holding ~15% of the bricks active (`c2-cold` starts from an empty volume).
+`--warmup` untimed steps (default 100) run first, then `--steps` steps
+(default 100) are timed: a "step" is fx_step + the raymarch, with every input
+already resident in HBM.
 
 Prints one JSON line (see the keys below).
 """
@@ -46,6 +48,14 @@ STAGE_BYTES = {
 # 602.2 + raymarch 16 B/voxel (+ 8 B per pixel, added per workload).
 STEP_BYTES_SIM = 602.2
 RAY_BYTES_PER_VOXEL = 16.0
+# Accesses of the density/temperature/fuel planes per active voxel, per stage: with
+# 16-bit storage each one moves 2 bytes fewer than the FP32 figure above.
+SCALAR_ACCESSES = {"advect_scalars": 6, "combustion": 6, "buoyancy": 2, "confinement": 2}
+# Whole-step simulation bytes with 16-bit d/T/f (the stages above plus the two
+# activation passes, which read d and T at 65/512 of the voxels).
+STEP_BYTES_SIM_16 = 569.2
+RAY_SCALAR_ACCESSES = 3
+PRECISIONS = {"fp32": (abi.FX_PRECISION_FP32, 4), "bf16": (abi.FX_PRECISION_BF16, 2), "fp16": (abi.FX_PRECISION_FP16, 2)}
 
 
 class HipBuffer:
@@ -64,18 +74,28 @@ class HipBuffer:
             self.ptr = C.c_void_p()
 
 
-def workload(name):
+def workload(name, precision="fp32"):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import scenes
-    if name == "c2":
+    prec = PRECISIONS[precision][0]
+    tag = precision.upper()
+    if name in ("c2", "c2-cold"):
         n = 256
-        return dict(n=n, desc=scenes.c2_desc(n), sources=scenes.c2_sources(n), render=(1024, 1024, 128),
-                    label="sparse 256^3 FP32, 8^3 bricks, 2 floor emitters r=10, threshold activation, "
-                          "4-level V-cycle, 1024x1024x128 raymarch")
+        srcs = scenes.c2_sources(n)
+        wl = dict(n=n, desc=scenes.c2_desc(n, precision=prec), sources=srcs, render=(1024, 1024, 128), state=None,
+                  label=f"sparse 256^3 {tag} storage, 8^3 bricks, 2 floor emitters r=10, threshold activation, "
+                        "4-level V-cycle, 1024x1024x128 raymarch")
+        if name == "c2":
+            # The config's ~15% active set: start from the developed plume (seeded).
+            wl["state"], frac = scenes.c2_plume_state(n, srcs, target=0.15)
+            wl["label"] += f", seeded plume start ({frac:.1%} of bricks)"
+        else:
+            wl["label"] += ", empty start"
+        return wl
     if name == "c1":
         n = 64
-        return dict(n=n, desc=scenes.c1_desc(n), sources=scenes.c1_sources(), render=(256, 256, 64),
-                    label="dense 64^3 FP32 reference semantics, 256x256x64 raymarch")
+        return dict(n=n, desc=scenes.c1_desc(n, precision=prec), sources=scenes.c1_sources(), render=(256, 256, 64), state=None,
+                    label=f"dense 64^3 {tag} storage, reference semantics, 256x256x64 raymarch")
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -120,15 +140,21 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=10, help="extra steps timed per stage with HIP events")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=sorted(PRECISIONS), default="fp32",
+                    help="storage of density/temperature/fuel (arithmetic is FP32 throughout)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: there is no multi-GPU path (the sharding plan is design-only).",
               file=sys.stderr)
         sys.exit(2)
 
-    wl = workload(args.workload)
+    wl = workload(args.workload, args.precision)
+    saved = 4 - PRECISIONS[args.precision][1]
     sim = CombustionVolumeSimulation(wl["desc"])
     sim.updateSources(wl["sources"])
+    for name, values in (wl["state"] or {}).items():
+        sim.write_field(name, values)
+    wl["state"] = None
     w, h, sc = wl["render"]
     rp = VolumeRaymarchPass((wl["n"],) * 3)
     rp.setRenderParams(stepCount=sc)
@@ -169,12 +195,14 @@ def main():
         raise SystemExit("--timing-steps must be >= 1 to find the dominant kernel")
     dom = max(candidates, key=candidates.get)
     avg_ms = stage_ms[dom]
+    ray_bytes = RAY_BYTES_PER_VOXEL - saved * RAY_SCALAR_ACCESSES
+    step_bytes_voxel = (STEP_BYTES_SIM if saved == 0 else STEP_BYTES_SIM_16) + ray_bytes
     if dom == "raymarch":
-        bytes_launch = RAY_BYTES_PER_VOXEL * active_t + 8.0 * w * h
+        bytes_launch = ray_bytes * active_t + 8.0 * w * h
     else:
-        bytes_launch = STAGE_BYTES[dom] * active_t
+        bytes_launch = (STAGE_BYTES[dom] - saved * SCALAR_ACCESSES.get(dom, 0)) * active_t
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    step_bytes = (STEP_BYTES_SIM + RAY_BYTES_PER_VOXEL) * voxels + 8.0 * w * h
+    step_bytes = step_bytes_voxel * voxels + 8.0 * w * h
     step_gbs = step_bytes / (ms * 1e-3) / 1e9
 
     out = {
@@ -190,13 +218,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": wl["label"], "path": "plain FP32 single-scene", "active_bricks": active,
+        "config": {"workload": wl["label"], "storage": args.precision, "active_bricks": active,
                    "active_fraction": active / sim.brick_count, "render": f"{w}x{h}x{sc}"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": bytes_launch, "avg_launch_ms": avg_ms},
         "step_hbm": {"algorithmic_gbs": step_gbs, "frac": step_gbs / HBM_PEAK_GBS,
-                     "bytes_per_active_voxel": STEP_BYTES_SIM + RAY_BYTES_PER_VOXEL},
+                     "bytes_per_active_voxel": step_bytes_voxel},
         "stage_ms": stage_ms,
     }
     if not args.no_cpu:

@@ -815,70 +815,153 @@ __global__ __launch_bounds__(512) void k_jacobi(const uint32_t* __restrict__ lis
 // __syncthreads between sweeps): p0 -> p1 -> p0 ... The first sweep starts from
 // zero when zero_start is set. Used for the coarsest level, whose dispatches
 // would otherwise be launch-bound.
-constexpr uint32_t kSmoothSlots = 8;      // active bricks staged in LDS
-constexpr uint32_t kSmoothTable = 4096;   // level bricks addressable through the LDS slot table
+//
+// Staged path: the active interior voxels of the listed bricks are compacted
+// into LDS slots (per-brick 512-bit masks, one 64-bit ballot per z-slice, and an
+// exclusive scan of the per-brick counts); each thread owns up to kSmoothPer
+// slots and resolves their six neighbour slots once, so every sweep is LDS-only.
+// A neighbour outside the slots (inactive, boundary, or in an unlisted brick)
+// holds 0, as in the sparse invariant.
+constexpr uint32_t kSmoothCap = 8192;      // active voxels staged in LDS
+constexpr uint32_t kSmoothBricks = 512;    // listed bricks whose voxel masks fit in LDS
+constexpr uint32_t kSmoothTable = 4096;    // level bricks addressable through the LDS slot table
+constexpr int kSmoothPer = int(kSmoothCap / 1024);
+
+// Bit position of the r-th (0-based) set bit of w; w has more than r set bits.
+__device__ __forceinline__ uint32_t select_bit(uint64_t w, uint32_t r) {
+  uint32_t pos = 0;
+  for (uint32_t width = 32; width >= 1; width >>= 1) {
+    const uint32_t c = uint32_t(__popcll(w & ((1ull << width) - 1ull)));
+    if (r >= c) { r -= c; w >>= width; pos += width; }
+  }
+  return pos;
+}
+// Active voxels of listed brick k that precede local voxel t.
+__device__ __forceinline__ uint32_t rank_in_brick(const uint64_t* am, uint32_t k, uint32_t t) {
+  uint32_t r = 0;
+  for (uint32_t j = 0; j < (t >> 6); ++j) r += uint32_t(__popcll(am[k * 8 + j]));
+  return r + uint32_t(__popcll(am[k * 8 + (t >> 6)] & ((1ull << (t & 63u)) - 1ull)));
+}
+
 __global__ __launch_bounds__(1024) void k_smooth_block(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
                                                        LvMask m, const StepConst* __restrict__ C, const float* __restrict__ rhs,
                                                        float* p0, float* p1, int sweeps, int weighted, int zero_start) {
-  __shared__ float sp[2][kSmoothSlots * 512];
-  __shared__ float sr[kSmoothSlots * 512];
+  __shared__ float sp[2][kSmoothCap];
+  __shared__ float sr[kSmoothCap];
+  __shared__ uint64_t amask[kSmoothBricks * 8];
+  __shared__ uint32_t base[kSmoothBricks];
   __shared__ int16_t slot[kSmoothTable];
+  __shared__ uint32_t wtot[16];
   const float om = C->omega;
   const uint32_t n = *count, nbl = g.BX * g.BY * g.BZ;
   const uint32_t total = n * 512u;
-  if (n <= kSmoothSlots && nbl <= kSmoothTable) {
-    // Stage the active bricks: bricks outside the list hold zeros (sparse invariant).
-    for (uint32_t q = threadIdx.x; q < nbl; q += blockDim.x) slot[q] = -1;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  uint32_t nact = 0;
+  if (n <= kSmoothBricks && nbl <= kSmoothTable) {
+    for (uint32_t q = tid; q < nbl; q += 1024) slot[q] = -1;
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) slot[list[k]] = int16_t(k);
-    for (uint32_t q = threadIdx.x; q < total; q += blockDim.x) {
-      const uint32_t gi = (list[q >> 9] << 9) | (q & 511u);
-      sr[q] = rhs[gi];
-      sp[0][q] = zero_start ? 0.f : p0[gi];
+    for (uint32_t k = tid; k < n; k += 1024) slot[list[k]] = int16_t(k);
+    // one ballot per (brick, z-slice): bit = x + 8y of an active interior voxel
+    for (uint32_t wd = wave; wd < n * 8; wd += 16) {
+      const uint32_t b = list[wd >> 3], lz = wd & 7u;
+      const uint32_t x = (b % g.BX) * 8 + (lane & 7u), y = ((b / g.BX) % g.BY) * 8 + (lane >> 3),
+                     z = (b / (g.BX * g.BY)) * 8 + lz;
+      const bool act = x < g.W && y < g.H && z < g.D && !on_boundary(g, x, y, z) && lv_active(m, x, y, z);
+      const uint64_t bits = __ballot(act);
+      if (lane == 0) amask[wd] = bits;
     }
     __syncthreads();
-    // Per-thread voxels (<= 4 with 1024 threads and 8 slots): neighbour offsets
-    // into the staged slots are resolved once; -1 marks a zero neighbour, -2 a
-    // voxel that is always 0 (boundary, inactive or padding).
-    constexpr int kPer = kSmoothSlots * 512 / 1024;
-    int nbr[kPer][6];
-    int self[kPer];
-    for (int j = 0; j < kPer; ++j) {
-      const uint32_t q = threadIdx.x + uint32_t(j) * 1024u;
-      self[j] = -2;
-      if (q >= total) continue;
-      const uint32_t b = list[q >> 9], t = q & 511u;
-      uint32_t x = (b % g.BX) * 8 + (t & 7u), y = ((b / g.BX) % g.BY) * 8 + ((t >> 3) & 7u), z = (b / (g.BX * g.BY)) * 8 + (t >> 6);
-      if (x >= g.W || y >= g.H || z >= g.D || on_boundary(g, x, y, z) || !lv_active(m, x, y, z)) continue;
-      self[j] = int(q);
+    // exclusive scan of the per-brick counts (n <= 512 <= blockDim)
+    uint32_t cnt = 0;
+    if (tid < n)
+      for (uint32_t j = 0; j < 8; ++j) cnt += uint32_t(__popcll(amask[tid * 8 + j]));
+    uint32_t incl = cnt;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+      const uint32_t v = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += v;
+    }
+    if (lane == 63) wtot[wave] = incl;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (uint32_t w = 0; w < wave; ++w) pre += wtot[w];
+    if (tid < n) base[tid] = pre + incl - cnt;
+    for (uint32_t w = 0; w < 16; ++w) nact += wtot[w];
+    __syncthreads();
+  }
+  if (n <= kSmoothBricks && nbl <= kSmoothTable && nact <= kSmoothCap) {
+    // stage the right-hand side and the start value of every active voxel
+    for (uint32_t q = tid; q < total; q += 1024) {
+      const uint32_t k = q >> 9, t = q & 511u;
+      if (!((amask[k * 8 + (t >> 6)] >> (t & 63u)) & 1ull)) continue;
+      const uint32_t sl = base[k] + rank_in_brick(amask, k, t);
+      const uint32_t gi = (list[k] << 9) | t;
+      sr[sl] = rhs[gi];
+      sp[0][sl] = zero_start ? 0.f : p0[gi];
+    }
+    // owned slots: global index and neighbour slots (-1: the neighbour holds 0)
+    int nbr[kSmoothPer][6];
+    uint32_t gid[kSmoothPer];
+#pragma unroll
+    for (int j = 0; j < kSmoothPer; ++j) {
+      const uint32_t q = tid + uint32_t(j) * 1024u;
+      gid[j] = 0xffffffffu;
+      if (q >= nact) continue;
+      uint32_t lo = 0, hi = n - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= q) lo = mid; else hi = mid - 1;
+      }
+      uint32_t r = q - base[lo], wd = 0;
+      for (; wd < 7; ++wd) {
+        const uint32_t c = uint32_t(__popcll(amask[lo * 8 + wd]));
+        if (r < c) break;
+        r -= c;
+      }
+      const uint32_t t = wd * 64 + select_bit(amask[lo * 8 + wd], r);
+      const uint32_t b = list[lo];
+      gid[j] = (b << 9) | t;
+      const uint32_t x = (b % g.BX) * 8 + (t & 7u), y = ((b / g.BX) % g.BY) * 8 + ((t >> 3) & 7u),
+                     z = (b / (g.BX * g.BY)) * 8 + (t >> 6);
       const uint32_t nx[6] = {x + 1, x - 1, x, x, x, x}, ny[6] = {y, y, y + 1, y - 1, y, y}, nz[6] = {z, z, z, z, z + 1, z - 1};
+#pragma unroll
       for (int k = 0; k < 6; ++k) {
-        int sl = slot[(nx[k] >> 3) + g.BX * ((ny[k] >> 3) + g.BY * (nz[k] >> 3))];
-        nbr[j][k] = sl < 0 ? -1 : int((uint32_t(sl) << 9) | ((nz[k] & 7u) << 6) | ((ny[k] & 7u) << 3) | (nx[k] & 7u));
+        const int sl = slot[(nx[k] >> 3) + g.BX * ((ny[k] >> 3) + g.BY * (nz[k] >> 3))];
+        nbr[j][k] = -1;
+        if (sl < 0) continue;
+        const uint32_t tn = ((nz[k] & 7u) << 6) | ((ny[k] & 7u) << 3) | (nx[k] & 7u);
+        if ((amask[uint32_t(sl) * 8 + (tn >> 6)] >> (tn & 63u)) & 1ull) nbr[j][k] = int(base[sl] + rank_in_brick(amask, uint32_t(sl), tn));
       }
     }
+    __syncthreads();
     int cur = 0;
     for (int sw = 0; sw < sweeps; ++sw) {
       const float* A = sp[cur];
       float* B = sp[1 - cur];
-      for (int j = 0; j < kPer; ++j) {
-        const uint32_t q = threadIdx.x + uint32_t(j) * 1024u;
-        if (q >= total) break;
-        if (self[j] < 0) { B[q] = 0.f; continue; }
+#pragma unroll
+      for (int j = 0; j < kSmoothPer; ++j) {
+        const uint32_t q = tid + uint32_t(j) * 1024u;
+        if (q >= nact) break;
         float v[6];
+#pragma unroll
         for (int k = 0; k < 6; ++k) v[k] = nbr[j][k] < 0 ? 0.f : A[nbr[j][k]];
-        float sum = v[0] + v[1] + v[2] + v[3] + v[4] + v[5];
-        float pj = (sum - sr[q]) / 6.f;
+        const float sum = v[0] + v[1] + v[2] + v[3] + v[4] + v[5];
+        const float pj = (sum - sr[q]) / 6.f;
         B[q] = weighted ? mixf(A[q], pj, om) : pj;
       }
       __syncthreads();
       cur = 1 - cur;
     }
     float* dst = (sweeps & 1) ? p1 : p0;
-    for (uint32_t q = threadIdx.x; q < total; q += blockDim.x) {
-      const uint32_t b = list[q >> 9], t = q & 511u;
-      uint32_t x = (b % g.BX) * 8 + (t & 7u), y = ((b / g.BX) % g.BY) * 8 + ((t >> 3) & 7u), z = (b / (g.BX * g.BY)) * 8 + (t >> 6);
-      if (x < g.W && y < g.H && z < g.D) dst[(b << 9) | t] = sp[cur][q];
+#pragma unroll
+    for (int j = 0; j < kSmoothPer; ++j)
+      if (gid[j] != 0xffffffffu) dst[gid[j]] = sp[cur][tid + uint32_t(j) * 1024u];
+    // every other in-domain voxel of the listed bricks is 0
+    for (uint32_t q = tid; q < total; q += 1024) {
+      const uint32_t k = q >> 9, t = q & 511u;
+      if ((amask[k * 8 + (t >> 6)] >> (t & 63u)) & 1ull) continue;
+      const uint32_t b = list[k];
+      const uint32_t x = (b % g.BX) * 8 + (t & 7u), y = ((b / g.BX) % g.BY) * 8 + ((t >> 3) & 7u), z = (b / (g.BX * g.BY)) * 8 + (t >> 6);
+      if (x < g.W && y < g.H && z < g.D) dst[(b << 9) | t] = 0.f;
     }
     return;
   }

@@ -133,3 +133,89 @@ def random_state(shape, seed):
 
 def packed(sources):
     return pack_emitter_sources(sources), len(sources)
+
+
+def splitmix_uniform(index, seed):
+    """Vectorised SplitMix64 of (seed, index) -> uniform [0, 1), same mixing as SplitMix64."""
+    with np.errstate(over="ignore"):
+        z = (np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15)) + (index.astype(np.uint64) + np.uint64(1)) * np.uint64(
+            0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(40)).astype(np.float32) * np.float32(1.0 / (1 << 24))
+
+
+def brick_activity(density, speed=None, act_density=1e-4, act_speed=1e-3):
+    """Brick flags of the threshold rule (density bricks dilate by one brick, velocity-only
+    bricks mark themselves), from row-major (z, y, x) fields."""
+    d, h, w = density.shape
+    bz, by, bx = (d + 7) // 8, (h + 7) // 8, (w + 7) // 8
+    pad = np.zeros((bz * 8, by * 8, bx * 8), np.float32)
+    pad[:d, :h, :w] = density
+    dens = pad.reshape(bz, 8, by, 8, bx, 8).max(axis=(1, 3, 5)) > act_density
+    act = np.zeros_like(dens)
+    for oz in (-1, 0, 1):
+        for oy in (-1, 0, 1):
+            for ox in (-1, 0, 1):
+                src = dens[max(0, -oz):bz - max(0, oz), max(0, -oy):by - max(0, oy), max(0, -ox):bx - max(0, ox)]
+                act[max(0, oz):bz + min(0, oz), max(0, oy):by + min(0, oy), max(0, ox):bx + min(0, ox)] |= src
+    if speed is not None:
+        pad[:] = 0
+        pad[:d, :h, :w] = speed
+        act |= pad.reshape(bz, 8, by, 8, bx, 8).max(axis=(1, 3, 5)) > act_speed
+    return act
+
+
+def plume_fields(n, sources, top_radius, seed=2):
+    """Developed rising plumes above floor emitters: cones whose radius grows from the
+    emitter radius to `top_radius` voxels at the ceiling, Gaussian cross-section cut at
+    exp(-2), seeded +-20% density noise, updraft 10 voxels/s with +-0.75 lateral noise."""
+    y = (np.arange(n, dtype=np.float32) + 0.5)[None, :, None]
+    x = (np.arange(n, dtype=np.float32) + 0.5)[None, None, :]
+    z = (np.arange(n, dtype=np.float32) + 0.5)[:, None, None]
+    prof = np.zeros((n, n, n), np.float32)
+    for s in sources:
+        xe, ze, r0 = s.position[0] * n, s.position[2] * n, s.radius * n
+        r = r0 + (top_radius - r0) * (y / n) ** 0.8
+        q = ((x - xe) ** 2 + (z - ze) ** 2) / (r * r)
+        prof = np.maximum(prof, np.where(q < 1.0, np.exp(-2.0 * q), 0.0).astype(np.float32))
+    idx = np.arange(n ** 3, dtype=np.uint64).reshape(n, n, n)
+    height = 1.0 - 0.4 * (y / n)
+    st = {"density": 0.5 * prof * height * (0.8 + 0.4 * splitmix_uniform(idx, seed))}
+    st["temperature"] = np.clip(1.1 * prof * (1.0 - 0.6 * y / n), 0.0, 1.0)
+    st["fuel"] = 0.4 * prof * np.maximum(0.0, 1.0 - 2.0 * y / n)
+    st["velocity_y"] = 10.0 * prof
+    st["velocity_x"] = 1.5 * (splitmix_uniform(idx, seed + 1) - 0.5) * prof
+    st["velocity_z"] = 1.5 * (splitmix_uniform(idx, seed + 2) - 0.5) * prof
+    return {k: v.astype(np.float32) for k, v in st.items()}
+
+
+def plume_support(n, sources, top_radius):
+    """Voxels inside any plume cone of plume_fields (density > 0 there)."""
+    y = (np.arange(n, dtype=np.float32) + 0.5)[None, :, None]
+    x = (np.arange(n, dtype=np.float32) + 0.5)[None, None, :]
+    z = (np.arange(n, dtype=np.float32) + 0.5)[:, None, None]
+    inside = np.zeros((n, n, n), bool)
+    for s in sources:
+        xe, ze, r0 = s.position[0] * n, s.position[2] * n, s.radius * n
+        r = r0 + (top_radius - r0) * (y / n) ** 0.8
+        inside |= ((x - xe) ** 2 + (z - ze) ** 2) < r * r
+    return inside
+
+
+def c2_plume_state(n=256, sources=None, target=0.15, seed=2):
+    """C2 initial state: the rising fire plume developed to about `target` of the bricks
+    active (the config's ~15%), found by bisection on the plume's top radius."""
+    sources = sources if sources is not None else c2_sources(n)
+    lo, hi = sources[0].radius * n, float(n)
+    best = None
+    for _ in range(12):
+        mid = 0.5 * (lo + hi)
+        frac = float(brick_activity(plume_support(n, sources, mid).astype(np.float32)).mean())
+        best = (abs(frac - target), mid, frac) if best is None or abs(frac - target) < best[0] else best
+        if frac < target:
+            lo = mid
+        else:
+            hi = mid
+    return plume_fields(n, sources, best[1], seed), best[2]

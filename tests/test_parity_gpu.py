@@ -101,22 +101,26 @@ def test_semi_lagrangian_parity():
     compare_all(sim, orc, "semi-lagrangian")
 
 
-@pytest.mark.parametrize("solver,quality", [
-    (abi.FX_SOLVER_REFERENCE, {}),
-    (abi.FX_SOLVER_REFERENCE, {"finePreIterations": 3, "midPreIterations": 1, "coarseIterations": 7, "finePostIterations": 0}),
-    (abi.FX_SOLVER_VCYCLE, {}),
+@pytest.mark.parametrize("solver,quality,grid,nlev", [
+    (abi.FX_SOLVER_REFERENCE, {}, (32, 24, 16), 3),
+    (abi.FX_SOLVER_REFERENCE, {"finePreIterations": 3, "midPreIterations": 1, "coarseIterations": 7, "finePostIterations": 0},
+     (32, 24, 16), 3),
+    (abi.FX_SOLVER_VCYCLE, {}, (32, 24, 16), 3),
+    # coarsest level 20x16x16: 3528 active voxels staged in LDS, several slots per thread
+    (abi.FX_SOLVER_VCYCLE, {}, (160, 128, 128), 4),
+    # coarsest level 32^3: 27000 active voxels exceed the LDS staging, global sweeps
+    (abi.FX_SOLVER_VCYCLE, {}, (64, 64, 64), 2),
 ])
-def test_pressure_parity(solver, quality):
-    desc = make_desc((32, 24, 16), solver=solver, quality=quality, vcycle={"levels": 3, "cycles": 3})
+def test_pressure_parity(solver, quality, grid, nlev):
+    desc = make_desc(grid, solver=solver, quality=quality, vcycle={"levels": nlev, "cycles": 3})
     sim, orc = make_pair(desc)
-    st = scenes.random_state((16, 24, 32), seed=5)
+    st = scenes.random_state(grid[::-1], seed=5)
     sim.write_field("divergence", st["divergence"])
     orc.write("divergence", st["divergence"])
     p = scenes.sim_params()
     sim.run_pass("pressure", p)
     orc.run_pass("pressure", p)
     compare_all(sim, orc, "pressure", fields=["pressure", "divergence"])
-    nlev = 3
     # The reference schedule fixes which buffer holds what (quirks Q1-Q4); the
     # V-cycle's ping-pong buffers are an implementation detail, only the
     # per-level right-hand sides and the final pressure are defined.
@@ -186,6 +190,27 @@ def test_step_parity_sparse_config():
     bx = (n + 7) // 8
     mask = flags.reshape(bx, bx, bx).repeat(8, 0).repeat(8, 1).repeat(8, 2)[:n, :n, :n]
     assert np.all(d[mask == 0] == 0)
+
+
+def test_step_parity_sparse_plume():
+    """Sparse config from a developed plume (~15% of bricks active, the benchmark's start)."""
+    n = 64
+    srcs = scenes.c2_sources(n, radius_vox=2.5)
+    st, frac = scenes.c2_plume_state(n, srcs, target=0.15)
+    assert 0.1 < frac < 0.2
+    desc = scenes.c2_desc(n)
+    sim, orc = make_pair(desc, srcs)
+    load_state(sim, orc, st)
+    dt = 1.0 / 60.0
+    for k in range(10):
+        if k > 0:
+            for name in PERSISTENT:
+                sim.write_field(name, orc.read(name))
+        sim.step(None, k * dt, dt)
+        orc.step(k * dt, dt)
+        np.testing.assert_array_equal(sim.read_flags(), orc.flags(), err_msg=f"flags differ at step {k}")
+        compare_all(sim, orc, f"plume step {k}", fields=list(PERSISTENT) + ["pressure", "divergence"])
+    assert sim.active_brick_count() == orc.active_count() > 0.1 * sim.brick_count
 
 
 def test_raymarch_parity():
