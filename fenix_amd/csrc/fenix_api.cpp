@@ -33,6 +33,7 @@ struct Level {
   float* p[2] = {nullptr, nullptr};  // pressureA, pressureB
   uint32_t* flags = nullptr;         // sparse: level activity flags
   BrickList L{};                     // sparse: active list; dense: identity
+  uint64_t* bits = nullptr;          // voxel activity words, 8 per listed brick (V-cycle)
 };
 
 struct StageSlot {
@@ -94,6 +95,10 @@ struct fx_sim {
   LevelFlags lflags{};               // sparse: coarse flags marked by activation (n = 0 when dense)
   uint32_t* d_compact = nullptr;     // per-block counts and offsets
   uint32_t* d_bits = nullptr;        // packed render flags for the raymarch
+
+  // V-cycle kernel plan per level (bit l set: split legs of lean one-sweep
+  // kernels instead of the fused LDS legs). Part of the graph key.
+  uint32_t split_down = 0, split_up = 0;
 
   // hipGraph of the step body (rebuilt when the recorded sequence changes).
   bool use_graph = true;
@@ -171,6 +176,7 @@ LvMask mask_of(const fx_sim* s, int l) {
   if (!s->sparse) return LvMask{nullptr, 0, 0, 0, 0};
   return LvMask{s->flagsInj, s->g.BX, s->g.BY, s->g.BZ, uint32_t(l)};
 }
+LvBits bits_of(const fx_sim* s, int l) { return LvBits{s->lv[l].bits, s->sparse ? s->lv[l].L.page : nullptr}; }
 
 struct Buf { void* p; int esz; };
 std::vector<Buf> fine_buffers(fx_sim* s) {
@@ -385,6 +391,18 @@ int solve_vcycle(fx_sim* s) {
   const int L = int(s->lv.size());
   const fx_vcycle& v = s->desc.vcycle;
   std::vector<int> cur(L, 0);
+  {
+    LevelMaskArgs A{};
+    A.nlevels = L;
+    for (int l = 0; l < L; ++l) {
+      A.list[l] = level_list(s, l);
+      A.count[l] = level_count(s, l);
+      A.g[l] = s->lv[l].g;
+      A.m[l] = mask_of(s, l);
+      A.bits[l] = s->lv[l].bits;
+    }
+    launch_level_masks(s->stream, grid_for(s, s->lv[0].nb), A);
+  }
   // n sweeps at level l; zero_start: the first sweep starts from p = 0 (a
   // coarse level's correction), so neither a clear nor a read of p is needed.
   auto smoothW = [&](int l, int n, bool zero_start) {
@@ -395,51 +413,63 @@ int solve_vcycle(fx_sim* s) {
     }
     if (zero_start) cur[l] = 0;
     if (l > 0 && l == L - 1 && V.nv <= kBlockSmoothVoxels) {
-      launch_smooth_block(s->stream, level_list(s, l), level_count(s, l), V.g, mask_of(s, l), s->d_const, V.div, V.p[cur[l]],
-                          V.p[1 - cur[l]], n, 1, zero_start ? 1 : 0);
+      launch_smooth_block(s->stream, level_list(s, l), level_count(s, l), V.g, mask_of(s, l), bits_of(s, l), s->d_const, V.div,
+                          V.p[cur[l]], V.p[1 - cur[l]], n, 1, zero_start ? 1 : 0);
       if (n & 1) cur[l] = 1 - cur[l];
       return;
     }
     for (int k = 0; k < n; ++k) {
       StageTimer timer(s, l == 0 ? FX_TIME_JACOBI_FINE : -1);
       const float* src = (zero_start && k == 0) ? nullptr : V.p[cur[l]];
-      launch_jacobi(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, mask_of(s, l), s->d_const, V.div, src,
-                    V.p[1 - cur[l]], 1);
+      launch_sweep(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, bits_of(s, l), s->d_const, V.div, src,
+                   V.p[1 - cur[l]]);
       cur[l] = 1 - cur[l];
     }
   };
   int cycles = std::min(v.cycles, s->norm_cap);
-  if (cycles > 0) launch_clear_u32(s->stream, reinterpret_cast<uint32_t*>(s->d_norms), size_t(cycles) * 2, 0u);
+  if (cycles > 0) launch_clear_u32(s->stream, reinterpret_cast<uint32_t*>(s->d_norms), size_t(cycles) * kNormLanes * 2, 0u);
   s->norm_count = cycles;
   for (int c = 0; c < v.cycles; ++c) {
     for (int l = 0; l < L - 1; ++l) {
       Level& V = s->lv[l];
       Level& W = s->lv[l + 1];
-      double* norm = (l == 0 && c < s->norm_cap) ? s->d_norms + c : nullptr;
+      double* norm = (l == 0 && c < s->norm_cap) ? s->d_norms + size_t(c) * kNormLanes : nullptr;
       // Level 0 starts each solve from p = 0; coarse levels start every cycle from 0.
       const bool zero = l > 0 || c == 0;
       if (zero) cur[l] = 0;
-      if (launch_vdown(s->stream, grid_for(s, V.nb), v.pre_sweeps, level_list(s, l), level_count(s, l), V.g, mask_of(s, l), W.g,
-                       mask_of(s, l + 1), s->d_const, V.div, zero ? nullptr : V.p[cur[l]], V.p[1 - cur[l]], W.div, norm)) {
+      const bool split = (s->split_down >> l) & 1u;
+      if (!split && launch_vdown(s->stream, grid_for(s, V.nb), v.pre_sweeps, level_list(s, l), level_count(s, l), V.g,
+                                 bits_of(s, l), W.g, mask_of(s, l + 1), s->d_const, V.div, zero ? nullptr : V.p[cur[l]],
+                                 V.p[1 - cur[l]], W.div, norm)) {
         cur[l] = 1 - cur[l];
         continue;
       }
+      // split leg: lean sweeps, residual into the other pressure buffer, restriction
       smoothW(l, v.pre_sweeps, zero);
-      launch_residual(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, mask_of(s, l), V.div, V.p[cur[l]],
-                      V.p[1 - cur[l]], norm);
-      launch_restrict(s->stream, grid_for(s, W.nb), level_list(s, l + 1), level_count(s, l + 1), V.g, W.g, mask_of(s, l + 1),
-                      V.p[1 - cur[l]], W.div, 1);
+      if (zero && v.pre_sweeps == 0) zero_level_p(s, l, cur[l]);
+      launch_residual_lean(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, bits_of(s, l), V.div,
+                           V.p[cur[l]], V.p[1 - cur[l]], norm);
+      launch_restrict_lean(s->stream, grid_for(s, W.nb), level_list(s, l + 1), level_count(s, l + 1), V.g, W.g, bits_of(s, l + 1),
+                           V.p[1 - cur[l]], W.div);
     }
     smoothW(L - 1, v.coarse_sweeps, true);
     for (int l = L - 2; l >= 0; --l) {
       Level& V = s->lv[l];
-      if (launch_vup(s->stream, grid_for(s, V.nb), v.post_sweeps, level_list(s, l), level_count(s, l), s->lv[l + 1].g, V.g,
-                     mask_of(s, l), s->d_const, s->lv[l + 1].p[cur[l + 1]], V.div, V.p[cur[l]], V.p[1 - cur[l]])) {
+      const uint32_t* lst = level_list(s, l);
+      const uint32_t* cnt = level_count(s, l);
+      const float* e = s->lv[l + 1].p[cur[l + 1]];
+      if ((s->split_up >> l) & 1u) {
+        // split leg: in-place prolongation, then lean sweeps
+        launch_prolong_add(s->stream, grid_for(s, V.nb), lst, cnt, s->lv[l + 1].g, V.g, bits_of(s, l), e, V.p[cur[l]]);
+        smoothW(l, v.post_sweeps, false);
+        continue;
+      }
+      if (launch_vup(s->stream, grid_for(s, V.nb), v.post_sweeps, lst, cnt, s->lv[l + 1].g, V.g, bits_of(s, l), s->d_const, e,
+                     V.div, V.p[cur[l]], V.p[1 - cur[l]])) {
         cur[l] = 1 - cur[l];
         continue;
       }
-      launch_prolong(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), s->lv[l + 1].g, V.g, mask_of(s, l),
-                     s->lv[l + 1].p[cur[l + 1]], V.p[cur[l]]);
+      launch_prolong(s->stream, grid_for(s, V.nb), lst, cnt, s->lv[l + 1].g, V.g, mask_of(s, l), e, V.p[cur[l]]);
       smoothW(l, v.post_sweeps, false);
     }
   }
@@ -758,6 +788,7 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
     if (rc == FX_OK) rc = dalloc(s, &L.L.retire, L.nb);
     if (rc == FX_OK) rc = dalloc(s, &L.L.retire_count, 1);
     if (rc == FX_OK) rc = l == 0 ? FX_OK : dalloc(s, &L.flags, L.nb);
+    if (rc == FX_OK && desc->solver == FX_SOLVER_VCYCLE) rc = dalloc(s, &L.bits, L.nb * 8);
     if (l == 0) L.flags = s->flagsInj;
     s->lv.push_back(L);
     lg = make_grid(halve(lg.W), halve(lg.H), halve(lg.D));
@@ -788,9 +819,13 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
   {
     const char* g = std::getenv("FENIX_GRAPH");
     s->use_graph = !(g && g[0] == '0');
+    const char* d = std::getenv("FENIX_VSPLIT_DOWN");
+    const char* u = std::getenv("FENIX_VSPLIT_UP");
+    if (d) s->split_down = uint32_t(std::strtoul(d, nullptr, 0));
+    if (u) s->split_up = uint32_t(std::strtoul(u, nullptr, 0));
   }
   s->norm_cap = std::max(1, desc->vcycle.cycles);
-  if (dalloc(s, &s->d_norms, size_t(s->norm_cap)) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
+  if (dalloc(s, &s->d_norms, size_t(s->norm_cap) * kNormLanes) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
   s->src_cap = FX_FLOATS_PER_EMITTER;
   if (dalloc(s, &s->d_src, s->src_cap) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
   for (auto& slot : s->slots) {
@@ -840,7 +875,7 @@ void fx_destroy(fx_sim* s) {
   F(s->touched); F(s->d_bits); F(s->identity); F(s->identity_count); F(s->d_const); F(s->d_cand); F(s->d_src); F(s->d_norms);
   for (size_t l = 0; l < s->lv.size(); ++l) {
     Level& L = s->lv[l];
-    F(L.div); F(L.p[0]); F(L.p[1]); F(L.L.list); F(L.L.count); F(L.L.page); F(L.L.retire); F(L.L.retire_count);
+    F(L.div); F(L.p[0]); F(L.p[1]); F(L.bits); F(L.L.list); F(L.L.count); F(L.L.page); F(L.L.retire); F(L.L.retire_count);
     if (l > 0) F(L.flags);
   }
   for (auto& slot : s->slots) {
@@ -904,7 +939,7 @@ int fx_step(fx_sim* s, double elapsed_s, double step_s) {  // ORCH:230-305
   if (s->use_graph && !s->timing) {
     std::vector<int64_t> key = {vort, pres, dbg, s->mode, pres ? -1 : s->lastPressure, s->q.fine_pre_iterations,
                                 s->q.fine_post_iterations, s->q.mid_pre_iterations, s->q.mid_post_iterations,
-                                s->q.coarse_iterations};
+                                s->q.coarse_iterations, s->split_down, s->split_up};
     if (!s->graph_exec || key != s->graph_key) {
       rc = capture_step(s, vort, pres, dbg);
       if (rc != FX_OK) return rc;
@@ -1137,9 +1172,13 @@ int fx_solver_residuals(fx_sim* s, double* host, size_t capacity, uint32_t* coun
   uint32_t n = s->desc.solver == FX_SOLVER_VCYCLE ? uint32_t(s->norm_count) : 0u;
   *count = n;
   if (n && host && capacity) {
-    std::vector<double> t(n);
-    HIP_OK(s, hipMemcpy(t.data(), s->d_norms, n * sizeof(double), hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < std::min<size_t>(n, capacity); ++i) host[i] = std::sqrt(t[i]);
+    std::vector<double> t(size_t(n) * kNormLanes);
+    HIP_OK(s, hipMemcpy(t.data(), s->d_norms, t.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < std::min<size_t>(n, capacity); ++i) {
+      double sum = 0.0;
+      for (int k = 0; k < kNormLanes; ++k) sum += t[i * kNormLanes + k];
+      host[i] = std::sqrt(sum);
+    }
   }
   return FX_OK;
 }

@@ -8,6 +8,9 @@
 #include "../../include/fenix.h"
 
 #define FX_MAX_LEVELS_DEV 8
+// Residual norms: each cycle's FP64 sum of squares is spread over this many
+// accumulators (per-block atomics on one address serialise); the host adds them.
+constexpr int kNormLanes = 32;
 #define FX_MAX_ZERO_BUFS 24
 
 namespace fx {
@@ -55,6 +58,23 @@ struct LvMask {
   const uint32_t* flags;  // fine-level brick flags
   uint32_t FBX, FBY, FBZ;
   uint32_t shift;         // level index
+};
+
+// Voxel activity bits of one pressure level, built once per solve: 8 words per
+// listed brick (one per z-slice, bit x + 8y), indexed by list position. A voxel's
+// bit is set iff it lies in the domain, off the boundary faces, and is active
+// under the level's LvMask. page == nullptr: list position = brick index (dense).
+struct LvBits {
+  const uint64_t* bits;
+  const int32_t* page;
+};
+struct LevelMaskArgs {
+  int nlevels;
+  const uint32_t* list[FX_MAX_LEVELS_DEV];
+  const uint32_t* count[FX_MAX_LEVELS_DEV];
+  Grid g[FX_MAX_LEVELS_DEV];
+  LvMask m[FX_MAX_LEVELS_DEV];
+  uint64_t* bits[FX_MAX_LEVELS_DEV];
 };
 
 // Launch helpers implemented in fenix_kernels.hip. All are asynchronous on `s`.
@@ -106,13 +126,28 @@ struct CompactArgs {
   uint32_t* offsets;  // 2 per block
 };
 void launch_compact(hipStream_t s, const CompactArgs& A);
+void launch_level_masks(hipStream_t s, int blocks, const LevelMaskArgs& a);
 // Fused V-cycle legs (LDS temporal blocking); return false for unsupported sweep counts.
-bool launch_vdown(hipStream_t s, int blocks, int pre, const uint32_t* list, const uint32_t* count, Grid f, LvMask mf, Grid c,
+// vdown: p_out == nullptr skips the write of the (unchanged, pre == 0) pressure.
+bool launch_vdown(hipStream_t s, int blocks, int pre, const uint32_t* list, const uint32_t* count, Grid f, LvBits bf, Grid c,
                   LvMask mc, const StepConst* cst, const float* rhs, const float* p_in, float* p_out, float* crhs, double* norm2);
-bool launch_vup(hipStream_t s, int blocks, int post, const uint32_t* list, const uint32_t* count, Grid c, Grid f, LvMask mf,
+bool launch_vup(hipStream_t s, int blocks, int post, const uint32_t* list, const uint32_t* count, Grid c, Grid f, LvBits bf,
                 const StepConst* cst, const float* e, const float* rhs, const float* p_in, float* p_out);
-void launch_smooth_block(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const StepConst* c,
-                         const float* rhs, float* p0, float* p1, int sweeps, int weighted, int zero_start);
+// Residual of a level into r (0 on inactive voxels), FP64 norm when norm2 != nullptr.
+void launch_residual_lean(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b,
+                          const float* rhs, const float* p, float* r, double* norm2);
+// Full-weighting restriction of r into the coarse right-hand side over the coarse level's bricks.
+void launch_restrict_lean(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid f, Grid c, LvBits cb,
+                          const float* r, float* crhs);
+// p += prolongation of the coarse correction e, in place, on the level's active voxels.
+void launch_prolong_add(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid c, Grid f, LvBits b,
+                        const float* e, float* p);
+// One weighted-Jacobi sweep of a V-cycle level; src == nullptr starts from p = 0.
+void launch_sweep(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
+                  const float* rhs, const float* src, float* dst);
+// bits == nullptr in `b`: voxel activity is evaluated from `m` inside the kernel.
+void launch_smooth_block(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, LvBits b,
+                         const StepConst* c, const float* rhs, float* p0, float* p1, int sweeps, int weighted, int zero_start);
 void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
                             const StepConst* c, Vec3 u, Vec3 us);
 void launch_advect_scalars(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,

@@ -844,10 +844,10 @@ __device__ __forceinline__ uint32_t rank_in_brick(const uint64_t* am, uint32_t k
 }
 
 __global__ __launch_bounds__(1024) void k_smooth_block(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                       LvMask m, const StepConst* __restrict__ C, const float* __restrict__ rhs,
-                                                       float* p0, float* p1, int sweeps, int weighted, int zero_start) {
-  __shared__ float sp[2][kSmoothCap];
-  __shared__ float sr[kSmoothCap];
+                                                       LvMask m, LvBits lb, const StepConst* __restrict__ C,
+                                                       const float* __restrict__ rhs, float* p0, float* p1, int sweeps, int weighted,
+                                                       int zero_start) {
+  __shared__ float sp[2][kSmoothCap + 1];  // slot kSmoothCap holds 0: the missing neighbour
   __shared__ uint64_t amask[kSmoothBricks * 8];
   __shared__ uint32_t base[kSmoothBricks];
   __shared__ int16_t slot[kSmoothTable];
@@ -862,7 +862,9 @@ __global__ __launch_bounds__(1024) void k_smooth_block(const uint32_t* __restric
     __syncthreads();
     for (uint32_t k = tid; k < n; k += 1024) slot[list[k]] = int16_t(k);
     // one ballot per (brick, z-slice): bit = x + 8y of an active interior voxel
-    for (uint32_t wd = wave; wd < n * 8; wd += 16) {
+    if (lb.bits) {
+      for (uint32_t wd = tid; wd < n * 8; wd += 1024) amask[wd] = lb.bits[wd];
+    } else for (uint32_t wd = wave; wd < n * 8; wd += 16) {
       const uint32_t b = list[wd >> 3], lz = wd & 7u;
       const uint32_t x = (b % g.BX) * 8 + (lane & 7u), y = ((b / g.BX) % g.BY) * 8 + (lane >> 3),
                      z = (b / (g.BX * g.BY)) * 8 + lz;
@@ -889,16 +891,7 @@ __global__ __launch_bounds__(1024) void k_smooth_block(const uint32_t* __restric
     __syncthreads();
   }
   if (n <= kSmoothBricks && nbl <= kSmoothTable && nact <= kSmoothCap) {
-    // stage the right-hand side and the start value of every active voxel
-    for (uint32_t q = tid; q < total; q += 1024) {
-      const uint32_t k = q >> 9, t = q & 511u;
-      if (!((amask[k * 8 + (t >> 6)] >> (t & 63u)) & 1ull)) continue;
-      const uint32_t sl = base[k] + rank_in_brick(amask, k, t);
-      const uint32_t gi = (list[k] << 9) | t;
-      sr[sl] = rhs[gi];
-      sp[0][sl] = zero_start ? 0.f : p0[gi];
-    }
-    // owned slots: global index and neighbour slots (-1: the neighbour holds 0)
+    // owned slots: global index and neighbour slots (kSmoothCap: the neighbour holds 0)
     int nbr[kSmoothPer][6];
     uint32_t gid[kSmoothPer];
 #pragma unroll
@@ -926,12 +919,25 @@ __global__ __launch_bounds__(1024) void k_smooth_block(const uint32_t* __restric
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
         const int sl = slot[(nx[k] >> 3) + g.BX * ((ny[k] >> 3) + g.BY * (nz[k] >> 3))];
-        nbr[j][k] = -1;
+        nbr[j][k] = int(kSmoothCap);
         if (sl < 0) continue;
         const uint32_t tn = ((nz[k] & 7u) << 6) | ((ny[k] & 7u) << 3) | (nx[k] & 7u);
         if ((amask[uint32_t(sl) * 8 + (tn >> 6)] >> (tn & 63u)) & 1ull) nbr[j][k] = int(base[sl] + rank_in_brick(amask, uint32_t(sl), tn));
       }
     }
+    // the right-hand side and the start value of every owned slot (all loads in
+    // flight together) stay in registers; LDS carries the values neighbours read
+    float rv[kSmoothPer], pv[kSmoothPer];
+#pragma unroll
+    for (int j = 0; j < kSmoothPer; ++j) {
+      const bool own = gid[j] != 0xffffffffu;
+      rv[j] = own ? rhs[gid[j]] : 0.f;
+      pv[j] = (own && !zero_start) ? p0[gid[j]] : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < kSmoothPer; ++j)
+      if (gid[j] != 0xffffffffu) sp[0][tid + uint32_t(j) * 1024u] = pv[j];
+    if (tid == 0) { sp[0][kSmoothCap] = 0.f; sp[1][kSmoothCap] = 0.f; }
     __syncthreads();
     int cur = 0;
     for (int sw = 0; sw < sweeps; ++sw) {
@@ -939,14 +945,11 @@ __global__ __launch_bounds__(1024) void k_smooth_block(const uint32_t* __restric
       float* B = sp[1 - cur];
 #pragma unroll
       for (int j = 0; j < kSmoothPer; ++j) {
-        const uint32_t q = tid + uint32_t(j) * 1024u;
-        if (q >= nact) break;
-        float v[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) v[k] = nbr[j][k] < 0 ? 0.f : A[nbr[j][k]];
-        const float sum = v[0] + v[1] + v[2] + v[3] + v[4] + v[5];
-        const float pj = (sum - sr[q]) / 6.f;
-        B[q] = weighted ? mixf(A[q], pj, om) : pj;
+        if (gid[j] == 0xffffffffu) continue;
+        const float sum = A[nbr[j][0]] + A[nbr[j][1]] + A[nbr[j][2]] + A[nbr[j][3]] + A[nbr[j][4]] + A[nbr[j][5]];
+        const float pj = (sum - rv[j]) / 6.f;
+        pv[j] = weighted ? mixf(pv[j], pj, om) : pj;
+        B[tid + uint32_t(j) * 1024u] = pv[j];
       }
       __syncthreads();
       cur = 1 - cur;
@@ -954,7 +957,7 @@ __global__ __launch_bounds__(1024) void k_smooth_block(const uint32_t* __restric
     float* dst = (sweeps & 1) ? p1 : p0;
 #pragma unroll
     for (int j = 0; j < kSmoothPer; ++j)
-      if (gid[j] != 0xffffffffu) dst[gid[j]] = sp[cur][tid + uint32_t(j) * 1024u];
+      if (gid[j] != 0xffffffffu) dst[gid[j]] = pv[j];
     // every other in-domain voxel of the listed bricks is 0
     for (uint32_t q = tid; q < total; q += 1024) {
       const uint32_t k = q >> 9, t = q & 511u;
@@ -1004,7 +1007,7 @@ __global__ __launch_bounds__(512) void k_residual(const uint32_t* __restrict__ l
   if (threadIdx.x == 0) {
     double s = 0.0;
     for (int k = 0; k < 8; ++k) s += part[k];
-    atomicAdd(norm2, s);
+    atomicAdd(norm2 + (blockIdx.x % kNormLanes), s);
   }
 }
 // SH/pressure-restriction.wgsl.ts:27-50 (8-child mean) or (1,3,3,1)/8 x 4 (V-cycle).
@@ -1048,125 +1051,231 @@ __global__ __launch_bounds__(512) void k_prolong(const uint32_t* __restrict__ li
     dst[v.i] += tri_eval(t, src);
   }
 }
+// ------------------------------------------------ level activity bits --
+// One ballot per (listed brick, z-slice) of every level: bit x + 8y is set iff
+// the voxel is in the domain, off the boundary faces and active under the
+// level's mask. Built once per solve; every V-cycle kernel reads these bits.
+__global__ __launch_bounds__(512) void k_level_masks(LevelMaskArgs a) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t start[FX_MAX_LEVELS_DEV + 1];
+  start[0] = 0;
+  for (int l = 0; l < a.nlevels; ++l) start[l + 1] = start[l] + *a.count[l];
+  const uint32_t total = start[a.nlevels];
+  // one grid-stride loop over the bricks of all levels, 4 bricks per block and
+  // round so that their list and flag loads overlap
+  constexpr int U = 4;
+  for (uint32_t e0 = blockIdx.x; e0 < total; e0 += gridDim.x * U) {
+    int lv[U];
+    uint32_t kk[U], bb[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t e = e0 + uint32_t(u) * gridDim.x;
+      lv[u] = -1;
+      kk[u] = 0;
+      bb[u] = 0;
+      if (e < total) {
+        int l = 0;
+        while (e >= start[l + 1]) ++l;
+        lv[u] = l;
+        kk[u] = e - start[l];
+        bb[u] = a.list[l][kk[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (lv[u] < 0) continue;
+      const Grid& g = a.g[lv[u]];
+      const uint32_t b = bb[u];
+      const uint32_t x = (b % g.BX) * 8 + (lane & 7u), y = ((b / g.BX) % g.BY) * 8 + (lane >> 3),
+                     z = (b / (g.BX * g.BY)) * 8 + wave;
+      const bool act = x < g.W && y < g.H && z < g.D && !on_boundary(g, x, y, z) && lv_active(a.m[lv[u]], x, y, z);
+      const uint64_t w = __ballot(act);
+      if (lane == 0) a.bits[lv[u]][size_t(kk[u]) * 8 + wave] = w;
+    }
+  }
+}
+
+// Activity words of the 3x3x3 bricks around brick (bx, by, bz): nb[c * 8 + z],
+// c = (dx+1) + 3 (dy+1) + 9 (dz+1); bricks outside the domain or unlisted are 0.
+__device__ __forceinline__ void nbhd_load(const Grid& g, const LvBits& lb, int bx, int by, int bz, uint64_t* nb) {
+  for (int q = threadIdx.x; q < 216; q += blockDim.x) {
+    const int c = q >> 3, w = q & 7;
+    const int nx = bx + c % 3 - 1, ny = by + (c / 3) % 3 - 1, nz = bz + c / 9 - 1;
+    uint64_t v = 0;
+    if (nx >= 0 && ny >= 0 && nz >= 0 && nx < int(g.BX) && ny < int(g.BY) && nz < int(g.BZ)) {
+      const uint32_t nbk = uint32_t(nx) + g.BX * (uint32_t(ny) + g.BY * uint32_t(nz));
+      const int pos = lb.page ? lb.page[nbk] : int(nbk);
+      if (pos >= 0) v = lb.bits[size_t(pos) * 8 + w];
+    }
+    nb[q] = v;
+  }
+}
+
 // ------------------------------------------------ fused V-cycle kernels --
 // Temporal blocking in LDS: one 512-thread workgroup per brick of the level.
 // Tiles share one origin (brick origin - R0) and side S = 8 + 2*R0; stage s of
 // a chain computes on the brick +- (R0 - s). Halo voxels are recomputed with the
 // same arithmetic their own brick uses, so results equal the unfused sweeps.
+// Inactive voxels (and everything outside the domain) hold 0 in every tile.
 template <int R0>
 struct Tile {
+  static_assert(R0 <= 8, "tiles reach at most one brick beyond their own");
   static constexpr int S = 8 + 2 * R0;
   static constexpr int N = S * S * S;
 };
-
-// Loads f (level grid g) on brick +- R0 into t; 0 outside the domain.
+// Activity of tile voxel (tx, ty, tz) from the neighbourhood words.
 template <int R0>
-__device__ __forceinline__ void tile_load(const Grid& g, const float* __restrict__ f, int ox, int oy, int oz, float* t) {
+__device__ __forceinline__ bool tile_on(const uint64_t* nb, int tx, int ty, int tz) {
+  const int lx = tx - R0 + 8, ly = ty - R0 + 8, lz = tz - R0 + 8;
+  return (nb[((lx >> 3) + 3 * ((ly >> 3) + 3 * (lz >> 3))) * 8 + (lz & 7)] >> ((lx & 7) | ((ly & 7) << 3))) & 1ull;
+}
+// Tile loads in two phases so that every global load of a tile (and the
+// neighbourhood words) is in flight at once: tile_fetch reads f on brick +- R0
+// into registers (0 outside the domain); tile_store writes them to LDS after the
+// neighbourhood words have landed, keeping active voxels only (inactive voxels
+// hold 0 in memory as well, by the sparse invariant).
+template <int R0>
+struct TileRegs {
+  static constexpr int IT = (Tile<R0>::N + 511) / 512;
+  float v[IT];
+};
+template <int R0>
+__device__ __forceinline__ void tile_fetch(const Grid& g, const float* __restrict__ f, int ox, int oy, int oz, TileRegs<R0>& r) {
   constexpr int S = Tile<R0>::S, N = Tile<R0>::N;
-  for (int q = threadIdx.x; q < N; q += 512) {
-    int x = ox + q % S, y = oy + (q / S) % S, z = oz + q / (S * S);
-    t[q] = (x >= 0 && y >= 0 && z >= 0 && x < int(g.W) && y < int(g.H) && z < int(g.D)) ? f[vidx(g, x, y, z)] : 0.f;
+#pragma unroll
+  for (int j = 0; j < TileRegs<R0>::IT; ++j) {
+    const int q = int(threadIdx.x) + j * 512;
+    const int x = ox + q % S, y = oy + (q / S) % S, z = oz + q / (S * S);
+    r.v[j] = (q < N && x >= 0 && y >= 0 && z >= 0 && x < int(g.W) && y < int(g.H) && z < int(g.D))
+                 ? f[vidx(g, uint32_t(x), uint32_t(y), uint32_t(z))]
+                 : 0.f;
+  }
+}
+template <int R0>
+__device__ __forceinline__ void tile_store(const uint64_t* nb, const TileRegs<R0>& r, float* t) {
+  constexpr int S = Tile<R0>::S, N = Tile<R0>::N;
+#pragma unroll
+  for (int j = 0; j < TileRegs<R0>::IT; ++j) {
+    const int q = int(threadIdx.x) + j * 512;
+    if (q < N) t[q] = tile_on<R0>(nb, q % S, (q / S) % S, q / (S * S)) ? r.v[j] : 0.f;
   }
 }
 template <int R0>
 __device__ __forceinline__ void tile_zero(float* t) {
   for (int q = threadIdx.x; q < Tile<R0>::N; q += 512) t[q] = 0.f;
 }
-// One weighted-Jacobi sweep A -> B on brick +- r (tiles of radius R0).
-template <int R0>
-__device__ __forceinline__ void tile_sweep(const Grid& g, const LvMask& m, int ox, int oy, int oz, int r, const float* A,
-                                           const float* rhs, float* B, float om) {
-  constexpr int S = Tile<R0>::S;
-  const int side = 8 + 2 * r, n = side * side * side, off = R0 - r;
+// One weighted-Jacobi sweep A -> B on brick +- R (tiles of radius R0).
+template <int R0, int R>
+__device__ __forceinline__ void tile_sweep(const uint64_t* nb, const float* A, const float* rhs, float* B, float om) {
+  constexpr int S = Tile<R0>::S, side = 8 + 2 * R, n = side * side * side, off = R0 - R;
   for (int q = threadIdx.x; q < n; q += 512) {
-    int tx = off + q % side, ty = off + (q / side) % side, tz = off + q / (side * side);
-    int i = tx + S * (ty + S * tz);
-    int x = ox + tx, y = oy + ty, z = oz + tz;
-    if (x < 0 || y < 0 || z < 0 || x >= int(g.W) || y >= int(g.H) || z >= int(g.D) ||
-        on_boundary(g, x, y, z) || !lv_active(m, x, y, z)) { B[i] = 0.f; continue; }
-    float sum = A[i + 1] + A[i - 1] + A[i + S] + A[i - S] + A[i + S * S] + A[i - S * S];
-    float pj = (sum - rhs[i]) / 6.f;
+    const int tx = off + q % side, ty = off + (q / side) % side, tz = off + q / (side * side);
+    const int i = tx + S * (ty + S * tz);
+    if (!tile_on<R0>(nb, tx, ty, tz)) { B[i] = 0.f; continue; }
+    const float sum = A[i + 1] + A[i - 1] + A[i + S] + A[i - S] + A[i + S * S] + A[i - S * S];
+    const float pj = (sum - rhs[i]) / 6.f;
     B[i] = mixf(A[i], pj, om);
   }
+}
+__device__ __forceinline__ void brick_coords(const Grid& g, uint32_t b, int& bx, int& by, int& bz) {
+  bx = int(b % g.BX); by = int((b / g.BX) % g.BY); bz = int(b / (g.BX * g.BY));
 }
 
 // Pre-smoothing (PRE sweeps), residual, FP64 norm and (1,3,3,1)/8 x 4 restriction.
 // p_in == nullptr: start from p = 0. Writes the smoothed p of the brick to p_out
-// and the coarse right-hand side of the 4^3 coarse voxels the brick covers.
+// (skipped when p_out == nullptr) and the coarse right-hand side of the 4^3
+// coarse voxels the brick covers.
 template <int PRE>
 __global__ __launch_bounds__(512) void k_vdown(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid f,
-                                               LvMask mf, Grid c, LvMask mc, const StepConst* __restrict__ C,
+                                               LvBits bf, Grid c, LvMask mc, const StepConst* __restrict__ C,
                                                const float* __restrict__ rhs, const float* __restrict__ p_in,
                                                float* __restrict__ p_out, float* __restrict__ crhs, double* norm2) {
   constexpr int R0 = PRE + 2, S = Tile<R0>::S, N = Tile<R0>::N;
   __shared__ float ta[N], tb[N], tr[N];
+  __shared__ uint64_t nb[216];
   __shared__ double part[8];
   const float om = C->omega;
   double acc = 0.0;
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
-    const int bx = int(b % f.BX), by = int((b / f.BX) % f.BY), bz = int(b / (f.BX * f.BY));
+    int bx, by, bz;
+    brick_coords(f, b, bx, by, bz);
     const int ox = bx * 8 - R0, oy = by * 8 - R0, oz = bz * 8 - R0;
+    // activity of the coarse voxel this thread restricts (threads 0..63)
+    bool cact = false;
+    if (threadIdx.x < 64) {
+      const int t = threadIdx.x;
+      const uint32_t cx = uint32_t(bx * 4 + (t & 3)), cy = uint32_t(by * 4 + ((t >> 2) & 3)), cz = uint32_t(bz * 4 + (t >> 4));
+      cact = cx < c.W && cy < c.H && cz < c.D && !on_boundary(c, cx, cy, cz) && lv_active(mc, cx, cy, cz);
+    }
+    TileRegs<R0> rr, rp;
+    tile_fetch<R0>(f, rhs, ox, oy, oz, rr);
+    if (p_in) tile_fetch<R0>(f, p_in, ox, oy, oz, rp);
     __syncthreads();  // previous brick's tiles are no longer read
-    tile_load<R0>(f, rhs, ox, oy, oz, tr);
-    if (p_in) tile_load<R0>(f, p_in, ox, oy, oz, ta); else tile_zero<R0>(ta);
+    nbhd_load(f, bf, bx, by, bz, nb);
+    __syncthreads();
+    tile_store<R0>(nb, rr, tr);
+    if (p_in) tile_store<R0>(nb, rp, ta); else tile_zero<R0>(ta);
     __syncthreads();
     float* A = ta;
     float* B = tb;
-    for (int sw = 1; sw <= PRE; ++sw) {
-      tile_sweep<R0>(f, mf, ox, oy, oz, R0 - sw, A, tr, B, om);
+    if constexpr (PRE >= 1) {
+      tile_sweep<R0, R0 - 1>(nb, A, tr, B, om);
+      __syncthreads();
+      float* t = A; A = B; B = t;
+    }
+    if constexpr (PRE >= 2) {
+      tile_sweep<R0, R0 - 2>(nb, A, tr, B, om);
       __syncthreads();
       float* t = A; A = B; B = t;
     }
     // residual on brick +- 1 into B
     {
-      const int side = 10, n = 1000, off = R0 - 1;
+      constexpr int side = 10, n = 1000, off = R0 - 1;
       for (int q = threadIdx.x; q < n; q += 512) {
-        int tx = off + q % side, ty = off + (q / side) % side, tz = off + q / (side * side);
-        int i = tx + S * (ty + S * tz);
-        int x = ox + tx, y = oy + ty, z = oz + tz;
+        const int tx = off + q % side, ty = off + (q / side) % side, tz = off + q / (side * side);
+        const int i = tx + S * (ty + S * tz);
         float res = 0.f;
-        if (!(x < 0 || y < 0 || z < 0 || x >= int(f.W) || y >= int(f.H) || z >= int(f.D) || on_boundary(f, x, y, z) ||
-              !lv_active(mf, x, y, z))) {
-          float sum = A[i + 1] + A[i - 1] + A[i + S] + A[i - S] + A[i + S * S] + A[i - S * S];
-          float lap = sum - A[i] * 6.f;
+        if (tile_on<R0>(nb, tx, ty, tz)) {
+          const float sum = A[i + 1] + A[i - 1] + A[i + S] + A[i - S] + A[i + S * S] + A[i - S * S];
+          const float lap = sum - A[i] * 6.f;
           res = tr[i] - lap;
         }
         B[i] = res;
-        bool inside = tx >= R0 && ty >= R0 && tz >= R0 && tx < R0 + 8 && ty < R0 + 8 && tz < R0 + 8;
+        const bool inside = tx >= R0 && ty >= R0 && tz >= R0 && tx < R0 + 8 && ty < R0 + 8 && tz < R0 + 8;
         if (inside) acc += double(res) * double(res);
       }
     }
     // smoothed p of the brick
-    {
+    if (p_out) {
       const int t = threadIdx.x;
-      int tx = R0 + (t & 7), ty = R0 + ((t >> 3) & 7), tz = R0 + (t >> 6);
-      int x = ox + tx, y = oy + ty, z = oz + tz;
-      if (x < int(f.W) && y < int(f.H) && z < int(f.D)) p_out[(b << 9) | t] = A[tx + S * (ty + S * tz)];
+      const int tx = R0 + (t & 7), ty = R0 + ((t >> 3) & 7), tz = R0 + (t >> 6);
+      if (ox + tx < int(f.W) && oy + ty < int(f.H) && oz + tz < int(f.D)) p_out[(b << 9) | t] = A[tx + S * (ty + S * tz)];
     }
     __syncthreads();
-    // restriction of the brick's 4^3 coarse voxels
+    // (1,3,3,1)/8 x 4 restriction of the brick's 4^3 coarse voxels (the tap order
+    // of the unfused restriction, taps clamped to the domain)
     if (threadIdx.x < 64) {
       const int t = threadIdx.x;
-      int cx = bx * 4 + (t & 3), cy = by * 4 + ((t >> 2) & 3), cz = bz * 4 + (t >> 4);
+      const int cx = bx * 4 + (t & 3), cy = by * 4 + ((t >> 2) & 3), cz = bz * 4 + (t >> 4);
       if (cx < int(c.W) && cy < int(c.H) && cz < int(c.D)) {
         float out = 0.f;
-        if (!on_boundary(c, cx, cy, cz) && lv_active(mc, cx, cy, cz)) {
+        if (cact) {
           const float wt[4] = {0.125f, 0.375f, 0.375f, 0.125f};
           float sum = 0.f;
           for (int k = 0; k < 4; ++k) {
-            int fz = min(max(2 * cz - 1 + k, 0), int(f.D) - 1) - oz;
+            const int fz = min(max(2 * cz - 1 + k, 0), int(f.D) - 1) - oz;
             for (int j = 0; j < 4; ++j) {
-              int fy = min(max(2 * cy - 1 + j, 0), int(f.H) - 1) - oy;
+              const int fy = min(max(2 * cy - 1 + j, 0), int(f.H) - 1) - oy;
               for (int a = 0; a < 4; ++a) {
-                int fx = min(max(2 * cx - 1 + a, 0), int(f.W) - 1) - ox;
+                const int fx = min(max(2 * cx - 1 + a, 0), int(f.W) - 1) - ox;
                 sum += (wt[k] * wt[j] * wt[a]) * B[fx + S * (fy + S * fz)];
               }
             }
           }
           out = sum * 4.f;
         }
-        crhs[vidx(c, cx, cy, cz)] = out;
+        crhs[vidx(c, uint32_t(cx), uint32_t(cy), uint32_t(cz))] = out;
       }
     }
   }
@@ -1177,7 +1286,7 @@ __global__ __launch_bounds__(512) void k_vdown(const uint32_t* __restrict__ list
   if (threadIdx.x == 0) {
     double sacc = 0.0;
     for (int k = 0; k < 8; ++k) sacc += part[k];
-    atomicAdd(norm2, sacc);
+    atomicAdd(norm2 + (blockIdx.x % kNormLanes), sacc);
   }
 }
 
@@ -1185,57 +1294,223 @@ __global__ __launch_bounds__(512) void k_vdown(const uint32_t* __restrict__ list
 // sampler) added to p_in on brick +- POST, then POST sweeps; writes p_out.
 template <int POST>
 __global__ __launch_bounds__(512) void k_vup(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid c, Grid f,
-                                             LvMask mf, const StepConst* __restrict__ C, const float* __restrict__ e,
+                                             LvBits bf, const StepConst* __restrict__ C, const float* __restrict__ e,
                                              const float* __restrict__ rhs, const float* __restrict__ p_in,
                                              float* __restrict__ p_out) {
   constexpr int R0 = POST > 0 ? POST : 0, S = Tile<R0>::S, N = Tile<R0>::N;
   constexpr int CR = (POST + 3) / 2, CS = 4 + 2 * CR, CN = CS * CS * CS;
   __shared__ float ta[N], tb[N], tr[N], tc[CN];
+  __shared__ uint64_t nb[216];
   const float om = C->omega;
   const float mcx = float(c.W - 1), mcy = float(c.H - 1), mcz = float(c.D - 1);
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
-    const int bx = int(b % f.BX), by = int((b / f.BX) % f.BY), bz = int(b / (f.BX * f.BY));
+    int bx, by, bz;
+    brick_coords(f, b, bx, by, bz);
     const int ox = bx * 8 - R0, oy = by * 8 - R0, oz = bz * 8 - R0;
     const int cox = bx * 4 - CR, coy = by * 4 - CR, coz = bz * 4 - CR;
-    __syncthreads();
-    for (int q = threadIdx.x; q < CN; q += 512) {
-      int x = min(max(cox + q % CS, 0), int(c.W) - 1), y = min(max(coy + (q / CS) % CS, 0), int(c.H) - 1),
-          z = min(max(coz + q / (CS * CS), 0), int(c.D) - 1);
-      tc[q] = e[vidx(c, x, y, z)];
+    constexpr int CIT = (CN + 511) / 512;
+    float cv[CIT];
+#pragma unroll
+    for (int j = 0; j < CIT; ++j) {
+      const int q = int(threadIdx.x) + j * 512;
+      const int x = min(max(cox + q % CS, 0), int(c.W) - 1), y = min(max(coy + (q / CS) % CS, 0), int(c.H) - 1),
+                z = min(max(coz + q / (CS * CS), 0), int(c.D) - 1);
+      cv[j] = q < CN ? e[vidx(c, uint32_t(x), uint32_t(y), uint32_t(z))] : 0.f;
     }
-    tile_load<R0>(f, p_in, ox, oy, oz, ta);
-    if (POST > 0) tile_load<R0>(f, rhs, ox, oy, oz, tr);
+    TileRegs<R0> rp, rr;
+    tile_fetch<R0>(f, p_in, ox, oy, oz, rp);
+    if (POST > 0) tile_fetch<R0>(f, rhs, ox, oy, oz, rr);
+    __syncthreads();
+    nbhd_load(f, bf, bx, by, bz, nb);
+#pragma unroll
+    for (int j = 0; j < CIT; ++j) {
+      const int q = int(threadIdx.x) + j * 512;
+      if (q < CN) tc[q] = cv[j];
+    }
+    __syncthreads();
+    tile_store<R0>(nb, rp, ta);
+    if (POST > 0) tile_store<R0>(nb, rr, tr);
     __syncthreads();
     // p + P(e) on brick +- R0 into tb
     for (int q = threadIdx.x; q < N; q += 512) {
-      int tx = q % S, ty = (q / S) % S, tz = q / (S * S);
-      int x = ox + tx, y = oy + ty, z = oz + tz;
-      if (x < 0 || y < 0 || z < 0 || x >= int(f.W) || y >= int(f.H) || z >= int(f.D) || on_boundary(f, x, y, z) ||
-          !lv_active(mf, x, y, z)) { tb[q] = 0.f; continue; }
-      float px = clampf((float(x) + 0.5f) * 0.5f - 0.5f, 0.f, mcx), py = clampf((float(y) + 0.5f) * 0.5f - 0.5f, 0.f, mcy),
-            pz = clampf((float(z) + 0.5f) * 0.5f - 0.5f, 0.f, mcz);
-      int x0 = int(floorf(px)), y0 = int(floorf(py)), z0 = int(floorf(pz));
-      int x1 = min(x0 + 1, int(c.W) - 1), y1 = min(y0 + 1, int(c.H) - 1), z1 = min(z0 + 1, int(c.D) - 1);
-      float fx = fractf(px), fy = fractf(py), fz = fractf(pz);
+      const int tx = q % S, ty = (q / S) % S, tz = q / (S * S);
+      if (!tile_on<R0>(nb, tx, ty, tz)) { tb[q] = 0.f; continue; }
+      const int x = ox + tx, y = oy + ty, z = oz + tz;
+      const float px = clampf((float(x) + 0.5f) * 0.5f - 0.5f, 0.f, mcx), py = clampf((float(y) + 0.5f) * 0.5f - 0.5f, 0.f, mcy),
+                  pz = clampf((float(z) + 0.5f) * 0.5f - 0.5f, 0.f, mcz);
+      const int x0 = int(floorf(px)), y0 = int(floorf(py)), z0 = int(floorf(pz));
+      const int x1 = min(x0 + 1, int(c.W) - 1), y1 = min(y0 + 1, int(c.H) - 1), z1 = min(z0 + 1, int(c.D) - 1);
+      const float fx = fractf(px), fy = fractf(py), fz = fractf(pz);
       auto at = [&](int a, int bb, int cc) { return tc[(a - cox) + CS * ((bb - coy) + CS * (cc - coz))]; };
-      float x00 = mixf(at(x0, y0, z0), at(x1, y0, z0), fx);
-      float x10 = mixf(at(x0, y1, z0), at(x1, y1, z0), fx);
-      float x01 = mixf(at(x0, y0, z1), at(x1, y0, z1), fx);
-      float x11 = mixf(at(x0, y1, z1), at(x1, y1, z1), fx);
+      const float x00 = mixf(at(x0, y0, z0), at(x1, y0, z0), fx);
+      const float x10 = mixf(at(x0, y1, z0), at(x1, y1, z0), fx);
+      const float x01 = mixf(at(x0, y0, z1), at(x1, y0, z1), fx);
+      const float x11 = mixf(at(x0, y1, z1), at(x1, y1, z1), fx);
       tb[q] = ta[q] + mixf(mixf(x00, x10, fy), mixf(x01, x11, fy), fz);
     }
     __syncthreads();
     float* A = tb;
     float* B = ta;
-    for (int sw = 1; sw <= POST; ++sw) {
-      tile_sweep<R0>(f, mf, ox, oy, oz, R0 - sw, A, tr, B, om);
+    if constexpr (POST >= 1) {
+      tile_sweep<R0, R0 - 1>(nb, A, tr, B, om);
+      __syncthreads();
+      float* t = A; A = B; B = t;
+    }
+    if constexpr (POST >= 2) {
+      tile_sweep<R0, R0 - 2>(nb, A, tr, B, om);
+      __syncthreads();
+      float* t = A; A = B; B = t;
+    }
+    if constexpr (POST >= 3) {
+      tile_sweep<R0, R0 - 3>(nb, A, tr, B, om);
       __syncthreads();
       float* t = A; A = B; B = t;
     }
     const int t = threadIdx.x;
-    int tx = R0 + (t & 7), ty = R0 + ((t >> 3) & 7), tz = R0 + (t >> 6);
+    const int tx = R0 + (t & 7), ty = R0 + ((t >> 3) & 7), tz = R0 + (t >> 6);
     if (ox + tx < int(f.W) && oy + ty < int(f.H) && oz + tz < int(f.D)) p_out[(b << 9) | t] = A[tx + S * (ty + S * tz)];
+  }
+}
+
+// p += P(e) on the active voxels of a level, one thread per voxel: the
+// prolongation sampler of pressure-prolongation.wgsl.ts:29-44 with the same
+// arithmetic as the fused up-leg, in place.
+__global__ __launch_bounds__(512) void k_prolong_add(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid c,
+                                                     Grid f, LvBits lb, const float* __restrict__ e, float* __restrict__ p) {
+  const uint32_t t = threadIdx.x;
+  const float mcx = float(c.W - 1), mcy = float(c.H - 1), mcz = float(c.D - 1);
+  FX_BRICK_LOOP(list, count) {
+    const uint32_t b = list[fx_k];
+    if (!((lb.bits[size_t(fx_k) * 8 + (t >> 6)] >> (t & 63u)) & 1ull)) continue;
+    const uint32_t x = (b % f.BX) * 8 + (t & 7u), y = ((b / f.BX) % f.BY) * 8 + ((t >> 3) & 7u), z = (b / (f.BX * f.BY)) * 8 + (t >> 6);
+    const float px = clampf((float(x) + 0.5f) * 0.5f - 0.5f, 0.f, mcx), py = clampf((float(y) + 0.5f) * 0.5f - 0.5f, 0.f, mcy),
+                pz = clampf((float(z) + 0.5f) * 0.5f - 0.5f, 0.f, mcz);
+    const uint32_t x0 = uint32_t(floorf(px)), y0 = uint32_t(floorf(py)), z0 = uint32_t(floorf(pz));
+    const uint32_t x1 = min(x0 + 1u, c.W - 1u), y1 = min(y0 + 1u, c.H - 1u), z1 = min(z0 + 1u, c.D - 1u);
+    const float fx = fractf(px), fy = fractf(py), fz = fractf(pz);
+    const float x00 = mixf(e[vidx(c, x0, y0, z0)], e[vidx(c, x1, y0, z0)], fx);
+    const float x10 = mixf(e[vidx(c, x0, y1, z0)], e[vidx(c, x1, y1, z0)], fx);
+    const float x01 = mixf(e[vidx(c, x0, y0, z1)], e[vidx(c, x1, y0, z1)], fx);
+    const float x11 = mixf(e[vidx(c, x0, y1, z1)], e[vidx(c, x1, y1, z1)], fx);
+    const uint32_t i = (b << 9) | t;
+    p[i] = p[i] + mixf(mixf(x00, x10, fy), mixf(x01, x11, fy), fz);
+  }
+}
+
+// Residual r = rhs - (sum of 6 neighbours - 6 p) of a V-cycle level
+// (pressure-residual.wgsl.ts:16-38), one thread per voxel, with the FP64 sum of
+// squares over the active voxels when norm2 != nullptr.
+__global__ __launch_bounds__(512) void k_residual_lean(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                       LvBits lb, const float* __restrict__ rhs, const float* __restrict__ p,
+                                                       float* __restrict__ r, double* norm2) {
+  __shared__ double part[8];
+  const uint32_t t = threadIdx.x, lx = t & 7u, ly = (t >> 3) & 7u, lz = t >> 6;
+  const uint32_t BX = g.BX, BXY = g.BX * g.BY;
+  double acc = 0.0;
+  FX_BRICK_LOOP(list, count) {
+    const uint32_t b = list[fx_k];
+    const uint32_t i = (b << 9) | t;
+    if (!((lb.bits[size_t(fx_k) * 8 + lz] >> (t & 63u)) & 1ull)) { r[i] = 0.f; continue; }
+    const uint32_t xp = lx < 7u ? i + 1u : (((b + 1u) << 9) | (t - 7u));
+    const uint32_t xm = lx > 0u ? i - 1u : (((b - 1u) << 9) | (t + 7u));
+    const uint32_t yp = ly < 7u ? i + 8u : (((b + BX) << 9) | (t - 56u));
+    const uint32_t ym = ly > 0u ? i - 8u : (((b - BX) << 9) | (t + 56u));
+    const uint32_t zp = lz < 7u ? i + 64u : (((b + BXY) << 9) | (t - 448u));
+    const uint32_t zm = lz > 0u ? i - 64u : (((b - BXY) << 9) | (t + 448u));
+    const float sum = p[xp] + p[xm] + p[yp] + p[ym] + p[zp] + p[zm];
+    const float lap = sum - p[i] * 6.f;
+    const float res = rhs[i] - lap;
+    r[i] = res;
+    acc += double(res) * double(res);
+  }
+  if (!norm2) return;
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sacc = 0.0;
+    for (int k = 0; k < 8; ++k) sacc += part[k];
+    atomicAdd(norm2 + (blockIdx.x % kNormLanes), sacc);
+  }
+}
+// (1,3,3,1)/8 x 4 restriction of r into the coarse right-hand side, one
+// workgroup per listed coarse brick: the 18^3 fine region its taps reach is
+// staged in LDS (every load in flight at once), then each thread sums the 64
+// taps of its coarse voxel in the oracle's order (taps clamped to the domain).
+__global__ __launch_bounds__(512) void k_restrict_lean(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid f,
+                                                       Grid c, LvBits cb, const float* __restrict__ r, float* __restrict__ crhs) {
+  constexpr int S = 18, N = S * S * S, IT = (N + 511) / 512;
+  __shared__ float tf[N];
+  const uint32_t t = threadIdx.x;
+  const float wt[4] = {0.125f, 0.375f, 0.375f, 0.125f};
+  FX_BRICK_LOOP(list, count) {
+    const uint32_t b = list[fx_k];
+    const int cbx = int(b % c.BX), cby = int((b / c.BX) % c.BY), cbz = int(b / (c.BX * c.BY));
+    // fine origin of the region: 2 * (coarse brick origin) - 1
+    const int ox = cbx * 16 - 1, oy = cby * 16 - 1, oz = cbz * 16 - 1;
+    float v[IT];
+#pragma unroll
+    for (int j = 0; j < IT; ++j) {
+      const int q = int(t) + j * 512;
+      const int x = min(max(ox + q % S, 0), int(f.W) - 1), y = min(max(oy + (q / S) % S, 0), int(f.H) - 1),
+                z = min(max(oz + q / (S * S), 0), int(f.D) - 1);
+      v[j] = q < N ? r[vidx(f, uint32_t(x), uint32_t(y), uint32_t(z))] : 0.f;
+    }
+    const bool act = (cb.bits[size_t(fx_k) * 8 + (t >> 6)] >> (t & 63u)) & 1ull;
+    __syncthreads();  // the previous brick's taps have been read
+#pragma unroll
+    for (int j = 0; j < IT; ++j) {
+      const int q = int(t) + j * 512;
+      if (q < N) tf[q] = v[j];
+    }
+    __syncthreads();
+    const uint32_t i = (b << 9) | t;
+    if (!act) { crhs[i] = 0.f; continue; }
+    const int cx = cbx * 8 + int(t & 7u), cy = cby * 8 + int((t >> 3) & 7u), cz = cbz * 8 + int(t >> 6);
+    float sum = 0.f;
+    for (int k = 0; k < 4; ++k) {
+      const int fz = min(max(2 * cz - 1 + k, 0), int(f.D) - 1) - oz;
+      for (int j = 0; j < 4; ++j) {
+        const int fy = min(max(2 * cy - 1 + j, 0), int(f.H) - 1) - oy;
+        for (int a = 0; a < 4; ++a) {
+          const int fx = min(max(2 * cx - 1 + a, 0), int(f.W) - 1) - ox;
+          sum += (wt[k] * wt[j] * wt[a]) * tf[fx + S * (fy + S * fz)];
+        }
+      }
+    }
+    crhs[i] = sum * 4.f;
+  }
+}
+
+// One weighted-Jacobi sweep of a V-cycle level (pressure-jacobi.wgsl.ts:15-36,
+// mix(p, jacobi, omega)); one thread per voxel, neighbours addressed through
+// home-page brick arithmetic (active voxels are interior, so every neighbour
+// brick exists; unlisted ones hold 0). ZERO: the sweep starts from p = 0.
+template <bool ZERO>
+__global__ __launch_bounds__(512) void k_sweep(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                               LvBits lb, const StepConst* __restrict__ C, const float* __restrict__ rhs,
+                                               const float* __restrict__ src, float* __restrict__ dst) {
+  const float om = C->omega;
+  const uint32_t t = threadIdx.x, lx = t & 7u, ly = (t >> 3) & 7u, lz = t >> 6;
+  const uint32_t BX = g.BX, BXY = g.BX * g.BY;
+  FX_BRICK_LOOP(list, count) {
+    const uint32_t b = list[fx_k];
+    const uint32_t i = (b << 9) | t;
+    if (!((lb.bits[size_t(fx_k) * 8 + lz] >> (t & 63u)) & 1ull)) { dst[i] = 0.f; continue; }
+    float sum = 0.f, self = 0.f;
+    if (!ZERO) {
+      const uint32_t xp = lx < 7u ? i + 1u : (((b + 1u) << 9) | (t - 7u));
+      const uint32_t xm = lx > 0u ? i - 1u : (((b - 1u) << 9) | (t + 7u));
+      const uint32_t yp = ly < 7u ? i + 8u : (((b + BX) << 9) | (t - 56u));
+      const uint32_t ym = ly > 0u ? i - 8u : (((b - BX) << 9) | (t + 56u));
+      const uint32_t zp = lz < 7u ? i + 64u : (((b + BXY) << 9) | (t - 448u));
+      const uint32_t zm = lz > 0u ? i - 64u : (((b - BXY) << 9) | (t + 448u));
+      sum = src[xp] + src[xm] + src[yp] + src[ym] + src[zp] + src[zm];
+      self = src[i];
+    }
+    const float pj = (sum - rhs[i]) / 6.f;
+    dst[i] = mixf(self, pj, om);
   }
 }
 
@@ -1555,27 +1830,48 @@ void launch_jacobi(hipStream_t s, int blocks, const uint32_t* list, const uint32
                    const float* rhs, const float* src, float* dst, int weighted) {
   FX_LAUNCH(k_jacobi, blocks, 512, s, list, count, g, m, c, rhs, src, dst, weighted);
 }
-void launch_smooth_block(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const StepConst* c,
-                         const float* rhs, float* p0, float* p1, int sweeps, int weighted, int zero_start) {
-  FX_LAUNCH(k_smooth_block, 1, 1024, s, list, count, g, m, c, rhs, p0, p1, sweeps, weighted, zero_start);
+void launch_level_masks(hipStream_t s, int blocks, const LevelMaskArgs& a) { FX_LAUNCH(k_level_masks, blocks, 512, s, a); }
+void launch_smooth_block(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, LvBits b,
+                         const StepConst* c, const float* rhs, float* p0, float* p1, int sweeps, int weighted, int zero_start) {
+  FX_LAUNCH(k_smooth_block, 1, 1024, s, list, count, g, m, b, c, rhs, p0, p1, sweeps, weighted, zero_start);
 }
-bool launch_vdown(hipStream_t s, int blocks, int pre, const uint32_t* list, const uint32_t* count, Grid f, LvMask mf, Grid c,
+bool launch_vdown(hipStream_t s, int blocks, int pre, const uint32_t* list, const uint32_t* count, Grid f, LvBits bf, Grid c,
                   LvMask mc, const StepConst* cst, const float* rhs, const float* p_in, float* p_out, float* crhs, double* norm2) {
   switch (pre) {
-    case 0: FX_LAUNCH(k_vdown<0>, blocks, 512, s, list, count, f, mf, c, mc, cst, rhs, p_in, p_out, crhs, norm2); return true;
-    case 1: FX_LAUNCH(k_vdown<1>, blocks, 512, s, list, count, f, mf, c, mc, cst, rhs, p_in, p_out, crhs, norm2); return true;
-    case 2: FX_LAUNCH(k_vdown<2>, blocks, 512, s, list, count, f, mf, c, mc, cst, rhs, p_in, p_out, crhs, norm2); return true;
+    case 0: FX_LAUNCH(k_vdown<0>, blocks, 512, s, list, count, f, bf, c, mc, cst, rhs, p_in, p_out, crhs, norm2); return true;
+    case 1: FX_LAUNCH(k_vdown<1>, blocks, 512, s, list, count, f, bf, c, mc, cst, rhs, p_in, p_out, crhs, norm2); return true;
+    case 2: FX_LAUNCH(k_vdown<2>, blocks, 512, s, list, count, f, bf, c, mc, cst, rhs, p_in, p_out, crhs, norm2); return true;
     default: return false;  // larger halos exceed the LDS budget: unfused sweeps
   }
 }
-bool launch_vup(hipStream_t s, int blocks, int post, const uint32_t* list, const uint32_t* count, Grid c, Grid f, LvMask mf,
+bool launch_vup(hipStream_t s, int blocks, int post, const uint32_t* list, const uint32_t* count, Grid c, Grid f, LvBits bf,
                 const StepConst* cst, const float* e, const float* rhs, const float* p_in, float* p_out) {
   switch (post) {
-    case 0: FX_LAUNCH(k_vup<0>, blocks, 512, s, list, count, c, f, mf, cst, e, rhs, p_in, p_out); return true;
-    case 1: FX_LAUNCH(k_vup<1>, blocks, 512, s, list, count, c, f, mf, cst, e, rhs, p_in, p_out); return true;
-    case 2: FX_LAUNCH(k_vup<2>, blocks, 512, s, list, count, c, f, mf, cst, e, rhs, p_in, p_out); return true;
-    case 3: FX_LAUNCH(k_vup<3>, blocks, 512, s, list, count, c, f, mf, cst, e, rhs, p_in, p_out); return true;
+    case 0: FX_LAUNCH(k_vup<0>, blocks, 512, s, list, count, c, f, bf, cst, e, rhs, p_in, p_out); return true;
+    case 1: FX_LAUNCH(k_vup<1>, blocks, 512, s, list, count, c, f, bf, cst, e, rhs, p_in, p_out); return true;
+    case 2: FX_LAUNCH(k_vup<2>, blocks, 512, s, list, count, c, f, bf, cst, e, rhs, p_in, p_out); return true;
+    case 3: FX_LAUNCH(k_vup<3>, blocks, 512, s, list, count, c, f, bf, cst, e, rhs, p_in, p_out); return true;
     default: return false;
+  }
+}
+void launch_residual_lean(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b,
+                          const float* rhs, const float* p, float* r, double* norm2) {
+  FX_LAUNCH(k_residual_lean, blocks, 512, s, list, count, g, b, rhs, p, r, norm2);
+}
+void launch_restrict_lean(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid f, Grid c, LvBits cb,
+                          const float* r, float* crhs) {
+  FX_LAUNCH(k_restrict_lean, blocks, 512, s, list, count, f, c, cb, r, crhs);
+}
+void launch_prolong_add(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid c, Grid f, LvBits b,
+                        const float* e, float* p) {
+  FX_LAUNCH(k_prolong_add, blocks, 512, s, list, count, c, f, b, e, p);
+}
+void launch_sweep(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
+                  const float* rhs, const float* src, float* dst) {
+  if (src) {
+    FX_LAUNCH(k_sweep<false>, blocks, 512, s, list, count, g, b, c, rhs, src, dst);
+  } else {
+    FX_LAUNCH(k_sweep<true>, blocks, 512, s, list, count, g, b, c, rhs, src, dst);
   }
 }
 void launch_residual(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const float* rhs,
