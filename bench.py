@@ -140,6 +140,9 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=10, help="extra steps timed per stage with HIP events")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of this many MiB (a known-size reference for the "
+                         "HBM traffic counters)")
     ap.add_argument("--precision", choices=sorted(PRECISIONS), default="fp32",
                     help="storage of density/temperature/fuel (arithmetic is FP32 throughout)")
     args = ap.parse_args()
@@ -227,6 +230,15 @@ def main():
                      "bytes_per_active_voxel": step_bytes_voxel},
         "stage_ms": stage_ms,
     }
+    if args.copy_probe > 0:
+        nbytes = args.copy_probe << 20
+        src, dst = HipBuffer(nbytes), HipBuffer(nbytes)
+        if src.hip.hipMemcpy(dst.ptr, src.ptr, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("probe copy failed")
+        src.hip.hipDeviceSynchronize()
+        src.free()
+        dst.free()
+        out["copy_probe_bytes"] = nbytes
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, sim, (base + args.timing_steps) * dt, args.cpu_steps)
     img.free()

@@ -8,7 +8,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libfenix.so")
+LIB_PATH = os.environ.get("FENIX_LIB") or os.path.join(HERE, "libfenix.so")
 
 # ---- constants (include/fenix.h) -------------------------------------------
 FX_OK = 0

@@ -34,6 +34,8 @@ struct Level {
   uint32_t* flags = nullptr;         // sparse: level activity flags
   BrickList L{};                     // sparse: active list; dense: identity
   uint64_t* bits = nullptr;          // voxel activity words, 8 per listed brick (V-cycle)
+  void* tab = nullptr;               // coarsest level: slot table of the block smoother
+  uint32_t* tab_n = nullptr;
 };
 
 struct StageSlot {
@@ -403,6 +405,11 @@ int solve_vcycle(fx_sim* s) {
     }
     launch_level_masks(s->stream, grid_for(s, s->lv[0].nb), A);
   }
+  Level& LC = s->lv[L - 1];
+  const bool block_smoother = L > 1 && LC.nv <= kBlockSmoothVoxels;
+  if (block_smoother)
+    launch_coarse_setup(s->stream, level_list(s, L - 1), level_count(s, L - 1), LC.g, bits_of(s, L - 1), LC.tab, LC.tab_n, LC.p[0],
+                        LC.p[1]);
   // n sweeps at level l; zero_start: the first sweep starts from p = 0 (a
   // coarse level's correction), so neither a clear nor a read of p is needed.
   auto smoothW = [&](int l, int n, bool zero_start) {
@@ -412,9 +419,9 @@ int solve_vcycle(fx_sim* s) {
       return;
     }
     if (zero_start) cur[l] = 0;
-    if (l > 0 && l == L - 1 && V.nv <= kBlockSmoothVoxels) {
-      launch_smooth_block(s->stream, level_list(s, l), level_count(s, l), V.g, mask_of(s, l), bits_of(s, l), s->d_const, V.div,
-                          V.p[cur[l]], V.p[1 - cur[l]], n, 1, zero_start ? 1 : 0);
+    if (l == L - 1 && block_smoother) {
+      launch_smooth_block(s->stream, level_list(s, l), level_count(s, l), V.g, mask_of(s, l), s->d_const, V.div, V.p[cur[l]],
+                          V.p[1 - cur[l]], n, 1, zero_start ? 1 : 0, V.tab, V.tab_n);
       if (n & 1) cur[l] = 1 - cur[l];
       return;
     }
@@ -789,6 +796,12 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
     if (rc == FX_OK) rc = dalloc(s, &L.L.retire_count, 1);
     if (rc == FX_OK) rc = l == 0 ? FX_OK : dalloc(s, &L.flags, L.nb);
     if (rc == FX_OK && desc->solver == FX_SOLVER_VCYCLE) rc = dalloc(s, &L.bits, L.nb * 8);
+    if (rc == FX_OK && desc->solver == FX_SOLVER_VCYCLE && l > 0) {
+      uint8_t* t = nullptr;
+      rc = dalloc(s, &t, kCoarseTableBytes);
+      L.tab = t;
+      if (rc == FX_OK) rc = dalloc(s, &L.tab_n, 1);
+    }
     if (l == 0) L.flags = s->flagsInj;
     s->lv.push_back(L);
     lg = make_grid(halve(lg.W), halve(lg.H), halve(lg.D));
@@ -875,7 +888,7 @@ void fx_destroy(fx_sim* s) {
   F(s->touched); F(s->d_bits); F(s->identity); F(s->identity_count); F(s->d_const); F(s->d_cand); F(s->d_src); F(s->d_norms);
   for (size_t l = 0; l < s->lv.size(); ++l) {
     Level& L = s->lv[l];
-    F(L.div); F(L.p[0]); F(L.p[1]); F(L.bits); F(L.L.list); F(L.L.count); F(L.L.page); F(L.L.retire); F(L.L.retire_count);
+    F(L.div); F(L.p[0]); F(L.p[1]); F(L.bits); F(L.tab); F(L.tab_n); F(L.L.list); F(L.L.count); F(L.L.page); F(L.L.retire); F(L.L.retire_count);
     if (l > 0) F(L.flags);
   }
   for (auto& slot : s->slots) {

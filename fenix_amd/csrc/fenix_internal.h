@@ -145,9 +145,15 @@ void launch_prolong_add(hipStream_t s, int blocks, const uint32_t* list, const u
 // One weighted-Jacobi sweep of a V-cycle level; src == nullptr starts from p = 0.
 void launch_sweep(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
                   const float* rhs, const float* src, float* dst);
-// bits == nullptr in `b`: voxel activity is evaluated from `m` inside the kernel.
-void launch_smooth_block(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, LvBits b,
-                         const StepConst* c, const float* rhs, float* p0, float* p1, int sweeps, int weighted, int zero_start);
+// Coarsest-level slot table (kCoarseTableBytes at tab, count at tab_n), built
+// once per solve; zeroes the inactive voxels of the level's listed bricks in p0/p1.
+constexpr size_t kCoarseTableBytes = 8192 * 16;
+void launch_coarse_setup(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, void* tab, uint32_t* tab_n,
+                         float* p0, float* p1);
+// tab == nullptr (or an overflowed table): global-memory sweeps in one workgroup.
+void launch_smooth_block(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const StepConst* c,
+                         const float* rhs, float* p0, float* p1, int sweeps, int weighted, int zero_start, const void* tab,
+                         const uint32_t* tab_n);
 void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
                             const StepConst* c, Vec3 u, Vec3 us);
 void launch_advect_scalars(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,

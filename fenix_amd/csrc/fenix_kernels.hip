@@ -85,15 +85,22 @@ struct Tri {
   uint32_t i[8];
   float fx, fy, fz;
 };
+// Corners of the clamped trilinear sample at p. The upper corner along each axis
+// is min(c + 1, n - 1); in brick-major order it sits 1 / 8 / 64 elements on,
+// or in the next brick when the local coordinate is 7, so the eight indices are
+// one brick-major index plus per-axis steps.
 __device__ __forceinline__ Tri tri(const Grid& g, F3 p) {
   float mx = float(g.W - 1), my = float(g.H - 1), mz = float(g.D - 1);
   float cx = clampf(p.x, 0.f, mx), cy = clampf(p.y, 0.f, my), cz = clampf(p.z, 0.f, mz);
   uint32_t bx = uint32_t(floorf(cx)), by = uint32_t(floorf(cy)), bz = uint32_t(floorf(cz));
-  uint32_t ux = min(bx + 1u, g.W - 1), uy = min(by + 1u, g.H - 1), uz = min(bz + 1u, g.D - 1);
+  const uint32_t i0 = vidx(g, bx, by, bz);
+  const uint32_t sx = bx + 1u < g.W ? ((bx & 7u) < 7u ? 1u : 505u) : 0u;
+  const uint32_t sy = by + 1u < g.H ? ((by & 7u) < 7u ? 8u : g.BX * 512u - 56u) : 0u;
+  const uint32_t sz = bz + 1u < g.D ? ((bz & 7u) < 7u ? 64u : g.BX * g.BY * 512u - 448u) : 0u;
   Tri t;
   t.fx = fractf(cx); t.fy = fractf(cy); t.fz = fractf(cz);
-  t.i[0] = vidx(g, bx, by, bz); t.i[1] = vidx(g, ux, by, bz); t.i[2] = vidx(g, bx, uy, bz); t.i[3] = vidx(g, ux, uy, bz);
-  t.i[4] = vidx(g, bx, by, uz); t.i[5] = vidx(g, ux, by, uz); t.i[6] = vidx(g, bx, uy, uz); t.i[7] = vidx(g, ux, uy, uz);
+  t.i[0] = i0; t.i[1] = i0 + sx; t.i[2] = i0 + sy; t.i[3] = i0 + sx + sy;
+  t.i[4] = i0 + sz; t.i[5] = i0 + sx + sz; t.i[6] = i0 + sy + sz; t.i[7] = i0 + sx + sy + sz;
   return t;
 }
 __device__ __forceinline__ float tri_eval(const Tri& t, const float* __restrict__ f) {
@@ -170,6 +177,19 @@ struct Src {
 };
 
 // SH/inject-source-common.wgsl.ts:102-121
+// Sine and cosine for the smooth procedural fields (thermal noise, buoyancy
+// shear): two-part Cody-Waite reduction by 2*pi, then the hardware
+// v_sin_f32 / v_cos_f32 (argument in revolutions). The arguments stay below a
+// few thousand radians, where this keeps the error near 1e-6; hash31 below keeps
+// the correctly rounded sinf because it scales the result by 43758.
+__device__ __forceinline__ float rev2pi(float x) {
+  const float k = rintf(x * 0.15915494309189535f);
+  float r = fmaf(-k, 6.28318548202514648f, x);
+  r = fmaf(-k, -1.7484556000744487e-07f, r);
+  return r * 0.15915494309189535f;
+}
+__device__ __forceinline__ float sin_s(float x) { return __builtin_amdgcn_sinf(rev2pi(x)); }
+__device__ __forceinline__ float cos_s(float x) { return __builtin_amdgcn_cosf(rev2pi(x)); }
 __device__ __forceinline__ float hash31(F3 p) { return fractf(sinf(dot3(p, f3(127.1f, 311.7f, 74.7f))) * 43758.5453123f); }
 __device__ float value_noise3(F3 p) {
   F3 c = f3(floorf(p.x), floorf(p.y), floorf(p.z));
@@ -561,8 +581,8 @@ __global__ __launch_bounds__(512) void k_advect_velocity(const uint32_t* __restr
 // SH/advect-scalar-maccormack.wgsl.ts:52-57 (and semilagrangian :69-74)
 __device__ __forceinline__ float thermal_noise(F3 pos, float time) {
   F3 p = pos + f3(time * 1.7f, -time * 3.1f, time * 1.1f);
-  float a = 0.5f + 0.5f * sinf(dot3(p, f3(0.097f, 0.061f, -0.083f)));
-  float b = 0.5f + 0.5f * sinf(dot3(p, f3(-0.043f, 0.137f, 0.071f)) + sinf(p.y * 0.079f) * 1.6f);
+  float a = 0.5f + 0.5f * sin_s(dot3(p, f3(0.097f, 0.061f, -0.083f)));
+  float b = 0.5f + 0.5f * sin_s(dot3(p, f3(-0.043f, 0.137f, 0.071f)) + sin_s(p.y * 0.079f) * 1.6f);
   return clampf(a * 0.58f + b * 0.42f, 0.f, 1.f);
 }
 // SH/advect-scalar-maccormack.wgsl.ts:75-100
@@ -625,9 +645,9 @@ __global__ __launch_bounds__(512) void k_advect_scalars(const uint32_t* __restri
 // render set; voxels outside active bricks are copied.
 __device__ __forceinline__ float thermal_cluster(F3 c, float time) {
   F3 p = c + f3(time * 2.2f, time * 5.2f, -time * 1.7f);
-  float body = 0.5f + 0.5f * sinf(dot3(p, f3(0.071f, 0.093f, -0.064f)) + sinf(p.y * 0.057f) * 1.8f);
-  float pocket = 0.5f + 0.5f * sinf(dot3(p, f3(0.143f, -0.081f, 0.119f)) + sinf(p.x * 0.11f + p.z * 0.07f) * 1.2f);
-  float rising = 0.5f + 0.5f * sinf(p.x * 0.18f + p.y * 0.13f - p.z * 0.16f + time * 1.4f);
+  float body = 0.5f + 0.5f * sin_s(dot3(p, f3(0.071f, 0.093f, -0.064f)) + sin_s(p.y * 0.057f) * 1.8f);
+  float pocket = 0.5f + 0.5f * sin_s(dot3(p, f3(0.143f, -0.081f, 0.119f)) + sin_s(p.x * 0.11f + p.z * 0.07f) * 1.2f);
+  float rising = 0.5f + 0.5f * sin_s(p.x * 0.18f + p.y * 0.13f - p.z * 0.16f + time * 1.4f);
   return clampf(body * 0.46f + pocket * 0.34f + rising * 0.2f, 0.f, 1.f);
 }
 template <class ST>
@@ -696,7 +716,7 @@ __global__ __launch_bounds__(512) void k_buoyancy(const uint32_t* __restrict__ l
     F3 gravityForce = gdir * den * P.gravity[3];
     float entrain = den * smoothf(0.04f, 0.3f, hotGas);
     float ny = (float(y) + 0.5f) / float(g.H);
-    float shear = sinf(float(y) * 0.073f + P.time * 0.9f) * 0.35f + sinf(float(x) * 0.041f - float(z) * 0.052f + P.time * 0.43f) * 0.22f;
+    float shear = sin_s(float(y) * 0.073f + P.time * 0.9f) * 0.35f + sin_s(float(x) * 0.041f - float(z) * 0.052f + P.time * 0.43f) * 0.22f;
     float windAmount = P.wind[3] * (0.18f + den * 0.55f + ny * 0.45f);
     F3 wdir = f3(P.wind[0], P.wind[1], P.wind[2]);
     F3 wind = wdir * windAmount + f3(-P.wind[2], 0.f, P.wind[0]) * windAmount * shear;
@@ -707,8 +727,8 @@ __global__ __launch_bounds__(512) void k_buoyancy(const uint32_t* __restrict__ l
     float dgl = fmaxf(len3(dg), 0.0001f);
     float coldSmoke = den * (1.f - smoothf(0.12f, 0.72f, hotGas));
     float surfaceBreakup = smoothf(0.008f, 0.06f, dgl) * coldSmoke;
-    F3 shred = norm3(f3(sinf(float(y) * 0.117f + P.time * 1.9f), sinf(float(x) * 0.071f - float(z) * 0.083f + P.time * 1.35f) * 0.35f,
-                        cosf(float(x) * 0.061f + float(y) * 0.049f - P.time * 1.6f)));
+    F3 shred = norm3(f3(sin_s(float(y) * 0.117f + P.time * 1.9f), sin_s(float(x) * 0.071f - float(z) * 0.083f + P.time * 1.35f) * 0.35f,
+                        cos_s(float(x) * 0.061f + float(y) * 0.049f - P.time * 1.6f)));
     float collapse = den * (1.f - smoothf(0.16f, 0.64f, hotGas));
     float smokeDrag = den * smoothf(0.03f, 0.45f, hotGas);
     F3 vel = f3(us.x[i], us.y[i], us.z[i]) * (1.f - dt * 0.08f);
@@ -816,15 +836,18 @@ __global__ __launch_bounds__(512) void k_jacobi(const uint32_t* __restrict__ lis
 // zero when zero_start is set. Used for the coarsest level, whose dispatches
 // would otherwise be launch-bound.
 //
-// Staged path: the active interior voxels of the listed bricks are compacted
-// into LDS slots (per-brick 512-bit masks, one 64-bit ballot per z-slice, and an
-// exclusive scan of the per-brick counts); each thread owns up to kSmoothPer
-// slots and resolves their six neighbour slots once, so every sweep is LDS-only.
-// A neighbour outside the slots (inactive, boundary, or in an unlisted brick)
-// holds 0, as in the sparse invariant.
+// Staged path: k_coarse_setup (once per solve) compacts the active voxels of the
+// listed bricks into slots (per-brick 512-bit masks, an exclusive scan of the
+// per-brick counts) and writes one 16-byte table entry per slot: the voxel's
+// global index and its six neighbour slots (16 bits each; kSmoothCap names the
+// always-zero slot: inactive, boundary, or in an unlisted brick, as in the
+// sparse invariant). It also zeroes the inactive voxels of both buffers, which
+// the sweeps then never write. Each k_smooth_block thread owns up to kSmoothPer
+// slots; every sweep is LDS-only.
 constexpr uint32_t kSmoothCap = 8192;      // active voxels staged in LDS
 constexpr uint32_t kSmoothBricks = 512;    // listed bricks whose voxel masks fit in LDS
 constexpr uint32_t kSmoothTable = 4096;    // level bricks addressable through the LDS slot table
+constexpr uint32_t kSmoothOverflow = 0xffffffffu;  // table header: the level takes the global path
 constexpr int kSmoothPer = int(kSmoothCap / 1024);
 
 // Bit position of the r-th (0-based) set bit of w; w has more than r set bits.
@@ -843,91 +866,105 @@ __device__ __forceinline__ uint32_t rank_in_brick(const uint64_t* am, uint32_t k
   return r + uint32_t(__popcll(am[k * 8 + (t >> 6)] & ((1ull << (t & 63u)) - 1ull)));
 }
 
-__global__ __launch_bounds__(1024) void k_smooth_block(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                       LvMask m, LvBits lb, const StepConst* __restrict__ C,
-                                                       const float* __restrict__ rhs, float* p0, float* p1, int sweeps, int weighted,
-                                                       int zero_start) {
-  __shared__ float sp[2][kSmoothCap + 1];  // slot kSmoothCap holds 0: the missing neighbour
+__global__ __launch_bounds__(1024) void k_coarse_setup(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                       LvBits lb, uint4* __restrict__ tab, uint32_t* __restrict__ tab_n, float* p0,
+                                                       float* p1) {
   __shared__ uint64_t amask[kSmoothBricks * 8];
   __shared__ uint32_t base[kSmoothBricks];
   __shared__ int16_t slot[kSmoothTable];
   __shared__ uint32_t wtot[16];
-  const float om = C->omega;
   const uint32_t n = *count, nbl = g.BX * g.BY * g.BZ;
-  const uint32_t total = n * 512u;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-  uint32_t nact = 0;
-  if (n <= kSmoothBricks && nbl <= kSmoothTable) {
-    for (uint32_t q = tid; q < nbl; q += 1024) slot[q] = -1;
-    __syncthreads();
-    for (uint32_t k = tid; k < n; k += 1024) slot[list[k]] = int16_t(k);
-    // one ballot per (brick, z-slice): bit = x + 8y of an active interior voxel
-    if (lb.bits) {
-      for (uint32_t wd = tid; wd < n * 8; wd += 1024) amask[wd] = lb.bits[wd];
-    } else for (uint32_t wd = wave; wd < n * 8; wd += 16) {
-      const uint32_t b = list[wd >> 3], lz = wd & 7u;
-      const uint32_t x = (b % g.BX) * 8 + (lane & 7u), y = ((b / g.BX) % g.BY) * 8 + (lane >> 3),
-                     z = (b / (g.BX * g.BY)) * 8 + lz;
-      const bool act = x < g.W && y < g.H && z < g.D && !on_boundary(g, x, y, z) && lv_active(m, x, y, z);
-      const uint64_t bits = __ballot(act);
-      if (lane == 0) amask[wd] = bits;
-    }
-    __syncthreads();
-    // exclusive scan of the per-brick counts (n <= 512 <= blockDim)
-    uint32_t cnt = 0;
-    if (tid < n)
-      for (uint32_t j = 0; j < 8; ++j) cnt += uint32_t(__popcll(amask[tid * 8 + j]));
-    uint32_t incl = cnt;
-    for (uint32_t off = 1; off < 64; off <<= 1) {
-      const uint32_t v = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += v;
-    }
-    if (lane == 63) wtot[wave] = incl;
-    __syncthreads();
-    uint32_t pre = 0;
-    for (uint32_t w = 0; w < wave; ++w) pre += wtot[w];
-    if (tid < n) base[tid] = pre + incl - cnt;
-    for (uint32_t w = 0; w < 16; ++w) nact += wtot[w];
-    __syncthreads();
+  if (n == 0 || n > kSmoothBricks || nbl > kSmoothTable) {
+    if (tid == 0) *tab_n = n == 0 ? 0u : kSmoothOverflow;
+    return;
   }
-  if (n <= kSmoothBricks && nbl <= kSmoothTable && nact <= kSmoothCap) {
-    // owned slots: global index and neighbour slots (kSmoothCap: the neighbour holds 0)
-    int nbr[kSmoothPer][6];
-    uint32_t gid[kSmoothPer];
+  for (uint32_t q = tid; q < nbl; q += 1024) slot[q] = -1;
+  __syncthreads();
+  for (uint32_t k = tid; k < n; k += 1024) slot[list[k]] = int16_t(k);
+  for (uint32_t wd = tid; wd < n * 8; wd += 1024) amask[wd] = lb.bits[wd];
+  __syncthreads();
+  // exclusive scan of the per-brick counts (n <= 512 <= blockDim)
+  uint32_t cnt = 0;
+  if (tid < n)
+    for (uint32_t j = 0; j < 8; ++j) cnt += uint32_t(__popcll(amask[tid * 8 + j]));
+  uint32_t incl = cnt;
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const uint32_t v = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += v;
+  }
+  if (lane == 63) wtot[wave] = incl;
+  __syncthreads();
+  uint32_t pre = 0, nact = 0;
+  for (uint32_t w = 0; w < wave; ++w) pre += wtot[w];
+  for (uint32_t w = 0; w < 16; ++w) nact += wtot[w];
+  if (tid < n) base[tid] = pre + incl - cnt;
+  if (nact > kSmoothCap) {
+    if (tid == 0) *tab_n = kSmoothOverflow;
+    return;
+  }
+  __syncthreads();
+  for (uint32_t q = tid; q < nact; q += 1024) {
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (base[mid] <= q) lo = mid; else hi = mid - 1;
+    }
+    uint32_t r = q - base[lo], wd = 0;
+    for (; wd < 7; ++wd) {
+      const uint32_t c = uint32_t(__popcll(amask[lo * 8 + wd]));
+      if (r < c) break;
+      r -= c;
+    }
+    const uint32_t t = wd * 64 + select_bit(amask[lo * 8 + wd], r);
+    const uint32_t b = list[lo];
+    const uint32_t x = (b % g.BX) * 8 + (t & 7u), y = ((b / g.BX) % g.BY) * 8 + ((t >> 3) & 7u),
+                   z = (b / (g.BX * g.BY)) * 8 + (t >> 6);
+    const uint32_t nx[6] = {x + 1, x - 1, x, x, x, x}, ny[6] = {y, y, y + 1, y - 1, y, y}, nz[6] = {z, z, z, z, z + 1, z - 1};
+    uint32_t nb[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int sl = slot[(nx[k] >> 3) + g.BX * ((ny[k] >> 3) + g.BY * (nz[k] >> 3))];
+      nb[k] = kSmoothCap;
+      if (sl < 0) continue;
+      const uint32_t tn = ((nz[k] & 7u) << 6) | ((ny[k] & 7u) << 3) | (nx[k] & 7u);
+      if ((amask[uint32_t(sl) * 8 + (tn >> 6)] >> (tn & 63u)) & 1ull) nb[k] = base[sl] + rank_in_brick(amask, uint32_t(sl), tn);
+    }
+    tab[q] = make_uint4((b << 9) | t, nb[0] | (nb[1] << 16), nb[2] | (nb[3] << 16), nb[4] | (nb[5] << 16));
+  }
+  // the inactive in-domain voxels of the listed bricks hold 0 in both buffers
+  for (uint32_t q = tid; q < n * 512u; q += 1024) {
+    const uint32_t k = q >> 9, t = q & 511u;
+    if ((amask[k * 8 + (t >> 6)] >> (t & 63u)) & 1ull) continue;
+    const uint32_t b = list[k];
+    const uint32_t x = (b % g.BX) * 8 + (t & 7u), y = ((b / g.BX) % g.BY) * 8 + ((t >> 3) & 7u), z = (b / (g.BX * g.BY)) * 8 + (t >> 6);
+    if (x < g.W && y < g.H && z < g.D) { p0[(b << 9) | t] = 0.f; p1[(b << 9) | t] = 0.f; }
+  }
+  if (tid == 0) *tab_n = nact;
+}
+
+__global__ __launch_bounds__(1024) void k_smooth_block(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                       LvMask m, const StepConst* __restrict__ C, const float* __restrict__ rhs,
+                                                       float* p0, float* p1, int sweeps, int weighted, int zero_start,
+                                                       const uint4* __restrict__ tab, const uint32_t* __restrict__ tab_n) {
+  __shared__ float sp[2][kSmoothCap + 1];  // slot kSmoothCap holds 0: the missing neighbour
+  const float om = C->omega;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nact = tab ? *tab_n : kSmoothOverflow;
+  if (nact != kSmoothOverflow) {
+    uint32_t gid[kSmoothPer], nbr[kSmoothPer][6];
+    float rv[kSmoothPer], pv[kSmoothPer];
 #pragma unroll
     for (int j = 0; j < kSmoothPer; ++j) {
       const uint32_t q = tid + uint32_t(j) * 1024u;
-      gid[j] = 0xffffffffu;
-      if (q >= nact) continue;
-      uint32_t lo = 0, hi = n - 1;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (base[mid] <= q) lo = mid; else hi = mid - 1;
-      }
-      uint32_t r = q - base[lo], wd = 0;
-      for (; wd < 7; ++wd) {
-        const uint32_t c = uint32_t(__popcll(amask[lo * 8 + wd]));
-        if (r < c) break;
-        r -= c;
-      }
-      const uint32_t t = wd * 64 + select_bit(amask[lo * 8 + wd], r);
-      const uint32_t b = list[lo];
-      gid[j] = (b << 9) | t;
-      const uint32_t x = (b % g.BX) * 8 + (t & 7u), y = ((b / g.BX) % g.BY) * 8 + ((t >> 3) & 7u),
-                     z = (b / (g.BX * g.BY)) * 8 + (t >> 6);
-      const uint32_t nx[6] = {x + 1, x - 1, x, x, x, x}, ny[6] = {y, y, y + 1, y - 1, y, y}, nz[6] = {z, z, z, z, z + 1, z - 1};
-#pragma unroll
-      for (int k = 0; k < 6; ++k) {
-        const int sl = slot[(nx[k] >> 3) + g.BX * ((ny[k] >> 3) + g.BY * (nz[k] >> 3))];
-        nbr[j][k] = int(kSmoothCap);
-        if (sl < 0) continue;
-        const uint32_t tn = ((nz[k] & 7u) << 6) | ((ny[k] & 7u) << 3) | (nx[k] & 7u);
-        if ((amask[uint32_t(sl) * 8 + (tn >> 6)] >> (tn & 63u)) & 1ull) nbr[j][k] = int(base[sl] + rank_in_brick(amask, uint32_t(sl), tn));
-      }
+      const uint4 e = q < nact ? tab[q] : make_uint4(0xffffffffu, 0u, 0u, 0u);
+      gid[j] = e.x;
+      nbr[j][0] = e.y & 0xffffu; nbr[j][1] = e.y >> 16;
+      nbr[j][2] = e.z & 0xffffu; nbr[j][3] = e.z >> 16;
+      nbr[j][4] = e.w & 0xffffu; nbr[j][5] = e.w >> 16;
     }
-    // the right-hand side and the start value of every owned slot (all loads in
-    // flight together) stay in registers; LDS carries the values neighbours read
-    float rv[kSmoothPer], pv[kSmoothPer];
+    // the right-hand side and the start value of every owned slot stay in
+    // registers; LDS carries the values neighbours read
 #pragma unroll
     for (int j = 0; j < kSmoothPer; ++j) {
       const bool own = gid[j] != 0xffffffffu;
@@ -958,16 +995,9 @@ __global__ __launch_bounds__(1024) void k_smooth_block(const uint32_t* __restric
 #pragma unroll
     for (int j = 0; j < kSmoothPer; ++j)
       if (gid[j] != 0xffffffffu) dst[gid[j]] = pv[j];
-    // every other in-domain voxel of the listed bricks is 0
-    for (uint32_t q = tid; q < total; q += 1024) {
-      const uint32_t k = q >> 9, t = q & 511u;
-      if ((amask[k * 8 + (t >> 6)] >> (t & 63u)) & 1ull) continue;
-      const uint32_t b = list[k];
-      const uint32_t x = (b % g.BX) * 8 + (t & 7u), y = ((b / g.BX) % g.BY) * 8 + ((t >> 3) & 7u), z = (b / (g.BX * g.BY)) * 8 + (t >> 6);
-      if (x < g.W && y < g.H && z < g.D) dst[(b << 9) | t] = 0.f;
-    }
     return;
   }
+  const uint32_t total = *count * 512u;
   for (int sw = 0; sw < sweeps; ++sw) {
     const float* src = (sw & 1) ? p1 : p0;
     float* dst = (sw & 1) ? p0 : p1;
@@ -1551,9 +1581,18 @@ __device__ __forceinline__ F3 smoke_albedo(float den, float tem, float hot) {
   float soot = smoothf(0.16f, 0.9f, den) * (1.f - smoothf(0.2f, 0.74f, hot));
   return mix3(mix3(f3(0.26f, 0.275f, 0.295f), f3(0.04f, 0.041f, 0.045f), soot * 0.86f), f3(0.36f, 0.27f, 0.18f), warm * (0.35f + hot * 0.45f));
 }
+// Shading transcendentals on the hardware base-2 units (v_exp_f32 / v_log_f32,
+// about 1 ulp): the image is FP16 and its parity tolerance is 2e-3, so the
+// correctly rounded library forms buy nothing here and cost ~100 instructions.
+__device__ __forceinline__ float exp_hw(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+// x^y for x >= 0, y > 0 (0^y = 0).
+__device__ __forceinline__ float pow_hw(float x, float y) {
+  return x > 0.f ? __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)) : 0.f;
+}
 __device__ __forceinline__ float phase_hg(float c, float gg) {
   float g2 = gg * gg;
-  float denom = powf(fmaxf(1.f + g2 - 2.f * gg * c, 0.001f), 1.5f);
+  float base = fmaxf(1.f + g2 - 2.f * gg * c, 0.001f);
+  float denom = base * sqrtf(base);  // base^1.5
   return (1.f - g2) / (12.566370614359172f * denom);
 }
 
@@ -1581,7 +1620,9 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, SS<ST> S, 
     uint32_t b = bx + g.BX * (by + g.BY * bz);
     return lds ? ((sbits[b >> 5] >> (b & 31u)) & 1u) != 0u : flags[b] != 0u;
   };
-  const uint32_t px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
+  // each wave64 marches a compact 8x8 pixel tile (coherent rays), 2x2 tiles per block
+  const uint32_t px = blockIdx.x * 16 + ((threadIdx.x >> 6) & 1u) * 8 + (threadIdx.x & 7u),
+                 py = blockIdx.y * 16 + (threadIdx.x >> 7) * 8 + ((threadIdx.x >> 3) & 7u);
   if (px >= R.width || py >= R.height) return;
   __half* o = reinterpret_cast<__half*>(out) + (size_t(py) * R.width + px) * 4;
   const F3 dims = f3(float(g.W) - 1.f, float(g.H) - 1.f, float(g.D) - 1.f);
@@ -1615,6 +1656,20 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, SS<ST> S, 
     return 1.f / (1.f + dist * 0.09f + dist * dist * 0.02f);
   };
   F3 primary = nl > 0 ? lightDir(0, camPos) : f3(0.f, 1.f, 0.f);
+  // light 0 when directional: direction, power, phase terms and rim factor are
+  // fixed per ray (other lights are evaluated per sample)
+  const bool dir0 = nl > 0 && R.light_kind[0] <= 0.5f;
+  F3 dirL0 = f3(0.f, 1.f, 0.f);
+  float powL0 = 0.f, fsL0 = 0.f, bsL0 = 0.f, rimL0 = 0.f;
+  if (dir0) {
+    dirL0 = lightDir(0, camPos);
+    float raw = fmaxf(R.light_intensity[0], 0.f);
+    powL0 = raw / (1.f + raw * 0.18f);
+    float lvc = clampf(dot3(dirL0, -rayDir), -1.f, 1.f);
+    fsL0 = phase_hg(lvc, R.scat_f) * 5.2f * powL0;
+    bsL0 = phase_hg(lvc, R.scat_b) * 1.55f * powL0;
+    rimL0 = pow_hw(clampf(dot3(dirL0, -rayDir) * 0.5f + 0.5f, 0.f, 1.f), 1.2f);
+  }
   F3 lightStep = mul3(primary, dims) * 0.036f;
   F3 vp = mul3(mul3(camPos + rayDir * bx - bmin, invExt), dims);
   float travelled = 0.f;
@@ -1628,9 +1683,8 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, SS<ST> S, 
     if (alpha > 0.985f) break;
     float vstep = voxDelta;
     F3 sp = vp;
-    F3 uvw = f3(sp.x / dims.x, sp.y / dims.y, sp.z / dims.z);
-    uint32_t ax = uint32_t(floorf(clampf(sp.x, 0.f, dims.x))), ay = uint32_t(floorf(clampf(sp.y, 0.f, dims.y))),
-             az = uint32_t(floorf(clampf(sp.z, 0.f, dims.z)));
+    const float cxf = clampf(sp.x, 0.f, dims.x), cyf = clampf(sp.y, 0.f, dims.y), czf = clampf(sp.z, 0.f, dims.z);
+    const uint32_t ax = uint32_t(floorf(cxf)), ay = uint32_t(floorf(cyf)), az = uint32_t(floorf(czf));
     if (active_brick(ax, ay, az)) {
       Tri t = tri(g, sp);
       float den = fmaxf(tri_evalP<ST>(t, S.d) - 0.004f, 0.f);
@@ -1648,7 +1702,8 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, SS<ST> S, 
           float gl = fmaxf(len3(dg), 0.0001f);
           F3 normal = gl > 0.00011f ? (-dg) / gl : f3(0.f, 1.f, 0.f);
           float detail = clampf(gl * 5.2f + hot * 0.28f + rea * 0.18f, 0.f, 1.f);
-          float dd = fmaxf(powf(den, 0.96f) * mixf(0.82f, 1.42f, detail), 0.f);
+          float dd = fmaxf(pow_hw(den, 0.96f) * mixf(0.82f, 1.42f, detail), 0.f);
+          F3 uvw = f3(sp.x / dims.x, sp.y / dims.y, sp.z / dims.z);
           F3 wp = f3(mixf(bmin.x, bmax.x, uvw.x), mixf(bmin.y, bmax.y, uvw.y), mixf(bmin.z, bmax.z, uvw.z));
           float topFade = 1.f - smoothf(0.9f, 0.995f, uvw.y);
           F3 total = f3(0, 0, 0);
@@ -1662,26 +1717,35 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, SS<ST> S, 
             float d2 = fmaxf(nearest(sp + lightStep * 2.f) - 0.004f, 0.f);
             float d3 = fmaxf(nearest(sp + lightStep * 3.6f) - 0.004f, 0.f);
             float od = dd * 0.34f + d1 * 1.15f + d2 * 0.72f + d3 * 0.4f - hot * 0.06f;
-            primT = expf(-fmaxf(od, 0.f) * 2.15f);
+            primT = exp_hw(-fmaxf(od, 0.f) * 2.15f);
           }
+          const float rimView = pow_hw(1.f - fabsf(dot3(normal, -rayDir)), 2.4f);
+#pragma unroll
           for (uint32_t li = 0; li < 3; ++li) {
-            if (li >= nl) break;
-            F3 dl = lightDir(int(li), wp);
-            float raw = fmaxf(R.light_intensity[li], 0.f) * lightAtt(int(li), wp);
-            float power = raw / (1.f + raw * 0.18f);
+            if (li >= nl) continue;
+            F3 dl;
+            float power, fs, bs, rimLight;
+            if (li == 0 && dir0) {  // per-ray constants
+              dl = dirL0; power = powL0; fs = fsL0; bs = bsL0; rimLight = rimL0;
+            } else {
+              dl = lightDir(int(li), wp);
+              float raw = fmaxf(R.light_intensity[li], 0.f) * lightAtt(int(li), wp);
+              power = raw / (1.f + raw * 0.18f);
+              float lvc = clampf(dot3(dl, -rayDir), -1.f, 1.f);
+              fs = phase_hg(lvc, R.scat_f) * 5.2f * power;
+              bs = phase_hg(lvc, R.scat_b) * 1.55f * power;
+              rimLight = pow_hw(clampf(dot3(dl, -rayDir) * 0.5f + 0.5f, 0.f, 1.f), 1.2f);
+            }
             F3 color = f3(R.light_color[li][0], R.light_color[li][1], R.light_color[li][2]);
-            float lvc = clampf(dot3(dl, -rayDir), -1.f, 1.f);
-            float fs = phase_hg(lvc, R.scat_f) * 5.2f * power;
-            float bs = phase_hg(lvc, R.scat_b) * 1.55f * power;
             float nf = clampf(dot3(normal, dl) * 0.5f + 0.5f, 0.f, 1.f);
-            float surface = powf(nf, 1.35f) * mixf(1.f, 0.56f, smoothf(0.16f, 0.9f, dd));
-            float rim = powf(1.f - fabsf(dot3(normal, -rayDir)), 2.4f) * powf(clampf(dot3(dl, -rayDir) * 0.5f + 0.5f, 0.f, 1.f), 1.2f);
+            float surface = pow_hw(nf, 1.35f) * mixf(1.f, 0.56f, smoothf(0.16f, 0.9f, dd));
+            float rim = rimView * rimLight;
             float directional = (surface * 0.42f + rim * 0.22f) * power;
             float scatter = (fs + bs) * mixf(0.36f, 0.74f, hot);
             float trans = li > 0 ? 1.f : primT;
             total = total + color * trans * (directional + scatter);
           }
-          F3 tl = f3(1.f - expf(-total.x * 0.72f), 1.f - expf(-total.y * 0.72f), 1.f - expf(-total.z * 0.72f));
+          F3 tl = f3(1.f - exp_hw(-total.x * 0.72f), 1.f - exp_hw(-total.y * 0.72f), 1.f - exp_hw(-total.z * 0.72f));
           F3 smoke = smoke_albedo(dd, tem, hot);
           float absorb = mixf(0.92f, 0.34f, smoothf(0.26f, 1.05f, dd));
           float heat = clampf(tem * 1.25f + fu * 0.18f + rea * 0.34f, 0.f, 1.f);
@@ -1694,13 +1758,13 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, SS<ST> S, 
           float stepDistance = worldDelta * mixf(1.02f, 0.76f, contribution);
           vstep = voxDelta * mixf(1.02f, 0.76f, contribution);
           F3 comp = lit + emissive;
-          float opacity = (1.f - expf(-(dd * 4.75f + hot * 0.42f) * stepDistance)) * topFade;
+          float opacity = (1.f - exp_hw(-(dd * 4.75f + hot * 0.42f) * stepDistance)) * topFade;
           if (R.mode > 0.5f && R.mode < 1.5f) {
             comp = mix3(f3(0.01f, 0.01f, 0.012f), f3(0.88f, 0.9f, 0.94f), clampf(dd * 1.4f, 0.f, 1.f));
-            opacity = 1.f - expf(-dd * stepDistance * 8.6f);
+            opacity = 1.f - exp_hw(-dd * stepDistance * 8.6f);
           } else if (R.mode >= 1.5f) {
             comp = mix3(f3(0.04f, 0.012f, 0.f), f3(1.f, 0.72f, 0.22f), clampf(fu * 1.3f, 0.f, 1.f));
-            opacity = 1.f - expf(-fu * stepDistance * 7.f);
+            opacity = 1.f - exp_hw(-fu * stepDistance * 7.f);
           }
           acc = acc + comp * ((1.f - alpha) * opacity);
           alpha = alpha + (1.f - alpha) * opacity;
@@ -1831,9 +1895,15 @@ void launch_jacobi(hipStream_t s, int blocks, const uint32_t* list, const uint32
   FX_LAUNCH(k_jacobi, blocks, 512, s, list, count, g, m, c, rhs, src, dst, weighted);
 }
 void launch_level_masks(hipStream_t s, int blocks, const LevelMaskArgs& a) { FX_LAUNCH(k_level_masks, blocks, 512, s, a); }
-void launch_smooth_block(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, LvBits b,
-                         const StepConst* c, const float* rhs, float* p0, float* p1, int sweeps, int weighted, int zero_start) {
-  FX_LAUNCH(k_smooth_block, 1, 1024, s, list, count, g, m, b, c, rhs, p0, p1, sweeps, weighted, zero_start);
+void launch_coarse_setup(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, void* tab, uint32_t* tab_n,
+                         float* p0, float* p1) {
+  FX_LAUNCH(k_coarse_setup, 1, 1024, s, list, count, g, b, static_cast<uint4*>(tab), tab_n, p0, p1);
+}
+void launch_smooth_block(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const StepConst* c,
+                         const float* rhs, float* p0, float* p1, int sweeps, int weighted, int zero_start, const void* tab,
+                         const uint32_t* tab_n) {
+  FX_LAUNCH(k_smooth_block, 1, 1024, s, list, count, g, m, c, rhs, p0, p1, sweeps, weighted, zero_start,
+            static_cast<const uint4*>(tab), tab_n);
 }
 bool launch_vdown(hipStream_t s, int blocks, int pre, const uint32_t* list, const uint32_t* count, Grid f, LvBits bf, Grid c,
                   LvMask mc, const StepConst* cst, const float* rhs, const float* p_in, float* p_out, float* crhs, double* norm2) {
