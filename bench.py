@@ -140,6 +140,8 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=10, help="extra steps timed per stage with HIP events")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default V-cycle kernel plan instead of timing the candidates after warm-up")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of this many MiB (a known-size reference for the "
                          "HBM traffic counters)")
@@ -172,6 +174,7 @@ def main():
     for k in range(args.warmup):
         full_step(k)
     sim.synchronize()
+    plan = sim.autotune_vcycle(4) if args.autotune else sim.vcycle_plan()
     active0 = sim.active_brick_count()
     t0 = time.perf_counter()
     for k in range(args.warmup, args.warmup + args.steps):
@@ -222,6 +225,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": wl["label"], "storage": args.precision, "active_bricks": active,
+                   "vcycle_plan": {"down_split": plan[0], "up_split": plan[1], "autotuned": args.autotune},
                    "active_fraction": active / sim.brick_count, "render": f"{w}x{h}x{sc}"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -187,6 +187,9 @@ FUNCTIONS = {
     "fx_raymarch_device": (C.c_int, [P, C.POINTER(Camera), C.POINTER(RenderParams), u32, u32, C.c_void_p]),
     "fx_set_stage_timing": (C.c_int, [P, i32]),
     "fx_read_stage_times": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(u32), C.c_size_t]),
+    "fx_set_vcycle_plan": (C.c_int, [P, u32, u32]),
+    "fx_get_vcycle_plan": (C.c_int, [P, C.POINTER(u32), C.POINTER(u32)]),
+    "fx_autotune_vcycle": (C.c_int, [P, u32, C.POINTER(u32), C.POINTER(u32)]),
     "fx_run_pass": (C.c_int, [P, i32, C.POINTER(SimParams)]),
     "fx_last_error": (C.c_char_p, [P]),
 }

@@ -102,10 +102,27 @@ struct fx_sim {
   // kernels instead of the fused LDS legs). Part of the graph key.
   uint32_t split_down = 0, split_up = 0;
 
-  // hipGraph of the step body (rebuilt when the recorded sequence changes).
+  // hipGraphs of the step body, split where the scalars and render flags are
+  // final (after combustion); rebuilt when the recorded sequence changes.
   bool use_graph = true;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr, graph_b = nullptr;
+  hipGraphExec_t graph_exec = nullptr, graph_exec_b = nullptr;
+
+  // Rendering runs on its own stream, overlapped with the simulation: after
+  // combustion (where density, temperature, fuel, reaction and the render flags
+  // are final for the step) the simulation stream packs them into a snapshot
+  // (d_render, d_bits) and records ev_scalars; the raymarch reads only the
+  // snapshot, so it overlaps the rest of the step and the first half of the
+  // next one. ev_render marks the last raymarch done; the next pack waits for it.
+  hipStream_t rstream = nullptr;
+  hipEvent_t ev_scalars = nullptr, ev_render = nullptr;
+  void* d_render = nullptr;      // float4 per voxel
+  uint32_t* d_rqueue = nullptr;  // raymarch tile counter
+  int ray_blocks = 1;            // resident raymarch workgroups
+  bool render_overlap = true;    // false: the raymarch runs on the simulation stream
+  bool render_pending = false;
+  bool scalars_marked = false;  // the snapshot matches all simulation work enqueued so far
+  bool render_full = true;      // the next pack rewrites every brick (lists changed outside fx_step)
   std::vector<int64_t> graph_key;
   int graph_last_pressure = 0;
 
@@ -279,6 +296,7 @@ bool debug_each_frame(const fx_sim* s) {
 struct StageTimer {
   fx_sim* s;
   int id;
+  hipStream_t st;
   hipEvent_t a = nullptr, b = nullptr;
   static hipEvent_t take(fx_sim* s) {
     if (!s->pool.empty()) { hipEvent_t e = s->pool.back(); s->pool.pop_back(); return e; }
@@ -286,14 +304,14 @@ struct StageTimer {
     if (hipEventCreate(&e) != hipSuccess) return nullptr;
     return e;
   }
-  StageTimer(fx_sim* sim, int stage) : s(sim), id(stage) {
+  StageTimer(fx_sim* sim, int stage, hipStream_t on = nullptr) : s(sim), id(stage), st(on ? on : sim->stream) {
     if (!s->timing) return;
     a = take(s); b = take(s);
-    if (a) (void)hipEventRecord(a, s->stream);
+    if (a) (void)hipEventRecord(a, st);
   }
   ~StageTimer() {
     if (!s->timing || !a || !b) return;
-    (void)hipEventRecord(b, s->stream);
+    (void)hipEventRecord(b, st);
     s->recs.push_back(fx_sim::Rec{id, a, b});
   }
 };
@@ -645,14 +663,19 @@ int read_brickmajor(fx_sim* s, const Grid& g, const float* dev, float* host, siz
   return FX_OK;
 }
 
-// The pass sequence of one step (ORCH:270-301) on the simulation stream.
-void record_step(fx_sim* s, bool vort, bool pres, bool dbg) {
+// The pass sequence of one step (ORCH:270-301) on the simulation stream, in
+// two parts: (a) up to combustion, after which the render scalars and flags are
+// final, and (b) the rest, which reads them but writes only velocity, vorticity,
+// pressure and diagnostics.
+void record_step_a(fx_sim* s) {
   pass_inject(s, 7);                                   // ORCH:270
   pass_activate_render(s);                             // ORCH:271-272
   run_stage(s, FX_PASS_ADVECT_VELOCITY);               // ORCH:273
   run_stage(s, FX_PASS_ADVECT_SCALARS);                // ORCH:274-279
   if (!s->sparse) pass_activate_advected(s);           // ORCH:280-281
   run_stage(s, FX_PASS_COMBUSTION);                    // ORCH:282
+}
+void record_step_b(fx_sim* s, bool vort, bool pres, bool dbg) {
   run_stage(s, FX_PASS_BUOYANCY);                      // ORCH:283
   if (vort) {                                          // ORCH:285-288
     run_stage(s, FX_PASS_VORTICITY);
@@ -669,19 +692,51 @@ void record_step(fx_sim* s, bool vort, bool pres, bool dbg) {
   if (dbg) run_stage(s, FX_PASS_DEBUG_FIELDS);         // ORCH:298-301
 }
 
-// Captures record_step into a hipGraph (kernel nodes only: every size the
-// kernels need is read from device memory, so one graph serves every step).
+// Captures the two step parts into hipGraphs (kernel nodes only: every size the
+// kernels need is read from device memory, so one pair serves every step).
 int capture_step(fx_sim* s, bool vort, bool pres, bool dbg) {
   if (s->graph_exec) { (void)hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
   if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+  if (s->graph_exec_b) { (void)hipGraphExecDestroy(s->graph_exec_b); s->graph_exec_b = nullptr; }
+  if (s->graph_b) { (void)hipGraphDestroy(s->graph_b); s->graph_b = nullptr; }
   HIP_OK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-  record_step(s, vort, pres, dbg);
+  record_step_a(s);
   hipGraph_t g = nullptr;
   HIP_OK(s, hipStreamEndCapture(s->stream, &g));
   s->graph = g;
   HIP_OK(s, hipGraphInstantiateWithFlags(&s->graph_exec, s->graph, 0));
+  HIP_OK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  record_step_b(s, vort, pres, dbg);
+  g = nullptr;
+  HIP_OK(s, hipStreamEndCapture(s->stream, &g));
+  s->graph_b = g;
+  HIP_OK(s, hipGraphInstantiateWithFlags(&s->graph_exec_b, s->graph_b, 0));
   s->graph_last_pressure = s->lastPressure;
   return FX_OK;
+}
+// The simulation stream waits for the last raymarch, then rewrites the render
+// snapshot: the listed bricks and this step's retired ones, or every brick.
+int pack_render(fx_sim* s, bool full) {
+  if (s->render_pending) {
+    HIP_OK(s, hipStreamWaitEvent(s->stream, s->ev_render, 0));
+    s->render_pending = false;
+  }
+  {
+    StageTimer timer(s, FX_TIME_RAYMARCH);
+    const bool all = full || !s->sparse;
+    launch_render_pack(s->stream, grid_for(s, s->nb), all ? s->identity : s->lv[0].L.list,
+                       all ? s->identity_count : s->lv[0].L.count, set_of(s, 0), all ? nullptr : s->lv[0].L.retire,
+                       all ? nullptr : s->lv[0].L.retire_count, s->d_render, s->flagsRender, uint32_t(s->nb), s->d_bits);
+  }
+  HIP_OK(s, hipEventRecord(s->ev_scalars, s->stream));
+  s->render_full = false;
+  s->scalars_marked = true;
+  return FX_OK;
+}
+// Simulation work outside fx_step: the snapshot is stale and lists may have changed.
+void snapshot_stale(fx_sim* s) {
+  s->scalars_marked = false;
+  s->render_full = true;
 }
 
 }  // namespace
@@ -750,7 +805,19 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, s->device) != hipSuccess) return bail(fail(s, FX_ERR_DEVICE, "hipGetDeviceProperties failed"));
   s->blocks = std::max(1, prop.multiProcessorCount * 4);
-  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(s, FX_ERR_DEVICE, "stream creation failed"));
+  {
+    // resident raymarch workgroups per CU (256 threads each); FENIX_RAY_BLOCKS_PER_CU overrides
+    const char* r = std::getenv("FENIX_RAY_BLOCKS_PER_CU");
+    const int per_cu = r ? std::max(1, std::atoi(r)) : 1;
+    s->ray_blocks = std::max(1, prop.multiProcessorCount * per_cu);
+    const char* o = std::getenv("FENIX_RENDER_OVERLAP");
+    s->render_overlap = !(o && o[0] == '0');
+  }
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->rstream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&s->ev_scalars, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->ev_render, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(s, FX_ERR_DEVICE, "stream creation failed"));
 
   const size_t nsv = storage_count(s->g);
   int rc = FX_OK;
@@ -832,13 +899,16 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
   {
     const char* g = std::getenv("FENIX_GRAPH");
     s->use_graph = !(g && g[0] == '0');
-    const char* d = std::getenv("FENIX_VSPLIT_DOWN");
-    const char* u = std::getenv("FENIX_VSPLIT_UP");
-    if (d) s->split_down = uint32_t(std::strtoul(d, nullptr, 0));
-    if (u) s->split_up = uint32_t(std::strtoul(u, nullptr, 0));
+
   }
   s->norm_cap = std::max(1, desc->vcycle.cycles);
   if (dalloc(s, &s->d_norms, size_t(s->norm_cap) * kNormLanes) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
+  if (dalloc(s, &s->d_rqueue, 1) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
+  {
+    float* rb = nullptr;
+    if (dalloc(s, &rb, storage_count(s->g) * 4) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
+    s->d_render = rb;
+  }
   s->src_cap = FX_FLOATS_PER_EMITTER;
   if (dalloc(s, &s->d_src, s->src_cap) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
   for (auto& slot : s->slots) {
@@ -879,13 +949,14 @@ void fx_destroy(fx_sim* s) {
   if (!s) return;
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->rstream) (void)hipStreamSynchronize(s->rstream);
   auto F = [](void* p) { if (p) (void)hipFree(p); };
   for (int k = 0; k < 2; ++k) for (int f = 0; f < 4; ++f) F(s->S[k][f]);
   for (int c = 0; c < 3; ++c) { F(s->u[c]); F(s->us[c]); F(s->w[c]); }
   F(s->wmag); F(s->cmag); F(s->umag);
   F(s->flagsInj);
   if (!s->sparse) F(s->flagsRender);
-  F(s->touched); F(s->d_bits); F(s->identity); F(s->identity_count); F(s->d_const); F(s->d_cand); F(s->d_src); F(s->d_norms);
+  F(s->touched); F(s->d_bits); F(s->d_rqueue); F(s->d_render); F(s->identity); F(s->identity_count); F(s->d_const); F(s->d_cand); F(s->d_src); F(s->d_norms);
   for (size_t l = 0; l < s->lv.size(); ++l) {
     Level& L = s->lv[l];
     F(L.div); F(L.p[0]); F(L.p[1]); F(L.bits); F(L.tab); F(L.tab_n); F(L.L.list); F(L.L.count); F(L.L.page); F(L.L.retire); F(L.L.retire_count);
@@ -898,9 +969,14 @@ void fx_destroy(fx_sim* s) {
   }
   if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
+  if (s->graph_exec_b) (void)hipGraphExecDestroy(s->graph_exec_b);
+  if (s->graph_b) (void)hipGraphDestroy(s->graph_b);
   F(s->d_compact);
   drain_timing(s);
   for (hipEvent_t e : s->pool) (void)hipEventDestroy(e);
+  if (s->ev_scalars) (void)hipEventDestroy(s->ev_scalars);
+  if (s->ev_render) (void)hipEventDestroy(s->ev_render);
+  if (s->rstream) (void)hipStreamDestroy(s->rstream);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -908,6 +984,7 @@ void fx_destroy(fx_sim* s) {
 int fx_reset(fx_sim* s) {  // ORCH:206-226
   if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
   HIP_OK(s, hipSetDevice(s->device));
+  snapshot_stale(s);
   s->curSet = 0; s->stepIndex = 0; s->lastPressure = 0; s->initialized = false;
   s->startS = s->lastS;
   s->pending.clear();
@@ -961,8 +1038,14 @@ int fx_step(fx_sim* s, double elapsed_s, double step_s) {  // ORCH:230-305
       s->lastPressure = s->graph_last_pressure;
     }
     HIP_OK(s, hipGraphLaunch(s->graph_exec, s->stream));
+    rc = pack_render(s, s->render_full);
+    if (rc != FX_OK) return rc;
+    HIP_OK(s, hipGraphLaunch(s->graph_exec_b, s->stream));
   } else {
-    record_step(s, vort, pres, dbg);
+    record_step_a(s);
+    rc = pack_render(s, s->render_full);
+    if (rc != FX_OK) return rc;
+    record_step_b(s, vort, pres, dbg);
   }
   s->curSet = s->curSet == 0 ? 1 : 0;
   s->lastS = elapsed_s;
@@ -977,6 +1060,7 @@ int fx_run_pass(fx_sim* s, int32_t pass, const fx_sim_params* params) {
   std::vector<uint32_t> cand;
   uint32_t inj = 0;
   if (s->sparse) build_candidates(s, *params, cand, inj);
+  snapshot_stale(s);
   int rc = upload_step(s, *params, s->sparse ? &cand : nullptr, inj);
   if (rc != FX_OK) return rc;
   switch (pass) {
@@ -1002,7 +1086,9 @@ int fx_run_pass(fx_sim* s, int32_t pass, const fx_sim_params* params) {
 
 int fx_synchronize(fx_sim* s) {
   if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
+  HIP_OK(s, hipSetDevice(s->device));
   HIP_OK(s, hipStreamSynchronize(s->stream));
+  HIP_OK(s, hipStreamSynchronize(s->rstream));
   return FX_OK;
 }
 
@@ -1085,6 +1171,7 @@ int fx_write_field(fx_sim* s, int32_t id, const float* host, size_t bytes) {
   if (!f) return fail(s, FX_ERR_INVALID_ARG, "unknown field id");
   if (bytes != s->nv * 4) return fail(s, FX_ERR_INVALID_ARG, "buffer size mismatch");
   HIP_OK(s, hipSetDevice(s->device));
+  snapshot_stale(s);
   std::vector<float> tmp(storage_count(s->g), 0.f);
   scatter_brickmajor(s->g, host, tmp.data());
   HIP_OK(s, hipStreamSynchronize(s->stream));
@@ -1237,8 +1324,20 @@ int fx_raymarch_device(fx_sim* s, const fx_camera* c, const fx_render_params* p,
   int rc = build_ray_const(s, c, p, w, h, R);
   if (rc != FX_OK) return rc;
   HIP_OK(s, hipSetDevice(s->device));
-  StageTimer timer(s, FX_TIME_RAYMARCH);
-  launch_raymarch(s->stream, s->g, R, set_of(s, 0), s->flagsRender, s->d_bits, static_cast<uint16_t*>(dev));
+  if (!s->scalars_marked) {  // simulation work since the last step: snapshot everything now
+    rc = pack_render(s, true);
+    if (rc != FX_OK) return rc;
+  }
+  hipStream_t rs = s->render_overlap ? s->rstream : s->stream;
+  if (s->render_overlap) HIP_OK(s, hipStreamWaitEvent(s->rstream, s->ev_scalars, 0));
+  {
+    StageTimer timer(s, FX_TIME_RAYMARCH, rs);
+    launch_raymarch(rs, s->g, R, s->d_render, s->d_bits, s->d_rqueue, s->ray_blocks, static_cast<uint16_t*>(dev));
+  }
+  if (s->render_overlap) {
+    HIP_OK(s, hipEventRecord(s->ev_render, s->rstream));
+    s->render_pending = true;
+  }
   HIP_OK(s, hipGetLastError());
   return FX_OK;
 }
@@ -1250,7 +1349,7 @@ int fx_raymarch(fx_sim* s, const fx_camera* c, const fx_render_params* p, uint32
   HIP_OK(s, hipMalloc(&dev, size_t(w) * h * 8));
   int rc = fx_raymarch_device(s, c, p, w, h, dev);
   if (rc == FX_OK) {
-    hipError_t e = hipStreamSynchronize(s->stream);
+    hipError_t e = hipStreamSynchronize(s->rstream);
     if (e == hipSuccess) e = hipMemcpy(host, dev, size_t(w) * h * 8, hipMemcpyDeviceToHost);
     if (e != hipSuccess) rc = fail(s, FX_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
   }
@@ -1258,10 +1357,80 @@ int fx_raymarch(fx_sim* s, const fx_camera* c, const fx_render_params* p, uint32
   return rc;
 }
 
+int fx_set_vcycle_plan(fx_sim* s, uint32_t down_split, uint32_t up_split) {
+  if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
+  s->split_down = down_split;
+  s->split_up = up_split;
+  return FX_OK;
+}
+int fx_get_vcycle_plan(fx_sim* s, uint32_t* down_split, uint32_t* up_split) {
+  if (!s || !down_split || !up_split) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  *down_split = s->split_down;
+  *up_split = s->split_up;
+  return FX_OK;
+}
+int fx_autotune_vcycle(fx_sim* s, uint32_t repeats, uint32_t* down_split, uint32_t* up_split) {
+  if (!s || !down_split || !up_split) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  if (s->desc.solver != FX_SOLVER_VCYCLE) return fail(s, FX_ERR_UNSUPPORTED, "the autotuner plans the V-cycle solver");
+  if (s->timing) return fail(s, FX_ERR_UNSUPPORTED, "stage timing is enabled");
+  HIP_OK(s, hipSetDevice(s->device));
+  HIP_OK(s, hipStreamSynchronize(s->stream));
+  HIP_OK(s, hipStreamSynchronize(s->rstream));
+  repeats = std::max(1u, repeats);
+  hipEvent_t a = nullptr, b = nullptr;
+  HIP_OK(s, hipEventCreate(&a));
+  HIP_OK(s, hipEventCreate(&b));
+  // Mean time of one graph-captured solve under the current plan.
+  auto time_plan = [&](float* ms) -> int {
+    HIP_OK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    const int last = solve_vcycle(s);
+    hipGraph_t g = nullptr;
+    HIP_OK(s, hipStreamEndCapture(s->stream, &g));
+    hipGraphExec_t ge = nullptr;
+    hipError_t e = hipGraphInstantiateWithFlags(&ge, g, 0);
+    if (e == hipSuccess) e = hipGraphLaunch(ge, s->stream);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(a, s->stream);
+    for (uint32_t r = 0; r < repeats && e == hipSuccess; ++r) e = hipGraphLaunch(ge, s->stream);
+    if (e == hipSuccess) e = hipEventRecord(b, s->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, a, b);
+    if (ge) (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(s, FX_ERR_DEVICE, std::string("autotune: ") + hipGetErrorString(e));
+    *ms /= float(repeats);
+    s->lastPressure = last;
+    return FX_OK;
+  };
+  int rc = FX_OK;
+  const int L = int(s->lv.size());
+  for (int l = 0; l + 1 < L && rc == FX_OK; ++l) {
+    float best = 0.f;
+    uint32_t best_c = 0;
+    for (uint32_t c = 0; c < 4 && rc == FX_OK; ++c) {
+      s->split_down = (s->split_down & ~(1u << l)) | ((c & 1u) << l);
+      s->split_up = (s->split_up & ~(1u << l)) | (((c >> 1) & 1u) << l);
+      float ms = 0.f;
+      rc = time_plan(&ms);
+      if (rc == FX_OK && (c == 0 || ms < best)) { best = ms; best_c = c; }
+    }
+    s->split_down = (s->split_down & ~(1u << l)) | ((best_c & 1u) << l);
+    s->split_up = (s->split_up & ~(1u << l)) | (((best_c >> 1) & 1u) << l);
+  }
+  // leave the pressure buffers as the chosen plan computes them
+  float ms = 0.f;
+  if (rc == FX_OK) rc = time_plan(&ms);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *down_split = s->split_down;
+  *up_split = s->split_up;
+  return rc;
+}
+
 int fx_set_stage_timing(fx_sim* s, int32_t enable) {
   if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
   HIP_OK(s, hipSetDevice(s->device));
   HIP_OK(s, hipStreamSynchronize(s->stream));
+  HIP_OK(s, hipStreamSynchronize(s->rstream));
   drain_timing(s);
   std::memset(s->acc_ms, 0, sizeof(s->acc_ms));
   std::memset(s->acc_calls, 0, sizeof(s->acc_calls));
@@ -1273,6 +1442,7 @@ int fx_read_stage_times(fx_sim* s, double* total_ms, uint32_t* calls, size_t n) 
   if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
   HIP_OK(s, hipSetDevice(s->device));
   HIP_OK(s, hipStreamSynchronize(s->stream));
+  HIP_OK(s, hipStreamSynchronize(s->rstream));
   drain_timing(s);
   for (size_t i = 0; i < std::min<size_t>(n, FX_TIME_COUNT); ++i) {
     if (total_ms) total_ms[i] = s->acc_ms[i];

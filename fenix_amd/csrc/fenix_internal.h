@@ -209,8 +209,14 @@ struct RayConst {
   float light_color[3][3];
   uint32_t width, height;
 };
-// bits: scratch of ceil(bricks/32) words for the LDS copy of the flags (nullptr: read flags from memory).
-void launch_raymarch(hipStream_t s, Grid g, const RayConst& rc, Scalars set0, const uint32_t* flags, uint32_t* bits,
+// Render snapshot (16 bytes per voxel, storage order of the fields) of the listed
+// bricks, zero on the retire list (rlist may be nullptr), and the brick flags
+// packed 32 per word into bits.
+void launch_render_pack(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Scalars set0, const uint32_t* rlist,
+                        const uint32_t* rcount, void* rb, const uint32_t* flags, uint32_t nb, uint32_t* bits);
+// Raymarch of the snapshot rb with packed flags bits; queue: one device word
+// (tile counter); blocks: resident workgroups taking tiles.
+void launch_raymarch(hipStream_t s, Grid g, const RayConst& rc, const void* rb, const uint32_t* bits, uint32_t* queue, int blocks,
                      uint16_t* out);
 
 }  // namespace fx

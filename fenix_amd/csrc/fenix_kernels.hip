@@ -408,24 +408,36 @@ __global__ __launch_bounds__(512) void k_activate_threshold(const uint32_t* __re
                                                             const uint32_t* __restrict__ listB, const uint32_t* __restrict__ countB,
                                                             Grid g, const StepConst* __restrict__ C, const typename ST::T* __restrict__ den,
                                                             Vec3 vel, uint32_t* flags, LevelFlags lf) {
+  // One wave per brick (8 bricks per block): lane (x, y) of the brick reads its
+  // column of 8 voxels, ballots reduce; flags are only ever set to 1, so the
+  // 27 neighbour marks run on 27 lanes at once.
   const uint32_t na = *countA, nb = listB ? *countB : 0u;
   const float dth = C->act_density, sth = C->act_speed;
-  for (uint32_t k = blockIdx.x; k < na + nb; k += gridDim.x) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  for (uint32_t k = blockIdx.x * 8 + wave; k < na + nb; k += gridDim.x * 8) {
     const uint32_t b = k < na ? listA[k] : listB[k - na];
-    Vox v = vox_of(g, b);
-    int dens = 0, fast = 0;
-    if (v.in) {
-      F3 w = f3(vel.x[v.i], vel.y[v.i], vel.z[v.i]);
-      dens = ST::ld(den, v.i) > dth ? 1 : 0;
-      fast = len3(w) > sth ? 1 : 0;
+    const uint32_t bx = b % g.BX, by = (b / g.BX) % g.BY, bz = b / (g.BX * g.BY);
+    const uint32_t x = bx * 8 + (lane & 7u), y = by * 8 + (lane >> 3);
+    bool dens = false, fast = false;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) {
+      const uint32_t i = (b << 9) | (j << 6) | lane;
+      if (x < g.W && y < g.H && bz * 8 + j < g.D) {
+        dens |= ST::ld(den, i) > dth;
+        fast |= len3(f3(vel.x[i], vel.y[i], vel.z[i])) > sth;
+      }
     }
     // Density-active bricks dilate by one brick; velocity-only bricks stay
     // active without dilating (velocity lives inside the active set only).
-    const int any_dens = __syncthreads_or(dens);
-    const int any_fast = __syncthreads_or(fast);
-    if (threadIdx.x == 0) {
-      if (any_dens) dilate(g, b, 1, flags, lf);
-      else if (any_fast) mark(g, flags, lf, b % g.BX, (b / g.BX) % g.BY, b / (g.BX * g.BY));
+    if (__ballot(dens) != 0ull) {
+      if (lane < 27) {
+        const int nx = min(max(int(bx) + int(lane % 3) - 1, 0), int(g.BX) - 1);
+        const int ny = min(max(int(by) + int((lane / 3) % 3) - 1, 0), int(g.BY) - 1);
+        const int nz = min(max(int(bz) + int(lane / 9) - 1, 0), int(g.BZ) - 1);
+        mark(g, flags, lf, uint32_t(nx), uint32_t(ny), uint32_t(nz));
+      }
+    } else if (__ballot(fast) != 0ull && lane == 0) {
+      mark(g, flags, lf, bx, by, bz);
     }
   }
 }
@@ -1523,24 +1535,47 @@ __global__ __launch_bounds__(512) void k_sweep(const uint32_t* __restrict__ list
                                                const float* __restrict__ src, float* __restrict__ dst) {
   const float om = C->omega;
   const uint32_t t = threadIdx.x, lx = t & 7u, ly = (t >> 3) & 7u, lz = t >> 6;
-  const uint32_t BX = g.BX, BXY = g.BX * g.BY;
-  FX_BRICK_LOOP(list, count) {
-    const uint32_t b = list[fx_k];
-    const uint32_t i = (b << 9) | t;
-    if (!((lb.bits[size_t(fx_k) * 8 + lz] >> (t & 63u)) & 1ull)) { dst[i] = 0.f; continue; }
-    float sum = 0.f, self = 0.f;
-    if (!ZERO) {
-      const uint32_t xp = lx < 7u ? i + 1u : (((b + 1u) << 9) | (t - 7u));
-      const uint32_t xm = lx > 0u ? i - 1u : (((b - 1u) << 9) | (t + 7u));
-      const uint32_t yp = ly < 7u ? i + 8u : (((b + BX) << 9) | (t - 56u));
-      const uint32_t ym = ly > 0u ? i - 8u : (((b - BX) << 9) | (t + 56u));
-      const uint32_t zp = lz < 7u ? i + 64u : (((b + BXY) << 9) | (t - 448u));
-      const uint32_t zm = lz > 0u ? i - 64u : (((b - BXY) << 9) | (t + 448u));
-      sum = src[xp] + src[xm] + src[yp] + src[ym] + src[zp] + src[zm];
-      self = src[i];
+  const uint32_t BX = g.BX, BXY = g.BX * g.BY, last = g.BX * g.BY * g.BZ - 1u;
+  const uint32_t n = *count;
+  // Two bricks per round, every load issued before any result is needed; the
+  // neighbour addresses are clamped to the allocation so that they can be read
+  // before the activity word says whether the voxel uses them.
+  for (uint32_t k0 = blockIdx.x; k0 < n; k0 += 2u * gridDim.x) {
+    const uint32_t kk[2] = {k0, k0 + gridDim.x};
+    uint32_t ii[2];
+    uint64_t w[2];
+    float sum[2], self[2], r[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const bool ok = kk[u] < n;
+      const uint32_t b = list[ok ? kk[u] : k0];
+      const uint32_t i = (b << 9) | t;
+      ii[u] = i;
+      w[u] = ok ? lb.bits[size_t(kk[u]) * 8 + lz] : 0ull;
+      sum[u] = 0.f;
+      self[u] = 0.f;
+      if (!ZERO) {
+        const uint32_t xp = lx < 7u ? i + 1u : ((min(b + 1u, last) << 9) | (t - 7u));
+        const uint32_t xm = lx > 0u ? i - 1u : (((b > 0u ? b - 1u : 0u) << 9) | (t + 7u));
+        const uint32_t yp = ly < 7u ? i + 8u : ((min(b + BX, last) << 9) | (t - 56u));
+        const uint32_t ym = ly > 0u ? i - 8u : (((b >= BX ? b - BX : 0u) << 9) | (t + 56u));
+        const uint32_t zp = lz < 7u ? i + 64u : ((min(b + BXY, last) << 9) | (t - 448u));
+        const uint32_t zm = lz > 0u ? i - 64u : (((b >= BXY ? b - BXY : 0u) << 9) | (t + 448u));
+        sum[u] = src[xp] + src[xm] + src[yp] + src[ym] + src[zp] + src[zm];
+        self[u] = src[i];
+      }
+      r[u] = rhs[i];
     }
-    const float pj = (sum - rhs[i]) / 6.f;
-    dst[i] = mixf(self, pj, om);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      if (kk[u] >= n) continue;
+      float v = 0.f;
+      if ((w[u] >> (t & 63u)) & 1ull) {
+        const float pj = (sum[u] - r[u]) / 6.f;
+        v = mixf(self[u], pj, om);
+      }
+      dst[ii[u]] = v;
+    }
   }
 }
 
@@ -1605,24 +1640,18 @@ __global__ void k_pack_flags(const uint32_t* __restrict__ flags, uint32_t nb, ui
   bits[w] = v;
 }
 
-template <class ST>
-__global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, SS<ST> S, const uint32_t* __restrict__ flags,
-                                                  const uint32_t* __restrict__ bits, uint32_t nwords, uint16_t* out) {
-  extern __shared__ uint32_t sbits[];
-  // isActiveSample (RP:378-381) reads the brick flags from LDS when they fit.
-  const bool lds = bits != nullptr;
-  if (lds) {
-    for (uint32_t q = threadIdx.x; q < nwords; q += 256) sbits[q] = bits[q];
-    __syncthreads();
-  }
+// One ray (createVolumeRaymarchPass.ts fsMain) for pixel (px, py), reading the
+// render snapshot RB (density, temperature, fuel, reaction per voxel) and the
+// packed brick flags (sbits in LDS, else gbits in memory).
+__device__ __forceinline__ void march_pixel(const Grid& g, const RayConst& R, const float4* __restrict__ RB,
+                                            const uint32_t* __restrict__ gbits, const uint32_t* sbits, uint32_t px, uint32_t py,
+                                            uint16_t* out) {
+  // isActiveSample (RP:378-381)
   auto active_brick = [&](uint32_t x, uint32_t y, uint32_t z) {
     uint32_t bx = min(x >> 3, g.BX - 1), by = min(y >> 3, g.BY - 1), bz = min(z >> 3, g.BZ - 1);
     uint32_t b = bx + g.BX * (by + g.BY * bz);
-    return lds ? ((sbits[b >> 5] >> (b & 31u)) & 1u) != 0u : flags[b] != 0u;
+    return (((sbits ? sbits[b >> 5] : gbits[b >> 5]) >> (b & 31u)) & 1u) != 0u;
   };
-  // each wave64 marches a compact 8x8 pixel tile (coherent rays), 2x2 tiles per block
-  const uint32_t px = blockIdx.x * 16 + ((threadIdx.x >> 6) & 1u) * 8 + (threadIdx.x & 7u),
-                 py = blockIdx.y * 16 + (threadIdx.x >> 7) * 8 + ((threadIdx.x >> 3) & 7u);
   if (px >= R.width || py >= R.height) return;
   __half* o = reinterpret_cast<__half*>(out) + (size_t(py) * R.width + px) * 4;
   const F3 dims = f3(float(g.W) - 1.f, float(g.H) - 1.f, float(g.D) - 1.f);
@@ -1687,8 +1716,18 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, SS<ST> S, 
     const uint32_t ax = uint32_t(floorf(cxf)), ay = uint32_t(floorf(cyf)), az = uint32_t(floorf(czf));
     if (active_brick(ax, ay, az)) {
       Tri t = tri(g, sp);
-      float den = fmaxf(tri_evalP<ST>(t, S.d) - 0.004f, 0.f);
-      float tem = tri_evalP<ST>(t, S.t), fu = tri_evalP<ST>(t, S.f), rea = tri_eval(t, S.r);
+      float4 cr[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) cr[k] = RB[t.i[k]];
+      // the tri_eval order per channel
+      auto tri4 = [&](float c0, float c1, float c2, float c3, float c4, float c5, float c6, float c7) {
+        const float x00 = mixf(c0, c1, t.fx), x10 = mixf(c2, c3, t.fx), x01 = mixf(c4, c5, t.fx), x11 = mixf(c6, c7, t.fx);
+        return mixf(mixf(x00, x10, t.fy), mixf(x01, x11, t.fy), t.fz);
+      };
+      float den = fmaxf(tri4(cr[0].x, cr[1].x, cr[2].x, cr[3].x, cr[4].x, cr[5].x, cr[6].x, cr[7].x) - 0.004f, 0.f);
+      float tem = tri4(cr[0].y, cr[1].y, cr[2].y, cr[3].y, cr[4].y, cr[5].y, cr[6].y, cr[7].y);
+      float fu = tri4(cr[0].z, cr[1].z, cr[2].z, cr[3].z, cr[4].z, cr[5].z, cr[6].z, cr[7].z);
+      float rea = tri4(cr[0].w, cr[1].w, cr[2].w, cr[3].w, cr[4].w, cr[5].w, cr[6].w, cr[7].w);
       float hot = clampf(tem * 1.45f + fu * 0.22f + rea * 0.38f, 0.f, 1.f);
       if (den > 0.001f || hot > 0.018f) {
         if (den < 0.012f && hot < 0.04f) {
@@ -1696,9 +1735,9 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, SS<ST> S, 
         } else {
           uint32_t cx = min(uint32_t(fmaxf(rintf(sp.x), 0.f)), mxu), cy = min(uint32_t(fmaxf(rintf(sp.y), 0.f)), myu),
                    cz = min(uint32_t(fmaxf(rintf(sp.z), 0.f)), mzu);
-          F3 dg = f3(ST::ld(S.d, vidx(g, min(cx + 1, mxu), cy, cz)) - ST::ld(S.d, vidx(g, cx == 0 ? 0 : cx - 1, cy, cz)),
-                     ST::ld(S.d, vidx(g, cx, min(cy + 1, myu), cz)) - ST::ld(S.d, vidx(g, cx, cy == 0 ? 0 : cy - 1, cz)),
-                     ST::ld(S.d, vidx(g, cx, cy, min(cz + 1, mzu))) - ST::ld(S.d, vidx(g, cx, cy, cz == 0 ? 0 : cz - 1)));
+          F3 dg = f3(RB[vidx(g, min(cx + 1, mxu), cy, cz)].x - RB[vidx(g, cx == 0 ? 0 : cx - 1, cy, cz)].x,
+                     RB[vidx(g, cx, min(cy + 1, myu), cz)].x - RB[vidx(g, cx, cy == 0 ? 0 : cy - 1, cz)].x,
+                     RB[vidx(g, cx, cy, min(cz + 1, mzu))].x - RB[vidx(g, cx, cy, cz == 0 ? 0 : cz - 1)].x);
           float gl = fmaxf(len3(dg), 0.0001f);
           F3 normal = gl > 0.00011f ? (-dg) / gl : f3(0.f, 1.f, 0.f);
           float detail = clampf(gl * 5.2f + hot * 0.28f + rea * 0.18f, 0.f, 1.f);
@@ -1711,7 +1750,7 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, SS<ST> S, 
           if (nl > 0) {
             auto nearest = [&](F3 p) {
               F3 c = f3(clampf(p.x, 0.f, dims.x), clampf(p.y, 0.f, dims.y), clampf(p.z, 0.f, dims.z));
-              return ST::ld(S.d, vidx(g, min(uint32_t(floorf(c.x)), mxu), min(uint32_t(floorf(c.y)), myu), min(uint32_t(floorf(c.z)), mzu)));
+              return RB[vidx(g, min(uint32_t(floorf(c.x)), mxu), min(uint32_t(floorf(c.y)), myu), min(uint32_t(floorf(c.z)), mzu))].x;
             };
             float d1 = fmaxf(nearest(sp + lightStep * 0.9f) - 0.004f, 0.f);
             float d2 = fmaxf(nearest(sp + lightStep * 2.f) - 0.004f, 0.f);
@@ -1787,6 +1826,47 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, SS<ST> S, 
   }
   if (alpha < 0.01f) { o[0] = zero; o[1] = zero; o[2] = zero; o[3] = zero; return; }
   o[0] = __float2half(acc.x); o[1] = __float2half(acc.y); o[2] = __float2half(acc.z); o[3] = __float2half(alpha);
+}
+
+
+// Render snapshot: RB = (density, temperature, fuel, reaction) of the listed
+// bricks (FP32 whatever the storage), 0 on the bricks in the retire list. The
+// raymarch reads only the snapshot and the packed flags, so the next step may
+// rewrite the simulation fields while a render is still running.
+template <class ST>
+__global__ __launch_bounds__(512) void k_pack_render(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, SS<ST> S,
+                                                     const uint32_t* __restrict__ rlist, const uint32_t* __restrict__ rcount,
+                                                     float4* __restrict__ RB) {
+  FX_BRICK_LOOP(list, count) {
+    const uint32_t i = (list[fx_k] << 9) | threadIdx.x;
+    RB[i] = make_float4(ST::ld(S.d, i), ST::ld(S.t, i), ST::ld(S.f, i), S.r[i]);
+  }
+  if (!rlist) return;
+  const uint32_t nr = *rcount;
+  for (uint32_t k = blockIdx.x; k < nr; k += gridDim.x) RB[(rlist[k] << 9) | threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Work-queue raymarch: a bounded number of resident workgroups (so that the
+// simulation's own kernels keep CUs while both run) take 16x16-pixel tiles from
+// `queue` (zeroed before launch) until none are left; each wave64 marches a
+// compact 8x8 pixel tile (coherent rays), 2x2 of them per workgroup tile.
+__global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, const float4* __restrict__ RB, const uint32_t* __restrict__ gbits,
+                                                  uint32_t nwords, int use_lds, uint16_t* out, uint32_t* __restrict__ queue) {
+  extern __shared__ uint32_t sbits[];
+  __shared__ uint32_t tile;
+  if (use_lds)
+    for (uint32_t q = threadIdx.x; q < nwords; q += 256) sbits[q] = gbits[q];
+  const uint32_t tiles_x = (R.width + 15) / 16, ntiles = tiles_x * ((R.height + 15) / 16);
+  for (;;) {
+    __syncthreads();  // flags staged; the previous tile index has been read
+    if (threadIdx.x == 0) tile = atomicAdd(queue, 1u);
+    __syncthreads();
+    const uint32_t tl = tile;
+    if (tl >= ntiles) break;
+    const uint32_t px = (tl % tiles_x) * 16 + ((threadIdx.x >> 6) & 1u) * 8 + (threadIdx.x & 7u),
+                   py = (tl / tiles_x) * 16 + (threadIdx.x >> 7) * 8 + ((threadIdx.x >> 3) & 7u);
+    march_pixel(g, R, RB, gbits, use_lds ? sbits : nullptr, px, py, out);
+  }
 }
 
 // ------------------------------------------------------------ launchers --
@@ -1963,19 +2043,26 @@ void launch_projection(hipStream_t s, int blocks, const uint32_t* list, const ui
 void launch_debug(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 u, float* umag) {
   FX_LAUNCH(k_debug, blocks, 512, s, list, count, g, u, umag);
 }
-void launch_raymarch(hipStream_t s, Grid g, const RayConst& rc, Scalars set0, const uint32_t* flags, uint32_t* bits,
+void launch_render_pack(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Scalars set0, const uint32_t* rlist,
+                        const uint32_t* rcount, void* rb, const uint32_t* flags, uint32_t nb, uint32_t* bits) {
+  float4* RB = static_cast<float4*>(rb);
+  switch (set0.prec) {
+    case FX_PRECISION_BF16: FX_LAUNCH(k_pack_render<StBF16>, blocks, 512, s, list, count, typed<StBF16>(set0), rlist, rcount, RB); break;
+    case FX_PRECISION_FP16: FX_LAUNCH(k_pack_render<StF16>, blocks, 512, s, list, count, typed<StF16>(set0), rlist, rcount, RB); break;
+    default: FX_LAUNCH(k_pack_render<StF32>, blocks, 512, s, list, count, typed<StF32>(set0), rlist, rcount, RB);
+  }
+  const uint32_t nwords = (nb + 31) / 32;
+  FX_LAUNCH(k_pack_flags, (nwords + 255) / 256, 256, s, flags, nb, bits);
+}
+void launch_raymarch(hipStream_t s, Grid g, const RayConst& rc, const void* rb, const uint32_t* bits, uint32_t* queue, int blocks,
                      uint16_t* out) {
   const uint32_t nb = uint32_t(brick_count(g)), nwords = (nb + 31) / 32;
-  const bool lds = bits && nwords <= 8192;  // up to 32 KB of flag bits in LDS
-  if (lds) FX_LAUNCH(k_pack_flags, (nwords + 255) / 256, 256, s, flags, nb, bits);
-  dim3 grid((rc.width + 15) / 16, (rc.height + 15) / 16);
-  const uint32_t* b = lds ? bits : nullptr;
+  const int lds = nwords <= 8192 ? 1 : 0;  // up to 32 KB of flag bits in LDS
+  launch_clear_u32(s, queue, 1, 0u);
+  const uint32_t ntiles = ((rc.width + 15) / 16) * ((rc.height + 15) / 16);
+  dim3 grid(std::max(1u, std::min(ntiles, uint32_t(std::max(blocks, 1)))));
   const size_t sh = lds ? nwords * 4 : 0;
-  switch (set0.prec) {
-    case FX_PRECISION_BF16: hipLaunchKernelGGL(k_raymarch<StBF16>, grid, dim3(256), sh, s, g, rc, typed<StBF16>(set0), flags, b, nwords, out); break;
-    case FX_PRECISION_FP16: hipLaunchKernelGGL(k_raymarch<StF16>, grid, dim3(256), sh, s, g, rc, typed<StF16>(set0), flags, b, nwords, out); break;
-    default: hipLaunchKernelGGL(k_raymarch<StF32>, grid, dim3(256), sh, s, g, rc, typed<StF32>(set0), flags, b, nwords, out);
-  }
+  hipLaunchKernelGGL(k_raymarch, grid, dim3(256), sh, s, g, rc, static_cast<const float4*>(rb), bits, nwords, lds, out, queue);
 }
 
 }  // namespace fx

@@ -292,6 +292,21 @@ class CombustionVolumeSimulation:
         pid = abi.PASSES[name_or_id] if isinstance(name_or_id, str) else int(name_or_id)
         self._check(self._lib.fx_run_pass(self._h, pid, C.byref(params)))
 
+    def set_vcycle_plan(self, down_split: int, up_split: int) -> None:
+        """Per-level V-cycle kernel plan (bit l: split legs at level l); results are plan-independent."""
+        self._check(self._lib.fx_set_vcycle_plan(self._h, int(down_split), int(up_split)))
+
+    def vcycle_plan(self):
+        d, u = C.c_uint32(), C.c_uint32()
+        self._check(self._lib.fx_get_vcycle_plan(self._h, C.byref(d), C.byref(u)))
+        return d.value, u.value
+
+    def autotune_vcycle(self, repeats: int = 4):
+        """Times the per-level leg plans on the current state and keeps the fastest; returns (down, up)."""
+        d, u = C.c_uint32(), C.c_uint32()
+        self._check(self._lib.fx_autotune_vcycle(self._h, int(repeats), C.byref(d), C.byref(u)))
+        return d.value, u.value
+
     def set_stage_timing(self, enable: bool) -> None:
         self._check(self._lib.fx_set_stage_timing(self._h, 1 if enable else 0))
 

@@ -354,6 +354,23 @@ int fx_set_stage_timing(fx_sim* sim, int32_t enable);
  * counts since timing was enabled. `n` entries are written (<= FX_TIME_COUNT). */
 int fx_read_stage_times(fx_sim* sim, double* total_ms, uint32_t* calls, size_t n);
 
+/* ---- V-cycle kernel plan (no reference counterpart) ---------------------- */
+
+/* Kernel plan of the V-cycle legs, one bit per level (bit l = level l):
+ * down_split: pre-sweeps as one-sweep kernels + residual + restriction instead of
+ * one fused LDS kernel (tiles of the brick plus a pre_sweeps + 2 halo);
+ * up_split: prolongation + one-sweep kernels instead of one fused LDS kernel
+ * (halo post_sweeps). Every plan gives bit-identical results; only speed differs. */
+int fx_set_vcycle_plan(fx_sim* sim, uint32_t down_split, uint32_t up_split);
+int fx_get_vcycle_plan(fx_sim* sim, uint32_t* down_split, uint32_t* up_split);
+
+/* Autotuner: times each level's four leg plans (greedily, finest level first)
+ * with `repeats` graph-captured pressure solves of the current state and keeps
+ * the fastest; returns the chosen masks. Solves start from p = 0 and every plan
+ * is bit-identical, so the simulation's results do not depend on the outcome.
+ * Synchronises; V-cycle solver only (FX_ERR_UNSUPPORTED otherwise). */
+int fx_autotune_vcycle(fx_sim* sim, uint32_t repeats, uint32_t* down_split, uint32_t* up_split);
+
 /* ---- test hook ---------------------------------------------------------- */
 
 /* Runs one pass of the step with explicit uniforms (no reference counterpart:

@@ -213,7 +213,7 @@ def test_step_parity_sparse_plume():
     assert sim.active_brick_count() == orc.active_count() > 0.1 * sim.brick_count
 
 
-def test_vcycle_plans_bit_identical(monkeypatch):
+def test_vcycle_plans_bit_identical():
     """Split V-cycle legs (lean one-sweep kernels + residual/restriction and
     prolongation + one sweep) give bit-identical results to the fused LDS legs,
     and both match the oracle."""
@@ -223,9 +223,9 @@ def test_vcycle_plans_bit_identical(monkeypatch):
     desc = scenes.c2_desc(n)
     out, res = {}, {}
     for plan in ((0, 0), (0xF, 0), (0, 0xF), (0xF, 0xF)):
-        monkeypatch.setenv("FENIX_VSPLIT_DOWN", str(plan[0]))
-        monkeypatch.setenv("FENIX_VSPLIT_UP", str(plan[1]))
         sim = CombustionVolumeSimulation(desc)
+        sim.set_vcycle_plan(*plan)
+        assert sim.vcycle_plan() == plan
         sim.updateSources(srcs)
         for k, v in st.items():
             sim.write_field(k, v)
@@ -247,6 +247,29 @@ def test_vcycle_plans_bit_identical(monkeypatch):
         orc.step(k / 60.0, 1 / 60.0)
     for name in PERSISTENT + ("pressure",):
         assert rel_l2(out[(0xF, 0xF)][name], orc.read(name)) <= 1e-5, name
+
+
+def test_autotune_keeps_results():
+    """The autotuner only picks kernels: a run that autotunes mid-way equals a
+    run that never does, bit for bit."""
+    n = 64
+    srcs = scenes.c2_sources(n, radius_vox=2.5)
+    st, _ = scenes.c2_plume_state(n, srcs, target=0.15)
+    outs = []
+    for tune in (False, True):
+        sim = CombustionVolumeSimulation(scenes.c2_desc(n))
+        sim.updateSources(srcs)
+        for k, v in st.items():
+            sim.write_field(k, v)
+        for k in range(6):
+            sim.step(None, k / 60.0, 1 / 60.0)
+            if tune and k == 2:
+                plan = sim.autotune_vcycle(2)
+                assert sim.vcycle_plan() == plan
+        outs.append({f: sim.read_field(f) for f in PERSISTENT + ("pressure",)})
+        sim.dispose()
+    for name in outs[0]:
+        np.testing.assert_array_equal(outs[1][name], outs[0][name], err_msg=name)
 
 
 def test_raymarch_parity():
@@ -272,6 +295,48 @@ def test_raymarch_parity():
         diff = np.abs(img - ref)
         assert rel_l2(img, ref) < 2e-3, f"{mode}: image rel L2 {rel_l2(img, ref):.3e}"
         assert np.mean(diff.max(axis=-1) < 2e-3) > 0.99
+
+
+def test_render_overlap_matches_serial():
+    """The raymarch runs on its own stream, overlapped with the second half of
+    the step; images equal those rendered after a full synchronisation, and a
+    following step never overwrites what a pending render reads."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n, w, h = 48, 96, 80
+    desc = scenes.c2_desc(n)
+    srcs = scenes.c2_sources(n, radius_vox=4.0)
+    st, _ = scenes.c2_plume_state(n, srcs, target=0.15)
+    sims, bufs = [], []
+    for _ in range(2):
+        sim = CombustionVolumeSimulation(desc)
+        sim.updateSources(srcs)
+        for k, v in st.items():
+            sim.write_field(k, v)
+        ptr = C.c_void_p()
+        assert hip.hipMalloc(C.byref(ptr), C.c_size_t(w * h * 8)) == 0
+        sims.append(sim)
+        bufs.append(ptr)
+    rp = VolumeRaymarchPass((n, n, n))
+    rp.setRenderParams(stepCount=64)
+    cam = orbit_camera_snapshot()
+    imgs = [[], []]
+    for k in range(6):
+        for j, sim in enumerate(sims):
+            sim.step(None, k / 60.0, 1 / 60.0)
+            if j == 1:
+                sim.synchronize()
+            rp.render_device(sim, cam, bufs[j].value, "temperature", w, h, k / 60.0)
+        for j, sim in enumerate(sims):
+            sim.synchronize()
+            out = np.empty((h, w, 4), np.float16)
+            assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), bufs[j], C.c_size_t(out.nbytes), 2) == 0
+            imgs[j].append(out)
+    for k in range(6):
+        np.testing.assert_array_equal(imgs[0][k], imgs[1][k], err_msg=f"frame {k}")
+    assert float(imgs[0][-1][..., 3].astype(np.float32).max()) > 0.05
+    for ptr in bufs:
+        hip.hipFree(ptr)
 
 
 def test_deterministic_repeat():
