@@ -55,6 +55,11 @@ SCALAR_ACCESSES = {"advect_scalars": 6, "combustion": 6, "buoyancy": 2, "confine
 # activation passes, which read d and T at 65/512 of the voxels).
 STEP_BYTES_SIM_16 = 569.2
 RAY_SCALAR_ACCESSES = 3
+# HBM bytes per active voxel per launch from rocprofv3 counter runs of this
+# benchmark: FETCH_SIZE x 2.0 (a 256 MiB device copy reads 128 MiB on that counter)
+# + WRITE_SIZE x 1.0 (the copy's writes read exactly). Summaries with the commands
+# are in profiles/.
+TRAFFIC_PER_VOXEL = {"jacobi_fine": 13.56, "advect_scalars": 65.2}
 PRECISIONS = {"fp32": (abi.FX_PRECISION_FP32, 4), "bf16": (abi.FX_PRECISION_BF16, 2), "fp16": (abi.FX_PRECISION_FP16, 2)}
 
 
@@ -141,7 +146,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
-                    help="keep the default V-cycle kernel plan instead of timing the candidates after warm-up")
+                    help="keep the V-cycle kernel plan (default or --vcycle-plan) instead of timing the candidates")
+    ap.add_argument("--vcycle-plan", default=None, metavar="DOWN,UP",
+                    help="per-level split masks of the V-cycle legs, e.g. 1,3 (implies --no-autotune); used to profile "
+                         "the plan an autotuned run chose, since profiler overhead changes what the autotuner picks")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of this many MiB (a known-size reference for the "
                          "HBM traffic counters)")
@@ -171,10 +179,18 @@ def main():
         sim.step(None, k * dt, dt)
         rp.render_device(sim, cam, img.ptr.value, "temperature", w, h, k * dt)
 
-    for k in range(args.warmup):
-        full_step(k)
+    # The V-cycle kernel plan is tuned on the first step's active set (results do
+    # not depend on the plan), so every warm-up and timed step runs the final plan.
+    if args.vcycle_plan:
+        down, up = (int(v, 0) for v in args.vcycle_plan.split(","))
+        sim.set_vcycle_plan(down, up)
+        args.autotune = False
+    full_step(0)
     sim.synchronize()
     plan = sim.autotune_vcycle(4) if args.autotune else sim.vcycle_plan()
+    for k in range(1, args.warmup):
+        full_step(k)
+    sim.synchronize()
     active0 = sim.active_brick_count()
     t0 = time.perf_counter()
     for k in range(args.warmup, args.warmup + args.steps):
@@ -196,18 +212,24 @@ def main():
     active_t = sim.active_brick_count() * 512
     sim.set_stage_timing(False)
     stage_ms = {k: v[0] / v[1] for k, v in stages.items()}
-    candidates = {k: stages[k][0] for k in stages if k in STAGE_BYTES or k == "raymarch"}
-    if not candidates:
+    # Dominant kernel: the single-kernel stage with the most device time per step on
+    # the simulation stream (the raymarch runs overlapped on its own stream). The
+    # fine-level sweep launches ~15 times per step; its mean launch time comes from
+    # the library's kernel clock (100 back-to-back production launches between one
+    # event pair), the others from the per-stage events.
+    totals = {k: stages[k][0] for k in stages if k in STAGE_BYTES}
+    if not totals:
         raise SystemExit("--timing-steps must be >= 1 to find the dominant kernel")
-    dom = max(candidates, key=candidates.get)
-    avg_ms = stage_ms[dom]
+    dom = max(totals, key=totals.get)
+    if dom == "jacobi_fine":
+        avg_ms, vox_launch = sim.time_fine_sweep(100)
+    else:
+        avg_ms, vox_launch = stage_ms[dom], active_t
     ray_bytes = RAY_BYTES_PER_VOXEL - saved * RAY_SCALAR_ACCESSES
     step_bytes_voxel = (STEP_BYTES_SIM if saved == 0 else STEP_BYTES_SIM_16) + ray_bytes
-    if dom == "raymarch":
-        bytes_launch = ray_bytes * active_t + 8.0 * w * h
-    else:
-        bytes_launch = (STAGE_BYTES[dom] - saved * SCALAR_ACCESSES.get(dom, 0)) * active_t
+    bytes_launch = (STAGE_BYTES[dom] - saved * SCALAR_ACCESSES.get(dom, 0)) * vox_launch
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    traffic = TRAFFIC_PER_VOXEL[dom] * vox_launch if dom in TRAFFIC_PER_VOXEL and saved == 0 else None
     step_bytes = step_bytes_voxel * voxels + 8.0 * w * h
     step_gbs = step_bytes / (ms * 1e-3) / 1e9
 
@@ -228,8 +250,8 @@ def main():
                    "vcycle_plan": {"down_split": plan[0], "up_split": plan[1], "autotuned": args.autotune},
                    "active_fraction": active / sim.brick_count, "render": f"{w}x{h}x{sc}"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": bytes_launch, "avg_launch_ms": avg_ms},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_launch": bytes_launch, "avg_launch_ms": avg_ms, "voxels_per_launch": vox_launch},
         "step_hbm": {"algorithmic_gbs": step_gbs, "frac": step_gbs / HBM_PEAK_GBS,
                      "bytes_per_active_voxel": step_bytes_voxel},
         "stage_ms": stage_ms,

@@ -190,6 +190,7 @@ FUNCTIONS = {
     "fx_set_vcycle_plan": (C.c_int, [P, u32, u32]),
     "fx_get_vcycle_plan": (C.c_int, [P, C.POINTER(u32), C.POINTER(u32)]),
     "fx_autotune_vcycle": (C.c_int, [P, u32, C.POINTER(u32), C.POINTER(u32)]),
+    "fx_time_fine_sweep": (C.c_int, [P, u32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "fx_run_pass": (C.c_int, [P, i32, C.POINTER(SimParams)]),
     "fx_last_error": (C.c_char_p, [P]),
 }

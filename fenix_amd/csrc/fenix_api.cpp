@@ -447,7 +447,7 @@ int solve_vcycle(fx_sim* s) {
       StageTimer timer(s, l == 0 ? FX_TIME_JACOBI_FINE : -1);
       const float* src = (zero_start && k == 0) ? nullptr : V.p[cur[l]];
       launch_sweep(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, bits_of(s, l), s->d_const, V.div, src,
-                   V.p[1 - cur[l]]);
+                   V.p[1 - cur[l]], l == 0);
       cur[l] = 1 - cur[l];
     }
   };
@@ -1424,6 +1424,38 @@ int fx_autotune_vcycle(fx_sim* s, uint32_t repeats, uint32_t* down_split, uint32
   *down_split = s->split_down;
   *up_split = s->split_up;
   return rc;
+}
+
+int fx_time_fine_sweep(fx_sim* s, uint32_t repeats, double* avg_ms, uint64_t* voxels) {
+  if (!s || !avg_ms || !voxels) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  if (s->desc.solver != FX_SOLVER_VCYCLE) return fail(s, FX_ERR_UNSUPPORTED, "the fine sweep belongs to the V-cycle solver");
+  HIP_OK(s, hipSetDevice(s->device));
+  HIP_OK(s, hipStreamSynchronize(s->stream));
+  repeats = std::max(1u, repeats);
+  Level& V = s->lv[0];
+  const float* src = V.p[s->lastPressure];
+  float* dst = V.p[1 - s->lastPressure];
+  hipEvent_t a = nullptr, b = nullptr;
+  HIP_OK(s, hipEventCreate(&a));
+  HIP_OK(s, hipEventCreate(&b));
+  launch_sweep(s->stream, grid_for(s, V.nb), level_list(s, 0), level_count(s, 0), V.g, bits_of(s, 0), s->d_const, V.div, src, dst,
+               true);
+  hipError_t e = hipEventRecord(a, s->stream);
+  for (uint32_t r = 0; r < repeats; ++r)
+    launch_sweep(s->stream, grid_for(s, V.nb), level_list(s, 0), level_count(s, 0), V.g, bits_of(s, 0), s->d_const, V.div, src, dst,
+                 true);
+  if (e == hipSuccess) e = hipEventRecord(b, s->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(b);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+  uint32_t n = 0;
+  if (e == hipSuccess) e = hipMemcpy(&n, level_count(s, 0), 4, hipMemcpyDeviceToHost);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (e != hipSuccess) return fail(s, FX_ERR_DEVICE, std::string("fine sweep clock: ") + hipGetErrorString(e));
+  *avg_ms = double(ms) / double(repeats);
+  *voxels = uint64_t(n) * 512u;
+  return FX_OK;
 }
 
 int fx_set_stage_timing(fx_sim* s, int32_t enable) {

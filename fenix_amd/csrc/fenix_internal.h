@@ -144,7 +144,7 @@ void launch_prolong_add(hipStream_t s, int blocks, const uint32_t* list, const u
                         const float* e, float* p);
 // One weighted-Jacobi sweep of a V-cycle level; src == nullptr starts from p = 0.
 void launch_sweep(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
-                  const float* rhs, const float* src, float* dst);
+                  const float* rhs, const float* src, float* dst, bool fine);
 // Coarsest-level slot table (kCoarseTableBytes at tab, count at tab_n), built
 // once per solve; zeroes the inactive voxels of the level's listed bricks in p0/p1.
 constexpr size_t kCoarseTableBytes = 8192 * 16;

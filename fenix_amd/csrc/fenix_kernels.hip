@@ -156,9 +156,25 @@ __device__ __forceinline__ F3 sample_u(const Grid& g, const Vec3& u, F3 p) {
   return F3{tri_eval(t, u.x), tri_eval(t, u.y), tri_eval(t, u.z)};
 }
 
+// XCD-aware brick loop. Workgroups are dispatched to the 8 XCDs round-robin by
+// block id, and each XCD has its own L2: a plain grid-stride loop sends list
+// neighbours (x-neighbour bricks) to different XCDs, so every stencil halo is
+// fetched from memory once per XCD. Here XCD x (blocks b with b % 8 == x) walks
+// one contiguous eighth of the ascending brick list, i.e. a z-slab, whose
+// neighbours mostly sit in the same L2. Grids that are not a multiple of 8 blocks
+// use the plain loop.
+struct BrickRange {
+  uint32_t k, end, step;
+};
+__device__ __forceinline__ BrickRange brick_range(uint32_t n) {
+  if (gridDim.x < 8u || (gridDim.x & 7u) != 0u) return BrickRange{blockIdx.x, n, gridDim.x};
+  const uint32_t x = blockIdx.x & 7u, per = gridDim.x >> 3, seg = (n + 7u) >> 3;
+  const uint32_t lo = min(x * seg, n), hi = min(lo + seg, n);
+  return BrickRange{lo + (blockIdx.x >> 3), hi, per};
+}
 #define FX_BRICK_LOOP(list, count)                                       \
-  const uint32_t fx_n = *(count);                                        \
-  for (uint32_t fx_k = blockIdx.x; fx_k < fx_n; fx_k += gridDim.x)
+  const BrickRange fx_r = brick_range(*(count));                         \
+  for (uint32_t fx_k = fx_r.k; fx_k < fx_r.end; fx_k += fx_r.step)
 
 // ------------------------------------------------------------ emitters --
 // EmitterSource (SH/inject-source-common.wgsl.ts:3-12), 32 floats per source.
@@ -191,7 +207,7 @@ __device__ __forceinline__ float rev2pi(float x) {
 __device__ __forceinline__ float sin_s(float x) { return __builtin_amdgcn_sinf(rev2pi(x)); }
 __device__ __forceinline__ float cos_s(float x) { return __builtin_amdgcn_cosf(rev2pi(x)); }
 __device__ __forceinline__ float hash31(F3 p) { return fractf(sinf(dot3(p, f3(127.1f, 311.7f, 74.7f))) * 43758.5453123f); }
-__device__ float value_noise3(F3 p) {
+__device__ __noinline__ float value_noise3(F3 p) {
   F3 c = f3(floorf(p.x), floorf(p.y), floorf(p.z));
   F3 f = f3(fractf(p.x), fractf(p.y), fractf(p.z));
   F3 u = f3(f.x * f.x * (3.f - 2.f * f.x), f.y * f.y * (3.f - 2.f * f.y), f.z * f.z * (3.f - 2.f * f.z));
@@ -216,7 +232,7 @@ __device__ __forceinline__ float source_feather(const fx_sim_params& P, float r)
   return fmaxf(r * 0.24f, vw * 3.f);
 }
 __device__ __forceinline__ float outer_limit(const fx_sim_params& P, float r) { return r + source_feather(P, r); }
-__device__ float eruptive_falloff(const fx_sim_params& P, F3 pos, float dist, Src s, float falloff) {
+__device__ __noinline__ float eruptive_falloff(const fx_sim_params& P, F3 pos, float dist, Src s, float falloff) {
   float targetRadius = s.radius();
   float growthSeconds = clampf(targetRadius * 5.5f, 0.1f, 0.75f);
   float age = fmaxf(P.time - s.start(), 0.f);
@@ -233,7 +249,7 @@ __device__ float eruptive_falloff(const fx_sim_params& P, F3 pos, float dist, Sr
   float body = sphere_falloff(dist, fmaxf(reach + feather, feather), falloff) * front;
   return clampf(fmaxf(core, body) * mixf(0.72f, 1.2f, lobe), 0.f, 1.f);
 }
-__device__ F3 turbulent_dir(F3 pos, float seed) {  // SH/inject-velocity.wgsl.ts:39-44
+__device__ __noinline__ F3 turbulent_dir(F3 pos, float seed) {  // SH/inject-velocity.wgsl.ts:39-44
   float nx = emitter_noise(pos, 4.f, seed);
   float ny = emitter_noise(pos, 4.f, seed + 17.3f);
   float nz = emitter_noise(pos, 4.f, seed + 31.7f);
@@ -1529,19 +1545,23 @@ __global__ __launch_bounds__(512) void k_restrict_lean(const uint32_t* __restric
 // mix(p, jacobi, omega)); one thread per voxel, neighbours addressed through
 // home-page brick arithmetic (active voxels are interior, so every neighbour
 // brick exists; unlisted ones hold 0). ZERO: the sweep starts from p = 0.
-template <bool ZERO>
+// FINE: the finest level, where every in-domain voxel of a listed brick off the
+// boundary faces is active; the test uses the voxel's coordinates instead of the
+// activity words.
+template <bool ZERO, bool FINE>
 __global__ __launch_bounds__(512) void k_sweep(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
                                                LvBits lb, const StepConst* __restrict__ C, const float* __restrict__ rhs,
                                                const float* __restrict__ src, float* __restrict__ dst) {
   const float om = C->omega;
   const uint32_t t = threadIdx.x, lx = t & 7u, ly = (t >> 3) & 7u, lz = t >> 6;
   const uint32_t BX = g.BX, BXY = g.BX * g.BY, last = g.BX * g.BY * g.BZ - 1u;
-  const uint32_t n = *count;
+  const BrickRange br = brick_range(*count);
+  const uint32_t n = br.end;
   // Two bricks per round, every load issued before any result is needed; the
   // neighbour addresses are clamped to the allocation so that they can be read
   // before the activity word says whether the voxel uses them.
-  for (uint32_t k0 = blockIdx.x; k0 < n; k0 += 2u * gridDim.x) {
-    const uint32_t kk[2] = {k0, k0 + gridDim.x};
+  for (uint32_t k0 = br.k; k0 < n; k0 += 2u * br.step) {
+    const uint32_t kk[2] = {k0, k0 + br.step};
     uint32_t ii[2];
     uint64_t w[2];
     float sum[2], self[2], r[2];
@@ -1551,7 +1571,13 @@ __global__ __launch_bounds__(512) void k_sweep(const uint32_t* __restrict__ list
       const uint32_t b = list[ok ? kk[u] : k0];
       const uint32_t i = (b << 9) | t;
       ii[u] = i;
-      w[u] = ok ? lb.bits[size_t(kk[u]) * 8 + lz] : 0ull;
+      if (FINE) {
+        const uint32_t x = (b % g.BX) * 8 + lx, y = ((b / g.BX) % g.BY) * 8 + ly, z = (b / BXY) * 8 + lz;
+        const bool act = ok && x < g.W && y < g.H && z < g.D && !on_boundary(g, x, y, z);
+        w[u] = act ? (1ull << (t & 63u)) : 0ull;
+      } else {
+        w[u] = ok ? lb.bits[size_t(kk[u]) * 8 + lz] : 0ull;
+      }
       sum[u] = 0.f;
       self[u] = 0.f;
       if (!ZERO) {
@@ -2017,11 +2043,15 @@ void launch_prolong_add(hipStream_t s, int blocks, const uint32_t* list, const u
   FX_LAUNCH(k_prolong_add, blocks, 512, s, list, count, c, f, b, e, p);
 }
 void launch_sweep(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
-                  const float* rhs, const float* src, float* dst) {
-  if (src) {
-    FX_LAUNCH(k_sweep<false>, blocks, 512, s, list, count, g, b, c, rhs, src, dst);
+                  const float* rhs, const float* src, float* dst, bool fine) {
+  if (src && fine) {
+    FX_LAUNCH((k_sweep<false, true>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
+  } else if (src) {
+    FX_LAUNCH((k_sweep<false, false>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
+  } else if (fine) {
+    FX_LAUNCH((k_sweep<true, true>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
   } else {
-    FX_LAUNCH(k_sweep<true>, blocks, 512, s, list, count, g, b, c, rhs, src, dst);
+    FX_LAUNCH((k_sweep<true, false>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
   }
 }
 void launch_residual(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const float* rhs,

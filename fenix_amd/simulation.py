@@ -307,6 +307,12 @@ class CombustionVolumeSimulation:
         self._check(self._lib.fx_autotune_vcycle(self._h, int(repeats), C.byref(d), C.byref(u)))
         return d.value, u.value
 
+    def time_fine_sweep(self, repeats: int = 50):
+        """(mean ms per launch, voxels per launch) of the finest level's pressure sweep on the current state."""
+        ms, vox = C.c_double(), C.c_uint64()
+        self._check(self._lib.fx_time_fine_sweep(self._h, int(repeats), C.byref(ms), C.byref(vox)))
+        return ms.value, vox.value
+
     def set_stage_timing(self, enable: bool) -> None:
         self._check(self._lib.fx_set_stage_timing(self._h, 1 if enable else 0))
 

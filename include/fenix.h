@@ -371,6 +371,14 @@ int fx_get_vcycle_plan(fx_sim* sim, uint32_t* down_split, uint32_t* up_split);
  * Synchronises; V-cycle solver only (FX_ERR_UNSUPPORTED otherwise). */
 int fx_autotune_vcycle(fx_sim* sim, uint32_t repeats, uint32_t* down_split, uint32_t* up_split);
 
+/* Kernel clock (no reference counterpart): times `repeats` back-to-back launches
+ * of the finest level's weighted-Jacobi sweep on the current active set (reading
+ * the last pressure solution, writing the other pressure buffer, so the solution
+ * the projection uses is left intact) between one pair of HIP events on the
+ * simulation stream, and returns the mean launch time in milliseconds and the
+ * voxel count one launch updates. Synchronises; V-cycle solver only. */
+int fx_time_fine_sweep(fx_sim* sim, uint32_t repeats, double* avg_ms, uint64_t* voxels);
+
 /* ---- test hook ---------------------------------------------------------- */
 
 /* Runs one pass of the step with explicit uniforms (no reference counterpart:
