@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 
 from fenix_amd import abi  # noqa: E402
 from fenix_amd.render import VolumeRaymarchPass, orbit_camera_snapshot  # noqa: E402
-from fenix_amd.simulation import CombustionVolumeSimulation  # noqa: E402
+from fenix_amd.simulation import CombustionVolumeBatch, CombustionVolumeSimulation  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
@@ -136,6 +136,114 @@ def cpu_baseline(wl, sim, elapsed0, steps):
                       f"after the timed window; {per_step:.2f} s/step"}
 
 
+def run_batch(args):
+    """Batched scenes (fx_batch): aggregate active voxel-updates/s of one batch step
+    (every scene steps once; with --workload c2 every scene also renders), against the
+    plain path (the same scenes stepped one at a time, each alone on the GPU)."""
+    # Each scene runs on two streams; give them hardware queues of their own
+    # (read by the HIP runtime at initialisation, before any device call).
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(args.hw_queues))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import scenes
+    prec = PRECISIONS[args.precision][0]
+    render = None
+    if args.workload == "c3":
+        specs = scenes.c3_scenes(args.batch or 64)
+        label = (f"C3 batch of {len(specs)} sparse scenes, edge 32-192 (log-uniform), 5-40% target active, "
+                 f"1-4 floor emitters, {args.precision.upper()} storage, simulation only")
+    elif args.workload == "c2":
+        k = args.batch
+        specs = [dict(n=256, target=0.15, seed=2 + i, sources=scenes.c2_sources(256, seed=2 + i)) for i in range(k)]
+        render = (1024, 1024, 128)
+        label = (f"{k} sparse 256^3 plume scenes (seeds 2..{1 + k}), {args.precision.upper()} storage, "
+                 "each with a 1024x1024x128 raymarch")
+    else:
+        raise SystemExit("--batch applies to the c2 and c3 workloads")
+    descs = [scenes.c2_desc(sp["n"], precision=prec) for sp in specs]
+    batch = CombustionVolumeBatch(descs)
+    for sim, sp in zip(batch.scenes, specs):
+        sim.updateSources(sp["sources"])
+        if sp["n"] == 256:
+            st, _ = scenes.c2_plume_state(256, sp["sources"], target=sp["target"], seed=sp["seed"])
+        else:
+            st = scenes.plume_state_fast(sp["n"], sp["sources"], sp["target"], sp["seed"])
+        for name, values in st.items():
+            sim.write_field(name, values)
+        del st
+    rp, cam, imgs = None, orbit_camera_snapshot(), []
+    if render:
+        w, h, sc = render
+        rp = {}
+        for sim, sp in zip(batch.scenes, specs):
+            rp[id(sim)] = VolumeRaymarchPass((sp["n"],) * 3)
+            rp[id(sim)].setRenderParams(stepCount=sc)
+            imgs.append(HipBuffer(w * h * 8))
+    dt = 1.0 / 60.0
+
+    def batch_step(k):
+        batch.step(k * dt, dt)
+        if render:
+            for j, sim in enumerate(batch.scenes):
+                rp[id(sim)].render_device(sim, cam, imgs[j].ptr.value, "temperature", render[0], render[1], k * dt)
+
+    def scene_step(j, k):
+        sim = batch.scenes[j]
+        sim.step(None, k * dt, dt)
+        if render:
+            rp[id(sim)].render_device(sim, cam, imgs[j].ptr.value, "temperature", render[0], render[1], k * dt)
+
+    batch_step(0)
+    batch.synchronize()
+    if args.autotune:
+        for sim in batch.scenes:
+            sim.autotune_vcycle(4)
+    for k in range(1, args.warmup):
+        batch_step(k)
+    batch.synchronize()
+    a0 = sum(sim.active_brick_count() for sim in batch.scenes)
+    t0 = time.perf_counter()
+    for k in range(args.warmup, args.warmup + args.steps):
+        batch_step(k)
+    batch.synchronize()
+    t1 = time.perf_counter()
+    a1 = sum(sim.active_brick_count() for sim in batch.scenes)
+    ms = (t1 - t0) * 1e3 / args.steps
+    voxels = 0.5 * (a0 + a1) * 512
+    # plain: each scene alone for the same number of steps, one after the other
+    base = args.warmup + args.steps
+    plain_s = 0.0
+    for j in range(len(batch.scenes)):
+        t0 = time.perf_counter()
+        for k in range(base, base + args.steps):
+            scene_step(j, k)
+        batch.scenes[j].synchronize()
+        plain_s += time.perf_counter() - t0
+    plain_ms = plain_s * 1e3 / args.steps
+    out = {
+        "metric": "aggregate active voxel-updates/s per batch step (batched vs plain single-scene path)",
+        "value": voxels / (ms * 1e-3),
+        "unit": "active voxel-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": label, "scenes": len(specs), "storage": args.precision,
+                   "edges": sorted(sp["n"] for sp in specs), "active_bricks": 0.5 * (a0 + a1),
+                   "voxels": sum(sp["n"] ** 3 for sp in specs)},
+        "plain": {"ms_per_batch_step": plain_ms, "value": voxels / (plain_ms * 1e-3)},
+        "vs_plain": plain_ms / ms,
+    }
+    for b in imgs:
+        b.free()
+    batch.dispose()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +258,10 @@ def main():
     ap.add_argument("--vcycle-plan", default=None, metavar="DOWN,UP",
                     help="per-level split masks of the V-cycle legs, e.g. 1,3 (implies --no-autotune); used to profile "
                          "the plan an autotuned run chose, since profiler overhead changes what the autotuner picks")
+    ap.add_argument("--batch", type=int, default=0, metavar="K",
+                    help="batched scenes: K 256^3 plume scenes (c2) or the C3 batch (c3, default 64 scenes)")
+    ap.add_argument("--hw-queues", type=int, default=16, choices=range(1, 33), metavar="Q",
+                    help="hardware queues per process for batched runs (GPU_MAX_HW_QUEUES, at most 32)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of this many MiB (a known-size reference for the "
                          "HBM traffic counters)")
@@ -160,6 +272,8 @@ def main():
         print("bench.py runs on exactly one GPU: there is no multi-GPU path (the sharding plan is design-only).",
               file=sys.stderr)
         sys.exit(2)
+    if args.batch or args.workload == "c3":
+        return run_batch(args)
 
     wl = workload(args.workload, args.precision)
     saved = 4 - PRECISIONS[args.precision][1]

@@ -191,6 +191,12 @@ FUNCTIONS = {
     "fx_get_vcycle_plan": (C.c_int, [P, C.POINTER(u32), C.POINTER(u32)]),
     "fx_autotune_vcycle": (C.c_int, [P, u32, C.POINTER(u32), C.POINTER(u32)]),
     "fx_time_fine_sweep": (C.c_int, [P, u32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "fx_batch_create": (C.c_int, [C.POINTER(CreateDesc), u32, C.POINTER(C.c_void_p)]),
+    "fx_batch_destroy": (None, [C.c_void_p]),
+    "fx_batch_size": (u32, [C.c_void_p]),
+    "fx_batch_scene": (C.c_void_p, [C.c_void_p, u32]),
+    "fx_batch_step": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "fx_batch_synchronize": (C.c_int, [C.c_void_p]),
     "fx_run_pass": (C.c_int, [P, i32, C.POINTER(SimParams)]),
     "fx_last_error": (C.c_char_p, [P]),
 }

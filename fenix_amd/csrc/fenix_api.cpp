@@ -1357,6 +1357,56 @@ int fx_raymarch(fx_sim* s, const fx_camera* c, const fx_render_params* p, uint32
   return rc;
 }
 
+}  // extern "C"
+
+struct fx_batch {
+  std::vector<fx_sim*> scenes;
+};
+
+extern "C" {
+
+int fx_batch_create(const fx_create_desc* descs, uint32_t n, fx_batch** out) {
+  if (!descs || !out || n == 0) return fail(nullptr, FX_ERR_INVALID_ARG, "invalid batch arguments");
+  *out = nullptr;
+  fx_batch* b = new (std::nothrow) fx_batch;
+  if (!b) return fail(nullptr, FX_ERR_OUT_OF_MEMORY, "batch allocation failed");
+  for (uint32_t i = 0; i < n; ++i) {
+    fx_sim* sim = nullptr;
+    const int rc = fx_create(&descs[i], &sim);
+    if (rc != FX_OK) {
+      for (fx_sim* q : b->scenes) fx_destroy(q);
+      delete b;
+      return rc;  // the message stays in fx_last_error(NULL)
+    }
+    b->scenes.push_back(sim);
+  }
+  *out = b;
+  return FX_OK;
+}
+void fx_batch_destroy(fx_batch* b) {
+  if (!b) return;
+  for (fx_sim* q : b->scenes) fx_destroy(q);
+  delete b;
+}
+uint32_t fx_batch_size(const fx_batch* b) { return b ? uint32_t(b->scenes.size()) : 0u; }
+fx_sim* fx_batch_scene(fx_batch* b, uint32_t i) { return (b && i < b->scenes.size()) ? b->scenes[i] : nullptr; }
+int fx_batch_step(fx_batch* b, double elapsed_s, double step_s) {
+  if (!b) return fail(nullptr, FX_ERR_INVALID_ARG, "null batch");
+  for (fx_sim* q : b->scenes) {
+    const int rc = fx_step(q, elapsed_s, step_s);
+    if (rc != FX_OK) return rc;
+  }
+  return FX_OK;
+}
+int fx_batch_synchronize(fx_batch* b) {
+  if (!b) return fail(nullptr, FX_ERR_INVALID_ARG, "null batch");
+  for (fx_sim* q : b->scenes) {
+    const int rc = fx_synchronize(q);
+    if (rc != FX_OK) return rc;
+  }
+  return FX_OK;
+}
+
 int fx_set_vcycle_plan(fx_sim* s, uint32_t down_split, uint32_t up_split) {
   if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
   s->split_down = down_split;

@@ -124,13 +124,17 @@ class DeviceField:
 class CombustionVolumeSimulation:
     """CombustionVolumeSimulation (CVS/types.ts:62-76) on one MI355X."""
 
-    def __init__(self, desc: abi.CreateDesc, lib=None):
+    def __init__(self, desc: abi.CreateDesc, lib=None, borrowed_handle=None):
         self._lib = lib or abi.load()
-        self._h = C.c_void_p()
-        rc = self._lib.fx_create(C.byref(desc), C.byref(self._h))
-        if rc != abi.FX_OK:
-            msg = self._lib.fx_last_error(None)
-            raise abi.FenixError(rc, msg.decode() if msg else "")
+        self._owned = borrowed_handle is None
+        if borrowed_handle is not None:  # a scene of a CombustionVolumeBatch
+            self._h = C.c_void_p(borrowed_handle)
+        else:
+            self._h = C.c_void_p()
+            rc = self._lib.fx_create(C.byref(desc), C.byref(self._h))
+            if rc != abi.FX_OK:
+                msg = self._lib.fx_last_error(None)
+                raise abi.FenixError(rc, msg.decode() if msg else "")
         self.desc = desc
         self.resolution = VolumeResolution(desc.width, desc.height, desc.depth)
         self.brick_counts = tuple((n + 7) // 8 for n in (desc.width, desc.height, desc.depth))
@@ -226,9 +230,9 @@ class CombustionVolumeSimulation:
         self._check(self._lib.fx_reset(self._h))
 
     def dispose(self) -> None:  # ORCH:384-400
-        if self._h:
+        if self._h and self._owned:
             self._lib.fx_destroy(self._h)
-            self._h = C.c_void_p()
+        self._h = C.c_void_p()
 
     def __del__(self):
         try:
@@ -327,6 +331,53 @@ class CombustionVolumeSimulation:
         """packSparseBrickLayout (src/engine/simulation/sparse/createSparseBrickLayout.ts:28-39)."""
         bx, by, bz = self.brick_counts
         return np.array([bx, by, bz, self.brick_count, 8, 0, 0, 0], dtype=np.uint32)
+
+
+class CombustionVolumeBatch:
+    """Many independent scenes stepped together (fx_batch_*): each scene keeps its
+    own streams and step graphs, so the GPU runs them concurrently. `scenes[i]` is a
+    CombustionVolumeSimulation over the borrowed handle of scene i."""
+
+    def __init__(self, descs: Sequence[abi.CreateDesc], lib=None):
+        self._lib = lib or abi.load()
+        arr = (abi.CreateDesc * len(descs))(*descs)
+        self._h = C.c_void_p()
+        rc = self._lib.fx_batch_create(arr, len(descs), C.byref(self._h))
+        if rc != abi.FX_OK:
+            msg = self._lib.fx_last_error(None)
+            raise abi.FenixError(rc, msg.decode() if msg else "")
+        self.scenes = [CombustionVolumeSimulation(d, self._lib, borrowed_handle=self._lib.fx_batch_scene(self._h, i))
+                       for i, d in enumerate(descs)]
+
+    def __len__(self):
+        return len(self.scenes)
+
+    def step(self, elapsedSeconds: float, stepSeconds: float) -> None:
+        rc = self._lib.fx_batch_step(self._h, float(elapsedSeconds), float(stepSeconds))
+        if rc != abi.FX_OK:
+            for sim in self.scenes:  # the failing scene carries the message
+                msg = self._lib.fx_last_error(sim.handle)
+                if msg:
+                    raise abi.FenixError(rc, msg.decode())
+            raise abi.FenixError(rc, "batch step failed")
+
+    def synchronize(self) -> None:
+        rc = self._lib.fx_batch_synchronize(self._h)
+        if rc != abi.FX_OK:
+            raise abi.FenixError(rc, "batch synchronize failed")
+
+    def dispose(self) -> None:
+        if self._h:
+            for sim in self.scenes:
+                sim.dispose()
+            self._lib.fx_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.dispose()
+        except Exception:
+            pass
 
 
 def createCombustionVolumeSimulation(device=None, options: Optional[Dict] = None, **build_knobs) -> CombustionVolumeSimulation:

@@ -268,6 +268,28 @@ int fx_step(fx_sim* sim, double elapsed_s, double step_s);
 /* Waits for all work queued on the simulation's stream. */
 int fx_synchronize(fx_sim* sim);
 
+/* ---- batches (no reference counterpart) ---------------------------------- */
+
+/* A batch of independent scenes (BASELINE C3: many scenes of different sizes).
+ * Every scene is a full fx_sim with its own simulation and render streams and its
+ * own captured step graphs, so one batch step runs the scenes concurrently on the
+ * GPU: the latency-bound launches of one scene fill the CUs another leaves idle. */
+typedef struct fx_batch fx_batch;
+
+/* Creates n scenes from descs[0..n-1]. On failure nothing is left allocated. */
+int fx_batch_create(const fx_create_desc* descs, uint32_t n, fx_batch** out);
+/* Destroys every scene of the batch. Accepts NULL. */
+void fx_batch_destroy(fx_batch* batch);
+uint32_t fx_batch_size(const fx_batch* batch);
+/* Borrowed handle of scene i (NULL if out of range): every fx_* call applies to it,
+ * except fx_destroy. */
+fx_sim* fx_batch_scene(fx_batch* batch, uint32_t i);
+/* fx_step on every scene with the same clock; stops at the first failure and
+ * returns its code (the message is on that scene's fx_last_error). */
+int fx_batch_step(fx_batch* batch, double elapsed_s, double step_s);
+/* Waits for every scene's streams. */
+int fx_batch_synchronize(fx_batch* batch);
+
 /* ---- parameters --------------------------------------------------------- */
 
 /* getRuntimeParams / setRuntimeParams(Partial<...>) — ORCH:338-365. */

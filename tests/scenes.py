@@ -219,3 +219,63 @@ def c2_plume_state(n=256, sources=None, target=0.15, seed=2):
         else:
             hi = mid
     return plume_fields(n, sources, best[1], seed), best[2]
+
+
+def plume_brick_fraction(n, sources, top_radius):
+    """Fraction of bricks the threshold rule activates for plume_fields(n, sources,
+    top_radius), at brick resolution: a brick holds density when its xz box comes
+    within the cone radius of an axis at the brick's y-range (conservative box test),
+    then dense bricks dilate by one brick."""
+    nb = (n + 7) // 8
+    b = np.arange(nb, dtype=np.float32)
+    lo, hi = b * 8, np.minimum(b * 8 + 8, n)
+    dens = np.zeros((nb, nb, nb), bool)  # (z, y, x)
+    for s in sources:
+        xe, ze, r0 = s.position[0] * n, s.position[2] * n, s.radius * n
+        ytop = hi - 0.5  # the widest cross-section of the brick's rows
+        r = r0 + (top_radius - r0) * (ytop / n) ** 0.8
+        dx = np.maximum(np.maximum(lo - 0.5 - xe, xe - (hi - 0.5)), 0.0)  # distance from axis to the brick's voxel centres
+        dz = np.maximum(np.maximum(lo - 0.5 - ze, ze - (hi - 0.5)), 0.0)
+        d2 = dz[:, None, None] ** 2 + dx[None, None, :] ** 2
+        dens |= d2 < (r * r)[None, :, None]
+    act = np.zeros_like(dens)
+    for oz in (-1, 0, 1):
+        for oy in (-1, 0, 1):
+            for ox in (-1, 0, 1):
+                src = dens[max(0, -oz):nb - max(0, oz), max(0, -oy):nb - max(0, oy), max(0, -ox):nb - max(0, ox)]
+                act[max(0, oz):nb + min(0, oz), max(0, oy):nb + min(0, oy), max(0, ox):nb + min(0, ox)] |= src
+    return float(act.mean())
+
+
+def plume_state_fast(n, sources, target, seed=2):
+    """plume_fields with the top radius bisected at brick resolution (for many scenes)."""
+    lo, hi = sources[0].radius * n, 4.0 * n
+    best = (1e9, lo)
+    for _ in range(14):
+        mid = 0.5 * (lo + hi)
+        frac = plume_brick_fraction(n, sources, mid)
+        if abs(frac - target) < best[0]:
+            best = (abs(frac - target), mid)
+        if frac < target:
+            lo = mid
+        else:
+            hi = mid
+    return plume_fields(n, sources, best[1], seed)
+
+
+def c3_scenes(count=64, seed=3):
+    """C3 batch: edge length log-uniform in [32, 192] voxels (multiple of 8),
+    active-brick fraction uniform in [5%, 40%] (a seeded developed plume), 1-4
+    seeded floor emitters of radius max(2.5, n/25.6) voxels. Each entry: n,
+    target fraction, sources (config mode, FP32)."""
+    import math
+    rng = SplitMix64(seed)
+    out = []
+    for i in range(count):
+        n = int(round(math.exp(math.log(32.0) + (math.log(192.0) - math.log(32.0)) * rng.uniform()) / 8.0)) * 8
+        n = min(max(n, 32), 192)
+        frac = 0.05 + 0.35 * rng.uniform()
+        ne = 1 + min(3, int(rng.uniform() * 4))
+        srcs = c2_sources(n, seed=seed * 1000 + i, count=ne, radius_vox=max(2.5, n / 25.6))
+        out.append(dict(n=n, target=frac, sources=srcs, seed=seed * 1000 + i))
+    return out
