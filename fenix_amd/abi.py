@@ -118,6 +118,16 @@ class VCycle(C.Structure):
     ]
 
 
+class DiffDesc(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32), ("height", C.c_uint32), ("depth", C.c_uint32),
+        ("steps", C.c_uint32), ("checkpoint_every", C.c_uint32), ("control_n", C.c_uint32), ("slab", C.c_uint32),
+        ("dt", C.c_float), ("buoyancy", C.c_float), ("confinement", C.c_float),
+        ("vcycle", VCycle),
+        ("device", C.c_int32),
+    ]
+
+
 class CreateDesc(C.Structure):
     _fields_ = [
         ("width", C.c_uint32), ("height", C.c_uint32), ("depth", C.c_uint32),
@@ -197,6 +207,14 @@ FUNCTIONS = {
     "fx_batch_scene": (C.c_void_p, [C.c_void_p, u32]),
     "fx_batch_step": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     "fx_batch_synchronize": (C.c_int, [C.c_void_p]),
+    "fx_diff_default_desc": (None, [C.POINTER(DiffDesc)]),
+    "fx_diff_create": (C.c_int, [C.POINTER(DiffDesc), C.POINTER(C.c_void_p)]),
+    "fx_diff_destroy": (None, [C.c_void_p]),
+    "fx_diff_set_target": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t]),
+    "fx_diff_forward": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_double)]),
+    "fx_diff_gradient": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_float)]),
+    "fx_diff_read_field": (C.c_int, [C.c_void_p, i32, C.POINTER(C.c_float), C.c_size_t]),
+    "fx_diff_last_error": (C.c_char_p, [C.c_void_p]),
     "fx_run_pass": (C.c_int, [P, i32, C.POINTER(SimParams)]),
     "fx_last_error": (C.c_char_p, [P]),
 }

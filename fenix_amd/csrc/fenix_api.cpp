@@ -1536,3 +1536,27 @@ int fx_read_stage_times(fx_sim* s, double* total_ms, uint32_t* calls, size_t n) 
 const char* fx_last_error(fx_sim* s) { return s ? s->err.c_str() : g_create_error.c_str(); }
 
 }  // extern "C"
+
+// ---- hooks for the differentiable step (fenix_diff.cpp) ---------------------
+namespace fx {
+
+SimView sim_view(fx_sim* s) {
+  return SimView{s->stream, s->g, fine_list(s), fine_count(s), grid_for(s, s->nb), s->nb, storage_count(s->g)};
+}
+int sim_prepare_projection(fx_sim* s) {
+  fx_sim_params p{};
+  return upload_step(s, p, nullptr, 0);
+}
+int sim_project(fx_sim* s, Vec3 us, Vec3 u) {
+  const SimView v = sim_view(s);
+  launch_divergence(v.stream, v.blocks, v.list, v.count, v.g, us, s->lv[0].div);
+  const int cur = solve_vcycle(s);
+  launch_projection(v.stream, v.blocks, v.list, v.count, v.g, us, s->lv[0].p[cur], u);
+  s->lastPressure = cur;
+  HIP_OK(s, hipGetLastError());
+  return FX_OK;
+}
+void bricks_from_rowmajor(const Grid& g, const float* rm, float* bm) { scatter_brickmajor(g, rm, bm); }
+void bricks_to_rowmajor(const Grid& g, const float* bm, float* rm) { gather_rowmajor(g, bm, rm); }
+
+}  // namespace fx

@@ -219,4 +219,42 @@ void launch_render_pack(hipStream_t s, int blocks, const uint32_t* list, const u
 void launch_raymarch(hipStream_t s, Grid g, const RayConst& rc, const void* rb, const uint32_t* bits, uint32_t* queue, int blocks,
                      uint16_t* out);
 
+// ---- differentiable smoke step (fenix_diff.cpp) ----
+void launch_df_source(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const float* ctrl, uint32_t cn,
+                      uint32_t slab, float dt, const float* d, Vec3 u, float* ds, Vec3 us);
+// u_a = semi-Lagrangian(u_s), d' = MacCormack(d_s, u_a)
+void launch_df_advect(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 us, Vec3 ua,
+                      const float* ds, float* dn, float dt);
+// u_b = u_a + buoyancy(d'); w, |w| = curl u_b; u_c = u_b + confinement
+void launch_df_forces(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const float* dn, Vec3 ua, Vec3 ub,
+                      Vec3 w, float* wmag, Vec3 uc, float beta_dt, float eps_dt);
+// out[kNormLanes] += 0.5 sum (d - target)^2 (FP64); db = d - target when non-null
+void launch_df_loss(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const float* d, const float* target,
+                    float* db, double* out);
+void launch_da_zero_boundary(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 a);
+// abar: adjoint of u_c in, of u_a out; dnb += buoyancy adjoint. w, wmag, gbar, wbar: scratch.
+void launch_da_forces(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 ub, Vec3 w, float* wmag,
+                      Vec3 abar, Vec3 gbar, Vec3 wbar, float* dnb, float beta_dt, float eps_dt);
+// dsb, usb (cleared by the caller) += adjoints of d_s, u_s; abar gains the MacCormack velocity terms first.
+void launch_da_advect(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const float* ds, Vec3 us, Vec3 ua,
+                      const float* dnb, float* dsb, Vec3 abar, Vec3 usb, float dt);
+void launch_da_source(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, uint32_t cn, uint32_t slab,
+                      float dt, const float* dsb, Vec3 usb, float* cbar);
+
+// Simulation internals used by the differentiable step: a dense simulation's
+// stream, grid and brick list, and its V-cycle projection u = us - grad P(div us).
+struct SimView {
+  hipStream_t stream;
+  Grid g;
+  const uint32_t* list;
+  const uint32_t* count;
+  int blocks;
+  size_t nb, storage;  // bricks, floats per brick-major field
+};
+SimView sim_view(fx_sim* s);
+int sim_prepare_projection(fx_sim* s);  // uploads the step constants the V-cycle reads
+int sim_project(fx_sim* s, Vec3 us, Vec3 u);
+void bricks_from_rowmajor(const Grid& g, const float* rm, float* bm);
+void bricks_to_rowmajor(const Grid& g, const float* bm, float* rm);
+
 }  // namespace fx

@@ -1895,6 +1895,355 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, const floa
   }
 }
 
+// ------------------------------------------------ differentiable smoke --
+// A differentiable smoke step for inflow-control optimisation (no counterpart in
+// the reference, which has no gradients). State: density d and velocity u on a
+// dense grid. Per step:
+//   source:      d_s = d + dt S_rho, u_s = u + dt S_u on the floor slab (interior
+//                voxels with 1 <= y <= slab); S is the trilinear upsampling of a
+//                cn^3 x 4 control, row-major [channel][z][y][x], channels
+//                (u_x, u_y, u_z, rho), cell-centred over the slab box;
+//   advection:   u_a = u_s(x - u_s(x) dt) (semi-Lagrangian, 0 on the boundary);
+//                d' = MacCormack(d_s, u_a) with the min/max(original, predicted)
+//                limiter (the scalar advection's formula without the decay terms);
+//   buoyancy:    u_b = u_a + dt beta d' e_y (interior);
+//   confinement: u_c = u_b + dt eps (N x w), w = curl u_b, N = grad|w| / max(|grad|w||, 1e-4)
+//                (the confinement kernel's direction, without its masks and clamp);
+//   projection:  u' = u_c - grad P(div u_c), P the V-cycle (0 on the boundary).
+// The adjoint kernels reverse each stage; transposed gathers are float atomics.
+__device__ __forceinline__ bool interior(const Grid& g, uint32_t x, uint32_t y, uint32_t z) {
+  return x > 0 && y > 0 && z > 0 && x + 1 < g.W && y + 1 < g.H && z + 1 < g.D;
+}
+__device__ __forceinline__ bool in_slab(const Grid& g, uint32_t slab, uint32_t x, uint32_t y, uint32_t z) {
+  return y >= 1 && y <= slab && interior(g, x, y, z);
+}
+// The 8 control cells (per channel) and weights of slab voxel (x, y, z).
+__device__ __forceinline__ void control_taps(const Grid& g, uint32_t cn, uint32_t slab, uint32_t x, uint32_t y, uint32_t z,
+                                             uint32_t idx[8], float w[8]) {
+  const float m = float(cn - 1), c = float(cn);
+  const float px = clampf((float(x) + 0.5f) / float(g.W) * c - 0.5f, 0.f, m);
+  const float py = clampf((float(y - 1) + 0.5f) / float(slab) * c - 0.5f, 0.f, m);
+  const float pz = clampf((float(z) + 0.5f) / float(g.D) * c - 0.5f, 0.f, m);
+  const uint32_t x0 = uint32_t(floorf(px)), y0 = uint32_t(floorf(py)), z0 = uint32_t(floorf(pz));
+  const uint32_t x1 = min(x0 + 1u, cn - 1u), y1 = min(y0 + 1u, cn - 1u), z1 = min(z0 + 1u, cn - 1u);
+  const float fx = px - float(x0), fy = py - float(y0), fz = pz - float(z0);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    idx[k] = ((k & 1) ? x1 : x0) + cn * (((k & 2) ? y1 : y0) + cn * ((k & 4) ? z1 : z0));
+    w[k] = ((k & 1) ? fx : 1.f - fx) * ((k & 2) ? fy : 1.f - fy) * ((k & 4) ? fz : 1.f - fz);
+  }
+}
+__global__ __launch_bounds__(512) void k_df_source(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                   const float* __restrict__ ctrl, uint32_t cn, uint32_t slab, float dt,
+                                                   const float* __restrict__ d, Vec3 u, float* __restrict__ ds, Vec3 us) {
+  const size_t plane = size_t(cn) * cn * cn;
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
+    if (in_slab(g, slab, v.x, v.y, v.z)) {
+      uint32_t idx[8];
+      float w[8];
+      control_taps(g, cn, slab, v.x, v.y, v.z, idx, w);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) s[ch] += w[k] * ctrl[ch * plane + idx[k]];
+    }
+    us.x[v.i] = u.x[v.i] + dt * s[0];
+    us.y[v.i] = u.y[v.i] + dt * s[1];
+    us.z[v.i] = u.z[v.i] + dt * s[2];
+    ds[v.i] = d[v.i] + dt * s[3];
+  }
+}
+__global__ __launch_bounds__(512) void k_df_advect_u(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                     Vec3 u, Vec3 ua, float dt) {
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    if (!interior(g, v.x, v.y, v.z)) { ua.x[v.i] = 0.f; ua.y[v.i] = 0.f; ua.z[v.i] = 0.f; continue; }
+    const Tri t = tri(g, f3(float(v.x) - u.x[v.i] * dt, float(v.y) - u.y[v.i] * dt, float(v.z) - u.z[v.i] * dt));
+    ua.x[v.i] = tri_eval(t, u.x);
+    ua.y[v.i] = tri_eval(t, u.y);
+    ua.z[v.i] = tri_eval(t, u.z);
+  }
+}
+__global__ __launch_bounds__(512) void k_df_advect_d(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                     const float* __restrict__ d, Vec3 ua, float* __restrict__ dn, float dt) {
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    const F3 sp = f3(float(v.x) - ua.x[v.i] * dt, float(v.y) - ua.y[v.i] * dt, float(v.z) - ua.z[v.i] * dt);
+    const Tri tp = tri(g, sp);
+    const F3 fwd = f3(tri_eval(tp, ua.x), tri_eval(tp, ua.y), tri_eval(tp, ua.z));
+    const Tri tr = tri(g, sp + fwd * dt);
+    const float pred = tri_eval(tp, d), rec = tri_eval(tr, d), orig = d[v.i];
+    const float corr = pred + 0.5f * (orig - rec);
+    dn[v.i] = clampf(corr, fminf(orig, pred), fmaxf(orig, pred));
+  }
+}
+__global__ __launch_bounds__(512) void k_df_buoyancy(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                     const float* __restrict__ d, Vec3 ua, Vec3 ub, float beta_dt) {
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    ub.x[v.i] = ua.x[v.i];
+    ub.y[v.i] = ua.y[v.i] + (interior(g, v.x, v.y, v.z) ? beta_dt * d[v.i] : 0.f);
+    ub.z[v.i] = ua.z[v.i];
+  }
+}
+constexpr float kConfineFloor = 1e-4f;  // SH/apply-vorticity-confinement.wgsl.ts: max(length(grad), 0.0001)
+// grad|w| at an interior voxel (central differences; |w| is 0 on the boundary)
+__device__ __forceinline__ F3 grad_mag(const Grid& g, const float* wmag, uint32_t x, uint32_t y, uint32_t z) {
+  return f3(wmag[vidx(g, x + 1, y, z)] - wmag[vidx(g, x - 1, y, z)], wmag[vidx(g, x, y + 1, z)] - wmag[vidx(g, x, y - 1, z)],
+            wmag[vidx(g, x, y, z + 1)] - wmag[vidx(g, x, y, z - 1)]) * 0.5f;
+}
+__global__ __launch_bounds__(512) void k_df_confine(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                    Vec3 w, const float* __restrict__ wmag, Vec3 ub, Vec3 uc, float eps_dt) {
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    F3 f = f3(0.f, 0.f, 0.f);
+    if (interior(g, v.x, v.y, v.z)) {
+      const F3 gm = grad_mag(g, wmag, v.x, v.y, v.z);
+      const F3 N = gm / fmaxf(len3(gm), kConfineFloor);
+      f = cross3(N, f3(w.x[v.i], w.y[v.i], w.z[v.i])) * eps_dt;
+    }
+    uc.x[v.i] = ub.x[v.i] + f.x;
+    uc.y[v.i] = ub.y[v.i] + f.y;
+    uc.z[v.i] = ub.z[v.i] + f.z;
+  }
+}
+// Loss 0.5 sum (d - target)^2 over the grid: FP64 per-workgroup sums into
+// out[blockIdx % kNormLanes]; db (optional) receives the adjoint seed d - target.
+__global__ __launch_bounds__(512) void k_df_loss(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                 const float* __restrict__ d, const float* __restrict__ target, float* db,
+                                                 double* out) {
+  __shared__ double part[8];
+  double acc = 0.0;
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    const float r = d[v.i] - target[v.i];
+    if (db) db[v.i] = r;
+    acc += 0.5 * double(r) * double(r);
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63u) == 0u) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < 8; ++k) t += part[k];
+    atomicAdd(out + (blockIdx.x % kNormLanes), t);
+  }
+}
+
+// ---- adjoints ----
+// Derivative of the trilinear sample t of f with respect to its position p (0
+// along an axis where tri() clamped p).
+__device__ __forceinline__ F3 tri_grad(const Grid& g, F3 p, const Tri& t, const float* __restrict__ f) {
+  const float c0 = f[t.i[0]], c1 = f[t.i[1]], c2 = f[t.i[2]], c3 = f[t.i[3]];
+  const float c4 = f[t.i[4]], c5 = f[t.i[5]], c6 = f[t.i[6]], c7 = f[t.i[7]];
+  const float ax = 1.f - t.fx, ay = 1.f - t.fy, az = 1.f - t.fz;
+  F3 r;
+  r.x = ((c1 - c0) * ay + (c3 - c2) * t.fy) * az + ((c5 - c4) * ay + (c7 - c6) * t.fy) * t.fz;
+  r.y = ((c2 - c0) * ax + (c3 - c1) * t.fx) * az + ((c6 - c4) * ax + (c7 - c5) * t.fx) * t.fz;
+  r.z = ((c4 - c0) * ax + (c5 - c1) * t.fx) * ay + ((c6 - c2) * ax + (c7 - c3) * t.fx) * t.fy;
+  if (!(p.x > 0.f && p.x < float(g.W - 1))) r.x = 0.f;
+  if (!(p.y > 0.f && p.y < float(g.H - 1))) r.y = 0.f;
+  if (!(p.z > 0.f && p.z < float(g.D - 1))) r.z = 0.f;
+  return r;
+}
+// Transposed trilinear gather: f[corner] += weight * a.
+__device__ __forceinline__ void tri_scatter(const Tri& t, float* f, float a) {
+  if (a == 0.f) return;
+  const float ax = 1.f - t.fx, ay = 1.f - t.fy, az = 1.f - t.fz;
+  atomicAdd(&f[t.i[0]], ax * ay * az * a);
+  atomicAdd(&f[t.i[1]], t.fx * ay * az * a);
+  atomicAdd(&f[t.i[2]], ax * t.fy * az * a);
+  atomicAdd(&f[t.i[3]], t.fx * t.fy * az * a);
+  atomicAdd(&f[t.i[4]], ax * ay * t.fz * a);
+  atomicAdd(&f[t.i[5]], t.fx * ay * t.fz * a);
+  atomicAdd(&f[t.i[6]], ax * t.fy * t.fz * a);
+  atomicAdd(&f[t.i[7]], t.fx * t.fy * t.fz * a);
+}
+// Zero a velocity adjoint on the boundary faces (the forward velocity there is
+// the constant 0 written by the projection, so its adjoint is discarded).
+__global__ __launch_bounds__(512) void k_da_zero_boundary(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                          Vec3 a) {
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (v.in && !interior(g, v.x, v.y, v.z)) { a.x[v.i] = 0.f; a.y[v.i] = 0.f; a.z[v.i] = 0.f; }
+  }
+}
+// Confinement adjoint, part 1. With f = eps dt (N x w) and abar = adjoint of u_c:
+// Nbar = w x fbar, wbar = fbar x N, gbar = J_N(g)^T Nbar where N = g / max(|g|, floor).
+__global__ __launch_bounds__(512) void k_da_confine1(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                     Vec3 w, const float* __restrict__ wmag, Vec3 abar, float eps_dt, Vec3 gbar,
+                                                     Vec3 wbar) {
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    const uint32_t i = v.i;
+    F3 gb = f3(0.f, 0.f, 0.f), wb = f3(0.f, 0.f, 0.f);
+    if (interior(g, v.x, v.y, v.z)) {
+      const F3 fb = f3(abar.x[i], abar.y[i], abar.z[i]) * eps_dt;
+      const F3 om = f3(w.x[i], w.y[i], w.z[i]);
+      const F3 gm = grad_mag(g, wmag, v.x, v.y, v.z);
+      const float n = len3(gm);
+      const F3 Nb = cross3(om, fb);
+      if (n > kConfineFloor) {
+        const F3 N = gm / n;
+        wb = cross3(fb, N);
+        gb = (Nb - N * dot3(N, Nb)) / n;
+      } else {
+        wb = cross3(fb, gm / kConfineFloor);
+        gb = Nb / kConfineFloor;
+      }
+    }
+    gbar.x[i] = gb.x; gbar.y[i] = gb.y; gbar.z[i] = gb.z;
+    wbar.x[i] = wb.x; wbar.y[i] = wb.y; wbar.z[i] = wb.z;
+  }
+}
+// Part 2: |w|bar = (grad)^T gbar = 0.5 sum_a (gbar_a[x - e_a] - gbar_a[x + e_a])
+// (gbar is 0 off the interior), then wbar += |w|bar w / |w|.
+__global__ __launch_bounds__(512) void k_da_confine2(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                     Vec3 w, const float* __restrict__ wmag, Vec3 gbar, Vec3 wbar) {
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in || !interior(g, v.x, v.y, v.z)) continue;
+    const uint32_t x = v.x, y = v.y, z = v.z, i = v.i;
+    const float m = wmag[i];
+    if (!(m > 0.f)) continue;
+    const float mb = 0.5f * ((gbar.x[vidx(g, x - 1, y, z)] - gbar.x[vidx(g, x + 1, y, z)]) +
+                             (gbar.y[vidx(g, x, y - 1, z)] - gbar.y[vidx(g, x, y + 1, z)]) +
+                             (gbar.z[vidx(g, x, y, z - 1)] - gbar.z[vidx(g, x, y, z + 1)]));
+    const float s = mb / m;
+    wbar.x[i] += s * w.x[i];
+    wbar.y[i] += s * w.y[i];
+    wbar.z[i] += s * w.z[i];
+  }
+}
+// Part 3: abar += curl^T wbar. The central-difference curl is its own transpose
+// on fields that vanish off the interior, so this is curl(wbar) at interior voxels.
+__global__ __launch_bounds__(512) void k_da_confine3(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                     Vec3 wbar, Vec3 abar) {
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in || !interior(g, v.x, v.y, v.z)) continue;
+    const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
+    auto rv = [&](uint32_t a, uint32_t b2, uint32_t c2) {
+      const uint32_t k = vidx(g, a, b2, c2);
+      return f3(wbar.x[k], wbar.y[k], wbar.z[k]);
+    };
+    const F3 x0 = rv(x - 1, y, z), x1 = rv(x + 1, y, z), y0 = rv(x, y - 1, z), y1 = rv(x, y + 1, z);
+    const F3 z0 = rv(x, y, z - 1), z1 = rv(x, y, z + 1);
+    const F3 c = f3((y1.z - y0.z) - (z1.y - z0.y), (z1.x - z0.x) - (x1.z - x0.z), (x1.y - x0.y) - (y1.x - y0.x)) * 0.5f;
+    abar.x[i] += c.x;
+    abar.y[i] += c.y;
+    abar.z[i] += c.z;
+  }
+}
+// Buoyancy adjoint: dbar' += beta dt abar_y (interior).
+__global__ __launch_bounds__(512) void k_da_buoyancy(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                     const float* __restrict__ aby, float* dbar, float beta_dt) {
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (v.in && interior(g, v.x, v.y, v.z)) dbar[v.i] += beta_dt * aby[v.i];
+  }
+}
+// MacCormack adjoint. From dnb (adjoint of d') it scatter-adds into dsb (adjoint
+// of d_s, cleared by the caller) and abar (adjoint of u_a). The limiter is a
+// clamp to [min(orig, pred), max(orig, pred)]; its subgradient passes the adjoint
+// to the corrected value inside the interval and to the active bound outside.
+__global__ __launch_bounds__(512) void k_da_advect_d(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                     const float* __restrict__ d, Vec3 ua, const float* __restrict__ dnb, float* dsb,
+                                                     Vec3 abar, float dt) {
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in) continue;
+    const float ob = dnb[v.i];
+    if (ob == 0.f) continue;
+    const F3 sp = f3(float(v.x) - ua.x[v.i] * dt, float(v.y) - ua.y[v.i] * dt, float(v.z) - ua.z[v.i] * dt);
+    const Tri tp = tri(g, sp);
+    const F3 fwd = f3(tri_eval(tp, ua.x), tri_eval(tp, ua.y), tri_eval(tp, ua.z));
+    const F3 sr = sp + fwd * dt;
+    const Tri tr = tri(g, sr);
+    const float pred = tri_eval(tp, d), rec = tri_eval(tr, d), orig = d[v.i];
+    const float corr = pred + 0.5f * (orig - rec);
+    const float lo = fminf(orig, pred), hi = fmaxf(orig, pred);
+    float cb = 0.f, predb = 0.f, origb = 0.f;
+    if (corr < lo) {
+      if (orig <= pred) origb = ob; else predb = ob;
+    } else if (corr > hi) {
+      if (orig >= pred) origb = ob; else predb = ob;
+    } else {
+      cb = ob;
+    }
+    predb += cb;
+    origb += 0.5f * cb;
+    const float recb = -0.5f * cb;
+    atomicAdd(&dsb[v.i], origb);
+    tri_scatter(tp, dsb, predb);
+    tri_scatter(tr, dsb, recb);
+    // sp = x - u_a(x) dt; sr = sp + u_a(sp) dt
+    const F3 srb = recb != 0.f ? tri_grad(g, sr, tr, d) * recb : f3(0.f, 0.f, 0.f);
+    F3 spb = srb;
+    if (predb != 0.f) spb = spb + tri_grad(g, sp, tp, d) * predb;
+    if (recb != 0.f) {
+      const F3 fb = srb * dt;
+      tri_scatter(tp, abar.x, fb.x);
+      tri_scatter(tp, abar.y, fb.y);
+      tri_scatter(tp, abar.z, fb.z);
+      spb = spb + tri_grad(g, sp, tp, ua.x) * fb.x + tri_grad(g, sp, tp, ua.y) * fb.y + tri_grad(g, sp, tp, ua.z) * fb.z;
+    }
+    if (spb.x != 0.f) atomicAdd(&abar.x[v.i], -dt * spb.x);
+    if (spb.y != 0.f) atomicAdd(&abar.y[v.i], -dt * spb.y);
+    if (spb.z != 0.f) atomicAdd(&abar.z[v.i], -dt * spb.z);
+  }
+}
+// Velocity advection adjoint: from abar (adjoint of u_a) scatter-adds into usb
+// (adjoint of u_s, cleared by the caller).
+__global__ __launch_bounds__(512) void k_da_advect_u(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                     Vec3 u, Vec3 abar, Vec3 usb, float dt) {
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in || !interior(g, v.x, v.y, v.z)) continue;
+    const F3 a = f3(abar.x[v.i], abar.y[v.i], abar.z[v.i]);
+    if (a.x == 0.f && a.y == 0.f && a.z == 0.f) continue;
+    const F3 sp = f3(float(v.x) - u.x[v.i] * dt, float(v.y) - u.y[v.i] * dt, float(v.z) - u.z[v.i] * dt);
+    const Tri t = tri(g, sp);
+    tri_scatter(t, usb.x, a.x);
+    tri_scatter(t, usb.y, a.y);
+    tri_scatter(t, usb.z, a.z);
+    const F3 spb = tri_grad(g, sp, t, u.x) * a.x + tri_grad(g, sp, t, u.y) * a.y + tri_grad(g, sp, t, u.z) * a.z;
+    if (spb.x != 0.f) atomicAdd(&usb.x[v.i], -dt * spb.x);
+    if (spb.y != 0.f) atomicAdd(&usb.y[v.i], -dt * spb.y);
+    if (spb.z != 0.f) atomicAdd(&usb.z[v.i], -dt * spb.z);
+  }
+}
+// Source adjoint: cbar[ch] += dt * taps^T (u_s bar, d_s bar) over the slab.
+__global__ __launch_bounds__(512) void k_da_source(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                   uint32_t cn, uint32_t slab, float dt, const float* __restrict__ dsb, Vec3 usb,
+                                                   float* cbar) {
+  const size_t plane = size_t(cn) * cn * cn;
+  FX_BRICK_LOOP(list, count) {
+    Vox v = vox_of(g, list[fx_k]);
+    if (!v.in || !in_slab(g, slab, v.x, v.y, v.z)) continue;
+    uint32_t idx[8];
+    float w[8];
+    control_taps(g, cn, slab, v.x, v.y, v.z, idx, w);
+    const float a[4] = {dt * usb.x[v.i], dt * usb.y[v.i], dt * usb.z[v.i], dt * dsb[v.i]};
+#pragma unroll
+    for (int ch = 0; ch < 4; ++ch) {
+      if (a[ch] == 0.f) continue;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (w[k] != 0.f) atomicAdd(&cbar[ch * plane + idx[k]], w[k] * a[ch]);
+    }
+  }
+}
+
 // ------------------------------------------------------------ launchers --
 #define FX_LAUNCH(kernel, blocks, threads, stream, ...) \
   hipLaunchKernelGGL(kernel, dim3(blocks), dim3(threads), 0, stream, __VA_ARGS__)
@@ -2093,6 +2442,47 @@ void launch_raymarch(hipStream_t s, Grid g, const RayConst& rc, const void* rb, 
   dim3 grid(std::max(1u, std::min(ntiles, uint32_t(std::max(blocks, 1)))));
   const size_t sh = lds ? nwords * 4 : 0;
   hipLaunchKernelGGL(k_raymarch, grid, dim3(256), sh, s, g, rc, static_cast<const float4*>(rb), bits, nwords, lds, out, queue);
+}
+
+// ---------------------------------------------- differentiable smoke launchers --
+void launch_df_source(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const float* ctrl, uint32_t cn,
+                      uint32_t slab, float dt, const float* d, Vec3 u, float* ds, Vec3 us) {
+  FX_LAUNCH(k_df_source, blocks, 512, s, list, count, g, ctrl, cn, slab, dt, d, u, ds, us);
+}
+void launch_df_advect(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 us, Vec3 ua,
+                      const float* ds, float* dn, float dt) {
+  FX_LAUNCH(k_df_advect_u, blocks, 512, s, list, count, g, us, ua, dt);
+  FX_LAUNCH(k_df_advect_d, blocks, 512, s, list, count, g, ds, ua, dn, dt);
+}
+void launch_df_forces(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const float* dn, Vec3 ua, Vec3 ub,
+                      Vec3 w, float* wmag, Vec3 uc, float beta_dt, float eps_dt) {
+  FX_LAUNCH(k_df_buoyancy, blocks, 512, s, list, count, g, dn, ua, ub, beta_dt);
+  FX_LAUNCH(k_vorticity, blocks, 512, s, list, count, g, ub, w, wmag);
+  FX_LAUNCH(k_df_confine, blocks, 512, s, list, count, g, w, wmag, ub, uc, eps_dt);
+}
+void launch_df_loss(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const float* d, const float* target,
+                    float* db, double* out) {
+  FX_LAUNCH(k_df_loss, blocks, 512, s, list, count, g, d, target, db, out);
+}
+void launch_da_zero_boundary(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 a) {
+  FX_LAUNCH(k_da_zero_boundary, blocks, 512, s, list, count, g, a);
+}
+void launch_da_forces(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 ub, Vec3 w, float* wmag,
+                      Vec3 abar, Vec3 gbar, Vec3 wbar, float* dnb, float beta_dt, float eps_dt) {
+  FX_LAUNCH(k_vorticity, blocks, 512, s, list, count, g, ub, w, wmag);
+  FX_LAUNCH(k_da_confine1, blocks, 512, s, list, count, g, w, wmag, abar, eps_dt, gbar, wbar);
+  FX_LAUNCH(k_da_confine2, blocks, 512, s, list, count, g, w, wmag, gbar, wbar);
+  FX_LAUNCH(k_da_confine3, blocks, 512, s, list, count, g, wbar, abar);
+  FX_LAUNCH(k_da_buoyancy, blocks, 512, s, list, count, g, abar.y, dnb, beta_dt);
+}
+void launch_da_advect(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const float* ds, Vec3 us, Vec3 ua,
+                      const float* dnb, float* dsb, Vec3 abar, Vec3 usb, float dt) {
+  FX_LAUNCH(k_da_advect_d, blocks, 512, s, list, count, g, ds, ua, dnb, dsb, abar, dt);
+  FX_LAUNCH(k_da_advect_u, blocks, 512, s, list, count, g, us, abar, usb, dt);
+}
+void launch_da_source(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, uint32_t cn, uint32_t slab,
+                      float dt, const float* dsb, Vec3 usb, float* cbar) {
+  FX_LAUNCH(k_da_source, blocks, 512, s, list, count, g, cn, slab, dt, dsb, usb, cbar);
 }
 
 }  // namespace fx

@@ -401,6 +401,60 @@ int fx_autotune_vcycle(fx_sim* sim, uint32_t repeats, uint32_t* down_split, uint
  * voxel count one launch updates. Synchronises; V-cycle solver only. */
 int fx_time_fine_sweep(fx_sim* sim, uint32_t repeats, double* avg_ms, uint64_t* voxels);
 
+/* ---- differentiable smoke step (no reference counterpart) ---------------- */
+
+/* Inflow-control optimisation needs gradients of a rollout with respect to a
+ * source control; the reference computes none. fx_diff runs a differentiable
+ * model of the step on a dense grid: state density d and velocity u (voxel
+ * units, 0 initially), and per step
+ *   source       d += dt S_rho, u += dt S_u on the floor slab (interior voxels with
+ *                1 <= y <= slab), S the cell-centred trilinear upsampling of the
+ *                control over the slab box;
+ *   advection    u_a = u(x - u(x) dt) (semi-Lagrangian, 0 on the boundary faces),
+ *                d' = MacCormack(d, u_a) with the min/max(original, predicted)
+ *                limiter of the scalar advection, without its decay terms;
+ *   buoyancy     u_y += dt buoyancy d';
+ *   confinement  u += dt confinement (N x w), w = curl u, N = grad|w| / max(|grad|w||, 1e-4);
+ *   projection   u -= grad P(div u), P the multigrid V-cycle of fx_vcycle.
+ * The control holds 4 x control_n^3 floats, row-major [channel][z][y][x], channels
+ * (u_x, u_y, u_z, rho). The loss of a rollout is 0.5 sum (d_T - target)^2 over the
+ * grid (FP64 reduction). fx_diff_gradient runs the rollout storing (d, u) every
+ * checkpoint_every steps, then walks the segments backwards: each segment is
+ * recomputed from its checkpoint with the per-step intermediates kept, and the
+ * adjoint kernels run it in reverse (transposed trilinear gathers are float
+ * atomics, so gradients vary in the last bits from run to run; the forward pass is
+ * deterministic). The pressure projection is its own adjoint because the V-cycle
+ * is symmetric (equal pre- and post-sweeps, restriction proportional to the
+ * transposed prolongation), so pre_sweeps must equal post_sweeps. */
+typedef struct fx_diff fx_diff;
+typedef struct fx_diff_desc {
+  uint32_t width, height, depth; /* dense grid, each >= 8 */
+  uint32_t steps;                /* steps per rollout */
+  uint32_t checkpoint_every;     /* steps per recomputed segment */
+  uint32_t control_n;            /* control edge, >= 2 */
+  uint32_t slab;                 /* source slab height (voxels above the floor face) */
+  float dt, buoyancy, confinement;
+  fx_vcycle vcycle;
+  int32_t device;
+} fx_diff_desc;
+
+/* 64^3, 64 steps, checkpoint 8, control 32, slab 4, dt 1, buoyancy 0.05,
+ * confinement 0.3, V-cycle 4 levels 2/2 x 4, 16 coarse sweeps, omega 2/3. */
+void fx_diff_default_desc(fx_diff_desc* desc);
+int fx_diff_create(const fx_diff_desc* desc, fx_diff** out);
+void fx_diff_destroy(fx_diff* diff);
+/* Target density, row-major [z][y][x] floats (bytes = 4 W H D). Zero until set. */
+int fx_diff_set_target(fx_diff* diff, const float* host, size_t bytes);
+/* One rollout from the zero state under `control` (host, 4 control_n^3 floats);
+ * writes the loss. The final state stays readable with fx_diff_read_field. */
+int fx_diff_forward(fx_diff* diff, const float* control, double* loss);
+/* Loss and its gradient with respect to the control (host, 4 control_n^3 floats). */
+int fx_diff_gradient(fx_diff* diff, const float* control, double* loss, float* grad);
+/* Final state of the last rollout: field 0 density, 1..3 velocity x, y, z;
+ * row-major [z][y][x] floats (bytes = 4 W H D). */
+int fx_diff_read_field(fx_diff* diff, int32_t field, float* host, size_t bytes);
+const char* fx_diff_last_error(fx_diff* diff);
+
 /* ---- test hook ---------------------------------------------------------- */
 
 /* Runs one pass of the step with explicit uniforms (no reference counterpart:

@@ -133,3 +133,43 @@ class Oracle:
         out = np.zeros((h, w, 4), dtype=np.float32)
         self._ok(self.L.orc_raymarch(self.h, C.byref(cam), C.byref(params), w, h, out.ctypes.data_as(C.POINTER(C.c_float))))
         return out
+
+
+_DIFF = None
+
+
+def diff_model():
+    """The FP64 restatement of the differentiable step (oracle/diff_model.py)."""
+    global _DIFF
+    if _DIFF is None:
+        import importlib.util
+        spec = importlib.util.spec_from_file_location("fenix_oracle_diff_model",
+                                                      os.path.join(ROOT, "oracle", "diff_model.py"))
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        _DIFF = mod
+    return _DIFF
+
+
+class OracleProjector:
+    """Pressure projection u = us - grad P(div us) of a dense grid through the
+    oracle's divergence, V-cycle and projection passes (FP64 by default)."""
+
+    def __init__(self, shape, vcycle=None, bits=64):
+        from fenix_amd.simulation import make_desc
+        D, H, W = shape
+        v = {"levels": 4, "pre_sweeps": 2, "post_sweeps": 2, "cycles": 4, "coarse_sweeps": 16, "omega": 2.0 / 3.0}
+        v.update(vcycle or {})
+        self.orc = Oracle(make_desc((W, H, D), solver=abi.FX_SOLVER_VCYCLE, vcycle=v), bits)
+        p = abi.SimParams()
+        p.delta_time, p.dx, p.world_size = 1.0, 1.0, float(max(W, D))
+        self.params = p
+        self.calls = 0
+
+    def __call__(self, us):
+        for name, comp in zip(("scratch_x", "scratch_y", "scratch_z"), us):
+            self.orc.write(name, comp)
+        for name in ("divergence", "pressure", "projection"):
+            self.orc.run_pass(name, self.params)
+        self.calls += 1
+        return [self.orc.read(n) for n in ("velocity_x", "velocity_y", "velocity_z")]

@@ -41,6 +41,7 @@ STRUCTS = {
     "fx_camera": (abi.Camera, ["aspect", "tan_half_fov_y"]),
     "fx_light": (abi.Light, ["intensity"]),
     "fx_render_params": (abi.RenderParams, ["lights", "scattering_back", "elapsed_s"]),
+    "fx_diff_desc": (abi.DiffDesc, ["slab", "dt", "confinement", "vcycle", "device"]),
 }
 
 
@@ -107,3 +108,17 @@ def test_product_does_not_reference_the_oracle(path):
         assert "orc_" not in out
     else:
         assert os.path.exists(os.path.join(ROOT, "oracle", "oracle.cpp"))
+
+
+def test_diff_create_rejects_bad_args_without_a_gpu():
+    lib = abi.load()
+    from fenix_amd.differentiable import diff_desc
+    h = C.c_void_p()
+    d = diff_desc(lib=lib)
+    d.vcycle.post_sweeps = d.vcycle.pre_sweeps + 1  # the adjoint projection needs a symmetric V-cycle
+    assert lib.fx_diff_create(C.byref(d), C.byref(h)) == abi.FX_ERR_INVALID_ARG
+    assert b"symmetric" in lib.fx_diff_last_error(None)
+    d = diff_desc(lib=lib, slab=0)
+    assert lib.fx_diff_create(C.byref(d), C.byref(h)) == abi.FX_ERR_INVALID_ARG
+    assert lib.fx_diff_create(None, C.byref(h)) == abi.FX_ERR_INVALID_ARG
+    lib.fx_diff_destroy(None)  # accepted

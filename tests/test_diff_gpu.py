@@ -1,0 +1,167 @@
+"""The differentiable step on the GPU (fx_diff_* through the C ABI):
+
+* forward rollouts against the FP64 restatement (oracle/diff_model.py with the
+  FP64 oracle's V-cycle projection);
+* adjoint gradients against the FP64 adjoint (itself pinned by central
+  differences on the CPU), for several checkpoint intervals;
+* FP32 central differences (h = 1e-3) on random control entries at the
+  inflow-control configuration (128^3, 64 steps, control 32^3 x 4, checkpoint 8);
+* the torch Function pair and an Adam inflow optimisation that lowers the loss.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from fenix_amd.differentiable import (SmokeRollout, diff_desc, gaussian_blobs, optimise_inflow, plume_control,
+                                      rollout_loss)
+from oracle import OracleProjector, diff_model
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _report(name, payload):
+    out = os.path.join(ROOT, "gpurun_out")
+    os.makedirs(out, exist_ok=True)
+    with open(os.path.join(out, name), "w") as fh:
+        json.dump(payload, fh, indent=1)
+
+
+SMALL = dict(resolution=(32, 32, 32), steps=8, control_n=8, slab=4, dt=1.0, buoyancy=0.05, confinement=0.3)
+
+
+def _small_pair(checkpoint_every=3):
+    n = SMALL["resolution"][0]
+    desc = diff_desc(checkpoint_every=checkpoint_every, **SMALL)
+    gpu = SmokeRollout(desc)
+    shape = (n, n, n)
+    ref = diff_model().DiffModel(shape, steps=SMALL["steps"], control_n=SMALL["control_n"], slab=SMALL["slab"],
+                                 dt=SMALL["dt"], buoyancy=SMALL["buoyancy"], confinement=SMALL["confinement"],
+                                 project=OracleProjector(shape))
+    target = gaussian_blobs(shape, [(16, 20, 16), (10, 12, 20)], 4.0, amplitude=0.5)
+    gpu.set_target(target)
+    ref.target = target.astype(np.float64)
+    rng = np.random.default_rng(21)
+    ctrl = (plume_control(SMALL["control_n"], rise=0.6, density=0.8, radius=0.3)
+            + 0.1 * rng.standard_normal((4,) + (SMALL["control_n"],) * 3)).astype(np.float32)
+    return gpu, ref, ctrl
+
+
+def test_forward_matches_fp64():
+    gpu, ref, ctrl = _small_pair()
+    loss = gpu.forward(ctrl)
+    d, u, _ = ref.rollout(ctrl.astype(np.float64))
+    errs = {"density": rel_l2(gpu.read_field(0), d)}
+    for c in range(3):
+        errs[f"velocity_{'xyz'[c]}"] = rel_l2(gpu.read_field(1 + c), u[c])
+    ref_loss = 0.5 * float(np.sum((d - ref.target) ** 2))
+    print("forward rel L2 vs FP64:", errs, "loss", loss, ref_loss, flush=True)
+    for k, e in errs.items():
+        assert e <= 1e-4, (k, e)
+    assert abs(loss - ref_loss) <= 1e-4 * ref_loss
+
+
+@pytest.mark.parametrize("checkpoint_every", [1, 3, 8])
+def test_gradient_matches_fp64_adjoint(checkpoint_every):
+    gpu, ref, ctrl = _small_pair(checkpoint_every)
+    loss, grad = gpu.gradient(ctrl)
+    ref_loss, ref_grad = ref.gradient(ctrl.astype(np.float64))
+    e = rel_l2(grad, ref_grad)
+    print(f"checkpoint {checkpoint_every}: gradient rel L2 vs FP64 adjoint {e:.3e}, loss {loss} vs {ref_loss}", flush=True)
+    assert np.all(np.isfinite(grad))
+    assert np.abs(ref_grad).max() > 0
+    assert e <= 1e-3, e
+    assert abs(loss - ref_loss) <= 1e-4 * ref_loss
+    # the state after the gradient call is the rollout's final state
+    d, _, _ = ref.rollout(ctrl.astype(np.float64))
+    assert rel_l2(gpu.read_field(0), d) <= 1e-4
+
+
+def test_gradient_is_independent_of_checkpoint_interval():
+    grads = []
+    for ce in (1, 5, 8):
+        gpu, _, ctrl = _small_pair(ce)
+        grads.append(gpu.gradient(ctrl)[1])
+        gpu.dispose()
+    for g in grads[1:]:
+        # float atomics reorder the scatter sums; nothing else differs
+        assert rel_l2(g, grads[0]) <= 1e-5
+
+
+C5 = dict(resolution=(128, 128, 128), steps=64, checkpoint_every=8, control_n=32, slab=4, dt=1.0, buoyancy=0.05,
+          confinement=0.3)
+
+
+def c5_problem():
+    """Inflow control at 128^3: 64 steps, control 32^3 x 4 into the slab y in [1, 4],
+    target = 3 Gaussian blobs (sigma 12 voxels, centres drawn with seed 5)."""
+    n = C5["resolution"][0]
+    rng = np.random.default_rng(5)
+    centres = [(rng.uniform(0.3, 0.7) * n, rng.uniform(0.3, 0.55) * n, rng.uniform(0.3, 0.7) * n) for _ in range(3)]
+    target = gaussian_blobs((n, n, n), centres, 12.0, amplitude=0.5)
+    ctrl = plume_control(C5["control_n"], rise=0.5, density=1.0, radius=0.3)
+    return target, ctrl, centres
+
+
+def test_fp32_central_differences_at_inflow_configuration():
+    """32 control entries (seed 6) among those the slab samples: FP32 central
+    differences with h = 1e-3 against the adjoint gradient."""
+    gpu = SmokeRollout(diff_desc(**C5))
+    target, ctrl, centres = c5_problem()
+    gpu.set_target(target)
+    loss, grad = gpu.gradient(ctrl)
+    flat = grad.reshape(-1)
+    live = np.nonzero(flat != 0)[0]
+    rng = np.random.default_rng(6)
+    pick = rng.choice(live, size=32, replace=False)
+    h = 1e-3
+    fd = []
+    for idx in pick:
+        c = ctrl.reshape(-1).copy()
+        c[idx] += h
+        lp = gpu.forward(c)
+        c[idx] -= 2 * h
+        lm = gpu.forward(c)
+        fd.append((lp - lm) / (2 * h))
+    fd = np.array(fd)
+    ad = flat[pick].astype(np.float64)
+    rel = rel_l2(ad, fd)
+    per = np.abs(ad - fd) / np.maximum(np.abs(fd), 1e-30)
+    report = {"config": "128^3 dense FP32, 64 steps, checkpoint 8, control 32^3x4, slab y in [1,4], dt 1, "
+                        "buoyancy 0.05, confinement 0.3, target 3 blobs sigma 12 (seed 5)",
+              "loss": loss, "h": h, "entries": [int(i) for i in pick], "adjoint": ad.tolist(), "fd": fd.tolist(),
+              "rel_l2": rel, "median_rel": float(np.median(per)), "max_rel": float(per.max()),
+              "within_1e-3": int(np.sum(per <= 1e-3))}
+    _report("diff_fd_report.json", report)
+    print(json.dumps({k: report[k] for k in ("loss", "rel_l2", "median_rel", "max_rel", "within_1e-3")}), flush=True)
+    assert rel <= 1e-2, report
+
+
+def test_torch_function_and_adam_lower_the_loss():
+    import torch
+    n = 48
+    desc = diff_desc(resolution=(n, n, n), steps=24, checkpoint_every=8, control_n=16, slab=4)
+    gpu = SmokeRollout(desc)
+    gpu.set_target(gaussian_blobs((n, n, n), [(24, 28, 24)], 6.0, amplitude=0.5))
+    ctrl = plume_control(16, rise=0.5, density=1.0, radius=0.3)
+    # the Function pair returns the adjoint gradient
+    param = torch.nn.Parameter(torch.from_numpy(ctrl.copy()))
+    loss = rollout_loss(param, gpu)
+    loss.backward()
+    l2, g2 = gpu.gradient(ctrl)
+    assert abs(float(loss) - l2) <= 1e-9 * l2
+    assert rel_l2(param.grad.numpy(), g2) <= 1e-5
+    with torch.no_grad():
+        assert abs(float(rollout_loss(param, gpu)) - l2) <= 1e-9 * l2
+    _, losses = optimise_inflow(gpu, ctrl, 12, lr=1e-2)
+    print("adam losses", losses, flush=True)
+    assert losses[-1] < 0.97 * losses[0], losses
