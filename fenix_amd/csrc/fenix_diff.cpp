@@ -55,6 +55,7 @@ struct fx_diff {
   float* aA[3] = {};               // adjoint of the velocity inside the step
   float* gbar[3] = {};
   float* wbar[3] = {};
+  float* stage = nullptr;          // staged scatter tiles of the advection adjoints
   double* loss_acc = nullptr;      // kNormLanes partial sums
   std::string err;
 };
@@ -145,7 +146,8 @@ int adjoint_step(fx_diff* f, const Tape& T) {
   // advection of density and velocity
   launch_clear_f32(v.stream, f->dB, f->n);
   for (int c = 0; c < 3; ++c) launch_clear_f32(v.stream, f->uB[c], f->n);
-  launch_da_advect(v.stream, v.blocks, v.list, v.count, v.g, T.ds, v3(T.us), v3(T.ua), f->dA, f->dB, v3(f->aA), v3(f->uB), D.dt);
+  launch_da_advect(v.stream, v.blocks, v.list, v.count, v.g, v.nb, T.ds, v3(T.us), v3(T.ua), f->dA, f->dB, v3(f->aA), v3(f->uB), D.dt,
+                   f->stage);
   // source
   launch_da_source(v.stream, v.blocks, v.list, v.count, v.g, D.control_n, D.slab, D.dt, f->dB, v3(f->uB), f->cbar);
   std::swap(f->dA, f->dB);
@@ -167,8 +169,8 @@ void fx_diff_default_desc(fx_diff_desc* d) {
   d->control_n = 32;
   d->slab = 4;
   d->dt = 1.0f;
-  d->buoyancy = 0.05f;
-  d->confinement = 0.3f;
+  d->buoyancy = 0.02f;
+  d->confinement = 0.02f;
   d->vcycle.levels = 4;
   d->vcycle.pre_sweeps = 2;
   d->vcycle.post_sweeps = 2;
@@ -225,7 +227,7 @@ int fx_diff_create(const fx_diff_desc* desc, fx_diff** out) {
   const uint32_t nslots = std::min(D.checkpoint_every, D.steps);
   auto A = [&](float** p, size_t cnt) { return rc == FX_OK ? (rc = alloc(f, p, cnt)) : rc; };
   A(&f->d, f->n); A(&f->rd, f->n); A(&f->target, f->n); A(&f->wmag, f->n); A(&f->dA, f->n); A(&f->dB, f->n);
-  A(&f->ctrl, f->cfl); A(&f->cbar, f->cfl);
+  A(&f->ctrl, f->cfl); A(&f->cbar, f->cfl); A(&f->stage, da_stage_floats(f->v.nb));
   for (int c = 0; c < 3; ++c) {
     A(&f->u[c], f->n); A(&f->ru[c], f->n); A(&f->uc[c], f->n); A(&f->w[c], f->n);
     A(&f->uA[c], f->n); A(&f->uB[c], f->n); A(&f->aA[c], f->n); A(&f->gbar[c], f->n); A(&f->wbar[c], f->n);
@@ -247,6 +249,12 @@ int fx_diff_create(const fx_diff_desc* desc, fx_diff** out) {
     }
   }
   if (rc == FX_OK && (rc = sim_prepare_projection(f->sim)) != FX_OK) fail(f, rc, fx_last_error(f->sim));
+  if (rc == FX_OK) {
+    // Kernel plan of the projection's V-cycle for this dense grid (every plan
+    // gives bit-identical results; timing does not depend on the field values).
+    uint32_t down = 0, up = 0;
+    if ((rc = fx_autotune_vcycle(f->sim, 3, &down, &up)) != FX_OK) fail(f, rc, fx_last_error(f->sim));
+  }
   if (rc == FX_OK) {
     launch_clear_f32(f->v.stream, f->target, f->n);
     if (hipStreamSynchronize(f->v.stream) != hipSuccess) rc = fail(f, FX_ERR_DEVICE, "initialisation failed");

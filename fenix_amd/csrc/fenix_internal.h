@@ -236,8 +236,10 @@ void launch_da_zero_boundary(hipStream_t s, int blocks, const uint32_t* list, co
 void launch_da_forces(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 ub, Vec3 w, float* wmag,
                       Vec3 abar, Vec3 gbar, Vec3 wbar, float* dnb, float beta_dt, float eps_dt);
 // dsb, usb (cleared by the caller) += adjoints of d_s, u_s; abar gains the MacCormack velocity terms first.
-void launch_da_advect(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const float* ds, Vec3 us, Vec3 ua,
-                      const float* dnb, float* dsb, Vec3 abar, Vec3 usb, float dt);
+// The dense identity list over nb bricks; stage: da_stage_floats(nb) floats of scratch.
+void launch_da_advect(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, size_t nb, const float* ds,
+                      Vec3 us, Vec3 ua, const float* dnb, float* dsb, Vec3 abar, Vec3 usb, float dt, float* stage);
+size_t da_stage_floats(size_t nb);
 void launch_da_source(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, uint32_t cn, uint32_t slab,
                       float dt, const float* dsb, Vec3 usb, float* cbar);
 

@@ -2054,19 +2054,6 @@ __device__ __forceinline__ F3 tri_grad(const Grid& g, F3 p, const Tri& t, const 
   if (!(p.z > 0.f && p.z < float(g.D - 1))) r.z = 0.f;
   return r;
 }
-// Transposed trilinear gather: f[corner] += weight * a.
-__device__ __forceinline__ void tri_scatter(const Tri& t, float* f, float a) {
-  if (a == 0.f) return;
-  const float ax = 1.f - t.fx, ay = 1.f - t.fy, az = 1.f - t.fz;
-  atomicAdd(&f[t.i[0]], ax * ay * az * a);
-  atomicAdd(&f[t.i[1]], t.fx * ay * az * a);
-  atomicAdd(&f[t.i[2]], ax * t.fy * az * a);
-  atomicAdd(&f[t.i[3]], t.fx * t.fy * az * a);
-  atomicAdd(&f[t.i[4]], ax * ay * t.fz * a);
-  atomicAdd(&f[t.i[5]], t.fx * ay * t.fz * a);
-  atomicAdd(&f[t.i[6]], ax * t.fy * t.fz * a);
-  atomicAdd(&f[t.i[7]], t.fx * t.fy * t.fz * a);
-}
 // Zero a velocity adjoint on the boundary faces (the forward velocity there is
 // the constant 0 written by the projection, so its adjoint is discarded).
 __global__ __launch_bounds__(512) void k_da_zero_boundary(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
@@ -2152,74 +2139,186 @@ __global__ __launch_bounds__(512) void k_da_buoyancy(const uint32_t* __restrict_
     if (v.in && interior(g, v.x, v.y, v.z)) dbar[v.i] += beta_dt * aby[v.i];
   }
 }
+// Transposed gathers staged in LDS, summed without atomics. The adjoint of a
+// semi-Lagrangian sample is a scatter to the 8 corners of the sample; with
+// |u dt| < kScatHalo every corner of a brick's voxels lies in the brick grown by
+// kScatHalo voxels. Phase 1 (the adjoint kernels below): each target field gets
+// a (8 + 2 kScatHalo)^3 LDS tile around the brick, the voxels add into it with
+// LDS atomics, and the tile is stored whole to a staging buffer (plain
+// coalesced stores); corners outside the tile go to the output with a global
+// atomic. Phase 2 (k_da_gather): every voxel adds the entries of the (at most 8)
+// tiles that cover it, in a fixed order. Global float atomics on runs of
+// neighbouring addresses serialise in L2: a per-entry atomic flush of the tiles
+// cost ~10x the rest of the kernel, and so does a halo of 1 when the flow moves
+// faster than a voxel per step.
+constexpr int kScatHalo = 2;
+static_assert(kScatHalo <= 4, "a voxel is covered by at most one neighbour tile per axis");
+constexpr int kScatT = 8 + 2 * kScatHalo;
+constexpr int kScatN = kScatT * kScatT * kScatT;
+struct TileOrigin { int x, y, z; };
+__device__ __forceinline__ TileOrigin tile_origin(const Grid& g, uint32_t b) {
+  const uint32_t bx = b % g.BX, by = (b / g.BX) % g.BY, bz = b / (g.BX * g.BY);
+  return TileOrigin{int(bx * 8) - kScatHalo, int(by * 8) - kScatHalo, int(bz * 8) - kScatHalo};
+}
+// Corner coordinates and fractions of the clamped trilinear sample at p (as tri()).
+struct TriC { int x0, y0, z0, dx, dy, dz; float fx, fy, fz; };
+__device__ __forceinline__ TriC tri_coords(const Grid& g, F3 p) {
+  const float cx = clampf(p.x, 0.f, float(g.W - 1)), cy = clampf(p.y, 0.f, float(g.H - 1)), cz = clampf(p.z, 0.f, float(g.D - 1));
+  TriC c;
+  c.x0 = int(floorf(cx)); c.y0 = int(floorf(cy)); c.z0 = int(floorf(cz));
+  c.dx = c.x0 + 1 < int(g.W) ? 1 : 0; c.dy = c.y0 + 1 < int(g.H) ? 1 : 0; c.dz = c.z0 + 1 < int(g.D) ? 1 : 0;
+  c.fx = fractf(cx); c.fy = fractf(cy); c.fz = fractf(cz);
+  return c;
+}
+// The tiles are passed as generic pointers; the explicit LDS address space keeps
+// the staged adds on ds_add_f32 instead of flat atomics.
+typedef __attribute__((address_space(3))) float lds_float;
+__device__ __forceinline__ void stage_add(float* tile, const TileOrigin& o, float* glob, const Grid& g, int x, int y, int z, float v) {
+  const int lx = x - o.x, ly = y - o.y, lz = z - o.z;
+  if (unsigned(lx) < unsigned(kScatT) && unsigned(ly) < unsigned(kScatT) && unsigned(lz) < unsigned(kScatT))
+    __hip_atomic_fetch_add((lds_float*)tile + ((lz * kScatT + ly) * kScatT + lx), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  else
+    atomicAdd(&glob[vidx(g, uint32_t(x), uint32_t(y), uint32_t(z))], v);
+}
+__device__ __forceinline__ void stage_scatter(float* tile, const TileOrigin& o, float* glob, const Grid& g, const TriC& c, float a) {
+  if (a == 0.f) return;
+  const float wx[2] = {1.f - c.fx, c.fx}, wy[2] = {1.f - c.fy, c.fy}, wz[2] = {1.f - c.fz, c.fz};
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    stage_add(tile, o, glob, g, c.x0 + ((k & 1) ? c.dx : 0), c.y0 + ((k & 2) ? c.dy : 0), c.z0 + ((k & 4) ? c.dz : 0),
+              wx[k & 1] * wy[(k >> 1) & 1] * wz[k >> 2] * a);
+}
+// Stores the tile of list entry k to stage (layout [list entry][kScatN]) and clears it.
+__device__ __forceinline__ void stage_store(float* tile, float* __restrict__ stage, uint32_t k) {
+  float* dst = stage + size_t(k) * kScatN;
+  for (int t = int(threadIdx.x); t < kScatN; t += int(blockDim.x)) {
+    dst[t] = tile[t];
+    tile[t] = 0.f;
+  }
+}
+// Phase 2: out[f][v] += sum of the staged tiles covering v, for nf fields
+// (stage[f] holds one tile per list entry; page maps a brick to its list entry,
+// or the list is the identity when page is null).
+struct GatherArgs {
+  const float* stage[4];
+  float* out[4];
+  int nf;
+};
+__global__ __launch_bounds__(512) void k_da_gather(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                   const int32_t* __restrict__ page, GatherArgs A) {
+  FX_BRICK_LOOP(list, count) {
+    const uint32_t b = list[fx_k];
+    const Vox v = vox_of(g, b);
+    if (!v.in) continue;
+    const int lx = int(v.x & 7u), ly = int(v.y & 7u), lz = int(v.z & 7u);
+    const int bx = int(b % g.BX), by = int((b / g.BX) % g.BY), bz = int(b / (g.BX * g.BY));
+    // neighbour offsets whose tile covers this voxel (own brick first)
+    auto side = [](int l) { return l < kScatHalo ? -1 : (l >= 8 - kScatHalo ? 1 : 0); };
+    const int ox = side(lx), oy = side(ly), oz = side(lz);
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < 8; ++k) {
+      const int sx = (k & 1) ? ox : 0, sy = (k & 2) ? oy : 0, sz = (k & 4) ? oz : 0;
+      if (((k & 1) && !ox) || ((k & 2) && !oy) || ((k & 4) && !oz)) continue;
+      const int nbx = bx + sx, nby = by + sy, nbz = bz + sz;
+      if (nbx < 0 || nby < 0 || nbz < 0 || nbx >= int(g.BX) || nby >= int(g.BY) || nbz >= int(g.BZ)) continue;
+      const uint32_t nb = uint32_t(nbx) + g.BX * (uint32_t(nby) + g.BY * uint32_t(nbz));
+      const int32_t e = page ? page[nb] : int32_t(nb);
+      if (e < 0) continue;
+      const int tx = lx - 8 * sx + kScatHalo, ty = ly - 8 * sy + kScatHalo, tz = lz - 8 * sz + kScatHalo;
+      const size_t off = size_t(e) * kScatN + size_t((tz * kScatT + ty) * kScatT + tx);
+      for (int f = 0; f < A.nf; ++f) acc[f] += A.stage[f][off];
+    }
+    for (int f = 0; f < A.nf; ++f) A.out[f][v.i] += acc[f];
+  }
+}
 // MacCormack adjoint. From dnb (adjoint of d') it scatter-adds into dsb (adjoint
 // of d_s, cleared by the caller) and abar (adjoint of u_a). The limiter is a
 // clamp to [min(orig, pred), max(orig, pred)]; its subgradient passes the adjoint
 // to the corrected value inside the interval and to the active bound outside.
 __global__ __launch_bounds__(512) void k_da_advect_d(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
                                                      const float* __restrict__ d, Vec3 ua, const float* __restrict__ dnb, float* dsb,
-                                                     Vec3 abar, float dt) {
+                                                     Vec3 abar, float dt, float* __restrict__ stage, size_t stage_field) {
+  __shared__ float tile[4][kScatN];  // dsb, abar x, y, z
+  for (int t = int(threadIdx.x); t < 4 * kScatN; t += int(blockDim.x)) (&tile[0][0])[t] = 0.f;
+  __syncthreads();
   FX_BRICK_LOOP(list, count) {
-    Vox v = vox_of(g, list[fx_k]);
-    if (!v.in) continue;
-    const float ob = dnb[v.i];
-    if (ob == 0.f) continue;
-    const F3 sp = f3(float(v.x) - ua.x[v.i] * dt, float(v.y) - ua.y[v.i] * dt, float(v.z) - ua.z[v.i] * dt);
-    const Tri tp = tri(g, sp);
-    const F3 fwd = f3(tri_eval(tp, ua.x), tri_eval(tp, ua.y), tri_eval(tp, ua.z));
-    const F3 sr = sp + fwd * dt;
-    const Tri tr = tri(g, sr);
-    const float pred = tri_eval(tp, d), rec = tri_eval(tr, d), orig = d[v.i];
-    const float corr = pred + 0.5f * (orig - rec);
-    const float lo = fminf(orig, pred), hi = fmaxf(orig, pred);
-    float cb = 0.f, predb = 0.f, origb = 0.f;
-    if (corr < lo) {
-      if (orig <= pred) origb = ob; else predb = ob;
-    } else if (corr > hi) {
-      if (orig >= pred) origb = ob; else predb = ob;
-    } else {
-      cb = ob;
+    const uint32_t b = list[fx_k];
+    const Vox v = vox_of(g, b);
+    const TileOrigin o = tile_origin(g, b);
+    const float ob = v.in ? dnb[v.i] : 0.f;
+    if (ob != 0.f) {
+      const F3 sp = f3(float(v.x) - ua.x[v.i] * dt, float(v.y) - ua.y[v.i] * dt, float(v.z) - ua.z[v.i] * dt);
+      const Tri tp = tri(g, sp);
+      const F3 fwd = f3(tri_eval(tp, ua.x), tri_eval(tp, ua.y), tri_eval(tp, ua.z));
+      const F3 sr = sp + fwd * dt;
+      const Tri tr = tri(g, sr);
+      const float pred = tri_eval(tp, d), rec = tri_eval(tr, d), orig = d[v.i];
+      const float corr = pred + 0.5f * (orig - rec);
+      const float lo = fminf(orig, pred), hi = fmaxf(orig, pred);
+      float cb = 0.f, predb = 0.f, origb = 0.f;
+      if (corr < lo) {
+        if (orig <= pred) origb = ob; else predb = ob;
+      } else if (corr > hi) {
+        if (orig >= pred) origb = ob; else predb = ob;
+      } else {
+        cb = ob;
+      }
+      predb += cb;
+      origb += 0.5f * cb;
+      const float recb = -0.5f * cb;
+      const TriC cp = tri_coords(g, sp);
+      stage_add(tile[0], o, dsb, g, int(v.x), int(v.y), int(v.z), origb);
+      stage_scatter(tile[0], o, dsb, g, cp, predb);
+      // sp = x - u_a(x) dt; sr = sp + u_a(sp) dt
+      F3 spb = f3(0.f, 0.f, 0.f);
+      if (predb != 0.f) spb = tri_grad(g, sp, tp, d) * predb;
+      if (recb != 0.f) {
+        stage_scatter(tile[0], o, dsb, g, tri_coords(g, sr), recb);
+        const F3 srb = tri_grad(g, sr, tr, d) * recb;
+        const F3 fb = srb * dt;
+        stage_scatter(tile[1], o, abar.x, g, cp, fb.x);
+        stage_scatter(tile[2], o, abar.y, g, cp, fb.y);
+        stage_scatter(tile[3], o, abar.z, g, cp, fb.z);
+        spb = spb + srb + tri_grad(g, sp, tp, ua.x) * fb.x + tri_grad(g, sp, tp, ua.y) * fb.y + tri_grad(g, sp, tp, ua.z) * fb.z;
+      }
+      if (spb.x != 0.f) stage_add(tile[1], o, abar.x, g, int(v.x), int(v.y), int(v.z), -dt * spb.x);
+      if (spb.y != 0.f) stage_add(tile[2], o, abar.y, g, int(v.x), int(v.y), int(v.z), -dt * spb.y);
+      if (spb.z != 0.f) stage_add(tile[3], o, abar.z, g, int(v.x), int(v.y), int(v.z), -dt * spb.z);
     }
-    predb += cb;
-    origb += 0.5f * cb;
-    const float recb = -0.5f * cb;
-    atomicAdd(&dsb[v.i], origb);
-    tri_scatter(tp, dsb, predb);
-    tri_scatter(tr, dsb, recb);
-    // sp = x - u_a(x) dt; sr = sp + u_a(sp) dt
-    const F3 srb = recb != 0.f ? tri_grad(g, sr, tr, d) * recb : f3(0.f, 0.f, 0.f);
-    F3 spb = srb;
-    if (predb != 0.f) spb = spb + tri_grad(g, sp, tp, d) * predb;
-    if (recb != 0.f) {
-      const F3 fb = srb * dt;
-      tri_scatter(tp, abar.x, fb.x);
-      tri_scatter(tp, abar.y, fb.y);
-      tri_scatter(tp, abar.z, fb.z);
-      spb = spb + tri_grad(g, sp, tp, ua.x) * fb.x + tri_grad(g, sp, tp, ua.y) * fb.y + tri_grad(g, sp, tp, ua.z) * fb.z;
-    }
-    if (spb.x != 0.f) atomicAdd(&abar.x[v.i], -dt * spb.x);
-    if (spb.y != 0.f) atomicAdd(&abar.y[v.i], -dt * spb.y);
-    if (spb.z != 0.f) atomicAdd(&abar.z[v.i], -dt * spb.z);
+    __syncthreads();
+    for (int f = 0; f < 4; ++f) stage_store(tile[f], stage + f * stage_field, fx_k);
+    __syncthreads();
   }
 }
 // Velocity advection adjoint: from abar (adjoint of u_a) scatter-adds into usb
 // (adjoint of u_s, cleared by the caller).
 __global__ __launch_bounds__(512) void k_da_advect_u(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                     Vec3 u, Vec3 abar, Vec3 usb, float dt) {
+                                                     Vec3 u, Vec3 abar, Vec3 usb, float dt, float* __restrict__ stage,
+                                                     size_t stage_field) {
+  __shared__ float tile[3][kScatN];
+  for (int t = int(threadIdx.x); t < 3 * kScatN; t += int(blockDim.x)) (&tile[0][0])[t] = 0.f;
+  __syncthreads();
   FX_BRICK_LOOP(list, count) {
-    Vox v = vox_of(g, list[fx_k]);
-    if (!v.in || !interior(g, v.x, v.y, v.z)) continue;
-    const F3 a = f3(abar.x[v.i], abar.y[v.i], abar.z[v.i]);
-    if (a.x == 0.f && a.y == 0.f && a.z == 0.f) continue;
-    const F3 sp = f3(float(v.x) - u.x[v.i] * dt, float(v.y) - u.y[v.i] * dt, float(v.z) - u.z[v.i] * dt);
-    const Tri t = tri(g, sp);
-    tri_scatter(t, usb.x, a.x);
-    tri_scatter(t, usb.y, a.y);
-    tri_scatter(t, usb.z, a.z);
-    const F3 spb = tri_grad(g, sp, t, u.x) * a.x + tri_grad(g, sp, t, u.y) * a.y + tri_grad(g, sp, t, u.z) * a.z;
-    if (spb.x != 0.f) atomicAdd(&usb.x[v.i], -dt * spb.x);
-    if (spb.y != 0.f) atomicAdd(&usb.y[v.i], -dt * spb.y);
-    if (spb.z != 0.f) atomicAdd(&usb.z[v.i], -dt * spb.z);
+    const uint32_t b = list[fx_k];
+    const Vox v = vox_of(g, b);
+    const TileOrigin o = tile_origin(g, b);
+    const bool live = v.in && interior(g, v.x, v.y, v.z);
+    const F3 a = live ? f3(abar.x[v.i], abar.y[v.i], abar.z[v.i]) : f3(0.f, 0.f, 0.f);
+    if (a.x != 0.f || a.y != 0.f || a.z != 0.f) {
+      const F3 sp = f3(float(v.x) - u.x[v.i] * dt, float(v.y) - u.y[v.i] * dt, float(v.z) - u.z[v.i] * dt);
+      const Tri t = tri(g, sp);
+      const TriC c = tri_coords(g, sp);
+      stage_scatter(tile[0], o, usb.x, g, c, a.x);
+      stage_scatter(tile[1], o, usb.y, g, c, a.y);
+      stage_scatter(tile[2], o, usb.z, g, c, a.z);
+      const F3 spb = tri_grad(g, sp, t, u.x) * a.x + tri_grad(g, sp, t, u.y) * a.y + tri_grad(g, sp, t, u.z) * a.z;
+      if (spb.x != 0.f) stage_add(tile[0], o, usb.x, g, int(v.x), int(v.y), int(v.z), -dt * spb.x);
+      if (spb.y != 0.f) stage_add(tile[1], o, usb.y, g, int(v.x), int(v.y), int(v.z), -dt * spb.y);
+      if (spb.z != 0.f) stage_add(tile[2], o, usb.z, g, int(v.x), int(v.y), int(v.z), -dt * spb.z);
+    }
+    __syncthreads();
+    for (int f = 0; f < 3; ++f) stage_store(tile[f], stage + f * stage_field, fx_k);
+    __syncthreads();
   }
 }
 // Source adjoint: cbar[ch] += dt * taps^T (u_s bar, d_s bar) over the slab.
@@ -2475,11 +2574,17 @@ void launch_da_forces(hipStream_t s, int blocks, const uint32_t* list, const uin
   FX_LAUNCH(k_da_confine3, blocks, 512, s, list, count, g, wbar, abar);
   FX_LAUNCH(k_da_buoyancy, blocks, 512, s, list, count, g, abar.y, dnb, beta_dt);
 }
-void launch_da_advect(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const float* ds, Vec3 us, Vec3 ua,
-                      const float* dnb, float* dsb, Vec3 abar, Vec3 usb, float dt) {
-  FX_LAUNCH(k_da_advect_d, blocks, 512, s, list, count, g, ds, ua, dnb, dsb, abar, dt);
-  FX_LAUNCH(k_da_advect_u, blocks, 512, s, list, count, g, us, abar, usb, dt);
+void launch_da_advect(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, size_t nb, const float* ds,
+                      Vec3 us, Vec3 ua, const float* dnb, float* dsb, Vec3 abar, Vec3 usb, float dt, float* stage) {
+  const size_t sf = nb * size_t(kScatN);
+  FX_LAUNCH(k_da_advect_d, blocks, 512, s, list, count, g, ds, ua, dnb, dsb, abar, dt, stage, sf);
+  GatherArgs A{{stage, stage + sf, stage + 2 * sf, stage + 3 * sf}, {dsb, abar.x, abar.y, abar.z}, 4};
+  FX_LAUNCH(k_da_gather, blocks, 512, s, list, count, g, nullptr, A);
+  FX_LAUNCH(k_da_advect_u, blocks, 512, s, list, count, g, us, abar, usb, dt, stage, sf);
+  GatherArgs B{{stage, stage + sf, stage + 2 * sf, nullptr}, {usb.x, usb.y, usb.z, nullptr}, 3};
+  FX_LAUNCH(k_da_gather, blocks, 512, s, list, count, g, nullptr, B);
 }
+size_t da_stage_floats(size_t nb) { return 4 * nb * size_t(kScatN); }
 void launch_da_source(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, uint32_t cn, uint32_t slab,
                       float dt, const float* dsb, Vec3 usb, float* cbar) {
   FX_LAUNCH(k_da_source, blocks, 512, s, list, count, g, cn, slab, dt, dsb, usb, cbar);

@@ -16,8 +16,8 @@ import numpy as np
 from . import abi
 
 
-def diff_desc(resolution=(64, 64, 64), steps=64, checkpoint_every=8, control_n=32, slab=4, dt=1.0, buoyancy=0.05,
-              confinement=0.3, vcycle=None, device=0, lib=None) -> abi.DiffDesc:
+def diff_desc(resolution=(64, 64, 64), steps=64, checkpoint_every=8, control_n=32, slab=4, dt=1.0, buoyancy=0.02,
+              confinement=0.02, vcycle=None, device=0, lib=None) -> abi.DiffDesc:
     lib = lib or abi.load()
     d = abi.DiffDesc()
     lib.fx_diff_default_desc(C.byref(d))
@@ -139,7 +139,7 @@ def gaussian_blobs(shape, centres, sigma, amplitude=1.0):
     return out.astype(np.float32)
 
 
-def plume_control(control_n, rise=0.5, density=0.5, radius=0.25):
+def plume_control(control_n, rise=0.05, density=0.1, radius=0.3):
     """A starting control: a disc of upward velocity and density at the slab centre."""
     n = control_n
     c = (np.arange(n) + 0.5) / n - 0.5

@@ -438,8 +438,11 @@ typedef struct fx_diff_desc {
   int32_t device;
 } fx_diff_desc;
 
-/* 64^3, 64 steps, checkpoint 8, control 32, slab 4, dt 1, buoyancy 0.05,
- * confinement 0.3, V-cycle 4 levels 2/2 x 4, 16 coarse sweeps, omega 2/3. */
+/* 64^3, 64 steps, checkpoint 8, control 32, slab 4, dt 1, buoyancy 0.02,
+ * confinement 0.02, V-cycle 4 levels 2/2 x 4, 16 coarse sweeps, omega 2/3.
+ * Confinement amplifies small perturbations: with strong confinement over long
+ * rollouts (0.3 over 64 steps at 128^3) the loss is chaotic and its gradient,
+ * exact for the discrete model, no longer predicts finite changes. */
 void fx_diff_default_desc(fx_diff_desc* desc);
 int fx_diff_create(const fx_diff_desc* desc, fx_diff** out);
 void fx_diff_destroy(fx_diff* diff);
