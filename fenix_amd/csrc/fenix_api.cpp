@@ -117,6 +117,13 @@ struct fx_sim {
   hipStream_t rstream = nullptr;
   hipEvent_t ev_scalars = nullptr, ev_render = nullptr;
   void* d_render = nullptr;      // float4 per voxel
+  // Gather mirrors of the scalar advection (float4 per voxel): the scalars of
+  // set 0 and the advected velocity. Inside a step the mirror of the scalars is
+  // packed over the listed bricks and zeroed in the retired ones; outside a step
+  // (fx_run_pass) both mirrors are packed over every brick.
+  void* d_sq = nullptr;
+  void* d_uq = nullptr;
+  bool in_step = false;          // recording the step's pass sequence
   uint32_t* d_rqueue = nullptr;  // raymarch tile counter
   int ray_blocks = 1;            // resident raymarch workgroups
   bool render_overlap = true;    // false: the raymarch runs on the simulation stream
@@ -514,12 +521,16 @@ void run_stage(fx_sim* s, int pass) {
   const int blocks = grid_for(s, s->nb);
   switch (pass) {
     case FX_PASS_ADVECT_VELOCITY:
-      launch_advect_velocity(s->stream, blocks, list, count, s->g, s->d_const, vec_of(s->u), vec_of(s->us));
+      launch_advect_velocity(s->stream, blocks, list, count, s->g, s->d_const, vec_of(s->u), vec_of(s->us), s->d_uq);
       break;
-    case FX_PASS_ADVECT_SCALARS:
-      launch_advect_scalars(s->stream, blocks, list, count, s->g, s->d_const, s->mode, set_of(s, 0), vec_of(s->us), set_of(s, 1),
-                            s->flagsInj);
+    case FX_PASS_ADVECT_SCALARS: {
+      const bool all = !s->in_step || !s->sparse;
+      launch_pack_gather(s->stream, grid_for(s, s->nb), all ? s->identity : list, all ? s->identity_count : count, set_of(s, 0),
+                         s->in_step ? Vec3{nullptr, nullptr, nullptr} : vec_of(s->us), all ? nullptr : s->lv[0].L.retire,
+                         all ? nullptr : s->lv[0].L.retire_count, s->d_sq, s->d_uq);
+      launch_advect_scalars(s->stream, blocks, list, count, s->g, s->d_const, s->mode, s->d_sq, s->d_uq, set_of(s, 1), s->flagsInj);
       break;
+    }
     case FX_PASS_COMBUSTION:
       launch_combustion(s->stream, blocks, list, count, s->g, s->d_const, set_of(s, 1), set_of(s, 0), s->flagsRender);
       break;
@@ -668,12 +679,14 @@ int read_brickmajor(fx_sim* s, const Grid& g, const float* dev, float* host, siz
 // final, and (b) the rest, which reads them but writes only velocity, vorticity,
 // pressure and diagnostics.
 void record_step_a(fx_sim* s) {
+  s->in_step = true;
   pass_inject(s, 7);                                   // ORCH:270
   pass_activate_render(s);                             // ORCH:271-272
   run_stage(s, FX_PASS_ADVECT_VELOCITY);               // ORCH:273
   run_stage(s, FX_PASS_ADVECT_SCALARS);                // ORCH:274-279
   if (!s->sparse) pass_activate_advected(s);           // ORCH:280-281
   run_stage(s, FX_PASS_COMBUSTION);                    // ORCH:282
+  s->in_step = false;
 }
 void record_step_b(fx_sim* s, bool vort, bool pres, bool dbg) {
   run_stage(s, FX_PASS_BUOYANCY);                      // ORCH:283
@@ -904,11 +917,13 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
   s->norm_cap = std::max(1, desc->vcycle.cycles);
   if (dalloc(s, &s->d_norms, size_t(s->norm_cap) * kNormLanes) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
   if (dalloc(s, &s->d_rqueue, 1) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
-  {
-    float* rb = nullptr;
-    if (dalloc(s, &rb, storage_count(s->g) * 4) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
-    s->d_render = rb;
+  for (void** mirror : {&s->d_render, &s->d_sq, &s->d_uq}) {
+    float* q = nullptr;
+    if (dalloc(s, &q, storage_count(s->g) * 4) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
+    *mirror = q;
   }
+  launch_clear_f32(s->stream, static_cast<float*>(s->d_sq), storage_count(s->g) * 4);
+  launch_clear_f32(s->stream, static_cast<float*>(s->d_uq), storage_count(s->g) * 4);
   s->src_cap = FX_FLOATS_PER_EMITTER;
   if (dalloc(s, &s->d_src, s->src_cap) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
   for (auto& slot : s->slots) {
@@ -956,7 +971,7 @@ void fx_destroy(fx_sim* s) {
   F(s->wmag); F(s->cmag); F(s->umag);
   F(s->flagsInj);
   if (!s->sparse) F(s->flagsRender);
-  F(s->touched); F(s->d_bits); F(s->d_rqueue); F(s->d_render); F(s->identity); F(s->identity_count); F(s->d_const); F(s->d_cand); F(s->d_src); F(s->d_norms);
+  F(s->touched); F(s->d_bits); F(s->d_rqueue); F(s->d_render); F(s->d_sq); F(s->d_uq); F(s->identity); F(s->identity_count); F(s->d_const); F(s->d_cand); F(s->d_src); F(s->d_norms);
   for (size_t l = 0; l < s->lv.size(); ++l) {
     Level& L = s->lv[l];
     F(L.div); F(L.p[0]); F(L.p[1]); F(L.bits); F(L.tab); F(L.tab_n); F(L.L.list); F(L.L.count); F(L.L.page); F(L.L.retire); F(L.L.retire_count);
@@ -990,6 +1005,8 @@ int fx_reset(fx_sim* s) {  // ORCH:206-226
   s->pending.clear();
   const size_t nsv = storage_count(s->g);
   for (const Buf& b : fine_buffers(s)) launch_clear_u32(s->stream, static_cast<uint32_t*>(b.p), nsv * size_t(b.esz) / 4, 0u);
+  launch_clear_f32(s->stream, static_cast<float*>(s->d_sq), nsv * 4);
+  launch_clear_f32(s->stream, static_cast<float*>(s->d_uq), nsv * 4);
   launch_clear_u32(s->stream, s->flagsInj, s->nb, 0u);
   if (!s->sparse) launch_clear_u32(s->stream, s->flagsRender, s->nb, 0u);
   launch_clear_u32(s->stream, s->touched, s->nb, 0u);
@@ -1453,16 +1470,25 @@ int fx_autotune_vcycle(fx_sim* s, uint32_t repeats, uint32_t* down_split, uint32
   };
   int rc = FX_OK;
   const int L = int(s->lv.size());
+  {
+    // bring the clocks up before the first measurement
+    float ms = 0.f;
+    for (int w = 0; w < 3 && rc == FX_OK; ++w) rc = time_plan(&ms);
+  }
   for (int l = 0; l + 1 < L && rc == FX_OK; ++l) {
-    float best = 0.f;
+    // each plan's best of two interleaved rounds
+    float t[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int round = 0; round < 2 && rc == FX_OK; ++round)
+      for (uint32_t c = 0; c < 4 && rc == FX_OK; ++c) {
+        s->split_down = (s->split_down & ~(1u << l)) | ((c & 1u) << l);
+        s->split_up = (s->split_up & ~(1u << l)) | (((c >> 1) & 1u) << l);
+        float ms = 0.f;
+        rc = time_plan(&ms);
+        if (rc == FX_OK && (round == 0 || ms < t[c])) t[c] = ms;
+      }
     uint32_t best_c = 0;
-    for (uint32_t c = 0; c < 4 && rc == FX_OK; ++c) {
-      s->split_down = (s->split_down & ~(1u << l)) | ((c & 1u) << l);
-      s->split_up = (s->split_up & ~(1u << l)) | (((c >> 1) & 1u) << l);
-      float ms = 0.f;
-      rc = time_plan(&ms);
-      if (rc == FX_OK && (c == 0 || ms < best)) { best = ms; best_c = c; }
-    }
+    for (uint32_t c = 1; c < 4; ++c)
+      if (t[c] < t[best_c]) best_c = c;
     s->split_down = (s->split_down & ~(1u << l)) | ((best_c & 1u) << l);
     s->split_up = (s->split_up & ~(1u << l)) | (((best_c >> 1) & 1u) << l);
   }

@@ -154,10 +154,15 @@ void launch_coarse_setup(hipStream_t s, const uint32_t* list, const uint32_t* co
 void launch_smooth_block(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const StepConst* c,
                          const float* rhs, float* p0, float* p1, int sweeps, int weighted, int zero_start, const void* tab,
                          const uint32_t* tab_n);
+// uq: float4 mirror of us (written with it)
 void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
-                            const StepConst* c, Vec3 u, Vec3 us);
+                            const StepConst* c, Vec3 u, Vec3 us, void* uq);
+// Gather mirrors for the scalar advection: sq = float4 (d, T, f, r) of set0 over the
+// list, uq = float4 of us when us.x is non-null; both zeroed in the retired bricks.
+void launch_pack_gather(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Scalars set0, Vec3 us,
+                        const uint32_t* rlist, const uint32_t* rcount, void* sq, void* uq);
 void launch_advect_scalars(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
-                           const StepConst* c, int mode, Scalars src, Vec3 us, Scalars dst, const uint32_t* flags);
+                           const StepConst* c, int mode, const void* sq, const void* uq, Scalars dst, const uint32_t* flags);
 void launch_combustion(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
                        const StepConst* c, Scalars src, Scalars dst, const uint32_t* flags);
 void launch_buoyancy(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,

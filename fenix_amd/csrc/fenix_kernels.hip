@@ -592,17 +592,25 @@ __global__ __launch_bounds__(512) void k_zero_bricks(const uint32_t* __restrict_
 // ----------------------------------------------------------- advection --
 // SH/advect-velocity-semilagrangian.wgsl.ts:23-41
 __global__ __launch_bounds__(512) void k_advect_velocity(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                         const StepConst* __restrict__ C, Vec3 u, Vec3 us) {
+                                                         const StepConst* __restrict__ C, Vec3 u, Vec3 us, float4* __restrict__ uq) {
   const float dt = C->p.delta_time;
   FX_BRICK_LOOP(list, count) {
     Vox v = vox_of(g, list[fx_k]);
     if (!v.in) continue;
-    if (on_boundary(g, v.x, v.y, v.z)) { us.x[v.i] = 0.f; us.y[v.i] = 0.f; us.z[v.i] = 0.f; continue; }
+    if (on_boundary(g, v.x, v.y, v.z)) {
+      us.x[v.i] = 0.f; us.y[v.i] = 0.f; us.z[v.i] = 0.f;
+      uq[v.i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      continue;
+    }
     F3 coord = f3(float(v.x) + 0.5f, float(v.y) + 0.5f, float(v.z) + 0.5f);
     F3 half = f3(0.5f, 0.5f, 0.5f);
-    F3 back = coord - sample_u(g, u, coord - half) * dt;
+    // sampleVelocity(coord - 0.5) lands on the voxel with all fractions 0; the
+    // trilinear blend a*(1-0) + b*0 returns a exactly for finite fields, so the
+    // voxel's own value is read instead of 8 corners per component.
+    F3 back = coord - f3(u.x[v.i], u.y[v.i], u.z[v.i]) * dt;
     F3 adv = sample_u(g, u, back - half) * (1.f - dt * 0.025f);
     us.x[v.i] = adv.x; us.y[v.i] = adv.y; us.z[v.i] = adv.z;
+    uq[v.i] = make_float4(adv.x, adv.y, adv.z, 0.f);
   }
 }
 
@@ -629,13 +637,48 @@ __device__ __forceinline__ void decay_write(const Grid& g, uint32_t i, uint32_t 
   ST::st(D.f, i, clamp01(fmaxf(0.f, fu - dt * mixf(0.012f, 0.034f, tn))));
   D.r[i] = clamp01(fmaxf(0.f, rea - dt * mixf(0.46f, 0.82f, tn)));
 }
+// Gather mirrors for the scalar advection: the four scalars (d, T, f, r) and the
+// advected velocity as one float4 per voxel, so that each trilinear corner is one
+// 16-byte load instead of four (or three) 4-byte ones; the advection is bound by
+// the number of gather instructions, not by bytes. tri_eval4 blends each
+// component in tri_eval's order, so results are unchanged.
+__device__ __forceinline__ float4 tri_eval4(const Tri& t, const float4* __restrict__ f) {
+  const float4 c0 = f[t.i[0]], c1 = f[t.i[1]], c2 = f[t.i[2]], c3 = f[t.i[3]];
+  const float4 c4 = f[t.i[4]], c5 = f[t.i[5]], c6 = f[t.i[6]], c7 = f[t.i[7]];
+  auto blend = [&](float a0, float a1, float a2, float a3, float a4, float a5, float a6, float a7) {
+    const float x00 = mixf(a0, a1, t.fx), x10 = mixf(a2, a3, t.fx), x01 = mixf(a4, a5, t.fx), x11 = mixf(a6, a7, t.fx);
+    return mixf(mixf(x00, x10, t.fy), mixf(x01, x11, t.fy), t.fz);
+  };
+  return make_float4(blend(c0.x, c1.x, c2.x, c3.x, c4.x, c5.x, c6.x, c7.x), blend(c0.y, c1.y, c2.y, c3.y, c4.y, c5.y, c6.y, c7.y),
+                     blend(c0.z, c1.z, c2.z, c3.z, c4.z, c5.z, c6.z, c7.z), blend(c0.w, c1.w, c2.w, c3.w, c4.w, c5.w, c6.w, c7.w));
+}
+// Packs the scalar mirror of the listed bricks (and the velocity mirror from us
+// when us.x is set), and zeroes both mirrors in the retired bricks, so that every
+// unlisted brick reads 0 there as it does in the fields (sparse zero invariant).
+template <class ST>
+__global__ __launch_bounds__(512) void k_pack_gather(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, SS<ST> S,
+                                                     Vec3 us, const uint32_t* __restrict__ rlist, const uint32_t* __restrict__ rcount,
+                                                     float4* __restrict__ sq, float4* __restrict__ uq) {
+  FX_BRICK_LOOP(list, count) {
+    const uint32_t i = (list[fx_k] << 9) | threadIdx.x;
+    sq[i] = make_float4(ST::ld(S.d, i), ST::ld(S.t, i), ST::ld(S.f, i), S.r[i]);
+    if (us.x) uq[i] = make_float4(us.x[i], us.y[i], us.z[i], 0.f);
+  }
+  if (!rlist) return;
+  const uint32_t nr = *rcount;
+  for (uint32_t k = blockIdx.x; k < nr; k += gridDim.x) {
+    const uint32_t i = (rlist[k] << 9) | threadIdx.x;
+    sq[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    uq[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
 // SH/advect-scalar-maccormack.wgsl.ts:33-101 / SH/advect-scalar-semilagrangian.wgsl.ts:26-44.
 // The four advect<Name> calls share every position; sampleVelocity(coord-0.5)
 // and the `original` sample hit the voxel exactly (fraction 0).
 template <class ST>
 __global__ __launch_bounds__(512) void k_advect_scalars(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                        const StepConst* __restrict__ C, int mode, SS<ST> S, Vec3 us, SS<ST> D,
-                                                        const uint32_t* __restrict__ flags) {
+                                                        const StepConst* __restrict__ C, int mode, const float4* __restrict__ sq,
+                                                        const float4* __restrict__ uq, SS<ST> D, const uint32_t* __restrict__ flags) {
   const float dt = C->p.delta_time, time = C->p.time;
   FX_BRICK_LOOP(list, count) {
     Vox v = vox_of(g, list[fx_k]);
@@ -644,26 +687,30 @@ __global__ __launch_bounds__(512) void k_advect_scalars(const uint32_t* __restri
     if (!flag_on(g, flags, v.x, v.y, v.z)) { ST::st(D.d, i, 0.f); ST::st(D.t, i, 0.f); ST::st(D.f, i, 0.f); D.r[i] = 0.f; continue; }
     F3 coord = f3(float(v.x) + 0.5f, float(v.y) + 0.5f, float(v.z) + 0.5f);
     F3 half = f3(0.5f, 0.5f, 0.5f);
-    F3 velocity = f3(us.x[i], us.y[i], us.z[i]);
+    const float4 uv = uq[i];
+    F3 velocity = f3(uv.x, uv.y, uv.z);
     F3 back = coord - velocity * dt;
     F3 sp = back - half;
     Tri tp = tri(g, sp);
     if (mode == FX_ADVECT_SEMI_LAGRANGIAN) {
-      decay_write<ST>(g, i, v.y, tri_evalP<ST>(tp, S.d), tri_evalP<ST>(tp, S.t), tri_evalP<ST>(tp, S.f), tri_eval(tp, S.r), thermal_noise(sp, time), dt, D);
+      const float4 s4 = tri_eval4(tp, sq);
+      decay_write<ST>(g, i, v.y, s4.x, s4.y, s4.z, s4.w, thermal_noise(sp, time), dt, D);
       continue;
     }
-    F3 fwd = sample_u(g, us, sp);
+    const float4 f4 = tri_eval4(tp, uq);
+    F3 fwd = f3(f4.x, f4.y, f4.z);
     F3 rec = back + fwd * dt;
     Tri tr = tri(g, rec - half);
     auto mac = [&](float predicted, float recovered, float original) {
       float corrected = predicted + 0.5f * (original - recovered);
       return clampf(corrected, fminf(original, predicted), fmaxf(original, predicted));
     };
+    const float4 pr = tri_eval4(tp, sq), rc = tri_eval4(tr, sq), og = sq[i];
     float out[4];
-    out[0] = mac(tri_evalP<ST>(tp, S.d), tri_evalP<ST>(tr, S.d), ST::ld(S.d, i));
-    out[1] = mac(tri_evalP<ST>(tp, S.t), tri_evalP<ST>(tr, S.t), ST::ld(S.t, i));
-    out[2] = mac(tri_evalP<ST>(tp, S.f), tri_evalP<ST>(tr, S.f), ST::ld(S.f, i));
-    out[3] = mac(tri_eval(tp, S.r), tri_eval(tr, S.r), S.r[i]);
+    out[0] = mac(pr.x, rc.x, og.x);
+    out[1] = mac(pr.y, rc.y, og.y);
+    out[2] = mac(pr.z, rc.z, og.z);
+    out[3] = mac(pr.w, rc.w, og.w);
     decay_write<ST>(g, i, v.y, out[0], out[1], out[2], out[3], thermal_noise(coord, time), dt, D);
   }
 }
@@ -2394,15 +2441,27 @@ void launch_compact(hipStream_t s, const CompactArgs& A) {
   FX_LAUNCH(k_compact_write, blocks, 256, s, A);
 }
 void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
-                            Vec3 u, Vec3 us) {
-  FX_LAUNCH(k_advect_velocity, blocks, 512, s, list, count, g, c, u, us);
+                            Vec3 u, Vec3 us, void* uq) {
+  FX_LAUNCH(k_advect_velocity, blocks, 512, s, list, count, g, c, u, us, static_cast<float4*>(uq));
+}
+void launch_pack_gather(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Scalars set0, Vec3 us,
+                        const uint32_t* rlist, const uint32_t* rcount, void* sq, void* uq) {
+  float4* S4 = static_cast<float4*>(sq);
+  float4* U4 = static_cast<float4*>(uq);
+  switch (set0.prec) {
+    case FX_PRECISION_BF16: FX_LAUNCH(k_pack_gather<StBF16>, blocks, 512, s, list, count, typed<StBF16>(set0), us, rlist, rcount, S4, U4); break;
+    case FX_PRECISION_FP16: FX_LAUNCH(k_pack_gather<StF16>, blocks, 512, s, list, count, typed<StF16>(set0), us, rlist, rcount, S4, U4); break;
+    default: FX_LAUNCH(k_pack_gather<StF32>, blocks, 512, s, list, count, typed<StF32>(set0), us, rlist, rcount, S4, U4);
+  }
 }
 void launch_advect_scalars(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
-                           int mode, Scalars src, Vec3 us, Scalars dst, const uint32_t* flags) {
-  switch (src.prec) {
-    case FX_PRECISION_BF16: FX_LAUNCH(k_advect_scalars<StBF16>, blocks, 512, s, list, count, g, c, mode, typed<StBF16>(src), us, typed<StBF16>(dst), flags); break;
-    case FX_PRECISION_FP16: FX_LAUNCH(k_advect_scalars<StF16>, blocks, 512, s, list, count, g, c, mode, typed<StF16>(src), us, typed<StF16>(dst), flags); break;
-    default: FX_LAUNCH(k_advect_scalars<StF32>, blocks, 512, s, list, count, g, c, mode, typed<StF32>(src), us, typed<StF32>(dst), flags);
+                           int mode, const void* sq, const void* uq, Scalars dst, const uint32_t* flags) {
+  const float4* S4 = static_cast<const float4*>(sq);
+  const float4* U4 = static_cast<const float4*>(uq);
+  switch (dst.prec) {
+    case FX_PRECISION_BF16: FX_LAUNCH(k_advect_scalars<StBF16>, blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StBF16>(dst), flags); break;
+    case FX_PRECISION_FP16: FX_LAUNCH(k_advect_scalars<StF16>, blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StF16>(dst), flags); break;
+    default: FX_LAUNCH(k_advect_scalars<StF32>, blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StF32>(dst), flags);
   }
 }
 void launch_combustion(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
