@@ -950,8 +950,9 @@ __global__ __launch_bounds__(1024) void k_coarse_setup(const uint32_t* __restric
   __shared__ uint32_t wtot[16];
   const uint32_t n = *count, nbl = g.BX * g.BY * g.BZ;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const bool lead = blockIdx.x == 0 && tid == 0;
   if (n == 0 || n > kSmoothBricks || nbl > kSmoothTable) {
-    if (tid == 0) *tab_n = n == 0 ? 0u : kSmoothOverflow;
+    if (lead) *tab_n = n == 0 ? 0u : kSmoothOverflow;
     return;
   }
   for (uint32_t q = tid; q < nbl; q += 1024) slot[q] = -1;
@@ -975,11 +976,12 @@ __global__ __launch_bounds__(1024) void k_coarse_setup(const uint32_t* __restric
   for (uint32_t w = 0; w < 16; ++w) nact += wtot[w];
   if (tid < n) base[tid] = pre + incl - cnt;
   if (nact > kSmoothCap) {
-    if (tid == 0) *tab_n = kSmoothOverflow;
+    if (lead) *tab_n = kSmoothOverflow;
     return;
   }
   __syncthreads();
-  for (uint32_t q = tid; q < nact; q += 1024) {
+  const uint32_t first = blockIdx.x * 1024u + tid, stride = gridDim.x * 1024u;
+  for (uint32_t q = first; q < nact; q += stride) {
     uint32_t lo = 0, hi = n - 1;
     while (lo < hi) {
       const uint32_t mid = (lo + hi + 1) >> 1;
@@ -1008,14 +1010,14 @@ __global__ __launch_bounds__(1024) void k_coarse_setup(const uint32_t* __restric
     tab[q] = make_uint4((b << 9) | t, nb[0] | (nb[1] << 16), nb[2] | (nb[3] << 16), nb[4] | (nb[5] << 16));
   }
   // the inactive in-domain voxels of the listed bricks hold 0 in both buffers
-  for (uint32_t q = tid; q < n * 512u; q += 1024) {
+  for (uint32_t q = first; q < n * 512u; q += stride) {
     const uint32_t k = q >> 9, t = q & 511u;
     if ((amask[k * 8 + (t >> 6)] >> (t & 63u)) & 1ull) continue;
     const uint32_t b = list[k];
     const uint32_t x = (b % g.BX) * 8 + (t & 7u), y = ((b / g.BX) % g.BY) * 8 + ((t >> 3) & 7u), z = (b / (g.BX * g.BY)) * 8 + (t >> 6);
     if (x < g.W && y < g.H && z < g.D) { p0[(b << 9) | t] = 0.f; p1[(b << 9) | t] = 0.f; }
   }
-  if (tid == 0) *tab_n = nact;
+  if (lead) *tab_n = nact;
 }
 
 __global__ __launch_bounds__(1024) void k_smooth_block(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
@@ -2510,7 +2512,9 @@ void launch_jacobi(hipStream_t s, int blocks, const uint32_t* list, const uint32
 void launch_level_masks(hipStream_t s, int blocks, const LevelMaskArgs& a) { FX_LAUNCH(k_level_masks, blocks, 512, s, a); }
 void launch_coarse_setup(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, void* tab, uint32_t* tab_n,
                          float* p0, float* p1) {
-  FX_LAUNCH(k_coarse_setup, 1, 1024, s, list, count, g, b, static_cast<uint4*>(tab), tab_n, p0, p1);
+  // every workgroup rebuilds the brick scan (at most kSmoothBricks bricks) and
+  // writes its share of the slots and of the zeroing
+  FX_LAUNCH(k_coarse_setup, 16, 1024, s, list, count, g, b, static_cast<uint4*>(tab), tab_n, p0, p1);
 }
 void launch_smooth_block(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const StepConst* c,
                          const float* rhs, float* p0, float* p1, int sweeps, int weighted, int zero_start, const void* tab,
