@@ -47,6 +47,8 @@ STAGE_BYTES = {
 # Whole-step algorithmic bytes per active voxel (config V-cycle, FP32): simulation
 # 602.2 + raymarch 16 B/voxel (+ 8 B per pixel, added per workload).
 STEP_BYTES_SIM = 602.2
+# The same for the reference's parity pressure schedule (plain C1): 351.6 B (318.6 with 16-bit d/T/f).
+STEP_BYTES_SIM_REF = {0: 351.6, 2: 318.6}
 RAY_BYTES_PER_VOXEL = 16.0
 # Accesses of the density/temperature/fuel planes per active voxel, per stage: with
 # 16-bit storage each one moves 2 bytes fewer than the FP32 figure above.
@@ -59,7 +61,7 @@ RAY_SCALAR_ACCESSES = 3
 # benchmark: FETCH_SIZE x 2.0 (a 256 MiB device copy reads 128 MiB on that counter)
 # + WRITE_SIZE x 1.0 (the copy's writes read exactly). Summaries with the commands
 # are in profiles/.
-TRAFFIC_PER_VOXEL = {"jacobi_fine": 13.44, "advect_scalars": 65.0}
+TRAFFIC_PER_VOXEL = {"jacobi_fine": 13.44, "advect_scalars": 62.5}
 PRECISIONS = {"fp32": (abi.FX_PRECISION_FP32, 4), "bf16": (abi.FX_PRECISION_BF16, 2), "fp16": (abi.FX_PRECISION_FP16, 2)}
 
 
@@ -357,7 +359,10 @@ def main():
         args.autotune = False
     full_step(0)
     sim.synchronize()
-    plan = sim.autotune_vcycle(4) if args.autotune else sim.vcycle_plan()
+    vcycle = wl["desc"].solver == abi.FX_SOLVER_VCYCLE
+    plan = None
+    if vcycle:
+        plan = sim.autotune_vcycle(4) if args.autotune else sim.vcycle_plan()
     for k in range(1, args.warmup):
         full_step(k)
     sim.synchronize()
@@ -396,7 +401,8 @@ def main():
     else:
         avg_ms, vox_launch = stage_ms[dom], active_t
     ray_bytes = RAY_BYTES_PER_VOXEL - saved * RAY_SCALAR_ACCESSES
-    step_bytes_voxel = (STEP_BYTES_SIM if saved == 0 else STEP_BYTES_SIM_16) + ray_bytes
+    sim_bytes = (STEP_BYTES_SIM if saved == 0 else STEP_BYTES_SIM_16) if vcycle else STEP_BYTES_SIM_REF[saved]
+    step_bytes_voxel = sim_bytes + ray_bytes
     bytes_launch = (STAGE_BYTES[dom] - saved * SCALAR_ACCESSES.get(dom, 0)) * vox_launch
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     traffic = TRAFFIC_PER_VOXEL[dom] * vox_launch if dom in TRAFFIC_PER_VOXEL and saved == 0 else None
@@ -417,7 +423,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": wl["label"], "storage": args.precision, "active_bricks": active,
-                   "vcycle_plan": {"down_split": plan[0], "up_split": plan[1], "autotuned": args.autotune},
+                   "vcycle_plan": ({"down_split": plan[0], "up_split": plan[1], "autotuned": args.autotune} if plan
+                                   else "reference pressure schedule"),
                    "active_fraction": active / sim.brick_count, "render": f"{w}x{h}x{sc}"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
