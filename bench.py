@@ -396,7 +396,7 @@ def main():
     if not totals:
         raise SystemExit("--timing-steps must be >= 1 to find the dominant kernel")
     dom = max(totals, key=totals.get)
-    if dom == "jacobi_fine":
+    if dom == "jacobi_fine" and vcycle:
         avg_ms, vox_launch = sim.time_fine_sweep(100)
     else:
         avg_ms, vox_launch = stage_ms[dom], active_t
