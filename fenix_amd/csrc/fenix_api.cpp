@@ -819,10 +819,12 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
   if (hipGetDeviceProperties(&prop, s->device) != hipSuccess) return bail(fail(s, FX_ERR_DEVICE, "hipGetDeviceProperties failed"));
   s->blocks = std::max(1, prop.multiProcessorCount * 4);
   {
-    // resident raymarch workgroups per CU (256 threads each); FENIX_RAY_BLOCKS_PER_CU overrides
+    // resident raymarch workgroups (256 threads each): one per CU;
+    // FENIX_RAY_BLOCKS_PER_CU or the absolute FENIX_RAY_BLOCKS override
     const char* r = std::getenv("FENIX_RAY_BLOCKS_PER_CU");
     const int per_cu = r ? std::max(1, std::atoi(r)) : 1;
     s->ray_blocks = std::max(1, prop.multiProcessorCount * per_cu);
+    if (const char* a = std::getenv("FENIX_RAY_BLOCKS")) s->ray_blocks = std::max(1, std::atoi(a));
     const char* o = std::getenv("FENIX_RENDER_OVERLAP");
     s->render_overlap = !(o && o[0] == '0');
   }
