@@ -26,7 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from fenix_amd import abi  # noqa: E402
-from fenix_amd.render import VolumeRaymarchPass, orbit_camera_snapshot  # noqa: E402
+from fenix_amd.render import BloomPass, VolumeRaymarchPass, orbit_camera_snapshot  # noqa: E402
 from fenix_amd.simulation import CombustionVolumeBatch, CombustionVolumeSimulation  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
@@ -318,6 +318,8 @@ def main():
                     help="batched scenes: K 256^3 plume scenes (c2) or the C3 batch (c3, default 64 scenes)")
     ap.add_argument("--hw-queues", type=int, default=16, choices=range(1, 33), metavar="Q",
                     help="hardware queues per process for batched runs (GPU_MAX_HW_QUEUES, at most 32)")
+    ap.add_argument("--bloom", action="store_true",
+                    help="a step also runs the bloom/composite passes into an RGBA8 canvas image (the reference's frame)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of this many MiB (a known-size reference for the "
                          "HBM traffic counters)")
@@ -345,11 +347,15 @@ def main():
     rp.setRenderParams(stepCount=sc)
     cam = orbit_camera_snapshot()
     img = HipBuffer(w * h * 8)
+    canvas = HipBuffer(w * h * 4) if args.bloom else None
+    bloom = BloomPass() if args.bloom else None
     dt = 1.0 / 60.0
 
     def full_step(k):
         sim.step(None, k * dt, dt)
         rp.render_device(sim, cam, img.ptr.value, "temperature", w, h, k * dt)
+        if bloom:
+            bloom.render_device(sim, img.ptr.value, canvas.ptr.value, w, h)
 
     # The V-cycle kernel plan is tuned on the first step's active set (results do
     # not depend on the plan), so every warm-up and timed step runs the final plan.
@@ -425,7 +431,8 @@ def main():
         "config": {"workload": wl["label"], "storage": args.precision, "active_bricks": active,
                    "vcycle_plan": ({"down_split": plan[0], "up_split": plan[1], "autotuned": args.autotune} if plan
                                    else "reference pressure schedule"),
-                   "active_fraction": active / sim.brick_count, "render": f"{w}x{h}x{sc}"},
+                   "active_fraction": active / sim.brick_count,
+                   "render": f"{w}x{h}x{sc}" + (" + bloom/composite" if args.bloom else "")},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": bytes_launch, "avg_launch_ms": avg_ms, "voxels_per_launch": vox_launch},
@@ -445,6 +452,8 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, sim, (base + args.timing_steps) * dt, args.cpu_steps)
     img.free()
+    if canvas:
+        canvas.free()
     sim.dispose()
     print(json.dumps(out), flush=True)
 

@@ -167,6 +167,10 @@ class RenderParams(C.Structure):
     ]
 
 
+class BloomParams(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("threshold", C.c_float), ("strength", C.c_float), ("radius", C.c_float)]
+
+
 P = C.c_void_p
 i32, u32 = C.c_int32, C.c_uint32
 FUNCTIONS = {
@@ -195,6 +199,8 @@ FUNCTIONS = {
     "fx_active_brick_count": (C.c_int, [P, C.POINTER(u32)]),
     "fx_raymarch": (C.c_int, [P, C.POINTER(Camera), C.POINTER(RenderParams), u32, u32, C.POINTER(C.c_uint16)]),
     "fx_raymarch_device": (C.c_int, [P, C.POINTER(Camera), C.POINTER(RenderParams), u32, u32, C.c_void_p]),
+    "fx_bloom_device": (C.c_int, [P, C.POINTER(BloomParams), u32, u32, C.c_void_p, C.c_void_p]),
+    "fx_bloom": (C.c_int, [P, C.POINTER(BloomParams), u32, u32, C.POINTER(C.c_uint16), C.POINTER(C.c_uint8)]),
     "fx_set_stage_timing": (C.c_int, [P, i32]),
     "fx_read_stage_times": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(u32), C.c_size_t]),
     "fx_set_vcycle_plan": (C.c_int, [P, u32, u32]),

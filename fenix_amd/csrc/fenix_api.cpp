@@ -123,6 +123,12 @@ struct fx_sim {
   // (fx_run_pass) both mirrors are packed over every brick.
   void* d_sq = nullptr;
   void* d_uq = nullptr;
+  // Bloom: three half-resolution rgba16f images (bright, horizontal and vertical
+  // blur) for the last size, and staging for the host-memory variant.
+  void* d_bloom = nullptr;
+  uint32_t bloom_w = 0, bloom_h = 0;
+  void* d_bloom_io = nullptr;
+  size_t bloom_io_bytes = 0;
   bool in_step = false;          // recording the step's pass sequence
   uint32_t* d_rqueue = nullptr;  // raymarch tile counter
   int ray_blocks = 1;            // resident raymarch workgroups
@@ -973,7 +979,7 @@ void fx_destroy(fx_sim* s) {
   F(s->wmag); F(s->cmag); F(s->umag);
   F(s->flagsInj);
   if (!s->sparse) F(s->flagsRender);
-  F(s->touched); F(s->d_bits); F(s->d_rqueue); F(s->d_render); F(s->d_sq); F(s->d_uq); F(s->identity); F(s->identity_count); F(s->d_const); F(s->d_cand); F(s->d_src); F(s->d_norms);
+  F(s->touched); F(s->d_bits); F(s->d_rqueue); F(s->d_render); F(s->d_sq); F(s->d_uq); F(s->d_bloom); F(s->d_bloom_io); F(s->identity); F(s->identity_count); F(s->d_const); F(s->d_cand); F(s->d_src); F(s->d_norms);
   for (size_t l = 0; l < s->lv.size(); ++l) {
     Level& L = s->lv[l];
     F(L.div); F(L.p[0]); F(L.p[1]); F(L.bits); F(L.tab); F(L.tab_n); F(L.L.list); F(L.L.count); F(L.L.page); F(L.L.retire); F(L.L.retire_count);
@@ -1358,6 +1364,52 @@ int fx_raymarch_device(fx_sim* s, const fx_camera* c, const fx_render_params* p,
     s->render_pending = true;
   }
   HIP_OK(s, hipGetLastError());
+  return FX_OK;
+}
+
+int fx_bloom_device(fx_sim* s, const fx_bloom_params* p, uint32_t w, uint32_t h, const void* src, void* dst) {
+  if (!s || !p || !src || !dst) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  if (w == 0 || h == 0 || uint64_t(w) * h > (1ull << 28)) return fail(s, FX_ERR_INVALID_ARG, "image size out of range");
+  HIP_OK(s, hipSetDevice(s->device));
+  hipStream_t rs = s->render_overlap ? s->rstream : s->stream;
+  const uint32_t hw = std::max(1u, w / 2), hh = std::max(1u, h / 2);  // createBloomPass.ts:84-89
+  if (w != s->bloom_w || h != s->bloom_h) {
+    if (s->d_bloom) {
+      HIP_OK(s, hipStreamSynchronize(rs));
+      HIP_OK(s, hipFree(s->d_bloom));
+      s->d_bloom = nullptr;
+    }
+    HIP_OK(s, hipMalloc(&s->d_bloom, size_t(hw) * hh * 8 * 3));
+    s->bloom_w = w;
+    s->bloom_h = h;
+  }
+  BloomArgs a{p->enabled ? 1 : 0, p->threshold, p->strength, p->radius, w, h, hw, hh};
+  char* b = static_cast<char*>(s->d_bloom);
+  const size_t img = size_t(hw) * hh * 8;
+  launch_bloom(rs, a, src, b, b + img, b + 2 * img, dst);
+  HIP_OK(s, hipGetLastError());
+  return FX_OK;
+}
+
+int fx_bloom(fx_sim* s, const fx_bloom_params* p, uint32_t w, uint32_t h, const uint16_t* host_src, uint8_t* host_dst) {
+  if (!s || !p || !host_src || !host_dst) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  if (w == 0 || h == 0 || uint64_t(w) * h > (1ull << 28)) return fail(s, FX_ERR_INVALID_ARG, "image size out of range");
+  HIP_OK(s, hipSetDevice(s->device));
+  hipStream_t rs = s->render_overlap ? s->rstream : s->stream;
+  const size_t in = size_t(w) * h * 8, out = size_t(w) * h * 4;
+  if (in + out > s->bloom_io_bytes) {
+    HIP_OK(s, hipStreamSynchronize(rs));
+    if (s->d_bloom_io) HIP_OK(s, hipFree(s->d_bloom_io));
+    s->d_bloom_io = nullptr;
+    HIP_OK(s, hipMalloc(&s->d_bloom_io, in + out));
+    s->bloom_io_bytes = in + out;
+  }
+  char* io = static_cast<char*>(s->d_bloom_io);
+  HIP_OK(s, hipMemcpyAsync(io, host_src, in, hipMemcpyHostToDevice, rs));
+  int rc = fx_bloom_device(s, p, w, h, io, io + in);
+  if (rc != FX_OK) return rc;
+  HIP_OK(s, hipMemcpyAsync(host_dst, io + in, out, hipMemcpyDeviceToHost, rs));
+  HIP_OK(s, hipStreamSynchronize(rs));
   return FX_OK;
 }
 

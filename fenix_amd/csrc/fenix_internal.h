@@ -264,4 +264,14 @@ int sim_project(fx_sim* s, Vec3 us, Vec3 u);
 void bricks_from_rowmajor(const Grid& g, const float* rm, float* bm);
 void bricks_to_rowmajor(const Grid& g, const float* bm, float* rm);
 
+// Bloom and composite after the raymarch (createBloomPass.ts): src is the w x h
+// rgba16f image, bright/blur_h/blur_v hold hw x hh rgba16f each, out is w x h
+// RGBA8 (R in the low byte).
+struct BloomArgs {
+  int enabled;
+  float threshold, strength, radius;
+  uint32_t w, h, hw, hh;
+};
+void launch_bloom(hipStream_t s, const BloomArgs& a, const void* src, void* bright, void* blur_h, void* blur_v, void* out);
+
 }  // namespace fx

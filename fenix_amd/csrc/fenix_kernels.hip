@@ -1944,6 +1944,99 @@ __global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, const floa
   }
 }
 
+// --------------------------------------------------------------- bloom --
+// src/engine/render/passes/createBloomPass.ts: threshold into a half-resolution
+// image (:119), 9-tap separable blur (:121-125, blurShader :237-262), composite
+// with the source into the 8-bit canvas (:128, compositeShader :276-293), or the
+// plain blit when bloom is off (:114-116, blitShader :264-274). Each render pass
+// draws a full-screen triangle: the fragment of target pixel (i, j) reads its
+// sources at uv = ((i + 0.5) / W, (j + 0.5) / H), row 0 at the top, through a
+// linear, clamp-to-edge sampler. Intermediate targets are rgba16float.
+struct BloomConst {
+  float threshold, strength, radius;
+  uint32_t w, h, hw, hh;
+};
+__device__ __forceinline__ float4 ld_h4(const __half* img, uint32_t i) {
+  const __half* p = img + size_t(i) * 4;
+  return make_float4(__half2float(p[0]), __half2float(p[1]), __half2float(p[2]), __half2float(p[3]));
+}
+__device__ __forceinline__ void st_h4(__half* img, uint32_t i, float4 v) {
+  __half* p = img + size_t(i) * 4;
+  p[0] = __float2half(v.x); p[1] = __float2half(v.y); p[2] = __float2half(v.z); p[3] = __float2half(v.w);
+}
+// Linear, clamp-to-edge sample of a w x h rgba16f image at uv.
+__device__ __forceinline__ float4 sample_lin(const __half* img, uint32_t w, uint32_t h, float u, float v) {
+  const float tx = u * float(w) - 0.5f, ty = v * float(h) - 0.5f;
+  const float fx0 = floorf(tx), fy0 = floorf(ty);
+  const float ax = tx - fx0, ay = ty - fy0;
+  const int x0 = int(fx0), y0 = int(fy0);
+  const int xa = min(max(x0, 0), int(w) - 1), xb = min(max(x0 + 1, 0), int(w) - 1);
+  const int ya = min(max(y0, 0), int(h) - 1), yb = min(max(y0 + 1, 0), int(h) - 1);
+  const float4 c00 = ld_h4(img, uint32_t(ya) * w + uint32_t(xa)), c10 = ld_h4(img, uint32_t(ya) * w + uint32_t(xb));
+  const float4 c01 = ld_h4(img, uint32_t(yb) * w + uint32_t(xa)), c11 = ld_h4(img, uint32_t(yb) * w + uint32_t(xb));
+  auto lerp2 = [&](float a, float b, float c, float d) { return mixf(mixf(a, b, ax), mixf(c, d, ax), ay); };
+  return make_float4(lerp2(c00.x, c10.x, c01.x, c11.x), lerp2(c00.y, c10.y, c01.y, c11.y), lerp2(c00.z, c10.z, c01.z, c11.z),
+                     lerp2(c00.w, c10.w, c01.w, c11.w));
+}
+// WebGPU float -> unorm8: clamp to [0, 1], round to nearest.
+__device__ __forceinline__ uint32_t unorm8(float v) { return uint32_t(rintf(clampf(v, 0.f, 1.f) * 255.f)); }
+__device__ __forceinline__ uint32_t pack_rgba8(float r, float g, float b, float a) {
+  return unorm8(r) | (unorm8(g) << 8) | (unorm8(b) << 16) | (unorm8(a) << 24);
+}
+__global__ __launch_bounds__(256) void k_bloom_threshold(BloomConst P, const __half* __restrict__ src, __half* __restrict__ bright) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= P.hw * P.hh) return;
+  const uint32_t x = i % P.hw, y = i / P.hw;
+  const float4 c = sample_lin(src, P.w, P.h, (float(x) + 0.5f) / float(P.hw), (float(y) + 0.5f) / float(P.hh));
+  const float lum = c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f;
+  const float knee = P.threshold * 0.5f;
+  const float rq = clampf(lum - P.threshold + knee, 0.f, 2.f * knee);
+  const float wgt = (rq * rq) / (4.f * knee * knee + 0.00001f);
+  const float mask = fmaxf(wgt, stepf(P.threshold, lum));
+  st_h4(bright, i, make_float4(c.x * mask * c.w, c.y * mask * c.w, c.z * mask * c.w, c.w * mask));
+}
+__global__ __launch_bounds__(256) void k_bloom_blur(BloomConst P, int horizontal, const __half* __restrict__ src, __half* __restrict__ dst) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= P.hw * P.hh) return;
+  const uint32_t x = i % P.hw, y = i / P.hw;
+  const float u = (float(x) + 0.5f) / float(P.hw), v = (float(y) + 0.5f) / float(P.hh);
+  const float scale = 1.f + P.radius * 2.f;
+  const float su = horizontal ? (1.f / float(P.hw)) * scale : 0.f, sv = horizontal ? 0.f : (1.f / float(P.hh)) * scale;
+  const float wt[5] = {0.2270270270f, 0.1945945946f, 0.1216216216f, 0.0540540541f, 0.0162162162f};
+  float4 col = sample_lin(src, P.hw, P.hh, u, v);
+  col = make_float4(col.x * wt[0], col.y * wt[0], col.z * wt[0], col.w * wt[0]);
+#pragma unroll
+  for (int k = 1; k <= 4; ++k) {
+    const float4 a = sample_lin(src, P.hw, P.hh, u + su * float(k), v + sv * float(k));
+    col = make_float4(col.x + a.x * wt[k], col.y + a.y * wt[k], col.z + a.z * wt[k], col.w + a.w * wt[k]);
+    const float4 b = sample_lin(src, P.hw, P.hh, u - su * float(k), v - sv * float(k));
+    col = make_float4(col.x + b.x * wt[k], col.y + b.y * wt[k], col.z + b.z * wt[k], col.w + b.w * wt[k]);
+  }
+  st_h4(dst, i, col);
+}
+// glow == nullptr: the blit (bloom disabled).
+__global__ __launch_bounds__(256) void k_bloom_composite(BloomConst P, const __half* __restrict__ src, const __half* __restrict__ glow,
+                                                         uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= P.w * P.h) return;
+  const uint32_t x = i % P.w, y = i / P.w;
+  const float u = (float(x) + 0.5f) / float(P.w), v = (float(y) + 0.5f) / float(P.h);
+  const float4 base = sample_lin(src, P.w, P.h, u, v);
+  float3 hdr;
+  float alpha;
+  if (glow) {
+    const float4 g = sample_lin(glow, P.hw, P.hh, u, v);
+    hdr = make_float3(fmaxf(base.x + fmaxf(g.x, 0.f) * P.strength, 0.f), fmaxf(base.y + fmaxf(g.y, 0.f) * P.strength, 0.f),
+                      fmaxf(base.z + fmaxf(g.z, 0.f) * P.strength, 0.f));
+    const float ga = clampf(g.w * P.strength, 0.f, 1.f);
+    alpha = clampf(base.w + ga * (1.f - base.w), 0.f, 1.f);
+  } else {
+    hdr = make_float3(fmaxf(base.x, 0.f), fmaxf(base.y, 0.f), fmaxf(base.z, 0.f));
+    alpha = clampf(base.w, 0.f, 1.f);
+  }
+  out[i] = pack_rgba8(pow_hw(hdr.x, 0.92f), pow_hw(hdr.y, 0.92f), pow_hw(hdr.z, 0.92f), alpha);
+}
+
 // ------------------------------------------------ differentiable smoke --
 // A differentiable smoke step for inflow-control optimisation (no counterpart in
 // the reference, which has no gradients). State: density d and velocity u on a
@@ -2651,6 +2744,20 @@ size_t da_stage_floats(size_t nb) { return 4 * nb * size_t(kScatN); }
 void launch_da_source(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, uint32_t cn, uint32_t slab,
                       float dt, const float* dsb, Vec3 usb, float* cbar) {
   FX_LAUNCH(k_da_source, blocks, 512, s, list, count, g, cn, slab, dt, dsb, usb, cbar);
+}
+
+// ----------------------------------------------------------- bloom launchers --
+void launch_bloom(hipStream_t s, const BloomArgs& a, const void* src, void* bright, void* blur_h, void* blur_v, void* out) {
+  BloomConst P{a.threshold, a.strength, a.radius, a.w, a.h, a.hw, a.hh};
+  const __half* S = static_cast<const __half*>(src);
+  const uint32_t nh = a.hw * a.hh, nf = a.w * a.h;
+  if (a.enabled) {
+    FX_LAUNCH(k_bloom_threshold, (nh + 255) / 256, 256, s, P, S, static_cast<__half*>(bright));
+    FX_LAUNCH(k_bloom_blur, (nh + 255) / 256, 256, s, P, 1, static_cast<const __half*>(bright), static_cast<__half*>(blur_h));
+    FX_LAUNCH(k_bloom_blur, (nh + 255) / 256, 256, s, P, 0, static_cast<const __half*>(blur_h), static_cast<__half*>(blur_v));
+  }
+  FX_LAUNCH(k_bloom_composite, (nf + 255) / 256, 256, s, P, S, a.enabled ? static_cast<const __half*>(blur_v) : nullptr,
+            static_cast<uint32_t*>(out));
 }
 
 }  // namespace fx

@@ -164,3 +164,34 @@ class VolumeRaymarchPass:
         p = self.params_struct(displayMode, elapsedSeconds)
         sim._check(sim._lib.fx_raymarch_device(sim.handle, C.byref(cam), C.byref(p), renderWidth, renderHeight,
                                                C.c_void_p(device_ptr)))
+
+
+class BloomPass:
+    """BloomPass (src/engine/render/passes/createBloomPass.ts:26-146) over libfenix:
+    the raymarch's RGBA float16 image -> the canvas image (RGBA8 unorm)."""
+
+    def __init__(self, lib=None):
+        self._lib = lib or abi.load()
+
+    @staticmethod
+    def params_struct(enabled=True, threshold=0.6, strength=0.9, radius=0.4) -> abi.BloomParams:
+        p = abi.BloomParams()
+        p.enabled, p.threshold, p.strength, p.radius = int(bool(enabled)), threshold, strength, radius
+        return p
+
+    def render(self, sim, src_rgba_f16: np.ndarray, params: Optional[abi.BloomParams] = None) -> np.ndarray:
+        """Host image in, uint8[h, w, 4] out (synchronous)."""
+        src = np.ascontiguousarray(np.asarray(src_rgba_f16, dtype=np.float16))
+        h, w = src.shape[:2]
+        out = np.zeros((h, w, 4), dtype=np.uint8)
+        p = params or self.params_struct()
+        rc = self._lib.fx_bloom(sim._h, C.byref(p), w, h, src.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                out.ctypes.data_as(C.POINTER(C.c_uint8)))
+        abi.check(self._lib, sim._h, rc)
+        return out
+
+    def render_device(self, sim, src_ptr, dst_ptr, width, height, params: Optional[abi.BloomParams] = None) -> None:
+        """Device images (asynchronous, on the render stream after the raymarch)."""
+        p = params or self.params_struct()
+        rc = self._lib.fx_bloom_device(sim._h, C.byref(p), int(width), int(height), C.c_void_p(src_ptr), C.c_void_p(dst_ptr))
+        abi.check(self._lib, sim._h, rc)

@@ -285,6 +285,12 @@ class Scene:
     render_output: Dict
     edges: List[Dict] = field(default_factory=list)
 
+    def bloom(self):
+        """The file's bloom settings (RenderOutputNodeProps) as (enabled, threshold, strength, radius)."""
+        ro = self.render_output
+        return (bool(ro.get("bloomEnabled", True)), float(ro.get("bloomThreshold", 0.6)),
+                float(ro.get("bloomStrength", 0.9)), float(ro.get("bloomRadius", 0.4)))
+
     def apply(self, sim, raymarch=None) -> None:
         """applySerializedGraph for the simulation (and the raymarch pass, when given):
         runtime params, quality, advection mode, sources; then reset."""

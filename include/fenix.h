@@ -358,6 +358,27 @@ int fx_raymarch(fx_sim* sim, const fx_camera* camera, const fx_render_params* pa
 int fx_raymarch_device(fx_sim* sim, const fx_camera* camera, const fx_render_params* params,
                        uint32_t width, uint32_t height, void* device_rgba_f16);
 
+/* ---- bloom and composite ------------------------------------------------ */
+
+/* BloomParams, src/engine/render/passes/createBloomPass.ts:1-6. */
+typedef struct fx_bloom_params {
+  int32_t enabled;
+  float threshold, strength, radius;
+} fx_bloom_params;
+
+/* BloomPass.render(encoder, sourceView, targetView, w, h, params) — createBloomPass.ts:99-129:
+ * the raymarch's w x h RGBA float16 image (device) -> the w x h canvas image as RGBA8
+ * unorm (R in the lowest byte of each 32-bit pixel, row 0 = top). Enabled: soft
+ * luminance threshold into half resolution, 9-tap horizontal and vertical blur,
+ * composite with pow(hdr, 0.92); disabled: the blit with pow(hdr, 0.92). Enqueued
+ * on the render stream after the raymarch; the library keeps the half-resolution
+ * images (resized when w, h change). */
+int fx_bloom_device(fx_sim* sim, const fx_bloom_params* params, uint32_t width, uint32_t height,
+                    const void* device_src_rgba_f16, void* device_dst_rgba8);
+/* Same from and to host memory (synchronous). */
+int fx_bloom(fx_sim* sim, const fx_bloom_params* params, uint32_t width, uint32_t height,
+             const uint16_t* host_src_rgba_f16, uint8_t* host_dst_rgba8);
+
 /* ---- stage timing ------------------------------------------------------- */
 
 /* Timing ids: the FX_PASS_* ids (whole pass; FX_PASS_ACTIVATE_RENDER includes the

@@ -173,3 +173,18 @@ class OracleProjector:
             self.orc.run_pass(name, self.params)
         self.calls += 1
         return [self.orc.read(n) for n in ("velocity_x", "velocity_y", "velocity_z")]
+
+
+_BLOOM = None
+
+
+def bloom_model():
+    """The numpy restatement of the bloom and composite passes (oracle/bloom.py)."""
+    global _BLOOM
+    if _BLOOM is None:
+        import importlib.util
+        spec = importlib.util.spec_from_file_location("fenix_oracle_bloom", os.path.join(ROOT, "oracle", "bloom.py"))
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        _BLOOM = mod
+    return _BLOOM

@@ -42,6 +42,7 @@ STRUCTS = {
     "fx_light": (abi.Light, ["intensity"]),
     "fx_render_params": (abi.RenderParams, ["lights", "scattering_back", "elapsed_s"]),
     "fx_diff_desc": (abi.DiffDesc, ["slab", "dt", "confinement", "vcycle", "device"]),
+    "fx_bloom_params": (abi.BloomParams, ["threshold", "radius"]),
 }
 
 
