@@ -172,7 +172,14 @@ __device__ __forceinline__ BrickRange brick_range(uint32_t n) {
   const uint32_t lo = min(x * seg, n), hi = min(lo + seg, n);
   return BrickRange{lo + (blockIdx.x >> 3), hi, per};
 }
+// Simulation kernels share the CUs with the raymarch, which runs overlapped on
+// its own stream. s_setprio raises the wave's instruction-issue priority (0..3,
+// default 0): where both kinds of waves are resident the simulation's issue
+// first. The raymarch keeps priority 0 and finishes within the step anyway
+// (measured on the 256^3 plume: 1.135 -> 1.074 ms per step).
+#define FX_SIM_PRIORITY() __builtin_amdgcn_s_setprio(2)
 #define FX_BRICK_LOOP(list, count)                                       \
+  FX_SIM_PRIORITY();                                                     \
   const BrickRange fx_r = brick_range(*(count));                         \
   for (uint32_t fx_k = fx_r.k; fx_k < fx_r.end; fx_k += fx_r.step)
 
@@ -358,12 +365,15 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
 
 // ----------------------------------------------------------- clearing --
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t value) {
+  FX_SIM_PRIORITY();
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = value;
 }
 __global__ void k_mark(const uint32_t* __restrict__ list, uint32_t n, uint32_t* touched) {
+  FX_SIM_PRIORITY();
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) touched[list[k]] = 1u;
 }
 __global__ void k_fill_f32(float* p, size_t n) {
+  FX_SIM_PRIORITY();
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = 0.f;
 }
 
@@ -393,6 +403,7 @@ template <class ST>
 __global__ __launch_bounds__(256) void k_activate_reference(const uint32_t* __restrict__ listA, const uint32_t* __restrict__ countA,
                                                             const uint32_t* __restrict__ listB, const uint32_t* __restrict__ countB,
                                                             Grid g, SS<ST> S, uint32_t* flags, LevelFlags lf) {
+  FX_SIM_PRIORITY();
   const uint32_t na = *countA, nb = listB ? *countB : 0u;
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na + nb; k += gridDim.x * blockDim.x) {
     const uint32_t b = k < na ? listA[k] : listB[k - na];
@@ -424,6 +435,7 @@ __global__ __launch_bounds__(512) void k_activate_threshold(const uint32_t* __re
                                                             const uint32_t* __restrict__ listB, const uint32_t* __restrict__ countB,
                                                             Grid g, const StepConst* __restrict__ C, const typename ST::T* __restrict__ den,
                                                             Vec3 vel, uint32_t* flags, LevelFlags lf) {
+  FX_SIM_PRIORITY();
   // One wave per brick (8 bricks per block): lane (x, y) of the brick reads its
   // column of 8 voxels, ballots reduce; flags are only ever set to 1, so the
   // 27 neighbour marks run on 27 lanes at once.
@@ -497,6 +509,7 @@ __device__ __forceinline__ uint32_t scan256(uint32_t v, uint32_t* excl, uint32_t
   return total;
 }
 __global__ __launch_bounds__(256) void k_compact_count(CompactArgs A) {
+  FX_SIM_PRIORITY();
   __shared__ uint32_t lds[8];
   int l;
   uint32_t base;
@@ -518,6 +531,7 @@ __global__ __launch_bounds__(256) void k_compact_count(CompactArgs A) {
   if (threadIdx.x == 0) { A.counts[2 * blockIdx.x] = ta; A.counts[2 * blockIdx.x + 1] = tr; }
 }
 __global__ __launch_bounds__(1024) void k_compact_scan(CompactArgs A) {
+  FX_SIM_PRIORITY();
   if (threadIdx.x != 0) return;
   for (int l = 0; l < A.nlevels; ++l) {
     uint32_t a = 0, r = 0;
@@ -531,6 +545,7 @@ __global__ __launch_bounds__(1024) void k_compact_scan(CompactArgs A) {
   }
 }
 __global__ __launch_bounds__(256) void k_compact_write(CompactArgs A) {
+  FX_SIM_PRIORITY();
   __shared__ uint32_t lds[8];
   int l;
   uint32_t base;
@@ -568,6 +583,7 @@ __global__ __launch_bounds__(256) void k_compact_write(CompactArgs A) {
 // Zero the retiring bricks of every level in one launch: level l's buffers are
 // Z.b[Z.first[l] .. Z.first[l+1]) and its list is retire[l] / count[l].
 __global__ __launch_bounds__(512) void k_retire(RetireArgs R) {
+  FX_SIM_PRIORITY();
   for (int l = 0; l < R.nlevels; ++l) {
     const uint32_t n = *R.count[l];
     for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
@@ -944,6 +960,7 @@ __device__ __forceinline__ uint32_t rank_in_brick(const uint64_t* am, uint32_t k
 __global__ __launch_bounds__(1024) void k_coarse_setup(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
                                                        LvBits lb, uint4* __restrict__ tab, uint32_t* __restrict__ tab_n, float* p0,
                                                        float* p1) {
+  FX_SIM_PRIORITY();
   __shared__ uint64_t amask[kSmoothBricks * 8];
   __shared__ uint32_t base[kSmoothBricks];
   __shared__ int16_t slot[kSmoothTable];
@@ -1024,6 +1041,7 @@ __global__ __launch_bounds__(1024) void k_smooth_block(const uint32_t* __restric
                                                        LvMask m, const StepConst* __restrict__ C, const float* __restrict__ rhs,
                                                        float* p0, float* p1, int sweeps, int weighted, int zero_start,
                                                        const uint4* __restrict__ tab, const uint32_t* __restrict__ tab_n) {
+  FX_SIM_PRIORITY();
   __shared__ float sp[2][kSmoothCap + 1];  // slot kSmoothCap holds 0: the missing neighbour
   const float om = C->omega;
   const uint32_t tid = threadIdx.x;
@@ -1163,6 +1181,7 @@ __global__ __launch_bounds__(512) void k_prolong(const uint32_t* __restrict__ li
 // the voxel is in the domain, off the boundary faces and active under the
 // level's mask. Built once per solve; every V-cycle kernel reads these bits.
 __global__ __launch_bounds__(512) void k_level_masks(LevelMaskArgs a) {
+  FX_SIM_PRIORITY();
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   uint32_t start[FX_MAX_LEVELS_DEV + 1];
   start[0] = 0;
@@ -1601,6 +1620,7 @@ template <bool ZERO, bool FINE>
 __global__ __launch_bounds__(512) void k_sweep(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
                                                LvBits lb, const StepConst* __restrict__ C, const float* __restrict__ rhs,
                                                const float* __restrict__ src, float* __restrict__ dst) {
+  FX_SIM_PRIORITY();
   const float om = C->omega;
   const uint32_t t = threadIdx.x, lx = t & 7u, ly = (t >> 3) & 7u, lz = t >> 6;
   const uint32_t BX = g.BX, BXY = g.BX * g.BY, last = g.BX * g.BY * g.BZ - 1u;
@@ -1708,6 +1728,7 @@ __device__ __forceinline__ float phase_hg(float c, float gg) {
 
 // Brick flags packed 32 per word (the raymarch keeps them in LDS).
 __global__ void k_pack_flags(const uint32_t* __restrict__ flags, uint32_t nb, uint32_t* __restrict__ bits) {
+  FX_SIM_PRIORITY();
   uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w * 32 >= nb) return;
   uint32_t v = 0;
