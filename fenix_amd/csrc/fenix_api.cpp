@@ -54,6 +54,8 @@ struct fx_sim {
   Grid g{};
   size_t nb = 0, nv = 0;
   int blocks = 1;          // persistent grid for brick kernels
+  int cus = 1;             // compute units of the device
+  bool blocks_env = false; // FENIX_SIM_BLOCKS_PER_CU set
   bool sparse = false;
 
   void* S[2][4] = {};               // d, t, f in the storage precision; r FP32
@@ -443,7 +445,7 @@ int solve_vcycle(fx_sim* s) {
                         LC.p[1]);
   // n sweeps at level l; zero_start: the first sweep starts from p = 0 (a
   // coarse level's correction), so neither a clear nor a read of p is needed.
-  auto smoothW = [&](int l, int n, bool zero_start) {
+  auto smoothW = [&](int l, int n, bool zero_start, bool pair) {
     Level& V = s->lv[l];
     if (n == 0) {
       if (zero_start) { cur[l] = 0; zero_level_p(s, l, 0); }
@@ -457,8 +459,15 @@ int solve_vcycle(fx_sim* s) {
       return;
     }
     for (int k = 0; k < n; ++k) {
-      StageTimer timer(s, l == 0 ? FX_TIME_JACOBI_FINE : -1);
       const float* src = (zero_start && k == 0) ? nullptr : V.p[cur[l]];
+      if (pair && k + 1 < n) {
+        launch_sweep_pair(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, bits_of(s, l), s->d_const, V.div,
+                          src, V.p[1 - cur[l]]);
+        cur[l] = 1 - cur[l];
+        ++k;
+        continue;
+      }
+      StageTimer timer(s, l == 0 ? FX_TIME_JACOBI_FINE : -1);
       launch_sweep(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, bits_of(s, l), s->d_const, V.div, src,
                    V.p[1 - cur[l]], l == 0);
       cur[l] = 1 - cur[l];
@@ -483,14 +492,14 @@ int solve_vcycle(fx_sim* s) {
         continue;
       }
       // split leg: lean sweeps, residual into the other pressure buffer, restriction
-      smoothW(l, v.pre_sweeps, zero);
+      smoothW(l, v.pre_sweeps, zero, (s->split_down >> (kPlanPairShift + l)) & 1u);
       if (zero && v.pre_sweeps == 0) zero_level_p(s, l, cur[l]);
       launch_residual_lean(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, bits_of(s, l), V.div,
                            V.p[cur[l]], V.p[1 - cur[l]], norm);
       launch_restrict_lean(s->stream, grid_for(s, W.nb), level_list(s, l + 1), level_count(s, l + 1), V.g, W.g, bits_of(s, l + 1),
                            V.p[1 - cur[l]], W.div);
     }
-    smoothW(L - 1, v.coarse_sweeps, true);
+    smoothW(L - 1, v.coarse_sweeps, true, false);
     for (int l = L - 2; l >= 0; --l) {
       Level& V = s->lv[l];
       const uint32_t* lst = level_list(s, l);
@@ -499,7 +508,7 @@ int solve_vcycle(fx_sim* s) {
       if ((s->split_up >> l) & 1u) {
         // split leg: in-place prolongation, then lean sweeps
         launch_prolong_add(s->stream, grid_for(s, V.nb), lst, cnt, s->lv[l + 1].g, V.g, bits_of(s, l), e, V.p[cur[l]]);
-        smoothW(l, v.post_sweeps, false);
+        smoothW(l, v.post_sweeps, false, (s->split_up >> (kPlanPairShift + l)) & 1u);
         continue;
       }
       if (launch_vup(s->stream, grid_for(s, V.nb), v.post_sweeps, lst, cnt, s->lv[l + 1].g, V.g, bits_of(s, l), s->d_const, e,
@@ -508,7 +517,7 @@ int solve_vcycle(fx_sim* s) {
         continue;
       }
       launch_prolong(s->stream, grid_for(s, V.nb), lst, cnt, s->lv[l + 1].g, V.g, mask_of(s, l), e, V.p[cur[l]]);
-      smoothW(l, v.post_sweeps, false);
+      smoothW(l, v.post_sweeps, false, false);
     }
   }
   return cur[0];
@@ -823,7 +832,17 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
   if (hipSetDevice(s->device) != hipSuccess) return bail(fail(s, FX_ERR_DEVICE, "hipSetDevice failed"));
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, s->device) != hipSuccess) return bail(fail(s, FX_ERR_DEVICE, "hipGetDeviceProperties failed"));
-  s->blocks = std::max(1, prop.multiProcessorCount * 4);
+  {
+    // persistent brick-kernel workgroups (512 threads each): 3 per CU, which a
+    // CU holds beside a raymarch workgroup (128 VGPRs) for kernels of at most 64
+    // VGPRs; 4 would leave a quarter of the grid waiting while the overlapped
+    // raymarch runs (plume step 1.045 -> 1.018 ms). FENIX_SIM_BLOCKS_PER_CU
+    // overrides; the grid stays a multiple of the 8 XCDs through the CU count.
+    const char* b = std::getenv("FENIX_SIM_BLOCKS_PER_CU");
+    s->cus = std::max(1, prop.multiProcessorCount);
+    s->blocks_env = b != nullptr;
+    s->blocks = s->cus * (b ? std::max(1, std::atoi(b)) : 3);
+  }
   {
     // resident raymarch workgroups (256 threads each): one per CU;
     // FENIX_RAY_BLOCKS_PER_CU or the absolute FENIX_RAY_BLOCKS override
@@ -1529,22 +1548,31 @@ int fx_autotune_vcycle(fx_sim* s, uint32_t repeats, uint32_t* down_split, uint32
     float ms = 0.f;
     for (int w = 0; w < 3 && rc == FX_OK; ++w) rc = time_plan(&ms);
   }
+  // Per leg: 0 fused LDS kernel, 1 split with one sweep per launch, 2 split with
+  // two sweeps per launch (only where the leg has two or more sweeps).
+  auto set_leg = [&](uint32_t& mask, int l, uint32_t opt) {
+    mask &= ~((1u << l) | (1u << (kPlanPairShift + l)));
+    if (opt >= 1) mask |= 1u << l;
+    if (opt == 2) mask |= 1u << (kPlanPairShift + l);
+  };
+  const fx_vcycle& vc = s->desc.vcycle;
+  const uint32_t nd = vc.pre_sweeps >= 2 ? 3u : 2u, nu = vc.post_sweeps >= 2 ? 3u : 2u;
   for (int l = 0; l + 1 < L && rc == FX_OK; ++l) {
     // each plan's best of two interleaved rounds
-    float t[4] = {0.f, 0.f, 0.f, 0.f};
+    float t[9] = {};
     for (int round = 0; round < 2 && rc == FX_OK; ++round)
-      for (uint32_t c = 0; c < 4 && rc == FX_OK; ++c) {
-        s->split_down = (s->split_down & ~(1u << l)) | ((c & 1u) << l);
-        s->split_up = (s->split_up & ~(1u << l)) | (((c >> 1) & 1u) << l);
+      for (uint32_t c = 0; c < nd * nu && rc == FX_OK; ++c) {
+        set_leg(s->split_down, l, c % nd);
+        set_leg(s->split_up, l, c / nd);
         float ms = 0.f;
         rc = time_plan(&ms);
         if (rc == FX_OK && (round == 0 || ms < t[c])) t[c] = ms;
       }
     uint32_t best_c = 0;
-    for (uint32_t c = 1; c < 4; ++c)
+    for (uint32_t c = 1; c < nd * nu; ++c)
       if (t[c] < t[best_c]) best_c = c;
-    s->split_down = (s->split_down & ~(1u << l)) | ((best_c & 1u) << l);
-    s->split_up = (s->split_up & ~(1u << l)) | (((best_c >> 1) & 1u) << l);
+    set_leg(s->split_down, l, best_c % nd);
+    set_leg(s->split_up, l, best_c / nd);
   }
   // leave the pressure buffers as the chosen plan computes them
   float ms = 0.f;
@@ -1622,6 +1650,9 @@ namespace fx {
 
 SimView sim_view(fx_sim* s) {
   return SimView{s->stream, s->g, fine_list(s), fine_count(s), grid_for(s, s->nb), s->nb, storage_count(s->g)};
+}
+void sim_set_blocks_per_cu(fx_sim* s, int per_cu) {
+  if (!s->blocks_env) s->blocks = s->cus * std::max(1, per_cu);
 }
 int sim_prepare_projection(fx_sim* s) {
   fx_sim_params p{};

@@ -220,6 +220,7 @@ int fx_diff_create(const fx_diff_desc* desc, fx_diff** out) {
     delete f;
     return fail(nullptr, rc, "internal simulation: " + m);
   }
+  sim_set_blocks_per_cu(f->sim, 4);  // nothing renders beside the differentiable step
   f->v = sim_view(f->sim);
   f->n = f->v.storage;
   f->cfl = size_t(4) * D.control_n * D.control_n * D.control_n;

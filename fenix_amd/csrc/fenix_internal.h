@@ -145,6 +145,13 @@ void launch_prolong_add(hipStream_t s, int blocks, const uint32_t* list, const u
 // One weighted-Jacobi sweep of a V-cycle level; src == nullptr starts from p = 0.
 void launch_sweep(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
                   const float* rhs, const float* src, float* dst, bool fine);
+// Two sweeps in one launch (LDS temporal blocking), bit-identical to two
+// launch_sweep calls; src == nullptr starts from p = 0.
+void launch_sweep_pair(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
+                       const float* rhs, const float* src, float* dst);
+// Bits 8.. of a V-cycle plan mask: the split leg of level l runs its sweeps two
+// per launch (launch_sweep_pair).
+constexpr int kPlanPairShift = 8;
 // Coarsest-level slot table (kCoarseTableBytes at tab, count at tab_n), built
 // once per solve; zeroes the inactive voxels of the level's listed bricks in p0/p1.
 constexpr size_t kCoarseTableBytes = 8192 * 16;
@@ -259,6 +266,9 @@ struct SimView {
   size_t nb, storage;  // bricks, floats per brick-major field
 };
 SimView sim_view(fx_sim* s);
+// Persistent workgroups per CU of the brick kernels (a simulation that never
+// renders fills the CUs with 4); FENIX_SIM_BLOCKS_PER_CU takes precedence.
+void sim_set_blocks_per_cu(fx_sim* s, int per_cu);
 int sim_prepare_projection(fx_sim* s);  // uploads the step constants the V-cycle reads
 int sim_project(fx_sim* s, Vec3 us, Vec3 u);
 void bricks_from_rowmajor(const Grid& g, const float* rm, float* bm);

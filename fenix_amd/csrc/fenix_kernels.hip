@@ -785,7 +785,7 @@ __global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__
 // ------------------------------------------------------------ buoyancy --
 // SH/apply-buoyancy.wgsl.ts:27-110 (render set, velocityScratch in place).
 template <class ST>
-__global__ __launch_bounds__(512) void k_buoyancy(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_buoyancy(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
                                                   const StepConst* __restrict__ C, SS<ST> S, Vec3 us, const uint32_t* __restrict__ flags) {
   const fx_sim_params& P = C->p;
   FX_BRICK_LOOP(list, count) {
@@ -1223,18 +1223,19 @@ __global__ __launch_bounds__(512) void k_level_masks(LevelMaskArgs a) {
 
 // Activity words of the 3x3x3 bricks around brick (bx, by, bz): nb[c * 8 + z],
 // c = (dx+1) + 3 (dy+1) + 9 (dz+1); bricks outside the domain or unlisted are 0.
-__device__ __forceinline__ void nbhd_load(const Grid& g, const LvBits& lb, int bx, int by, int bz, uint64_t* nb) {
-  for (int q = threadIdx.x; q < 216; q += blockDim.x) {
-    const int c = q >> 3, w = q & 7;
-    const int nx = bx + c % 3 - 1, ny = by + (c / 3) % 3 - 1, nz = bz + c / 9 - 1;
-    uint64_t v = 0;
-    if (nx >= 0 && ny >= 0 && nz >= 0 && nx < int(g.BX) && ny < int(g.BY) && nz < int(g.BZ)) {
-      const uint32_t nbk = uint32_t(nx) + g.BX * (uint32_t(ny) + g.BY * uint32_t(nz));
-      const int pos = lb.page ? lb.page[nbk] : int(nbk);
-      if (pos >= 0) v = lb.bits[size_t(pos) * 8 + w];
-    }
-    nb[q] = v;
+__device__ __forceinline__ uint64_t nbhd_word(const Grid& g, const LvBits& lb, int bx, int by, int bz, int q) {
+  const int c = q >> 3, w = q & 7;
+  const int nx = bx + c % 3 - 1, ny = by + (c / 3) % 3 - 1, nz = bz + c / 9 - 1;
+  uint64_t v = 0;
+  if (nx >= 0 && ny >= 0 && nz >= 0 && nx < int(g.BX) && ny < int(g.BY) && nz < int(g.BZ)) {
+    const uint32_t nbk = uint32_t(nx) + g.BX * (uint32_t(ny) + g.BY * uint32_t(nz));
+    const int pos = lb.page ? lb.page[nbk] : int(nbk);
+    if (pos >= 0) v = lb.bits[size_t(pos) * 8 + w];
   }
+  return v;
+}
+__device__ __forceinline__ void nbhd_load(const Grid& g, const LvBits& lb, int bx, int by, int bz, uint64_t* nb) {
+  for (int q = threadIdx.x; q < 216; q += blockDim.x) nb[q] = nbhd_word(g, lb, bx, by, bz, q);
 }
 
 // ------------------------------------------------ fused V-cycle kernels --
@@ -1564,7 +1565,7 @@ __global__ __launch_bounds__(512) void k_residual_lean(const uint32_t* __restric
 // workgroup per listed coarse brick: the 18^3 fine region its taps reach is
 // staged in LDS (every load in flight at once), then each thread sums the 64
 // taps of its coarse voxel in the oracle's order (taps clamped to the domain).
-__global__ __launch_bounds__(512) void k_restrict_lean(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid f,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_restrict_lean(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid f,
                                                        Grid c, LvBits cb, const float* __restrict__ r, float* __restrict__ crhs) {
   constexpr int S = 18, N = S * S * S, IT = (N + 511) / 512;
   __shared__ float tf[N];
@@ -1671,6 +1672,65 @@ __global__ __launch_bounds__(512) void k_sweep(const uint32_t* __restrict__ list
       }
       dst[ii[u]] = v;
     }
+  }
+}
+
+// Two weighted-Jacobi sweeps of a V-cycle level in one launch, temporally
+// blocked in LDS: tiles of radius 2 around the brick, the first sweep on the
+// brick +- 1, the second on the brick. The next brick's tile loads and
+// neighbourhood words are issued before the current brick's sweeps, so their
+// latency overlaps the LDS work. Every voxel sees the same arithmetic as in two
+// k_sweep launches (same neighbour order, inactive voxels 0), so results are
+// bit-identical. ZERO: the first sweep starts from p = 0.
+template <bool ZERO>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_sweep2(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+                                                LvBits lb, const StepConst* __restrict__ C, const float* __restrict__ rhs,
+                                                const float* __restrict__ src, float* __restrict__ dst) {
+  FX_SIM_PRIORITY();
+  constexpr int R0 = 2, S = Tile<R0>::S;
+  __shared__ float ta[Tile<R0>::N], tb[Tile<R0>::N], tr[Tile<R0>::N];
+  __shared__ uint64_t nb[216];
+  const float om = C->omega;
+  const int tid = int(threadIdx.x);
+  const BrickRange br = brick_range(*count);
+  TileRegs<R0> rr, rp;
+  uint64_t nw = 0;
+  uint32_t b = 0;
+  auto fetch = [&](uint32_t k) {
+    b = list[k];
+    int bx, by, bz;
+    brick_coords(g, b, bx, by, bz);
+    const int ox = bx * 8 - R0, oy = by * 8 - R0, oz = bz * 8 - R0;
+#pragma unroll
+    for (int j = 0; j < TileRegs<R0>::IT; ++j) {
+      const int q = tid + j * 512;
+      const int x = ox + q % S, y = oy + (q / S) % S, z = oz + q / (S * S);
+      const bool in = q < Tile<R0>::N && x >= 0 && y >= 0 && z >= 0 && x < int(g.W) && y < int(g.H) && z < int(g.D);
+      const uint32_t id = in ? vidx(g, uint32_t(x), uint32_t(y), uint32_t(z)) : 0u;
+      rr.v[j] = in ? rhs[id] : 0.f;
+      rp.v[j] = (!ZERO && in) ? src[id] : 0.f;
+    }
+    nw = tid < 216 ? nbhd_word(g, lb, bx, by, bz, tid) : 0ull;
+  };
+  if (br.k < br.end) fetch(br.k);
+  for (uint32_t k = br.k; k < br.end; k += br.step) {
+    const uint32_t cb = b;
+    __syncthreads();  // the previous brick's tiles are no longer read
+    if (tid < 216) nb[tid] = nw;
+    __syncthreads();
+    tile_store<R0>(nb, rr, tr);
+    if (!ZERO) tile_store<R0>(nb, rp, ta); else tile_zero<R0>(ta);
+    __syncthreads();
+    if (k + br.step < br.end) fetch(k + br.step);
+    tile_sweep<R0, 1>(nb, ta, tr, tb, om);
+    __syncthreads();
+    tile_sweep<R0, 0>(nb, tb, tr, ta, om);
+    __syncthreads();
+    int bx, by, bz;
+    brick_coords(g, cb, bx, by, bz);
+    const int x = bx * 8 + (tid & 7), y = by * 8 + ((tid >> 3) & 7), z = bz * 8 + (tid >> 6);
+    if (x < int(g.W) && y < int(g.H) && z < int(g.D))
+      dst[(cb << 9) | uint32_t(tid)] = ta[(R0 + (tid & 7)) + S * ((R0 + ((tid >> 3) & 7)) + S * (R0 + (tid >> 6)))];
   }
 }
 
@@ -1946,7 +2006,7 @@ __global__ __launch_bounds__(512) void k_pack_render(const uint32_t* __restrict_
 // simulation's own kernels keep CUs while both run) take 16x16-pixel tiles from
 // `queue` (zeroed before launch) until none are left; each wave64 marches a
 // compact 8x8 pixel tile (coherent rays), 2x2 of them per workgroup tile.
-__global__ __launch_bounds__(256) void k_raymarch(Grid g, RayConst R, const float4* __restrict__ RB, const uint32_t* __restrict__ gbits,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_raymarch(Grid g, RayConst R, const float4* __restrict__ RB, const uint32_t* __restrict__ gbits,
                                                   uint32_t nwords, int use_lds, uint16_t* out, uint32_t* __restrict__ queue) {
   extern __shared__ uint32_t sbits[];
   __shared__ uint32_t tile;
@@ -2678,6 +2738,11 @@ void launch_sweep(hipStream_t s, int blocks, const uint32_t* list, const uint32_
   } else {
     FX_LAUNCH((k_sweep<true, false>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
   }
+}
+void launch_sweep_pair(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
+                       const float* rhs, const float* src, float* dst) {
+  if (src) FX_LAUNCH(k_sweep2<false>, blocks, 512, s, list, count, g, b, c, rhs, src, dst);
+  else FX_LAUNCH(k_sweep2<true>, blocks, 512, s, list, count, g, b, c, rhs, src, dst);
 }
 void launch_residual(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const float* rhs,
                      const float* p, float* r, double* norm2) {

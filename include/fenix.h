@@ -403,11 +403,15 @@ int fx_read_stage_times(fx_sim* sim, double* total_ms, uint32_t* calls, size_t n
  * down_split: pre-sweeps as one-sweep kernels + residual + restriction instead of
  * one fused LDS kernel (tiles of the brick plus a pre_sweeps + 2 halo);
  * up_split: prolongation + one-sweep kernels instead of one fused LDS kernel
- * (halo post_sweeps). Every plan gives bit-identical results; only speed differs. */
+ * (halo post_sweeps).
+ * Bit 8 + l of either mask: that split leg of level l runs its sweeps two per
+ * launch (LDS tiles of halo 2) instead of one per launch; ignored where the leg
+ * is fused. Every plan gives bit-identical results; only speed differs. */
 int fx_set_vcycle_plan(fx_sim* sim, uint32_t down_split, uint32_t up_split);
 int fx_get_vcycle_plan(fx_sim* sim, uint32_t* down_split, uint32_t* up_split);
 
-/* Autotuner: times each level's four leg plans (greedily, finest level first)
+/* Autotuner: times each level's leg plans (fused, split, split with paired
+ * sweeps, for the down and the up leg; greedily, finest level first)
  * with `repeats` graph-captured pressure solves of the current state and keeps
  * the fastest; returns the chosen masks. Solves start from p = 0 and every plan
  * is bit-identical, so the simulation's results do not depend on the outcome.

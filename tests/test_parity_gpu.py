@@ -215,14 +215,14 @@ def test_step_parity_sparse_plume():
 
 def test_vcycle_plans_bit_identical():
     """Split V-cycle legs (lean one-sweep kernels + residual/restriction and
-    prolongation + one sweep) give bit-identical results to the fused LDS legs,
-    and both match the oracle."""
+    prolongation + one sweep, or paired sweeps in LDS tiles) give bit-identical
+    results to the fused LDS legs, and all match the oracle."""
     n = 64
     srcs = scenes.c2_sources(n, radius_vox=2.5)
     st, _ = scenes.c2_plume_state(n, srcs, target=0.15)
     desc = scenes.c2_desc(n)
     out, res = {}, {}
-    for plan in ((0, 0), (0xF, 0), (0, 0xF), (0xF, 0xF)):
+    for plan in ((0, 0), (0xF, 0), (0, 0xF), (0xF, 0xF), (0xF0F, 0xF0F), (0x10F, 0x30F)):
         sim = CombustionVolumeSimulation(desc)
         sim.set_vcycle_plan(*plan)
         assert sim.vcycle_plan() == plan
