@@ -172,6 +172,15 @@ __device__ __forceinline__ BrickRange brick_range(uint32_t n) {
   const uint32_t lo = min(x * seg, n), hi = min(lo + seg, n);
   return BrickRange{lo + (blockIdx.x >> 3), hi, per};
 }
+// brick_range for kernels that process `lanes` bricks side by side in one
+// workgroup (blockDim / lanes threads per brick): slot s of workgroup w walks the
+// list as virtual workgroup w * lanes + s of its XCD's segment.
+__device__ __forceinline__ BrickRange brick_range_slots(uint32_t n, uint32_t lanes, uint32_t slot) {
+  if (gridDim.x < 8u || (gridDim.x & 7u) != 0u) return BrickRange{blockIdx.x * lanes + slot, n, gridDim.x * lanes};
+  const uint32_t x = blockIdx.x & 7u, per = (gridDim.x >> 3) * lanes, seg = (n + 7u) >> 3;
+  const uint32_t lo = min(x * seg, n), hi = min(lo + seg, n);
+  return BrickRange{lo + (blockIdx.x >> 3) * lanes + slot, hi, per};
+}
 // Simulation kernels share the CUs with the raymarch, which runs overlapped on
 // its own stream. s_setprio raises the wave's instruction-issue priority (0..3,
 // default 0): where both kinds of waves are resident the simulation's issue
@@ -1611,67 +1620,82 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
 }
 
 // One weighted-Jacobi sweep of a V-cycle level (pressure-jacobi.wgsl.ts:15-36,
-// mix(p, jacobi, omega)); one thread per voxel, neighbours addressed through
-// home-page brick arithmetic (active voxels are interior, so every neighbour
-// brick exists; unlisted ones hold 0). ZERO: the sweep starts from p = 0.
-// FINE: the finest level, where every in-domain voxel of a listed brick off the
-// boundary faces is active; the test uses the voxel's coordinates instead of the
-// activity words.
+// mix(p, jacobi, omega)). Each thread owns 4 consecutive voxels of a brick row
+// (x = 0..3 or 4..7): 128 threads per brick, 4 bricks side by side in a
+// workgroup. The row and its +-y and +-z neighbour rows are float4 loads; the
+// in-row x neighbours come from the registers and the partner lane (the other
+// half of the row, one shuffle); only the x neighbour across the brick face is a
+// scalar load. Every voxel sums its neighbours in the same order as a
+// one-voxel-per-thread sweep (xp, xm, yp, ym, zp, zm). Neighbours are addressed
+// through home-page brick arithmetic (active voxels are interior, so every
+// neighbour brick exists; unlisted ones hold 0), with brick ids clamped to the
+// allocation so that rows can be read before activity says whether they are
+// used. ZERO: the sweep starts from p = 0. FINE: the finest level, where every
+// in-domain voxel of a listed brick off the boundary faces is active; the test
+// uses coordinates instead of the activity words.
 template <bool ZERO, bool FINE>
 __global__ __launch_bounds__(512) void k_sweep(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
                                                LvBits lb, const StepConst* __restrict__ C, const float* __restrict__ rhs,
                                                const float* __restrict__ src, float* __restrict__ dst) {
   FX_SIM_PRIORITY();
   const float om = C->omega;
-  const uint32_t t = threadIdx.x, lx = t & 7u, ly = (t >> 3) & 7u, lz = t >> 6;
+  const uint32_t slot = threadIdx.x >> 7, t = threadIdx.x & 127u;
+  const uint32_t h = t & 1u, row = t >> 1, ly = row & 7u, lz = row >> 3, x0 = h * 4u;
   const uint32_t BX = g.BX, BXY = g.BX * g.BY, last = g.BX * g.BY * g.BZ - 1u;
-  const BrickRange br = brick_range(*count);
-  const uint32_t n = br.end;
-  // Two bricks per round, every load issued before any result is needed; the
-  // neighbour addresses are clamped to the allocation so that they can be read
-  // before the activity word says whether the voxel uses them.
-  for (uint32_t k0 = br.k; k0 < n; k0 += 2u * br.step) {
-    const uint32_t kk[2] = {k0, k0 + br.step};
-    uint32_t ii[2];
-    uint64_t w[2];
-    float sum[2], self[2], r[2];
+  const BrickRange br = brick_range_slots(*count, 4u, slot);
+  const uint32_t rofs = (lz << 6) | (ly << 3) | x0;  // element 0 of this thread within the brick
+  for (uint32_t k = br.k; k < br.end; k += br.step) {
+    const uint32_t b = list[k];
+    const uint32_t i0 = (b << 9) | rofs;
+    uint32_t act = 0;  // bit e: element e is active
+    if (FINE) {
+      const uint32_t x = (b % g.BX) * 8 + x0, y = ((b / g.BX) % g.BY) * 8 + ly, z = (b / BXY) * 8 + lz;
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const bool ok = kk[u] < n;
-      const uint32_t b = list[ok ? kk[u] : k0];
-      const uint32_t i = (b << 9) | t;
-      ii[u] = i;
-      if (FINE) {
-        const uint32_t x = (b % g.BX) * 8 + lx, y = ((b / g.BX) % g.BY) * 8 + ly, z = (b / BXY) * 8 + lz;
-        const bool act = ok && x < g.W && y < g.H && z < g.D && !on_boundary(g, x, y, z);
-        w[u] = act ? (1ull << (t & 63u)) : 0ull;
-      } else {
-        w[u] = ok ? lb.bits[size_t(kk[u]) * 8 + lz] : 0ull;
-      }
-      sum[u] = 0.f;
-      self[u] = 0.f;
-      if (!ZERO) {
-        const uint32_t xp = lx < 7u ? i + 1u : ((min(b + 1u, last) << 9) | (t - 7u));
-        const uint32_t xm = lx > 0u ? i - 1u : (((b > 0u ? b - 1u : 0u) << 9) | (t + 7u));
-        const uint32_t yp = ly < 7u ? i + 8u : ((min(b + BX, last) << 9) | (t - 56u));
-        const uint32_t ym = ly > 0u ? i - 8u : (((b >= BX ? b - BX : 0u) << 9) | (t + 56u));
-        const uint32_t zp = lz < 7u ? i + 64u : ((min(b + BXY, last) << 9) | (t - 448u));
-        const uint32_t zm = lz > 0u ? i - 64u : (((b >= BXY ? b - BXY : 0u) << 9) | (t + 448u));
-        sum[u] = src[xp] + src[xm] + src[yp] + src[ym] + src[zp] + src[zm];
-        self[u] = src[i];
-      }
-      r[u] = rhs[i];
+      for (uint32_t e = 0; e < 4; ++e)
+        act |= (x + e < g.W && y < g.H && z < g.D && !on_boundary(g, x + e, y, z)) ? (1u << e) : 0u;
+    } else {
+      act = uint32_t(lb.bits[size_t(k) * 8 + lz] >> ((ly << 3) | x0)) & 0xFu;
     }
+    const float4 r = *reinterpret_cast<const float4*>(rhs + i0);
+    float4 c = make_float4(0.f, 0.f, 0.f, 0.f), yp = c, ym = c, zp = c, zm = c;
+    float edge = 0.f;
+    if (!ZERO) {
+      const uint32_t iyp = ly < 7u ? i0 + 8u : ((min(b + BX, last) << 9) | (lz << 6) | x0);
+      const uint32_t iym = ly > 0u ? i0 - 8u : (((b >= BX ? b - BX : 0u) << 9) | (lz << 6) | (7u << 3) | x0);
+      const uint32_t izp = lz < 7u ? i0 + 64u : ((min(b + BXY, last) << 9) | (ly << 3) | x0);
+      const uint32_t izm = lz > 0u ? i0 - 64u : (((b >= BXY ? b - BXY : 0u) << 9) | (7u << 6) | (ly << 3) | x0);
+      // x neighbour across the brick face: x = -1 (brick b - 1, x = 7) for the
+      // first half of the row, x = 8 (brick b + 1, x = 0) for the second
+      const uint32_t iedge = h ? ((min(b + 1u, last) << 9) | (lz << 6) | (ly << 3))
+                               : (((b > 0u ? b - 1u : 0u) << 9) | (lz << 6) | (ly << 3) | 7u);
+      c = *reinterpret_cast<const float4*>(src + i0);
+      yp = *reinterpret_cast<const float4*>(src + iyp);
+      ym = *reinterpret_cast<const float4*>(src + iym);
+      zp = *reinterpret_cast<const float4*>(src + izp);
+      zm = *reinterpret_cast<const float4*>(src + izm);
+      edge = src[iedge];
+    }
+    // the partner lane holds the other half of the row: x = 4 for the first
+    // half's last element, x = 3 for the second half's first element
+    const float mine = h ? c.x : c.w;
+    const float other = __shfl_xor(mine, 1, 64);
+    const float xm0 = h ? other : edge, xp3 = h ? edge : other;
+    const float cs[4] = {c.x, c.y, c.z, c.w};
+    const float xps[4] = {c.y, c.z, c.w, xp3}, xms[4] = {xm0, c.x, c.y, c.z};
+    const float yps[4] = {yp.x, yp.y, yp.z, yp.w}, yms[4] = {ym.x, ym.y, ym.z, ym.w};
+    const float zps[4] = {zp.x, zp.y, zp.z, zp.w}, zms[4] = {zm.x, zm.y, zm.z, zm.w};
+    const float rs[4] = {r.x, r.y, r.z, r.w};
+    float out[4];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      if (kk[u] >= n) continue;
-      float v = 0.f;
-      if ((w[u] >> (t & 63u)) & 1ull) {
-        const float pj = (sum[u] - r[u]) / 6.f;
-        v = mixf(self[u], pj, om);
+    for (int e = 0; e < 4; ++e) {
+      out[e] = 0.f;
+      if ((act >> e) & 1u) {
+        const float sum = ZERO ? 0.f : xps[e] + xms[e] + yps[e] + yms[e] + zps[e] + zms[e];
+        const float pj = (sum - rs[e]) / 6.f;
+        out[e] = mixf(ZERO ? 0.f : cs[e], pj, om);
       }
-      dst[ii[u]] = v;
     }
+    *reinterpret_cast<float4*>(dst + i0) = make_float4(out[0], out[1], out[2], out[3]);
   }
 }
 
