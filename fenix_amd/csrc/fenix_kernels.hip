@@ -1509,56 +1509,127 @@ __global__ __launch_bounds__(512) void k_vup(const uint32_t* __restrict__ list, 
   }
 }
 
-// p += P(e) on the active voxels of a level, one thread per voxel: the
-// prolongation sampler of pressure-prolongation.wgsl.ts:29-44 with the same
-// arithmetic as the fused up-leg, in place.
+// p += P(e) on the active voxels of a level: the prolongation sampler of
+// pressure-prolongation.wgsl.ts:29-44 with the same arithmetic as the fused
+// up-leg, in place, on float4 brick rows (the layout of k_sweep). Voxel
+// x = 4m + q samples coarse columns x0 = floor(px), x1 = x0 + 1 with
+// px = 2m + q/2 - 1/4: columns (2m-1, 2m), (2m, 2m+1), (2m, 2m+1), (2m+1, 2m+2)
+// for q = 0..3. The thread loads columns 2m-1 .. 2m+2 (clamped to the level) at
+// the 4 (y, z) corners once. Where px is clamped to the level, the weight fx is
+// 0 and the clamped column holds the value the clamped x0 reads, so every blend
+// equals a one-voxel-per-thread prolongation's.
 __global__ __launch_bounds__(512) void k_prolong_add(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid c,
                                                      Grid f, LvBits lb, const float* __restrict__ e, float* __restrict__ p) {
-  const uint32_t t = threadIdx.x;
+  FX_SIM_PRIORITY();
+  const uint32_t slot = threadIdx.x >> 7, t = threadIdx.x & 127u;
+  const uint32_t h = t & 1u, ly = (t >> 1) & 7u, lz = t >> 4;
   const float mcx = float(c.W - 1), mcy = float(c.H - 1), mcz = float(c.D - 1);
-  FX_BRICK_LOOP(list, count) {
-    const uint32_t b = list[fx_k];
-    if (!((lb.bits[size_t(fx_k) * 8 + (t >> 6)] >> (t & 63u)) & 1ull)) continue;
-    const uint32_t x = (b % f.BX) * 8 + (t & 7u), y = ((b / f.BX) % f.BY) * 8 + ((t >> 3) & 7u), z = (b / (f.BX * f.BY)) * 8 + (t >> 6);
-    const float px = clampf((float(x) + 0.5f) * 0.5f - 0.5f, 0.f, mcx), py = clampf((float(y) + 0.5f) * 0.5f - 0.5f, 0.f, mcy),
-                pz = clampf((float(z) + 0.5f) * 0.5f - 0.5f, 0.f, mcz);
-    const uint32_t x0 = uint32_t(floorf(px)), y0 = uint32_t(floorf(py)), z0 = uint32_t(floorf(pz));
-    const uint32_t x1 = min(x0 + 1u, c.W - 1u), y1 = min(y0 + 1u, c.H - 1u), z1 = min(z0 + 1u, c.D - 1u);
-    const float fx = fractf(px), fy = fractf(py), fz = fractf(pz);
-    const float x00 = mixf(e[vidx(c, x0, y0, z0)], e[vidx(c, x1, y0, z0)], fx);
-    const float x10 = mixf(e[vidx(c, x0, y1, z0)], e[vidx(c, x1, y1, z0)], fx);
-    const float x01 = mixf(e[vidx(c, x0, y0, z1)], e[vidx(c, x1, y0, z1)], fx);
-    const float x11 = mixf(e[vidx(c, x0, y1, z1)], e[vidx(c, x1, y1, z1)], fx);
-    const uint32_t i = (b << 9) | t;
-    p[i] = p[i] + mixf(mixf(x00, x10, fy), mixf(x01, x11, fy), fz);
+  const BrickRange br = brick_range_slots(*count, 4u, slot);
+  for (uint32_t k = br.k; k < br.end; k += br.step) {
+    const uint32_t b = list[k];
+    const uint32_t act = uint32_t(lb.bits[size_t(k) * 8 + lz] >> ((ly << 3) | (h * 4u))) & 0xFu;
+    if (!act) continue;
+    const uint32_t xb = (b % f.BX) * 8 + h * 4u, y = ((b / f.BX) % f.BY) * 8 + ly, z = (b / (f.BX * f.BY)) * 8 + lz;
+    const float py = clampf((float(y) + 0.5f) * 0.5f - 0.5f, 0.f, mcy), pz = clampf((float(z) + 0.5f) * 0.5f - 0.5f, 0.f, mcz);
+    const uint32_t y0 = uint32_t(floorf(py)), z0 = uint32_t(floorf(pz));
+    const uint32_t y1 = min(y0 + 1u, c.H - 1u), z1 = min(z0 + 1u, c.D - 1u);
+    const float fy = fractf(py), fz = fractf(pz);
+    const int base = int(xb / 2u) - 1;  // 2m - 1
+    float col[4][4];                    // [column j][corner y0z0, y1z0, y0z1, y1z1]
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t xc = uint32_t(min(max(base + j, 0), int(c.W) - 1));
+      col[j][0] = e[vidx(c, xc, y0, z0)];
+      col[j][1] = e[vidx(c, xc, y1, z0)];
+      col[j][2] = e[vidx(c, xc, y0, z1)];
+      col[j][3] = e[vidx(c, xc, y1, z1)];
+    }
+    const uint32_t i0 = (b << 9) | (lz << 6) | (ly << 3) | (h * 4u);
+    float4 pv = *reinterpret_cast<const float4*>(p + i0);
+    float pr[4] = {pv.x, pv.y, pv.z, pv.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int j0 = q == 0 ? 0 : q == 3 ? 2 : 1, j1 = j0 + 1;
+      const float fx = fractf(clampf((float(xb + uint32_t(q)) + 0.5f) * 0.5f - 0.5f, 0.f, mcx));
+      const float x00 = mixf(col[j0][0], col[j1][0], fx);
+      const float x10 = mixf(col[j0][1], col[j1][1], fx);
+      const float x01 = mixf(col[j0][2], col[j1][2], fx);
+      const float x11 = mixf(col[j0][3], col[j1][3], fx);
+      const float add = pr[q] + mixf(mixf(x00, x10, fy), mixf(x01, x11, fy), fz);
+      pr[q] = ((act >> q) & 1u) ? add : pr[q];
+    }
+    *reinterpret_cast<float4*>(p + i0) = make_float4(pr[0], pr[1], pr[2], pr[3]);
   }
 }
 
+// Rows of a brick as float4s (4 voxels per thread, 128 threads per brick): the
+// layout of k_sweep. RowNbrs holds a row's own values and its six neighbour
+// sets in the order xp, xm, yp, ym, zp, zm; brick ids are clamped to the
+// allocation so that rows can be read before activity says whether they are used.
+struct RowNbrs {
+  float c[4], xp[4], xm[4], yp[4], ym[4], zp[4], zm[4];
+};
+__device__ __forceinline__ void row_load(const float* __restrict__ f, uint32_t b, uint32_t h, uint32_t ly, uint32_t lz,
+                                         uint32_t BX, uint32_t BXY, uint32_t last, RowNbrs& R) {
+  const uint32_t x0 = h * 4u, i0 = (b << 9) | (lz << 6) | (ly << 3) | x0;
+  const uint32_t iyp = ly < 7u ? i0 + 8u : ((min(b + BX, last) << 9) | (lz << 6) | x0);
+  const uint32_t iym = ly > 0u ? i0 - 8u : (((b >= BX ? b - BX : 0u) << 9) | (lz << 6) | (7u << 3) | x0);
+  const uint32_t izp = lz < 7u ? i0 + 64u : ((min(b + BXY, last) << 9) | (ly << 3) | x0);
+  const uint32_t izm = lz > 0u ? i0 - 64u : (((b >= BXY ? b - BXY : 0u) << 9) | (7u << 6) | (ly << 3) | x0);
+  const uint32_t iedge = h ? ((min(b + 1u, last) << 9) | (lz << 6) | (ly << 3))
+                           : (((b > 0u ? b - 1u : 0u) << 9) | (lz << 6) | (ly << 3) | 7u);
+  const float4 c = *reinterpret_cast<const float4*>(f + i0);
+  const float4 yp = *reinterpret_cast<const float4*>(f + iyp);
+  const float4 ym = *reinterpret_cast<const float4*>(f + iym);
+  const float4 zp = *reinterpret_cast<const float4*>(f + izp);
+  const float4 zm = *reinterpret_cast<const float4*>(f + izm);
+  const float edge = f[iedge];
+  const float other = __shfl_xor(h ? c.x : c.w, 1, 64);
+  R.c[0] = c.x; R.c[1] = c.y; R.c[2] = c.z; R.c[3] = c.w;
+  R.xp[0] = c.y; R.xp[1] = c.z; R.xp[2] = c.w; R.xp[3] = h ? edge : other;
+  R.xm[0] = h ? other : edge; R.xm[1] = c.x; R.xm[2] = c.y; R.xm[3] = c.z;
+  R.yp[0] = yp.x; R.yp[1] = yp.y; R.yp[2] = yp.z; R.yp[3] = yp.w;
+  R.ym[0] = ym.x; R.ym[1] = ym.y; R.ym[2] = ym.z; R.ym[3] = ym.w;
+  R.zp[0] = zp.x; R.zp[1] = zp.y; R.zp[2] = zp.z; R.zp[3] = zp.w;
+  R.zm[0] = zm.x; R.zm[1] = zm.y; R.zm[2] = zm.z; R.zm[3] = zm.w;
+}
+
 // Residual r = rhs - (sum of 6 neighbours - 6 p) of a V-cycle level
-// (pressure-residual.wgsl.ts:16-38), one thread per voxel, with the FP64 sum of
-// squares over the active voxels when norm2 != nullptr.
+// (pressure-residual.wgsl.ts:16-38) on float4 brick rows, with the FP64 sum of
+// squares over the active voxels when norm2 != nullptr. Each voxel's arithmetic
+// is that of a one-voxel-per-thread residual; only the FP64 summation order of
+// the norm differs.
 __global__ __launch_bounds__(512) void k_residual_lean(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
                                                        LvBits lb, const float* __restrict__ rhs, const float* __restrict__ p,
                                                        float* __restrict__ r, double* norm2) {
+  FX_SIM_PRIORITY();
   __shared__ double part[8];
-  const uint32_t t = threadIdx.x, lx = t & 7u, ly = (t >> 3) & 7u, lz = t >> 6;
-  const uint32_t BX = g.BX, BXY = g.BX * g.BY;
+  const uint32_t slot = threadIdx.x >> 7, t = threadIdx.x & 127u;
+  const uint32_t h = t & 1u, ly = (t >> 1) & 7u, lz = t >> 4;
+  const uint32_t BX = g.BX, BXY = g.BX * g.BY, last = g.BX * g.BY * g.BZ - 1u;
+  const uint32_t rofs = (lz << 6) | (ly << 3) | (h * 4u);
   double acc = 0.0;
-  FX_BRICK_LOOP(list, count) {
-    const uint32_t b = list[fx_k];
-    const uint32_t i = (b << 9) | t;
-    if (!((lb.bits[size_t(fx_k) * 8 + lz] >> (t & 63u)) & 1ull)) { r[i] = 0.f; continue; }
-    const uint32_t xp = lx < 7u ? i + 1u : (((b + 1u) << 9) | (t - 7u));
-    const uint32_t xm = lx > 0u ? i - 1u : (((b - 1u) << 9) | (t + 7u));
-    const uint32_t yp = ly < 7u ? i + 8u : (((b + BX) << 9) | (t - 56u));
-    const uint32_t ym = ly > 0u ? i - 8u : (((b - BX) << 9) | (t + 56u));
-    const uint32_t zp = lz < 7u ? i + 64u : (((b + BXY) << 9) | (t - 448u));
-    const uint32_t zm = lz > 0u ? i - 64u : (((b - BXY) << 9) | (t + 448u));
-    const float sum = p[xp] + p[xm] + p[yp] + p[ym] + p[zp] + p[zm];
-    const float lap = sum - p[i] * 6.f;
-    const float res = rhs[i] - lap;
-    r[i] = res;
-    acc += double(res) * double(res);
+  const BrickRange br = brick_range_slots(*count, 4u, slot);
+  for (uint32_t k = br.k; k < br.end; k += br.step) {
+    const uint32_t b = list[k];
+    const uint32_t i0 = (b << 9) | rofs;
+    const uint32_t act = uint32_t(lb.bits[size_t(k) * 8 + lz] >> ((ly << 3) | (h * 4u))) & 0xFu;
+    const float4 rh = *reinterpret_cast<const float4*>(rhs + i0);
+    RowNbrs R;
+    row_load(p, b, h, ly, lz, BX, BXY, last, R);
+    const float rs[4] = {rh.x, rh.y, rh.z, rh.w};
+    float out[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      out[e] = 0.f;
+      if ((act >> e) & 1u) {
+        const float sum = R.xp[e] + R.xm[e] + R.yp[e] + R.ym[e] + R.zp[e] + R.zm[e];
+        const float lap = sum - R.c[e] * 6.f;
+        out[e] = rs[e] - lap;
+        acc += double(out[e]) * double(out[e]);
+      }
+    }
+    *reinterpret_cast<float4*>(r + i0) = make_float4(out[0], out[1], out[2], out[3]);
   }
   if (!norm2) return;
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
@@ -1620,19 +1691,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
 }
 
 // One weighted-Jacobi sweep of a V-cycle level (pressure-jacobi.wgsl.ts:15-36,
-// mix(p, jacobi, omega)). Each thread owns 4 consecutive voxels of a brick row
-// (x = 0..3 or 4..7): 128 threads per brick, 4 bricks side by side in a
-// workgroup. The row and its +-y and +-z neighbour rows are float4 loads; the
-// in-row x neighbours come from the registers and the partner lane (the other
-// half of the row, one shuffle); only the x neighbour across the brick face is a
-// scalar load. Every voxel sums its neighbours in the same order as a
-// one-voxel-per-thread sweep (xp, xm, yp, ym, zp, zm). Neighbours are addressed
-// through home-page brick arithmetic (active voxels are interior, so every
-// neighbour brick exists; unlisted ones hold 0), with brick ids clamped to the
-// allocation so that rows can be read before activity says whether they are
-// used. ZERO: the sweep starts from p = 0. FINE: the finest level, where every
-// in-domain voxel of a listed brick off the boundary faces is active; the test
-// uses coordinates instead of the activity words.
+// mix(p, jacobi, omega)) on float4 brick rows (row_load): each thread owns 4
+// consecutive voxels of a brick row (x = 0..3 or 4..7), 128 threads per brick, 4
+// bricks side by side in a workgroup. The row and its +-y and +-z neighbour rows
+// are float4 loads; the in-row x neighbours come from the registers and the
+// partner lane (the other half of the row, one shuffle); only the x neighbour
+// across the brick face is a scalar load. Every voxel sums its neighbours in the
+// same order as a one-voxel-per-thread sweep (xp, xm, yp, ym, zp, zm).
+// Neighbours are addressed through home-page brick arithmetic (active voxels are
+// interior, so every neighbour brick exists; unlisted ones hold 0). ZERO: the
+// sweep starts from p = 0. FINE: the finest level, where every in-domain voxel
+// of a listed brick off the boundary faces is active; the test uses coordinates
+// instead of the activity words.
 template <bool ZERO, bool FINE>
 __global__ __launch_bounds__(512) void k_sweep(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
                                                LvBits lb, const StepConst* __restrict__ C, const float* __restrict__ rhs,
@@ -1657,42 +1727,17 @@ __global__ __launch_bounds__(512) void k_sweep(const uint32_t* __restrict__ list
       act = uint32_t(lb.bits[size_t(k) * 8 + lz] >> ((ly << 3) | x0)) & 0xFu;
     }
     const float4 r = *reinterpret_cast<const float4*>(rhs + i0);
-    float4 c = make_float4(0.f, 0.f, 0.f, 0.f), yp = c, ym = c, zp = c, zm = c;
-    float edge = 0.f;
-    if (!ZERO) {
-      const uint32_t iyp = ly < 7u ? i0 + 8u : ((min(b + BX, last) << 9) | (lz << 6) | x0);
-      const uint32_t iym = ly > 0u ? i0 - 8u : (((b >= BX ? b - BX : 0u) << 9) | (lz << 6) | (7u << 3) | x0);
-      const uint32_t izp = lz < 7u ? i0 + 64u : ((min(b + BXY, last) << 9) | (ly << 3) | x0);
-      const uint32_t izm = lz > 0u ? i0 - 64u : (((b >= BXY ? b - BXY : 0u) << 9) | (7u << 6) | (ly << 3) | x0);
-      // x neighbour across the brick face: x = -1 (brick b - 1, x = 7) for the
-      // first half of the row, x = 8 (brick b + 1, x = 0) for the second
-      const uint32_t iedge = h ? ((min(b + 1u, last) << 9) | (lz << 6) | (ly << 3))
-                               : (((b > 0u ? b - 1u : 0u) << 9) | (lz << 6) | (ly << 3) | 7u);
-      c = *reinterpret_cast<const float4*>(src + i0);
-      yp = *reinterpret_cast<const float4*>(src + iyp);
-      ym = *reinterpret_cast<const float4*>(src + iym);
-      zp = *reinterpret_cast<const float4*>(src + izp);
-      zm = *reinterpret_cast<const float4*>(src + izm);
-      edge = src[iedge];
-    }
-    // the partner lane holds the other half of the row: x = 4 for the first
-    // half's last element, x = 3 for the second half's first element
-    const float mine = h ? c.x : c.w;
-    const float other = __shfl_xor(mine, 1, 64);
-    const float xm0 = h ? other : edge, xp3 = h ? edge : other;
-    const float cs[4] = {c.x, c.y, c.z, c.w};
-    const float xps[4] = {c.y, c.z, c.w, xp3}, xms[4] = {xm0, c.x, c.y, c.z};
-    const float yps[4] = {yp.x, yp.y, yp.z, yp.w}, yms[4] = {ym.x, ym.y, ym.z, ym.w};
-    const float zps[4] = {zp.x, zp.y, zp.z, zp.w}, zms[4] = {zm.x, zm.y, zm.z, zm.w};
+    RowNbrs R{};
+    if (!ZERO) row_load(src, b, h, ly, lz, BX, BXY, last, R);
     const float rs[4] = {r.x, r.y, r.z, r.w};
     float out[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       out[e] = 0.f;
       if ((act >> e) & 1u) {
-        const float sum = ZERO ? 0.f : xps[e] + xms[e] + yps[e] + yms[e] + zps[e] + zms[e];
+        const float sum = ZERO ? 0.f : R.xp[e] + R.xm[e] + R.yp[e] + R.ym[e] + R.zp[e] + R.zm[e];
         const float pj = (sum - rs[e]) / 6.f;
-        out[e] = mixf(ZERO ? 0.f : cs[e], pj, om);
+        out[e] = mixf(ZERO ? 0.f : R.c[e], pj, om);
       }
     }
     *reinterpret_cast<float4*>(dst + i0) = make_float4(out[0], out[1], out[2], out[3]);
