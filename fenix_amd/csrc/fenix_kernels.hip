@@ -112,24 +112,47 @@ __device__ __forceinline__ float tri_eval(const Tri& t, const float* __restrict_
 }
 // Storage of density, temperature and fuel: FP32, BF16 (round to nearest even)
 // or FP16. Every load widens to FP32; all arithmetic is FP32.
+// ld4/st4: elements i .. i+3 (i a multiple of 4) as one 16- or 8-byte access.
 struct StF32 {
   using T = float;
   static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return p[i]; }
   static __device__ __forceinline__ void st(T* p, uint32_t i, float v) { p[i] = v; }
+  static __device__ __forceinline__ float4 ld4(const T* p, uint32_t i) { return *reinterpret_cast<const float4*>(p + i); }
+  static __device__ __forceinline__ void st4(T* p, uint32_t i, float4 v) { *reinterpret_cast<float4*>(p + i) = v; }
 };
 struct StBF16 {
   using T = uint16_t;
-  static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return __uint_as_float(uint32_t(p[i]) << 16); }
-  static __device__ __forceinline__ void st(T* p, uint32_t i, float v) {
+  static __device__ __forceinline__ uint16_t enc(float v) {
     uint32_t u = __float_as_uint(v);
     u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite values)
-    p[i] = uint16_t(u >> 16);
+    return uint16_t(u >> 16);
+  }
+  static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return __uint_as_float(uint32_t(p[i]) << 16); }
+  static __device__ __forceinline__ void st(T* p, uint32_t i, float v) { p[i] = enc(v); }
+  static __device__ __forceinline__ float4 ld4(const T* p, uint32_t i) {
+    const uint2 w = *reinterpret_cast<const uint2*>(p + i);
+    return make_float4(__uint_as_float(w.x << 16), __uint_as_float(w.x & 0xffff0000u), __uint_as_float(w.y << 16),
+                       __uint_as_float(w.y & 0xffff0000u));
+  }
+  static __device__ __forceinline__ void st4(T* p, uint32_t i, float4 v) {
+    *reinterpret_cast<uint2*>(p + i) = make_uint2(uint32_t(enc(v.x)) | (uint32_t(enc(v.y)) << 16),
+                                                  uint32_t(enc(v.z)) | (uint32_t(enc(v.w)) << 16));
   }
 };
 struct StF16 {
   using T = __half;
   static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return __half2float(p[i]); }
   static __device__ __forceinline__ void st(T* p, uint32_t i, float v) { p[i] = __float2half(v); }
+  static __device__ __forceinline__ float4 ld4(const T* p, uint32_t i) {
+    const uint2 w = *reinterpret_cast<const uint2*>(p + i);
+    return make_float4(__half2float(__ushort_as_half(uint16_t(w.x))), __half2float(__ushort_as_half(uint16_t(w.x >> 16))),
+                       __half2float(__ushort_as_half(uint16_t(w.y))), __half2float(__ushort_as_half(uint16_t(w.y >> 16))));
+  }
+  static __device__ __forceinline__ void st4(T* p, uint32_t i, float4 v) {
+    const uint32_t a = __half_as_ushort(__float2half(v.x)), b = __half_as_ushort(__float2half(v.y));
+    const uint32_t c = __half_as_ushort(__float2half(v.z)), d = __half_as_ushort(__float2half(v.w));
+    *reinterpret_cast<uint2*>(p + i) = make_uint2(a | (b << 16), c | (d << 16));
+  }
 };
 template <class ST>
 struct SS {
@@ -181,12 +204,86 @@ __device__ __forceinline__ BrickRange brick_range_slots(uint32_t n, uint32_t lan
   const uint32_t lo = min(x * seg, n), hi = min(lo + seg, n);
   return BrickRange{lo + (blockIdx.x >> 3) * lanes + slot, hi, per};
 }
+// Float4 brick rows with the domain clamps of the reference's stencils
+// (incu/decu + cidx: a neighbour past a domain face is the voxel itself). A
+// thread owns the 4 voxels x = x .. x+3 of one brick row (128 threads per
+// brick); RowCtx holds the row's index, the indices of its +-y and +-z
+// neighbour rows (the row itself where the neighbour is past a face) and of the
+// x neighbour across the brick face. Every index lies in the allocation.
+struct RowCtx {
+  uint32_t i0, iyp, iym, izp, izm, iedge;
+  uint32_t x, y, z, h;
+  uint32_t nin;  // elements inside the domain (x + e < W, y < H, z < D)
+};
+__device__ __forceinline__ RowCtx row_ctx(const Grid& g, uint32_t b, uint32_t t) {
+  RowCtx R;
+  const uint32_t h = t & 1u, ly = (t >> 1) & 7u, lz = t >> 4, x0 = h * 4u;
+  const uint32_t BX = g.BX, BXY = g.BX * g.BY;
+  R.h = h;
+  R.x = (b % g.BX) * 8 + x0;
+  R.y = ((b / g.BX) % g.BY) * 8 + ly;
+  R.z = (b / BXY) * 8 + lz;
+  R.i0 = (b << 9) | (lz << 6) | (ly << 3) | x0;
+  R.iyp = R.y + 1u < g.H ? (ly < 7u ? R.i0 + 8u : ((b + BX) << 9) | (lz << 6) | x0) : R.i0;
+  R.iym = R.y > 0u ? (ly > 0u ? R.i0 - 8u : ((b - BX) << 9) | (lz << 6) | (7u << 3) | x0) : R.i0;
+  R.izp = R.z + 1u < g.D ? (lz < 7u ? R.i0 + 64u : ((b + BXY) << 9) | (ly << 3) | x0) : R.i0;
+  R.izm = R.z > 0u ? (lz > 0u ? R.i0 - 64u : ((b - BXY) << 9) | (7u << 6) | (ly << 3) | x0) : R.i0;
+  if (h) R.iedge = R.x + 4u < g.W ? ((b + 1u) << 9) | (lz << 6) | (ly << 3) : R.i0 + 3u;
+  else R.iedge = R.x > 0u ? ((b - 1u) << 9) | (lz << 6) | (ly << 3) | 7u : R.i0;
+  R.nin = (R.y < g.H && R.z < g.D) ? min(g.W - min(R.x, g.W), 4u) : 0u;
+  return R;
+}
+__device__ __forceinline__ void unpack4(float4 v, float* a) { a[0] = v.x; a[1] = v.y; a[2] = v.z; a[3] = v.w; }
+// The row's own values and its clamped x neighbours (one shuffle with the
+// partner lane, which holds the other half of the row; call with both lanes
+// active).
+template <class ST>
+__device__ __forceinline__ void row_x(const Grid& g, const RowCtx& R, const typename ST::T* __restrict__ f, float* c, float* xp,
+                                      float* xm) {
+  unpack4(ST::ld4(f, R.i0), c);
+  const float edge = ST::ld(f, R.iedge);
+  const float other = __shfl_xor(R.h ? c[0] : c[3], 1, 64);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const uint32_t x = R.x + uint32_t(e);
+    const float next = e < 3 ? c[e + 1] : (R.h ? edge : other);
+    const float prev = e > 0 ? c[e - 1] : (R.h ? other : edge);
+    xp[e] = x + 1u < g.W ? next : c[e];
+    xm[e] = x > 0u ? prev : c[e];
+  }
+}
+template <class ST>
+__device__ __forceinline__ void row_y(const RowCtx& R, const typename ST::T* __restrict__ f, float* yp, float* ym) {
+  unpack4(ST::ld4(f, R.iyp), yp);
+  unpack4(ST::ld4(f, R.iym), ym);
+}
+template <class ST>
+__device__ __forceinline__ void row_z(const RowCtx& R, const typename ST::T* __restrict__ f, float* zp, float* zm) {
+  unpack4(ST::ld4(f, R.izp), zp);
+  unpack4(ST::ld4(f, R.izm), zm);
+}
+// Store the row's in-domain elements.
+template <class ST>
+__device__ __forceinline__ void row_store(const RowCtx& R, typename ST::T* __restrict__ f, const float* v) {
+  if (R.nin == 4u) {
+    ST::st4(f, R.i0, make_float4(v[0], v[1], v[2], v[3]));
+  } else {
+    for (uint32_t e = 0; e < R.nin; ++e) ST::st(f, R.i0 + e, v[e]);
+  }
+}
 // Simulation kernels share the CUs with the raymarch, which runs overlapped on
 // its own stream. s_setprio raises the wave's instruction-issue priority (0..3,
 // default 0): where both kinds of waves are resident the simulation's issue
 // first. The raymarch keeps priority 0 and finishes within the step anyway
 // (measured on the 256^3 plume: 1.135 -> 1.074 ms per step).
 #define FX_SIM_PRIORITY() __builtin_amdgcn_s_setprio(2)
+// Row kernels: 4 bricks side by side per 512-thread workgroup, thread fx_t
+// (0..127) of its brick owns one half brick row (row_ctx).
+#define FX_ROW_LOOP(list, count)                                           \
+  FX_SIM_PRIORITY();                                                       \
+  const uint32_t fx_t = threadIdx.x & 127u;                                \
+  const BrickRange fx_r = brick_range_slots(*(count), 4u, threadIdx.x >> 7); \
+  for (uint32_t fx_k = fx_r.k; fx_k < fx_r.end; fx_k += fx_r.step)
 #define FX_BRICK_LOOP(list, count)                                       \
   FX_SIM_PRIORITY();                                                     \
   const BrickRange fx_r = brick_range(*(count));                         \
@@ -848,21 +945,31 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
 }
 
 // ------------------------------------------------ vorticity/confinement --
-// SH/compute-vorticity.wgsl.ts:23-50
+// SH/compute-vorticity.wgsl.ts:23-50, on float4 brick rows: x neighbours of
+// u_y and u_z, y neighbours of u_x and u_z, z neighbours of u_x and u_y.
 __global__ __launch_bounds__(512) void k_vorticity(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g, Vec3 us,
                                                    Vec3 w, float* wmag) {
-  FX_BRICK_LOOP(list, count) {
-    Vox v = vox_of(g, list[fx_k]);
-    if (!v.in) continue;
-    const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
-    if (on_boundary(g, x, y, z)) { w.x[i] = 0.f; w.y[i] = 0.f; w.z[i] = 0.f; wmag[i] = 0.f; continue; }
-    auto rv = [&](uint32_t a, uint32_t b2, uint32_t c2) { uint32_t k = cidx(g, a, b2, c2); return f3(us.x[k], us.y[k], us.z[k]); };
-    F3 x0 = rv(decu(x), y, z), x1 = rv(incu(x, g.W - 1), y, z);
-    F3 y0 = rv(x, decu(y), z), y1 = rv(x, incu(y, g.H - 1), z);
-    F3 z0 = rv(x, y, decu(z)), z1 = rv(x, y, incu(z, g.D - 1));
-    F3 curl = f3((y1.z - y0.z) - (z1.y - z0.y), (z1.x - z0.x) - (x1.z - x0.z), (x1.y - x0.y) - (y1.x - y0.x)) * 0.5f;
-    w.x[i] = curl.x; w.y[i] = curl.y; w.z[i] = curl.z;
-    wmag[i] = len3(curl);
+  FX_ROW_LOOP(list, count) {
+    const RowCtx R = row_ctx(g, list[fx_k], fx_t);
+    float c[4], yxp[4], yxm[4], zxp[4], zxm[4], xyp[4], xym[4], zyp[4], zym[4], xzp[4], xzm[4], yzp[4], yzm[4];
+    row_x<StF32>(g, R, us.y, c, yxp, yxm);
+    row_x<StF32>(g, R, us.z, c, zxp, zxm);
+    row_y<StF32>(R, us.x, xyp, xym);
+    row_y<StF32>(R, us.z, zyp, zym);
+    row_z<StF32>(R, us.x, xzp, xzm);
+    row_z<StF32>(R, us.y, yzp, yzm);
+    float ox[4], oy[4], oz[4], om[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (on_boundary(g, R.x + uint32_t(e), R.y, R.z)) { ox[e] = oy[e] = oz[e] = om[e] = 0.f; continue; }
+      F3 curl = f3((zyp[e] - zym[e]) - (yzp[e] - yzm[e]), (xzp[e] - xzm[e]) - (zxp[e] - zxm[e]), (yxp[e] - yxm[e]) - (xyp[e] - xym[e])) * 0.5f;
+      ox[e] = curl.x; oy[e] = curl.y; oz[e] = curl.z;
+      om[e] = len3(curl);
+    }
+    row_store<StF32>(R, w.x, ox);
+    row_store<StF32>(R, w.y, oy);
+    row_store<StF32>(R, w.z, oz);
+    row_store<StF32>(R, wmag, om);
   }
 }
 // SH/apply-vorticity-confinement.wgsl.ts:30-66 (render set)
@@ -893,18 +1000,22 @@ __global__ __launch_bounds__(512) void k_confinement(const uint32_t* __restrict_
 }
 
 // ------------------------------------------------ divergence / pressure --
-// SH/compute-divergence.wgsl.ts:22-42
-__global__ __launch_bounds__(512) void k_divergence(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g, Vec3 us,
+// SH/compute-divergence.wgsl.ts:22-42, on float4 brick rows.
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_divergence(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g, Vec3 us,
                                                     float* div) {
-  FX_BRICK_LOOP(list, count) {
-    Vox v = vox_of(g, list[fx_k]);
-    if (!v.in) continue;
-    const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
-    if (on_boundary(g, x, y, z)) { div[i] = 0.f; continue; }
-    float dx = us.x[cidx(g, incu(x, g.W - 1), y, z)] - us.x[cidx(g, decu(x), y, z)];
-    float dy = us.y[cidx(g, x, incu(y, g.H - 1), z)] - us.y[cidx(g, x, decu(y), z)];
-    float dz = us.z[cidx(g, x, y, incu(z, g.D - 1))] - us.z[cidx(g, x, y, decu(z))];
-    div[i] = (dx + dy + dz) * 0.5f;
+  FX_ROW_LOOP(list, count) {
+    const RowCtx R = row_ctx(g, list[fx_k], fx_t);
+    float c[4], xp[4], xm[4], yp[4], ym[4], zp[4], zm[4], out[4];
+    row_x<StF32>(g, R, us.x, c, xp, xm);
+    row_y<StF32>(R, us.y, yp, ym);
+    row_z<StF32>(R, us.z, zp, zm);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (on_boundary(g, R.x + uint32_t(e), R.y, R.z)) { out[e] = 0.f; continue; }
+      const float dx = xp[e] - xm[e], dy = yp[e] - ym[e], dz = zp[e] - zm[e];
+      out[e] = (dx + dy + dz) * 0.5f;
+    }
+    row_store<StF32>(R, div, out);
   }
 }
 __device__ __forceinline__ bool lv_active(const LvMask& m, uint32_t x, uint32_t y, uint32_t z) {
@@ -1803,18 +1914,25 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   }
 }
 
-// SH/project-velocity.wgsl.ts:21-43
+// SH/project-velocity.wgsl.ts:21-43, on float4 brick rows.
 __global__ __launch_bounds__(512) void k_projection(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g, Vec3 us,
                                                     const float* __restrict__ p, Vec3 u) {
-  FX_BRICK_LOOP(list, count) {
-    Vox v = vox_of(g, list[fx_k]);
-    if (!v.in) continue;
-    const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
-    if (on_boundary(g, x, y, z)) { u.x[i] = 0.f; u.y[i] = 0.f; u.z[i] = 0.f; continue; }
-    float gx = (p[cidx(g, incu(x, g.W - 1), y, z)] - p[cidx(g, decu(x), y, z)]) * 0.5f;
-    float gy = (p[cidx(g, x, incu(y, g.H - 1), z)] - p[cidx(g, x, decu(y), z)]) * 0.5f;
-    float gz = (p[cidx(g, x, y, incu(z, g.D - 1))] - p[cidx(g, x, y, decu(z))]) * 0.5f;
-    u.x[i] = us.x[i] - gx; u.y[i] = us.y[i] - gy; u.z[i] = us.z[i] - gz;
+  FX_ROW_LOOP(list, count) {
+    const RowCtx R = row_ctx(g, list[fx_k], fx_t);
+    float c[4], xp[4], xm[4], yp[4], ym[4], zp[4], zm[4], ux[4], uy[4], uz[4];
+    row_x<StF32>(g, R, p, c, xp, xm);
+    row_y<StF32>(R, p, yp, ym);
+    row_z<StF32>(R, p, zp, zm);
+    unpack4(StF32::ld4(us.x, R.i0), ux); unpack4(StF32::ld4(us.y, R.i0), uy); unpack4(StF32::ld4(us.z, R.i0), uz);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (on_boundary(g, R.x + uint32_t(e), R.y, R.z)) { ux[e] = uy[e] = uz[e] = 0.f; continue; }
+      const float gx = (xp[e] - xm[e]) * 0.5f, gy = (yp[e] - ym[e]) * 0.5f, gz = (zp[e] - zm[e]) * 0.5f;
+      ux[e] = ux[e] - gx; uy[e] = uy[e] - gy; uz[e] = uz[e] - gz;
+    }
+    row_store<StF32>(R, u.x, ux);
+    row_store<StF32>(R, u.y, uy);
+    row_store<StF32>(R, u.z, uz);
   }
 }
 // SH/compute-debug-fields.wgsl.ts:18-25
