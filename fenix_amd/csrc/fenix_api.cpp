@@ -436,6 +436,10 @@ int solve_vcycle(fx_sim* s) {
       A.m[l] = mask_of(s, l);
       A.bits[l] = s->lv[l].bits;
     }
+    // the FP64 residual-norm accumulators of this solve start at zero
+    const int ncyc = std::min(v.cycles, s->norm_cap);
+    A.zero = ncyc > 0 ? reinterpret_cast<uint32_t*>(s->d_norms) : nullptr;
+    A.zero_words = size_t(std::max(ncyc, 0)) * kNormLanes * 2;
     launch_level_masks(s->stream, grid_for(s, s->lv[0].nb), A);
   }
   Level& LC = s->lv[L - 1];
@@ -473,9 +477,7 @@ int solve_vcycle(fx_sim* s) {
       cur[l] = 1 - cur[l];
     }
   };
-  int cycles = std::min(v.cycles, s->norm_cap);
-  if (cycles > 0) launch_clear_u32(s->stream, reinterpret_cast<uint32_t*>(s->d_norms), size_t(cycles) * kNormLanes * 2, 0u);
-  s->norm_count = cycles;
+  s->norm_count = std::min(v.cycles, s->norm_cap);
   for (int c = 0; c < v.cycles; ++c) {
     for (int l = 0; l < L - 1; ++l) {
       Level& V = s->lv[l];

@@ -75,6 +75,8 @@ struct LevelMaskArgs {
   Grid g[FX_MAX_LEVELS_DEV];
   LvMask m[FX_MAX_LEVELS_DEV];
   uint64_t* bits[FX_MAX_LEVELS_DEV];
+  uint32_t* zero;  // zero_words words cleared by workgroup 0 (the solve's FP64 norm accumulators)
+  size_t zero_words;
 };
 
 // Launch helpers implemented in fenix_kernels.hip. All are asynchronous on `s`.

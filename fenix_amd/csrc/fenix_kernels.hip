@@ -636,18 +636,36 @@ __global__ __launch_bounds__(256) void k_compact_count(CompactArgs A) {
   uint32_t tr = scan256(ret_n, &e, lds + 4);
   if (threadIdx.x == 0) { A.counts[2 * blockIdx.x] = ta; A.counts[2 * blockIdx.x + 1] = tr; }
 }
+// Exclusive offsets of every block's active and retiring counts, per level, and
+// the level totals: one 1024-thread workgroup, a workgroup-wide scan per chunk of
+// 1024 blocks with a running carry (one chunk up to 1024^2 bricks per level).
 __global__ __launch_bounds__(1024) void k_compact_scan(CompactArgs A) {
   FX_SIM_PRIORITY();
-  if (threadIdx.x != 0) return;
+  __shared__ uint32_t wa[16], wr[16];
+  const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
   for (int l = 0; l < A.nlevels; ++l) {
-    uint32_t a = 0, r = 0;
-    for (uint32_t j = A.block_start[l]; j < A.block_start[l + 1]; ++j) {
-      uint32_t ca = A.counts[2 * j], cr = A.counts[2 * j + 1];
-      A.offsets[2 * j] = a; A.offsets[2 * j + 1] = r;
-      a += ca; r += cr;
+    uint32_t carry_a = 0, carry_r = 0;
+    for (uint32_t c0 = A.block_start[l]; c0 < A.block_start[l + 1]; c0 += 1024u) {
+      const uint32_t j = c0 + t;
+      const bool in = j < A.block_start[l + 1];
+      const uint32_t ca = in ? A.counts[2 * j] : 0u, cr = in ? A.counts[2 * j + 1] : 0u;
+      uint32_t xa = ca, xr = cr;
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t ya = __shfl_up(xa, off, 64), yr = __shfl_up(xr, off, 64);
+        if (lane >= uint32_t(off)) { xa += ya; xr += yr; }
+      }
+      if (lane == 63u) { wa[wv] = xa; wr[wv] = xr; }
+      __syncthreads();
+      uint32_t ba = 0, br = 0, ta = 0, tr = 0;
+      for (uint32_t k = 0; k < 16; ++k) {
+        if (k < wv) { ba += wa[k]; br += wr[k]; }
+        ta += wa[k]; tr += wr[k];
+      }
+      if (in) { A.offsets[2 * j] = carry_a + ba + xa - ca; A.offsets[2 * j + 1] = carry_r + br + xr - cr; }
+      carry_a += ta; carry_r += tr;
+      __syncthreads();  // wa / wr are rewritten by the next chunk
     }
-    *A.L[l].count = a;
-    *A.L[l].retire_count = r;
+    if (t == 0) { *A.L[l].count = carry_a; *A.L[l].retire_count = carry_r; }
   }
 }
 __global__ __launch_bounds__(256) void k_compact_write(CompactArgs A) {
@@ -1303,6 +1321,8 @@ __global__ __launch_bounds__(512) void k_prolong(const uint32_t* __restrict__ li
 __global__ __launch_bounds__(512) void k_level_masks(LevelMaskArgs a) {
   FX_SIM_PRIORITY();
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  if (blockIdx.x == 0 && a.zero)
+    for (size_t q = threadIdx.x; q < a.zero_words; q += blockDim.x) a.zero[q] = 0u;
   uint32_t start[FX_MAX_LEVELS_DEV + 1];
   start[0] = 0;
   for (int l = 0; l < a.nlevels; ++l) start[l + 1] = start[l] + *a.count[l];
@@ -2800,7 +2820,7 @@ void launch_activate(hipStream_t s, int blocks, const uint32_t* listA, const uin
 void launch_compact(hipStream_t s, const CompactArgs& A) {
   const uint32_t blocks = A.block_start[A.nlevels];
   FX_LAUNCH(k_compact_count, blocks, 256, s, A);
-  FX_LAUNCH(k_compact_scan, 1, 64, s, A);
+  FX_LAUNCH(k_compact_scan, 1, 1024, s, A);
   FX_LAUNCH(k_compact_write, blocks, 256, s, A);
 }
 void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
