@@ -403,3 +403,50 @@ def test_graph_replay_matches_eager(sparse, monkeypatch):
         for name in PERSISTENT + ("pressure",):
             np.testing.assert_array_equal(sims["1"].read_field(name), sims["0"].read_field(name), err_msg=f"{name} step {k}")
         np.testing.assert_array_equal(sims["1"].read_flags(), sims["0"].read_flags())
+
+
+def test_velocity_written_through_device_pointer_reaches_the_advection():
+    """The velocity advection gathers a float4 mirror of u. A velocity written
+    through fx_field_device_ptr (a borrowed pointer) must be seen by the next
+    step exactly like one written with fx_write_field. Dense domain: in a sparse
+    one fx_write_field also makes the written bricks activation candidates, which
+    a write through the pointer cannot."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = 40
+    desc = scenes.c2_desc(n, domain_mode=abi.FX_DOMAIN_DENSE)
+    srcs = scenes.c2_sources(n, radius_vox=3.0)
+    st, _ = scenes.c2_plume_state(n, srcs, target=0.2)
+    other = {k: v for k, v in scenes.random_state((n, n, n), 11).items() if k.startswith("velocity_")}
+    sims = []
+    for _ in range(3):
+        sim = CombustionVolumeSimulation(desc)
+        sim.updateSources(srcs)
+        for k, v in st.items():
+            sim.write_field(k, v)
+        sims.append(sim)
+    a, b, c = sims
+    for k, v in other.items():
+        c.write_field(k, v)
+    for k in range(2):
+        a.step(None, k / 60.0, 1 / 60.0)
+        b.step(None, k / 60.0, 1 / 60.0)
+    for s in sims:
+        s.synchronize()
+    nbytes = ((n + 7) // 8) ** 3 * 512 * 4
+    for comp in ("velocity_x", "velocity_y", "velocity_z"):
+        fid = abi.FIELDS[comp]
+        src = c._lib.fx_field_device_ptr(c._h, fid)
+        dst = b._lib.fx_field_device_ptr(b._h, fid)
+        assert src and dst
+        assert hip.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes), 3) == 0  # device to device
+        a.write_field(comp, c.read_field(comp))
+    assert hip.hipDeviceSynchronize() == 0
+    for k in range(2, 4):
+        a.step(None, k / 60.0, 1 / 60.0)
+        b.step(None, k / 60.0, 1 / 60.0)
+    for name in PERSISTENT:
+        np.testing.assert_array_equal(b.read_field(name), a.read_field(name), err_msg=name)
+    assert float(np.abs(a.read_field("velocity_y")).max()) > 0.1
+    for s in sims:
+        s.dispose()
