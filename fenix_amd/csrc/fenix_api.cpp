@@ -125,6 +125,7 @@ struct fx_sim {
   // (fx_run_pass) both mirrors are packed over every brick.
   void* d_sq = nullptr;
   void* d_uq = nullptr;
+  bool dense_tables = false;  // dense V-cycle: level activity words and coarsest table built
   // Bloom: three half-resolution rgba16f images (bright, horizontal and vertical
   // blur) for the last size, and staging for the host-memory variant.
   void* d_bloom = nullptr;
@@ -440,13 +441,20 @@ int solve_vcycle(fx_sim* s) {
     const int ncyc = std::min(v.cycles, s->norm_cap);
     A.zero = ncyc > 0 ? reinterpret_cast<uint32_t*>(s->d_norms) : nullptr;
     A.zero_words = size_t(std::max(ncyc, 0)) * kNormLanes * 2;
-    launch_level_masks(s->stream, grid_for(s, s->lv[0].nb), A);
+    // A dense domain's levels list every brick and activity is geometric, so the
+    // activity words and the coarsest slot table (whose inactive voxels the
+    // sweeps never write) are built once; later solves only clear the norms.
+    if (s->sparse || !s->dense_tables)
+      launch_level_masks(s->stream, grid_for(s, s->lv[0].nb), A);
+    else if (A.zero)
+      launch_clear_u32(s->stream, A.zero, A.zero_words, 0u);
   }
   Level& LC = s->lv[L - 1];
   const bool block_smoother = L > 1 && LC.nv <= kBlockSmoothVoxels;
-  if (block_smoother)
+  if (block_smoother && (s->sparse || !s->dense_tables))
     launch_coarse_setup(s->stream, level_list(s, L - 1), level_count(s, L - 1), LC.g, bits_of(s, L - 1), LC.tab, LC.tab_n, LC.p[0],
                         LC.p[1]);
+  if (!s->sparse) s->dense_tables = true;
   // n sweeps at level l; zero_start: the first sweep starts from p = 0 (a
   // coarse level's correction), so neither a clear nor a read of p is needed.
   auto smoothW = [&](int l, int n, bool zero_start, bool pair) {
