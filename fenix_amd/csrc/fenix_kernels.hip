@@ -2618,6 +2618,75 @@ __device__ __forceinline__ void stage_scatter(float* tile, const TileOrigin& o, 
     stage_add(tile, o, glob, g, c.x0 + ((k & 1) ? c.dx : 0), c.y0 + ((k & 2) ? c.dy : 0), c.z0 + ((k & 4) ? c.dz : 0),
               wx[k & 1] * wy[(k >> 1) & 1] * wz[k >> 2] * a);
 }
+// Scatter of NF fields sharing one trilinear cell c, with the 8 corner values of
+// neighbouring lanes merged first. The 64 lanes of a wave are the 8x8 voxels of a
+// brick z-slice (lane = x + 8 y). Where the cell of lane - 1 is the cell of this
+// lane shifted by -1 in x, its +x corners are this lane's -x corners: they are
+// added here through a shuffle and the left lane drops them; then the same along
+// y with lane - 8. In smooth flow this leaves about a third of the LDS atomics.
+// Every lane of the wave must call this (zero values where nothing scatters);
+// the addition order differs from a plain scatter only as atomics already do.
+template <int NF>
+__device__ __forceinline__ void stage_scatter_merged(float* const* tiles, const TileOrigin& o, float* const* glob, const Grid& g,
+                                                     const TriC& c, const float* a) {
+  const int lane = int(threadIdx.x & 63u), lx = lane & 7, ly = lane >> 3;
+  const float wx[2] = {1.f - c.fx, c.fx}, wy[2] = {1.f - c.fy, c.fy}, wz[2] = {1.f - c.fz, c.fz};
+  float v[NF][8];
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[f][k] = wx[k & 1] * wy[(k >> 1) & 1] * wz[k >> 2] * a[f];
+  // x: lane - 1
+  {
+    const int px = __shfl_up(c.x0, 1, 64), py = __shfl_up(c.y0, 1, 64), pz = __shfl_up(c.z0, 1, 64);
+    const int pdx = __shfl_up(c.dx, 1, 64), pdy = __shfl_up(c.dy, 1, 64), pdz = __shfl_up(c.dz, 1, 64);
+    const bool from_prev = lx > 0 && pdx == 1 && px + 1 == c.x0 && py == c.y0 && pz == c.z0 && pdy == c.dy && pdz == c.dz;
+    // shuffles run with every lane active (a lane reading an inactive lane gets 0)
+    const int next_takes = __shfl_down(from_prev ? 1 : 0, 1, 64);
+    const bool to_next = lx < 7 && next_takes != 0;
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int k = 1; k < 8; k += 2) {
+        const float pv = __shfl_up(v[f][k], 1, 64);
+        v[f][k - 1] += from_prev ? pv : 0.f;
+      }
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int k = 1; k < 8; k += 2)
+        if (to_next) v[f][k] = 0.f;
+  }
+  // y: lane - 8
+  {
+    const int px = __shfl_up(c.x0, 8, 64), py = __shfl_up(c.y0, 8, 64), pz = __shfl_up(c.z0, 8, 64);
+    const int pdx = __shfl_up(c.dx, 8, 64), pdy = __shfl_up(c.dy, 8, 64), pdz = __shfl_up(c.dz, 8, 64);
+    const bool from_prev = ly > 0 && pdy == 1 && py + 1 == c.y0 && px == c.x0 && pz == c.z0 && pdx == c.dx && pdz == c.dz;
+    // shuffles run with every lane active (a lane reading an inactive lane gets 0)
+    const int next_takes = __shfl_down(from_prev ? 1 : 0, 8, 64);
+    const bool to_next = ly < 7 && next_takes != 0;
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (!(k & 2)) continue;
+        const float pv = __shfl_up(v[f][k], 8, 64);
+        v[f][k - 2] += from_prev ? pv : 0.f;
+      }
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if ((k & 2) && to_next) v[f][k] = 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int x = c.x0 + ((k & 1) ? c.dx : 0), y = c.y0 + ((k & 2) ? c.dy : 0), z = c.z0 + ((k & 4) ? c.dz : 0);
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+      if (v[f][k] != 0.f) stage_add(tiles[f], o, glob[f], g, x, y, z, v[f][k]);
+  }
+}
 // Stores the tile of list entry k to stage (layout [list entry][kScatN]) and clears it.
 __device__ __forceinline__ void stage_store(float* tile, float* __restrict__ stage, uint32_t k) {
   float* dst = stage + size_t(k) * kScatN;
@@ -2676,16 +2745,22 @@ __global__ __launch_bounds__(512) void k_da_advect_d(const uint32_t* __restrict_
     const Vox v = vox_of(g, b);
     const TileOrigin o = tile_origin(g, b);
     const float ob = v.in ? dnb[v.i] : 0.f;
+    // Every lane takes part in the merged scatters below, so the per-lane work
+    // leaves its scatter values here (zero where nothing scatters): predb and the
+    // forward-velocity adjoint fb at the backtrace cell cp, recb at the cell cr.
+    float predb = 0.f, recb = 0.f;
+    F3 fb = f3(0.f, 0.f, 0.f);
+    F3 sp = f3(float(v.x), float(v.y), float(v.z)), sr = sp;
     if (ob != 0.f) {
-      const F3 sp = f3(float(v.x) - ua.x[v.i] * dt, float(v.y) - ua.y[v.i] * dt, float(v.z) - ua.z[v.i] * dt);
+      sp = f3(float(v.x) - ua.x[v.i] * dt, float(v.y) - ua.y[v.i] * dt, float(v.z) - ua.z[v.i] * dt);
       const Tri tp = tri(g, sp);
       const F3 fwd = f3(tri_eval(tp, ua.x), tri_eval(tp, ua.y), tri_eval(tp, ua.z));
-      const F3 sr = sp + fwd * dt;
+      sr = sp + fwd * dt;
       const Tri tr = tri(g, sr);
       const float pred = tri_eval(tp, d), rec = tri_eval(tr, d), orig = d[v.i];
       const float corr = pred + 0.5f * (orig - rec);
       const float lo = fminf(orig, pred), hi = fmaxf(orig, pred);
-      float cb = 0.f, predb = 0.f, origb = 0.f;
+      float cb = 0.f, origb = 0.f;
       if (corr < lo) {
         if (orig <= pred) origb = ob; else predb = ob;
       } else if (corr > hi) {
@@ -2695,25 +2770,27 @@ __global__ __launch_bounds__(512) void k_da_advect_d(const uint32_t* __restrict_
       }
       predb += cb;
       origb += 0.5f * cb;
-      const float recb = -0.5f * cb;
-      const TriC cp = tri_coords(g, sp);
+      recb = -0.5f * cb;
       stage_add(tile[0], o, dsb, g, int(v.x), int(v.y), int(v.z), origb);
-      stage_scatter(tile[0], o, dsb, g, cp, predb);
       // sp = x - u_a(x) dt; sr = sp + u_a(sp) dt
       F3 spb = f3(0.f, 0.f, 0.f);
       if (predb != 0.f) spb = tri_grad(g, sp, tp, d) * predb;
       if (recb != 0.f) {
-        stage_scatter(tile[0], o, dsb, g, tri_coords(g, sr), recb);
         const F3 srb = tri_grad(g, sr, tr, d) * recb;
-        const F3 fb = srb * dt;
-        stage_scatter(tile[1], o, abar.x, g, cp, fb.x);
-        stage_scatter(tile[2], o, abar.y, g, cp, fb.y);
-        stage_scatter(tile[3], o, abar.z, g, cp, fb.z);
+        fb = srb * dt;
         spb = spb + srb + tri_grad(g, sp, tp, ua.x) * fb.x + tri_grad(g, sp, tp, ua.y) * fb.y + tri_grad(g, sp, tp, ua.z) * fb.z;
       }
       if (spb.x != 0.f) stage_add(tile[1], o, abar.x, g, int(v.x), int(v.y), int(v.z), -dt * spb.x);
       if (spb.y != 0.f) stage_add(tile[2], o, abar.y, g, int(v.x), int(v.y), int(v.z), -dt * spb.y);
       if (spb.z != 0.f) stage_add(tile[3], o, abar.z, g, int(v.x), int(v.y), int(v.z), -dt * spb.z);
+    }
+    {
+      float* tiles[4] = {tile[0], tile[1], tile[2], tile[3]};
+      float* globs[4] = {dsb, abar.x, abar.y, abar.z};
+      const float ap[4] = {predb, fb.x, fb.y, fb.z};
+      stage_scatter_merged<4>(tiles, o, globs, g, tri_coords(g, sp), ap);
+      const float ar[1] = {recb};
+      stage_scatter_merged<1>(tiles, o, globs, g, tri_coords(g, sr), ar);
     }
     __syncthreads();
     for (int f = 0; f < 4; ++f) stage_store(tile[f], stage + f * stage_field, fx_k);
@@ -2734,13 +2811,19 @@ __global__ __launch_bounds__(512) void k_da_advect_u(const uint32_t* __restrict_
     const TileOrigin o = tile_origin(g, b);
     const bool live = v.in && interior(g, v.x, v.y, v.z);
     const F3 a = live ? f3(abar.x[v.i], abar.y[v.i], abar.z[v.i]) : f3(0.f, 0.f, 0.f);
-    if (a.x != 0.f || a.y != 0.f || a.z != 0.f) {
-      const F3 sp = f3(float(v.x) - u.x[v.i] * dt, float(v.y) - u.y[v.i] * dt, float(v.z) - u.z[v.i] * dt);
+    const bool any = a.x != 0.f || a.y != 0.f || a.z != 0.f;
+    // every lane takes part in the merged scatter (zero values where nothing scatters)
+    const F3 sp = any ? f3(float(v.x) - u.x[v.i] * dt, float(v.y) - u.y[v.i] * dt, float(v.z) - u.z[v.i] * dt)
+                      : f3(float(v.x), float(v.y), float(v.z));
+    const TriC c = tri_coords(g, sp);
+    {
+      float* tiles[3] = {tile[0], tile[1], tile[2]};
+      float* globs[3] = {usb.x, usb.y, usb.z};
+      const float av[3] = {a.x, a.y, a.z};
+      stage_scatter_merged<3>(tiles, o, globs, g, c, av);
+    }
+    if (any) {
       const Tri t = tri(g, sp);
-      const TriC c = tri_coords(g, sp);
-      stage_scatter(tile[0], o, usb.x, g, c, a.x);
-      stage_scatter(tile[1], o, usb.y, g, c, a.y);
-      stage_scatter(tile[2], o, usb.z, g, c, a.z);
       const F3 spb = tri_grad(g, sp, t, u.x) * a.x + tri_grad(g, sp, t, u.y) * a.y + tri_grad(g, sp, t, u.z) * a.z;
       if (spb.x != 0.f) stage_add(tile[0], o, usb.x, g, int(v.x), int(v.y), int(v.z), -dt * spb.x);
       if (spb.y != 0.f) stage_add(tile[1], o, usb.y, g, int(v.x), int(v.y), int(v.z), -dt * spb.y);
