@@ -364,6 +364,8 @@ void pass_activate_render(fx_sim* s) {
       R.first[l] = nb;
       if (l == 0) {
         for (const Buf& b : fine_buffers(s)) { R.esz[nb] = uint8_t(b.esz); R.b[nb++] = b.p; }
+        // the float4 gather mirrors, which the step packs for listed bricks only
+        for (void* m : {s->d_sq, s->d_uq}) { R.esz[nb] = 16; R.b[nb++] = m; }
       } else {
         for (float* p : {s->lv[l].div, s->lv[l].p[0], s->lv[l].p[1]}) { R.esz[nb] = 4; R.b[nb++] = p; }
       }
@@ -546,13 +548,15 @@ void run_stage(fx_sim* s, int pass) {
   const int blocks = grid_for(s, s->nb);
   switch (pass) {
     case FX_PASS_ADVECT_VELOCITY:
-      launch_advect_velocity(s->stream, blocks, list, count, s->g, s->d_const, vec_of(s->u), vec_of(s->us), s->d_uq);
+      // inside a step the scalar gather mirror is packed here (retired bricks were
+      // zeroed by the activation pass); outside one, the scalar advection packs it
+      launch_advect_velocity(s->stream, blocks, list, count, s->g, s->d_const, vec_of(s->u), vec_of(s->us), s->d_uq, set_of(s, 0),
+                             s->in_step ? s->d_sq : nullptr);
       break;
     case FX_PASS_ADVECT_SCALARS: {
-      const bool all = !s->in_step || !s->sparse;
-      launch_pack_gather(s->stream, grid_for(s, s->nb), all ? s->identity : list, all ? s->identity_count : count, set_of(s, 0),
-                         s->in_step ? Vec3{nullptr, nullptr, nullptr} : vec_of(s->us), all ? nullptr : s->lv[0].L.retire,
-                         all ? nullptr : s->lv[0].L.retire_count, s->d_sq, s->d_uq);
+      if (!s->in_step)  // a pass run alone: both mirrors over every brick
+        launch_pack_gather(s->stream, grid_for(s, s->nb), s->identity, s->identity_count, set_of(s, 0), vec_of(s->us), nullptr,
+                           nullptr, s->d_sq, s->d_uq);
       launch_advect_scalars(s->stream, blocks, list, count, s->g, s->d_const, s->mode, s->d_sq, s->d_uq, set_of(s, 1), s->flagsInj);
       break;
     }

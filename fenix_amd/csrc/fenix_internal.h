@@ -164,8 +164,9 @@ void launch_smooth_block(hipStream_t s, const uint32_t* list, const uint32_t* co
                          const float* rhs, float* p0, float* p1, int sweeps, int weighted, int zero_start, const void* tab,
                          const uint32_t* tab_n);
 // uq: float4 mirror of us (written with it)
+// sq (optional): also packs the scalar gather mirror of set0 over the listed bricks.
 void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
-                            const StepConst* c, Vec3 u, Vec3 us, void* uq);
+                            const StepConst* c, Vec3 u, Vec3 us, void* uq, Scalars set0, void* sq);
 // Gather mirrors for the scalar advection: sq = float4 (d, T, f, r) of set0 over the
 // list, uq = float4 of us when us.x is non-null; both zeroed in the retired bricks.
 void launch_pack_gather(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Scalars set0, Vec3 us,
@@ -192,7 +193,7 @@ struct RetireArgs {
   const uint32_t* count[FX_MAX_LEVELS_DEV];
   int first[FX_MAX_LEVELS_DEV + 1];
   void* b[FX_MAX_ZERO_BUFS + 3 * FX_MAX_LEVELS_DEV];
-  uint8_t esz[FX_MAX_ZERO_BUFS + 3 * FX_MAX_LEVELS_DEV];  // element bytes (4 or 2)
+  uint8_t esz[FX_MAX_ZERO_BUFS + 3 * FX_MAX_LEVELS_DEV];  // element bytes (4, 2 or 16)
 };
 void launch_retire(hipStream_t s, int blocks, const RetireArgs& R);
 void launch_jacobi(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvMask m,

@@ -714,6 +714,7 @@ __global__ __launch_bounds__(512) void k_retire(RetireArgs R) {
       const uint32_t i = (R.list[l][k] << 9) | threadIdx.x;
       for (int q = R.first[l]; q < R.first[l + 1]; ++q) {
         if (R.esz[q] == 4) static_cast<float*>(R.b[q])[i] = 0.f;
+        else if (R.esz[q] == 16) static_cast<float4*>(R.b[q])[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         else static_cast<uint16_t*>(R.b[q])[i] = 0;
       }
     }
@@ -731,11 +732,16 @@ __global__ __launch_bounds__(512) void k_zero_bricks(const uint32_t* __restrict_
 
 // ----------------------------------------------------------- advection --
 // SH/advect-velocity-semilagrangian.wgsl.ts:23-41
+// sq (optional): also packs the scalar gather mirror (d, T, f, r) of set S for
+// every voxel of the listed bricks, which the scalar advection reads next.
+template <class ST>
 __global__ __launch_bounds__(512) void k_advect_velocity(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                         const StepConst* __restrict__ C, Vec3 u, Vec3 us, float4* __restrict__ uq) {
+                                                         const StepConst* __restrict__ C, Vec3 u, Vec3 us, float4* __restrict__ uq,
+                                                         SS<ST> S, float4* __restrict__ sq) {
   const float dt = C->p.delta_time;
   FX_BRICK_LOOP(list, count) {
     Vox v = vox_of(g, list[fx_k]);
+    if (sq) sq[v.i] = make_float4(ST::ld(S.d, v.i), ST::ld(S.t, v.i), ST::ld(S.f, v.i), S.r[v.i]);
     if (!v.in) continue;
     if (on_boundary(g, v.x, v.y, v.z)) {
       us.x[v.i] = 0.f; us.y[v.i] = 0.f; us.z[v.i] = 0.f;
@@ -2907,8 +2913,14 @@ void launch_compact(hipStream_t s, const CompactArgs& A) {
   FX_LAUNCH(k_compact_write, blocks, 256, s, A);
 }
 void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
-                            Vec3 u, Vec3 us, void* uq) {
-  FX_LAUNCH(k_advect_velocity, blocks, 512, s, list, count, g, c, u, us, static_cast<float4*>(uq));
+                            Vec3 u, Vec3 us, void* uq, Scalars set0, void* sq) {
+  float4* U4 = static_cast<float4*>(uq);
+  float4* S4 = static_cast<float4*>(sq);
+  switch (set0.prec) {
+    case FX_PRECISION_BF16: FX_LAUNCH(k_advect_velocity<StBF16>, blocks, 512, s, list, count, g, c, u, us, U4, typed<StBF16>(set0), S4); break;
+    case FX_PRECISION_FP16: FX_LAUNCH(k_advect_velocity<StF16>, blocks, 512, s, list, count, g, c, u, us, U4, typed<StF16>(set0), S4); break;
+    default: FX_LAUNCH(k_advect_velocity<StF32>, blocks, 512, s, list, count, g, c, u, us, U4, typed<StF32>(set0), S4);
+  }
 }
 void launch_pack_gather(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Scalars set0, Vec3 us,
                         const uint32_t* rlist, const uint32_t* rcount, void* sq, void* uq) {
