@@ -246,6 +246,66 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def run_c4(args):
+    """C4 precision at full size: sparse 512^3, 4 floor emitters (seed 4) under a
+    seeded developed plume of ~8% of the bricks, 100 steps (or --steps) in config
+    mode, once per storage precision (FP32, BF16, FP16 for density, temperature and
+    fuel; FP32 arithmetic). Reports each run's step time and throughput, and the
+    16-bit runs' relative L2 and max error against the FP32 run after the last
+    step. The FP64 oracle cannot run 512^3, so accuracy against FP64 is pinned at
+    64^3 by the precision tests, where FP32 stays within ~4e-6 of FP64."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import scenes
+    n, nsteps = 512, max(1, args.steps)
+    srcs = scenes.c2_sources(n, seed=4, count=4, radius_vox=20.0)
+    print("c4: building the 512^3 start state", file=sys.stderr, flush=True)
+    state = scenes.plume_state_fast(n, srcs, target=0.08, seed=4)
+    fields = ("density", "temperature", "fuel", "reaction", "velocity_x", "velocity_y", "velocity_z")
+    dt = 1.0 / 60.0
+    runs, ref = {}, None
+    for name in ("fp32", "bf16", "fp16"):
+        sim = CombustionVolumeSimulation(scenes.c2_desc(n, precision=PRECISIONS[name][0]))
+        sim.updateSources(srcs)
+        for k, v in state.items():
+            sim.write_field(k, v)
+        sim.step(None, 0.0, dt)
+        sim.synchronize()
+        sim.autotune_vcycle(4)
+        active0 = sim.active_brick_count()
+        t0 = time.perf_counter()
+        for k in range(1, nsteps + 1):
+            sim.step(None, k * dt, dt)
+        sim.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / nsteps
+        active = 0.5 * (active0 + sim.active_brick_count())
+        out = {f: sim.read_field(f) for f in fields}
+        sim.dispose()
+        run = {"ms_per_step": ms, "value": active * 512 / (ms * 1e-3), "active_bricks": active}
+        if ref is None:
+            ref = out
+        else:
+            err = {}
+            for f in fields:
+                a, b = out[f].astype(np.float64), ref[f].astype(np.float64)
+                nb = float(np.linalg.norm(b))
+                err[f] = {"rel_l2": float(np.linalg.norm(a - b)) / nb if nb > 0 else 0.0, "max_abs": float(np.abs(a - b).max())}
+            run["vs_fp32"] = err
+        runs[name] = run
+        print(f"c4 {name}: {ms:.3f} ms/step, {active:.0f} active bricks", file=sys.stderr, flush=True)
+    sel = runs[args.precision]
+    nb = ((n + 7) // 8) ** 3
+    print(json.dumps({
+        "metric": "active voxel-updates/s per step @512^3 sparse (C4 precision runs)",
+        "value": sel["value"], "unit": "active voxel-updates/s", "n_gpus": 1, "steps": nsteps, "warmup": 1,
+        "ms_per_step": sel["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "sparse 512^3, 8^3 bricks, 4 floor emitters r=20 (seed 4), threshold activation, "
+                               "4-level V-cycle, seeded plume start (~8% of bricks), no render",
+                   "storage": args.precision, "active_fraction": sel["active_bricks"] / nb},
+        "runs": runs,
+    }))
+
+
 def run_c5(args):
     """Inflow-control gradients (fx_diff): dense 128^3 FP32, 64 steps, checkpoint 8,
     control 32^3 x 4 into the floor slab, target 3 Gaussian blobs (sigma 12, seed 5).
@@ -304,7 +364,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c2-cold", "c1", "c3", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c2-cold", "c1", "c3", "c4", "c5"])
     ap.add_argument("--adam-iters", type=int, default=16, help="c5: Adam iterations after the timed gradients")
     ap.add_argument("--timing-steps", type=int, default=10, help="extra steps timed per stage with HIP events")
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -334,6 +394,8 @@ def main():
         return run_batch(args)
     if args.workload == "c5":
         return run_c5(args)
+    if args.workload == "c4":
+        return run_c4(args)
 
     wl = workload(args.workload, args.precision)
     saved = 4 - PRECISIONS[args.precision][1]
