@@ -11,6 +11,10 @@ holding ~15% of the bricks active (`c2-cold` starts from an empty volume).
 (default 100) are timed: a "step" is fx_step + the raymarch, with every input
 already resident in HBM.
 
+Other workloads: `c1` (dense 64^3, reference semantics), `c3` and `--batch K`
+(batched scenes against one at a time), `c4` (sparse 512^3 precision runs),
+`c5` (loss + gradient of the differentiable step).
+
 Prints one JSON line (see the keys below).
 """
 import argparse
