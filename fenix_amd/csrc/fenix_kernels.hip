@@ -1841,43 +1841,83 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
 // of a listed brick off the boundary faces is active; the test uses coordinates
 // instead of the activity words.
 template <bool ZERO, bool FINE>
-__global__ __launch_bounds__(512) void k_sweep(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_sweep(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
                                                LvBits lb, const StepConst* __restrict__ C, const float* __restrict__ rhs,
                                                const float* __restrict__ src, float* __restrict__ dst) {
   FX_SIM_PRIORITY();
   const float om = C->omega;
-  const uint32_t slot = threadIdx.x >> 7, t = threadIdx.x & 127u;
-  const uint32_t h = t & 1u, row = t >> 1, ly = row & 7u, lz = row >> 3, x0 = h * 4u;
+  // one wave per brick, 8 bricks per workgroup: lane t owns x = 4h .. 4h+3 of row
+  // y in the two z-layers z0 = 2 zq and z1 = z0 + 1 (each layer's z neighbour on
+  // that side is the other layer, in registers); a wave's loads of one layer pair
+  // stay contiguous (1 KB per instruction)
+  const uint32_t slot = threadIdx.x >> 6, t = threadIdx.x & 63u;
+  const uint32_t h = t & 1u, ly = (t >> 1) & 7u, zq = t >> 4, x0 = h * 4u, z0 = 2u * zq;
   const uint32_t BX = g.BX, BXY = g.BX * g.BY, last = g.BX * g.BY * g.BZ - 1u;
   const BrickRange br = brick_range_slots(*count, 4u, slot);
-  const uint32_t rofs = (lz << 6) | (ly << 3) | x0;  // element 0 of this thread within the brick
+  const uint32_t rofs = (z0 << 6) | (ly << 3) | x0;
   for (uint32_t k = br.k; k < br.end; k += br.step) {
     const uint32_t b = list[k];
-    const uint32_t i0 = (b << 9) | rofs;
-    uint32_t act = 0;  // bit e: element e is active
+    const uint32_t i0 = (b << 9) | rofs, i1 = i0 + 64u;
+    uint32_t act = 0;  // bits 0-3: layer z0, bits 4-7: layer z1
     if (FINE) {
-      const uint32_t x = (b % g.BX) * 8 + x0, y = ((b / g.BX) % g.BY) * 8 + ly, z = (b / BXY) * 8 + lz;
+      // in the domain and off its faces: 1 <= c <= n - 2 on every axis (separable)
+      const uint32_t x = (b % g.BX) * 8 + x0, y = ((b / g.BX) % g.BY) * 8 + ly, z = (b / BXY) * 8 + z0;
+      uint32_t xs = 0;
 #pragma unroll
-      for (uint32_t e = 0; e < 4; ++e)
-        act |= (x + e < g.W && y < g.H && z < g.D && !on_boundary(g, x + e, y, z)) ? (1u << e) : 0u;
+      for (uint32_t e = 0; e < 4; ++e) xs |= (x + e >= 1u && x + e + 2u <= g.W) ? (1u << e) : 0u;
+      const bool yok = y >= 1u && y + 2u <= g.H;
+      const bool z0ok = z >= 1u && z + 2u <= g.D, z1ok = z + 3u <= g.D;
+      act = yok ? ((z0ok ? xs : 0u) | (z1ok ? xs << 4 : 0u)) : 0u;
     } else {
-      act = uint32_t(lb.bits[size_t(k) * 8 + lz] >> ((ly << 3) | x0)) & 0xFu;
+      act = (uint32_t(lb.bits[size_t(k) * 8 + z0] >> ((ly << 3) | x0)) & 0xFu) |
+            ((uint32_t(lb.bits[size_t(k) * 8 + z0 + 1] >> ((ly << 3) | x0)) & 0xFu) << 4);
     }
-    const float4 r = *reinterpret_cast<const float4*>(rhs + i0);
-    RowNbrs R{};
-    if (!ZERO) row_load(src, b, h, ly, lz, BX, BXY, last, R);
-    const float rs[4] = {r.x, r.y, r.z, r.w};
-    float out[4];
+    float r[8], c[8], yp[8], ym[8], zm0[4], zp1[4];
+    float e0 = 0.f, e1 = 0.f;
+    unpack4(*reinterpret_cast<const float4*>(rhs + i0), r);
+    unpack4(*reinterpret_cast<const float4*>(rhs + i1), r + 4);
+    if (!ZERO) {
+      const uint32_t iyp0 = ly < 7u ? i0 + 8u : ((min(b + BX, last) << 9) | (z0 << 6) | x0);
+      const uint32_t iym0 = ly > 0u ? i0 - 8u : (((b >= BX ? b - BX : 0u) << 9) | (z0 << 6) | (7u << 3) | x0);
+      const uint32_t izm0 = z0 > 0u ? i0 - 64u : (((b >= BXY ? b - BXY : 0u) << 9) | (7u << 6) | (ly << 3) | x0);
+      const uint32_t izp1 = z0 + 1u < 7u ? i1 + 64u : ((min(b + BXY, last) << 9) | (ly << 3) | x0);
+      const uint32_t ie0 = h ? ((min(b + 1u, last) << 9) | (z0 << 6) | (ly << 3)) : (((b > 0u ? b - 1u : 0u) << 9) | (z0 << 6) | (ly << 3) | 7u);
+      unpack4(*reinterpret_cast<const float4*>(src + i0), c);
+      unpack4(*reinterpret_cast<const float4*>(src + i1), c + 4);
+      unpack4(*reinterpret_cast<const float4*>(src + iyp0), yp);
+      unpack4(*reinterpret_cast<const float4*>(src + iyp0 + 64u), yp + 4);
+      unpack4(*reinterpret_cast<const float4*>(src + iym0), ym);
+      unpack4(*reinterpret_cast<const float4*>(src + iym0 + 64u), ym + 4);
+      unpack4(*reinterpret_cast<const float4*>(src + izm0), zm0);
+      unpack4(*reinterpret_cast<const float4*>(src + izp1), zp1);
+      e0 = src[ie0];
+      e1 = src[ie0 + 64u];
+    } else {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      out[e] = 0.f;
-      if ((act >> e) & 1u) {
-        const float sum = ZERO ? 0.f : R.xp[e] + R.xm[e] + R.yp[e] + R.ym[e] + R.zp[e] + R.zm[e];
-        const float pj = (sum - rs[e]) / 6.f;
-        out[e] = mixf(ZERO ? 0.f : R.c[e], pj, om);
+      for (int e = 0; e < 8; ++e) c[e] = yp[e] = ym[e] = 0.f;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) zm0[e] = zp1[e] = 0.f;
+    }
+    // the partner lane holds the other half of each row
+    const float o0 = __shfl_xor(h ? c[0] : c[3], 1, 64), o1 = __shfl_xor(h ? c[4] : c[7], 1, 64);
+    float out[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = j & 3, L = j >> 2;
+      const float edge = L ? e1 : e0, other = L ? o1 : o0;
+      const float xpv = e < 3 ? c[j + 1] : (h ? edge : other);
+      const float xmv = e > 0 ? c[j - 1] : (h ? other : edge);
+      const float zpv = L ? zp1[e] : c[4 + e];
+      const float zmv = L ? c[e] : zm0[e];
+      out[j] = 0.f;
+      if ((act >> j) & 1u) {
+        const float sum = ZERO ? 0.f : xpv + xmv + yp[j] + ym[j] + zpv + zmv;
+        const float pj = (sum - r[j]) / 6.f;
+        out[j] = mixf(ZERO ? 0.f : c[j], pj, om);
       }
     }
     *reinterpret_cast<float4*>(dst + i0) = make_float4(out[0], out[1], out[2], out[3]);
+    *reinterpret_cast<float4*>(dst + i1) = make_float4(out[4], out[5], out[6], out[7]);
   }
 }
 
@@ -3031,14 +3071,17 @@ void launch_prolong_add(hipStream_t s, int blocks, const uint32_t* list, const u
 }
 void launch_sweep(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
                   const float* rhs, const float* src, float* dst, bool fine) {
+  // 256-thread workgroups of one wave per brick, 5/3 as many as the 512-thread
+  // persistent grid (five per CU fit beside a raymarch workgroup at 72 VGPRs)
+  const int wg = blocks >= 8 ? ((blocks * 5 / 3) & ~7) : blocks;
   if (src && fine) {
-    FX_LAUNCH((k_sweep<false, true>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
+    FX_LAUNCH((k_sweep<false, true>), wg, 256, s, list, count, g, b, c, rhs, src, dst);
   } else if (src) {
-    FX_LAUNCH((k_sweep<false, false>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
+    FX_LAUNCH((k_sweep<false, false>), wg, 256, s, list, count, g, b, c, rhs, src, dst);
   } else if (fine) {
-    FX_LAUNCH((k_sweep<true, true>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
+    FX_LAUNCH((k_sweep<true, true>), wg, 256, s, list, count, g, b, c, rhs, src, dst);
   } else {
-    FX_LAUNCH((k_sweep<true, false>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
+    FX_LAUNCH((k_sweep<true, false>), wg, 256, s, list, count, g, b, c, rhs, src, dst);
   }
 }
 void launch_sweep_pair(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
