@@ -65,7 +65,7 @@ RAY_SCALAR_ACCESSES = 3
 # benchmark: FETCH_SIZE x 2.0 (a 256 MiB device copy reads 128 MiB on that counter)
 # + WRITE_SIZE x 1.0 (the copy's writes read exactly). Summaries with the commands
 # are in profiles/.
-TRAFFIC_PER_VOXEL = {"jacobi_fine": 13.44, "advect_scalars": 62.5}
+TRAFFIC_PER_VOXEL = {"jacobi_fine": 13.44, "advect_scalars": 63.5}
 PRECISIONS = {"fp32": (abi.FX_PRECISION_FP32, 4), "bf16": (abi.FX_PRECISION_BF16, 2), "fp16": (abi.FX_PRECISION_FP16, 2)}
 
 

@@ -24,6 +24,10 @@ namespace {
 thread_local std::string g_create_error;
 
 constexpr int kStageSlots = 8;
+// The step constants and the candidate list share one staging block (pinned on
+// the host, device-resident on the GPU): the constants at offset 0, the list at
+// kConstBytes, so that a step uploads both with one copy.
+constexpr size_t kConstBytes = (sizeof(StepConst) + 255) & ~size_t(255);
 constexpr size_t kBlockSmoothVoxels = 65536;  // coarsest levels up to this size run in one workgroup
 
 struct Level {
@@ -39,8 +43,8 @@ struct Level {
 };
 
 struct StageSlot {
-  StepConst* c = nullptr;      // pinned
-  uint32_t* cand = nullptr;    // pinned, capacity nb
+  StepConst* c = nullptr;      // pinned staging block (kConstBytes + 4 nb bytes)
+  uint32_t* cand = nullptr;    // inside the block at kConstBytes, capacity nb
   hipEvent_t done = nullptr;
   bool used = false;
 };
@@ -72,8 +76,8 @@ struct fx_sim {
   uint32_t* identity = nullptr;     // 0..nb-1
   uint32_t* identity_count = nullptr;
 
-  StepConst* d_const = nullptr;     // device copy of the step constants
-  uint32_t* d_cand = nullptr;       // device candidate list (injection + host-written bricks)
+  StepConst* d_const = nullptr;     // device copy of the step constants (start of the staging block)
+  uint32_t* d_cand = nullptr;       // device candidate list (injection + host-written bricks), in the same block
   float* d_src = nullptr;
   size_t src_cap = 0;               // floats
   std::vector<float> h_src;         // packed, >= 1 source
@@ -292,11 +296,12 @@ int upload_step(fx_sim* s, const fx_sim_params& p, const std::vector<uint32_t>* 
   c.inj_count = inj_count;
   c.cand_count = cand ? uint32_t(cand->size()) : 0u;
   c.pad[0] = c.pad[1] = 0;
-  HIP_OK(s, hipMemcpyAsync(s->d_const, slot.c, sizeof(StepConst), hipMemcpyHostToDevice, s->stream));
+  size_t bytes = sizeof(StepConst);
   if (cand && !cand->empty()) {
     std::memcpy(slot.cand, cand->data(), cand->size() * 4);
-    HIP_OK(s, hipMemcpyAsync(s->d_cand, slot.cand, cand->size() * 4, hipMemcpyHostToDevice, s->stream));
+    bytes = kConstBytes + cand->size() * 4;
   }
+  HIP_OK(s, hipMemcpyAsync(s->d_const, slot.c, bytes, hipMemcpyHostToDevice, s->stream));
   HIP_OK(s, hipEventRecord(slot.done, s->stream));
   slot.used = true;
   return FX_OK;
@@ -894,8 +899,12 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
   if (rc == FX_OK) rc = dalloc(s, &s->touched, s->nb);
   if (rc == FX_OK) rc = dalloc(s, &s->identity, s->nb);
   if (rc == FX_OK) rc = dalloc(s, &s->identity_count, 1);
-  if (rc == FX_OK) rc = dalloc(s, &s->d_const, 1);
-  if (rc == FX_OK) rc = dalloc(s, &s->d_cand, s->nb);
+  uint8_t* stage = nullptr;
+  if (rc == FX_OK) rc = dalloc(s, &stage, kConstBytes + s->nb * 4);
+  if (rc == FX_OK) {
+    s->d_const = reinterpret_cast<StepConst*>(stage);
+    s->d_cand = reinterpret_cast<uint32_t*>(stage + kConstBytes);
+  }
   if (rc == FX_OK) rc = dalloc(s, &s->d_bits, (s->nb + 31) / 32);
   if (rc != FX_OK) return bail(rc);
   if (s->sparse) s->flagsRender = s->flagsInj;
@@ -968,10 +977,10 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
   s->src_cap = FX_FLOATS_PER_EMITTER;
   if (dalloc(s, &s->d_src, s->src_cap) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
   for (auto& slot : s->slots) {
-    if (hipHostMalloc(reinterpret_cast<void**>(&slot.c), sizeof(StepConst), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void**>(&slot.cand), std::max<size_t>(s->nb, 1) * 4, hipHostMallocDefault) != hipSuccess ||
+    if (hipHostMalloc(reinterpret_cast<void**>(&slot.c), kConstBytes + s->nb * 4, hipHostMallocDefault) != hipSuccess ||
         hipEventCreateWithFlags(&slot.done, hipEventDisableTiming) != hipSuccess)
       return bail(fail(s, FX_ERR_OUT_OF_MEMORY, "pinned staging allocation failed"));
+    slot.cand = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(slot.c) + kConstBytes);
   }
 
   // Identity lists (dense) and zeroed state.
@@ -1012,7 +1021,7 @@ void fx_destroy(fx_sim* s) {
   F(s->wmag); F(s->cmag); F(s->umag);
   F(s->flagsInj);
   if (!s->sparse) F(s->flagsRender);
-  F(s->touched); F(s->d_bits); F(s->d_rqueue); F(s->d_render); F(s->d_sq); F(s->d_uq); F(s->d_bloom); F(s->d_bloom_io); F(s->identity); F(s->identity_count); F(s->d_const); F(s->d_cand); F(s->d_src); F(s->d_norms);
+  F(s->touched); F(s->d_bits); F(s->d_rqueue); F(s->d_render); F(s->d_sq); F(s->d_uq); F(s->d_bloom); F(s->d_bloom_io); F(s->identity); F(s->identity_count); F(s->d_const); F(s->d_src); F(s->d_norms);
   for (size_t l = 0; l < s->lv.size(); ++l) {
     Level& L = s->lv[l];
     F(L.div); F(L.p[0]); F(L.p[1]); F(L.bits); F(L.tab); F(L.tab_n); F(L.L.list); F(L.L.count); F(L.L.page); F(L.L.retire); F(L.L.retire_count);
@@ -1020,7 +1029,6 @@ void fx_destroy(fx_sim* s) {
   }
   for (auto& slot : s->slots) {
     if (slot.c) (void)hipHostFree(slot.c);
-    if (slot.cand) (void)hipHostFree(slot.cand);
     if (slot.done) (void)hipEventDestroy(slot.done);
   }
   if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
