@@ -43,6 +43,29 @@ __device__ __forceinline__ float smoothf(float e0, float e1, float x) {
   float t = clampf((x - e0) / (e1 - e0), 0.f, 1.f);
   return t * t * (3.f - 2.f * t);
 }
+// n / d without the IEEE division sequence (~10 instructions): q0 = n * r with
+// r = RN(1 / d), then one Markstein correction with the exact remainder
+// fma(-q0, d, n), 3 instructions. Only used where the result is the correctly
+// rounded quotient for every input that reaches it: smoothk's numerators in
+// [0, d] for each literal edge pair, and (y + 0.5) / H for H <= 32768, both
+// checked exhaustively against the IEEE quotient on the CPU.
+__device__ __forceinline__ float div_mk(float n, float d, float r) {
+  const float q0 = n * r;
+  return fmaf(fmaf(-q0, d, n), r, q0);
+}
+// smoothf for literal edges: clamping x - e0 to [0, d] before dividing gives the
+// same t as clamping the quotient to [0, 1] (NaN clamps to 0 either way).
+__device__ __forceinline__ float smoothk(float e0, float e1, float x) {
+  const float d = e1 - e0;
+  const float t = div_mk(clampf(x - e0, 0.f, d), d, 1.f / d);
+  return t * t * (3.f - 2.f * t);
+}
+// (y + 0.5) / H, the normalised height of a voxel.
+__device__ __forceinline__ float height01(uint32_t y, uint32_t H) {
+  const float n = float(y) + 0.5f, d = float(H);
+  if (__builtin_expect(H <= 32768u, 1)) return div_mk(n, d, 1.f / d);
+  return n / d;
+}
 __device__ __forceinline__ float stepf(float edge, float x) { return edge <= x ? 1.f : 0.f; }
 __device__ __forceinline__ float fractf(float x) { return x - floorf(x); }
 __device__ __forceinline__ uint32_t decu(uint32_t v) { return v == 0u ? 0u : v - 1u; }
@@ -355,7 +378,7 @@ __device__ __noinline__ float eruptive_falloff(const fx_sim_params& P, F3 pos, f
   float coarse = emitter_noise(pos + f3(age * 0.17f, -age * 0.11f, age * 0.07f), 6.f, seed);
   float fine = emitter_noise(pos + f3(-age * 0.33f, age * 0.21f, -age * 0.15f), 18.f, seed + 41.f);
   float lobe = clampf(coarse * 0.72f + fine * 0.28f, 0.f, 1.f);
-  float reach = targetRadius * growth * mixf(0.62f, 1.f, smoothf(0.2f, 0.92f, lobe));
+  float reach = targetRadius * growth * mixf(0.62f, 1.f, smoothk(0.2f, 0.92f, lobe));
   float front = 1.f - smoothf(reach - feather, reach + feather, dist);
   float coreRadius = fmaxf(targetRadius * (0.18f + growth * 0.38f), feather);
   float core = sphere_falloff(dist, coreRadius, falloff) * (1.f - growth * 0.35f);
@@ -458,7 +481,7 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
         if (dist > (burst ? outer_limit(P, radius) : radius)) continue;
         float weight = burst ? eruptive_falloff(P, pos, dist, s, 0.6f) : sphere_falloff(dist, radius, 0.6f);
         float intensity = s.intensity();
-        float avail = burst ? 1.f : smoothf(0.02f, 0.22f, fuel);
+        float avail = burst ? 1.f : smoothk(0.02f, 0.22f, fuel);
         float release = burst ? 1.f / source_duration(s) : 1.6f;
         float seedv = intensity * weight * avail * P.delta_time * release;
         rea = clamp01(rea + seedv);
@@ -767,7 +790,9 @@ __device__ __forceinline__ float thermal_noise(F3 pos, float time) {
   float b = 0.5f + 0.5f * sin_s(dot3(p, f3(-0.043f, 0.137f, 0.071f)) + sin_s(p.y * 0.079f) * 1.6f);
   return clampf(a * 0.58f + b * 0.42f, 0.f, 1.f);
 }
-// SH/advect-scalar-maccormack.wgsl.ts:75-100
+// SH/advect-scalar-maccormack.wgsl.ts:75-100. Keeps smoothf's IEEE division: with
+// smoothk the scalar advection, which sits at 128 VGPRs and waits on its gathers,
+// measured 2-4 us slower.
 template <class ST>
 __device__ __forceinline__ void decay_write(const Grid& g, uint32_t i, uint32_t y, float den, float tem, float fu, float rea,
                                             float tn, float dt, SS<ST> D) {
@@ -886,23 +911,23 @@ __global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__
       uint32_t k = cidx(g, a, b2, c2);
       return clampf(ST::ld(S.t, k) * 1.55f + ST::ld(S.f, k) * 0.26f + S.r[k] * 0.44f, 0.f, 1.f);
     };
-    float ny = (float(y) + 0.5f) / float(g.H);
+    float ny = height01(y, g.H);
     float cluster = thermal_cluster(f3(float(x), float(y), float(z)), time);
     float nh = (hotgas(incu(x, g.W - 1), y, z) + hotgas(decu(x), y, z) + hotgas(x, incu(y, g.H - 1), z) + hotgas(x, decu(y), z) +
                 hotgas(x, y, incu(z, g.D - 1)) + hotgas(x, y, decu(z))) * 0.16666667f;
     float localBefore = hotgas(x, y, z);
     float den = ST::ld(S.d, i), tem = ST::ld(S.t, i), fu = ST::ld(S.f, i), rea = S.r[i];
-    float radiative = fmaxf(nh - localBefore * 0.55f, 0.f) * smoothf(0.04f, 0.76f, den) * (0.45f + (1.f - cluster) * 0.55f) * dt * 2.6f;
-    float hotPocket = smoothf(0.58f, 0.92f, cluster + rea * 0.22f);
-    float coolingPocket = smoothf(0.42f, 0.82f, 1.f - cluster) * smoothf(0.08f, 0.68f, den) * (1.f - smoothf(0.54f, 0.94f, rea));
+    float radiative = fmaxf(nh - localBefore * 0.55f, 0.f) * smoothk(0.04f, 0.76f, den) * (0.45f + (1.f - cluster) * 0.55f) * dt * 2.6f;
+    float hotPocket = smoothk(0.58f, 0.92f, cluster + rea * 0.22f);
+    float coolingPocket = smoothk(0.42f, 0.82f, 1.f - cluster) * smoothk(0.08f, 0.68f, den) * (1.f - smoothk(0.54f, 0.94f, rea));
     float ignition = smoothf(ilo, ihi, tem);
     float burn = fminf(fu, fu * ignition * dt * mixf(4.2f, 8.6f, hotPocket) * rate);
-    float hotSoot = smoothf(0.24f, 0.88f, tem + rea * 0.2f) * smoothf(0.36f, 0.9f, 1.f - cluster + den * 0.18f);
+    float hotSoot = smoothk(0.24f, 0.88f, tem + rea * 0.2f) * smoothk(0.36f, 0.9f, 1.f - cluster + den * 0.18f);
     float smokeYield = burn * mixf(0.16f, 0.58f, coolingPocket + (1.f - ignition) * 0.5f + hotSoot * 0.45f);
     float hotCooling = mixf(1.3f, 0.62f, hotPocket);
     float heatLoss = dt * (0.018f + ny * 0.052f) * hotCooling * mixf(1.f, 2.25f, coolingPocket);
     float densityLoss = dt * den * (0.02f + ny * 0.07f);
-    float clearing = den * smoothf(0.18f, 0.82f, tem + rea * 0.22f + fu * 0.1f) * dt * mixf(0.55f, 2.45f, hotPocket);
+    float clearing = den * smoothk(0.18f, 0.82f, tem + rea * 0.22f + fu * 0.1f) * dt * mixf(0.55f, 2.45f, hotPocket);
     float coolingSmoke = den * tem * (coolingPocket + hotSoot * 0.32f) * dt * 0.16f;
     float reactionDecay = rea * (1.f - dt * mixf(0.68f, 0.22f, hotPocket));
     ST::st(D.f, i, clamp01(fu - burn));
@@ -935,8 +960,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     float lift = hotGas * response;
     F3 gdir = f3(P.gravity[0], P.gravity[1], P.gravity[2]);
     F3 gravityForce = gdir * den * P.gravity[3];
-    float entrain = den * smoothf(0.04f, 0.3f, hotGas);
-    float ny = (float(y) + 0.5f) / float(g.H);
+    float entrain = den * smoothk(0.04f, 0.3f, hotGas);
+    float ny = height01(y, g.H);
     float shear = sin_s(float(y) * 0.073f + P.time * 0.9f) * 0.35f + sin_s(float(x) * 0.041f - float(z) * 0.052f + P.time * 0.43f) * 0.22f;
     float windAmount = P.wind[3] * (0.18f + den * 0.55f + ny * 0.45f);
     F3 wdir = f3(P.wind[0], P.wind[1], P.wind[2]);
@@ -946,12 +971,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     F3 dg = f3(rd(incu(x, g.W - 1), y, z) - rd(decu(x), y, z), rd(x, incu(y, g.H - 1), z) - rd(x, decu(y), z),
                rd(x, y, incu(z, g.D - 1)) - rd(x, y, decu(z))) * 0.5f;
     float dgl = fmaxf(len3(dg), 0.0001f);
-    float coldSmoke = den * (1.f - smoothf(0.12f, 0.72f, hotGas));
-    float surfaceBreakup = smoothf(0.008f, 0.06f, dgl) * coldSmoke;
+    float coldSmoke = den * (1.f - smoothk(0.12f, 0.72f, hotGas));
+    float surfaceBreakup = smoothk(0.008f, 0.06f, dgl) * coldSmoke;
     F3 shred = norm3(f3(sin_s(float(y) * 0.117f + P.time * 1.9f), sin_s(float(x) * 0.071f - float(z) * 0.083f + P.time * 1.35f) * 0.35f,
                         cos_s(float(x) * 0.061f + float(y) * 0.049f - P.time * 1.6f)));
-    float collapse = den * (1.f - smoothf(0.16f, 0.64f, hotGas));
-    float smokeDrag = den * smoothf(0.03f, 0.45f, hotGas);
+    float collapse = den * (1.f - smoothk(0.16f, 0.64f, hotGas));
+    float smokeDrag = den * smoothk(0.03f, 0.45f, hotGas);
     F3 vel = f3(us.x[i], us.y[i], us.z[i]) * (1.f - dt * 0.08f);
     vel = vel + wind * dt;
     vel = vel + gravityForce * dt;
@@ -1012,8 +1037,8 @@ __global__ __launch_bounds__(512) void k_confinement(const uint32_t* __restrict_
                  m(x, y, incu(z, g.D - 1)) - m(x, y, decu(z))) * 0.5f;
     F3 dir = grad / fmaxf(len3(grad), 0.0001f);
     float hotGas = clampf(ST::ld(S.t, i) + S.r[i] * 0.85f, 0.f, 1.f);
-    float mask = clampf(P.vorticity_mask[0] + smoothf(0.02f, 0.38f, wmag[i]) * P.vorticity_mask[1] +
-                            smoothf(0.04f, 0.7f, hotGas) * P.vorticity_mask[2] + smoothf(0.04f, 0.65f, ST::ld(S.d, i)) * P.vorticity_mask[3],
+    float mask = clampf(P.vorticity_mask[0] + smoothk(0.02f, 0.38f, wmag[i]) * P.vorticity_mask[1] +
+                            smoothk(0.04f, 0.7f, hotGas) * P.vorticity_mask[2] + smoothk(0.04f, 0.65f, ST::ld(S.d, i)) * P.vorticity_mask[3],
                         0.f, 4.f);
     F3 force = cross3(dir, f3(w.x[i], w.y[i], w.z[i])) * P.vorticity_strength * mask * 1.45f;
     F3 vel = f3(us.x[i], us.y[i], us.z[i]) + force * P.delta_time;
@@ -2016,12 +2041,12 @@ __global__ __launch_bounds__(512) void k_debug(const uint32_t* __restrict__ list
 // pixel, row 0 = top; discarded pixels are written as 0 (cleared target).
 __device__ __forceinline__ F3 fire_palette(float t) {
   F3 ember = f3(0.18f, 0.018f, 0.f), red = f3(1.f, 0.075f, 0.012f), orange = f3(1.f, 0.42f, 0.035f), white = f3(1.f, 0.86f, 0.56f);
-  float rm = smoothf(0.01f, 0.11f, t), om = smoothf(0.09f, 0.38f, t), wm = smoothf(0.58f, 0.92f, t);
+  float rm = smoothk(0.01f, 0.11f, t), om = smoothk(0.09f, 0.38f, t), wm = smoothk(0.58f, 0.92f, t);
   return mix3(mix3(mix3(ember, red, rm), orange, om), white, wm);
 }
 __device__ __forceinline__ F3 smoke_albedo(float den, float tem, float hot) {
   float warm = clampf(tem * 0.38f, 0.f, 1.f);
-  float soot = smoothf(0.16f, 0.9f, den) * (1.f - smoothf(0.2f, 0.74f, hot));
+  float soot = smoothk(0.16f, 0.9f, den) * (1.f - smoothk(0.2f, 0.74f, hot));
   return mix3(mix3(f3(0.26f, 0.275f, 0.295f), f3(0.04f, 0.041f, 0.045f), soot * 0.86f), f3(0.36f, 0.27f, 0.18f), warm * (0.35f + hot * 0.45f));
 }
 // Shading transcendentals on the hardware base-2 units (v_exp_f32 / v_log_f32,
@@ -2117,6 +2142,10 @@ __device__ __forceinline__ void march_pixel(const Grid& g, const RayConst& R, co
   F3 nrd = norm3(vrd);
   const float vrdLen = len3(vrd);
   const uint32_t mxu = uint32_t(dims.x), myu = uint32_t(dims.y), mzu = uint32_t(dims.z);
+  // per-ray reciprocals: the samples multiply where the shader divides (within
+  // an ulp or two; the image is FP16)
+  const F3 invDims = f3(1.f / dims.x, 1.f / dims.y, 1.f / dims.z);
+  const F3 invVrd = f3(1.f / vrd.x, 1.f / vrd.y, 1.f / vrd.z);
   for (uint32_t k = 0; k < stepCount && travelled <= rayLen; ++k) {
     if (alpha > 0.985f) break;
     float vstep = voxDelta;
@@ -2148,12 +2177,12 @@ __device__ __forceinline__ void march_pixel(const Grid& g, const RayConst& R, co
                      RB[vidx(g, cx, min(cy + 1, myu), cz)].x - RB[vidx(g, cx, cy == 0 ? 0 : cy - 1, cz)].x,
                      RB[vidx(g, cx, cy, min(cz + 1, mzu))].x - RB[vidx(g, cx, cy, cz == 0 ? 0 : cz - 1)].x);
           float gl = fmaxf(len3(dg), 0.0001f);
-          F3 normal = gl > 0.00011f ? (-dg) / gl : f3(0.f, 1.f, 0.f);
+          F3 normal = gl > 0.00011f ? (-dg) * (1.f / gl) : f3(0.f, 1.f, 0.f);
           float detail = clampf(gl * 5.2f + hot * 0.28f + rea * 0.18f, 0.f, 1.f);
           float dd = fmaxf(pow_hw(den, 0.96f) * mixf(0.82f, 1.42f, detail), 0.f);
-          F3 uvw = f3(sp.x / dims.x, sp.y / dims.y, sp.z / dims.z);
+          F3 uvw = mul3(sp, invDims);
           F3 wp = f3(mixf(bmin.x, bmax.x, uvw.x), mixf(bmin.y, bmax.y, uvw.y), mixf(bmin.z, bmax.z, uvw.z));
-          float topFade = 1.f - smoothf(0.9f, 0.995f, uvw.y);
+          float topFade = 1.f - smoothk(0.9f, 0.995f, uvw.y);
           F3 total = f3(0, 0, 0);
           float primT = 1.f;
           if (nl > 0) {
@@ -2186,7 +2215,7 @@ __device__ __forceinline__ void march_pixel(const Grid& g, const RayConst& R, co
             }
             F3 color = f3(R.light_color[li][0], R.light_color[li][1], R.light_color[li][2]);
             float nf = clampf(dot3(normal, dl) * 0.5f + 0.5f, 0.f, 1.f);
-            float surface = pow_hw(nf, 1.35f) * mixf(1.f, 0.56f, smoothf(0.16f, 0.9f, dd));
+            float surface = pow_hw(nf, 1.35f) * mixf(1.f, 0.56f, smoothk(0.16f, 0.9f, dd));
             float rim = rimView * rimLight;
             float directional = (surface * 0.42f + rim * 0.22f) * power;
             float scatter = (fs + bs) * mixf(0.36f, 0.74f, hot);
@@ -2195,11 +2224,11 @@ __device__ __forceinline__ void march_pixel(const Grid& g, const RayConst& R, co
           }
           F3 tl = f3(1.f - exp_hw(-total.x * 0.72f), 1.f - exp_hw(-total.y * 0.72f), 1.f - exp_hw(-total.z * 0.72f));
           F3 smoke = smoke_albedo(dd, tem, hot);
-          float absorb = mixf(0.92f, 0.34f, smoothf(0.26f, 1.05f, dd));
+          float absorb = mixf(0.92f, 0.34f, smoothk(0.26f, 1.05f, dd));
           float heat = clampf(tem * 1.25f + fu * 0.18f + rea * 0.34f, 0.f, 1.f);
           float flame = clampf(tem * 0.9f + fu * 0.12f + rea * 0.18f, 0.f, 1.f);
-          F3 heatScatter = fire_palette(flame) * heat * heat * smoothf(0.012f, 0.38f, dd) * 0.18f;
-          F3 lit = mul3(smoke, f3(0.045f, 0.048f, 0.052f)) + mul3(smoke, tl) * absorb + tl * smoothf(0.018f, 0.22f, dd) * 0.18f + heatScatter;
+          F3 heatScatter = fire_palette(flame) * heat * heat * smoothk(0.012f, 0.38f, dd) * 0.18f;
+          F3 lit = mul3(smoke, f3(0.045f, 0.048f, 0.052f)) + mul3(smoke, tl) * absorb + tl * smoothk(0.018f, 0.22f, dd) * 0.18f + heatScatter;
           float flameHeat = clampf(tem * 0.9f + fu * 0.14f + rea * 0.16f, 0.f, 1.f);
           F3 emissive = fire_palette(flameHeat) * hot * (0.48f + fu * 0.16f + rea * 0.24f);
           float contribution = clampf(dd * 1.35f + hot * 1.25f, 0.f, 1.f);
@@ -2225,9 +2254,9 @@ __device__ __forceinline__ void march_pixel(const Grid& g, const RayConst& R, co
       F3 bc = f3(floorf(sp.x / 8.f), floorf(sp.y / 8.f), floorf(sp.z / 8.f));
       F3 bnd = f3(vrd.x > 0.f ? (bc.x + 1.f) * 8.f : bc.x * 8.f, vrd.y > 0.f ? (bc.y + 1.f) * 8.f : bc.y * 8.f,
                   vrd.z > 0.f ? (bc.z + 1.f) * 8.f : bc.z * 8.f);
-      float tx = fabsf(vrd.x) > 0.00001f ? (bnd.x - sp.x) / vrd.x : 1e9f;
-      float ty = fabsf(vrd.y) > 0.00001f ? (bnd.y - sp.y) / vrd.y : 1e9f;
-      float tz = fabsf(vrd.z) > 0.00001f ? (bnd.z - sp.z) / vrd.z : 1e9f;
+      float tx = fabsf(vrd.x) > 0.00001f ? (bnd.x - sp.x) * invVrd.x : 1e9f;
+      float ty = fabsf(vrd.y) > 0.00001f ? (bnd.y - sp.y) * invVrd.y : 1e9f;
+      float tz = fabsf(vrd.z) > 0.00001f ? (bnd.z - sp.z) * invVrd.z : 1e9f;
       vstep = fmaxf(fminf(tx, fminf(ty, tz)) * vrdLen + 0.5f, voxDelta);
     }
     vp = vp + nrd * vstep;
