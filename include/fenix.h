@@ -270,7 +270,7 @@ int fx_synchronize(fx_sim* sim);
 
 /* ---- batches (no reference counterpart) ---------------------------------- */
 
-/* A batch of independent scenes (BASELINE C3: many scenes of different sizes).
+/* A batch of independent scenes (many scenes of different sizes and active sets).
  * Every scene is a full fx_sim with its own simulation and render streams and its
  * own captured step graphs, so one batch step runs the scenes concurrently on the
  * GPU: the latency-bound launches of one scene fill the CUs another leaves idle. */

@@ -1,7 +1,7 @@
 """Numpy restatement of the reference's bloom and composite passes
 (src/engine/render/passes/createBloomPass.ts).
 
-Test infrastructure only: the checker of the HIP bloom kernels. Each render pass
+Test infrastructure only: the oracle of the HIP bloom kernels. Each render pass
 draws a full-screen triangle whose fragment at target pixel (i, j) has
 uv = ((i + 0.5) / W, (j + 0.5) / H) (fullscreenVert :193-204, row 0 at the top),
 and reads its sources through a linear, clamp-to-edge sampler (:50). The

@@ -1,7 +1,7 @@
 """FP64 numpy restatement of the differentiable smoke step (fx_diff_* in
 include/fenix.h) and of its adjoint.
 
-Test infrastructure only: this is the checker of the HIP forward and adjoint
+Test infrastructure only: this is the oracle of the HIP forward and adjoint
 kernels, never a path the library runs. The reference has no gradients; the
 forward stages restate the library's own model, whose advection and
 confinement follow the reference's formulas:

@@ -1,6 +1,6 @@
 // oracle.cpp — CPU restatement of fenix's per-step combustion-volume pipeline.
 //
-// TEST INFRASTRUCTURE ONLY. This file is the checker for libfenix: only tests,
+// TEST INFRASTRUCTURE ONLY. This file is the oracle for libfenix: only tests,
 // the smoke check and the CPU-baseline leg of the benchmark load it. The product
 // (fenix_amd) never links or calls it.
 //

@@ -1,6 +1,6 @@
 """ctypes loader of the CPU oracle (oracle/build/liboracle.so).
 
-Test infrastructure only: the oracle is the checker, never the thing measured.
+Test infrastructure only: the oracle is what results are compared against, never the thing measured.
 """
 import ctypes as C
 import os

@@ -99,7 +99,7 @@ def test_null_handle_calls_return_errors():
 
 @pytest.mark.parametrize("path", ["oracle", "product"])
 def test_product_does_not_reference_the_oracle(path):
-    """The shipped library and host never link or import the checker."""
+    """The shipped library and host never link or import the oracle."""
     if path == "product":
         for name in os.listdir(os.path.join(ROOT, "fenix_amd")):
             if name.endswith(".py"):

@@ -12,7 +12,7 @@ that the oracle computes, for every float input that can reach it:
 - height01(y, H) = (y + 0.5) / H: every y < H for every H <= 32768 (larger heights
   take the IEEE division).
 
-The checker is C compiled with -ffp-contract=off, so products and sums round
+The check program is C compiled with -ffp-contract=off, so products and sums round
 exactly as the GPU code (built with the same flag) rounds them, and fmaf is a
 fused multiply-add on both.
 """
@@ -120,7 +120,7 @@ def test_height_division_exact(checker):
 
 
 def test_checker_catches_a_wrong_reciprocal(tmp_path):
-    """The checker must notice a planted error: q0 moved up by one ulp in place of
+    """The check program must notice a planted error: q0 moved up by one ulp in place of
     the correction step differs from the IEEE quotient, and from its smoothstep
     value, for many inputs."""
     if shutil.which("gcc") is None:

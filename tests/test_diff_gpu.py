@@ -30,7 +30,8 @@ def rel_l2(a, b):
 
 
 def _report(name, payload):
-    out = os.path.join(ROOT, "gpurun_out")
+    # reports go to FENIX_REPORT_DIR (default: reports/ at the repository root)
+    out = os.environ.get("FENIX_REPORT_DIR", os.path.join(ROOT, "reports"))
     os.makedirs(out, exist_ok=True)
     with open(os.path.join(out, name), "w") as fh:
         json.dump(payload, fh, indent=1)

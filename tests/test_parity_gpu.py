@@ -1,7 +1,7 @@
 """GPU parity: libfenix (HIP, gfx950) against the FP32 oracle on identical inputs.
 
 Bars: brick flags, active list and page table bit-exact; FP32 fields within
-1e-5 relative L2 per pass and per step (north-star tolerance).
+1e-5 relative L2 per pass and per step.
 """
 import numpy as np
 import pytest

@@ -116,7 +116,8 @@ def test_precision_report_against_fp64():
         report["fields"][label] = row
         report.setdefault("active_bricks", {})[label] = sim.active_brick_count()
         sim.dispose()
-    out = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
+    # reports go to FENIX_REPORT_DIR (default: reports/ at the repository root)
+    out = os.environ.get("FENIX_REPORT_DIR", os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "reports"))
     os.makedirs(out, exist_ok=True)
     with open(os.path.join(out, "precision_report.json"), "w") as fh:
         json.dump(report, fh, indent=1)
