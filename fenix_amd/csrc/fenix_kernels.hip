@@ -1405,8 +1405,15 @@ __device__ __forceinline__ uint64_t nbhd_word(const Grid& g, const LvBits& lb, i
   }
   return v;
 }
-__device__ __forceinline__ void nbhd_load(const Grid& g, const LvBits& lb, int bx, int by, int bz, uint64_t* nb) {
-  for (int q = threadIdx.x; q < 216; q += blockDim.x) nb[q] = nbhd_word(g, lb, bx, by, bz, q);
+// The neighbourhood words in two phases, like the tiles (blockDim >= 216):
+// nbhd_fetch issues the page and bit loads of word threadIdx.x into a register
+// before the barrier that frees the LDS array, so that they are in flight
+// together with the tile loads; nbhd_store writes the word after it.
+__device__ __forceinline__ uint64_t nbhd_fetch(const Grid& g, const LvBits& lb, int bx, int by, int bz) {
+  return threadIdx.x < 216u ? nbhd_word(g, lb, bx, by, bz, int(threadIdx.x)) : 0ull;
+}
+__device__ __forceinline__ void nbhd_store(uint64_t w, uint64_t* nb) {
+  if (threadIdx.x < 216u) nb[threadIdx.x] = w;
 }
 
 // ------------------------------------------------ fused V-cycle kernels --
@@ -1509,8 +1516,9 @@ __global__ __launch_bounds__(512) void k_vdown(const uint32_t* __restrict__ list
     TileRegs<R0> rr, rp;
     tile_fetch<R0>(f, rhs, ox, oy, oz, rr);
     if (p_in) tile_fetch<R0>(f, p_in, ox, oy, oz, rp);
+    const uint64_t nw = nbhd_fetch(f, bf, bx, by, bz);
     __syncthreads();  // previous brick's tiles are no longer read
-    nbhd_load(f, bf, bx, by, bz, nb);
+    nbhd_store(nw, nb);
     __syncthreads();
     tile_store<R0>(nb, rr, tr);
     if (p_in) tile_store<R0>(nb, rp, ta); else tile_zero<R0>(ta);
@@ -1619,8 +1627,9 @@ __global__ __launch_bounds__(512) void k_vup(const uint32_t* __restrict__ list, 
     TileRegs<R0> rp, rr;
     tile_fetch<R0>(f, p_in, ox, oy, oz, rp);
     if (POST > 0) tile_fetch<R0>(f, rhs, ox, oy, oz, rr);
+    const uint64_t nw = nbhd_fetch(f, bf, bx, by, bz);
     __syncthreads();
-    nbhd_load(f, bf, bx, by, bz, nb);
+    nbhd_store(nw, nb);
 #pragma unroll
     for (int j = 0; j < CIT; ++j) {
       const int q = int(threadIdx.x) + j * 512;
