@@ -8,7 +8,7 @@ import pytest
 
 import scenes
 from fenix_amd import abi
-from fenix_amd.render import VolumeRaymarchPass, camera_struct, orbit_camera_snapshot
+from fenix_amd.render import RenderLight, VolumeRaymarchPass, camera_struct, orbit_camera_snapshot
 from fenix_amd.simulation import CombustionVolumeBatch, CombustionVolumeSimulation, make_desc
 from oracle import Oracle
 
@@ -272,8 +272,8 @@ def test_autotune_keeps_results():
         np.testing.assert_array_equal(outs[1][name], outs[0][name], err_msg=name)
 
 
-def test_raymarch_parity():
-    n = 32
+def _raymarch_scene(n=32):
+    """A C1 scene after 31 oracle steps, written into the simulation (flags checked)."""
     desc = scenes.c1_desc(n)
     sim, orc = make_pair(desc, scenes.c1_sources())
     for k in range(30):
@@ -285,16 +285,50 @@ def test_raymarch_parity():
     for name in PERSISTENT:
         sim.write_field(name, orc.read(name))
     np.testing.assert_array_equal(sim.read_flags(), orc.flags())
-    rp = VolumeRaymarchPass((n, n, n))
-    rp.setRenderParams(stepCount=64)
+    return sim, orc
+
+
+def _check_images(sim, orc, rp, modes=("temperature", "density", "fuel")):
     cam = orbit_camera_snapshot()
-    for mode in ("temperature", "density", "fuel"):
+    for mode in modes:
         img = rp.render(sim, cam, mode, 64, 64).astype(np.float32)
         ref = orc.raymarch(camera_struct(cam), rp.params_struct(mode), 64, 64).astype(np.float16).astype(np.float32)
         assert ref[..., 3].max() > 0.05, "scene renders nothing"
         diff = np.abs(img - ref)
         assert rel_l2(img, ref) < 2e-3, f"{mode}: image rel L2 {rel_l2(img, ref):.3e}"
         assert np.mean(diff.max(axis=-1) < 2e-3) > 0.99
+
+
+def test_raymarch_parity():
+    n = 32
+    sim, orc = _raymarch_scene(n)
+    rp = VolumeRaymarchPass((n, n, n))
+    rp.setRenderParams(stepCount=64)
+    _check_images(sim, orc, rp)
+
+
+def test_raymarch_parity_point_and_several_lights():
+    """Point lights (direction to the light and distance attenuation per sample,
+    RP:503-520) and lights 1 and 2 (evaluated per sample, not per ray), which the
+    default single directional light never reaches."""
+    n = 32
+    sim, orc = _raymarch_scene(n)
+    rp = VolumeRaymarchPass((n, n, n))
+    lights = [RenderLight("point", position=(5.0, 20.0, 5.0), color=(1.0, 0.8, 0.6), intensity=4.0),
+              RenderLight("directional", direction=(0.2, -1.0, 0.3), color=(0.6, 0.7, 1.0), intensity=1.5),
+              RenderLight("point", position=(-12.0, 8.0, 14.0), color=(0.5, 1.0, 0.7), intensity=3.0)]
+    cam = orbit_camera_snapshot()
+    rp.setRenderParams(stepCount=64)
+    default_img = rp.render(sim, cam, "temperature", 64, 64).astype(np.float32)
+    rp.setRenderParams(lights=lights)
+    _check_images(sim, orc, rp)
+    lit = rp.render(sim, cam, "temperature", 64, 64).astype(np.float32)
+    assert rel_l2(lit, default_img) > 2e-2, "the lights do not change the image"
+    # a point light as light 0 (its terms are per sample too) and no light at all
+    rp.setRenderParams(lights=[RenderLight("point", position=(0.0, 30.0, -6.0), intensity=5.0)])
+    _check_images(sim, orc, rp, modes=("temperature",))
+    rp.setRenderParams(lights=[])
+    _check_images(sim, orc, rp, modes=("temperature",))
 
 
 def test_render_overlap_matches_serial():
