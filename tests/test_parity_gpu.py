@@ -329,6 +329,9 @@ def test_raymarch_parity_point_and_several_lights():
     _check_images(sim, orc, rp, modes=("temperature",))
     rp.setRenderParams(lights=[])
     _check_images(sim, orc, rp, modes=("temperature",))
+    # non-default Henyey-Greenstein asymmetries for the forward and back lobes
+    rp.setRenderParams(lights=lights, scatteringForward=0.7, scatteringBack=-0.5)
+    _check_images(sim, orc, rp, modes=("temperature",))
 
 
 def test_render_overlap_matches_serial():
