@@ -180,6 +180,7 @@ FUNCTIONS = {
     "fx_reset": (C.c_int, [P]),
     "fx_step": (C.c_int, [P, C.c_double, C.c_double]),
     "fx_synchronize": (C.c_int, [P]),
+    "fx_request_debug_fields": (C.c_int, [P]),
     "fx_get_runtime_params": (C.c_int, [P, C.POINTER(RuntimeParams)]),
     "fx_set_runtime_params": (C.c_int, [P, C.POINTER(RuntimeParams), u32]),
     "fx_get_quality": (C.c_int, [P, C.POINTER(Quality)]),

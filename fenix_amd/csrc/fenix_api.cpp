@@ -137,6 +137,7 @@ struct fx_sim {
   void* d_bloom_io = nullptr;
   size_t bloom_io_bytes = 0;
   bool in_step = false;          // recording the step's pass sequence
+  bool debug_requested = false;  // getDebugBuffers(): the next step runs the debug pass (ORCH:298-301, 316)
   uint32_t* d_rqueue = nullptr;  // raymarch tile counter
   int ray_blocks = 1;            // resident raymarch workgroups
   bool render_overlap = true;    // false: the raymarch runs on the simulation stream
@@ -1091,7 +1092,8 @@ int fx_step(fx_sim* s, double elapsed_s, double step_s) {  // ORCH:230-305
 
   const bool vort = s->stepIndex % uint64_t(s->q.vorticity_interval) == 0;
   const bool pres = s->stepIndex % uint64_t(s->q.pressure_interval) == 0;
-  const bool dbg = debug_each_frame(s);
+  const bool dbg = debug_each_frame(s) || s->debug_requested;
+  s->debug_requested = false;
   if (s->use_graph && !s->timing) {
     std::vector<int64_t> key = {vort, pres, dbg, s->mode, pres ? -1 : s->lastPressure, s->q.fine_pre_iterations,
                                 s->q.fine_post_iterations, s->q.mid_pre_iterations, s->q.mid_post_iterations,
@@ -1147,6 +1149,12 @@ int fx_run_pass(fx_sim* s, int32_t pass, const fx_sim_params* params) {
     default: return fail(s, FX_ERR_INVALID_ARG, "unknown pass id");
   }
   HIP_OK(s, hipGetLastError());
+  return FX_OK;
+}
+
+int fx_request_debug_fields(fx_sim* s) {  // getDebugBuffers() sets debugRequested, ORCH:316
+  if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
+  s->debug_requested = true;
   return FX_OK;
 }
 
@@ -1461,8 +1469,10 @@ int fx_raymarch(fx_sim* s, const fx_camera* c, const fx_render_params* p, uint32
   HIP_OK(s, hipMalloc(&dev, size_t(w) * h * 8));
   int rc = fx_raymarch_device(s, c, p, w, h, dev);
   if (rc == FX_OK) {
-    hipError_t e = hipStreamSynchronize(s->rstream);
-    if (e == hipSuccess) e = hipMemcpy(host, dev, size_t(w) * h * 8, hipMemcpyDeviceToHost);
+    // the stream the raymarch was enqueued on (the simulation stream when rendering is not overlapped)
+    hipStream_t rs = s->render_overlap ? s->rstream : s->stream;
+    hipError_t e = hipMemcpyAsync(host, dev, size_t(w) * h * 8, hipMemcpyDeviceToHost, rs);
+    if (e == hipSuccess) e = hipStreamSynchronize(rs);
     if (e != hipSuccess) rc = fail(s, FX_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
   }
   (void)hipFree(dev);

@@ -168,6 +168,9 @@ class CombustionVolumeSimulation:
         }
 
     def getDebugBuffers(self):  # ORCH:313-326
+        """The render buffers plus the debug fields; like the reference, the call also requests
+        the debug pass for the next step (debugRequested, ORCH:316), whatever the grid size."""
+        self._check(self._lib.fx_request_debug_fields(self._h))
         bufs = self.getRenderBuffers()
         for name in ("velocity_magnitude", "divergence", "pressure", "vorticity_magnitude", "confinement_magnitude"):
             bufs[name] = DeviceField(self, abi.FIELDS[name])

@@ -335,6 +335,12 @@ int fx_read_page_table(fx_sim* sim, int32_t* host, size_t count);
  * fx_step / fx_reset / fx_destroy). Element type follows the precision. */
 void* fx_field_device_ptr(fx_sim* sim, int32_t field_id);
 
+/* getDebugBuffers() also sets debugRequested (ORCH:313-326): the next fx_step runs the
+ * debug pass (velocity magnitude, ORCH:298-301) whatever the grid size, then clears
+ * the request. Without a request the pass runs every step only below 1.8M voxels
+ * (ORCH:193) or as fx_create_desc.debug_fields says. */
+int fx_request_debug_fields(fx_sim* sim);
+
 /* Pressure-hierarchy buffers (PressureSolvePass.ts:93-127): level 0 = fine. */
 int fx_read_level(fx_sim* sim, int32_t level, int32_t which, float* host, size_t bytes);
 int fx_level_dims(fx_sim* sim, int32_t level, uint32_t dims[3]);

@@ -29,10 +29,15 @@
 //     (1,3,3,1)/8 stencil (transpose of the trilinear prolongation / 8) and
 //     scaled by 4 (h^2), coarse levels start from zero.
 //
-// Pinning: tests/test_oracle.py checks each function against closed forms and
-// invariants (Jacobi eigenvectors, exact shifts under integer advection, central
-// differences on polynomials, activation special cases, restriction/prolongation
-// of constants/affine fields, zero-source identities, empty-volume raymarch).
+// Pinning: tests/test_oracle.py and tests/test_oracle_pins.py check each function
+// against closed forms and invariants: Jacobi eigenvectors, exact shifts under
+// integer advection, the trilinear sampler exact on affine fields at fractional
+// positions, the MacCormack limiter on cases where it bites, central differences
+// on polynomials, activation special cases, the prolongation exact on affine
+// coarse fields and the restriction as its scaled transpose, the combustion
+// 6-neighbour stencil and buoyancy gradient terms by differences of runs, the
+// raymarch's slab opacity/colour closed forms, phase normalisation, palette and
+// light helpers (orc_probe), zero-source identities.
 // Parity unpinned (no execution of the WGSL is possible here): hash noise
 // (inject-source-common.wgsl.ts:102-121) and the sin-based thermal noise beyond
 // tolerance, WGSL mix/FMA choices, the raymarch's pow/exp and fp16 rounding.
@@ -138,6 +143,40 @@ template <class T> inline T tri_eval(const Tri<T>& t, const T* f) {
   return mixT(mixT(x00, x10, t.fy), mixT(x01, x11, t.fy), t.fz);
 }
 
+// -------------------------------------------------------- shading helpers --
+// Raymarch shading functions (RP = src/engine/render/passes/createVolumeRaymarchPass.ts),
+// free functions so that tests can evaluate them alone (orc_probe).
+template <class T> inline V3<T> fire_palette(T t) {  // RP:477-486
+  V3<T> ember{T(0.18), T(0.018), T(0)}, red{T(1), T(0.075), T(0.012)}, orange{T(1), T(0.42), T(0.035)}, white{T(1), T(0.86), T(0.56)};
+  T rm = smooth(T(0.01), T(0.11), t), om = smooth(T(0.09), T(0.38), t), wm = smooth(T(0.58), T(0.92), t);
+  return mix3(mix3(mix3(ember, red, rm), orange, om), white, wm);
+}
+template <class T> inline V3<T> smoke_albedo(T den, T tem, T hot) {  // RP:488-495
+  T warm = clampT(tem * T(0.38), T(0), T(1));
+  T soot = smooth(T(0.16), T(0.9), den) * (T(1) - smooth(T(0.2), T(0.74), hot));
+  V3<T> cool{T(0.26), T(0.275), T(0.295)}, dsoot{T(0.04), T(0.041), T(0.045)}, warmS{T(0.36), T(0.27), T(0.18)};
+  return mix3(mix3(cool, dsoot, soot * T(0.86)), warmS, warm * (T(0.35) + hot * T(0.45)));
+}
+template <class T> inline T phase_hg(T c, T g) {  // RP:497-501
+  T g2 = g * g;
+  T denom = std::pow(std::max(T(1) + g2 - T(2) * g * c, T(0.001)), T(1.5));
+  return (T(1) - g2) / (T(12.566370614359172) * denom);
+}
+// kind > 0.5: point light at pos; else directional dir (default when |dir|^2 <= 1e-4). RP:503-510
+template <class T> inline V3<T> light_dir(T kind, V3<T> dir, V3<T> pos, V3<T> wp) {
+  if (kind > T(0.5)) return norm3(pos - wp);
+  return norm3(dot3(dir, dir) > T(0.0001) ? dir : V3<T>{T(-0.34), T(0.88), T(0.31)});
+}
+template <class T> inline T light_att(T kind, V3<T> pos, V3<T> wp) {  // RP:512-520
+  if (kind <= T(0.5)) return T(1);
+  T dist = len3(pos - wp);
+  return T(1) / (T(1) + dist * T(0.09) + dist * dist * T(0.02));
+}
+template <class T> inline T light_power(T intensity, T att) {  // RP:655-656, lightIntensity :522-524
+  T raw = std::max(intensity, T(0)) * att;
+  return raw / (T(1) + raw * T(0.18));
+}
+
 // ------------------------------------------------------------- emitters --
 // EmitterSource layout, SH/inject-source-common.wgsl.ts:3-12 / packEmitterSources.ts:69-121.
 struct SrcView {
@@ -241,6 +280,17 @@ struct Base {
   virtual int residuals(double* out, int cap) = 0;
   virtual int raymarch(const fx_camera* c, const fx_render_params* rp, uint32_t w, uint32_t h, float* out) = 0;
   virtual int active_count() = 0;
+  // Test hooks: write one level buffer; apply one level operator (LevelOp).
+  virtual int write_level(int level, int which, const double* in) = 0;
+  virtual int request_debug() = 0;
+  virtual int level_op(int op, int level, int src, int dst) = 0;
+};
+// Level operators exposed to the tests (orc_level_op).
+enum LevelOp {
+  OP_PROLONG = 0,        // pbuf(level, dst) += prolongation of pbuf(level + 1, src)
+  OP_RESTRICT_MEAN = 1,  // div(level + 1) = 8-child mean of buf(level, src)
+  OP_RESTRICT_FULL = 2,  // div(level + 1) = 4 * (1,3,3,1)/8 restriction of buf(level, src)
+  OP_JACOBI = 3,         // buf(level, dst) = one plain Jacobi sweep of buf(level, src) with rhs div(level)
 };
 
 inline int clampInt(double v, int lo, int hi) {
@@ -298,6 +348,7 @@ template <class T> struct Sim : Base {
   double startS = 0, lastS = 0;
   uint64_t stepIndex = 0;
   int lastPressure = 0;  // 0 = 'a', 1 = 'b'
+  bool debugRequested = false;  // ORCH:191, set by getDebugBuffers (:316)
   int curSet = 0;        // the reference's currentScalarSet (params.scalarSetIndex only)
   std::vector<double> resid;
 
@@ -1125,7 +1176,10 @@ template <class T> struct Sim : Base {
       lastPressure = pressure_solve();
     }
     projection();
-    if (debug_each_frame()) debug_fields();
+    if (debug_each_frame() || debugRequested) {  // ORCH:298-301
+      debug_fields();
+      debugRequested = false;
+    }
     curSet = curSet == 0 ? 1 : 0;
     lastS = elapsed;
     stepIndex += 1;
@@ -1249,6 +1303,30 @@ template <class T> struct Sim : Base {
     for (int i = 0; i < n; ++i) out[i] = resid[i];
     return int(resid.size());
   }
+  std::vector<T>& lbuf(Level<T>& L, int which) { return which == FX_LEVEL_DIVERGENCE ? L.div : which == FX_LEVEL_PRESSURE_A ? L.pA : L.pB; }
+  int write_level(int level, int which, const double* in) override {
+    if (level < 0 || level >= int(lv.size()) || which < 0 || which > 2) return FX_ERR_INVALID_ARG;
+    auto& f = lbuf(lv[level], which);
+    for (size_t i = 0; i < f.size(); ++i) f[i] = T(in[i]);
+    return 0;
+  }
+  int level_op(int op, int level, int src, int dst) override {
+    const bool needs_coarse = op != OP_JACOBI;
+    if (level < 0 || level + (needs_coarse ? 1 : 0) >= int(lv.size()) || src < 0 || src > 2 || dst < 0 || dst > 2) return FX_ERR_INVALID_ARG;
+    switch (op) {
+      case OP_PROLONG: prolong(lv[level + 1], lv[level], lbuf(lv[level + 1], src), lbuf(lv[level], dst)); return 0;
+      case OP_RESTRICT_MEAN: restrict_(lv[level], lv[level + 1], lbuf(lv[level], src), lv[level + 1].div, false); return 0;
+      case OP_RESTRICT_FULL: restrict_(lv[level], lv[level + 1], lbuf(lv[level], src), lv[level + 1].div, true); return 0;
+      case OP_JACOBI: {
+        std::vector<T> out = lbuf(lv[level], dst);
+        jacobi(lv[level], lv[level].div, lbuf(lv[level], src), out, false, T(1));
+        lbuf(lv[level], dst) = out;
+        return 0;
+      }
+      default: return FX_ERR_INVALID_ARG;
+    }
+  }
+  int request_debug() override { debugRequested = true; return 0; }
   int active_count() override {
     int n = 0;
     for (size_t b = 0; b < NB; ++b) n += bact[b] ? 1 : 0;
@@ -1336,33 +1414,11 @@ template <class T> int Sim<T>::raymarch(const fx_camera* cam, const fx_render_pa
     size_t zp = fl(x, y, std::min(z + 1, mz)), zn = fl(x, y, z == 0 ? 0 : z - 1);
     return V3<T>{Dn[xp] - Dn[xn], Dn[yp] - Dn[yn], Dn[zp] - Dn[zn]};
   };
-  auto firePalette = [&](T t) {  // RP:477-486
-    V3<T> ember{T(0.18), T(0.018), T(0)}, red{T(1), T(0.075), T(0.012)}, orange{T(1), T(0.42), T(0.035)}, white{T(1), T(0.86), T(0.56)};
-    T rm = smooth(T(0.01), T(0.11), t), om = smooth(T(0.09), T(0.38), t), wm = smooth(T(0.58), T(0.92), t);
-    return mix3(mix3(mix3(ember, red, rm), orange, om), white, wm);
-  };
-  auto smokeAlbedo = [&](T den, T tem, T hot) {  // RP:488-495
-    T warm = clampT(tem * T(0.38), T(0), T(1));
-    T soot = smooth(T(0.16), T(0.9), den) * (T(1) - smooth(T(0.2), T(0.74), hot));
-    V3<T> cool{T(0.26), T(0.275), T(0.295)}, dsoot{T(0.04), T(0.041), T(0.045)}, warmS{T(0.36), T(0.27), T(0.18)};
-    return mix3(mix3(cool, dsoot, soot * T(0.86)), warmS, warm * (T(0.35) + hot * T(0.45)));
-  };
-  auto phaseHG = [&](T c, T g) {  // RP:497-501
-    T g2 = g * g;
-    T denom = std::pow(std::max(T(1) + g2 - T(2) * g * c, T(0.001)), T(1.5));
-    return (T(1) - g2) / (T(12.566370614359172) * denom);
-  };
-  auto lightDir = [&](int i, V3<T> wp) {  // RP:503-510
-    if (lights[i].kind > T(0.5)) return norm3(lights[i].pos - wp);
-    V3<T> raw = lights[i].dir;
-    return norm3(dot3(raw, raw) > T(0.0001) ? raw : V3<T>{T(-0.34), T(0.88), T(0.31)});
-  };
-  auto lightAtt = [&](int i, V3<T> wp) {  // RP:512-520
-    if (lights[i].kind <= T(0.5)) return T(1);
-    V3<T> tl = lights[i].pos - wp;
-    T dist = len3(tl);
-    return T(1) / (T(1) + dist * T(0.09) + dist * dist * T(0.02));
-  };
+  auto firePalette = [&](T t) { return fire_palette(t); };
+  auto smokeAlbedo = [&](T den, T tem, T hot) { return smoke_albedo(den, tem, hot); };
+  auto phaseHG = [&](T c, T g) { return phase_hg(c, g); };
+  auto lightDir = [&](int i, V3<T> wp) { return light_dir(lights[i].kind, lights[i].dir, lights[i].pos, wp); };
+  auto lightAtt = [&](int i, V3<T> wp) { return light_att(lights[i].kind, lights[i].pos, wp); };
 
 #pragma omp parallel for schedule(dynamic, 4)
   for (int64_t py = 0; py < int64_t(Hpx); ++py)
@@ -1429,8 +1485,7 @@ template <class T> int Sim<T>::raymarch(const fx_camera* cam, const fx_render_pa
               for (uint32_t li = 0; li < 3; ++li) {
                 if (li >= nl) break;
                 V3<T> dl = lightDir(int(li), wp);
-                T raw = std::max(lights[li].intensity, T(0)) * lightAtt(int(li), wp);
-                T power = raw / (T(1) + raw * T(0.18));
+                T power = light_power(lights[li].intensity, lightAtt(int(li), wp));
                 V3<T> color = lights[li].color;
                 T lvc = clampT(dot3(dl, -rayDir), T(-1), T(1));
                 T fs = phaseHG(lvc, sF) * T(5.2) * power;
@@ -1488,6 +1543,30 @@ template <class T> int Sim<T>::raymarch(const fx_camera* cam, const fx_render_pa
 
 }  // namespace orc
 
+// Evaluates one raymarch shading helper in FP64 (bits 64) or FP32 (bits 32):
+//   fn 0 phase_hg(in[0] = cos, in[1] = g) -> out[0]
+//   fn 1 fire_palette(in[0]) -> out[0..2]
+//   fn 2 smoke_albedo(in[0] den, in[1] tem, in[2] hot) -> out[0..2]
+//   fn 3 light_dir(in[0] kind, in[1..3] dir, in[4..6] pos, in[7..9] world position) -> out[0..2]
+//   fn 4 light_att(in[0] kind, in[1..3] pos, in[4..6] world position) -> out[0]
+//   fn 5 light_power(in[0] intensity, in[1] attenuation) -> out[0]
+namespace orc {
+template <class T> int probe_t(int fn, const double* in, double* out) {
+  using orc::V3;
+  auto v3 = [&](int k) { return V3<T>{T(in[k]), T(in[k + 1]), T(in[k + 2])}; };
+  auto put3 = [&](V3<T> v) { out[0] = double(v.x); out[1] = double(v.y); out[2] = double(v.z); };
+  switch (fn) {
+    case 0: out[0] = double(orc::phase_hg(T(in[0]), T(in[1]))); return 0;
+    case 1: put3(orc::fire_palette(T(in[0]))); return 0;
+    case 2: put3(orc::smoke_albedo(T(in[0]), T(in[1]), T(in[2]))); return 0;
+    case 3: put3(orc::light_dir(T(in[0]), v3(1), v3(4), v3(7))); return 0;
+    case 4: out[0] = double(orc::light_att(T(in[0]), v3(1), v3(4))); return 0;
+    case 5: out[0] = double(orc::light_power(T(in[0]), T(in[1]))); return 0;
+    default: return FX_ERR_INVALID_ARG;
+  }
+}
+}  // namespace orc
+
 // ------------------------------------------------------------- C ABI --
 extern "C" {
 void* orc_create(int bits, const fx_create_desc* desc) {
@@ -1512,5 +1591,11 @@ int orc_residuals(void* h, double* out, int cap) { return static_cast<orc::Base*
 int orc_active_count(void* h) { return static_cast<orc::Base*>(h)->active_count(); }
 int orc_raymarch(void* h, const fx_camera* c, const fx_render_params* rp, uint32_t w, uint32_t hh, float* out) {
   return static_cast<orc::Base*>(h)->raymarch(c, rp, w, hh, out);
+}
+int orc_request_debug(void* h) { return static_cast<orc::Base*>(h)->request_debug(); }
+int orc_write_level(void* h, int l, int which, const double* in) { return static_cast<orc::Base*>(h)->write_level(l, which, in); }
+int orc_level_op(void* h, int op, int l, int src, int dst) { return static_cast<orc::Base*>(h)->level_op(op, l, src, dst); }
+int orc_probe(int bits, int fn, const double* in, double* out) {
+  return bits == 64 ? orc::probe_t<double>(fn, in, out) : orc::probe_t<float>(fn, in, out);
 }
 }

@@ -41,6 +41,10 @@ def lib():
             "orc_active_count": (C.c_int, [P]),
             "orc_raymarch": (C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams), C.c_uint32, C.c_uint32,
                                        C.POINTER(C.c_float)]),
+            "orc_write_level": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+            "orc_request_debug": (C.c_int, [P]),
+            "orc_level_op": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "orc_probe": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -121,6 +125,21 @@ class Oracle:
         self._ok(self.L.orc_read_level(self.h, level, which, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
+    def request_debug(self):
+        """getDebugBuffers()' request: the next step runs the debug pass (ORCH:316)."""
+        self._ok(self.L.orc_request_debug(self.h))
+
+    def write_level(self, level, which, values):
+        w, h, d = self.level_dims(level)
+        arr = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(d, h, w))
+        self._ok(self.L.orc_write_level(self.h, level, which, arr.ctypes.data_as(C.POINTER(C.c_double))))
+
+    # level operators (orc_level_op)
+    OP_PROLONG, OP_RESTRICT_MEAN, OP_RESTRICT_FULL, OP_JACOBI = 0, 1, 2, 3
+
+    def level_op(self, op, level, src=abi.FX_LEVEL_PRESSURE_A, dst=abi.FX_LEVEL_PRESSURE_A):
+        self._ok(self.L.orc_level_op(self.h, op, level, src, dst))
+
     def residuals(self):
         buf = (C.c_double * 64)()
         n = self.L.orc_residuals(self.h, buf, 64)
@@ -133,6 +152,22 @@ class Oracle:
         out = np.zeros((h, w, 4), dtype=np.float32)
         self._ok(self.L.orc_raymarch(self.h, C.byref(cam), C.byref(params), w, h, out.ctypes.data_as(C.POINTER(C.c_float))))
         return out
+
+
+# shading helpers of the raymarch (orc_probe)
+PROBE_PHASE_HG, PROBE_FIRE_PALETTE, PROBE_SMOKE_ALBEDO, PROBE_LIGHT_DIR, PROBE_LIGHT_ATT, PROBE_LIGHT_POWER = range(6)
+
+
+def probe(fn, *args, bits=64):
+    """Evaluates one raymarch shading helper of the oracle; returns a float or a 3-vector."""
+    inp = (C.c_double * 16)(*[float(a) for a in args])
+    out = (C.c_double * 4)()
+    rc = lib().orc_probe(bits, fn, inp, out)
+    if rc != 0:
+        raise RuntimeError(f"orc_probe failed: {rc}")
+    if fn in (PROBE_FIRE_PALETTE, PROBE_SMOKE_ALBEDO, PROBE_LIGHT_DIR):
+        return np.array(out[:3])
+    return out[0]
 
 
 _DIFF = None
