@@ -202,6 +202,7 @@ FUNCTIONS = {
     "fx_raymarch_device": (C.c_int, [P, C.POINTER(Camera), C.POINTER(RenderParams), u32, u32, C.c_void_p]),
     "fx_bloom_device": (C.c_int, [P, C.POINTER(BloomParams), u32, u32, C.c_void_p, C.c_void_p]),
     "fx_bloom": (C.c_int, [P, C.POINTER(BloomParams), u32, u32, C.POINTER(C.c_uint16), C.POINTER(C.c_uint8)]),
+    "fx_bloom_device_noalloc": (C.c_int, [P, C.POINTER(BloomParams), u32, u32, C.c_void_p, C.c_void_p]),
     "fx_set_stage_timing": (C.c_int, [P, i32]),
     "fx_read_stage_times": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(u32), C.c_size_t]),
     "fx_set_vcycle_plan": (C.c_int, [P, u32, u32]),

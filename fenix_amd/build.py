@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfenix.so")
-SOURCES = ["fenix_kernels.hip", "fenix_api.cpp", "fenix_diff.cpp"]
+SOURCES = ["fenix_kernels.hip", "fenix_api.cpp", "fenix_diff.cpp", "fenix_static.cpp"]
 
 HIPCC_FLAGS = [
     "-O3",

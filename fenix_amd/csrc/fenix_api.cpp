@@ -850,6 +850,9 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
   auto bail = [&](int code) { std::string e = s->err; fx_destroy(s); g_create_error = e; return code; };
 
   if (hipSetDevice(s->device) != hipSuccess) return bail(fail(s, FX_ERR_DEVICE, "hipSetDevice failed"));
+  // the static scratch of fx_bloom_device_noalloc: its address on this device, once per device
+  if (static_scratch_setup(s->device, nullptr) != FX_OK)
+    return bail(fail(s, FX_ERR_DEVICE, "hipGetSymbolAddress of the static scratch failed"));
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, s->device) != hipSuccess) return bail(fail(s, FX_ERR_DEVICE, "hipGetDeviceProperties failed"));
   {
@@ -1683,6 +1686,10 @@ namespace fx {
 SimView sim_view(fx_sim* s) {
   return SimView{s->stream, s->g, fine_list(s), fine_count(s), grid_for(s, s->nb), s->nb, storage_count(s->g)};
 }
+int sim_fail(fx_sim* s, int code, const char* msg) { return fail(s, code, msg); }
+int sim_device(fx_sim* s) { return s->device; }
+hipStream_t sim_render_stream(fx_sim* s) { return s->render_overlap ? s->rstream : s->stream; }
+
 void sim_set_blocks_per_cu(fx_sim* s, int per_cu) {
   if (!s->blocks_env) s->blocks = s->cus * std::max(1, per_cu);
 }

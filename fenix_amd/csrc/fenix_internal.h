@@ -287,4 +287,11 @@ struct BloomArgs {
 };
 void launch_bloom(hipStream_t s, const BloomArgs& a, const void* src, void* bright, void* blur_h, void* blur_v, void* out);
 
+// Simulation internals used by the entry points of fenix_static.cpp.
+int sim_fail(fx_sim* s, int code, const char* msg);  // records the message for fx_last_error; returns code
+int sim_device(fx_sim* s);
+hipStream_t sim_render_stream(fx_sim* s);            // where the raymarch and bloom run
+// Address of the static scratch array on `device` (current), resolved once per device (fenix_static.cpp).
+int static_scratch_setup(int device, void** out);
+
 }  // namespace fx

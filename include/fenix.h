@@ -385,6 +385,18 @@ int fx_bloom_device(fx_sim* sim, const fx_bloom_params* params, uint32_t width, 
 int fx_bloom(fx_sim* sim, const fx_bloom_params* params, uint32_t width, uint32_t height,
              const uint16_t* host_src_rgba_f16, uint8_t* host_dst_rgba8);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): fx_bloom_device without any allocation. The three half-resolution
+ * images live in a static device array of 24 MiB per GPU (held until the process
+ * ends), so the canvas is limited to 1024 x 1024 half-resolution pixels (up to
+ * 2048 x 2048); larger sizes fail with FX_ERR_INVALID_ARG. Every call on one GPU
+ * shares that array: calls that overlap on a GPU (several simulations rendering
+ * concurrently on different streams) race and must be serialised by the caller.
+ * Results equal fx_bloom_device's. Enqueued on the render stream; asynchronous. */
+int fx_bloom_device_noalloc(fx_sim* sim, const fx_bloom_params* params, uint32_t width, uint32_t height,
+                            const void* device_src_rgba_f16, void* device_dst_rgba8);
+
 /* ---- stage timing ------------------------------------------------------- */
 
 /* Timing ids: the FX_PASS_* ids (whole pass; FX_PASS_ACTIVATE_RENDER includes the
