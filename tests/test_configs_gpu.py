@@ -1,0 +1,256 @@
+"""GPU parity at each configuration's own size.
+
+C1: dense 64^3, reference semantics, 100 steps, 256 x 256 x 64 raymarch.
+C2: sparse 256^3 from the seeded rising plume (~15% of the bricks), 1024 x 1024 x 128 raymarch.
+C3: the 64-scene batch (edges 32-192, 5-40% active), every scene against its own oracle.
+C4: sparse 512^3, BF16 and FP16 storage of density/temperature/fuel, ~8% active.
+
+Per-step parity: before each step both sides hold the oracle's state; after it the brick flags
+(and in sparse mode the active list and page table) are bit-exact and the FP32 fields within 1e-5
+relative L2 (16-bit storage: 2e-3 BF16, 3e-4 FP16, against the FP32 oracle that rounds on store).
+Free-running drift against the FP64 oracle is written to $FENIX_REPORT_DIR (default reports/).
+"""
+import json
+import os
+import time
+
+import numpy as np
+import pytest
+
+import scenes
+from fenix_amd import abi
+from fenix_amd.render import VolumeRaymarchPass, camera_struct, orbit_camera_snapshot
+from fenix_amd.simulation import CombustionVolumeBatch, CombustionVolumeSimulation
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+REL_L2 = 1e-5
+TOL16 = {abi.FX_PRECISION_BF16: 2e-3, abi.FX_PRECISION_FP16: 3e-4}
+PERSISTENT = ("density", "temperature", "fuel", "reaction", "velocity_x", "velocity_y", "velocity_z")
+DT = 1.0 / 60.0
+
+
+def rel_l2(a, b):
+    den = float(np.linalg.norm(b))
+    num = float(np.linalg.norm(np.asarray(a, np.float64) - np.asarray(b, np.float64)))
+    return num / den if den > 0 else num
+
+
+def report_dir():
+    out = os.environ.get("FENIX_REPORT_DIR",
+                         os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "reports"))
+    os.makedirs(out, exist_ok=True)
+    return out
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def check_step(sim, orc, label, tol=REL_L2, fields=PERSISTENT + ("pressure", "divergence"), sparse=False):
+    flags = orc.flags(abi.FX_FLAGS_RENDER)
+    np.testing.assert_array_equal(sim.read_flags(abi.FX_FLAGS_RENDER), flags, err_msg=f"{label}: render flags")
+    if sparse:
+        lst = sim.read_active_list()
+        np.testing.assert_array_equal(lst, np.flatnonzero(flags).astype(np.uint32), err_msg=f"{label}: active list")
+        page = sim.read_page_table()
+        expect = np.full(flags.size, -1, np.int32)
+        expect[lst] = np.arange(lst.size, dtype=np.int32)
+        np.testing.assert_array_equal(page, expect, err_msg=f"{label}: page table")
+    worst = 0.0
+    for name in fields:
+        g = sim.read_field(name)
+        assert np.all(np.isfinite(g)), f"{label}: non-finite {name}"
+        e = rel_l2(g, orc.read(name))
+        worst = max(worst, e)
+        assert e <= tol, f"{label}: {name} rel L2 {e:.3e} > {tol:g}"
+    return worst
+
+
+def resync(sim, orc):
+    for name in PERSISTENT:
+        sim.write_field(name, orc.read(name))
+
+
+def check_image(img, ref, label):
+    img = img.astype(np.float32)
+    ref = ref.astype(np.float16).astype(np.float32)
+    assert ref[..., 3].max() > 0.05, f"{label}: the scene renders nothing"
+    e = rel_l2(img, ref)
+    assert e < 2e-3, f"{label}: image rel L2 {e:.3e}"
+    assert np.mean(np.abs(img - ref).max(axis=-1) < 2e-3) > 0.99, label
+    return e
+
+
+# --------------------------------------------------------------------- C1 --
+def test_c1_dense_64_100_steps():
+    """C1 at its own size: dense 64^3 with 8^3 bricks (all 512 allocated), reference semantics,
+    the source sphere r = 6 at (32, 8, 32), seeded velocity noise +-0.1 in it, 100 steps of 1/60;
+    every step at 1e-5 against the FP32 oracle; the 256 x 256 x 64 raymarch after the last step;
+    and the free-running 100-step drift of the GPU (FP32) against the FP64 oracle, bounded by the
+    north star's 1e-3 and written to c1_drift.json."""
+    n, steps = 64, 100
+    desc = scenes.c1_desc(n)
+    srcs = scenes.c1_sources()
+    u0 = scenes.c1_initial_velocity(n)
+    sim = CombustionVolumeSimulation(desc)
+    orc = Oracle(desc, 32)
+    sim.updateSources(srcs)
+    orc.update_sources(*scenes.packed(srcs))
+    for c, name in enumerate(("velocity_x", "velocity_y", "velocity_z")):
+        sim.write_field(name, u0[c])
+        orc.write(name, u0[c])
+    worst = 0.0
+    t0 = time.time()
+    for k in range(steps):
+        if k > 0:
+            resync(sim, orc)
+        sim.step(None, k * DT, DT)
+        orc.step(k * DT, DT)
+        worst = max(worst, check_step(sim, orc, f"C1 step {k}"))
+    log(f"C1 per-step parity: 100 steps, worst rel L2 {worst:.2e} ({time.time() - t0:.0f} s)")
+    assert orc.flags().sum() > 8  # the plume has grown beyond its source brick
+    rp = VolumeRaymarchPass((n, n, n))
+    rp.setRenderParams(stepCount=64)
+    cam = orbit_camera_snapshot()
+    resync(sim, orc)
+    sim.step(None, steps * DT, DT)
+    orc.step(steps * DT, DT)
+    resync(sim, orc)
+    img = rp.render(sim, cam, "temperature", 256, 256)
+    ref = orc.raymarch(camera_struct(cam), rp.params_struct("temperature"), 256, 256)
+    e_img = check_image(img, ref, "C1 256x256x64 raymarch")
+    sim.dispose()
+    # free-running drift: GPU FP32 against the FP64 oracle, same start
+    free = CombustionVolumeSimulation(desc)
+    ref64 = Oracle(desc, 64)
+    free.updateSources(srcs)
+    ref64.update_sources(*scenes.packed(srcs))
+    for c, name in enumerate(("velocity_x", "velocity_y", "velocity_z")):
+        free.write_field(name, u0[c])
+        ref64.write(name, u0[c])
+    for k in range(steps):
+        free.step(None, k * DT, DT)
+        ref64.step(k * DT, DT)
+    drift = {f: rel_l2(free.read_field(f), ref64.read(f)) for f in PERSISTENT}
+    with open(os.path.join(report_dir(), "c1_drift.json"), "w") as fh:
+        json.dump({"scene": "C1 dense 64^3, reference semantics, 100 steps of 1/60",
+                   "per_step_worst_rel_l2_vs_fp32_oracle": worst, "raymarch_rel_l2": e_img,
+                   "free_running_rel_l2_vs_fp64_after_100_steps": drift,
+                   "active_bricks_fp64": int(ref64.flags().sum())}, fh, indent=1)
+    log("C1 drift vs FP64 after 100 steps: " + ", ".join(f"{k} {v:.1e}" for k, v in drift.items()))
+    for f, e in drift.items():
+        assert e <= 1e-3, (f, e)
+    free.dispose()
+
+
+# --------------------------------------------------------------------- C2 --
+def test_c2_sparse_256_plume():
+    """C2 at its own size: sparse 256^3 from the seeded rising plume (the benchmark's start,
+    ~15% of the bricks), config mode (threshold activation, burn 0.5, ignition 0.6, 4-level
+    V-cycle); 3 steps at 1e-5 with flags, active list and page table bit-exact; then the
+    1024 x 1024 x 128 raymarch of the resulting state."""
+    n = 256
+    srcs = scenes.c2_sources(n)
+    st, frac = scenes.c2_plume_state(n, srcs, target=0.15)
+    log(f"C2 start: {frac:.3f} of the bricks active")
+    desc = scenes.c2_desc(n)
+    sim = CombustionVolumeSimulation(desc)
+    orc = Oracle(desc, 32)
+    sim.updateSources(srcs)
+    orc.update_sources(*scenes.packed(srcs))
+    for k, v in st.items():
+        sim.write_field(k, v)
+        orc.write(k, v)
+    for k in range(3):
+        if k > 0:
+            resync(sim, orc)
+        sim.step(None, k * DT, DT)
+        orc.step(k * DT, DT)
+        w = check_step(sim, orc, f"C2 step {k}", sparse=True)
+        log(f"C2 step {k}: {sim.active_brick_count()} active bricks, worst rel L2 {w:.2e}")
+    assert 0.1 * sim.brick_count < sim.active_brick_count() < 0.25 * sim.brick_count
+    resync(sim, orc)
+    sim.step(None, 3 * DT, DT)
+    orc.step(3 * DT, DT)
+    resync(sim, orc)
+    rp = VolumeRaymarchPass((n, n, n))
+    rp.setRenderParams(stepCount=128)
+    cam = orbit_camera_snapshot()
+    img = rp.render(sim, cam, "temperature", 1024, 1024)
+    ref = orc.raymarch(camera_struct(cam), rp.params_struct("temperature"), 1024, 1024)
+    check_image(img, ref, "C2 1024x1024x128 raymarch")
+    sim.dispose()
+
+
+# --------------------------------------------------------------------- C3 --
+def test_c3_batch_64_scenes_each_against_its_oracle():
+    """C3 at its own size: the 64 seeded scenes (edge log-uniform in 32-192, 5-40% of the bricks
+    active, 1-4 emitters) stepped together through the batch API, 2 steps; after each batch step
+    every scene matches its own FP32 oracle (flags, list and page table bit-exact, fields 1e-5)."""
+    specs = scenes.c3_scenes()
+    assert len(specs) == 64
+    descs = [scenes.c2_desc(sp["n"]) for sp in specs]
+    states = [scenes.plume_state_fast(sp["n"], sp["sources"], sp["target"], sp["seed"]) for sp in specs]
+    batch = CombustionVolumeBatch(descs)
+    oracles = []
+    for sim, sp, st, desc in zip(batch.scenes, specs, states, descs):
+        sim.updateSources(sp["sources"])
+        orc = Oracle(desc, 32)
+        orc.update_sources(*scenes.packed(sp["sources"]))
+        for k, v in st.items():
+            sim.write_field(k, v)
+            orc.write(k, v)
+        oracles.append(orc)
+    del states
+    worst = 0.0
+    for k in range(2):
+        if k > 0:
+            for sim, orc in zip(batch.scenes, oracles):
+                resync(sim, orc)
+        batch.step(k * DT, DT)
+        batch.synchronize()
+        for j, (sim, orc, sp) in enumerate(zip(batch.scenes, oracles, specs)):
+            orc.step(k * DT, DT)
+            worst = max(worst, check_step(sim, orc, f"C3 scene {j} ({sp['n']}^3) step {k}", sparse=True))
+    log(f"C3: 64 scenes x 2 steps, worst rel L2 {worst:.2e}")
+    batch.dispose()
+
+
+# --------------------------------------------------------------------- C4 --
+@pytest.mark.parametrize("prec", [abi.FX_PRECISION_BF16, abi.FX_PRECISION_FP16])
+def test_c4_sparse_512_16bit_storage(prec):
+    """C4 at its own size: sparse 512^3, 4 floor emitters r = 20 (seed 4), a seeded plume of ~8%
+    of the bricks (~21k), density/temperature/fuel stored in 16 bits. One step against the FP32
+    oracle that rounds to the same format on store (flags, list, page table bit-exact; fields
+    within the 16-bit tolerance), then full-size properties after a second step: every field
+    zero outside the active bricks, all finite."""
+    n = 512
+    srcs = scenes.c2_sources(n, seed=4, count=4, radius_vox=20.0)
+    st = scenes.plume_state_fast(n, srcs, target=0.08, seed=4)
+    desc = scenes.c2_desc(n, precision=prec)
+    sim = CombustionVolumeSimulation(desc)
+    sim.updateSources(srcs)
+    orc = Oracle(desc, 32)
+    orc.update_sources(*scenes.packed(srcs))
+    for k, v in st.items():
+        sim.write_field(k, v)
+        orc.write(k, v)
+    del st
+    sim.step(None, 0.0, DT)
+    orc.step(0.0, DT)
+    w = check_step(sim, orc, f"C4 {prec}", tol=TOL16[prec], fields=PERSISTENT, sparse=True)
+    nact = sim.active_brick_count()
+    log(f"C4 precision {prec}: {nact} active bricks ({nact / sim.brick_count:.3f}), worst rel L2 {w:.2e}")
+    assert 0.05 * sim.brick_count < nact < 0.12 * sim.brick_count
+    del orc
+    sim.step(None, DT, DT)
+    flags = sim.read_flags().astype(bool)
+    nb = n // 8
+    for name in ("density", "temperature", "velocity_y", "pressure"):
+        f = sim.read_field(name)
+        assert np.all(np.isfinite(f)), name
+        blocks = np.abs(f).reshape(nb, 8, nb, 8, nb, 8).max(axis=(1, 3, 5)).reshape(-1)
+        assert np.all(blocks[~flags] == 0), f"{name} is non-zero outside the active bricks"
+    sim.dispose()
