@@ -380,7 +380,7 @@ def main():
                          "the plan an autotuned run chose, since profiler overhead changes what the autotuner picks")
     ap.add_argument("--batch", type=int, default=0, metavar="K",
                     help="batched scenes: K 256^3 plume scenes (c2) or the C3 batch (c3, default 64 scenes)")
-    ap.add_argument("--hw-queues", type=int, default=16, choices=range(1, 33), metavar="Q",
+    ap.add_argument("--hw-queues", type=int, default=4, choices=range(1, 33), metavar="Q",
                     help="hardware queues per process for batched runs (GPU_MAX_HW_QUEUES, at most 32)")
     ap.add_argument("--bloom", action="store_true",
                     help="a step also runs the bloom/composite passes into an RGBA8 canvas image (the reference's frame)")
@@ -461,17 +461,16 @@ def main():
     stage_ms = {k: v[0] / v[1] for k, v in stages.items()}
     # Dominant kernel: the single-kernel stage with the most device time per step on
     # the simulation stream (the raymarch runs overlapped on its own stream). The
-    # fine-level sweep launches ~15 times per step; its mean launch time comes from
-    # the library's kernel clock (100 back-to-back production launches between one
-    # event pair), the others from the per-stage events.
+    # fine-level sweep launches ~15 times per step; its mean launch time is that of
+    # the production sweeps inside the stage-timed steps, each timed by HIP events
+    # that take the kernel's own start and end timestamps (hipExtLaunchKernelGGL), so
+    # it is the duration rocprof reports for the same kernel; other stages use their
+    # per-stage event pairs.
     totals = {k: stages[k][0] for k in stages if k in STAGE_BYTES}
     if not totals:
         raise SystemExit("--timing-steps must be >= 1 to find the dominant kernel")
     dom = max(totals, key=totals.get)
-    if dom == "jacobi_fine" and vcycle:
-        avg_ms, vox_launch = sim.time_fine_sweep(100)
-    else:
-        avg_ms, vox_launch = stage_ms[dom], active_t
+    avg_ms, vox_launch = stage_ms[dom], active_t
     ray_bytes = RAY_BYTES_PER_VOXEL - saved * RAY_SCALAR_ACCESSES
     sim_bytes = (STEP_BYTES_SIM if saved == 0 else STEP_BYTES_SIM_16) if vcycle else STEP_BYTES_SIM_REF[saved]
     step_bytes_voxel = sim_bytes + ray_bytes

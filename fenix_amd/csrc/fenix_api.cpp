@@ -107,6 +107,18 @@ struct fx_sim {
   // V-cycle kernel plan per level (bit l set: split legs of lean one-sweep
   // kernels instead of the fused LDS legs). Part of the graph key.
   uint32_t split_down = 0, split_up = 0;
+  // Coarsest level: the one-workgroup LDS smoother holds kCoarseCap active
+  // voxels (beyond that it falls back to a slow one-workgroup global path);
+  // coarse_split runs the coarsest sweeps as multi-workgroup launches instead.
+  // Decided per step from an upper bound of the coarsest active voxel count:
+  // exact when dense, from the fine active-brick count of an earlier step when
+  // sparse (copied into pinned memory at the end of every step and read once
+  // that step's event has completed, so no step waits for it). Results never
+  // depend on the choice. Part of the graph key.
+  bool coarse_split = false;
+  uint32_t* h_fine_count = nullptr;  // pinned
+  hipEvent_t ev_count = nullptr;
+  bool count_pending = false;
 
   // hipGraphs of the step body, split where the scalars and render flags are
   // final (after combustion); rebuilt when the recorded sequence changes.
@@ -458,7 +470,7 @@ int solve_vcycle(fx_sim* s) {
       launch_clear_u32(s->stream, A.zero, A.zero_words, 0u);
   }
   Level& LC = s->lv[L - 1];
-  const bool block_smoother = L > 1 && LC.nv <= kBlockSmoothVoxels;
+  const bool block_smoother = L > 1 && LC.nv <= kBlockSmoothVoxels && !s->coarse_split;
   if (block_smoother && (s->sparse || !s->dense_tables))
     launch_coarse_setup(s->stream, level_list(s, L - 1), level_count(s, L - 1), LC.g, bits_of(s, L - 1), LC.tab, LC.tab_n, LC.p[0],
                         LC.p[1]);
@@ -487,9 +499,15 @@ int solve_vcycle(fx_sim* s) {
         ++k;
         continue;
       }
-      StageTimer timer(s, l == 0 ? FX_TIME_JACOBI_FINE : -1);
-      launch_sweep(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, bits_of(s, l), s->d_const, V.div, src,
-                   V.p[1 - cur[l]], l == 0);
+      // fine sweeps reading a pressure are timed by the kernel's own start/end timestamps
+      const bool exact = s->timing && l == 0 && src != nullptr;
+      hipEvent_t ka = exact ? StageTimer::take(s) : nullptr, kb = exact ? StageTimer::take(s) : nullptr;
+      {
+        StageTimer timer(s, (l == 0 && !(ka && kb)) ? FX_TIME_JACOBI_FINE : -1);
+        launch_sweep(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, bits_of(s, l), s->d_const, V.div, src,
+                     V.p[1 - cur[l]], l == 0, ka, kb);
+      }
+      if (ka && kb) s->recs.push_back(fx_sim::Rec{FX_TIME_JACOBI_FINE, ka, kb});
       cur[l] = 1 - cur[l];
     }
   };
@@ -738,6 +756,35 @@ void record_step_b(fx_sim* s, bool vort, bool pres, bool dbg) {
   }
   run_stage(s, FX_PASS_PROJECTION);                    // ORCH:296
   if (dbg) run_stage(s, FX_PASS_DEBUG_FIELDS);         // ORCH:298-301
+  if (s->sparse && s->h_fine_count)                    // for the next steps' coarsest-level plan
+    (void)hipMemcpyAsync(s->h_fine_count, s->lv[0].L.count, 4, hipMemcpyDeviceToHost, s->stream);
+}
+
+// Upper bound of the coarsest level's active voxels: a level-l voxel J lies in
+// fine brick (J << l) >> 3, so a fine brick holds 8^(3-l) level-l voxels for
+// l <= 3 and at least one for l > 3.
+uint64_t coarse_active_bound(const fx_sim* s, uint64_t fine_bricks) {
+  const int l = int(s->lv.size()) - 1;
+  uint64_t per = 1;
+  for (int k = l; k < 3; ++k) per *= 8;
+  return fine_bricks * per;
+}
+// Chooses the coarsest-level plan before a step (see fx_sim::coarse_split).
+void update_coarse_plan(fx_sim* s) {
+  if (s->desc.solver != FX_SOLVER_VCYCLE || s->lv.size() < 2) return;
+  const Level& LC = s->lv.back();
+  if (!s->sparse) {  // every interior voxel is active
+    const uint64_t W = LC.g.W > 2 ? LC.g.W - 2 : 0, H = LC.g.H > 2 ? LC.g.H - 2 : 0, D = LC.g.D > 2 ? LC.g.D - 2 : 0;
+    s->coarse_split = W * H * D > kCoarseCap;
+    return;
+  }
+  if (s->count_pending && hipEventQuery(s->ev_count) == hipSuccess) {
+    s->count_pending = false;
+    const uint64_t bound = coarse_active_bound(s, *s->h_fine_count);
+    // hysteresis: a set hovering near the cap does not recapture the graph every step
+    if (bound > kCoarseCap) s->coarse_split = true;
+    else if (bound * 8 < uint64_t(kCoarseCap) * 7) s->coarse_split = false;
+  }
 }
 
 // Captures the two step parts into hipGraphs (kernel nodes only: every size the
@@ -881,6 +928,12 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
       hipEventCreateWithFlags(&s->ev_scalars, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&s->ev_render, hipEventDisableTiming) != hipSuccess)
     return bail(fail(s, FX_ERR_DEVICE, "stream creation failed"));
+  if (s->sparse && desc->solver == FX_SOLVER_VCYCLE) {
+    if (hipHostMalloc(reinterpret_cast<void**>(&s->h_fine_count), 4, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&s->ev_count, hipEventDisableTiming) != hipSuccess)
+      return bail(fail(s, FX_ERR_OUT_OF_MEMORY, "pinned count allocation failed"));
+    *s->h_fine_count = 0;
+  }
 
   const size_t nsv = storage_count(s->g);
   int rc = FX_OK;
@@ -1010,6 +1063,7 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
   int r = fx_reset(s);
   if (r != FX_OK) return bail(r);
   if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(s, FX_ERR_DEVICE, "initial clear failed"));
+  update_coarse_plan(s);  // dense: fixed by the grid
   *out = s;
   return FX_OK;
 }
@@ -1044,6 +1098,8 @@ void fx_destroy(fx_sim* s) {
   for (hipEvent_t e : s->pool) (void)hipEventDestroy(e);
   if (s->ev_scalars) (void)hipEventDestroy(s->ev_scalars);
   if (s->ev_render) (void)hipEventDestroy(s->ev_render);
+  if (s->ev_count) (void)hipEventDestroy(s->ev_count);
+  if (s->h_fine_count) (void)hipHostFree(s->h_fine_count);
   if (s->rstream) (void)hipStreamDestroy(s->rstream);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -1097,10 +1153,11 @@ int fx_step(fx_sim* s, double elapsed_s, double step_s) {  // ORCH:230-305
   const bool pres = s->stepIndex % uint64_t(s->q.pressure_interval) == 0;
   const bool dbg = debug_each_frame(s) || s->debug_requested;
   s->debug_requested = false;
+  update_coarse_plan(s);
   if (s->use_graph && !s->timing) {
     std::vector<int64_t> key = {vort, pres, dbg, s->mode, pres ? -1 : s->lastPressure, s->q.fine_pre_iterations,
                                 s->q.fine_post_iterations, s->q.mid_pre_iterations, s->q.mid_post_iterations,
-                                s->q.coarse_iterations, s->split_down, s->split_up};
+                                s->q.coarse_iterations, s->split_down, s->split_up, s->coarse_split};
     if (!s->graph_exec || key != s->graph_key) {
       rc = capture_step(s, vort, pres, dbg);
       if (rc != FX_OK) return rc;
@@ -1118,6 +1175,10 @@ int fx_step(fx_sim* s, double elapsed_s, double step_s) {  // ORCH:230-305
     if (rc != FX_OK) return rc;
     record_step_b(s, vort, pres, dbg);
   }
+  if (s->sparse && s->ev_count && !s->count_pending) {  // the count copied at the end of this step
+    HIP_OK(s, hipEventRecord(s->ev_count, s->stream));
+    s->count_pending = true;
+  }
   s->curSet = s->curSet == 0 ? 1 : 0;
   s->lastS = elapsed_s;
   s->stepIndex += 1;
@@ -1134,6 +1195,7 @@ int fx_run_pass(fx_sim* s, int32_t pass, const fx_sim_params* params) {
   snapshot_stale(s);
   int rc = upload_step(s, *params, s->sparse ? &cand : nullptr, inj);
   if (rc != FX_OK) return rc;
+  update_coarse_plan(s);
   switch (pass) {
     case FX_PASS_INJECT_SCALAR: pass_inject(s, 1); break;
     case FX_PASS_INJECT_VELOCITY: pass_inject(s, 2); break;

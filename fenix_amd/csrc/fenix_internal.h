@@ -145,8 +145,11 @@ void launch_restrict_lean(hipStream_t s, int blocks, const uint32_t* list, const
 void launch_prolong_add(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid c, Grid f, LvBits b,
                         const float* e, float* p);
 // One weighted-Jacobi sweep of a V-cycle level; src == nullptr starts from p = 0.
+// ev_start/ev_stop (fine level with src only): events that take the kernel's own
+// start and end timestamps.
 void launch_sweep(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
-                  const float* rhs, const float* src, float* dst, bool fine);
+                  const float* rhs, const float* src, float* dst, bool fine, hipEvent_t ev_start = nullptr,
+                  hipEvent_t ev_stop = nullptr);
 // Two sweeps in one launch (LDS temporal blocking), bit-identical to two
 // launch_sweep calls; src == nullptr starts from p = 0.
 void launch_sweep_pair(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
@@ -156,7 +159,8 @@ void launch_sweep_pair(hipStream_t s, int blocks, const uint32_t* list, const ui
 constexpr int kPlanPairShift = 8;
 // Coarsest-level slot table (kCoarseTableBytes at tab, count at tab_n), built
 // once per solve; zeroes the inactive voxels of the level's listed bricks in p0/p1.
-constexpr size_t kCoarseTableBytes = 8192 * 16;
+constexpr uint32_t kCoarseCap = 8192;  // active voxels the one-workgroup LDS smoother holds
+constexpr size_t kCoarseTableBytes = size_t(kCoarseCap) * 16;
 void launch_coarse_setup(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, void* tab, uint32_t* tab_n,
                          float* p0, float* p1);
 // tab == nullptr (or an overflowed table): global-memory sweeps in one workgroup.

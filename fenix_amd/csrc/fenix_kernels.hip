@@ -12,6 +12,7 @@
 // built with -ffp-contract=off so no FMA contraction changes the rounding.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 
@@ -1104,7 +1105,7 @@ __global__ __launch_bounds__(512) void k_jacobi(const uint32_t* __restrict__ lis
 // sparse invariant). It also zeroes the inactive voxels of both buffers, which
 // the sweeps then never write. Each k_smooth_block thread owns up to kSmoothPer
 // slots; every sweep is LDS-only.
-constexpr uint32_t kSmoothCap = 8192;      // active voxels staged in LDS
+constexpr uint32_t kSmoothCap = kCoarseCap;  // active voxels staged in LDS
 constexpr uint32_t kSmoothBricks = 512;    // listed bricks whose voxel masks fit in LDS
 constexpr uint32_t kSmoothTable = 4096;    // level bricks addressable through the LDS slot table
 constexpr uint32_t kSmoothOverflow = 0xffffffffu;  // table header: the level takes the global path
@@ -3108,11 +3109,15 @@ void launch_prolong_add(hipStream_t s, int blocks, const uint32_t* list, const u
   FX_LAUNCH(k_prolong_add, blocks, 512, s, list, count, c, f, b, e, p);
 }
 void launch_sweep(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
-                  const float* rhs, const float* src, float* dst, bool fine) {
+                  const float* rhs, const float* src, float* dst, bool fine, hipEvent_t ev_start, hipEvent_t ev_stop) {
   // 256-thread workgroups of one wave per brick, 5/3 as many as the 512-thread
   // persistent grid (five per CU fit beside a raymarch workgroup at 72 VGPRs)
   const int wg = blocks >= 8 ? ((blocks * 5 / 3) & ~7) : blocks;
-  if (src && fine) {
+  if (ev_start && ev_stop && src && fine) {
+    // the events take the kernel's own start and end timestamps (no dispatch time)
+    hipExtLaunchKernelGGL((k_sweep<false, true>), dim3(wg), dim3(256), 0, s, ev_start, ev_stop, 0, list, count, g, b, c, rhs, src,
+                          dst);
+  } else if (src && fine) {
     FX_LAUNCH((k_sweep<false, true>), wg, 256, s, list, count, g, b, c, rhs, src, dst);
   } else if (src) {
     FX_LAUNCH((k_sweep<false, false>), wg, 256, s, list, count, g, b, c, rhs, src, dst);
