@@ -333,8 +333,11 @@ struct Src {
 // Sine and cosine for the smooth procedural fields (thermal noise, buoyancy
 // shear): two-part Cody-Waite reduction by 2*pi, then the hardware
 // v_sin_f32 / v_cos_f32 (argument in revolutions). The arguments stay below a
-// few thousand radians, where this keeps the error near 1e-6; hash31 below keeps
-// the correctly rounded sinf because it scales the result by 43758.
+// few thousand radians, where this keeps the error near 1e-6. hash31 below
+// scales its sine by 43758 and takes the fraction, so a one-ulp difference in
+// the sine moves the hash by ~3e-3: it takes the double-precision sine rounded
+// to float (the correctly rounded float sine, barring double-rounding ties of
+// probability ~2^-29), which the oracle computes the same way.
 __device__ __forceinline__ float rev2pi(float x) {
   const float k = rintf(x * 0.15915494309189535f);
   float r = fmaf(-k, 6.28318548202514648f, x);
@@ -343,7 +346,10 @@ __device__ __forceinline__ float rev2pi(float x) {
 }
 __device__ __forceinline__ float sin_s(float x) { return __builtin_amdgcn_sinf(rev2pi(x)); }
 __device__ __forceinline__ float cos_s(float x) { return __builtin_amdgcn_cosf(rev2pi(x)); }
-__device__ __forceinline__ float hash31(F3 p) { return fractf(sinf(dot3(p, f3(127.1f, 311.7f, 74.7f))) * 43758.5453123f); }
+__device__ __forceinline__ float hash31(F3 p) {
+  const float s = float(sin(double(dot3(p, f3(127.1f, 311.7f, 74.7f)))));
+  return fractf(s * 43758.5453123f);
+}
 __device__ __noinline__ float value_noise3(F3 p) {
   F3 c = f3(floorf(p.x), floorf(p.y), floorf(p.z));
   F3 f = f3(fractf(p.x), fractf(p.y), fractf(p.z));

@@ -208,8 +208,11 @@ template <class T> struct Params {
 
 template <class T> struct Emit {
   const Params<T>& P;
-  // SH/inject-source-common.wgsl.ts:102-121
-  static T hash31(V3<T> p) { return fractT(std::sin(dot3(p, V3<T>{T(127.1), T(311.7), T(74.7)})) * T(43758.5453123)); }
+  // SH/inject-source-common.wgsl.ts:102-121. The sine is the correctly rounded one of
+  // the build's type: for float, the double sine rounded to float (WGSL leaves f32 sin
+  // of large arguments to the implementation; the kernels compute it the same way).
+  static T sinCR(T x) { return T(std::sin(double(x))); }
+  static T hash31(V3<T> p) { return fractT(sinCR(dot3(p, V3<T>{T(127.1), T(311.7), T(74.7)})) * T(43758.5453123)); }
   static T valueNoise3(V3<T> p) {
     V3<T> c{std::floor(p.x), std::floor(p.y), std::floor(p.z)};
     V3<T> f{fractT(p.x), fractT(p.y), fractT(p.z)};
@@ -1550,6 +1553,8 @@ template <class T> int Sim<T>::raymarch(const fx_camera* cam, const fx_render_pa
 //   fn 3 light_dir(in[0] kind, in[1..3] dir, in[4..6] pos, in[7..9] world position) -> out[0..2]
 //   fn 4 light_att(in[0] kind, in[1..3] pos, in[4..6] world position) -> out[0]
 //   fn 5 light_power(in[0] intensity, in[1] attenuation) -> out[0]
+//   fn 6 hash31(in[0..2]) -> out[0]
+//   fn 7 emitterNoise(in[0..2] position, in[3] scale, in[4] seed) -> out[0]
 namespace orc {
 template <class T> int probe_t(int fn, const double* in, double* out) {
   using orc::V3;
@@ -1562,6 +1567,8 @@ template <class T> int probe_t(int fn, const double* in, double* out) {
     case 3: put3(orc::light_dir(T(in[0]), v3(1), v3(4), v3(7))); return 0;
     case 4: out[0] = double(orc::light_att(T(in[0]), v3(1), v3(4))); return 0;
     case 5: out[0] = double(orc::light_power(T(in[0]), T(in[1]))); return 0;
+    case 6: out[0] = double(orc::Emit<T>::hash31(v3(0))); return 0;
+    case 7: out[0] = double(orc::Emit<T>::emitterNoise(v3(0), T(in[3]), T(in[4]))); return 0;
     default: return FX_ERR_INVALID_ARG;
   }
 }

@@ -502,3 +502,28 @@ def test_raymarch_secondary_lights_skip_primary_transmittance():
     np.testing.assert_array_equal(i0[..., 3], i1[..., 3])
     assert np.all(i0[..., :3] <= i1[..., :3] + 1e-7)
     assert (i1[..., :3] - i0[..., :3]).max() > 1e-3
+
+
+# ------------------------------------------------------------- hash noise --
+def test_hash_noise_uses_the_correctly_rounded_float_sine():
+    """SH/inject-source-common.wgsl.ts:102-121: hash31(p) = fract(sin(dot(p, (127.1, 311.7, 74.7)))
+    * 43758.5453123) in f32. WGSL leaves the precision of f32 sin at these arguments (hundreds to
+    thousands of radians) to the implementation, and the factor 43758 turns a one-ulp difference of
+    the sine into ~3e-3 of hash; both the oracle and the kernels take the double sine rounded to
+    float. Checked bit for bit against that definition (f32 dot product in WGSL order, no FMA);
+    value noise equals the hash at lattice points (u = 0) and stays in [0, 1).
+    Planted mistakes: the f32 library sine, the dot product in double, a missing fract."""
+    from oracle import PROBE_EMITTER_NOISE, PROBE_HASH31
+    rng = np.random.default_rng(12)
+    pts = np.round(rng.uniform(-40, 40, (400, 3)), 2).astype(np.float32)
+    for p in pts:
+        d = np.float32(np.float32(p[0] * np.float32(127.1)) + np.float32(p[1] * np.float32(311.7)))
+        d = np.float32(d + np.float32(p[2] * np.float32(74.7)))
+        s = np.float32(np.sin(np.float64(d)))
+        v = np.float32(s * np.float32(43758.5453123))
+        want = np.float32(v - np.floor(v))
+        got = probe(PROBE_HASH31, *p, bits=32)
+        assert got == float(want), (p, got, want)
+    for c in np.floor(pts[:50]):
+        n = probe(PROBE_EMITTER_NOISE, *c, 1.0, 0.0, bits=32)  # scale 1, seed 0: the lattice point itself
+        assert n == probe(PROBE_HASH31, *c, bits=32) and 0.0 <= n < 1.0
