@@ -73,3 +73,21 @@ def build_oracle(verbose=False):
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+
+
+def build_node_addon(verbose=False):
+    """Builds the Node-API addon of the JS host (fenix_amd/js/fenix_napi.node) with the
+    system Node headers, linked against libfenix.so (rpath: the package directory).
+    Returns the path, or None when no Node headers are installed."""
+    inc = next((d for d in ("/usr/include/node", "/usr/local/include/node") if os.path.exists(os.path.join(d, "node_api.h"))), None)
+    if inc is None:
+        return None
+    out = os.path.join(HERE, "js", "fenix_napi.node")
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall", f"-I{inc}", "-DNODE_GYP_MODULE_NAME=fenix_napi",
+           os.path.join(HERE, "js", "fenix_napi.cpp"), "-o", out + ".tmp", f"-L{HERE}", "-lfenix",
+           "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
+    return out

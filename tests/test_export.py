@@ -61,3 +61,50 @@ def test_export_frames_are_deterministic_and_match_manual_frames():
     # starting at frame 3: the first exported frame is simulation time 0 again
     c = list(export_frames(sim, rp, cam, 3, 4, 30.0, 48, 40, bloom=bloom, bloom_params=bp))
     np.testing.assert_array_equal(c[0], a[0])
+
+
+@pytest.mark.gpu
+def test_export_frames_match_the_oracle_chain():
+    """renderOffscreenFrame (createViewportRuntime.ts:205-232) per exported frame: step at
+    (start + i) / rate, raymarch into the HDR image, bloom/composite into the canvas. The GPU
+    frames follow the same chain run on the CPU (FP32 oracle step and raymarch, rounded to the
+    rgba16float HDR image, then the bloom restatement), free-running from the reset: canvases
+    within 2 of 255 everywhere and within 1 on >= 99.9% of the channel values."""
+    import scenes
+    from oracle import Oracle, bloom_model
+    from fenix_amd import abi
+    from fenix_amd.render import BloomPass, VolumeRaymarchPass, camera_struct, orbit_camera_snapshot
+    from fenix_amd.scene import load_scene
+    from fenix_amd.simulation import CombustionVolumeSimulation, make_desc
+    import ctypes as C
+    with open(os.path.join(HERE, "golden", "scenes", "twin_fires_v3.json")) as fh:
+        sc = load_scene(fh.read())
+    n, w, h, rate = 32, 48, 40, 30.0
+    desc = make_desc((n, n, n))
+    sim = CombustionVolumeSimulation(desc)
+    rp = VolumeRaymarchPass((n, n, n))
+    sc.apply(sim, rp)
+    orc = Oracle(desc, 32)
+    orc.update_sources(*scenes.packed(sc.sources))
+    rt, q = abi.RuntimeParams(), abi.Quality()
+    assert sim._lib.fx_get_runtime_params(sim._h, C.byref(rt)) == abi.FX_OK
+    assert sim._lib.fx_get_quality(sim._h, C.byref(q)) == abi.FX_OK
+    orc.set_runtime(rt, abi.FX_RP_ALL)
+    orc.set_quality(q)
+    cam = orbit_camera_snapshot()
+    bloom, bm = BloomPass(), bloom_model()
+    enabled, thr, strength, radius = sc.bloom()
+    frames = list(export_frames(sim, rp, cam, 2, 9, rate, w, h, bloom=bloom,
+                                bloom_params=bloom.params_struct(enabled, thr, strength, radius)))
+    orc.reset()
+    lit = 0
+    for i, frame in enumerate(frames):
+        t = (2 + i) / rate
+        orc.step(t, 1.0 / rate)
+        hdr = orc.raymarch(camera_struct(cam), rp.params_struct("temperature", t), w, h).astype(np.float16)
+        ref = bm.bloom(hdr, enabled=enabled, threshold_=thr, strength=strength, radius=radius)
+        diff = np.abs(frame.astype(np.int32) - ref.astype(np.int32))
+        assert diff.max() <= 2, (i, diff.max())
+        assert np.mean(diff <= 1) >= 0.999, (i, np.mean(diff <= 1))
+        lit += int(frame[..., 3].max() > 0)
+    assert lit >= 4, "the exported frames show nothing"
