@@ -29,6 +29,8 @@ FX_SOLVER_VCYCLE = 1
 FX_PRECISION_FP32 = 0
 FX_PRECISION_BF16 = 1
 FX_PRECISION_FP16 = 2
+FX_PRECISION_BF16_RNE = 3
+FX_PRECISION_FP16_RNE = 4
 
 FIELDS = {
     "density": 0, "temperature": 1, "fuel": 2, "reaction": 3,

@@ -61,6 +61,7 @@ struct fx_sim {
   int cus = 1;             // compute units of the device
   bool blocks_env = false; // FENIX_SIM_BLOCKS_PER_CU set
   bool sparse = false;
+  bool stochastic = false; // 16-bit stores round stochastically (FX_PRECISION_BF16/FP16)
 
   void* S[2][4] = {};               // d, t, f in the storage precision; r FP32
   float* u[3] = {};
@@ -308,7 +309,8 @@ int upload_step(fx_sim* s, const fx_sim_params& p, const std::vector<uint32_t>* 
   c.omega = s->desc.vcycle.omega;
   c.inj_count = inj_count;
   c.cand_count = cand ? uint32_t(cand->size()) : 0u;
-  c.pad[0] = c.pad[1] = 0;
+  c.step_key = uint32_t(s->stepIndex);
+  c.sr = s->stochastic ? 1u : 0u;
   size_t bytes = sizeof(StepConst);
   if (cand && !cand->empty()) {
     std::memcpy(slot.cand, cand->data(), cand->size() * 4);
@@ -872,7 +874,7 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
     return fail(nullptr, FX_ERR_INVALID_ARG, "resolution must be at least 4 in every axis");
   if (uint64_t(desc->width) * desc->height * desc->depth > (1ull << 31))
     return fail(nullptr, FX_ERR_INVALID_ARG, "resolution exceeds 2^31 voxels");
-  if (desc->precision != FX_PRECISION_FP32 && desc->precision != FX_PRECISION_BF16 && desc->precision != FX_PRECISION_FP16)
+  if (desc->precision < FX_PRECISION_FP32 || desc->precision > FX_PRECISION_FP16_RNE)
     return fail(nullptr, FX_ERR_INVALID_ARG, "unknown precision");
   if (desc->solver == FX_SOLVER_VCYCLE && (desc->vcycle.levels < 2 || desc->vcycle.levels > FX_MAX_LEVELS_DEV))
     return fail(nullptr, FX_ERR_INVALID_ARG, "vcycle.levels must be in [2, 8]");
@@ -885,6 +887,10 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
   fx_sim* s = new (std::nothrow) fx_sim();
   if (!s) return fail(nullptr, FX_ERR_OUT_OF_MEMORY, "host allocation failed");
   s->desc = *desc;
+  // storage type in desc.precision; the rounding mode apart
+  s->stochastic = desc->precision == FX_PRECISION_BF16 || desc->precision == FX_PRECISION_FP16;
+  if (desc->precision == FX_PRECISION_BF16_RNE) s->desc.precision = FX_PRECISION_BF16;
+  if (desc->precision == FX_PRECISION_FP16_RNE) s->desc.precision = FX_PRECISION_FP16;
   s->device = desc->device;
   s->sparse = desc->domain_mode == FX_DOMAIN_SPARSE;
   s->rt = desc->runtime;

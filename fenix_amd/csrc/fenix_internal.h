@@ -39,7 +39,8 @@ struct StepConst {
   float omega;            // V-cycle weighted-Jacobi factor
   uint32_t inj_count;     // sparse: bricks visited by injection (head of the candidate list)
   uint32_t cand_count;    // sparse: injection bricks + bricks written by the host
-  uint32_t pad[2];
+  uint32_t step_key;      // steps completed: the step term of the stochastic-rounding hash
+  uint32_t sr;            // 16-bit stores round stochastically (1) or to nearest even (0)
 };
 
 // Brick list of one level: ascending brick indices, device count, page table.

@@ -137,8 +137,41 @@ __device__ __forceinline__ float tri_eval(const Tri& t, const float* __restrict_
 // Storage of density, temperature and fuel: FP32, BF16 (round to nearest even)
 // or FP16. Every load widens to FP32; all arithmetic is FP32.
 // ld4/st4: elements i .. i+3 (i a multiple of 4) as one 16- or 8-byte access.
+// Stochastic rounding of 16-bit stores (StepConst::sr): a value between two
+// representable neighbours is stored as the upper one with probability equal to
+// its distance from the lower one in ulps, so the stored value is unbiased and
+// per-step increments below half an ulp survive on average. The decision compares
+// the dropped low bits with bits of a 32-bit hash of (brick-major voxel index,
+// step, pass, field) (MurmurHash3's fmix32 finalizer), so it is reproducible.
+__host__ __device__ __forceinline__ uint32_t fmix32(uint32_t x) {
+  x ^= x >> 16; x *= 0x85ebca6bu; x ^= x >> 13; x *= 0xc2b2ae35u; x ^= x >> 16;
+  return x;
+}
+// Per-site hash term: site = 4 * pass + field (pass 0 scalar injection, 1 igniter,
+// 2 scalar advection, 3 combustion; field 0 density, 1 temperature, 2 fuel).
+__device__ __forceinline__ uint32_t sr_site(const StepConst* C, uint32_t site) { return fmix32(C->step_key * 16u + site); }
+__device__ __forceinline__ uint32_t sr_hash(uint32_t i, uint32_t site_term) { return fmix32((i * 0x9e3779b1u) ^ site_term); }
+__device__ __forceinline__ uint16_t bf16_sr(float v, uint32_t h) {
+  const uint32_t u = __float_as_uint(v);
+  return uint16_t((u >> 16) + ((h & 0xffffu) < (u & 0xffffu) ? 1u : 0u));
+}
+__device__ __forceinline__ uint16_t f16_sr(float v, uint32_t h) {
+  const uint32_t u = __float_as_uint(v), sgn = (u >> 16) & 0x8000u, a = u & 0x7fffffffu;
+  if (a >= 0x477ff000u) return __half_as_ushort(__float2half(v));  // beyond the largest half: nearest
+  const uint32_t e = a >> 23;
+  if (e >= 113u) {  // normal half: 13 dropped bits
+    const uint32_t hb = ((e - 112u) << 10) | ((a >> 13) & 0x3ffu);
+    return uint16_t(sgn | (hb + ((h & 0x1fffu) < (a & 0x1fffu) ? 1u : 0u)));
+  }
+  const uint32_t d = 126u - e;  // subnormal half (unit 2^-24): 14 or more dropped bits
+  if (d >= 32u) return uint16_t(sgn);
+  const uint32_t m = (a & 0x7fffffu) | 0x800000u;
+  return uint16_t(sgn | ((m >> d) + ((h >> (32u - d)) < (m & ((1u << d) - 1u)) ? 1u : 0u)));
+}
+
 struct StF32 {
   using T = float;
+  static __device__ __forceinline__ void st_sr(T* p, uint32_t i, float v, uint32_t, uint32_t) { p[i] = v; }
   static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return p[i]; }
   static __device__ __forceinline__ void st(T* p, uint32_t i, float v) { p[i] = v; }
   static __device__ __forceinline__ float4 ld4(const T* p, uint32_t i) { return *reinterpret_cast<const float4*>(p + i); }
@@ -153,6 +186,10 @@ struct StBF16 {
   }
   static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return __uint_as_float(uint32_t(p[i]) << 16); }
   static __device__ __forceinline__ void st(T* p, uint32_t i, float v) { p[i] = enc(v); }
+  // sr != 0: stochastic rounding with the hash term of the store site
+  static __device__ __forceinline__ void st_sr(T* p, uint32_t i, float v, uint32_t sr, uint32_t site) {
+    p[i] = sr ? bf16_sr(v, sr_hash(i, site)) : enc(v);
+  }
   static __device__ __forceinline__ float4 ld4(const T* p, uint32_t i) {
     const uint2 w = *reinterpret_cast<const uint2*>(p + i);
     return make_float4(__uint_as_float(w.x << 16), __uint_as_float(w.x & 0xffff0000u), __uint_as_float(w.y << 16),
@@ -167,6 +204,9 @@ struct StF16 {
   using T = __half;
   static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return __half2float(p[i]); }
   static __device__ __forceinline__ void st(T* p, uint32_t i, float v) { p[i] = __float2half(v); }
+  static __device__ __forceinline__ void st_sr(T* p, uint32_t i, float v, uint32_t sr, uint32_t site) {
+    p[i] = sr ? __ushort_as_half(f16_sr(v, sr_hash(i, site))) : __float2half(v);
+  }
   static __device__ __forceinline__ float4 ld4(const T* p, uint32_t i) {
     const uint2 w = *reinterpret_cast<const uint2*>(p + i);
     return make_float4(__half2float(__ushort_as_half(uint16_t(w.x))), __half2float(__ushort_as_half(uint16_t(w.x >> 16))),
@@ -408,6 +448,8 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
                                                 SS<ST> S, Vec3 u, uint32_t* touched, int parts) {
   const fx_sim_params& P = C->p;
   const uint32_t ns = C->nsrc;
+  const uint32_t sr = C->sr;
+  const uint32_t s0 = sr_site(C, 0), s1 = sr_site(C, 1), s2 = sr_site(C, 2), s5 = sr_site(C, 5);
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
     if (touched && threadIdx.x == 0) touched[b] = 1u;
@@ -435,7 +477,7 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
         tem = clamp01(tem + s.yt() * contribution);
         fu = clamp01(fu + s.yf() * contribution);
       }
-      ST::st(S.d, i, den); ST::st(S.t, i, tem); ST::st(S.f, i, fu);
+      ST::st_sr(S.d, i, den, sr, s0); ST::st_sr(S.t, i, tem, sr, s1); ST::st_sr(S.f, i, fu, sr, s2);
     }
     if (parts & 2) {
       F3 vel = f3(u.x[i], u.y[i], u.z[i]);
@@ -494,7 +536,7 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
         rea = clamp01(rea + seedv);
         tem = clamp01(tem + intensity * weight * P.delta_time * 4.f);
       }
-      S.r[i] = rea; ST::st(S.t, i, tem);
+      S.r[i] = rea; ST::st_sr(S.t, i, tem, sr, s5);
     }
   }
 }
@@ -802,7 +844,7 @@ __device__ __forceinline__ float thermal_noise(F3 pos, float time) {
 // measured 2-4 us slower.
 template <class ST>
 __device__ __forceinline__ void decay_write(const Grid& g, uint32_t i, uint32_t y, float den, float tem, float fu, float rea,
-                                            float tn, float dt, SS<ST> D) {
+                                            float tn, float dt, SS<ST> D, uint32_t sr, const uint32_t* site) {
   float ny = (float(y) + 0.5f) / float(g.H);
   float hot = smoothf(0.22f, 0.78f, tem + fu * 0.2f + rea * 0.26f);
   float coolingPocket = smoothf(0.44f, 0.9f, 1.f - tn) * smoothf(0.08f, 0.74f, den) * (1.f - hot * 0.72f);
@@ -810,9 +852,9 @@ __device__ __forceinline__ void decay_write(const Grid& g, uint32_t i, uint32_t 
   float smokeLoss = dt * (0.008f + ny * 0.026f) * mixf(0.75f, 1.38f, tn);
   float coldBreakup = dt * den * (1.f - smoothf(0.2f, 0.86f, hot)) * smoothf(0.18f, 0.72f, den) * smoothf(0.34f, 0.92f, tn) * 0.34f;
   float clearing = dt * den * hot * mixf(0.1f, 0.72f, tn);
-  ST::st(D.d, i, clamp01(fmaxf(0.f, den * (1.f - smokeLoss) - clearing - coldBreakup)));
-  ST::st(D.t, i, clamp01(fmaxf(0.f, tem - cooling)));
-  ST::st(D.f, i, clamp01(fmaxf(0.f, fu - dt * mixf(0.012f, 0.034f, tn))));
+  ST::st_sr(D.d, i, clamp01(fmaxf(0.f, den * (1.f - smokeLoss) - clearing - coldBreakup)), sr, site[0]);
+  ST::st_sr(D.t, i, clamp01(fmaxf(0.f, tem - cooling)), sr, site[1]);
+  ST::st_sr(D.f, i, clamp01(fmaxf(0.f, fu - dt * mixf(0.012f, 0.034f, tn))), sr, site[2]);
   D.r[i] = clamp01(fmaxf(0.f, rea - dt * mixf(0.46f, 0.82f, tn)));
 }
 // Gather mirrors for the scalar advection: the four scalars (d, T, f, r) and the
@@ -858,6 +900,8 @@ __global__ __launch_bounds__(512) void k_advect_scalars(const uint32_t* __restri
                                                         const StepConst* __restrict__ C, int mode, const float4* __restrict__ sq,
                                                         const float4* __restrict__ uq, SS<ST> D, const uint32_t* __restrict__ flags) {
   const float dt = C->p.delta_time, time = C->p.time;
+  const uint32_t sr = C->sr;
+  const uint32_t site[3] = {sr_site(C, 8), sr_site(C, 9), sr_site(C, 10)};
   FX_BRICK_LOOP(list, count) {
     Vox v = vox_of(g, list[fx_k]);
     if (!v.in) continue;
@@ -872,7 +916,7 @@ __global__ __launch_bounds__(512) void k_advect_scalars(const uint32_t* __restri
     Tri tp = tri(g, sp);
     if (mode == FX_ADVECT_SEMI_LAGRANGIAN) {
       const float4 s4 = tri_eval4(tp, sq);
-      decay_write<ST>(g, i, v.y, s4.x, s4.y, s4.z, s4.w, thermal_noise(sp, time), dt, D);
+      decay_write<ST>(g, i, v.y, s4.x, s4.y, s4.z, s4.w, thermal_noise(sp, time), dt, D, sr, site);
       continue;
     }
     const float4 f4 = tri_eval4(tp, uq);
@@ -889,7 +933,7 @@ __global__ __launch_bounds__(512) void k_advect_scalars(const uint32_t* __restri
     out[1] = mac(pr.y, rc.y, og.y);
     out[2] = mac(pr.z, rc.z, og.z);
     out[3] = mac(pr.w, rc.w, og.w);
-    decay_write<ST>(g, i, v.y, out[0], out[1], out[2], out[3], thermal_noise(coord, time), dt, D);
+    decay_write<ST>(g, i, v.y, out[0], out[1], out[2], out[3], thermal_noise(coord, time), dt, D, sr, site);
   }
 }
 
@@ -909,6 +953,7 @@ __global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ flags) {
   const float dt = C->p.delta_time, time = C->p.time;
   const float ilo = C->ignition_lo, ihi = C->ignition_hi, rate = C->burn_rate;
+  const uint32_t sr = C->sr, sd = sr_site(C, 12), st = sr_site(C, 13), sf = sr_site(C, 14);
   FX_BRICK_LOOP(list, count) {
     Vox v = vox_of(g, list[fx_k]);
     if (!v.in) continue;
@@ -937,9 +982,9 @@ __global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__
     float clearing = den * smoothk(0.18f, 0.82f, tem + rea * 0.22f + fu * 0.1f) * dt * mixf(0.55f, 2.45f, hotPocket);
     float coolingSmoke = den * tem * (coolingPocket + hotSoot * 0.32f) * dt * 0.16f;
     float reactionDecay = rea * (1.f - dt * mixf(0.68f, 0.22f, hotPocket));
-    ST::st(D.f, i, clamp01(fu - burn));
-    ST::st(D.t, i, clamp01(tem + burn * mixf(2.1f, 3.25f, hotPocket) + radiative - heatLoss));
-    ST::st(D.d, i, clamp01(fmaxf(0.f, den + smokeYield + coolingSmoke - densityLoss - clearing * 0.72f)));
+    ST::st_sr(D.f, i, clamp01(fu - burn), sr, sf);
+    ST::st_sr(D.t, i, clamp01(tem + burn * mixf(2.1f, 3.25f, hotPocket) + radiative - heatLoss), sr, st);
+    ST::st_sr(D.d, i, clamp01(fmaxf(0.f, den + smokeYield + coolingSmoke - densityLoss - clearing * 0.72f)), sr, sd);
     D.r[i] = clamp01(fmaxf(reactionDecay, burn * mixf(4.f, 7.5f, hotPocket) + radiative * 1.8f));
   }
 }

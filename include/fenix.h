@@ -58,10 +58,16 @@ extern "C" {
 #define FX_SOLVER_VCYCLE 1
 
 /* Storage precision of density, temperature and fuel (reaction, velocity and
- * pressure always FP32; all arithmetic FP32). */
+ * pressure always FP32; all arithmetic FP32). BF16/FP16 round every value a pass
+ * stores stochastically: up with probability equal to the dropped fraction of an
+ * ulp, decided by an integer hash of (voxel, step, pass, field), so the rounding is
+ * unbiased and reproducible; the _RNE variants round to nearest even. Values
+ * uploaded by fx_write_field round to nearest even in every mode. */
 #define FX_PRECISION_FP32 0
 #define FX_PRECISION_BF16 1
 #define FX_PRECISION_FP16 2
+#define FX_PRECISION_BF16_RNE 3
+#define FX_PRECISION_FP16_RNE 4
 
 /* Field ids for fx_read_field / fx_write_field / fx_field_device_ptr. */
 #define FX_FIELD_DENSITY 0          /* render set (getRenderBuffers) */

@@ -103,6 +103,45 @@ inline float round_f16(float f) {
   return std::copysign(std::nearbyint(a / ulp) * ulp, f);
 }
 
+// Stochastic rounding (FX_PRECISION_BF16 / FP16): the stored value is the upper
+// representable neighbour with probability (dropped fraction of an ulp); the
+// decision compares the dropped low bits of the f32 value with bits of
+// fmix32(i * 0x9e3779b1 ^ fmix32(step * 16 + site)), i the brick-major index of the
+// voxel and site = 4 * pass + field (pass 0 scalar injection, 1 igniter,
+// 2 scalar advection, 3 combustion). fmix32 is MurmurHash3's 32-bit finalizer.
+inline uint32_t fmix32(uint32_t x) {
+  x ^= x >> 16; x *= 0x85ebca6bu; x ^= x >> 13; x *= 0xc2b2ae35u; x ^= x >> 16;
+  return x;
+}
+inline uint32_t sr_hash(uint32_t i, uint32_t step, uint32_t site) { return fmix32((i * 0x9e3779b1u) ^ fmix32(step * 16u + site)); }
+inline float sr_bf16(float f, uint32_t h) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u = (u & 0xffff0000u) + ((h & 0xffffu) < (u & 0xffffu) ? 0x10000u : 0u);
+  float r;
+  std::memcpy(&r, &u, 4);
+  return r;
+}
+inline float sr_f16(float f, uint32_t h) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  const uint32_t a = u & 0x7fffffffu;
+  if (a >= 0x477ff000u) return round_f16(f);
+  const uint32_t e = a >> 23;
+  uint32_t hb;  // half bits without the sign
+  if (e >= 113u) {
+    hb = (((e - 112u) << 10) | ((a >> 13) & 0x3ffu)) + ((h & 0x1fffu) < (a & 0x1fffu) ? 1u : 0u);
+  } else {
+    const uint32_t d = 126u - e;
+    if (d >= 32u) return std::copysign(0.f, f);
+    const uint32_t m = (a & 0x7fffffu) | 0x800000u;
+    hb = (m >> d) + ((h >> (32u - d)) < (m & ((1u << d) - 1u)) ? 1u : 0u);
+  }
+  const uint32_t he = hb >> 10, hm = hb & 0x3ffu;  // decode the half
+  const float mag = he == 0 ? std::ldexp(float(hm), -24) : std::ldexp(float(1024u + hm), int(he) - 25);
+  return std::copysign(mag, f);
+}
+
 struct Dims {
   uint32_t W, H, D;
   size_t N() const { return size_t(W) * H * D; }
@@ -352,6 +391,7 @@ template <class T> struct Sim : Base {
   uint64_t stepIndex = 0;
   int lastPressure = 0;  // 0 = 'a', 1 = 'b'
   bool debugRequested = false;  // ORCH:191, set by getDebugBuffers (:316)
+  bool stochastic = false;      // 16-bit stores of the passes round stochastically
   int curSet = 0;        // the reference's currentScalarSet (params.scalarSetIndex only)
   std::vector<double> resid;
 
@@ -363,14 +403,38 @@ template <class T> struct Sim : Base {
     const bool bf = desc.precision == FX_PRECISION_BF16;
     for (auto& v : a) v = T(bf ? round_bf16(float(v)) : round_f16(float(v)));
   }
-  void round_set(int set, bool with_fuel = true) {
-    round_store(S[set][0]); round_store(S[set][1]);
-    if (with_fuel) round_store(S[set][2]);
+  // The stores of one pass: stochastic rounding with that pass's sites, or nearest even.
+  void round_set(int set, bool with_fuel, int pass) {
+    if (sizeof(T) != sizeof(float) || desc.precision == FX_PRECISION_FP32) return;
+    if (!stochastic) {
+      round_store(S[set][0]); round_store(S[set][1]);
+      if (with_fuel) round_store(S[set][2]);
+      return;
+    }
+    const bool bf = desc.precision == FX_PRECISION_BF16;
+    const uint32_t step = uint32_t(stepIndex);
+    for (int f = 0; f < (with_fuel ? 3 : 2); ++f) {
+      auto& a = S[set][f];
+      const uint32_t site = uint32_t(4 * pass + f);
+#pragma omp parallel for schedule(static)
+      for (int64_t z = 0; z < d.D; ++z)
+        for (uint32_t y = 0; y < d.H; ++y)
+          for (uint32_t x = 0; x < d.W; ++x) {
+            const size_t i = d.idx(x, y, uint32_t(z));
+            const uint32_t bm = uint32_t(brickOf(x, y, uint32_t(z)) * 512u + (uint32_t(z) & 7u) * 64u + (y & 7u) * 8u + (x & 7u));
+            const uint32_t h = sr_hash(bm, step, site);
+            a[i] = T(bf ? sr_bf16(float(a[i]), h) : sr_f16(float(a[i]), h));
+          }
+    }
   }
   size_t brickOf(uint32_t x, uint32_t y, uint32_t z) const { return (x >> 3) + BX * ((y >> 3) + size_t(BY) * (z >> 3)); }
   bool on(uint32_t x, uint32_t y, uint32_t z) const { return bact[brickOf(x, y, z)] != 0; }
 
   explicit Sim(const fx_create_desc& de) : desc(de) {
+    // storage type in desc.precision, the rounding mode apart (as libfenix does)
+    stochastic = de.precision == FX_PRECISION_BF16 || de.precision == FX_PRECISION_FP16;
+    if (de.precision == FX_PRECISION_BF16_RNE) desc.precision = FX_PRECISION_BF16;
+    if (de.precision == FX_PRECISION_FP16_RNE) desc.precision = FX_PRECISION_FP16;
     d = Dims{de.width, de.height, de.depth};
     BX = (d.W + 7) / 8; BY = (d.H + 7) / 8; BZ = (d.D + 7) / 8;  // createSparseBrickLayout.ts:11-26
     NB = size_t(BX) * BY * BZ;
@@ -493,7 +557,7 @@ template <class T> struct Sim : Base {
           }
           D_[i] = den; Tm[i] = tem; F[i] = fu;
         }
-    round_set(0);
+    round_set(0, true, 0);
   }
 
   // SH/inject-velocity.wgsl.ts:46-96
@@ -579,7 +643,7 @@ template <class T> struct Sim : Base {
           }
           R[i] = rea; Tm[i] = tem;
         }
-    round_set(0, false);
+    round_set(0, false, 1);
   }
 
   // ------------------------------------------------------ activation --
@@ -745,7 +809,7 @@ template <class T> struct Sim : Base {
             decay_write(i, y, vals[0], vals[1], vals[2], vals[3], thermalNoise(coord, P.time), P);
           }
         }
-    round_set(1);
+    round_set(1, true, 2);
   }
 
   // ------------------------------------------------------ combustion --
@@ -802,7 +866,7 @@ template <class T> struct Sim : Base {
           S[0][0][i] = clamp01(std::max(T(0), den + smokeYield + coolingSmoke - densityLoss - clearing * T(0.72)));
           S[0][3][i] = clamp01(std::max(reactionDecay, burn * mixT(T(4), T(7.5), hotPocket) + radiative * T(1.8)));
         }
-    round_set(0);
+    round_set(0, true, 3);
   }
 
   // -------------------------------------------------------- buoyancy --
@@ -1555,6 +1619,8 @@ template <class T> int Sim<T>::raymarch(const fx_camera* cam, const fx_render_pa
 //   fn 5 light_power(in[0] intensity, in[1] attenuation) -> out[0]
 //   fn 6 hash31(in[0..2]) -> out[0]
 //   fn 7 emitterNoise(in[0..2] position, in[3] scale, in[4] seed) -> out[0]
+//   fn 8 / 9 stochastic rounding to bf16 / f16 of in[0] with hash in[1] -> out[0]
+//   fn 10 the rounding hash of (in[0] voxel, in[1] step, in[2] site) -> out[0]
 namespace orc {
 template <class T> int probe_t(int fn, const double* in, double* out) {
   using orc::V3;
@@ -1569,6 +1635,9 @@ template <class T> int probe_t(int fn, const double* in, double* out) {
     case 5: out[0] = double(orc::light_power(T(in[0]), T(in[1]))); return 0;
     case 6: out[0] = double(orc::Emit<T>::hash31(v3(0))); return 0;
     case 7: out[0] = double(orc::Emit<T>::emitterNoise(v3(0), T(in[3]), T(in[4]))); return 0;
+    case 8: out[0] = double(orc::sr_bf16(float(in[0]), uint32_t(in[1]))); return 0;
+    case 9: out[0] = double(orc::sr_f16(float(in[0]), uint32_t(in[1]))); return 0;
+    case 10: out[0] = double(orc::sr_hash(uint32_t(in[0]), uint32_t(in[1]), uint32_t(in[2]))); return 0;
     default: return FX_ERR_INVALID_ARG;
   }
 }

@@ -527,3 +527,37 @@ def test_hash_noise_uses_the_correctly_rounded_float_sine():
     for c in np.floor(pts[:50]):
         n = probe(PROBE_EMITTER_NOISE, *c, 1.0, 0.0, bits=32)  # scale 1, seed 0: the lattice point itself
         assert n == probe(PROBE_HASH31, *c, bits=32) and 0.0 <= n < 1.0
+
+
+# --------------------------------------------------------- 16-bit storage --
+@pytest.mark.parametrize("fmt", ["bf16", "f16"])
+def test_stochastic_rounding_is_unbiased_and_brackets_the_value(fmt):
+    """The 16-bit stores of FX_PRECISION_BF16/FP16 round up with probability equal to the dropped
+    fraction of an ulp, the decision taken from the low bits of a hash (bf16: 16 bits, half: 13
+    bits for normal values, the top d bits for subnormal halves with d dropped bits). Over every
+    value of those hash bits the mean of the stored values equals the input exactly, each stored
+    value is one of the two representable neighbours, and representable inputs are unchanged.
+    Planted mistakes: '<=' for '<' (biased up by 1/2^k), rounding down only, a wrong shift."""
+    from oracle import PROBE_SR_BF16, PROBE_SR_F16
+    fn = PROBE_SR_BF16 if fmt == "bf16" else PROBE_SR_F16
+    rng = np.random.default_rng(7)
+    vals = list(np.float32(rng.uniform(0, 1, 4))) + [np.float32(3.1e-6), np.float32(4.5e-5)]  # two subnormal halves
+    for v in vals:
+        v64 = float(v)
+        if fmt == "bf16":
+            hs = np.arange(65536)
+        elif v64 >= 2.0 ** -14:
+            hs = np.arange(8192)
+        else:
+            e = int(np.frexp(v)[1]) - 1 + 127          # biased f32 exponent
+            dbits = 126 - e
+            hs = np.arange(1 << min(dbits, 16)) << (32 - min(dbits, 16))
+        out = np.array([probe(fn, v64, int(h), bits=32) for h in hs])
+        lo, hi = out.min(), out.max()
+        assert set(np.unique(out)) <= {lo, hi}
+        assert lo <= v64 <= hi
+        if fmt == "bf16" or v64 >= 2.0 ** -14 or dbits <= 16:
+            assert abs(out.mean() - v64) <= 1e-12 * max(v64, 1e-30), (v, out.mean())
+    for v in (0.5, 0.25 + 2.0 ** -9, 0.0):  # representable in both formats
+        for h in (0, 1, 0xffffffff, 0x12345678):
+            assert probe(fn, v, h, bits=32) == v

@@ -21,7 +21,9 @@ pytestmark = pytest.mark.gpu
 
 PERSISTENT = ("density", "temperature", "fuel", "reaction", "velocity_x", "velocity_y", "velocity_z")
 # One 16-bit ulp flip in a handful of voxels moves rel L2 by ~1e-5 (bf16: 2^-8 relative).
-TOL = {abi.FX_PRECISION_BF16: 2e-3, abi.FX_PRECISION_FP16: 3e-4}
+TOL = {abi.FX_PRECISION_BF16: 2e-3, abi.FX_PRECISION_FP16: 3e-4, abi.FX_PRECISION_BF16_RNE: 2e-3,
+       abi.FX_PRECISION_FP16_RNE: 3e-4}
+SIXTEEN = [abi.FX_PRECISION_BF16, abi.FX_PRECISION_FP16, abi.FX_PRECISION_BF16_RNE, abi.FX_PRECISION_FP16_RNE]
 
 
 def rel_l2(a, b):
@@ -41,7 +43,7 @@ def c4_sources(n, seed=4, count=4, radius_vox=None):
     return out
 
 
-@pytest.mark.parametrize("prec", [abi.FX_PRECISION_BF16, abi.FX_PRECISION_FP16])
+@pytest.mark.parametrize("prec", SIXTEEN)
 @pytest.mark.parametrize("pass_name", ["inject_scalar", "inject_igniter", "advect_scalars", "combustion", "buoyancy",
                                        "confinement"])
 def test_pass_parity_16bit(prec, pass_name):
@@ -69,7 +71,7 @@ def test_pass_parity_16bit(prec, pass_name):
         assert e <= TOL[prec], f"{pass_name}/{prec}: {name} rel L2 {e:.3e}"
 
 
-@pytest.mark.parametrize("prec", [abi.FX_PRECISION_BF16, abi.FX_PRECISION_FP16])
+@pytest.mark.parametrize("prec", SIXTEEN)
 def test_step_parity_16bit_sparse(prec):
     n = 48
     desc = scenes.c2_desc(n, precision=prec)
@@ -92,8 +94,9 @@ def test_step_parity_16bit_sparse(prec):
 
 def test_precision_report_against_fp64():
     """C4-style sparse scene (4 seeded emitters, config mode) for 100 steps:
-    per-field relative L2 and max error of FP32 / BF16 / FP16 storage against
-    the FP64 oracle. FP32 must stay within 1e-3 relative L2."""
+    per-field relative L2 and max error of FP32 / BF16 / FP16 storage (stochastic
+    rounding, and round-to-nearest for comparison) against the FP64 oracle. FP32 must
+    stay within 1e-3 relative L2."""
     n, steps = 64, 100
     srcs = c4_sources(n)
     ref = Oracle(scenes.c2_desc(n), 64)
@@ -103,7 +106,8 @@ def test_precision_report_against_fp64():
     truth = {f: ref.read(f) for f in PERSISTENT}
     report = {"scene": f"sparse {n}^3, 4 floor emitters (seed 4), config mode, {steps} steps",
               "active_bricks_fp64": ref.active_count(), "fields": {}}
-    for label, prec in (("fp32", abi.FX_PRECISION_FP32), ("bf16", abi.FX_PRECISION_BF16), ("fp16", abi.FX_PRECISION_FP16)):
+    for label, prec in (("fp32", abi.FX_PRECISION_FP32), ("bf16", abi.FX_PRECISION_BF16), ("fp16", abi.FX_PRECISION_FP16),
+                        ("bf16_rne", abi.FX_PRECISION_BF16_RNE), ("fp16_rne", abi.FX_PRECISION_FP16_RNE)):
         sim = CombustionVolumeSimulation(scenes.c2_desc(n, precision=prec))
         sim.updateSources(srcs)
         for k in range(steps):
@@ -123,7 +127,7 @@ def test_precision_report_against_fp64():
         json.dump(report, fh, indent=1)
     for f in ("density", "temperature", "fuel", "velocity_y"):
         assert report["fields"]["fp32"][f]["rel_l2"] <= 1e-3, (f, report["fields"]["fp32"][f])
-    for label in ("bf16", "fp16"):
+    for label in ("bf16", "fp16", "bf16_rne", "fp16_rne"):
         for f in PERSISTENT:
             assert np.isfinite(report["fields"][label][f]["rel_l2"])
             assert report["fields"][label][f]["rel_l2"] < 0.2, (label, f, report["fields"][label][f])
