@@ -523,6 +523,15 @@ int solve_vcycle(fx_sim* s) {
       const bool zero = l > 0 || c == 0;
       if (zero) cur[l] = 0;
       const bool split = (s->split_down >> l) & 1u;
+      if (split && ((s->split_down >> (kPlanFuseShift + l)) & 1u) && v.pre_sweeps >= 2) {
+        // split leg whose last pre-sweep is fused with the residual and the restriction
+        smoothW(l, v.pre_sweeps - 1, zero, (s->split_down >> (kPlanPairShift + l)) & 1u);
+        if (launch_vdown(s->stream, grid_for(s, V.nb), 1, level_list(s, l), level_count(s, l), V.g, bits_of(s, l), W.g,
+                         mask_of(s, l + 1), s->d_const, V.div, V.p[cur[l]], V.p[1 - cur[l]], W.div, norm)) {
+          cur[l] = 1 - cur[l];
+          continue;
+        }
+      }
       if (!split && launch_vdown(s->stream, grid_for(s, V.nb), v.pre_sweeps, level_list(s, l), level_count(s, l), V.g,
                                  bits_of(s, l), W.g, mask_of(s, l + 1), s->d_const, V.div, zero ? nullptr : V.p[cur[l]],
                                  V.p[1 - cur[l]], W.div, norm)) {
@@ -543,6 +552,14 @@ int solve_vcycle(fx_sim* s) {
       const uint32_t* lst = level_list(s, l);
       const uint32_t* cnt = level_count(s, l);
       const float* e = s->lv[l + 1].p[cur[l + 1]];
+      if (((s->split_up >> l) & 1u) && ((s->split_up >> (kPlanFuseShift + l)) & 1u) && v.post_sweeps >= 1 &&
+          launch_vup(s->stream, grid_for(s, V.nb), 1, lst, cnt, s->lv[l + 1].g, V.g, bits_of(s, l), s->d_const, e, V.div,
+                     V.p[cur[l]], V.p[1 - cur[l]])) {
+        // split leg whose prolongation is fused with the first post-sweep, then lean sweeps
+        cur[l] = 1 - cur[l];
+        smoothW(l, v.post_sweeps - 1, false, (s->split_up >> (kPlanPairShift + l)) & 1u);
+        continue;
+      }
       if ((s->split_up >> l) & 1u) {
         // split leg: in-place prolongation, then lean sweeps
         launch_prolong_add(s->stream, grid_for(s, V.nb), lst, cnt, s->lv[l + 1].g, V.g, bits_of(s, l), e, V.p[cur[l]]);
@@ -1652,30 +1669,37 @@ int fx_autotune_vcycle(fx_sim* s, uint32_t repeats, uint32_t* down_split, uint32
     for (int w = 0; w < 3 && rc == FX_OK; ++w) rc = time_plan(&ms);
   }
   // Per leg: 0 fused LDS kernel, 1 split with one sweep per launch, 2 split with
-  // two sweeps per launch (only where the leg has two or more sweeps).
+  // two sweeps per launch (only where the leg has two or more sweeps), 3 split with
+  // the sweep next to the level change fused with the restriction (down) or the
+  // prolongation (up) in one LDS kernel (down: two or more sweeps; up: one or more).
   auto set_leg = [&](uint32_t& mask, int l, uint32_t opt) {
-    mask &= ~((1u << l) | (1u << (kPlanPairShift + l)));
+    mask &= ~((1u << l) | (1u << (kPlanPairShift + l)) | (1u << (kPlanFuseShift + l)));
     if (opt >= 1) mask |= 1u << l;
     if (opt == 2) mask |= 1u << (kPlanPairShift + l);
+    if (opt == 3) mask |= 1u << (kPlanFuseShift + l);
   };
   const fx_vcycle& vc = s->desc.vcycle;
-  const uint32_t nd = vc.pre_sweeps >= 2 ? 3u : 2u, nu = vc.post_sweeps >= 2 ? 3u : 2u;
+  std::vector<uint32_t> downs = {0u, 1u}, ups = {0u, 1u};
+  if (vc.pre_sweeps >= 2) { downs.push_back(2u); downs.push_back(3u); }
+  if (vc.post_sweeps >= 2) ups.push_back(2u);
+  if (vc.post_sweeps >= 1) ups.push_back(3u);
+  const uint32_t np = uint32_t(downs.size() * ups.size());
   for (int l = 0; l + 1 < L && rc == FX_OK; ++l) {
     // each plan's best of two interleaved rounds
-    float t[9] = {};
+    std::vector<float> t(np, 0.f);
     for (int round = 0; round < 2 && rc == FX_OK; ++round)
-      for (uint32_t c = 0; c < nd * nu && rc == FX_OK; ++c) {
-        set_leg(s->split_down, l, c % nd);
-        set_leg(s->split_up, l, c / nd);
+      for (uint32_t c = 0; c < np && rc == FX_OK; ++c) {
+        set_leg(s->split_down, l, downs[c % downs.size()]);
+        set_leg(s->split_up, l, ups[c / downs.size()]);
         float ms = 0.f;
         rc = time_plan(&ms);
         if (rc == FX_OK && (round == 0 || ms < t[c])) t[c] = ms;
       }
     uint32_t best_c = 0;
-    for (uint32_t c = 1; c < nd * nu; ++c)
+    for (uint32_t c = 1; c < np; ++c)
       if (t[c] < t[best_c]) best_c = c;
-    set_leg(s->split_down, l, best_c % nd);
-    set_leg(s->split_up, l, best_c / nd);
+    set_leg(s->split_down, l, downs[best_c % downs.size()]);
+    set_leg(s->split_up, l, ups[best_c / downs.size()]);
   }
   // leave the pressure buffers as the chosen plan computes them
   float ms = 0.f;

@@ -158,6 +158,10 @@ void launch_sweep_pair(hipStream_t s, int blocks, const uint32_t* list, const ui
 // Bits 8.. of a V-cycle plan mask: the split leg of level l runs its sweeps two
 // per launch (launch_sweep_pair).
 constexpr int kPlanPairShift = 8;
+// Bits 16.. of a V-cycle plan mask: the split leg of level l fuses its sweep next to
+// the level change with the restriction (down: k_vdown<1> after pre - 1 sweeps) or
+// with the prolongation (up: k_vup<1> before post - 1 sweeps).
+constexpr int kPlanFuseShift = 16;
 // Coarsest-level slot table (kCoarseTableBytes at tab, count at tab_n), built
 // once per solve; zeroes the inactive voxels of the level's listed bricks in p0/p1.
 constexpr uint32_t kCoarseCap = 8192;  // active voxels the one-workgroup LDS smoother holds

@@ -430,7 +430,11 @@ int fx_read_stage_times(fx_sim* sim, double* total_ms, uint32_t* calls, size_t n
  * (halo post_sweeps).
  * Bit 8 + l of either mask: that split leg of level l runs its sweeps two per
  * launch (LDS tiles of halo 2) instead of one per launch; ignored where the leg
- * is fused. Every plan gives bit-identical results; only speed differs. */
+ * is fused.
+ * Bit 16 + l: that split leg fuses the sweep next to the level change into one
+ * LDS kernel: the last pre-sweep with the residual and the restriction (down,
+ * tiles of halo 3), or the prolongation with the first post-sweep (up, halo 1).
+ * Every plan gives bit-identical results; only speed differs. */
 int fx_set_vcycle_plan(fx_sim* sim, uint32_t down_split, uint32_t up_split);
 int fx_get_vcycle_plan(fx_sim* sim, uint32_t* down_split, uint32_t* up_split);
 

@@ -215,14 +215,16 @@ def test_step_parity_sparse_plume():
 
 def test_vcycle_plans_bit_identical():
     """Split V-cycle legs (lean one-sweep kernels + residual/restriction and
-    prolongation + one sweep, or paired sweeps in LDS tiles) give bit-identical
-    results to the fused LDS legs, and all match the oracle."""
+    prolongation + one sweep, paired sweeps in LDS tiles, or the sweep next to the level
+    change fused with the restriction or the prolongation) give bit-identical results to
+    the fused LDS legs, and all match the oracle."""
     n = 64
     srcs = scenes.c2_sources(n, radius_vox=2.5)
     st, _ = scenes.c2_plume_state(n, srcs, target=0.15)
     desc = scenes.c2_desc(n)
     out, res = {}, {}
-    for plan in ((0, 0), (0xF, 0), (0, 0xF), (0xF, 0xF), (0xF0F, 0xF0F), (0x10F, 0x30F)):
+    for plan in ((0, 0), (0xF, 0), (0, 0xF), (0xF, 0xF), (0xF0F, 0xF0F), (0x10F, 0x30F),
+                 (0xF000F, 0xF000F), (0x3000F, 0x1020F)):  # fused boundary sweeps (bits 16 + l)
         sim = CombustionVolumeSimulation(desc)
         sim.set_vcycle_plan(*plan)
         assert sim.vcycle_plan() == plan
