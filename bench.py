@@ -272,9 +272,11 @@ def run_c4(args):
         sim.updateSources(srcs)
         for k, v in state.items():
             sim.write_field(k, v)
+        if args.vcycle_plan:
+            sim.set_vcycle_plan(*(int(v, 0) for v in args.vcycle_plan.split(",")))
         sim.step(None, 0.0, dt)
         sim.synchronize()
-        sim.autotune_vcycle(4)
+        plan = sim.vcycle_plan() if args.vcycle_plan or not args.autotune else sim.autotune_vcycle(4)
         active0 = sim.active_brick_count()
         t0 = time.perf_counter()
         for k in range(1, nsteps + 1):
@@ -284,7 +286,9 @@ def run_c4(args):
         active = 0.5 * (active0 + sim.active_brick_count())
         out = {f: sim.read_field(f) for f in fields}
         sim.dispose()
-        run = {"ms_per_step": ms, "value": active * 512 / (ms * 1e-3), "active_bricks": active}
+        run = {"ms_per_step": ms, "value": active * 512 / (ms * 1e-3), "active_bricks": active,
+               "vcycle_plan": {"down_split": plan[0], "up_split": plan[1],
+                               "autotuned": bool(args.autotune and not args.vcycle_plan)}}
         if ref is None:
             ref = out
         else:

@@ -1669,20 +1669,20 @@ int fx_autotune_vcycle(fx_sim* s, uint32_t repeats, uint32_t* down_split, uint32
     for (int w = 0; w < 3 && rc == FX_OK; ++w) rc = time_plan(&ms);
   }
   // Per leg: 0 fused LDS kernel, 1 split with one sweep per launch, 2 split with
-  // two sweeps per launch (only where the leg has two or more sweeps), 3 split with
-  // the sweep next to the level change fused with the restriction (down) or the
-  // prolongation (up) in one LDS kernel (down: two or more sweeps; up: one or more).
+  // two sweeps per launch (only where the leg has two or more sweeps). The split
+  // legs with the boundary sweep fused (plan bit 16 + l) are not timed: on a 15%
+  // active 256^3 plume they were slower at every level (0.985-1.18 against
+  // 0.941 ms per step) and would nearly double the tuning time; callers can still
+  // pin them with fx_set_vcycle_plan.
   auto set_leg = [&](uint32_t& mask, int l, uint32_t opt) {
     mask &= ~((1u << l) | (1u << (kPlanPairShift + l)) | (1u << (kPlanFuseShift + l)));
     if (opt >= 1) mask |= 1u << l;
     if (opt == 2) mask |= 1u << (kPlanPairShift + l);
-    if (opt == 3) mask |= 1u << (kPlanFuseShift + l);
   };
   const fx_vcycle& vc = s->desc.vcycle;
   std::vector<uint32_t> downs = {0u, 1u}, ups = {0u, 1u};
-  if (vc.pre_sweeps >= 2) { downs.push_back(2u); downs.push_back(3u); }
+  if (vc.pre_sweeps >= 2) downs.push_back(2u);
   if (vc.post_sweeps >= 2) ups.push_back(2u);
-  if (vc.post_sweeps >= 1) ups.push_back(3u);
   const uint32_t np = uint32_t(downs.size() * ups.size());
   for (int l = 0; l + 1 < L && rc == FX_OK; ++l) {
     // each plan's best of two interleaved rounds

@@ -439,7 +439,8 @@ int fx_set_vcycle_plan(fx_sim* sim, uint32_t down_split, uint32_t up_split);
 int fx_get_vcycle_plan(fx_sim* sim, uint32_t* down_split, uint32_t* up_split);
 
 /* Autotuner: times each level's leg plans (fused, split, split with paired
- * sweeps, for the down and the up leg; greedily, finest level first)
+ * sweeps, for the down and the up leg; greedily, finest level first; the
+ * bit 16 + l legs are not timed, only pinned by fx_set_vcycle_plan)
  * with `repeats` graph-captured pressure solves of the current state and keeps
  * the fastest; returns the chosen masks. Solves start from p = 0 and every plan
  * is bit-identical, so the simulation's results do not depend on the outcome.
