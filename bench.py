@@ -31,7 +31,7 @@ sys.path.insert(0, ROOT)
 
 from fenix_amd import abi  # noqa: E402
 from fenix_amd.render import BloomPass, VolumeRaymarchPass, orbit_camera_snapshot  # noqa: E402
-from fenix_amd.simulation import CombustionVolumeBatch, CombustionVolumeSimulation  # noqa: E402
+from fenix_amd.simulation import CombustionVolumeBatch, CombustionVolumeQueueBatch, CombustionVolumeSimulation  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
@@ -166,16 +166,49 @@ def run_batch(args):
     else:
         raise SystemExit("--batch applies to the c2 and c3 workloads")
     descs = [scenes.c2_desc(sp["n"], precision=prec) for sp in specs]
-    batch = CombustionVolumeBatch(descs)
-    for sim, sp in zip(batch.scenes, specs):
-        sim.updateSources(sp["sources"])
+    states = []
+    for sp in specs:
         if sp["n"] == 256:
             st, _ = scenes.c2_plume_state(256, sp["sources"], target=sp["target"], seed=sp["seed"])
         else:
             st = scenes.plume_state_fast(sp["n"], sp["sources"], sp["target"], sp["seed"])
-        for name, values in st.items():
-            sim.write_field(name, values)
-        del st
+        states.append(st)
+    queue = args.batch_mode == "queue" or (args.batch_mode is None and not render)
+    if queue and render:
+        raise SystemExit("a brick-queue batch renders nothing: use --batch-mode streams with --workload c2")
+    dt = 1.0 / 60.0
+
+    def load(sims):
+        for sim, sp, st in zip(sims, specs, states):
+            sim.updateSources(sp["sources"])
+            for name, values in st.items():
+                sim.write_field(name, values)
+
+    def timed(step_all, sync, count):
+        # warm-up, then the timed window; returns (ms per batch step, mean active bricks)
+        for k in range(1, args.warmup):
+            step_all(k)
+        sync()
+        a0 = count()
+        t0 = time.perf_counter()
+        for k in range(args.warmup, args.warmup + args.steps):
+            step_all(k)
+        sync()
+        t1 = time.perf_counter()
+        return (t1 - t0) * 1e3 / args.steps, 0.5 * (a0 + count())
+
+    queue_plan = None
+    if queue:
+        qb = CombustionVolumeQueueBatch(descs)
+        load(qb.scenes)
+        qb.step(0.0, dt)
+        qb.synchronize()
+        queue_plan = qb.autotune_vcycle(4) if args.autotune else qb.vcycle_plan()
+        ms, active = timed(lambda k: qb.step(k * dt, dt), qb.synchronize, qb.active_brick_count)
+        qb.dispose()
+    batch = CombustionVolumeBatch(descs)
+    load(batch.scenes)
+    del states
     rp, cam, imgs = None, orbit_camera_snapshot(), []
     if render:
         w, h, sc = render
@@ -184,7 +217,6 @@ def run_batch(args):
             rp[id(sim)] = VolumeRaymarchPass((sp["n"],) * 3)
             rp[id(sim)].setRenderParams(stepCount=sc)
             imgs.append(HipBuffer(w * h * 8))
-    dt = 1.0 / 60.0
 
     def batch_step(k):
         batch.step(k * dt, dt)
@@ -203,18 +235,11 @@ def run_batch(args):
     if args.autotune:
         for sim in batch.scenes:
             sim.autotune_vcycle(4)
-    for k in range(1, args.warmup):
-        batch_step(k)
-    batch.synchronize()
-    a0 = sum(sim.active_brick_count() for sim in batch.scenes)
-    t0 = time.perf_counter()
-    for k in range(args.warmup, args.warmup + args.steps):
-        batch_step(k)
-    batch.synchronize()
-    t1 = time.perf_counter()
-    a1 = sum(sim.active_brick_count() for sim in batch.scenes)
-    ms = (t1 - t0) * 1e3 / args.steps
-    voxels = 0.5 * (a0 + a1) * 512
+    streams_ms, streams_active = timed(batch_step, batch.synchronize,
+                                       lambda: sum(sim.active_brick_count() for sim in batch.scenes))
+    if not queue:
+        ms, active = streams_ms, streams_active
+    voxels = active * 512
     # plain: each scene alone for the same number of steps, one after the other
     base = args.warmup + args.steps
     plain_s = 0.0
@@ -239,11 +264,16 @@ def run_batch(args):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": label, "scenes": len(specs), "storage": args.precision,
-                   "edges": sorted(sp["n"] for sp in specs), "active_bricks": 0.5 * (a0 + a1),
+                   "batch": ("brick queue: one launch per pass over the bricks of all scenes (fx_create_scenes)"
+                             if queue else "independent scenes on their own streams (fx_batch_*)"),
+                   "edges": sorted(sp["n"] for sp in specs), "active_bricks": active,
                    "voxels": sum(sp["n"] ** 3 for sp in specs)},
         "plain": {"ms_per_batch_step": plain_ms, "value": voxels / (plain_ms * 1e-3)},
         "vs_plain": plain_ms / ms,
     }
+    if queue:
+        out["config"]["vcycle_plan"] = {"down_split": queue_plan[0], "up_split": queue_plan[1], "autotuned": args.autotune}
+        out["streams"] = {"ms_per_batch_step": streams_ms, "value": streams_active * 512 / (streams_ms * 1e-3)}
     for b in imgs:
         b.free()
     batch.dispose()
@@ -384,6 +414,10 @@ def main():
                          "the plan an autotuned run chose, since profiler overhead changes what the autotuner picks")
     ap.add_argument("--batch", type=int, default=0, metavar="K",
                     help="batched scenes: K 256^3 plume scenes (c2) or the C3 batch (c3, default 64 scenes)")
+    ap.add_argument("--batch-mode", choices=["streams", "queue"], default=None,
+                    help="batches: one brick queue over all scenes (fx_create_scenes; the default without render) or "
+                         "independent scenes on their own streams (fx_batch_*; the default with render); a queue run "
+                         "also times the streams batch ('streams') and every scene alone ('plain')")
     ap.add_argument("--hw-queues", type=int, default=4, choices=range(1, 33), metavar="Q",
                     help="hardware queues per process for batched runs (GPU_MAX_HW_QUEUES, at most 32)")
     ap.add_argument("--bloom", action="store_true",

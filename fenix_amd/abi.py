@@ -217,6 +217,17 @@ FUNCTIONS = {
     "fx_batch_scene": (C.c_void_p, [C.c_void_p, u32]),
     "fx_batch_step": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     "fx_batch_synchronize": (C.c_int, [C.c_void_p]),
+    "fx_create_scenes": (C.c_int, [C.POINTER(CreateDesc), u32, C.POINTER(P)]),
+    "fx_scene_count": (u32, [P]),
+    "fx_scene_update_sources": (C.c_int, [P, u32, C.POINTER(C.c_float), u32]),
+    "fx_scene_get_runtime_params": (C.c_int, [P, u32, C.POINTER(RuntimeParams)]),
+    "fx_scene_set_runtime_params": (C.c_int, [P, u32, C.POINTER(RuntimeParams), u32]),
+    "fx_scene_read_field": (C.c_int, [P, u32, i32, C.POINTER(C.c_float), C.c_size_t]),
+    "fx_scene_write_field": (C.c_int, [P, u32, i32, C.POINTER(C.c_float), C.c_size_t]),
+    "fx_scene_read_flags": (C.c_int, [P, u32, i32, C.POINTER(u32), C.c_size_t]),
+    "fx_scene_read_active_list": (C.c_int, [P, u32, C.POINTER(u32), C.c_size_t, C.POINTER(u32)]),
+    "fx_scene_read_page_table": (C.c_int, [P, u32, C.POINTER(i32), C.c_size_t]),
+    "fx_scene_active_brick_count": (C.c_int, [P, u32, C.POINTER(u32)]),
     "fx_diff_default_desc": (None, [C.POINTER(DiffDesc)]),
     "fx_diff_create": (C.c_int, [C.POINTER(DiffDesc), C.POINTER(C.c_void_p)]),
     "fx_diff_destroy": (None, [C.c_void_p]),

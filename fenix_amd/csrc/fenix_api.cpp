@@ -43,8 +43,8 @@ struct Level {
 };
 
 struct StageSlot {
-  StepConst* c = nullptr;      // pinned staging block (kConstBytes + 4 nb bytes)
-  uint32_t* cand = nullptr;    // inside the block at kConstBytes, capacity nb
+  StepConst* c = nullptr;      // pinned staging block (const_bytes + 4 nb bytes)
+  uint32_t* cand = nullptr;    // inside the block at const_bytes, capacity nb
   hipEvent_t done = nullptr;
   bool used = false;
 };
@@ -92,6 +92,29 @@ struct fx_sim {
   fx_runtime_params rt{};
   fx_quality q{};
   int mode = 0;
+
+  // Brick-queue batch (fx_create_scenes): every scene's bricks share each level's
+  // storage (scene k owns bricks [base, base + its bricks) of every level, in
+  // scene order), and every stage is one launch over the listed bricks of all
+  // scenes. g and lv[l].g are then selectors whose tables live on the device;
+  // the per-scene grids, runtime parameters and emitters are kept here. The
+  // step constants are one StepConst per scene (const_bytes apart from the
+  // candidate list), the emitters one buffer with per-scene offsets.
+  bool multi = false;
+  struct Scene {
+    Grid g[FX_MAX_LEVELS_DEV] = {};  // this scene's grid of every level (base and index set)
+    size_t nv = 0;                    // voxels of the finest level
+    fx_runtime_params rt{};
+    std::vector<float> h_src;         // packed, >= 1 source
+    uint32_t nsrc = 0;
+  };
+  std::vector<Scene> scenes;
+  Grid* d_gtab = nullptr;             // [level][scene] grids
+  uint16_t* d_owner = nullptr;        // scene of every brick, level after level
+  uint32_t* d_scene_count = nullptr;  // active fine bricks per scene (end of the last step)
+  uint32_t* h_scene_count = nullptr;  // pinned copy
+  size_t coarse_nv_max = 0;           // largest coarsest-level storage of one scene
+  size_t const_bytes = 0;             // bytes of the step constants before the candidate list
   bool initialized = false;
   double startS = 0, lastS = 0;
   uint64_t stepIndex = 0;
@@ -183,6 +206,13 @@ int fail(fx_sim* s, int code, const std::string& msg) {
       return fail(sim, FX_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));   \
   } while (0)
 
+// Calls that address one scene's fields, flags, emitters or image: a batch of
+// scenes takes the fx_scene_* forms.
+#define ONE_SCENE(sim)                                                                                       \
+  do {                                                                                                       \
+    if ((sim)->multi) return fail(sim, FX_ERR_UNSUPPORTED, "a batch of scenes is addressed per scene (fx_scene_*)"); \
+  } while (0)
+
 int clamp_int(double v, int lo, int hi) {  // CVS/types.ts:26-28
   double r = std::floor(v + 0.5);
   return int(std::max(double(lo), std::min(double(hi), r)));
@@ -226,8 +256,8 @@ const uint32_t* level_count(const fx_sim* s, int l) { return s->lv[l].L.count; }
 // Voxel activity of level l (sparse). Level 0 needs it too: fused V-cycle
 // tiles compute halo voxels that may lie in inactive bricks.
 LvMask mask_of(const fx_sim* s, int l) {
-  if (!s->sparse) return LvMask{nullptr, 0, 0, 0, 0};
-  return LvMask{s->flagsInj, s->g.BX, s->g.BY, s->g.BZ, uint32_t(l)};
+  if (!s->sparse) return LvMask{nullptr, s->g, 0u};
+  return LvMask{s->flagsInj, s->g, uint32_t(l)};
 }
 LvBits bits_of(const fx_sim* s, int l) { return LvBits{s->lv[l].bits, s->sparse ? s->lv[l].L.page : nullptr}; }
 
@@ -240,33 +270,37 @@ std::vector<Buf> fine_buffers(fx_sim* s) {
   return b;
 }
 
-// ORCH:236-266 — computed in double, stored as f32.
-fx_sim_params make_params(const fx_sim* s, double elapsed, double stepS) {
+// ORCH:236-266 — computed in double, stored as f32; `rt` and `g` are those of the
+// scene the constants are for.
+fx_sim_params make_params(const fx_sim* s, const fx_runtime_params& rt, const Grid& g, double elapsed, double stepS) {
   fx_sim_params p;
   double eff = std::min(stepS, 1.0 / 15.0);
   p.time = float(elapsed - s->startS);
   p.delta_time = float(eff);
   p.previous_time = float(s->lastS - s->startS);
   p.scalar_set_index = float(s->curSet);
-  p.wind[0] = s->rt.wind[0]; p.wind[1] = s->rt.wind[1]; p.wind[2] = s->rt.wind[2]; p.wind[3] = s->rt.wind_strength;
-  p.gravity[0] = s->rt.gravity[0]; p.gravity[1] = s->rt.gravity[1]; p.gravity[2] = s->rt.gravity[2];
-  p.gravity[3] = s->rt.gravity_strength;
-  p.buoyancy = s->rt.buoyancy;
-  p.vorticity_strength = s->rt.vorticity_strength;
-  p.dx = float(double(s->rt.world_size) / double(std::max(s->g.W, s->g.D)));
-  p.world_size = s->rt.world_size;
-  p.vorticity_mask[0] = s->rt.vorticity_constant_mask; p.vorticity_mask[1] = s->rt.vorticity_velocity_mask;
-  p.vorticity_mask[2] = s->rt.vorticity_heat_mask; p.vorticity_mask[3] = s->rt.vorticity_density_mask;
+  p.wind[0] = rt.wind[0]; p.wind[1] = rt.wind[1]; p.wind[2] = rt.wind[2]; p.wind[3] = rt.wind_strength;
+  p.gravity[0] = rt.gravity[0]; p.gravity[1] = rt.gravity[1]; p.gravity[2] = rt.gravity[2];
+  p.gravity[3] = rt.gravity_strength;
+  p.buoyancy = rt.buoyancy;
+  p.vorticity_strength = rt.vorticity_strength;
+  p.dx = float(double(rt.world_size) / double(std::max(g.W, g.D)));
+  p.world_size = rt.world_size;
+  p.vorticity_mask[0] = rt.vorticity_constant_mask; p.vorticity_mask[1] = rt.vorticity_velocity_mask;
+  p.vorticity_mask[2] = rt.vorticity_heat_mask; p.vorticity_mask[3] = rt.vorticity_density_mask;
   return p;
 }
+fx_sim_params make_params(const fx_sim* s, double elapsed, double stepS) { return make_params(s, s->rt, s->g, elapsed, stepS); }
 
 // Sparse injection bricks: every brick overlapping the voxel bounding box (+1
 // voxel) of a gate-active source's influence sphere (burst: outer limit,
 // inject-source-common.wgsl.ts:70-77).
-void injection_bricks(const fx_sim* s, const fx_sim_params& p, std::vector<uint8_t>& mark, std::vector<uint32_t>& out) {
-  uint32_t ns = std::max<uint32_t>(s->nsrc, 1);
+// g: the scene's fine grid (brick ids are offset by its base); src/nsrc its emitters.
+void injection_bricks(const Grid& g, const std::vector<float>& src, uint32_t nsrc, const fx_sim_params& p,
+                      std::vector<uint8_t>& mark, std::vector<uint32_t>& out) {
+  uint32_t ns = std::max<uint32_t>(nsrc, 1);
   for (uint32_t i = 0; i < ns; ++i) {
-    const float* e = &s->h_src[size_t(i) * FX_FLOATS_PER_EMITTER];
+    const float* e = &src[size_t(i) * FX_FLOATS_PER_EMITTER];
     float age = p.time - e[4];
     if (!(age >= 0.f && e[5] >= age)) continue;
     float r = e[3];
@@ -275,7 +309,7 @@ void injection_bricks(const fx_sim* s, const fx_sim_params& p, std::vector<uint8
       r = r + std::max(r * 0.24f, vw * 3.f);
     }
     if (!(r > 0.f)) continue;
-    const uint32_t n[3] = {s->g.W, s->g.H, s->g.D};
+    const uint32_t n[3] = {g.W, g.H, g.D};
     int lo[3], hi[3];
     bool empty = false;
     for (int a = 0; a < 3; ++a) {
@@ -289,7 +323,7 @@ void injection_bricks(const fx_sim* s, const fx_sim_params& p, std::vector<uint8
     for (int z = lo[2]; z <= hi[2]; ++z)
       for (int y = lo[1]; y <= hi[1]; ++y)
         for (int x = lo[0]; x <= hi[0]; ++x) {
-          uint32_t b = uint32_t(x) + s->g.BX * (uint32_t(y) + s->g.BY * uint32_t(z));
+          uint32_t b = g.base + uint32_t(x) + g.BX * (uint32_t(y) + g.BY * uint32_t(z));
           if (!mark[b]) { mark[b] = 1; out.push_back(b); }
         }
   }
@@ -297,24 +331,32 @@ void injection_bricks(const fx_sim* s, const fx_sim_params& p, std::vector<uint8
 
 // Uploads the step constants (and, sparse, the candidate list) through a pinned
 // staging slot; the slot is reused only after its previous copy completed.
-int upload_step(fx_sim* s, const fx_sim_params& p, const std::vector<uint32_t>* cand, uint32_t inj_count) {
+// p: the constants of every scene (one for a single scene). Scene 0's inj_count
+// and cand_count are the lengths of the whole candidate list.
+int upload_step(fx_sim* s, const std::vector<fx_sim_params>& p, const std::vector<uint32_t>* cand, uint32_t inj_count) {
   StageSlot& slot = s->slots[s->slot_next];
   s->slot_next = (s->slot_next + 1) % kStageSlots;
   if (slot.used) HIP_OK(s, hipEventSynchronize(slot.done));
-  StepConst& c = *slot.c;
-  c.p = p;
-  c.nsrc = std::max<uint32_t>(s->nsrc, 1);
-  c.ignition_lo = s->desc.ignition_lo; c.ignition_hi = s->desc.ignition_hi; c.burn_rate = s->desc.burn_rate;
-  c.act_density = s->desc.activation_density; c.act_speed = s->desc.activation_speed;
-  c.omega = s->desc.vcycle.omega;
-  c.inj_count = inj_count;
-  c.cand_count = cand ? uint32_t(cand->size()) : 0u;
-  c.step_key = uint32_t(s->stepIndex);
-  c.sr = s->stochastic ? 1u : 0u;
-  size_t bytes = sizeof(StepConst);
+  uint32_t src_off = 0;
+  for (size_t k = 0; k < p.size(); ++k) {
+    StepConst& c = slot.c[k];
+    const uint32_t nsrc = s->multi ? s->scenes[k].nsrc : s->nsrc;
+    c.p = p[k];
+    c.nsrc = std::max<uint32_t>(nsrc, 1);
+    c.ignition_lo = s->desc.ignition_lo; c.ignition_hi = s->desc.ignition_hi; c.burn_rate = s->desc.burn_rate;
+    c.act_density = s->desc.activation_density; c.act_speed = s->desc.activation_speed;
+    c.omega = s->desc.vcycle.omega;
+    c.inj_count = inj_count;
+    c.cand_count = cand ? uint32_t(cand->size()) : 0u;
+    c.step_key = uint32_t(s->stepIndex);
+    c.sr = s->stochastic ? 1u : 0u;
+    c.src_off = src_off;
+    src_off += c.nsrc * FX_FLOATS_PER_EMITTER;
+  }
+  size_t bytes = sizeof(StepConst) * p.size();
   if (cand && !cand->empty()) {
     std::memcpy(slot.cand, cand->data(), cand->size() * 4);
-    bytes = kConstBytes + cand->size() * 4;
+    bytes = s->const_bytes + cand->size() * 4;
   }
   HIP_OK(s, hipMemcpyAsync(s->d_const, slot.c, bytes, hipMemcpyHostToDevice, s->stream));
   HIP_OK(s, hipEventRecord(slot.done, s->stream));
@@ -323,7 +365,7 @@ int upload_step(fx_sim* s, const fx_sim_params& p, const std::vector<uint32_t>* 
 }
 
 bool debug_each_frame(const fx_sim* s) {
-  if (s->desc.debug_fields == 0) return false;
+  if (s->multi || s->desc.debug_fields == 0) return false;
   if (s->desc.debug_fields == 1) return true;
   return s->nv < 1800000ull;  // ORCH:193 (voxelCount)
 }
@@ -472,10 +514,12 @@ int solve_vcycle(fx_sim* s) {
       launch_clear_u32(s->stream, A.zero, A.zero_words, 0u);
   }
   Level& LC = s->lv[L - 1];
-  const bool block_smoother = L > 1 && LC.nv <= kBlockSmoothVoxels && !s->coarse_split;
+  const uint32_t nsc = s->multi ? uint32_t(s->scenes.size()) : 1u;
+  const size_t coarse_nv = s->multi ? s->coarse_nv_max : LC.nv;  // one scene's coarsest storage
+  const bool block_smoother = L > 1 && coarse_nv <= kBlockSmoothVoxels && !s->coarse_split;
   if (block_smoother && (s->sparse || !s->dense_tables))
     launch_coarse_setup(s->stream, level_list(s, L - 1), level_count(s, L - 1), LC.g, bits_of(s, L - 1), LC.tab, LC.tab_n, LC.p[0],
-                        LC.p[1]);
+                        LC.p[1], nsc);
   if (!s->sparse) s->dense_tables = true;
   // n sweeps at level l; zero_start: the first sweep starts from p = 0 (a
   // coarse level's correction), so neither a clear nor a read of p is needed.
@@ -488,7 +532,7 @@ int solve_vcycle(fx_sim* s) {
     if (zero_start) cur[l] = 0;
     if (l == L - 1 && block_smoother) {
       launch_smooth_block(s->stream, level_list(s, l), level_count(s, l), V.g, mask_of(s, l), s->d_const, V.div, V.p[cur[l]],
-                          V.p[1 - cur[l]], n, 1, zero_start ? 1 : 0, V.tab, V.tab_n);
+                          V.p[1 - cur[l]], n, 1, zero_start ? 1 : 0, V.tab, V.tab_n, nsc);
       if (n & 1) cur[l] = 1 - cur[l];
       return;
     }
@@ -502,7 +546,7 @@ int solve_vcycle(fx_sim* s) {
         continue;
       }
       // fine sweeps reading a pressure are timed by the kernel's own start/end timestamps
-      const bool exact = s->timing && l == 0 && src != nullptr;
+      const bool exact = s->timing && l == 0 && src != nullptr && !s->multi;
       hipEvent_t ka = exact ? StageTimer::take(s) : nullptr, kb = exact ? StageTimer::take(s) : nullptr;
       {
         StageTimer timer(s, (l == 0 && !(ka && kb)) ? FX_TIME_JACOBI_FINE : -1);
@@ -630,9 +674,14 @@ void run_stage(fx_sim* s, int pass) {
 }
 
 // Candidate list for the sparse activation: injection bricks then host-written bricks.
-void build_candidates(fx_sim* s, const fx_sim_params& p, std::vector<uint32_t>& cand, uint32_t& inj_count) {
+void build_candidates(fx_sim* s, const std::vector<fx_sim_params>& p, std::vector<uint32_t>& cand, uint32_t& inj_count) {
   std::vector<uint8_t> mark(s->nb, 0);
-  injection_bricks(s, p, mark, cand);
+  if (s->multi) {
+    for (size_t k = 0; k < s->scenes.size(); ++k)
+      injection_bricks(s->scenes[k].g[0], s->scenes[k].h_src, s->scenes[k].nsrc, p[k], mark, cand);
+  } else {
+    injection_bricks(s->g, s->h_src, s->nsrc, p[0], mark, cand);
+  }
   inj_count = uint32_t(cand.size());
   for (uint32_t b : s->pending)
     if (!mark[b]) { mark[b] = 1; cand.push_back(b); }
@@ -775,8 +824,13 @@ void record_step_b(fx_sim* s, bool vort, bool pres, bool dbg) {
   }
   run_stage(s, FX_PASS_PROJECTION);                    // ORCH:296
   if (dbg) run_stage(s, FX_PASS_DEBUG_FIELDS);         // ORCH:298-301
-  if (s->sparse && s->h_fine_count)                    // for the next steps' coarsest-level plan
+  if (s->multi && s->h_scene_count) {                  // for the next steps' coarsest-level plan
+    launch_scene_counts(s->stream, s->lv[0].L.list, s->lv[0].L.count, s->lv[0].g.tab, uint32_t(s->scenes.size()),
+                        s->d_scene_count);
+    (void)hipMemcpyAsync(s->h_scene_count, s->d_scene_count, s->scenes.size() * 4, hipMemcpyDeviceToHost, s->stream);
+  } else if (s->sparse && s->h_fine_count) {
     (void)hipMemcpyAsync(s->h_fine_count, s->lv[0].L.count, 4, hipMemcpyDeviceToHost, s->stream);
+  }
 }
 
 // Upper bound of the coarsest level's active voxels: a level-l voxel J lies in
@@ -799,7 +853,10 @@ void update_coarse_plan(fx_sim* s) {
   }
   if (s->count_pending && hipEventQuery(s->ev_count) == hipSuccess) {
     s->count_pending = false;
-    const uint64_t bound = coarse_active_bound(s, *s->h_fine_count);
+    uint64_t fine = s->multi ? 0 : *s->h_fine_count;  // the largest scene's count bounds every scene's smoother
+    if (s->multi)
+      for (size_t k = 0; k < s->scenes.size(); ++k) fine = std::max<uint64_t>(fine, s->h_scene_count[k]);
+    const uint64_t bound = coarse_active_bound(s, fine);
     // hysteresis: a set hovering near the cap does not recapture the graph every step
     if (bound > kCoarseCap) s->coarse_split = true;
     else if (bound * 8 < uint64_t(kCoarseCap) * 7) s->coarse_split = false;
@@ -884,9 +941,9 @@ void fx_default_create_desc(fx_create_desc* d) {
   d->device = 0;
 }
 
-int fx_create(const fx_create_desc* desc, fx_sim** out) {
-  if (!desc || !out) return fail(nullptr, FX_ERR_INVALID_ARG, "null argument");
-  *out = nullptr;
+namespace {
+// Checks one creation descriptor (fx_create's rules).
+int check_desc(const fx_create_desc* desc) {
   if (desc->width < 4 || desc->height < 4 || desc->depth < 4)
     return fail(nullptr, FX_ERR_INVALID_ARG, "resolution must be at least 4 in every axis");
   if (uint64_t(desc->width) * desc->height * desc->depth > (1ull << 31))
@@ -897,6 +954,35 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
     return fail(nullptr, FX_ERR_INVALID_ARG, "vcycle.levels must be in [2, 8]");
   if (desc->advection_mode != FX_ADVECT_MACCORMACK && desc->advection_mode != FX_ADVECT_SEMI_LAGRANGIAN)
     return fail(nullptr, FX_ERR_INVALID_ARG, "unknown advection mode");
+  return FX_OK;
+}
+// Scenes of one brick-queue batch share every setting that shapes the recorded
+// pass sequence or the kernels: domain, solver, activation, precision, advection,
+// quality, V-cycle, combustion and activation constants, device.
+int check_batch_desc(const fx_create_desc& d, const fx_create_desc& d0) {
+  if (d.domain_mode != FX_DOMAIN_SPARSE || d.solver != FX_SOLVER_VCYCLE || d.activation_rule != FX_ACTIVATION_THRESHOLD)
+    return fail(nullptr, FX_ERR_UNSUPPORTED, "a batch of scenes runs the sparse domain, the V-cycle solver and threshold activation");
+  const fx_quality q = clamp_quality(d.quality), q0 = clamp_quality(d0.quality);
+  if (d.precision != d0.precision || d.advection_mode != d0.advection_mode || d.device != d0.device ||
+      std::memcmp(&q, &q0, sizeof(q)) != 0 || std::memcmp(&d.vcycle, &d0.vcycle, sizeof(d.vcycle)) != 0 ||
+      d.ignition_lo != d0.ignition_lo || d.ignition_hi != d0.ignition_hi || d.burn_rate != d0.burn_rate ||
+      d.activation_density != d0.activation_density || d.activation_speed != d0.activation_speed)
+    return fail(nullptr, FX_ERR_INVALID_ARG,
+                "the scenes of a batch share precision, advection mode, quality, V-cycle, combustion and activation constants and device");
+  return FX_OK;
+}
+
+int create_impl(const fx_create_desc* descs, uint32_t nsc, bool multi, fx_sim** out) {
+  if (!descs || !out) return fail(nullptr, FX_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (nsc == 0) return fail(nullptr, FX_ERR_INVALID_ARG, "a batch needs at least one scene");
+  if (nsc > 65535) return fail(nullptr, FX_ERR_INVALID_ARG, "a batch holds at most 65535 scenes");
+  for (uint32_t k = 0; k < nsc; ++k) {
+    int r = check_desc(&descs[k]);
+    if (r == FX_OK && multi) r = check_batch_desc(descs[k], descs[0]);
+    if (r != FX_OK) return r;
+  }
+  const fx_create_desc* desc = &descs[0];
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, FX_ERR_DEVICE, "no HIP device");
   if (desc->device < 0 || desc->device >= ndev) return fail(nullptr, FX_ERR_INVALID_ARG, "device ordinal out of range");
@@ -904,6 +990,8 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
   fx_sim* s = new (std::nothrow) fx_sim();
   if (!s) return fail(nullptr, FX_ERR_OUT_OF_MEMORY, "host allocation failed");
   s->desc = *desc;
+  s->multi = multi;
+  if (multi) s->desc.debug_fields = 0;  // a batch records no debug pass
   // storage type in desc.precision; the rounding mode apart
   s->stochastic = desc->precision == FX_PRECISION_BF16 || desc->precision == FX_PRECISION_FP16;
   if (desc->precision == FX_PRECISION_BF16_RNE) s->desc.precision = FX_PRECISION_BF16;
@@ -913,9 +1001,45 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
   s->rt = desc->runtime;
   s->q = clamp_quality(desc->quality);
   s->mode = desc->advection_mode;
-  s->g = make_grid(desc->width, desc->height, desc->depth);
-  s->nb = brick_count(s->g);
-  s->nv = size_t(desc->width) * desc->height * desc->depth;
+  const int nlev = desc->solver == FX_SOLVER_VCYCLE ? desc->vcycle.levels : 3;  // PressureSolvePass.ts:32-35
+  // Level grids; several scenes: every scene's grid of every level at its base
+  // in the level's storage, and the level totals.
+  std::vector<size_t> lev_nb(size_t(nlev), 0);
+  if (multi) {
+    s->scenes.resize(nsc);
+    for (uint32_t k = 0; k < nsc; ++k) {
+      fx_sim::Scene& sc = s->scenes[k];
+      sc.rt = descs[k].runtime;
+      sc.nv = size_t(descs[k].width) * descs[k].height * descs[k].depth;
+      sc.h_src.assign(FX_FLOATS_PER_EMITTER, 0.f);
+      Grid lg = make_grid(descs[k].width, descs[k].height, descs[k].depth);
+      for (int l = 0; l < nlev; ++l) {
+        lg.base = uint32_t(lev_nb[size_t(l)]);
+        lg.scene = k;
+        sc.g[l] = lg;
+        lev_nb[size_t(l)] += brick_count(lg);
+        if (l == nlev - 1) s->coarse_nv_max = std::max(s->coarse_nv_max, storage_count(lg));
+        lg = make_grid(halve(lg.W), halve(lg.H), halve(lg.D));
+      }
+      s->nv += sc.nv;
+    }
+    if (lev_nb[0] * 512 > (size_t(1) << 32))
+      return fail(nullptr, FX_ERR_INVALID_ARG, "the scenes of a batch exceed 2^23 bricks");
+    for (uint32_t k = 0; k < nsc; ++k)
+      for (int l = 0; l < nlev; ++l) s->scenes[k].g[l].span = uint32_t(lev_nb[size_t(l)]);
+    s->g = s->scenes[0].g[0];
+    s->nb = lev_nb[0];
+  } else {
+    s->g = make_grid(desc->width, desc->height, desc->depth);
+    s->nb = brick_count(s->g);
+    s->nv = size_t(desc->width) * desc->height * desc->depth;
+    Grid lg = s->g;
+    for (int l = 0; l < nlev; ++l) {
+      lev_nb[size_t(l)] = brick_count(lg);
+      lg = make_grid(halve(lg.W), halve(lg.H), halve(lg.D));
+    }
+  }
+  s->const_bytes = (sizeof(StepConst) * nsc + 255) & ~size_t(255);
   s->h_src.assign(FX_FLOATS_PER_EMITTER, 0.f);
   auto bail = [&](int code) { std::string e = s->err; fx_destroy(s); g_create_error = e; return code; };
 
@@ -958,8 +1082,41 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
     *s->h_fine_count = 0;
   }
 
-  const size_t nsv = storage_count(s->g);
   int rc = FX_OK;
+  if (multi) {
+    // device grid tables and brick owners of every level
+    std::vector<Grid> gt(size_t(nlev) * nsc);
+    std::vector<uint16_t> own;
+    std::vector<size_t> own_at(size_t(nlev), 0);
+    for (int l = 0; l < nlev; ++l) {
+      own_at[size_t(l)] = own.size();
+      for (uint32_t k = 0; k < nsc; ++k) {
+        gt[size_t(l) * nsc + k] = s->scenes[k].g[l];
+        own.insert(own.end(), brick_count(s->scenes[k].g[l]), uint16_t(k));
+      }
+    }
+    rc = dalloc(s, &s->d_gtab, gt.size());
+    if (rc == FX_OK) rc = dalloc(s, &s->d_owner, own.size());
+    if (rc == FX_OK) rc = dalloc(s, &s->d_scene_count, nsc);
+    if (rc == FX_OK && hipHostMalloc(reinterpret_cast<void**>(&s->h_scene_count), size_t(nsc) * 4, hipHostMallocDefault) != hipSuccess)
+      rc = fail(s, FX_ERR_OUT_OF_MEMORY, "pinned count allocation failed");
+    if (rc == FX_OK && (hipMemcpy(s->d_gtab, gt.data(), gt.size() * sizeof(Grid), hipMemcpyHostToDevice) != hipSuccess ||
+                        hipMemcpy(s->d_owner, own.data(), own.size() * 2, hipMemcpyHostToDevice) != hipSuccess))
+      rc = fail(s, FX_ERR_DEVICE, "grid table upload failed");
+    if (rc != FX_OK) return bail(rc);
+    for (uint32_t k = 0; k < nsc; ++k) s->h_scene_count[k] = 0;
+    // selectors: the scene-0 grid, pointing at the level's table
+    s->g.tab = s->d_gtab;
+    s->g.owner = s->d_owner;
+    for (int l = 0; l < nlev; ++l) {
+      s->scenes[0].g[l].tab = s->d_gtab + size_t(l) * nsc;
+      s->scenes[0].g[l].owner = s->d_owner + own_at[size_t(l)];
+    }
+    for (uint32_t k = 1; k < nsc; ++k)
+      for (int l = 0; l < nlev; ++l) { s->scenes[k].g[l].tab = nullptr; s->scenes[k].g[l].owner = nullptr; }
+  }
+
+  const size_t nsv = s->nb * 512;
   for (int k = 0; k < 2 && rc == FX_OK; ++k)
     for (int f = 0; f < 4 && rc == FX_OK; ++f) {
       uint8_t* p = nullptr;
@@ -980,22 +1137,21 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
   if (rc == FX_OK) rc = dalloc(s, &s->identity, s->nb);
   if (rc == FX_OK) rc = dalloc(s, &s->identity_count, 1);
   uint8_t* stage = nullptr;
-  if (rc == FX_OK) rc = dalloc(s, &stage, kConstBytes + s->nb * 4);
+  if (rc == FX_OK) rc = dalloc(s, &stage, s->const_bytes + s->nb * 4);
   if (rc == FX_OK) {
     s->d_const = reinterpret_cast<StepConst*>(stage);
-    s->d_cand = reinterpret_cast<uint32_t*>(stage + kConstBytes);
+    s->d_cand = reinterpret_cast<uint32_t*>(stage + s->const_bytes);
   }
   if (rc == FX_OK) rc = dalloc(s, &s->d_bits, (s->nb + 31) / 32);
   if (rc != FX_OK) return bail(rc);
   if (s->sparse) s->flagsRender = s->flagsInj;
 
-  int nlev = desc->solver == FX_SOLVER_VCYCLE ? desc->vcycle.levels : 3;  // PressureSolvePass.ts:32-35
   Grid lg = s->g;
   for (int l = 0; l < nlev && rc == FX_OK; ++l) {
     Level L;
-    L.g = lg;
-    L.nb = brick_count(lg);
-    L.nv = storage_count(lg);
+    L.g = multi ? s->scenes[0].g[l] : lg;
+    L.nb = lev_nb[size_t(l)];
+    L.nv = L.nb * 512;
     rc = dalloc(s, &L.div, L.nv);
     if (rc == FX_OK) rc = dalloc(s, &L.p[0], L.nv);
     if (rc == FX_OK) rc = dalloc(s, &L.p[1], L.nv);
@@ -1007,10 +1163,11 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
     if (rc == FX_OK) rc = l == 0 ? FX_OK : dalloc(s, &L.flags, L.nb);
     if (rc == FX_OK && desc->solver == FX_SOLVER_VCYCLE) rc = dalloc(s, &L.bits, L.nb * 8);
     if (rc == FX_OK && desc->solver == FX_SOLVER_VCYCLE && l > 0) {
+      // one slot table per scene (only the coarsest level's is used)
       uint8_t* t = nullptr;
-      rc = dalloc(s, &t, kCoarseTableBytes);
+      rc = dalloc(s, &t, kCoarseTableBytes * nsc);
       L.tab = t;
-      if (rc == FX_OK) rc = dalloc(s, &L.tab_n, 1);
+      if (rc == FX_OK) rc = dalloc(s, &L.tab_n, nsc);
     }
     if (l == 0) L.flags = s->flagsInj;
     s->lv.push_back(L);
@@ -1022,7 +1179,7 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
     A.nlevels = int(s->lv.size());
     uint32_t blocks = 0;
     for (int l = 0; l < A.nlevels; ++l) {
-      A.g[l] = s->lv[l].g;
+      A.nb[l] = uint32_t(s->lv[l].nb);
       A.flags[l] = s->lv[l].flags;
       A.L[l] = s->lv[l].L;
       A.block_start[l] = blocks;
@@ -1033,7 +1190,7 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
     s->lflags.n = A.nlevels - 1;
     for (int l = 1; l < A.nlevels; ++l) {
       s->lflags.f[l] = s->lv[l].flags;
-      s->lflags.BX[l] = s->lv[l].g.BX; s->lflags.BY[l] = s->lv[l].g.BY; s->lflags.BZ[l] = s->lv[l].g.BZ;
+      s->lflags.g[l] = s->lv[l].g;
     }
     if (dalloc(s, &s->d_compact, size_t(blocks) * 4) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
     A.counts = s->d_compact;
@@ -1042,25 +1199,24 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
   {
     const char* g = std::getenv("FENIX_GRAPH");
     s->use_graph = !(g && g[0] == '0');
-
   }
   s->norm_cap = std::max(1, desc->vcycle.cycles);
   if (dalloc(s, &s->d_norms, size_t(s->norm_cap) * kNormLanes) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
   if (dalloc(s, &s->d_rqueue, 1) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
   for (void** mirror : {&s->d_render, &s->d_sq, &s->d_uq}) {
     float* q = nullptr;
-    if (dalloc(s, &q, storage_count(s->g) * 4) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
+    if (dalloc(s, &q, nsv * 4) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
     *mirror = q;
   }
-  launch_clear_f32(s->stream, static_cast<float*>(s->d_sq), storage_count(s->g) * 4);
-  launch_clear_f32(s->stream, static_cast<float*>(s->d_uq), storage_count(s->g) * 4);
-  s->src_cap = FX_FLOATS_PER_EMITTER;
+  launch_clear_f32(s->stream, static_cast<float*>(s->d_sq), nsv * 4);
+  launch_clear_f32(s->stream, static_cast<float*>(s->d_uq), nsv * 4);
+  s->src_cap = size_t(FX_FLOATS_PER_EMITTER) * nsc;
   if (dalloc(s, &s->d_src, s->src_cap) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
   for (auto& slot : s->slots) {
-    if (hipHostMalloc(reinterpret_cast<void**>(&slot.c), kConstBytes + s->nb * 4, hipHostMallocDefault) != hipSuccess ||
+    if (hipHostMalloc(reinterpret_cast<void**>(&slot.c), s->const_bytes + s->nb * 4, hipHostMallocDefault) != hipSuccess ||
         hipEventCreateWithFlags(&slot.done, hipEventDisableTiming) != hipSuccess)
       return bail(fail(s, FX_ERR_OUT_OF_MEMORY, "pinned staging allocation failed"));
-    slot.cand = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(slot.c) + kConstBytes);
+    slot.cand = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(slot.c) + s->const_bytes);
   }
 
   // Identity lists (dense) and zeroed state.
@@ -1080,9 +1236,11 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
         return bail(fail(s, FX_ERR_DEVICE, "level list upload failed"));
     }
   }
-  s->h_src.assign(FX_FLOATS_PER_EMITTER, 0.f);
-  if (hipMemcpy(s->d_src, s->h_src.data(), FX_FLOATS_PER_EMITTER * 4, hipMemcpyHostToDevice) != hipSuccess)
-    return bail(fail(s, FX_ERR_DEVICE, "source upload failed"));
+  {
+    std::vector<float> zero(s->src_cap, 0.f);
+    if (hipMemcpy(s->d_src, zero.data(), zero.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(s, FX_ERR_DEVICE, "source upload failed"));
+  }
   int r = fx_reset(s);
   if (r != FX_OK) return bail(r);
   if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(s, FX_ERR_DEVICE, "initial clear failed"));
@@ -1090,6 +1248,11 @@ int fx_create(const fx_create_desc* desc, fx_sim** out) {
   *out = s;
   return FX_OK;
 }
+}  // namespace
+
+int fx_create(const fx_create_desc* desc, fx_sim** out) { return create_impl(desc, 1, false, out); }
+
+int fx_create_scenes(const fx_create_desc* descs, uint32_t n, fx_sim** out) { return create_impl(descs, n, true, out); }
 
 void fx_destroy(fx_sim* s) {
   if (!s) return;
@@ -1116,7 +1279,8 @@ void fx_destroy(fx_sim* s) {
   if (s->graph) (void)hipGraphDestroy(s->graph);
   if (s->graph_exec_b) (void)hipGraphExecDestroy(s->graph_exec_b);
   if (s->graph_b) (void)hipGraphDestroy(s->graph_b);
-  F(s->d_compact);
+  F(s->d_compact); F(s->d_gtab); F(s->d_owner); F(s->d_scene_count);
+  if (s->h_scene_count) (void)hipHostFree(s->h_scene_count);
   drain_timing(s);
   for (hipEvent_t e : s->pool) (void)hipEventDestroy(e);
   if (s->ev_scalars) (void)hipEventDestroy(s->ev_scalars);
@@ -1135,7 +1299,7 @@ int fx_reset(fx_sim* s) {  // ORCH:206-226
   s->curSet = 0; s->stepIndex = 0; s->lastPressure = 0; s->initialized = false;
   s->startS = s->lastS;
   s->pending.clear();
-  const size_t nsv = storage_count(s->g);
+  const size_t nsv = s->nb * 512;
   for (const Buf& b : fine_buffers(s)) launch_clear_u32(s->stream, static_cast<uint32_t*>(b.p), nsv * size_t(b.esz) / 4, 0u);
   launch_clear_f32(s->stream, static_cast<float*>(s->d_sq), nsv * 4);
   launch_clear_f32(s->stream, static_cast<float*>(s->d_uq), nsv * 4);
@@ -1165,7 +1329,11 @@ int fx_step(fx_sim* s, double elapsed_s, double step_s) {  // ORCH:230-305
   if (!std::isfinite(elapsed_s) || !std::isfinite(step_s)) return fail(s, FX_ERR_INVALID_ARG, "non-finite time");
   HIP_OK(s, hipSetDevice(s->device));
   if (!s->initialized) { s->initialized = true; s->startS = elapsed_s; s->lastS = elapsed_s; }
-  fx_sim_params p = make_params(s, elapsed_s, step_s);
+  std::vector<fx_sim_params> p;
+  if (s->multi)
+    for (const auto& sc : s->scenes) p.push_back(make_params(s, sc.rt, sc.g[0], elapsed_s, step_s));
+  else
+    p.push_back(make_params(s, elapsed_s, step_s));
   std::vector<uint32_t> cand;
   uint32_t inj = 0;
   if (s->sparse) build_candidates(s, p, cand, inj);
@@ -1189,12 +1357,12 @@ int fx_step(fx_sim* s, double elapsed_s, double step_s) {  // ORCH:230-305
       s->lastPressure = s->graph_last_pressure;
     }
     HIP_OK(s, hipGraphLaunch(s->graph_exec, s->stream));
-    rc = pack_render(s, s->render_full);
+    rc = s->multi ? FX_OK : pack_render(s, s->render_full);  // a batch of scenes renders nothing
     if (rc != FX_OK) return rc;
     HIP_OK(s, hipGraphLaunch(s->graph_exec_b, s->stream));
   } else {
     record_step_a(s);
-    rc = pack_render(s, s->render_full);
+    rc = s->multi ? FX_OK : pack_render(s, s->render_full);
     if (rc != FX_OK) return rc;
     record_step_b(s, vort, pres, dbg);
   }
@@ -1212,11 +1380,13 @@ int fx_step(fx_sim* s, double elapsed_s, double step_s) {  // ORCH:230-305
 int fx_run_pass(fx_sim* s, int32_t pass, const fx_sim_params* params) {
   if (!s || !params) return fail(s, FX_ERR_INVALID_ARG, "null argument");
   HIP_OK(s, hipSetDevice(s->device));
+  if (s->multi) return fail(s, FX_ERR_UNSUPPORTED, "single passes run on a simulation of one scene");
   std::vector<uint32_t> cand;
   uint32_t inj = 0;
-  if (s->sparse) build_candidates(s, *params, cand, inj);
+  const std::vector<fx_sim_params> pv(1, *params);
+  if (s->sparse) build_candidates(s, pv, cand, inj);
   snapshot_stale(s);
-  int rc = upload_step(s, *params, s->sparse ? &cand : nullptr, inj);
+  int rc = upload_step(s, pv, s->sparse ? &cand : nullptr, inj);
   if (rc != FX_OK) return rc;
   update_coarse_plan(s);
   switch (pass) {
@@ -1242,6 +1412,7 @@ int fx_run_pass(fx_sim* s, int32_t pass, const fx_sim_params* params) {
 
 int fx_request_debug_fields(fx_sim* s) {  // getDebugBuffers() sets debugRequested, ORCH:316
   if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
+  ONE_SCENE(s);
   s->debug_requested = true;
   return FX_OK;
 }
@@ -1256,12 +1427,14 @@ int fx_synchronize(fx_sim* s) {
 
 int fx_get_runtime_params(fx_sim* s, fx_runtime_params* out) {
   if (!s || !out) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  ONE_SCENE(s);
   *out = s->rt;
   return FX_OK;
 }
-int fx_set_runtime_params(fx_sim* s, const fx_runtime_params* p, uint32_t m) {  // ORCH:353-365
-  if (!s || !p) return fail(s, FX_ERR_INVALID_ARG, "null argument");
-  fx_runtime_params& r = s->rt;
+}  // extern "C"
+namespace {
+// The fields of p selected by the present mask m (ORCH:353-365).
+void apply_runtime(fx_runtime_params& r, const fx_runtime_params* p, uint32_t m) {
   if (m & FX_RP_WIND) { r.wind[0] = p->wind[0]; r.wind[1] = p->wind[1]; r.wind[2] = p->wind[2]; }
   if (m & FX_RP_WIND_STRENGTH) r.wind_strength = p->wind_strength;
   if (m & FX_RP_GRAVITY) { r.gravity[0] = p->gravity[0]; r.gravity[1] = p->gravity[1]; r.gravity[2] = p->gravity[2]; }
@@ -1273,6 +1446,13 @@ int fx_set_runtime_params(fx_sim* s, const fx_runtime_params* p, uint32_t m) {  
   if (m & FX_RP_VORTICITY_HEAT_MASK) r.vorticity_heat_mask = p->vorticity_heat_mask;
   if (m & FX_RP_VORTICITY_DENSITY_MASK) r.vorticity_density_mask = p->vorticity_density_mask;
   if (m & FX_RP_WORLD_SIZE) r.world_size = p->world_size;
+}
+}  // namespace
+extern "C" {
+int fx_set_runtime_params(fx_sim* s, const fx_runtime_params* p, uint32_t m) {  // ORCH:353-365
+  if (!s || !p) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  ONE_SCENE(s);
+  apply_runtime(s->rt, p, m);
   return FX_OK;
 }
 int fx_get_quality(fx_sim* s, fx_quality* out) {
@@ -1299,6 +1479,7 @@ int fx_set_advection_mode(fx_sim* s, int32_t m) {
 
 int fx_update_sources(fx_sim* s, const float* packed, uint32_t count) {  // SourceInjectionPass.ts:71-83
   if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
+  ONE_SCENE(s);
   if (count && !packed) return fail(s, FX_ERR_INVALID_ARG, "null source data");
   HIP_OK(s, hipSetDevice(s->device));
   const size_t nf = size_t(std::max<uint32_t>(count, 1)) * FX_FLOATS_PER_EMITTER;
@@ -1319,6 +1500,7 @@ int fx_update_sources(fx_sim* s, const float* packed, uint32_t count) {  // Sour
 
 int fx_read_field(fx_sim* s, int32_t id, float* host, size_t bytes) {
   if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  ONE_SCENE(s);
   int esz = 4;
   void* f = field_ptr(s, id, &esz);
   if (!f) return fail(s, FX_ERR_INVALID_ARG, "unknown field id");
@@ -1328,6 +1510,7 @@ int fx_read_field(fx_sim* s, int32_t id, float* host, size_t bytes) {
 
 int fx_write_field(fx_sim* s, int32_t id, const float* host, size_t bytes) {
   if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  ONE_SCENE(s);
   int esz = 4;
   void* f = field_ptr(s, id, &esz);
   if (!f) return fail(s, FX_ERR_INVALID_ARG, "unknown field id");
@@ -1367,6 +1550,7 @@ int fx_write_field(fx_sim* s, int32_t id, const float* host, size_t bytes) {
 
 int fx_read_flags(fx_sim* s, int32_t which, uint32_t* host, size_t count) {
   if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  ONE_SCENE(s);
   if (count != s->nb) return fail(s, FX_ERR_INVALID_ARG, "count must equal the brick count");
   HIP_OK(s, hipSetDevice(s->device));
   HIP_OK(s, hipStreamSynchronize(s->stream));
@@ -1377,6 +1561,7 @@ int fx_read_flags(fx_sim* s, int32_t which, uint32_t* host, size_t count) {
 
 int fx_read_active_list(fx_sim* s, uint32_t* host, size_t capacity, uint32_t* count) {
   if (!s || !count) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  ONE_SCENE(s);
   HIP_OK(s, hipSetDevice(s->device));
   HIP_OK(s, hipStreamSynchronize(s->stream));
   if (s->sparse) {
@@ -1397,6 +1582,7 @@ int fx_read_active_list(fx_sim* s, uint32_t* host, size_t capacity, uint32_t* co
 
 int fx_read_page_table(fx_sim* s, int32_t* host, size_t count) {
   if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  ONE_SCENE(s);
   if (count != s->nb) return fail(s, FX_ERR_INVALID_ARG, "count must equal the brick count");
   HIP_OK(s, hipSetDevice(s->device));
   HIP_OK(s, hipStreamSynchronize(s->stream));
@@ -1411,10 +1597,11 @@ int fx_read_page_table(fx_sim* s, int32_t* host, size_t count) {
   return FX_OK;
 }
 
-void* fx_field_device_ptr(fx_sim* s, int32_t id) { return s ? field_ptr(s, id) : nullptr; }
+void* fx_field_device_ptr(fx_sim* s, int32_t id) { return s && !s->multi ? field_ptr(s, id) : nullptr; }
 
 int fx_read_level(fx_sim* s, int32_t level, int32_t which, float* host, size_t bytes) {
   if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  ONE_SCENE(s);
   if (level < 0 || level >= int(s->lv.size())) return fail(s, FX_ERR_INVALID_ARG, "level out of range");
   Level& L = s->lv[level];
   const float* f = which == FX_LEVEL_DIVERGENCE ? L.div : which == FX_LEVEL_PRESSURE_A ? L.p[0] : L.p[1];
@@ -1423,6 +1610,7 @@ int fx_read_level(fx_sim* s, int32_t level, int32_t which, float* host, size_t b
 }
 int fx_level_dims(fx_sim* s, int32_t level, uint32_t dims[3]) {
   if (!s || !dims) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  ONE_SCENE(s);
   if (level < 0 || level >= int(s->lv.size())) return fail(s, FX_ERR_INVALID_ARG, "level out of range");
   dims[0] = s->lv[level].g.W; dims[1] = s->lv[level].g.H; dims[2] = s->lv[level].g.D;
   return FX_OK;
@@ -1450,6 +1638,177 @@ int fx_active_brick_count(fx_sim* s, uint32_t* out) {
   HIP_OK(s, hipStreamSynchronize(s->stream));
   if (s->sparse) HIP_OK(s, hipMemcpy(out, s->lv[0].L.count, 4, hipMemcpyDeviceToHost));
   else *out = uint32_t(s->nb);
+  return FX_OK;
+}
+
+// ------------------------------------------------ brick-queue batches --
+uint32_t fx_scene_count(const fx_sim* s) { return s ? (s->multi ? uint32_t(s->scenes.size()) : 1u) : 0u; }
+
+}  // extern "C"
+namespace {
+// The scene's fine grid (one scene: the simulation's), or nullptr after a failure.
+const Grid* scene_grid(fx_sim* s, uint32_t k) {
+  if (!s->multi) {
+    if (k != 0) { fail(s, FX_ERR_INVALID_ARG, "scene index out of range"); return nullptr; }
+    return &s->g;
+  }
+  if (k >= s->scenes.size()) { fail(s, FX_ERR_INVALID_ARG, "scene index out of range"); return nullptr; }
+  return &s->scenes[k].g[0];
+}
+// The scene's bricks of the ascending fine list: list positions [k0, k0 + n).
+int scene_list_range(fx_sim* s, const Grid& g, std::vector<uint32_t>& list, uint32_t& k0, uint32_t& n) {
+  uint32_t total = 0;
+  HIP_OK(s, hipStreamSynchronize(s->stream));
+  HIP_OK(s, hipMemcpy(&total, s->lv[0].L.count, 4, hipMemcpyDeviceToHost));
+  list.resize(total);
+  if (total) HIP_OK(s, hipMemcpy(list.data(), s->lv[0].L.list, size_t(total) * 4, hipMemcpyDeviceToHost));
+  const uint32_t lo = g.base, hi = g.base + uint32_t(brick_count(g));
+  k0 = uint32_t(std::lower_bound(list.begin(), list.end(), lo) - list.begin());
+  n = uint32_t(std::lower_bound(list.begin(), list.end(), hi) - list.begin()) - k0;
+  return FX_OK;
+}
+}  // namespace
+extern "C" {
+
+int fx_scene_update_sources(fx_sim* s, uint32_t scene, const float* packed, uint32_t count) {
+  if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
+  if (!s->multi) return scene == 0 ? fx_update_sources(s, packed, count) : fail(s, FX_ERR_INVALID_ARG, "scene index out of range");
+  if (scene >= s->scenes.size()) return fail(s, FX_ERR_INVALID_ARG, "scene index out of range");
+  if (count && !packed) return fail(s, FX_ERR_INVALID_ARG, "null source data");
+  HIP_OK(s, hipSetDevice(s->device));
+  fx_sim::Scene& sc = s->scenes[scene];
+  sc.h_src.assign(size_t(std::max<uint32_t>(count, 1)) * FX_FLOATS_PER_EMITTER, 0.f);
+  if (count) std::memcpy(sc.h_src.data(), packed, size_t(count) * FX_FLOATS_PER_EMITTER * 4);
+  sc.nsrc = count;
+  // every scene's emitters, in scene order (the step constants carry the offsets)
+  std::vector<float> all;
+  for (const auto& x : s->scenes) all.insert(all.end(), x.h_src.begin(), x.h_src.end());
+  HIP_OK(s, hipStreamSynchronize(s->stream));
+  if (all.size() > s->src_cap) {
+    HIP_OK(s, hipFree(s->d_src));
+    s->d_src = nullptr;
+    if (dalloc(s, &s->d_src, all.size()) != FX_OK) return FX_ERR_OUT_OF_MEMORY;
+    s->src_cap = all.size();
+  }
+  HIP_OK(s, hipMemcpy(s->d_src, all.data(), all.size() * 4, hipMemcpyHostToDevice));
+  return FX_OK;
+}
+int fx_scene_get_runtime_params(fx_sim* s, uint32_t scene, fx_runtime_params* out) {
+  if (!s || !out) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  if (!scene_grid(s, scene)) return FX_ERR_INVALID_ARG;
+  *out = s->multi ? s->scenes[scene].rt : s->rt;
+  return FX_OK;
+}
+int fx_scene_set_runtime_params(fx_sim* s, uint32_t scene, const fx_runtime_params* p, uint32_t m) {
+  if (!s || !p) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  if (!scene_grid(s, scene)) return FX_ERR_INVALID_ARG;
+  apply_runtime(s->multi ? s->scenes[scene].rt : s->rt, p, m);
+  return FX_OK;
+}
+int fx_scene_read_field(fx_sim* s, uint32_t scene, int32_t id, float* host, size_t bytes) {
+  if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  const Grid* g = scene_grid(s, scene);
+  if (!g) return FX_ERR_INVALID_ARG;
+  int esz = 4;
+  const uint8_t* f = static_cast<const uint8_t*>(field_ptr(s, id, &esz));
+  if (!f) return fail(s, FX_ERR_INVALID_ARG, "unknown field id");
+  HIP_OK(s, hipSetDevice(s->device));
+  return read_brickmajor_typed(s, *g, f + size_t(g->base) * 512 * size_t(esz), esz, host, bytes);
+}
+int fx_scene_write_field(fx_sim* s, uint32_t scene, int32_t id, const float* host, size_t bytes) {
+  if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  const Grid* gp = scene_grid(s, scene);
+  if (!gp) return FX_ERR_INVALID_ARG;
+  if (!s->multi) return fx_write_field(s, id, host, bytes);
+  const Grid g = *gp;
+  int esz = 4;
+  uint8_t* f = static_cast<uint8_t*>(field_ptr(s, id, &esz));
+  if (!f) return fail(s, FX_ERR_INVALID_ARG, "unknown field id");
+  if (bytes != size_t(g.W) * g.H * g.D * 4) return fail(s, FX_ERR_INVALID_ARG, "buffer size mismatch");
+  HIP_OK(s, hipSetDevice(s->device));
+  const size_t nb = brick_count(g);
+  std::vector<float> tmp(nb * 512, 0.f);
+  scatter_brickmajor(g, host, tmp.data());
+  HIP_OK(s, hipStreamSynchronize(s->stream));
+  uint8_t* dst = f + size_t(g.base) * 512 * size_t(esz);
+  if (esz == 4) {
+    HIP_OK(s, hipMemcpy(dst, tmp.data(), tmp.size() * 4, hipMemcpyHostToDevice));
+  } else {
+    const bool bf = s->desc.precision == FX_PRECISION_BF16;
+    std::vector<uint16_t> h(tmp.size());
+    for (size_t i = 0; i < tmp.size(); ++i) h[i] = bf ? f32_to_bf16(tmp[i]) : f32_to_f16(tmp[i]);
+    HIP_OK(s, hipMemcpy(dst, h.data(), h.size() * 2, hipMemcpyHostToDevice));
+  }
+  // bricks holding non-zero data join the next activation's candidates
+  std::vector<uint32_t> nz;
+  for (size_t b = 0; b < nb; ++b) {
+    const float* q = &tmp[b << 9];
+    for (int k = 0; k < 512; ++k)
+      if (q[k] != 0.f) { nz.push_back(g.base + uint32_t(b)); break; }
+  }
+  if (!nz.empty()) {
+    HIP_OK(s, hipMemcpy(s->d_cand, nz.data(), nz.size() * 4, hipMemcpyHostToDevice));
+    launch_mark(s->stream, s->d_cand, uint32_t(nz.size()), s->touched);
+    HIP_OK(s, hipStreamSynchronize(s->stream));
+    std::vector<uint8_t> m(s->nb, 0);
+    for (uint32_t b : s->pending) m[b] = 1;
+    for (uint32_t b : nz) if (!m[b]) s->pending.push_back(b);
+  }
+  return FX_OK;
+}
+int fx_scene_read_flags(fx_sim* s, uint32_t scene, int32_t which, uint32_t* host, size_t count) {
+  if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  const Grid* g = scene_grid(s, scene);
+  if (!g) return FX_ERR_INVALID_ARG;
+  if (count != brick_count(*g)) return fail(s, FX_ERR_INVALID_ARG, "count must equal the scene's brick count");
+  HIP_OK(s, hipSetDevice(s->device));
+  HIP_OK(s, hipStreamSynchronize(s->stream));
+  const uint32_t* f = which == FX_FLAGS_INJECTED ? s->flagsInj : s->flagsRender;
+  HIP_OK(s, hipMemcpy(host, f + g->base, count * 4, hipMemcpyDeviceToHost));
+  return FX_OK;
+}
+int fx_scene_read_active_list(fx_sim* s, uint32_t scene, uint32_t* host, size_t capacity, uint32_t* count) {
+  if (!s || !count) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  const Grid* g = scene_grid(s, scene);
+  if (!g) return FX_ERR_INVALID_ARG;
+  if (!s->multi) return fx_read_active_list(s, host, capacity, count);
+  HIP_OK(s, hipSetDevice(s->device));
+  std::vector<uint32_t> list;
+  uint32_t k0 = 0, n = 0;
+  int rc = scene_list_range(s, *g, list, k0, n);
+  if (rc != FX_OK) return rc;
+  *count = n;
+  for (uint32_t i = 0; host && i < n && i < capacity; ++i) host[i] = list[k0 + i] - g->base;
+  return FX_OK;
+}
+int fx_scene_read_page_table(fx_sim* s, uint32_t scene, int32_t* host, size_t count) {
+  if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  const Grid* g = scene_grid(s, scene);
+  if (!g) return FX_ERR_INVALID_ARG;
+  if (!s->multi) return fx_read_page_table(s, host, count);
+  if (count != brick_count(*g)) return fail(s, FX_ERR_INVALID_ARG, "count must equal the scene's brick count");
+  HIP_OK(s, hipSetDevice(s->device));
+  std::vector<uint32_t> list;
+  uint32_t k0 = 0, n = 0;
+  int rc = scene_list_range(s, *g, list, k0, n);
+  if (rc != FX_OK) return rc;
+  HIP_OK(s, hipMemcpy(host, s->lv[0].L.page + g->base, count * 4, hipMemcpyDeviceToHost));
+  // positions within the scene's own list
+  for (size_t b = 0; b < count; ++b)
+    if (host[b] >= 0) host[b] -= int32_t(k0);
+  return FX_OK;
+}
+int fx_scene_active_brick_count(fx_sim* s, uint32_t scene, uint32_t* out) {
+  if (!s || !out) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  const Grid* g = scene_grid(s, scene);
+  if (!g) return FX_ERR_INVALID_ARG;
+  if (!s->multi) return fx_active_brick_count(s, out);
+  HIP_OK(s, hipSetDevice(s->device));
+  std::vector<uint32_t> list;
+  uint32_t k0 = 0, n = 0;
+  int rc = scene_list_range(s, *g, list, k0, n);
+  if (rc != FX_OK) return rc;
+  *out = n;
   return FX_OK;
 }
 
@@ -1482,6 +1841,7 @@ static int build_ray_const(fx_sim* s, const fx_camera* c, const fx_render_params
 
 int fx_raymarch_device(fx_sim* s, const fx_camera* c, const fx_render_params* p, uint32_t w, uint32_t h, void* dev) {
   if (!s || !dev) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  ONE_SCENE(s);
   RayConst R;
   int rc = build_ray_const(s, c, p, w, h, R);
   if (rc != FX_OK) return rc;
@@ -1552,6 +1912,7 @@ int fx_bloom(fx_sim* s, const fx_bloom_params* p, uint32_t w, uint32_t h, const 
 
 int fx_raymarch(fx_sim* s, const fx_camera* c, const fx_render_params* p, uint32_t w, uint32_t h, uint16_t* host) {
   if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  ONE_SCENE(s);
   HIP_OK(s, hipSetDevice(s->device));
   void* dev = nullptr;
   HIP_OK(s, hipMalloc(&dev, size_t(w) * h * 8));
@@ -1786,8 +2147,7 @@ void sim_set_blocks_per_cu(fx_sim* s, int per_cu) {
   if (!s->blocks_env) s->blocks = s->cus * std::max(1, per_cu);
 }
 int sim_prepare_projection(fx_sim* s) {
-  fx_sim_params p{};
-  return upload_step(s, p, nullptr, 0);
+  return upload_step(s, std::vector<fx_sim_params>(1), nullptr, 0);
 }
 int sim_project(fx_sim* s, Vec3 us, Vec3 u) {
   const SimView v = sim_view(s);

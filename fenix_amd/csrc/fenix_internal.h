@@ -18,13 +18,25 @@ namespace fx {
 // Dimensions of one grid level and its 8^3 brick tiling. Storage is
 // brick-major: brick b = bx + BX*(by + BY*bz) owns elements [b*512, b*512+512),
 // voxel (x,y,z) sits at b*512 + (z&7)*64 + (y&7)*8 + (x&7).
+// One level of one scene. Several scenes (a brick-queue batch) share each level's
+// storage: scene s owns bricks [base, base + BX*BY*BZ) of it, and a level's
+// launch walks the bricks of every scene. There the grid passed to a kernel is a
+// selector (tab != nullptr): the kernel reads the grid of brick b's scene,
+// tab[owner[b]], which carries that scene's base and index. `span` is the brick
+// count of the level's storage (all scenes), the bound of brick arithmetic.
 struct Grid {
   uint32_t W, H, D;
   uint32_t BX, BY, BZ;
+  uint32_t base;          // first brick of the scene in the level's storage
+  uint32_t scene;         // scene index (its step constants and sources)
+  uint32_t span;          // bricks in the level's storage
+  const Grid* tab;        // selector: device grid of every scene of the level
+  const uint16_t* owner;  // selector: scene of every brick of the level
 };
 
 inline Grid make_grid(uint32_t W, uint32_t H, uint32_t D) {
-  return Grid{W, H, D, (W + 7) / 8, (H + 7) / 8, (D + 7) / 8};
+  const uint32_t BX = (W + 7) / 8, BY = (H + 7) / 8, BZ = (D + 7) / 8;
+  return Grid{W, H, D, BX, BY, BZ, 0u, 0u, BX * BY * BZ, nullptr, nullptr};
 }
 inline size_t brick_count(const Grid& g) { return size_t(g.BX) * g.BY * g.BZ; }
 inline size_t storage_count(const Grid& g) { return brick_count(g) * 512; }
@@ -41,6 +53,7 @@ struct StepConst {
   uint32_t cand_count;    // sparse: injection bricks + bricks written by the host
   uint32_t step_key;      // steps completed: the step term of the stochastic-rounding hash
   uint32_t sr;            // 16-bit stores round stochastically (1) or to nearest even (0)
+  uint32_t src_off;       // first float of the scene's emitters in the source buffer
 };
 
 // Brick list of one level: ascending brick indices, device count, page table.
@@ -57,7 +70,7 @@ struct BrickList {
 // flags == nullptr: every voxel of every listed brick is active (dense).
 struct LvMask {
   const uint32_t* flags;  // fine-level brick flags
-  uint32_t FBX, FBY, FBZ;
+  Grid fine;              // fine level (a selector when several scenes share it)
   uint32_t shift;         // level index
 };
 
@@ -114,13 +127,13 @@ void launch_activate(hipStream_t s, int blocks, const uint32_t* listA, const uin
 struct LevelFlags {
   int n;                                 // coarse levels (0 when dense)
   uint32_t* f[FX_MAX_LEVELS_DEV];        // index 1..n
-  uint32_t BX[FX_MAX_LEVELS_DEV], BY[FX_MAX_LEVELS_DEV], BZ[FX_MAX_LEVELS_DEV];
+  Grid g[FX_MAX_LEVELS_DEV];             // index 1..n (selectors when several scenes share the levels)
 };
 
 // Brick-list compaction of every level (see k_compact_* in fenix_kernels.hip).
 struct CompactArgs {
   int nlevels;
-  Grid g[FX_MAX_LEVELS_DEV];
+  uint32_t nb[FX_MAX_LEVELS_DEV];          // bricks of each level's storage
   uint32_t* flags[FX_MAX_LEVELS_DEV];      // level 0: activation output; l > 0: written
   BrickList L[FX_MAX_LEVELS_DEV];
   uint32_t block_start[FX_MAX_LEVELS_DEV + 1];  // first block of each level (1024 bricks per block)
@@ -166,12 +179,13 @@ constexpr int kPlanFuseShift = 16;
 // once per solve; zeroes the inactive voxels of the level's listed bricks in p0/p1.
 constexpr uint32_t kCoarseCap = 8192;  // active voxels the one-workgroup LDS smoother holds
 constexpr size_t kCoarseTableBytes = size_t(kCoarseCap) * 16;
+// scenes > 1 (g a selector): one table per scene, kCoarseTableBytes apart, counts tab_n[scene].
 void launch_coarse_setup(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, void* tab, uint32_t* tab_n,
-                         float* p0, float* p1);
+                         float* p0, float* p1, uint32_t scenes = 1);
 // tab == nullptr (or an overflowed table): global-memory sweeps in one workgroup.
 void launch_smooth_block(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const StepConst* c,
                          const float* rhs, float* p0, float* p1, int sweeps, int weighted, int zero_start, const void* tab,
-                         const uint32_t* tab_n);
+                         const uint32_t* tab_n, uint32_t scenes = 1);
 // uq: float4 mirror of us (written with it)
 // sq (optional): also packs the scalar gather mirror of set0 over the listed bricks.
 void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
@@ -192,6 +206,10 @@ void launch_confinement(hipStream_t s, int blocks, const uint32_t* list, const u
                         const StepConst* c, Vec3 w, const float* wmag, Vec3 us, float* cmag, Scalars set0);
 void launch_divergence(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
                        Vec3 us, float* div);
+// Listed fine bricks of each scene of a brick-queue batch into out[scene] (tab: the
+// level's grid table).
+void launch_scene_counts(hipStream_t s, const uint32_t* list, const uint32_t* count, const Grid* tab, uint32_t scenes,
+                         uint32_t* out);
 void launch_zero_bricks(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count,
                         float* const* bufs, int nbufs);
 // Retiring bricks of every level, zeroed in one launch: level l's buffers are

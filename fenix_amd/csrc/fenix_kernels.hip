@@ -72,8 +72,38 @@ __device__ __forceinline__ float fractf(float x) { return x - floorf(x); }
 __device__ __forceinline__ uint32_t decu(uint32_t v) { return v == 0u ? 0u : v - 1u; }
 __device__ __forceinline__ uint32_t incu(uint32_t v, uint32_t m) { return min(v + 1u, m); }
 
+// Grid of brick b's scene. MS (several scenes share the level's storage): G is a
+// selector and the scene's grid comes from its table; otherwise G is the grid of
+// the only scene, whose base is 0.
+template <bool MS>
+__device__ __forceinline__ Grid grid_at(const Grid& G, uint32_t b) {
+  if constexpr (MS) {
+    return G.tab[G.owner[b]];
+  } else {
+    Grid g = G;
+    g.base = 0u;
+    g.scene = 0u;
+    return g;
+  }
+}
+// Grid of scene s on the level of selector G.
+template <bool MS>
+__device__ __forceinline__ Grid grid_scene(const Grid& G, uint32_t s) {
+  if constexpr (MS) {
+    return G.tab[s];
+  } else {
+    Grid g = G;
+    g.base = 0u;
+    g.scene = 0u;
+    return g;
+  }
+}
+// A value that is the same in every lane of the wave (brick ids of kernels that
+// give each wave one brick), so that the table loads it feeds are scalar.
+__device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
 __device__ __forceinline__ uint32_t vidx(const Grid& g, uint32_t x, uint32_t y, uint32_t z) {
-  return ((((z >> 3) * g.BY + (y >> 3)) * g.BX + (x >> 3)) << 9) | ((z & 7u) << 6) | ((y & 7u) << 3) | (x & 7u);
+  return ((g.base + ((z >> 3) * g.BY + (y >> 3)) * g.BX + (x >> 3)) << 9) | ((z & 7u) << 6) | ((y & 7u) << 3) | (x & 7u);
 }
 __device__ __forceinline__ uint32_t cidx(const Grid& g, uint32_t x, uint32_t y, uint32_t z) {
   return vidx(g, min(x, g.W - 1), min(y, g.H - 1), min(z, g.D - 1));
@@ -84,7 +114,7 @@ __device__ __forceinline__ bool on_boundary(const Grid& g, uint32_t x, uint32_t 
 // isActiveCoord, src/engine/simulation/shaders/common/active-brick-mask.wgsl.ts:13-18
 __device__ __forceinline__ bool flag_on(const Grid& g, const uint32_t* flags, uint32_t x, uint32_t y, uint32_t z) {
   uint32_t bx = min(x >> 3, g.BX - 1), by = min(y >> 3, g.BY - 1), bz = min(z >> 3, g.BZ - 1);
-  return flags[bx + g.BX * (by + g.BY * bz)] != 0u;
+  return flags[g.base + bx + g.BX * (by + g.BY * bz)] != 0u;
 }
 
 // Voxel owned by this thread inside brick b.
@@ -94,7 +124,8 @@ struct Vox {
 };
 __device__ __forceinline__ Vox vox_of(const Grid& g, uint32_t b) {
   uint32_t t = threadIdx.x;
-  uint32_t bx = b % g.BX, by = (b / g.BX) % g.BY, bz = b / (g.BX * g.BY);
+  const uint32_t lb = b - g.base;
+  uint32_t bx = lb % g.BX, by = (lb / g.BX) % g.BY, bz = lb / (g.BX * g.BY);
   Vox v;
   v.x = bx * 8 + (t & 7u);
   v.y = by * 8 + ((t >> 3) & 7u);
@@ -141,8 +172,8 @@ __device__ __forceinline__ float tri_eval(const Tri& t, const float* __restrict_
 // representable neighbours is stored as the upper one with probability equal to
 // its distance from the lower one in ulps, so the stored value is unbiased and
 // per-step increments below half an ulp survive on average. The decision compares
-// the dropped low bits with bits of a 32-bit hash of (brick-major voxel index,
-// step, pass, field) (MurmurHash3's fmix32 finalizer), so it is reproducible.
+// the dropped low bits with bits of a 32-bit hash of (brick-major voxel index
+// within the scene, step, pass, field) (MurmurHash3's fmix32 finalizer), so it is reproducible.
 __host__ __device__ __forceinline__ uint32_t fmix32(uint32_t x) {
   x ^= x >> 16; x *= 0x85ebca6bu; x ^= x >> 13; x *= 0xc2b2ae35u; x ^= x >> 16;
   return x;
@@ -171,7 +202,7 @@ __device__ __forceinline__ uint16_t f16_sr(float v, uint32_t h) {
 
 struct StF32 {
   using T = float;
-  static __device__ __forceinline__ void st_sr(T* p, uint32_t i, float v, uint32_t, uint32_t) { p[i] = v; }
+  static __device__ __forceinline__ void st_sr(T* p, uint32_t i, float v, uint32_t, uint32_t, uint32_t) { p[i] = v; }
   static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return p[i]; }
   static __device__ __forceinline__ void st(T* p, uint32_t i, float v) { p[i] = v; }
   static __device__ __forceinline__ float4 ld4(const T* p, uint32_t i) { return *reinterpret_cast<const float4*>(p + i); }
@@ -186,9 +217,10 @@ struct StBF16 {
   }
   static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return __uint_as_float(uint32_t(p[i]) << 16); }
   static __device__ __forceinline__ void st(T* p, uint32_t i, float v) { p[i] = enc(v); }
-  // sr != 0: stochastic rounding with the hash term of the store site
-  static __device__ __forceinline__ void st_sr(T* p, uint32_t i, float v, uint32_t sr, uint32_t site) {
-    p[i] = sr ? bf16_sr(v, sr_hash(i, site)) : enc(v);
+  // sr != 0: stochastic rounding with the hash term of the store site; hi is the
+  // voxel's brick-major index within its scene (i less the scene's first element)
+  static __device__ __forceinline__ void st_sr(T* p, uint32_t i, float v, uint32_t sr, uint32_t site, uint32_t hi) {
+    p[i] = sr ? bf16_sr(v, sr_hash(hi, site)) : enc(v);
   }
   static __device__ __forceinline__ float4 ld4(const T* p, uint32_t i) {
     const uint2 w = *reinterpret_cast<const uint2*>(p + i);
@@ -204,8 +236,8 @@ struct StF16 {
   using T = __half;
   static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return __half2float(p[i]); }
   static __device__ __forceinline__ void st(T* p, uint32_t i, float v) { p[i] = __float2half(v); }
-  static __device__ __forceinline__ void st_sr(T* p, uint32_t i, float v, uint32_t sr, uint32_t site) {
-    p[i] = sr ? __ushort_as_half(f16_sr(v, sr_hash(i, site))) : __float2half(v);
+  static __device__ __forceinline__ void st_sr(T* p, uint32_t i, float v, uint32_t sr, uint32_t site, uint32_t hi) {
+    p[i] = sr ? __ushort_as_half(f16_sr(v, sr_hash(hi, site))) : __float2half(v);
   }
   static __device__ __forceinline__ float4 ld4(const T* p, uint32_t i) {
     const uint2 w = *reinterpret_cast<const uint2*>(p + i);
@@ -282,11 +314,11 @@ struct RowCtx {
 __device__ __forceinline__ RowCtx row_ctx(const Grid& g, uint32_t b, uint32_t t) {
   RowCtx R;
   const uint32_t h = t & 1u, ly = (t >> 1) & 7u, lz = t >> 4, x0 = h * 4u;
-  const uint32_t BX = g.BX, BXY = g.BX * g.BY;
+  const uint32_t BX = g.BX, BXY = g.BX * g.BY, lb = b - g.base;
   R.h = h;
-  R.x = (b % g.BX) * 8 + x0;
-  R.y = ((b / g.BX) % g.BY) * 8 + ly;
-  R.z = (b / BXY) * 8 + lz;
+  R.x = (lb % g.BX) * 8 + x0;
+  R.y = ((lb / g.BX) % g.BY) * 8 + ly;
+  R.z = (lb / BXY) * 8 + lz;
   R.i0 = (b << 9) | (lz << 6) | (ly << 3) | x0;
   R.iyp = R.y + 1u < g.H ? (ly < 7u ? R.i0 + 8u : ((b + BX) << 9) | (lz << 6) | x0) : R.i0;
   R.iym = R.y > 0u ? (ly > 0u ? R.i0 - 8u : ((b - BX) << 9) | (lz << 6) | (7u << 3) | x0) : R.i0;
@@ -442,20 +474,23 @@ __device__ __noinline__ F3 turbulent_dir(F3 pos, float seed) {  // SH/inject-vel
 // SH/inject-scalar.wgsl.ts:32-70, SH/inject-velocity.wgsl.ts:46-96 and
 // SH/inject-igniter.wgsl.ts:32-66 fused per voxel in dispatch order (each pass
 // touches only its own voxel, so fusing keeps the sequential result).
-template <class ST>
-__global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                const StepConst* __restrict__ C, const float* __restrict__ srcs,
+template <class ST, bool MS>
+__global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
+                                                const StepConst* __restrict__ C0, const float* __restrict__ srcs0,
                                                 SS<ST> S, Vec3 u, uint32_t* touched, int parts) {
-  const fx_sim_params& P = C->p;
-  const uint32_t ns = C->nsrc;
-  const uint32_t sr = C->sr;
-  const uint32_t s0 = sr_site(C, 0), s1 = sr_site(C, 1), s2 = sr_site(C, 2), s5 = sr_site(C, 5);
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
+    const Grid g = grid_at<MS>(G, b);
+    const StepConst* __restrict__ C = C0 + g.scene;
+    const fx_sim_params& P = C->p;
+    const uint32_t ns = C->nsrc;
+    const uint32_t sr = C->sr;
+    const uint32_t s0 = sr_site(C, 0), s1 = sr_site(C, 1), s2 = sr_site(C, 2), s5 = sr_site(C, 5);
+    const float* __restrict__ srcs = srcs0 + C->src_off;
     if (touched && threadIdx.x == 0) touched[b] = 1u;
     Vox v = vox_of(g, b);
     if (!v.in) continue;
-    const uint32_t i = v.i;
+    const uint32_t i = v.i, hi = i - (g.base << 9);
     F3 pos = f3((float(v.x) + 0.5f) / float(g.W), (float(v.y) + 0.5f) / float(g.H), (float(v.z) + 0.5f) / float(g.D));
     if (parts & 1) {
       float den = ST::ld(S.d, i), tem = ST::ld(S.t, i), fu = ST::ld(S.f, i);
@@ -477,7 +512,7 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
         tem = clamp01(tem + s.yt() * contribution);
         fu = clamp01(fu + s.yf() * contribution);
       }
-      ST::st_sr(S.d, i, den, sr, s0); ST::st_sr(S.t, i, tem, sr, s1); ST::st_sr(S.f, i, fu, sr, s2);
+      ST::st_sr(S.d, i, den, sr, s0, hi); ST::st_sr(S.t, i, tem, sr, s1, hi); ST::st_sr(S.f, i, fu, sr, s2, hi);
     }
     if (parts & 2) {
       F3 vel = f3(u.x[i], u.y[i], u.z[i]);
@@ -536,7 +571,7 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
         rea = clamp01(rea + seedv);
         tem = clamp01(tem + intensity * weight * P.delta_time * 4.f);
       }
-      S.r[i] = rea; ST::st_sr(S.t, i, tem, sr, s5);
+      S.r[i] = rea; ST::st_sr(S.t, i, tem, sr, s5, hi);
     }
   }
 }
@@ -559,11 +594,14 @@ __global__ void k_fill_f32(float* p, size_t n) {
 // REFERENCE: SH/compute-active-bricks.wgsl.ts:24-79 (one thread per brick).
 // THRESHOLD: density > a or |u| > b anywhere in the brick, dilation 1 (one
 // workgroup per brick). Candidates = listA (+ listB).
+// Marks fine brick (x, y, z) of g's scene and the coarse bricks that hold it.
+template <bool MS>
 __device__ __forceinline__ void mark(const Grid& g, uint32_t* flags, const LevelFlags& lf, uint32_t x, uint32_t y, uint32_t z) {
-  flags[x + g.BX * (y + g.BY * z)] = 1u;
+  flags[g.base + x + g.BX * (y + g.BY * z)] = 1u;
   for (int l = 1; l <= lf.n; ++l) {
-    uint32_t cx = min(x >> l, lf.BX[l] - 1), cy = min(y >> l, lf.BY[l] - 1), cz = min(z >> l, lf.BZ[l] - 1);
-    lf.f[l][cx + lf.BX[l] * (cy + lf.BY[l] * cz)] = 1u;
+    const Grid c = grid_scene<MS>(lf.g[l], g.scene);
+    uint32_t cx = min(x >> l, c.BX - 1), cy = min(y >> l, c.BY - 1), cz = min(z >> l, c.BZ - 1);
+    lf.f[l][c.base + cx + c.BX * (cy + c.BY * cz)] = 1u;
   }
 }
 __device__ __forceinline__ void dilate(const Grid& g, uint32_t b, int r, uint32_t* flags, const LevelFlags& lf) {
@@ -574,7 +612,7 @@ __device__ __forceinline__ void dilate(const Grid& g, uint32_t b, int r, uint32_
         int nx = min(max(bx + x, 0), int(g.BX) - 1);
         int ny = min(max(by + y, 0), int(g.BY) - 1);
         int nz = min(max(bz + z, 0), int(g.BZ) - 1);
-        mark(g, flags, lf, uint32_t(nx), uint32_t(ny), uint32_t(nz));
+        mark<false>(g, flags, lf, uint32_t(nx), uint32_t(ny), uint32_t(nz));
       }
 }
 template <class ST>
@@ -608,21 +646,24 @@ __global__ __launch_bounds__(256) void k_activate_reference(const uint32_t* __re
     dilate(g, b, (peak > 0.05f || ce > 0.035f) ? 1 : 0, flags, lf);
   }
 }
-template <class ST>
+template <class ST, bool MS>
 __global__ __launch_bounds__(512) void k_activate_threshold(const uint32_t* __restrict__ listA, const uint32_t* __restrict__ countA,
                                                             const uint32_t* __restrict__ listB, const uint32_t* __restrict__ countB,
-                                                            Grid g, const StepConst* __restrict__ C, const typename ST::T* __restrict__ den,
+                                                            Grid G, const StepConst* __restrict__ C0, const typename ST::T* __restrict__ den,
                                                             Vec3 vel, uint32_t* flags, LevelFlags lf) {
   FX_SIM_PRIORITY();
   // One wave per brick (8 bricks per block): lane (x, y) of the brick reads its
   // column of 8 voxels, ballots reduce; flags are only ever set to 1, so the
   // 27 neighbour marks run on 27 lanes at once.
   const uint32_t na = *countA, nb = listB ? *countB : 0u;
-  const float dth = C->act_density, sth = C->act_speed;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   for (uint32_t k = blockIdx.x * 8 + wave; k < na + nb; k += gridDim.x * 8) {
-    const uint32_t b = k < na ? listA[k] : listB[k - na];
-    const uint32_t bx = b % g.BX, by = (b / g.BX) % g.BY, bz = b / (g.BX * g.BY);
+    const uint32_t b = wave_uniform(k < na ? listA[k] : listB[k - na]);
+    const Grid g = grid_at<MS>(G, b);
+    const StepConst* __restrict__ C = C0 + g.scene;
+    const float dth = C->act_density, sth = C->act_speed;
+    const uint32_t lb = b - g.base;
+    const uint32_t bx = lb % g.BX, by = (lb / g.BX) % g.BY, bz = lb / (g.BX * g.BY);
     const uint32_t x = bx * 8 + (lane & 7u), y = by * 8 + (lane >> 3);
     bool dens = false, fast = false;
 #pragma unroll
@@ -640,10 +681,10 @@ __global__ __launch_bounds__(512) void k_activate_threshold(const uint32_t* __re
         const int nx = min(max(int(bx) + int(lane % 3) - 1, 0), int(g.BX) - 1);
         const int ny = min(max(int(by) + int((lane / 3) % 3) - 1, 0), int(g.BY) - 1);
         const int nz = min(max(int(bz) + int(lane / 9) - 1, 0), int(g.BZ) - 1);
-        mark(g, flags, lf, uint32_t(nx), uint32_t(ny), uint32_t(nz));
+        mark<MS>(g, flags, lf, uint32_t(nx), uint32_t(ny), uint32_t(nz));
       }
     } else if (__ballot(fast) != 0ull && lane == 0) {
-      mark(g, flags, lf, bx, by, bz);
+      mark<MS>(g, flags, lf, bx, by, bz);
     }
   }
 }
@@ -692,7 +733,7 @@ __global__ __launch_bounds__(256) void k_compact_count(CompactArgs A) {
   int l;
   uint32_t base;
   block_of(A, blockIdx.x, &l, &base);
-  const uint32_t nb = A.g[l].BX * A.g[l].BY * A.g[l].BZ;
+  const uint32_t nb = A.nb[l];
   const BrickList L = A.L[l];
   uint32_t on_n = 0, ret_n = 0;
   for (uint32_t k = 0; k < 4; ++k) {
@@ -746,7 +787,7 @@ __global__ __launch_bounds__(256) void k_compact_write(CompactArgs A) {
   int l;
   uint32_t base;
   block_of(A, blockIdx.x, &l, &base);
-  const uint32_t nb = A.g[l].BX * A.g[l].BY * A.g[l].BZ;
+  const uint32_t nb = A.nb[l];
   const BrickList L = A.L[l];
   uint32_t on[4], ret[4];
   uint32_t on_n = 0, ret_n = 0;
@@ -806,13 +847,15 @@ __global__ __launch_bounds__(512) void k_zero_bricks(const uint32_t* __restrict_
 // SH/advect-velocity-semilagrangian.wgsl.ts:23-41
 // sq (optional): also packs the scalar gather mirror (d, T, f, r) of set S for
 // every voxel of the listed bricks, which the scalar advection reads next.
-template <class ST>
-__global__ __launch_bounds__(512) void k_advect_velocity(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                         const StepConst* __restrict__ C, Vec3 u, Vec3 us, float4* __restrict__ uq,
+template <class ST, bool MS>
+__global__ __launch_bounds__(512) void k_advect_velocity(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
+                                                         const StepConst* __restrict__ C0, Vec3 u, Vec3 us, float4* __restrict__ uq,
                                                          SS<ST> S, float4* __restrict__ sq) {
-  const float dt = C->p.delta_time;
   FX_BRICK_LOOP(list, count) {
-    Vox v = vox_of(g, list[fx_k]);
+    const uint32_t b = list[fx_k];
+    const Grid g = grid_at<MS>(G, b);
+    const float dt = C0[g.scene].p.delta_time;
+    Vox v = vox_of(g, b);
     if (sq) sq[v.i] = make_float4(ST::ld(S.d, v.i), ST::ld(S.t, v.i), ST::ld(S.f, v.i), S.r[v.i]);
     if (!v.in) continue;
     if (on_boundary(g, v.x, v.y, v.z)) {
@@ -852,9 +895,10 @@ __device__ __forceinline__ void decay_write(const Grid& g, uint32_t i, uint32_t 
   float smokeLoss = dt * (0.008f + ny * 0.026f) * mixf(0.75f, 1.38f, tn);
   float coldBreakup = dt * den * (1.f - smoothf(0.2f, 0.86f, hot)) * smoothf(0.18f, 0.72f, den) * smoothf(0.34f, 0.92f, tn) * 0.34f;
   float clearing = dt * den * hot * mixf(0.1f, 0.72f, tn);
-  ST::st_sr(D.d, i, clamp01(fmaxf(0.f, den * (1.f - smokeLoss) - clearing - coldBreakup)), sr, site[0]);
-  ST::st_sr(D.t, i, clamp01(fmaxf(0.f, tem - cooling)), sr, site[1]);
-  ST::st_sr(D.f, i, clamp01(fmaxf(0.f, fu - dt * mixf(0.012f, 0.034f, tn))), sr, site[2]);
+  const uint32_t hi = i - (g.base << 9);
+  ST::st_sr(D.d, i, clamp01(fmaxf(0.f, den * (1.f - smokeLoss) - clearing - coldBreakup)), sr, site[0], hi);
+  ST::st_sr(D.t, i, clamp01(fmaxf(0.f, tem - cooling)), sr, site[1], hi);
+  ST::st_sr(D.f, i, clamp01(fmaxf(0.f, fu - dt * mixf(0.012f, 0.034f, tn))), sr, site[2], hi);
   D.r[i] = clamp01(fmaxf(0.f, rea - dt * mixf(0.46f, 0.82f, tn)));
 }
 // Gather mirrors for the scalar advection: the four scalars (d, T, f, r) and the
@@ -895,15 +939,18 @@ __global__ __launch_bounds__(512) void k_pack_gather(const uint32_t* __restrict_
 // SH/advect-scalar-maccormack.wgsl.ts:33-101 / SH/advect-scalar-semilagrangian.wgsl.ts:26-44.
 // The four advect<Name> calls share every position; sampleVelocity(coord-0.5)
 // and the `original` sample hit the voxel exactly (fraction 0).
-template <class ST>
-__global__ __launch_bounds__(512) void k_advect_scalars(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                        const StepConst* __restrict__ C, int mode, const float4* __restrict__ sq,
+template <class ST, bool MS>
+__global__ __launch_bounds__(512) void k_advect_scalars(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
+                                                        const StepConst* __restrict__ C0, int mode, const float4* __restrict__ sq,
                                                         const float4* __restrict__ uq, SS<ST> D, const uint32_t* __restrict__ flags) {
-  const float dt = C->p.delta_time, time = C->p.time;
-  const uint32_t sr = C->sr;
-  const uint32_t site[3] = {sr_site(C, 8), sr_site(C, 9), sr_site(C, 10)};
   FX_BRICK_LOOP(list, count) {
-    Vox v = vox_of(g, list[fx_k]);
+    const uint32_t b = list[fx_k];
+    const Grid g = grid_at<MS>(G, b);
+    const StepConst* __restrict__ C = C0 + g.scene;
+    const float dt = C->p.delta_time, time = C->p.time;
+    const uint32_t sr = C->sr;
+    const uint32_t site[3] = {sr_site(C, 8), sr_site(C, 9), sr_site(C, 10)};
+    Vox v = vox_of(g, b);
     if (!v.in) continue;
     const uint32_t i = v.i;
     if (!flag_on(g, flags, v.x, v.y, v.z)) { ST::st(D.d, i, 0.f); ST::st(D.t, i, 0.f); ST::st(D.f, i, 0.f); D.r[i] = 0.f; continue; }
@@ -947,15 +994,18 @@ __device__ __forceinline__ float thermal_cluster(F3 c, float time) {
   float rising = 0.5f + 0.5f * sin_s(p.x * 0.18f + p.y * 0.13f - p.z * 0.16f + time * 1.4f);
   return clampf(body * 0.46f + pocket * 0.34f + rising * 0.2f, 0.f, 1.f);
 }
-template <class ST>
-__global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                    const StepConst* __restrict__ C, SS<ST> S, SS<ST> D,
+template <class ST, bool MS>
+__global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
+                                                    const StepConst* __restrict__ C0, SS<ST> S, SS<ST> D,
                                                     const uint32_t* __restrict__ flags) {
-  const float dt = C->p.delta_time, time = C->p.time;
-  const float ilo = C->ignition_lo, ihi = C->ignition_hi, rate = C->burn_rate;
-  const uint32_t sr = C->sr, sd = sr_site(C, 12), st = sr_site(C, 13), sf = sr_site(C, 14);
   FX_BRICK_LOOP(list, count) {
-    Vox v = vox_of(g, list[fx_k]);
+    const uint32_t b = list[fx_k];
+    const Grid g = grid_at<MS>(G, b);
+    const StepConst* __restrict__ C = C0 + g.scene;
+    const float dt = C->p.delta_time, time = C->p.time;
+    const float ilo = C->ignition_lo, ihi = C->ignition_hi, rate = C->burn_rate;
+    const uint32_t sr = C->sr, sd = sr_site(C, 12), st = sr_site(C, 13), sf = sr_site(C, 14);
+    Vox v = vox_of(g, b);
     if (!v.in) continue;
     const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
     if (!flag_on(g, flags, x, y, z)) { ST::st(D.d, i, ST::ld(S.d, i)); ST::st(D.t, i, ST::ld(S.t, i)); ST::st(D.f, i, ST::ld(S.f, i)); D.r[i] = S.r[i]; continue; }
@@ -982,21 +1032,24 @@ __global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__
     float clearing = den * smoothk(0.18f, 0.82f, tem + rea * 0.22f + fu * 0.1f) * dt * mixf(0.55f, 2.45f, hotPocket);
     float coolingSmoke = den * tem * (coolingPocket + hotSoot * 0.32f) * dt * 0.16f;
     float reactionDecay = rea * (1.f - dt * mixf(0.68f, 0.22f, hotPocket));
-    ST::st_sr(D.f, i, clamp01(fu - burn), sr, sf);
-    ST::st_sr(D.t, i, clamp01(tem + burn * mixf(2.1f, 3.25f, hotPocket) + radiative - heatLoss), sr, st);
-    ST::st_sr(D.d, i, clamp01(fmaxf(0.f, den + smokeYield + coolingSmoke - densityLoss - clearing * 0.72f)), sr, sd);
+    const uint32_t hi = i - (g.base << 9);
+    ST::st_sr(D.f, i, clamp01(fu - burn), sr, sf, hi);
+    ST::st_sr(D.t, i, clamp01(tem + burn * mixf(2.1f, 3.25f, hotPocket) + radiative - heatLoss), sr, st, hi);
+    ST::st_sr(D.d, i, clamp01(fmaxf(0.f, den + smokeYield + coolingSmoke - densityLoss - clearing * 0.72f)), sr, sd, hi);
     D.r[i] = clamp01(fmaxf(reactionDecay, burn * mixf(4.f, 7.5f, hotPocket) + radiative * 1.8f));
   }
 }
 
 // ------------------------------------------------------------ buoyancy --
 // SH/apply-buoyancy.wgsl.ts:27-110 (render set, velocityScratch in place).
-template <class ST>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_buoyancy(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                  const StepConst* __restrict__ C, SS<ST> S, Vec3 us, const uint32_t* __restrict__ flags) {
-  const fx_sim_params& P = C->p;
+template <class ST, bool MS>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_buoyancy(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
+                                                  const StepConst* __restrict__ C0, SS<ST> S, Vec3 us, const uint32_t* __restrict__ flags) {
   FX_BRICK_LOOP(list, count) {
-    Vox v = vox_of(g, list[fx_k]);
+    const uint32_t b = list[fx_k];
+    const Grid g = grid_at<MS>(G, b);
+    const fx_sim_params& P = C0[g.scene].p;
+    Vox v = vox_of(g, b);
     if (!v.in) continue;
     const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
     if (!flag_on(g, flags, x, y, z)) continue;
@@ -1048,10 +1101,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
 // ------------------------------------------------ vorticity/confinement --
 // SH/compute-vorticity.wgsl.ts:23-50, on float4 brick rows: x neighbours of
 // u_y and u_z, y neighbours of u_x and u_z, z neighbours of u_x and u_y.
-__global__ __launch_bounds__(512) void k_vorticity(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g, Vec3 us,
+template <bool MS>
+__global__ __launch_bounds__(512) void k_vorticity(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G, Vec3 us,
                                                    Vec3 w, float* wmag) {
   FX_ROW_LOOP(list, count) {
-    const RowCtx R = row_ctx(g, list[fx_k], fx_t);
+    const uint32_t b = wave_uniform(list[fx_k]);
+    const Grid g = grid_at<MS>(G, b);
+    const RowCtx R = row_ctx(g, b, fx_t);
     float c[4], yxp[4], yxm[4], zxp[4], zxm[4], xyp[4], xym[4], zyp[4], zym[4], xzp[4], xzm[4], yzp[4], yzm[4];
     row_x<StF32>(g, R, us.y, c, yxp, yxm);
     row_x<StF32>(g, R, us.z, c, zxp, zxm);
@@ -1074,13 +1130,15 @@ __global__ __launch_bounds__(512) void k_vorticity(const uint32_t* __restrict__ 
   }
 }
 // SH/apply-vorticity-confinement.wgsl.ts:30-66 (render set)
-template <class ST>
-__global__ __launch_bounds__(512) void k_confinement(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                     const StepConst* __restrict__ C, Vec3 w, const float* __restrict__ wmag, Vec3 us,
+template <class ST, bool MS>
+__global__ __launch_bounds__(512) void k_confinement(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
+                                                     const StepConst* __restrict__ C0, Vec3 w, const float* __restrict__ wmag, Vec3 us,
                                                      float* cmag, SS<ST> S) {
-  const fx_sim_params& P = C->p;
   FX_BRICK_LOOP(list, count) {
-    Vox v = vox_of(g, list[fx_k]);
+    const uint32_t b = list[fx_k];
+    const Grid g = grid_at<MS>(G, b);
+    const fx_sim_params& P = C0[g.scene].p;
+    Vox v = vox_of(g, b);
     if (!v.in) continue;
     const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
     if (on_boundary(g, x, y, z)) { cmag[i] = 0.f; continue; }
@@ -1102,10 +1160,13 @@ __global__ __launch_bounds__(512) void k_confinement(const uint32_t* __restrict_
 
 // ------------------------------------------------ divergence / pressure --
 // SH/compute-divergence.wgsl.ts:22-42, on float4 brick rows.
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_divergence(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g, Vec3 us,
+template <bool MS>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_divergence(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G, Vec3 us,
                                                     float* div) {
   FX_ROW_LOOP(list, count) {
-    const RowCtx R = row_ctx(g, list[fx_k], fx_t);
+    const uint32_t b = wave_uniform(list[fx_k]);
+    const Grid g = grid_at<MS>(G, b);
+    const RowCtx R = row_ctx(g, b, fx_t);
     float c[4], xp[4], xm[4], yp[4], ym[4], zp[4], zm[4], out[4];
     row_x<StF32>(g, R, us.x, c, xp, xm);
     row_y<StF32>(R, us.y, yp, ym);
@@ -1119,11 +1180,16 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     row_store<StF32>(R, div, out);
   }
 }
-__device__ __forceinline__ bool lv_active(const LvMask& m, uint32_t x, uint32_t y, uint32_t z) {
+// fg: the fine grid of the voxel's scene (grid_scene of m.fine).
+__device__ __forceinline__ bool lv_active(const LvMask& m, const Grid& fg, uint32_t x, uint32_t y, uint32_t z) {
   if (!m.flags) return true;
-  uint32_t fx = min((x << m.shift) >> 3, m.FBX - 1), fy = min((y << m.shift) >> 3, m.FBY - 1),
-           fz = min((z << m.shift) >> 3, m.FBZ - 1);
-  return m.flags[fx + m.FBX * (fy + m.FBY * fz)] != 0u;
+  uint32_t fx = min((x << m.shift) >> 3, fg.BX - 1), fy = min((y << m.shift) >> 3, fg.BY - 1),
+           fz = min((z << m.shift) >> 3, fg.BZ - 1);
+  return m.flags[fg.base + fx + fg.BX * (fy + fg.BY * fz)] != 0u;
+}
+// One scene only (the reference schedule's kernels).
+__device__ __forceinline__ bool lv_active(const LvMask& m, uint32_t x, uint32_t y, uint32_t z) {
+  return lv_active(m, grid_scene<false>(m.fine, 0u), x, y, z);
 }
 __device__ __forceinline__ float nsum(const Grid& g, const float* __restrict__ s, uint32_t x, uint32_t y, uint32_t z) {
   return s[cidx(g, incu(x, g.W - 1), y, z)] + s[cidx(g, decu(x), y, z)] + s[cidx(g, x, incu(y, g.H - 1), z)] +
@@ -1178,7 +1244,33 @@ __device__ __forceinline__ uint32_t rank_in_brick(const uint64_t* am, uint32_t k
   return r + uint32_t(__popcll(am[k * 8 + (t >> 6)] & ((1ull << (t & 63u)) - 1ull)));
 }
 
-__global__ __launch_bounds__(1024) void k_coarse_setup(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+// First list position at or after brick b in an ascending brick list of n entries.
+__device__ __forceinline__ uint32_t list_lower_bound(const uint32_t* __restrict__ list, uint32_t n, uint32_t b) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (list[mid] < b) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+// The listed bricks of g's scene: positions [k0, k0 + n) of the level's ascending list.
+template <bool MS>
+__device__ __forceinline__ void scene_list(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, const Grid& g,
+                                           uint32_t& k0, uint32_t& n) {
+  const uint32_t all = *count;
+  if constexpr (MS) {
+    k0 = list_lower_bound(list, all, g.base);
+    n = list_lower_bound(list, all, g.base + g.BX * g.BY * g.BZ) - k0;
+  } else {
+    k0 = 0u;
+    n = all;
+  }
+}
+
+// Several scenes (MS): workgroup row blockIdx.y builds the table of scene
+// blockIdx.y (tab + kSmoothCap * scene, tab_n + scene) from its bricks of the list.
+template <bool MS>
+__global__ __launch_bounds__(1024) void k_coarse_setup(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
                                                        LvBits lb, uint4* __restrict__ tab, uint32_t* __restrict__ tab_n, float* p0,
                                                        float* p1) {
   FX_SIM_PRIORITY();
@@ -1186,7 +1278,13 @@ __global__ __launch_bounds__(1024) void k_coarse_setup(const uint32_t* __restric
   __shared__ uint32_t base[kSmoothBricks];
   __shared__ int16_t slot[kSmoothTable];
   __shared__ uint32_t wtot[16];
-  const uint32_t n = *count, nbl = g.BX * g.BY * g.BZ;
+  const Grid g = grid_scene<MS>(G, blockIdx.y);
+  tab += size_t(kSmoothCap) * blockIdx.y;
+  tab_n += blockIdx.y;
+  uint32_t k0, n;
+  scene_list<MS>(list, count, g, k0, n);
+  list += k0;
+  const uint32_t nbl = g.BX * g.BY * g.BZ;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const bool lead = blockIdx.x == 0 && tid == 0;
   if (n == 0 || n > kSmoothBricks || nbl > kSmoothTable) {
@@ -1195,8 +1293,8 @@ __global__ __launch_bounds__(1024) void k_coarse_setup(const uint32_t* __restric
   }
   for (uint32_t q = tid; q < nbl; q += 1024) slot[q] = -1;
   __syncthreads();
-  for (uint32_t k = tid; k < n; k += 1024) slot[list[k]] = int16_t(k);
-  for (uint32_t wd = tid; wd < n * 8; wd += 1024) amask[wd] = lb.bits[wd];
+  for (uint32_t k = tid; k < n; k += 1024) slot[list[k] - g.base] = int16_t(k);
+  for (uint32_t wd = tid; wd < n * 8; wd += 1024) amask[wd] = lb.bits[size_t(k0) * 8 + wd];
   __syncthreads();
   // exclusive scan of the per-brick counts (n <= 512 <= blockDim)
   uint32_t cnt = 0;
@@ -1232,9 +1330,9 @@ __global__ __launch_bounds__(1024) void k_coarse_setup(const uint32_t* __restric
       r -= c;
     }
     const uint32_t t = wd * 64 + select_bit(amask[lo * 8 + wd], r);
-    const uint32_t b = list[lo];
-    const uint32_t x = (b % g.BX) * 8 + (t & 7u), y = ((b / g.BX) % g.BY) * 8 + ((t >> 3) & 7u),
-                   z = (b / (g.BX * g.BY)) * 8 + (t >> 6);
+    const uint32_t b = list[lo], lb0 = b - g.base;
+    const uint32_t x = (lb0 % g.BX) * 8 + (t & 7u), y = ((lb0 / g.BX) % g.BY) * 8 + ((t >> 3) & 7u),
+                   z = (lb0 / (g.BX * g.BY)) * 8 + (t >> 6);
     const uint32_t nx[6] = {x + 1, x - 1, x, x, x, x}, ny[6] = {y, y, y + 1, y - 1, y, y}, nz[6] = {z, z, z, z, z + 1, z - 1};
     uint32_t nb[6];
 #pragma unroll
@@ -1251,20 +1349,27 @@ __global__ __launch_bounds__(1024) void k_coarse_setup(const uint32_t* __restric
   for (uint32_t q = first; q < n * 512u; q += stride) {
     const uint32_t k = q >> 9, t = q & 511u;
     if ((amask[k * 8 + (t >> 6)] >> (t & 63u)) & 1ull) continue;
-    const uint32_t b = list[k];
-    const uint32_t x = (b % g.BX) * 8 + (t & 7u), y = ((b / g.BX) % g.BY) * 8 + ((t >> 3) & 7u), z = (b / (g.BX * g.BY)) * 8 + (t >> 6);
+    const uint32_t b = list[k], lb0 = b - g.base;
+    const uint32_t x = (lb0 % g.BX) * 8 + (t & 7u), y = ((lb0 / g.BX) % g.BY) * 8 + ((t >> 3) & 7u), z = (lb0 / (g.BX * g.BY)) * 8 + (t >> 6);
     if (x < g.W && y < g.H && z < g.D) { p0[(b << 9) | t] = 0.f; p1[(b << 9) | t] = 0.f; }
   }
   if (lead) *tab_n = nact;
 }
 
-__global__ __launch_bounds__(1024) void k_smooth_block(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                       LvMask m, const StepConst* __restrict__ C, const float* __restrict__ rhs,
+// Several scenes (MS): workgroup blockIdx.y smooths scene blockIdx.y.
+template <bool MS>
+__global__ __launch_bounds__(1024) void k_smooth_block(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
+                                                       LvMask m, const StepConst* __restrict__ C0, const float* __restrict__ rhs,
                                                        float* p0, float* p1, int sweeps, int weighted, int zero_start,
                                                        const uint4* __restrict__ tab, const uint32_t* __restrict__ tab_n) {
   FX_SIM_PRIORITY();
   __shared__ float sp[2][kSmoothCap + 1];  // slot kSmoothCap holds 0: the missing neighbour
-  const float om = C->omega;
+  const Grid g = grid_scene<MS>(G, blockIdx.y);
+  const float om = C0[g.scene].omega;
+  if (tab) {
+    tab += size_t(kSmoothCap) * blockIdx.y;
+    tab_n += blockIdx.y;
+  }
   const uint32_t tid = threadIdx.x;
   const uint32_t nact = tab ? *tab_n : kSmoothOverflow;
   if (nact != kSmoothOverflow) {
@@ -1313,18 +1418,21 @@ __global__ __launch_bounds__(1024) void k_smooth_block(const uint32_t* __restric
       if (gid[j] != 0xffffffffu) dst[gid[j]] = pv[j];
     return;
   }
-  const uint32_t total = *count * 512u;
+  uint32_t k0, nl;
+  scene_list<MS>(list, count, g, k0, nl);
+  const uint32_t total = nl * 512u;
+  const Grid fg = grid_scene<MS>(m.fine, g.scene);
   for (int sw = 0; sw < sweeps; ++sw) {
     const float* src = (sw & 1) ? p1 : p0;
     float* dst = (sw & 1) ? p0 : p1;
     const bool zero = zero_start && sw == 0;
     for (uint32_t q = threadIdx.x; q < total; q += blockDim.x) {
-      const uint32_t b = list[q >> 9], t = q & 511u;
-      uint32_t bx = b % g.BX, by = (b / g.BX) % g.BY, bz = b / (g.BX * g.BY);
+      const uint32_t b = list[k0 + (q >> 9)], t = q & 511u, lb0 = b - g.base;
+      uint32_t bx = lb0 % g.BX, by = (lb0 / g.BX) % g.BY, bz = lb0 / (g.BX * g.BY);
       uint32_t x = bx * 8 + (t & 7u), y = by * 8 + ((t >> 3) & 7u), z = bz * 8 + (t >> 6);
       if (x >= g.W || y >= g.H || z >= g.D) continue;
       const uint32_t i = (b << 9) | t;
-      if (on_boundary(g, x, y, z) || !lv_active(m, x, y, z)) { dst[i] = 0.f; continue; }
+      if (on_boundary(g, x, y, z) || !lv_active(m, fg, x, y, z)) { dst[i] = 0.f; continue; }
       float pj = ((zero ? 0.f : nsum(g, src, x, y, z)) - rhs[i]) / 6.f;
       dst[i] = weighted ? mixf(zero ? 0.f : src[i], pj, om) : pj;
     }
@@ -1401,6 +1509,7 @@ __global__ __launch_bounds__(512) void k_prolong(const uint32_t* __restrict__ li
 // One ballot per (listed brick, z-slice) of every level: bit x + 8y is set iff
 // the voxel is in the domain, off the boundary faces and active under the
 // level's mask. Built once per solve; every V-cycle kernel reads these bits.
+template <bool MS>
 __global__ __launch_bounds__(512) void k_level_masks(LevelMaskArgs a) {
   FX_SIM_PRIORITY();
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -1433,11 +1542,13 @@ __global__ __launch_bounds__(512) void k_level_masks(LevelMaskArgs a) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (lv[u] < 0) continue;
-      const Grid& g = a.g[lv[u]];
       const uint32_t b = bb[u];
-      const uint32_t x = (b % g.BX) * 8 + (lane & 7u), y = ((b / g.BX) % g.BY) * 8 + (lane >> 3),
-                     z = (b / (g.BX * g.BY)) * 8 + wave;
-      const bool act = x < g.W && y < g.H && z < g.D && !on_boundary(g, x, y, z) && lv_active(a.m[lv[u]], x, y, z);
+      const Grid g = grid_at<MS>(a.g[lv[u]], b);
+      const uint32_t lb = b - g.base;
+      const uint32_t x = (lb % g.BX) * 8 + (lane & 7u), y = ((lb / g.BX) % g.BY) * 8 + (lane >> 3),
+                     z = (lb / (g.BX * g.BY)) * 8 + wave;
+      const bool act = x < g.W && y < g.H && z < g.D && !on_boundary(g, x, y, z) &&
+                       lv_active(a.m[lv[u]], grid_scene<MS>(a.m[lv[u]].fine, g.scene), x, y, z);
       const uint64_t w = __ballot(act);
       if (lane == 0) a.bits[lv[u]][size_t(kk[u]) * 8 + wave] = w;
     }
@@ -1451,7 +1562,7 @@ __device__ __forceinline__ uint64_t nbhd_word(const Grid& g, const LvBits& lb, i
   const int nx = bx + c % 3 - 1, ny = by + (c / 3) % 3 - 1, nz = bz + c / 9 - 1;
   uint64_t v = 0;
   if (nx >= 0 && ny >= 0 && nz >= 0 && nx < int(g.BX) && ny < int(g.BY) && nz < int(g.BZ)) {
-    const uint32_t nbk = uint32_t(nx) + g.BX * (uint32_t(ny) + g.BY * uint32_t(nz));
+    const uint32_t nbk = g.base + uint32_t(nx) + g.BX * (uint32_t(ny) + g.BY * uint32_t(nz));
     const int pos = lb.page ? lb.page[nbk] : int(nbk);
     if (pos >= 0) v = lb.bits[size_t(pos) * 8 + w];
   }
@@ -1535,26 +1646,29 @@ __device__ __forceinline__ void tile_sweep(const uint64_t* nb, const float* A, c
   }
 }
 __device__ __forceinline__ void brick_coords(const Grid& g, uint32_t b, int& bx, int& by, int& bz) {
-  bx = int(b % g.BX); by = int((b / g.BX) % g.BY); bz = int(b / (g.BX * g.BY));
+  const uint32_t lb = b - g.base;
+  bx = int(lb % g.BX); by = int((lb / g.BX) % g.BY); bz = int(lb / (g.BX * g.BY));
 }
 
 // Pre-smoothing (PRE sweeps), residual, FP64 norm and (1,3,3,1)/8 x 4 restriction.
 // p_in == nullptr: start from p = 0. Writes the smoothed p of the brick to p_out
 // (skipped when p_out == nullptr) and the coarse right-hand side of the 4^3
 // coarse voxels the brick covers.
-template <int PRE>
-__global__ __launch_bounds__(512) void k_vdown(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid f,
-                                               LvBits bf, Grid c, LvMask mc, const StepConst* __restrict__ C,
+template <int PRE, bool MS>
+__global__ __launch_bounds__(512) void k_vdown(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid F,
+                                               LvBits bf, Grid CG, LvMask mc, const StepConst* __restrict__ C0,
                                                const float* __restrict__ rhs, const float* __restrict__ p_in,
                                                float* __restrict__ p_out, float* __restrict__ crhs, double* norm2) {
   constexpr int R0 = PRE + 2, S = Tile<R0>::S, N = Tile<R0>::N;
   __shared__ float ta[N], tb[N], tr[N];
   __shared__ uint64_t nb[216];
   __shared__ double part[8];
-  const float om = C->omega;
   double acc = 0.0;
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
+    const Grid f = grid_at<MS>(F, b);
+    const Grid c = grid_scene<MS>(CG, f.scene);
+    const float om = C0[f.scene].omega;
     int bx, by, bz;
     brick_coords(f, b, bx, by, bz);
     const int ox = bx * 8 - R0, oy = by * 8 - R0, oz = bz * 8 - R0;
@@ -1563,7 +1677,8 @@ __global__ __launch_bounds__(512) void k_vdown(const uint32_t* __restrict__ list
     if (threadIdx.x < 64) {
       const int t = threadIdx.x;
       const uint32_t cx = uint32_t(bx * 4 + (t & 3)), cy = uint32_t(by * 4 + ((t >> 2) & 3)), cz = uint32_t(bz * 4 + (t >> 4));
-      cact = cx < c.W && cy < c.H && cz < c.D && !on_boundary(c, cx, cy, cz) && lv_active(mc, cx, cy, cz);
+      cact = cx < c.W && cy < c.H && cz < c.D && !on_boundary(c, cx, cy, cz) &&
+             lv_active(mc, grid_scene<MS>(mc.fine, f.scene), cx, cy, cz);
     }
     TileRegs<R0> rr, rp;
     tile_fetch<R0>(f, rhs, ox, oy, oz, rr);
@@ -1650,19 +1765,21 @@ __global__ __launch_bounds__(512) void k_vdown(const uint32_t* __restrict__ list
 
 // Prolongation of the coarse correction e (pressure-prolongation.wgsl.ts:29-44
 // sampler) added to p_in on brick +- POST, then POST sweeps; writes p_out.
-template <int POST>
-__global__ __launch_bounds__(512) void k_vup(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid c, Grid f,
-                                             LvBits bf, const StepConst* __restrict__ C, const float* __restrict__ e,
+template <int POST, bool MS>
+__global__ __launch_bounds__(512) void k_vup(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid CG, Grid F,
+                                             LvBits bf, const StepConst* __restrict__ C0, const float* __restrict__ e,
                                              const float* __restrict__ rhs, const float* __restrict__ p_in,
                                              float* __restrict__ p_out) {
   constexpr int R0 = POST > 0 ? POST : 0, S = Tile<R0>::S, N = Tile<R0>::N;
   constexpr int CR = (POST + 3) / 2, CS = 4 + 2 * CR, CN = CS * CS * CS;
   __shared__ float ta[N], tb[N], tr[N], tc[CN];
   __shared__ uint64_t nb[216];
-  const float om = C->omega;
-  const float mcx = float(c.W - 1), mcy = float(c.H - 1), mcz = float(c.D - 1);
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
+    const Grid f = grid_at<MS>(F, b);
+    const Grid c = grid_scene<MS>(CG, f.scene);
+    const float om = C0[f.scene].omega;
+    const float mcx = float(c.W - 1), mcy = float(c.H - 1), mcz = float(c.D - 1);
     int bx, by, bz;
     brick_coords(f, b, bx, by, bz);
     const int ox = bx * 8 - R0, oy = by * 8 - R0, oz = bz * 8 - R0;
@@ -1741,18 +1858,22 @@ __global__ __launch_bounds__(512) void k_vup(const uint32_t* __restrict__ list, 
 // the 4 (y, z) corners once. Where px is clamped to the level, the weight fx is
 // 0 and the clamped column holds the value the clamped x0 reads, so every blend
 // equals a one-voxel-per-thread prolongation's.
-__global__ __launch_bounds__(512) void k_prolong_add(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid c,
-                                                     Grid f, LvBits lb, const float* __restrict__ e, float* __restrict__ p) {
+template <bool MS>
+__global__ __launch_bounds__(512) void k_prolong_add(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid CG,
+                                                     Grid F, LvBits lb, const float* __restrict__ e, float* __restrict__ p) {
   FX_SIM_PRIORITY();
   const uint32_t slot = threadIdx.x >> 7, t = threadIdx.x & 127u;
   const uint32_t h = t & 1u, ly = (t >> 1) & 7u, lz = t >> 4;
-  const float mcx = float(c.W - 1), mcy = float(c.H - 1), mcz = float(c.D - 1);
   const BrickRange br = brick_range_slots(*count, 4u, slot);
   for (uint32_t k = br.k; k < br.end; k += br.step) {
-    const uint32_t b = list[k];
+    const uint32_t b = wave_uniform(list[k]);
     const uint32_t act = uint32_t(lb.bits[size_t(k) * 8 + lz] >> ((ly << 3) | (h * 4u))) & 0xFu;
     if (!act) continue;
-    const uint32_t xb = (b % f.BX) * 8 + h * 4u, y = ((b / f.BX) % f.BY) * 8 + ly, z = (b / (f.BX * f.BY)) * 8 + lz;
+    const Grid f = grid_at<MS>(F, b);
+    const Grid c = grid_scene<MS>(CG, f.scene);
+    const float mcx = float(c.W - 1), mcy = float(c.H - 1), mcz = float(c.D - 1);
+    const uint32_t lbk = b - f.base;
+    const uint32_t xb = (lbk % f.BX) * 8 + h * 4u, y = ((lbk / f.BX) % f.BY) * 8 + ly, z = (lbk / (f.BX * f.BY)) * 8 + lz;
     const float py = clampf((float(y) + 0.5f) * 0.5f - 0.5f, 0.f, mcy), pz = clampf((float(z) + 0.5f) * 0.5f - 0.5f, 0.f, mcz);
     const uint32_t y0 = uint32_t(floorf(py)), z0 = uint32_t(floorf(pz));
     const uint32_t y1 = min(y0 + 1u, c.H - 1u), z1 = min(z0 + 1u, c.D - 1u);
@@ -1822,19 +1943,22 @@ __device__ __forceinline__ void row_load(const float* __restrict__ f, uint32_t b
 // squares over the active voxels when norm2 != nullptr. Each voxel's arithmetic
 // is that of a one-voxel-per-thread residual; only the FP64 summation order of
 // the norm differs.
-__global__ __launch_bounds__(512) void k_residual_lean(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
+template <bool MS>
+__global__ __launch_bounds__(512) void k_residual_lean(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
                                                        LvBits lb, const float* __restrict__ rhs, const float* __restrict__ p,
                                                        float* __restrict__ r, double* norm2) {
   FX_SIM_PRIORITY();
   __shared__ double part[8];
   const uint32_t slot = threadIdx.x >> 7, t = threadIdx.x & 127u;
   const uint32_t h = t & 1u, ly = (t >> 1) & 7u, lz = t >> 4;
-  const uint32_t BX = g.BX, BXY = g.BX * g.BY, last = g.BX * g.BY * g.BZ - 1u;
+  const uint32_t last = G.span - 1u;  // brick ids of unused neighbour rows stay in the allocation
   const uint32_t rofs = (lz << 6) | (ly << 3) | (h * 4u);
   double acc = 0.0;
   const BrickRange br = brick_range_slots(*count, 4u, slot);
   for (uint32_t k = br.k; k < br.end; k += br.step) {
-    const uint32_t b = list[k];
+    const uint32_t b = wave_uniform(list[k]);
+    const Grid g = grid_at<MS>(G, b);
+    const uint32_t BX = g.BX, BXY = g.BX * g.BY;
     const uint32_t i0 = (b << 9) | rofs;
     const uint32_t act = uint32_t(lb.bits[size_t(k) * 8 + lz] >> ((ly << 3) | (h * 4u))) & 0xFu;
     const float4 rh = *reinterpret_cast<const float4*>(rhs + i0);
@@ -1868,15 +1992,19 @@ __global__ __launch_bounds__(512) void k_residual_lean(const uint32_t* __restric
 // workgroup per listed coarse brick: the 18^3 fine region its taps reach is
 // staged in LDS (every load in flight at once), then each thread sums the 64
 // taps of its coarse voxel in the oracle's order (taps clamped to the domain).
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_restrict_lean(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid f,
-                                                       Grid c, LvBits cb, const float* __restrict__ r, float* __restrict__ crhs) {
+template <bool MS>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_restrict_lean(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid F,
+                                                       Grid CG, LvBits cb, const float* __restrict__ r, float* __restrict__ crhs) {
   constexpr int S = 18, N = S * S * S, IT = (N + 511) / 512;
   __shared__ float tf[N];
   const uint32_t t = threadIdx.x;
   const float wt[4] = {0.125f, 0.375f, 0.375f, 0.125f};
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
-    const int cbx = int(b % c.BX), cby = int((b / c.BX) % c.BY), cbz = int(b / (c.BX * c.BY));
+    const Grid c = grid_at<MS>(CG, b);
+    const Grid f = grid_scene<MS>(F, c.scene);
+    const uint32_t lb = b - c.base;
+    const int cbx = int(lb % c.BX), cby = int((lb / c.BX) % c.BY), cbz = int(lb / (c.BX * c.BY));
     // fine origin of the region: 2 * (coarse brick origin) - 1
     const int ox = cbx * 16 - 1, oy = cby * 16 - 1, oz = cbz * 16 - 1;
     float v[IT];
@@ -1926,28 +2054,31 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
 // sweep starts from p = 0. FINE: the finest level, where every in-domain voxel
 // of a listed brick off the boundary faces is active; the test uses coordinates
 // instead of the activity words.
-template <bool ZERO, bool FINE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_sweep(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                               LvBits lb, const StepConst* __restrict__ C, const float* __restrict__ rhs,
+template <bool ZERO, bool FINE, bool MS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_sweep(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
+                                               LvBits lb, const StepConst* __restrict__ C0, const float* __restrict__ rhs,
                                                const float* __restrict__ src, float* __restrict__ dst) {
   FX_SIM_PRIORITY();
-  const float om = C->omega;
   // one wave per brick, 8 bricks per workgroup: lane t owns x = 4h .. 4h+3 of row
   // y in the two z-layers z0 = 2 zq and z1 = z0 + 1 (each layer's z neighbour on
   // that side is the other layer, in registers); a wave's loads of one layer pair
   // stay contiguous (1 KB per instruction)
   const uint32_t slot = threadIdx.x >> 6, t = threadIdx.x & 63u;
   const uint32_t h = t & 1u, ly = (t >> 1) & 7u, zq = t >> 4, x0 = h * 4u, z0 = 2u * zq;
-  const uint32_t BX = g.BX, BXY = g.BX * g.BY, last = g.BX * g.BY * g.BZ - 1u;
+  const uint32_t last = G.span - 1u;  // brick ids of unused neighbour rows stay in the allocation
   const BrickRange br = brick_range_slots(*count, 4u, slot);
   const uint32_t rofs = (z0 << 6) | (ly << 3) | x0;
   for (uint32_t k = br.k; k < br.end; k += br.step) {
-    const uint32_t b = list[k];
+    const uint32_t b = wave_uniform(list[k]);
+    const Grid g = grid_at<MS>(G, b);
+    const float om = C0[g.scene].omega;
+    const uint32_t BX = g.BX, BXY = g.BX * g.BY;
     const uint32_t i0 = (b << 9) | rofs, i1 = i0 + 64u;
     uint32_t act = 0;  // bits 0-3: layer z0, bits 4-7: layer z1
     if (FINE) {
       // in the domain and off its faces: 1 <= c <= n - 2 on every axis (separable)
-      const uint32_t x = (b % g.BX) * 8 + x0, y = ((b / g.BX) % g.BY) * 8 + ly, z = (b / BXY) * 8 + z0;
+      const uint32_t lbk = b - g.base;
+      const uint32_t x = (lbk % g.BX) * 8 + x0, y = ((lbk / g.BX) % g.BY) * 8 + ly, z = (lbk / BXY) * 8 + z0;
       uint32_t xs = 0;
 #pragma unroll
       for (uint32_t e = 0; e < 4; ++e) xs |= (x + e >= 1u && x + e + 2u <= g.W) ? (1u << e) : 0u;
@@ -2014,15 +2145,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
 // latency overlaps the LDS work. Every voxel sees the same arithmetic as in two
 // k_sweep launches (same neighbour order, inactive voxels 0), so results are
 // bit-identical. ZERO: the first sweep starts from p = 0.
-template <bool ZERO>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_sweep2(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g,
-                                                LvBits lb, const StepConst* __restrict__ C, const float* __restrict__ rhs,
+template <bool ZERO, bool MS>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_sweep2(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
+                                                LvBits lb, const StepConst* __restrict__ C0, const float* __restrict__ rhs,
                                                 const float* __restrict__ src, float* __restrict__ dst) {
   FX_SIM_PRIORITY();
   constexpr int R0 = 2, S = Tile<R0>::S;
   __shared__ float ta[Tile<R0>::N], tb[Tile<R0>::N], tr[Tile<R0>::N];
   __shared__ uint64_t nb[216];
-  const float om = C->omega;
   const int tid = int(threadIdx.x);
   const BrickRange br = brick_range(*count);
   TileRegs<R0> rr, rp;
@@ -2030,6 +2160,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   uint32_t b = 0;
   auto fetch = [&](uint32_t k) {
     b = list[k];
+    const Grid g = grid_at<MS>(G, b);
     int bx, by, bz;
     brick_coords(g, b, bx, by, bz);
     const int ox = bx * 8 - R0, oy = by * 8 - R0, oz = bz * 8 - R0;
@@ -2054,6 +2185,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     if (!ZERO) tile_store<R0>(nb, rp, ta); else tile_zero<R0>(ta);
     __syncthreads();
     if (k + br.step < br.end) fetch(k + br.step);
+    const Grid g = grid_at<MS>(G, cb);
+    const float om = C0[g.scene].omega;
     tile_sweep<R0, 1>(nb, ta, tr, tb, om);
     __syncthreads();
     tile_sweep<R0, 0>(nb, tb, tr, ta, om);
@@ -2067,10 +2200,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
 }
 
 // SH/project-velocity.wgsl.ts:21-43, on float4 brick rows.
-__global__ __launch_bounds__(512) void k_projection(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid g, Vec3 us,
+template <bool MS>
+__global__ __launch_bounds__(512) void k_projection(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G, Vec3 us,
                                                     const float* __restrict__ p, Vec3 u) {
   FX_ROW_LOOP(list, count) {
-    const RowCtx R = row_ctx(g, list[fx_k], fx_t);
+    const uint32_t b = wave_uniform(list[fx_k]);
+    const Grid g = grid_at<MS>(G, b);
+    const RowCtx R = row_ctx(g, b, fx_t);
     float c[4], xp[4], xm[4], yp[4], ym[4], zp[4], zm[4], ux[4], uy[4], uz[4];
     row_x<StF32>(g, R, p, c, xp, xm);
     row_y<StF32>(R, p, yp, ym);
@@ -2996,12 +3132,29 @@ __global__ __launch_bounds__(512) void k_da_source(const uint32_t* __restrict__ 
 #define FX_LAUNCH(kernel, blocks, threads, stream, ...) \
   hipLaunchKernelGGL(kernel, dim3(blocks), dim3(threads), 0, stream, __VA_ARGS__)
 
+// Listed fine bricks of every scene of a brick-queue batch (the ascending list
+// holds each scene's bricks as one run): out[k] = bricks in [base_k, base_k + bricks_k).
+__global__ void k_scene_counts(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, const Grid* __restrict__ tab,
+                               uint32_t scenes, uint32_t* __restrict__ out) {
+  const uint32_t n = *count;
+  for (uint32_t k = threadIdx.x; k < scenes; k += blockDim.x) {
+    const Grid g = tab[k];
+    out[k] = list_lower_bound(list, n, g.base + g.BX * g.BY * g.BZ) - list_lower_bound(list, n, g.base);
+  }
+}
+
+// MS launches: kernel template KT (several scenes) when G is a selector, KF otherwise.
+#define FX_LAUNCH_MS(G, KT, KF, blocks, threads, stream, ...)    \
+  do {                                                          \
+    if ((G).tab) FX_LAUNCH(KT, blocks, threads, stream, __VA_ARGS__); \
+    else FX_LAUNCH(KF, blocks, threads, stream, __VA_ARGS__);   \
+  } while (0)
 void launch_inject(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
                    const float* src, Scalars set0, Vec3 u, uint32_t* touched, int parts) {
   switch (set0.prec) {
-    case FX_PRECISION_BF16: FX_LAUNCH(k_inject<StBF16>, blocks, 512, s, list, count, g, c, src, typed<StBF16>(set0), u, touched, parts); break;
-    case FX_PRECISION_FP16: FX_LAUNCH(k_inject<StF16>, blocks, 512, s, list, count, g, c, src, typed<StF16>(set0), u, touched, parts); break;
-    default: FX_LAUNCH(k_inject<StF32>, blocks, 512, s, list, count, g, c, src, typed<StF32>(set0), u, touched, parts);
+    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_inject<StBF16, true>), (k_inject<StBF16, false>), blocks, 512, s, list, count, g, c, src, typed<StBF16>(set0), u, touched, parts); break;
+    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_inject<StF16, true>), (k_inject<StF16, false>), blocks, 512, s, list, count, g, c, src, typed<StF16>(set0), u, touched, parts); break;
+    default: FX_LAUNCH_MS(g, (k_inject<StF32, true>), (k_inject<StF32, false>), blocks, 512, s, list, count, g, c, src, typed<StF32>(set0), u, touched, parts);
   }
 }
 void launch_clear_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t value) {
@@ -3021,7 +3174,7 @@ void launch_clear_f32(hipStream_t s, float* p, size_t n) {
 void launch_activate(hipStream_t s, int blocks, const uint32_t* listA, const uint32_t* countA, const uint32_t* listB,
                      const uint32_t* countB, Grid g, const StepConst* c, int rule, Scalars set, Vec3 vel, uint32_t* flags,
                      const LevelFlags& lf) {
-  if (rule == FX_ACTIVATION_REFERENCE) {
+  if (rule == FX_ACTIVATION_REFERENCE) {  // one scene only (the host rejects it for several)
     int b = std::max(1, std::min(blocks, int((brick_count(g) + 255) / 256) + 1));
     switch (set.prec) {
       case FX_PRECISION_BF16: FX_LAUNCH(k_activate_reference<StBF16>, b, 256, s, listA, countA, listB, countB, g, typed<StBF16>(set), flags, lf); break;
@@ -3030,9 +3183,9 @@ void launch_activate(hipStream_t s, int blocks, const uint32_t* listA, const uin
     }
   } else {
     switch (set.prec) {
-      case FX_PRECISION_BF16: FX_LAUNCH(k_activate_threshold<StBF16>, blocks, 512, s, listA, countA, listB, countB, g, c, typed<StBF16>(set).d, vel, flags, lf); break;
-      case FX_PRECISION_FP16: FX_LAUNCH(k_activate_threshold<StF16>, blocks, 512, s, listA, countA, listB, countB, g, c, typed<StF16>(set).d, vel, flags, lf); break;
-      default: FX_LAUNCH(k_activate_threshold<StF32>, blocks, 512, s, listA, countA, listB, countB, g, c, typed<StF32>(set).d, vel, flags, lf);
+      case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_activate_threshold<StBF16, true>), (k_activate_threshold<StBF16, false>), blocks, 512, s, listA, countA, listB, countB, g, c, typed<StBF16>(set).d, vel, flags, lf); break;
+      case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_activate_threshold<StF16, true>), (k_activate_threshold<StF16, false>), blocks, 512, s, listA, countA, listB, countB, g, c, typed<StF16>(set).d, vel, flags, lf); break;
+      default: FX_LAUNCH_MS(g, (k_activate_threshold<StF32, true>), (k_activate_threshold<StF32, false>), blocks, 512, s, listA, countA, listB, countB, g, c, typed<StF32>(set).d, vel, flags, lf);
     }
   }
 }
@@ -3047,9 +3200,9 @@ void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, con
   float4* U4 = static_cast<float4*>(uq);
   float4* S4 = static_cast<float4*>(sq);
   switch (set0.prec) {
-    case FX_PRECISION_BF16: FX_LAUNCH(k_advect_velocity<StBF16>, blocks, 512, s, list, count, g, c, u, us, U4, typed<StBF16>(set0), S4); break;
-    case FX_PRECISION_FP16: FX_LAUNCH(k_advect_velocity<StF16>, blocks, 512, s, list, count, g, c, u, us, U4, typed<StF16>(set0), S4); break;
-    default: FX_LAUNCH(k_advect_velocity<StF32>, blocks, 512, s, list, count, g, c, u, us, U4, typed<StF32>(set0), S4);
+    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_advect_velocity<StBF16, true>), (k_advect_velocity<StBF16, false>), blocks, 512, s, list, count, g, c, u, us, U4, typed<StBF16>(set0), S4); break;
+    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_advect_velocity<StF16, true>), (k_advect_velocity<StF16, false>), blocks, 512, s, list, count, g, c, u, us, U4, typed<StF16>(set0), S4); break;
+    default: FX_LAUNCH_MS(g, (k_advect_velocity<StF32, true>), (k_advect_velocity<StF32, false>), blocks, 512, s, list, count, g, c, u, us, U4, typed<StF32>(set0), S4);
   }
 }
 void launch_pack_gather(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Scalars set0, Vec3 us,
@@ -3067,40 +3220,44 @@ void launch_advect_scalars(hipStream_t s, int blocks, const uint32_t* list, cons
   const float4* S4 = static_cast<const float4*>(sq);
   const float4* U4 = static_cast<const float4*>(uq);
   switch (dst.prec) {
-    case FX_PRECISION_BF16: FX_LAUNCH(k_advect_scalars<StBF16>, blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StBF16>(dst), flags); break;
-    case FX_PRECISION_FP16: FX_LAUNCH(k_advect_scalars<StF16>, blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StF16>(dst), flags); break;
-    default: FX_LAUNCH(k_advect_scalars<StF32>, blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StF32>(dst), flags);
+    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_advect_scalars<StBF16, true>), (k_advect_scalars<StBF16, false>), blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StBF16>(dst), flags); break;
+    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_advect_scalars<StF16, true>), (k_advect_scalars<StF16, false>), blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StF16>(dst), flags); break;
+    default: FX_LAUNCH_MS(g, (k_advect_scalars<StF32, true>), (k_advect_scalars<StF32, false>), blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StF32>(dst), flags);
   }
 }
 void launch_combustion(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
                        Scalars src, Scalars dst, const uint32_t* flags) {
   switch (src.prec) {
-    case FX_PRECISION_BF16: FX_LAUNCH(k_combustion<StBF16>, blocks, 512, s, list, count, g, c, typed<StBF16>(src), typed<StBF16>(dst), flags); break;
-    case FX_PRECISION_FP16: FX_LAUNCH(k_combustion<StF16>, blocks, 512, s, list, count, g, c, typed<StF16>(src), typed<StF16>(dst), flags); break;
-    default: FX_LAUNCH(k_combustion<StF32>, blocks, 512, s, list, count, g, c, typed<StF32>(src), typed<StF32>(dst), flags);
+    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_combustion<StBF16, true>), (k_combustion<StBF16, false>), blocks, 512, s, list, count, g, c, typed<StBF16>(src), typed<StBF16>(dst), flags); break;
+    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_combustion<StF16, true>), (k_combustion<StF16, false>), blocks, 512, s, list, count, g, c, typed<StF16>(src), typed<StF16>(dst), flags); break;
+    default: FX_LAUNCH_MS(g, (k_combustion<StF32, true>), (k_combustion<StF32, false>), blocks, 512, s, list, count, g, c, typed<StF32>(src), typed<StF32>(dst), flags);
   }
 }
 void launch_buoyancy(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
                      Scalars set0, Vec3 us, const uint32_t* flags) {
   switch (set0.prec) {
-    case FX_PRECISION_BF16: FX_LAUNCH(k_buoyancy<StBF16>, blocks, 512, s, list, count, g, c, typed<StBF16>(set0), us, flags); break;
-    case FX_PRECISION_FP16: FX_LAUNCH(k_buoyancy<StF16>, blocks, 512, s, list, count, g, c, typed<StF16>(set0), us, flags); break;
-    default: FX_LAUNCH(k_buoyancy<StF32>, blocks, 512, s, list, count, g, c, typed<StF32>(set0), us, flags);
+    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_buoyancy<StBF16, true>), (k_buoyancy<StBF16, false>), blocks, 512, s, list, count, g, c, typed<StBF16>(set0), us, flags); break;
+    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_buoyancy<StF16, true>), (k_buoyancy<StF16, false>), blocks, 512, s, list, count, g, c, typed<StF16>(set0), us, flags); break;
+    default: FX_LAUNCH_MS(g, (k_buoyancy<StF32, true>), (k_buoyancy<StF32, false>), blocks, 512, s, list, count, g, c, typed<StF32>(set0), us, flags);
   }
 }
 void launch_vorticity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 us, Vec3 w, float* wmag) {
-  FX_LAUNCH(k_vorticity, blocks, 512, s, list, count, g, us, w, wmag);
+  FX_LAUNCH_MS(g, (k_vorticity<true>), (k_vorticity<false>), blocks, 512, s, list, count, g, us, w, wmag);
 }
 void launch_confinement(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c, Vec3 w,
                         const float* wmag, Vec3 us, float* cmag, Scalars set0) {
   switch (set0.prec) {
-    case FX_PRECISION_BF16: FX_LAUNCH(k_confinement<StBF16>, blocks, 512, s, list, count, g, c, w, wmag, us, cmag, typed<StBF16>(set0)); break;
-    case FX_PRECISION_FP16: FX_LAUNCH(k_confinement<StF16>, blocks, 512, s, list, count, g, c, w, wmag, us, cmag, typed<StF16>(set0)); break;
-    default: FX_LAUNCH(k_confinement<StF32>, blocks, 512, s, list, count, g, c, w, wmag, us, cmag, typed<StF32>(set0));
+    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_confinement<StBF16, true>), (k_confinement<StBF16, false>), blocks, 512, s, list, count, g, c, w, wmag, us, cmag, typed<StBF16>(set0)); break;
+    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_confinement<StF16, true>), (k_confinement<StF16, false>), blocks, 512, s, list, count, g, c, w, wmag, us, cmag, typed<StF16>(set0)); break;
+    default: FX_LAUNCH_MS(g, (k_confinement<StF32, true>), (k_confinement<StF32, false>), blocks, 512, s, list, count, g, c, w, wmag, us, cmag, typed<StF32>(set0));
   }
 }
 void launch_divergence(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 us, float* div) {
-  FX_LAUNCH(k_divergence, blocks, 512, s, list, count, g, us, div);
+  FX_LAUNCH_MS(g, (k_divergence<true>), (k_divergence<false>), blocks, 512, s, list, count, g, us, div);
+}
+void launch_scene_counts(hipStream_t s, const uint32_t* list, const uint32_t* count, const Grid* tab, uint32_t scenes,
+                         uint32_t* out) {
+  FX_LAUNCH(k_scene_counts, 1, 256, s, list, count, tab, scenes, out);
 }
 void launch_retire(hipStream_t s, int blocks, const RetireArgs& R) { FX_LAUNCH(k_retire, blocks, 512, s, R); }
 void launch_zero_bricks(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, float* const* bufs, int nbufs) {
@@ -3115,73 +3272,79 @@ void launch_jacobi(hipStream_t s, int blocks, const uint32_t* list, const uint32
                    const float* rhs, const float* src, float* dst, int weighted) {
   FX_LAUNCH(k_jacobi, blocks, 512, s, list, count, g, m, c, rhs, src, dst, weighted);
 }
-void launch_level_masks(hipStream_t s, int blocks, const LevelMaskArgs& a) { FX_LAUNCH(k_level_masks, blocks, 512, s, a); }
+void launch_level_masks(hipStream_t s, int blocks, const LevelMaskArgs& a) {
+  FX_LAUNCH_MS(a.g[0], k_level_masks<true>, k_level_masks<false>, blocks, 512, s, a);
+}
 void launch_coarse_setup(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, void* tab, uint32_t* tab_n,
-                         float* p0, float* p1) {
-  // every workgroup rebuilds the brick scan (at most kSmoothBricks bricks) and
-  // writes its share of the slots and of the zeroing
-  FX_LAUNCH(k_coarse_setup, 16, 1024, s, list, count, g, b, static_cast<uint4*>(tab), tab_n, p0, p1);
+                         float* p0, float* p1, uint32_t scenes) {
+  // every workgroup of a scene's row rebuilds its brick scan (at most kSmoothBricks
+  // bricks) and writes its share of the slots and of the zeroing
+  const dim3 grid(16, std::max(1u, scenes));
+  if (g.tab) hipLaunchKernelGGL(k_coarse_setup<true>, grid, dim3(1024), 0, s, list, count, g, b, static_cast<uint4*>(tab), tab_n, p0, p1);
+  else hipLaunchKernelGGL(k_coarse_setup<false>, grid, dim3(1024), 0, s, list, count, g, b, static_cast<uint4*>(tab), tab_n, p0, p1);
 }
 void launch_smooth_block(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const StepConst* c,
                          const float* rhs, float* p0, float* p1, int sweeps, int weighted, int zero_start, const void* tab,
-                         const uint32_t* tab_n) {
-  FX_LAUNCH(k_smooth_block, 1, 1024, s, list, count, g, m, c, rhs, p0, p1, sweeps, weighted, zero_start,
-            static_cast<const uint4*>(tab), tab_n);
+                         const uint32_t* tab_n, uint32_t scenes) {
+  const dim3 grid(1, std::max(1u, scenes));
+  const uint4* T = static_cast<const uint4*>(tab);
+  if (g.tab) hipLaunchKernelGGL(k_smooth_block<true>, grid, dim3(1024), 0, s, list, count, g, m, c, rhs, p0, p1, sweeps, weighted, zero_start, T, tab_n);
+  else hipLaunchKernelGGL(k_smooth_block<false>, grid, dim3(1024), 0, s, list, count, g, m, c, rhs, p0, p1, sweeps, weighted, zero_start, T, tab_n);
 }
 bool launch_vdown(hipStream_t s, int blocks, int pre, const uint32_t* list, const uint32_t* count, Grid f, LvBits bf, Grid c,
                   LvMask mc, const StepConst* cst, const float* rhs, const float* p_in, float* p_out, float* crhs, double* norm2) {
   switch (pre) {
-    case 0: FX_LAUNCH(k_vdown<0>, blocks, 512, s, list, count, f, bf, c, mc, cst, rhs, p_in, p_out, crhs, norm2); return true;
-    case 1: FX_LAUNCH(k_vdown<1>, blocks, 512, s, list, count, f, bf, c, mc, cst, rhs, p_in, p_out, crhs, norm2); return true;
-    case 2: FX_LAUNCH(k_vdown<2>, blocks, 512, s, list, count, f, bf, c, mc, cst, rhs, p_in, p_out, crhs, norm2); return true;
+    case 0: FX_LAUNCH_MS(f, (k_vdown<0, true>), (k_vdown<0, false>), blocks, 512, s, list, count, f, bf, c, mc, cst, rhs, p_in, p_out, crhs, norm2); return true;
+    case 1: FX_LAUNCH_MS(f, (k_vdown<1, true>), (k_vdown<1, false>), blocks, 512, s, list, count, f, bf, c, mc, cst, rhs, p_in, p_out, crhs, norm2); return true;
+    case 2: FX_LAUNCH_MS(f, (k_vdown<2, true>), (k_vdown<2, false>), blocks, 512, s, list, count, f, bf, c, mc, cst, rhs, p_in, p_out, crhs, norm2); return true;
     default: return false;  // larger halos exceed the LDS budget: unfused sweeps
   }
 }
 bool launch_vup(hipStream_t s, int blocks, int post, const uint32_t* list, const uint32_t* count, Grid c, Grid f, LvBits bf,
                 const StepConst* cst, const float* e, const float* rhs, const float* p_in, float* p_out) {
   switch (post) {
-    case 0: FX_LAUNCH(k_vup<0>, blocks, 512, s, list, count, c, f, bf, cst, e, rhs, p_in, p_out); return true;
-    case 1: FX_LAUNCH(k_vup<1>, blocks, 512, s, list, count, c, f, bf, cst, e, rhs, p_in, p_out); return true;
-    case 2: FX_LAUNCH(k_vup<2>, blocks, 512, s, list, count, c, f, bf, cst, e, rhs, p_in, p_out); return true;
-    case 3: FX_LAUNCH(k_vup<3>, blocks, 512, s, list, count, c, f, bf, cst, e, rhs, p_in, p_out); return true;
+    case 0: FX_LAUNCH_MS(f, (k_vup<0, true>), (k_vup<0, false>), blocks, 512, s, list, count, c, f, bf, cst, e, rhs, p_in, p_out); return true;
+    case 1: FX_LAUNCH_MS(f, (k_vup<1, true>), (k_vup<1, false>), blocks, 512, s, list, count, c, f, bf, cst, e, rhs, p_in, p_out); return true;
+    case 2: FX_LAUNCH_MS(f, (k_vup<2, true>), (k_vup<2, false>), blocks, 512, s, list, count, c, f, bf, cst, e, rhs, p_in, p_out); return true;
+    case 3: FX_LAUNCH_MS(f, (k_vup<3, true>), (k_vup<3, false>), blocks, 512, s, list, count, c, f, bf, cst, e, rhs, p_in, p_out); return true;
     default: return false;
   }
 }
 void launch_residual_lean(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b,
                           const float* rhs, const float* p, float* r, double* norm2) {
-  FX_LAUNCH(k_residual_lean, blocks, 512, s, list, count, g, b, rhs, p, r, norm2);
+  FX_LAUNCH_MS(g, k_residual_lean<true>, k_residual_lean<false>, blocks, 512, s, list, count, g, b, rhs, p, r, norm2);
 }
 void launch_restrict_lean(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid f, Grid c, LvBits cb,
                           const float* r, float* crhs) {
-  FX_LAUNCH(k_restrict_lean, blocks, 512, s, list, count, f, c, cb, r, crhs);
+  FX_LAUNCH_MS(c, k_restrict_lean<true>, k_restrict_lean<false>, blocks, 512, s, list, count, f, c, cb, r, crhs);
 }
 void launch_prolong_add(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid c, Grid f, LvBits b,
                         const float* e, float* p) {
-  FX_LAUNCH(k_prolong_add, blocks, 512, s, list, count, c, f, b, e, p);
+  FX_LAUNCH_MS(f, k_prolong_add<true>, k_prolong_add<false>, blocks, 512, s, list, count, c, f, b, e, p);
 }
 void launch_sweep(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
                   const float* rhs, const float* src, float* dst, bool fine, hipEvent_t ev_start, hipEvent_t ev_stop) {
   // 256-thread workgroups of one wave per brick, 5/3 as many as the 512-thread
   // persistent grid (five per CU fit beside a raymarch workgroup at 72 VGPRs)
   const int wg = blocks >= 8 ? ((blocks * 5 / 3) & ~7) : blocks;
-  if (ev_start && ev_stop && src && fine) {
+  if (ev_start && ev_stop && src && fine && !g.tab) {
     // the events take the kernel's own start and end timestamps (no dispatch time)
-    hipExtLaunchKernelGGL((k_sweep<false, true>), dim3(wg), dim3(256), 0, s, ev_start, ev_stop, 0, list, count, g, b, c, rhs, src,
-                          dst);
+    hipExtLaunchKernelGGL((k_sweep<false, true, false>), dim3(wg), dim3(256), 0, s, ev_start, ev_stop, 0, list, count, g, b, c, rhs,
+                          src, dst);
   } else if (src && fine) {
-    FX_LAUNCH((k_sweep<false, true>), wg, 256, s, list, count, g, b, c, rhs, src, dst);
+    FX_LAUNCH_MS(g, (k_sweep<false, true, true>), (k_sweep<false, true, false>), wg, 256, s, list, count, g, b, c, rhs, src, dst);
   } else if (src) {
-    FX_LAUNCH((k_sweep<false, false>), wg, 256, s, list, count, g, b, c, rhs, src, dst);
+    FX_LAUNCH_MS(g, (k_sweep<false, false, true>), (k_sweep<false, false, false>), wg, 256, s, list, count, g, b, c, rhs, src, dst);
   } else if (fine) {
-    FX_LAUNCH((k_sweep<true, true>), wg, 256, s, list, count, g, b, c, rhs, src, dst);
+    FX_LAUNCH_MS(g, (k_sweep<true, true, true>), (k_sweep<true, true, false>), wg, 256, s, list, count, g, b, c, rhs, src, dst);
   } else {
-    FX_LAUNCH((k_sweep<true, false>), wg, 256, s, list, count, g, b, c, rhs, src, dst);
+    FX_LAUNCH_MS(g, (k_sweep<true, false, true>), (k_sweep<true, false, false>), wg, 256, s, list, count, g, b, c, rhs, src, dst);
   }
 }
 void launch_sweep_pair(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
                        const float* rhs, const float* src, float* dst) {
-  if (src) FX_LAUNCH(k_sweep2<false>, blocks, 512, s, list, count, g, b, c, rhs, src, dst);
-  else FX_LAUNCH(k_sweep2<true>, blocks, 512, s, list, count, g, b, c, rhs, src, dst);
+  if (src) FX_LAUNCH_MS(g, (k_sweep2<false, true>), (k_sweep2<false, false>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
+  else FX_LAUNCH_MS(g, (k_sweep2<true, true>), (k_sweep2<true, false>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
 }
 void launch_residual(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const float* rhs,
                      const float* p, float* r, double* norm2) {
@@ -3197,7 +3360,7 @@ void launch_prolong(hipStream_t s, int blocks, const uint32_t* list, const uint3
 }
 void launch_projection(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 us, const float* p,
                        Vec3 u) {
-  FX_LAUNCH(k_projection, blocks, 512, s, list, count, g, us, p, u);
+  FX_LAUNCH_MS(g, k_projection<true>, k_projection<false>, blocks, 512, s, list, count, g, us, p, u);
 }
 void launch_debug(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 u, float* umag) {
   FX_LAUNCH(k_debug, blocks, 512, s, list, count, g, u, umag);
@@ -3237,7 +3400,7 @@ void launch_df_advect(hipStream_t s, int blocks, const uint32_t* list, const uin
 void launch_df_forces(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const float* dn, Vec3 ua, Vec3 ub,
                       Vec3 w, float* wmag, Vec3 uc, float beta_dt, float eps_dt) {
   FX_LAUNCH(k_df_buoyancy, blocks, 512, s, list, count, g, dn, ua, ub, beta_dt);
-  FX_LAUNCH(k_vorticity, blocks, 512, s, list, count, g, ub, w, wmag);
+  FX_LAUNCH(k_vorticity<false>, blocks, 512, s, list, count, g, ub, w, wmag);
   FX_LAUNCH(k_df_confine, blocks, 512, s, list, count, g, w, wmag, ub, uc, eps_dt);
 }
 void launch_df_loss(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const float* d, const float* target,
@@ -3249,7 +3412,7 @@ void launch_da_zero_boundary(hipStream_t s, int blocks, const uint32_t* list, co
 }
 void launch_da_forces(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, Vec3 ub, Vec3 w, float* wmag,
                       Vec3 abar, Vec3 gbar, Vec3 wbar, float* dnb, float beta_dt, float eps_dt) {
-  FX_LAUNCH(k_vorticity, blocks, 512, s, list, count, g, ub, w, wmag);
+  FX_LAUNCH(k_vorticity<false>, blocks, 512, s, list, count, g, ub, w, wmag);
   FX_LAUNCH(k_da_confine1, blocks, 512, s, list, count, g, w, wmag, abar, eps_dt, gbar, wbar);
   FX_LAUNCH(k_da_confine2, blocks, 512, s, list, count, g, w, wmag, gbar, wbar);
   FX_LAUNCH(k_da_confine3, blocks, 512, s, list, count, g, wbar, abar);

@@ -383,6 +383,155 @@ class CombustionVolumeBatch:
             pass
 
 
+class QueueScene:
+    """Scene `index` of a CombustionVolumeQueueBatch: the per-scene calls (fx_scene_*) with the
+    method names of CombustionVolumeSimulation."""
+
+    def __init__(self, batch, index: int, desc: abi.CreateDesc):
+        self._batch = batch
+        self.index = index
+        self.desc = desc
+        self.resolution = VolumeResolution(desc.width, desc.height, desc.depth)
+        self.brick_counts = tuple((n + 7) // 8 for n in (desc.width, desc.height, desc.depth))
+        self.brick_count = self.brick_counts[0] * self.brick_counts[1] * self.brick_counts[2]
+
+    def _call(self, name, *args):
+        b = self._batch
+        return b._check(getattr(b._lib, name)(b._h, self.index, *args))
+
+    def updateSources(self, sources: Sequence[EmitterSource]) -> None:  # ORCH:378-380
+        self.update_packed_sources(pack_emitter_sources(sources), len(sources))
+
+    def update_packed_sources(self, packed: np.ndarray, count: int) -> None:
+        packed = np.ascontiguousarray(packed, dtype=np.float32)
+        self._call("fx_scene_update_sources", packed.ctypes.data_as(C.POINTER(C.c_float)) if count else None, int(count))
+
+    def getRuntimeParams(self) -> Dict:  # ORCH:338-352
+        rp = abi.RuntimeParams()
+        self._call("fx_scene_get_runtime_params", C.byref(rp))
+        return {key: (tuple(getattr(rp, name)) if name in ("wind", "gravity") else float(getattr(rp, name)))
+                for key, (name, _) in _RUNTIME_KEYS.items()}
+
+    def setRuntimeParams(self, params: Dict) -> None:  # ORCH:353-365 (Partial<...>)
+        rp = abi.RuntimeParams()
+        mask = 0
+        for key, value in params.items():
+            if value is None:
+                continue
+            name, bit = _RUNTIME_KEYS[key]
+            if name in ("wind", "gravity"):
+                getattr(rp, name)[:] = tuple(float(v) for v in value)
+            else:
+                setattr(rp, name, float(value))
+            mask |= bit
+        self._call("fx_scene_set_runtime_params", C.byref(rp), mask)
+
+    def read_field(self, field) -> np.ndarray:
+        fid = abi.FIELDS[field] if isinstance(field, str) else int(field)
+        r = self.resolution
+        out = np.empty((r.depth, r.height, r.width), dtype=np.float32)
+        self._call("fx_scene_read_field", fid, out.ctypes.data_as(C.POINTER(C.c_float)), out.nbytes)
+        return out
+
+    def write_field(self, field, values: np.ndarray) -> None:
+        fid = abi.FIELDS[field] if isinstance(field, str) else int(field)
+        r = self.resolution
+        arr = np.ascontiguousarray(values, dtype=np.float32).reshape(r.depth, r.height, r.width)
+        self._call("fx_scene_write_field", fid, arr.ctypes.data_as(C.POINTER(C.c_float)), arr.nbytes)
+
+    def read_flags(self, which=abi.FX_FLAGS_RENDER) -> np.ndarray:
+        out = np.empty(self.brick_count, dtype=np.uint32)
+        self._call("fx_scene_read_flags", which, out.ctypes.data_as(C.POINTER(C.c_uint32)), out.size)
+        return out
+
+    def read_active_list(self) -> np.ndarray:
+        out = np.empty(self.brick_count, dtype=np.uint32)
+        n = C.c_uint32()
+        self._call("fx_scene_read_active_list", out.ctypes.data_as(C.POINTER(C.c_uint32)), out.size, C.byref(n))
+        return out[: n.value].copy()
+
+    def read_page_table(self) -> np.ndarray:
+        out = np.empty(self.brick_count, dtype=np.int32)
+        self._call("fx_scene_read_page_table", out.ctypes.data_as(C.POINTER(C.c_int32)), out.size)
+        return out
+
+    def active_brick_count(self) -> int:
+        n = C.c_uint32()
+        self._call("fx_scene_active_brick_count", C.byref(n))
+        return n.value
+
+
+class CombustionVolumeQueueBatch:
+    """A brick-queue batch (fx_create_scenes): scenes of any sizes whose bricks share every
+    level's storage, so that each pass of a step is one launch over the active bricks of all
+    scenes and the coarsest V-cycle level one workgroup per scene. `scenes[i]` reads and
+    writes scene i; step, reset and the solver controls apply to the whole batch."""
+
+    def __init__(self, descs: Sequence[abi.CreateDesc], lib=None):
+        self._lib = lib or abi.load()
+        arr = (abi.CreateDesc * len(descs))(*descs)
+        self._h = C.c_void_p()
+        rc = self._lib.fx_create_scenes(arr, len(descs), C.byref(self._h))
+        if rc != abi.FX_OK:
+            msg = self._lib.fx_last_error(None)
+            raise abi.FenixError(rc, msg.decode() if msg else "")
+        self.scenes = [QueueScene(self, i, d) for i, d in enumerate(descs)]
+
+    def _check(self, rc):
+        return abi.check(self._lib, self._h, rc)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __len__(self):
+        return len(self.scenes)
+
+    def step(self, elapsedSeconds: float, stepSeconds: float) -> None:
+        self._check(self._lib.fx_step(self._h, float(elapsedSeconds), float(stepSeconds)))
+
+    def synchronize(self) -> None:
+        self._check(self._lib.fx_synchronize(self._h))
+
+    def reset(self) -> None:
+        self._check(self._lib.fx_reset(self._h))
+
+    def active_brick_count(self) -> int:
+        n = C.c_uint32()
+        self._check(self._lib.fx_active_brick_count(self._h, C.byref(n)))
+        return n.value
+
+    def set_vcycle_plan(self, down_split: int, up_split: int) -> None:
+        self._check(self._lib.fx_set_vcycle_plan(self._h, int(down_split), int(up_split)))
+
+    def vcycle_plan(self):
+        d, u = C.c_uint32(), C.c_uint32()
+        self._check(self._lib.fx_get_vcycle_plan(self._h, C.byref(d), C.byref(u)))
+        return d.value, u.value
+
+    def autotune_vcycle(self, repeats: int = 4):
+        d, u = C.c_uint32(), C.c_uint32()
+        self._check(self._lib.fx_autotune_vcycle(self._h, int(repeats), C.byref(d), C.byref(u)))
+        return d.value, u.value
+
+    def set_stage_timing(self, enable: bool) -> None:
+        self._check(self._lib.fx_set_stage_timing(self._h, 1 if enable else 0))
+
+    def stage_times(self):
+        return CombustionVolumeSimulation.stage_times(self)
+
+    def dispose(self) -> None:
+        if self._h:
+            self._lib.fx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.dispose()
+        except Exception:
+            pass
+
+
 def createCombustionVolumeSimulation(device=None, options: Optional[Dict] = None, **build_knobs) -> CombustionVolumeSimulation:
     """createCombustionVolumeSimulation(device, options) — ORCH:50-226.
 

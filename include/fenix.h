@@ -296,6 +296,39 @@ int fx_batch_step(fx_batch* batch, double elapsed_s, double step_s);
 /* Waits for every scene's streams. */
 int fx_batch_synchronize(fx_batch* batch);
 
+/* A brick-queue batch: n scenes of any sizes in one simulation. Every level's
+ * storage holds the bricks of all scenes one after another, and the active-brick
+ * lists, the page tables and every stage of the step run over all scenes at once:
+ * each pass of fx_step is one launch over the (scene, brick) queue (the scene of a
+ * brick selects its grid, step constants and emitters), the coarsest V-cycle
+ * smoother runs one workgroup per scene, and the step is captured as one pair of
+ * hipGraphs. Each scene's results are those of the same scene simulated alone.
+ * The scenes must use the sparse domain, the V-cycle solver and threshold
+ * activation, and share precision, advection mode, quality, V-cycle settings,
+ * combustion and activation constants and device; their sizes, runtime
+ * parameters and emitters are free. A batch renders nothing and runs no debug
+ * pass. fx_step, fx_reset, fx_synchronize, fx_destroy, the quality, advection,
+ * V-cycle plan, autotuner and stage-timing calls apply to the whole batch; the
+ * calls that address one scene's fields, flags, emitters or runtime parameters
+ * take the fx_scene_* forms below (FX_ERR_UNSUPPORTED otherwise). */
+int fx_create_scenes(const fx_create_desc* descs, uint32_t n, fx_sim** out);
+/* Scenes of the simulation (1 for fx_create's). */
+uint32_t fx_scene_count(const fx_sim* sim);
+/* The per-scene forms of fx_update_sources, fx_get/set_runtime_params,
+ * fx_read_field, fx_write_field, fx_read_flags, fx_read_active_list,
+ * fx_read_page_table and fx_active_brick_count: same arguments and meaning on
+ * scene `scene` (its own grid; brick indices and list positions are the scene's).
+ * On a simulation of one scene, scene 0 is the simulation itself. */
+int fx_scene_update_sources(fx_sim* sim, uint32_t scene, const float* packed, uint32_t count);
+int fx_scene_get_runtime_params(fx_sim* sim, uint32_t scene, fx_runtime_params* out);
+int fx_scene_set_runtime_params(fx_sim* sim, uint32_t scene, const fx_runtime_params* params, uint32_t present_mask);
+int fx_scene_read_field(fx_sim* sim, uint32_t scene, int32_t field, float* host, size_t bytes);
+int fx_scene_write_field(fx_sim* sim, uint32_t scene, int32_t field, const float* host, size_t bytes);
+int fx_scene_read_flags(fx_sim* sim, uint32_t scene, int32_t which, uint32_t* host, size_t count);
+int fx_scene_read_active_list(fx_sim* sim, uint32_t scene, uint32_t* host, size_t capacity, uint32_t* count);
+int fx_scene_read_page_table(fx_sim* sim, uint32_t scene, int32_t* host, size_t count);
+int fx_scene_active_brick_count(fx_sim* sim, uint32_t scene, uint32_t* out);
+
 /* ---- parameters --------------------------------------------------------- */
 
 /* getRuntimeParams / setRuntimeParams(Partial<...>) — ORCH:338-365. */

@@ -20,7 +20,7 @@ import pytest
 import scenes
 from fenix_amd import abi
 from fenix_amd.render import VolumeRaymarchPass, camera_struct, orbit_camera_snapshot
-from fenix_amd.simulation import CombustionVolumeBatch, CombustionVolumeSimulation
+from fenix_amd.simulation import CombustionVolumeBatch, CombustionVolumeQueueBatch, CombustionVolumeSimulation
 from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -185,15 +185,18 @@ def test_c2_sparse_256_plume():
 
 
 # --------------------------------------------------------------------- C3 --
-def test_c3_batch_64_scenes_each_against_its_oracle():
+@pytest.mark.parametrize("kind", ["streams", "queue"])
+def test_c3_batch_64_scenes_each_against_its_oracle(kind):
     """C3 at its own size: the 64 seeded scenes (edge log-uniform in 32-192, 5-40% of the bricks
-    active, 1-4 emitters) stepped together through the batch API, 2 steps; after each batch step
-    every scene matches its own FP32 oracle (flags, list and page table bit-exact, fields 1e-5)."""
+    active, 1-4 emitters) stepped together, 2 steps, as independent simulations on their own
+    streams (fx_batch_*) and as one brick-queue batch (fx_create_scenes: one launch per pass over
+    the bricks of all scenes); after each batch step every scene matches its own FP32 oracle
+    (flags, list and page table bit-exact, fields 1e-5)."""
     specs = scenes.c3_scenes()
     assert len(specs) == 64
     descs = [scenes.c2_desc(sp["n"]) for sp in specs]
     states = [scenes.plume_state_fast(sp["n"], sp["sources"], sp["target"], sp["seed"]) for sp in specs]
-    batch = CombustionVolumeBatch(descs)
+    batch = CombustionVolumeBatch(descs) if kind == "streams" else CombustionVolumeQueueBatch(descs)
     oracles = []
     for sim, sp, st, desc in zip(batch.scenes, specs, states, descs):
         sim.updateSources(sp["sources"])
@@ -214,7 +217,7 @@ def test_c3_batch_64_scenes_each_against_its_oracle():
         for j, (sim, orc, sp) in enumerate(zip(batch.scenes, oracles, specs)):
             orc.step(k * DT, DT)
             worst = max(worst, check_step(sim, orc, f"C3 scene {j} ({sp['n']}^3) step {k}", sparse=True))
-    log(f"C3: 64 scenes x 2 steps, worst rel L2 {worst:.2e}")
+    log(f"C3 ({kind}): 64 scenes x 2 steps, worst rel L2 {worst:.2e}")
     batch.dispose()
 
 
