@@ -143,9 +143,11 @@ def cpu_baseline(wl, sim, elapsed0, steps):
 
 
 def run_batch(args):
-    """Batched scenes (fx_batch): aggregate active voxel-updates/s of one batch step
-    (every scene steps once; with --workload c2 every scene also renders), against the
-    plain path (the same scenes stepped one at a time, each alone on the GPU)."""
+    """Batched scenes: aggregate active voxel-updates/s of one batch step (every scene steps
+    once; with --workload c2 every scene also renders) as one brick-queue batch
+    (fx_create_scenes, the default) or as independent scenes on their own streams (fx_batch_*),
+    against the plain path (the same scenes stepped one at a time, each alone on the GPU). A
+    queue run also reports the streams batch."""
     # Each scene runs on two streams; give them hardware queues of their own
     # (read by the HIP runtime at initialisation, before any device call).
     os.environ.setdefault("GPU_MAX_HW_QUEUES", str(args.hw_queues))
@@ -173,10 +175,22 @@ def run_batch(args):
         else:
             st = scenes.plume_state_fast(sp["n"], sp["sources"], sp["target"], sp["seed"])
         states.append(st)
-    queue = args.batch_mode == "queue" or (args.batch_mode is None and not render)
-    if queue and render:
-        raise SystemExit("a brick-queue batch renders nothing: use --batch-mode streams with --workload c2")
+    queue = args.batch_mode != "streams"
     dt = 1.0 / 60.0
+    cam = orbit_camera_snapshot()
+    imgs = [HipBuffer(render[0] * render[1] * 8) for _ in specs] if render else []
+
+    def render_all(sims, k):
+        # every scene's raymarch into its own image (after the batch step that produced it)
+        if render:
+            for j, sim in enumerate(sims):
+                rp[j].render_device(sim, cam, imgs[j].ptr.value, "temperature", render[0], render[1], k * dt)
+
+    rp = []
+    if render:
+        for sp in specs:
+            rp.append(VolumeRaymarchPass((sp["n"],) * 3))
+            rp[-1].setRenderParams(stepCount=render[2])
 
     def load(sims):
         for sim, sp, st in zip(sims, specs, states):
@@ -202,33 +216,29 @@ def run_batch(args):
         qb = CombustionVolumeQueueBatch(descs)
         load(qb.scenes)
         qb.step(0.0, dt)
+        render_all(qb.scenes, 0)
         qb.synchronize()
         queue_plan = qb.autotune_vcycle(4) if args.autotune else qb.vcycle_plan()
-        ms, active = timed(lambda k: qb.step(k * dt, dt), qb.synchronize, qb.active_brick_count)
+
+        def queue_step(k):
+            qb.step(k * dt, dt)
+            render_all(qb.scenes, k)
+
+        ms, active = timed(queue_step, qb.synchronize, qb.active_brick_count)
         qb.dispose()
     batch = CombustionVolumeBatch(descs)
     load(batch.scenes)
     del states
-    rp, cam, imgs = None, orbit_camera_snapshot(), []
-    if render:
-        w, h, sc = render
-        rp = {}
-        for sim, sp in zip(batch.scenes, specs):
-            rp[id(sim)] = VolumeRaymarchPass((sp["n"],) * 3)
-            rp[id(sim)].setRenderParams(stepCount=sc)
-            imgs.append(HipBuffer(w * h * 8))
 
     def batch_step(k):
         batch.step(k * dt, dt)
-        if render:
-            for j, sim in enumerate(batch.scenes):
-                rp[id(sim)].render_device(sim, cam, imgs[j].ptr.value, "temperature", render[0], render[1], k * dt)
+        render_all(batch.scenes, k)
 
     def scene_step(j, k):
         sim = batch.scenes[j]
         sim.step(None, k * dt, dt)
         if render:
-            rp[id(sim)].render_device(sim, cam, imgs[j].ptr.value, "temperature", render[0], render[1], k * dt)
+            rp[j].render_device(sim, cam, imgs[j].ptr.value, "temperature", render[0], render[1], k * dt)
 
     batch_step(0)
     batch.synchronize()
@@ -414,10 +424,10 @@ def main():
                          "the plan an autotuned run chose, since profiler overhead changes what the autotuner picks")
     ap.add_argument("--batch", type=int, default=0, metavar="K",
                     help="batched scenes: K 256^3 plume scenes (c2) or the C3 batch (c3, default 64 scenes)")
-    ap.add_argument("--batch-mode", choices=["streams", "queue"], default=None,
-                    help="batches: one brick queue over all scenes (fx_create_scenes; the default without render) or "
-                         "independent scenes on their own streams (fx_batch_*; the default with render); a queue run "
-                         "also times the streams batch ('streams') and every scene alone ('plain')")
+    ap.add_argument("--batch-mode", choices=["streams", "queue"], default="queue",
+                    help="batches: one brick queue over all scenes (fx_create_scenes) or independent scenes on their "
+                         "own streams (fx_batch_*); a queue run also times the streams batch ('streams'); both time "
+                         "every scene alone ('plain')")
     ap.add_argument("--hw-queues", type=int, default=4, choices=range(1, 33), metavar="Q",
                     help="hardware queues per process for batched runs (GPU_MAX_HW_QUEUES, at most 32)")
     ap.add_argument("--bloom", action="store_true",

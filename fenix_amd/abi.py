@@ -228,6 +228,8 @@ FUNCTIONS = {
     "fx_scene_read_active_list": (C.c_int, [P, u32, C.POINTER(u32), C.c_size_t, C.POINTER(u32)]),
     "fx_scene_read_page_table": (C.c_int, [P, u32, C.POINTER(i32), C.c_size_t]),
     "fx_scene_active_brick_count": (C.c_int, [P, u32, C.POINTER(u32)]),
+    "fx_scene_raymarch": (C.c_int, [P, u32, C.POINTER(Camera), C.POINTER(RenderParams), u32, u32, C.POINTER(C.c_uint16)]),
+    "fx_scene_raymarch_device": (C.c_int, [P, u32, C.POINTER(Camera), C.POINTER(RenderParams), u32, u32, C.c_void_p]),
     "fx_diff_default_desc": (None, [C.POINTER(DiffDesc)]),
     "fx_diff_create": (C.c_int, [C.POINTER(DiffDesc), C.POINTER(C.c_void_p)]),
     "fx_diff_destroy": (None, [C.c_void_p]),

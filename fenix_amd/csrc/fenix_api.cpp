@@ -180,6 +180,7 @@ struct fx_sim {
   bool render_pending = false;
   bool scalars_marked = false;  // the snapshot matches all simulation work enqueued so far
   bool render_full = true;      // the next pack rewrites every brick (lists changed outside fx_step)
+  bool render_wanted = true;    // a raymarch read the last snapshot: the next step packs one
   std::vector<int64_t> graph_key;
   int graph_last_pressure = 0;
 
@@ -909,6 +910,17 @@ void snapshot_stale(fx_sim* s) {
   s->scalars_marked = false;
   s->render_full = true;
 }
+// The render snapshot of a step, taken between its two graphs when the previous
+// snapshot was rendered (or none was taken yet); a simulation that is not being
+// rendered skips it, and a raymarch after such steps repacks every brick first.
+int step_snapshot(fx_sim* s) {
+  if (!s->render_wanted) {
+    snapshot_stale(s);
+    return FX_OK;
+  }
+  s->render_wanted = false;
+  return pack_render(s, s->render_full);
+}
 
 }  // namespace
 
@@ -1357,12 +1369,12 @@ int fx_step(fx_sim* s, double elapsed_s, double step_s) {  // ORCH:230-305
       s->lastPressure = s->graph_last_pressure;
     }
     HIP_OK(s, hipGraphLaunch(s->graph_exec, s->stream));
-    rc = s->multi ? FX_OK : pack_render(s, s->render_full);  // a batch of scenes renders nothing
+    rc = step_snapshot(s);
     if (rc != FX_OK) return rc;
     HIP_OK(s, hipGraphLaunch(s->graph_exec_b, s->stream));
   } else {
     record_step_a(s);
-    rc = s->multi ? FX_OK : pack_render(s, s->render_full);
+    rc = step_snapshot(s);
     if (rc != FX_OK) return rc;
     record_step_b(s, vort, pres, dbg);
   }
@@ -1812,11 +1824,13 @@ int fx_scene_active_brick_count(fx_sim* s, uint32_t scene, uint32_t* out) {
   return FX_OK;
 }
 
-static int build_ray_const(fx_sim* s, const fx_camera* c, const fx_render_params* p, uint32_t w, uint32_t h, RayConst& R) {
+// g: the grid of the scene rendered.
+static int build_ray_const(fx_sim* s, const Grid& g, const fx_camera* c, const fx_render_params* p, uint32_t w, uint32_t h,
+                           RayConst& R) {
   if (!c || !p || w == 0 || h == 0) return fail(s, FX_ERR_INVALID_ARG, "invalid raymarch arguments");
   // createVolumeRaymarchPass.ts:83-88, 196-262; volumeWorldBounds.ts:14-31
-  double maxH = double(std::max(s->g.W, s->g.D));
-  float hx = float(2.05 * 13.5 * (s->g.W / maxH)), hy = float(2.05 * 13.5 * (s->g.H / maxH)), hz = float(2.05 * 13.5 * (s->g.D / maxH));
+  double maxH = double(std::max(g.W, g.D));
+  float hx = float(2.05 * 13.5 * (g.W / maxH)), hy = float(2.05 * 13.5 * (g.H / maxH)), hz = float(2.05 * 13.5 * (g.D / maxH));
   float cy = float(double(hy) - 0.25);
   std::memset(&R, 0, sizeof(R));
   for (int k = 0; k < 3; ++k) { R.cam_pos[k] = c->position[k]; R.right[k] = c->right[k]; R.up[k] = c->up[k]; R.fwd[k] = c->forward[k]; }
@@ -1839,11 +1853,13 @@ static int build_ray_const(fx_sim* s, const fx_camera* c, const fx_render_params
   return FX_OK;
 }
 
-int fx_raymarch_device(fx_sim* s, const fx_camera* c, const fx_render_params* p, uint32_t w, uint32_t h, void* dev) {
-  if (!s || !dev) return fail(s, FX_ERR_INVALID_ARG, "null argument");
-  ONE_SCENE(s);
+}  // extern "C"
+namespace {
+// The raymarch of the scene with grid g (one scene: the simulation's grid) from
+// the render snapshot, on the render stream.
+int raymarch_scene(fx_sim* s, const Grid& g, const fx_camera* c, const fx_render_params* p, uint32_t w, uint32_t h, void* dev) {
   RayConst R;
-  int rc = build_ray_const(s, c, p, w, h, R);
+  int rc = build_ray_const(s, g, c, p, w, h, R);
   if (rc != FX_OK) return rc;
   HIP_OK(s, hipSetDevice(s->device));
   if (!s->scalars_marked) {  // simulation work since the last step: snapshot everything now
@@ -1854,14 +1870,30 @@ int fx_raymarch_device(fx_sim* s, const fx_camera* c, const fx_render_params* p,
   if (s->render_overlap) HIP_OK(s, hipStreamWaitEvent(s->rstream, s->ev_scalars, 0));
   {
     StageTimer timer(s, FX_TIME_RAYMARCH, rs);
-    launch_raymarch(rs, s->g, R, s->d_render, s->d_bits, s->d_rqueue, s->ray_blocks, static_cast<uint16_t*>(dev));
+    launch_raymarch(rs, g, R, s->d_render, s->d_bits, s->d_rqueue, s->ray_blocks, static_cast<uint16_t*>(dev));
   }
   if (s->render_overlap) {
     HIP_OK(s, hipEventRecord(s->ev_render, s->rstream));
     s->render_pending = true;
   }
+  s->render_wanted = true;  // the next step packs its snapshot
   HIP_OK(s, hipGetLastError());
   return FX_OK;
+}
+}  // namespace
+extern "C" {
+
+int fx_raymarch_device(fx_sim* s, const fx_camera* c, const fx_render_params* p, uint32_t w, uint32_t h, void* dev) {
+  if (!s || !dev) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  ONE_SCENE(s);
+  return raymarch_scene(s, s->g, c, p, w, h, dev);
+}
+int fx_scene_raymarch_device(fx_sim* s, uint32_t scene, const fx_camera* c, const fx_render_params* p, uint32_t w, uint32_t h,
+                             void* dev) {
+  if (!s || !dev) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  const Grid* g = scene_grid(s, scene);
+  if (!g) return FX_ERR_INVALID_ARG;
+  return raymarch_scene(s, *g, c, p, w, h, dev);
 }
 
 int fx_bloom_device(fx_sim* s, const fx_bloom_params* p, uint32_t w, uint32_t h, const void* src, void* dst) {
@@ -1910,13 +1942,14 @@ int fx_bloom(fx_sim* s, const fx_bloom_params* p, uint32_t w, uint32_t h, const 
   return FX_OK;
 }
 
-int fx_raymarch(fx_sim* s, const fx_camera* c, const fx_render_params* p, uint32_t w, uint32_t h, uint16_t* host) {
-  if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
-  ONE_SCENE(s);
+}  // extern "C"
+namespace {
+// fx_raymarch of the scene with grid g into host memory (synchronous).
+int raymarch_host(fx_sim* s, const Grid& g, const fx_camera* c, const fx_render_params* p, uint32_t w, uint32_t h, uint16_t* host) {
   HIP_OK(s, hipSetDevice(s->device));
   void* dev = nullptr;
   HIP_OK(s, hipMalloc(&dev, size_t(w) * h * 8));
-  int rc = fx_raymarch_device(s, c, p, w, h, dev);
+  int rc = raymarch_scene(s, g, c, p, w, h, dev);
   if (rc == FX_OK) {
     // the stream the raymarch was enqueued on (the simulation stream when rendering is not overlapped)
     hipStream_t rs = s->render_overlap ? s->rstream : s->stream;
@@ -1926,6 +1959,20 @@ int fx_raymarch(fx_sim* s, const fx_camera* c, const fx_render_params* p, uint32
   }
   (void)hipFree(dev);
   return rc;
+}
+}  // namespace
+extern "C" {
+int fx_raymarch(fx_sim* s, const fx_camera* c, const fx_render_params* p, uint32_t w, uint32_t h, uint16_t* host) {
+  if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  ONE_SCENE(s);
+  return raymarch_host(s, s->g, c, p, w, h, host);
+}
+int fx_scene_raymarch(fx_sim* s, uint32_t scene, const fx_camera* c, const fx_render_params* p, uint32_t w, uint32_t h,
+                      uint16_t* host) {
+  if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  const Grid* g = scene_grid(s, scene);
+  if (!g) return FX_ERR_INVALID_ARG;
+  return raymarch_host(s, *g, c, p, w, h, host);
 }
 
 }  // extern "C"

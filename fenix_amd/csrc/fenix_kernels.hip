@@ -2273,14 +2273,16 @@ __global__ void k_pack_flags(const uint32_t* __restrict__ flags, uint32_t nb, ui
 
 // One ray (createVolumeRaymarchPass.ts fsMain) for pixel (px, py), reading the
 // render snapshot RB (density, temperature, fuel, reaction per voxel) and the
-// packed brick flags (sbits in LDS, else gbits in memory).
+// packed brick flags (sbits in LDS, else gbits in memory). g is one scene's grid;
+// its flags start at bit (g.base & 31) of gbits[0] (the launcher offsets gbits by
+// g.base / 32 words).
 __device__ __forceinline__ void march_pixel(const Grid& g, const RayConst& R, const float4* __restrict__ RB,
                                             const uint32_t* __restrict__ gbits, const uint32_t* sbits, uint32_t px, uint32_t py,
                                             uint16_t* out) {
   // isActiveSample (RP:378-381)
   auto active_brick = [&](uint32_t x, uint32_t y, uint32_t z) {
     uint32_t bx = min(x >> 3, g.BX - 1), by = min(y >> 3, g.BY - 1), bz = min(z >> 3, g.BZ - 1);
-    uint32_t b = bx + g.BX * (by + g.BY * bz);
+    uint32_t b = (g.base & 31u) + bx + g.BX * (by + g.BY * bz);
     return (((sbits ? sbits[b >> 5] : gbits[b >> 5]) >> (b & 31u)) & 1u) != 0u;
   };
   if (px >= R.width || py >= R.height) return;
@@ -3378,7 +3380,9 @@ void launch_render_pack(hipStream_t s, int blocks, const uint32_t* list, const u
 }
 void launch_raymarch(hipStream_t s, Grid g, const RayConst& rc, const void* rb, const uint32_t* bits, uint32_t* queue, int blocks,
                      uint16_t* out) {
-  const uint32_t nb = uint32_t(brick_count(g)), nwords = (nb + 31) / 32;
+  // the flag words of g's scene (its bricks start at bit base & 31 of the first)
+  const uint32_t nb = uint32_t(brick_count(g)), nwords = ((g.base & 31u) + nb + 31) / 32;
+  bits += g.base >> 5;
   const int lds = nwords <= 8192 ? 1 : 0;  // up to 32 KB of flag bits in LDS
   launch_clear_u32(s, queue, 1, 0u);
   const uint32_t ntiles = ((rc.width + 15) / 16) * ((rc.height + 15) / 16);

@@ -154,16 +154,22 @@ class VolumeRaymarchPass:
         out = np.zeros((renderHeight, renderWidth, 4), dtype=np.float16)
         cam = camera_struct(camera)
         p = self.params_struct(displayMode, elapsedSeconds)
-        sim._check(sim._lib.fx_raymarch(sim.handle, C.byref(cam), C.byref(p), renderWidth, renderHeight,
-                                        out.ctypes.data_as(C.POINTER(C.c_uint16))))
+        dst = out.ctypes.data_as(C.POINTER(C.c_uint16))
+        if hasattr(sim, "raymarch_call"):  # a scene of a brick-queue batch
+            sim.raymarch_call("fx_scene_raymarch", cam, p, renderWidth, renderHeight, dst)
+        else:
+            sim._check(sim._lib.fx_raymarch(sim.handle, C.byref(cam), C.byref(p), renderWidth, renderHeight, dst))
         return out
 
     def render_device(self, sim, camera, device_ptr, displayMode="temperature", renderWidth=256, renderHeight=256,
                       elapsedSeconds=0.0):
         cam = camera_struct(camera)
         p = self.params_struct(displayMode, elapsedSeconds)
-        sim._check(sim._lib.fx_raymarch_device(sim.handle, C.byref(cam), C.byref(p), renderWidth, renderHeight,
-                                               C.c_void_p(device_ptr)))
+        if hasattr(sim, "raymarch_call"):  # a scene of a brick-queue batch
+            sim.raymarch_call("fx_scene_raymarch_device", cam, p, renderWidth, renderHeight, C.c_void_p(device_ptr))
+        else:
+            sim._check(sim._lib.fx_raymarch_device(sim.handle, C.byref(cam), C.byref(p), renderWidth, renderHeight,
+                                                   C.c_void_p(device_ptr)))
 
 
 class BloomPass:

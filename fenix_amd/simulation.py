@@ -460,6 +460,10 @@ class QueueScene:
         self._call("fx_scene_active_brick_count", C.byref(n))
         return n.value
 
+    def raymarch_call(self, name, cam, params, width, height, dst):
+        """fx_scene_raymarch / fx_scene_raymarch_device of this scene (VolumeRaymarchPass uses it)."""
+        self._call(name, C.byref(cam), C.byref(params), int(width), int(height), dst)
+
 
 class CombustionVolumeQueueBatch:
     """A brick-queue batch (fx_create_scenes): scenes of any sizes whose bricks share every

@@ -306,8 +306,8 @@ int fx_batch_synchronize(fx_batch* batch);
  * The scenes must use the sparse domain, the V-cycle solver and threshold
  * activation, and share precision, advection mode, quality, V-cycle settings,
  * combustion and activation constants and device; their sizes, runtime
- * parameters and emitters are free. A batch renders nothing and runs no debug
- * pass. fx_step, fx_reset, fx_synchronize, fx_destroy, the quality, advection,
+ * parameters and emitters are free. Scenes render one at a time
+ * (fx_scene_raymarch*); a batch runs no debug pass. fx_step, fx_reset, fx_synchronize, fx_destroy, the quality, advection,
  * V-cycle plan, autotuner and stage-timing calls apply to the whole batch; the
  * calls that address one scene's fields, flags, emitters or runtime parameters
  * take the fx_scene_* forms below (FX_ERR_UNSUPPORTED otherwise). */
@@ -328,6 +328,12 @@ int fx_scene_read_flags(fx_sim* sim, uint32_t scene, int32_t which, uint32_t* ho
 int fx_scene_read_active_list(fx_sim* sim, uint32_t scene, uint32_t* host, size_t capacity, uint32_t* count);
 int fx_scene_read_page_table(fx_sim* sim, uint32_t scene, int32_t* host, size_t count);
 int fx_scene_active_brick_count(fx_sim* sim, uint32_t scene, uint32_t* out);
+/* fx_raymarch / fx_raymarch_device of one scene: its grid, world bounds and flags
+ * in the step's render snapshot (which every scene of a batch shares). */
+int fx_scene_raymarch(fx_sim* sim, uint32_t scene, const fx_camera* camera, const fx_render_params* params, uint32_t width,
+                      uint32_t height, uint16_t* host_rgba16f);
+int fx_scene_raymarch_device(fx_sim* sim, uint32_t scene, const fx_camera* camera, const fx_render_params* params,
+                             uint32_t width, uint32_t height, void* device_rgba16f);
 
 /* ---- parameters --------------------------------------------------------- */
 

@@ -336,6 +336,40 @@ def test_raymarch_parity_point_and_several_lights():
     _check_images(sim, orc, rp, modes=("temperature",))
 
 
+def test_snapshot_skipped_while_not_rendered():
+    """A step packs the render snapshot only when the previous one was rendered (a simulation
+    that is not being rendered skips it); a raymarch after unrendered steps repacks every brick
+    first. A scene rendered only after its fourth step gives the same image and fields as the
+    same scene rendered after every step."""
+    n, w, h = 48, 96, 80
+    desc = scenes.c2_desc(n)
+    srcs = scenes.c2_sources(n, radius_vox=4.0)
+    st, _ = scenes.c2_plume_state(n, srcs, target=0.15)
+    sims = []
+    for _ in range(2):
+        sim = CombustionVolumeSimulation(desc)
+        sim.updateSources(srcs)
+        for k, v in st.items():
+            sim.write_field(k, v)
+        sims.append(sim)
+    rp = VolumeRaymarchPass((n, n, n))
+    rp.setRenderParams(stepCount=64)
+    cam = orbit_camera_snapshot()
+    for k in range(4):
+        for j, sim in enumerate(sims):
+            sim.step(None, k / 60.0, 1 / 60.0)
+            if j == 0:
+                rp.render(sim, cam, "temperature", w, h, k / 60.0)
+    a = rp.render(sims[0], cam, "temperature", w, h, 3 / 60.0)
+    b = rp.render(sims[1], cam, "temperature", w, h, 3 / 60.0)
+    np.testing.assert_array_equal(a, b)
+    assert float(a[..., 3].astype(np.float32).max()) > 0.05
+    for name in ("density", "temperature", "velocity_y", "pressure"):
+        np.testing.assert_array_equal(sims[0].read_field(name), sims[1].read_field(name), err_msg=name)
+    for sim in sims:
+        sim.dispose()
+
+
 def test_render_overlap_matches_serial():
     """The raymarch runs on its own stream, overlapped with the second half of
     the step; images equal those rendered after a full synchronisation, and a

@@ -11,6 +11,7 @@ import pytest
 
 import scenes
 from fenix_amd import abi
+from fenix_amd.render import VolumeRaymarchPass, orbit_camera_snapshot
 from fenix_amd.simulation import CombustionVolumeQueueBatch, CombustionVolumeSimulation
 
 pytestmark = pytest.mark.gpu
@@ -87,6 +88,42 @@ def test_queue_batch_equals_scenes_alone(precision):
         sim.step(None, 0.0, DT)
     for j, (sc, sim) in enumerate(zip(batch.scenes, alone)):
         assert_same(sc, sim, f"scene {j} after reset")
+    batch.dispose()
+    for sim in alone:
+        sim.dispose()
+
+
+def test_queue_batch_scene_images_equal_scenes_alone():
+    """Each scene of a batch renders from the batch's shared snapshot with its own grid, world
+    bounds and brick flags (fx_scene_raymarch): the images equal those of the scenes alone, bit
+    for bit, with every scene rendered after each step and with renders skipped for two steps."""
+    spec = make_scenes(abi.FX_PRECISION_FP32)[:3]
+    batch = CombustionVolumeQueueBatch([d for d, _, _ in spec])
+    alone = [CombustionVolumeSimulation(d) for d, _, _ in spec]
+    passes = []
+    for sc, sim, (d, srcs, st) in zip(batch.scenes, alone, spec):
+        sc.updateSources(srcs)
+        sim.updateSources(srcs)
+        for f, v in st.items():
+            sc.write_field(f, v)
+            sim.write_field(f, v)
+        rp = VolumeRaymarchPass((d.width, d.height, d.depth))
+        rp.setRenderParams(stepCount=64)
+        passes.append(rp)
+    cam = orbit_camera_snapshot()
+    lit = 0.0
+    for k in range(5):
+        batch.step(k * DT, DT)
+        for sim in alone:
+            sim.step(None, k * DT, DT)
+        if k in (1, 2):  # nobody renders these steps
+            continue
+        for j, (sc, sim, rp) in enumerate(zip(batch.scenes, alone, passes)):
+            a = rp.render(sc, cam, "temperature", 80, 64, k * DT)
+            b = rp.render(sim, cam, "temperature", 80, 64, k * DT)
+            np.testing.assert_array_equal(a, b, err_msg=f"scene {j} step {k}")
+            lit = max(lit, float(a[..., 3].astype(np.float32).max()))
+    assert lit > 0.05
     batch.dispose()
     for sim in alone:
         sim.dispose()
