@@ -158,6 +158,14 @@ struct fx_sim {
   // next one. ev_render marks the last raymarch done; the next pack waits for it.
   hipStream_t rstream = nullptr;
   hipEvent_t ev_scalars = nullptr, ev_render = nullptr;
+  // A batch of scenes renders them on up to kRenderLanes streams (scene k on lane
+  // k % rlanes; lane 0 is rstream / ev_render), as concurrently as separate
+  // simulations would; each lane has its own tile counter (d_rqueue + lane).
+  static constexpr int kRenderLanes = 4;
+  int rlanes = 1;
+  hipStream_t rl_stream[kRenderLanes] = {};
+  hipEvent_t rl_ev[kRenderLanes] = {};
+  bool rl_pending[kRenderLanes] = {};
   void* d_render = nullptr;      // float4 per voxel
   // Gather mirrors of the scalar advection (float4 per voxel): the scalars of
   // set 0 and the advected velocity. Inside a step the mirror of the scalars is
@@ -181,6 +189,7 @@ struct fx_sim {
   bool scalars_marked = false;  // the snapshot matches all simulation work enqueued so far
   bool render_full = true;      // the next pack rewrites every brick (lists changed outside fx_step)
   bool render_wanted = true;    // a raymarch read the last snapshot: the next step packs one
+  hipStream_t last_render_stream = nullptr;  // where the last raymarch was enqueued
   std::vector<int64_t> graph_key;
   int graph_last_pressure = 0;
 
@@ -893,6 +902,11 @@ int pack_render(fx_sim* s, bool full) {
     HIP_OK(s, hipStreamWaitEvent(s->stream, s->ev_render, 0));
     s->render_pending = false;
   }
+  for (int r = 1; r < s->rlanes; ++r)
+    if (s->rl_pending[r]) {
+      HIP_OK(s, hipStreamWaitEvent(s->stream, s->rl_ev[r], 0));
+      s->rl_pending[r] = false;
+    }
   {
     StageTimer timer(s, FX_TIME_RAYMARCH);
     const bool all = full || !s->sparse;
@@ -1214,7 +1228,14 @@ int create_impl(const fx_create_desc* descs, uint32_t nsc, bool multi, fx_sim** 
   }
   s->norm_cap = std::max(1, desc->vcycle.cycles);
   if (dalloc(s, &s->d_norms, size_t(s->norm_cap) * kNormLanes) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
-  if (dalloc(s, &s->d_rqueue, 1) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
+  if (dalloc(s, &s->d_rqueue, fx_sim::kRenderLanes) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
+  if (multi) {  // render lanes 1.. of a batch
+    s->rlanes = int(std::min<uint32_t>(nsc, fx_sim::kRenderLanes));
+    for (int r = 1; r < s->rlanes; ++r)
+      if (hipStreamCreateWithFlags(&s->rl_stream[r], hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&s->rl_ev[r], hipEventDisableTiming) != hipSuccess)
+        return bail(fail(s, FX_ERR_DEVICE, "stream creation failed"));
+  }
   for (void** mirror : {&s->d_render, &s->d_sq, &s->d_uq}) {
     float* q = nullptr;
     if (dalloc(s, &q, nsv * 4) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
@@ -1271,6 +1292,8 @@ void fx_destroy(fx_sim* s) {
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   if (s->rstream) (void)hipStreamSynchronize(s->rstream);
+  for (int r = 1; r < fx_sim::kRenderLanes; ++r)
+    if (s->rl_stream[r]) (void)hipStreamSynchronize(s->rl_stream[r]);
   auto F = [](void* p) { if (p) (void)hipFree(p); };
   for (int k = 0; k < 2; ++k) for (int f = 0; f < 4; ++f) F(s->S[k][f]);
   for (int c = 0; c < 3; ++c) { F(s->u[c]); F(s->us[c]); F(s->w[c]); }
@@ -1300,6 +1323,10 @@ void fx_destroy(fx_sim* s) {
   if (s->ev_count) (void)hipEventDestroy(s->ev_count);
   if (s->h_fine_count) (void)hipHostFree(s->h_fine_count);
   if (s->rstream) (void)hipStreamDestroy(s->rstream);
+  for (int r = 1; r < fx_sim::kRenderLanes; ++r) {
+    if (s->rl_ev[r]) (void)hipEventDestroy(s->rl_ev[r]);
+    if (s->rl_stream[r]) (void)hipStreamDestroy(s->rl_stream[r]);
+  }
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -1434,6 +1461,7 @@ int fx_synchronize(fx_sim* s) {
   HIP_OK(s, hipSetDevice(s->device));
   HIP_OK(s, hipStreamSynchronize(s->stream));
   HIP_OK(s, hipStreamSynchronize(s->rstream));
+  for (int r = 1; r < s->rlanes; ++r) HIP_OK(s, hipStreamSynchronize(s->rl_stream[r]));
   return FX_OK;
 }
 
@@ -1866,16 +1894,21 @@ int raymarch_scene(fx_sim* s, const Grid& g, const fx_camera* c, const fx_render
     rc = pack_render(s, true);
     if (rc != FX_OK) return rc;
   }
-  hipStream_t rs = s->render_overlap ? s->rstream : s->stream;
-  if (s->render_overlap) HIP_OK(s, hipStreamWaitEvent(s->rstream, s->ev_scalars, 0));
+  const int lane = s->multi && s->render_overlap ? int(g.scene % uint32_t(s->rlanes)) : 0;
+  hipStream_t rs = !s->render_overlap ? s->stream : lane ? s->rl_stream[lane] : s->rstream;
+  if (s->render_overlap) HIP_OK(s, hipStreamWaitEvent(rs, s->ev_scalars, 0));
   {
     StageTimer timer(s, FX_TIME_RAYMARCH, rs);
-    launch_raymarch(rs, g, R, s->d_render, s->d_bits, s->d_rqueue, s->ray_blocks, static_cast<uint16_t*>(dev));
+    launch_raymarch(rs, g, R, s->d_render, s->d_bits, s->d_rqueue + lane, s->ray_blocks, static_cast<uint16_t*>(dev));
   }
-  if (s->render_overlap) {
+  if (s->render_overlap && lane) {
+    HIP_OK(s, hipEventRecord(s->rl_ev[lane], rs));
+    s->rl_pending[lane] = true;
+  } else if (s->render_overlap) {
     HIP_OK(s, hipEventRecord(s->ev_render, s->rstream));
     s->render_pending = true;
   }
+  s->last_render_stream = rs;
   s->render_wanted = true;  // the next step packs its snapshot
   HIP_OK(s, hipGetLastError());
   return FX_OK;
@@ -1898,6 +1931,7 @@ int fx_scene_raymarch_device(fx_sim* s, uint32_t scene, const fx_camera* c, cons
 
 int fx_bloom_device(fx_sim* s, const fx_bloom_params* p, uint32_t w, uint32_t h, const void* src, void* dst) {
   if (!s || !p || !src || !dst) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  ONE_SCENE(s);
   if (w == 0 || h == 0 || uint64_t(w) * h > (1ull << 28)) return fail(s, FX_ERR_INVALID_ARG, "image size out of range");
   HIP_OK(s, hipSetDevice(s->device));
   hipStream_t rs = s->render_overlap ? s->rstream : s->stream;
@@ -1922,6 +1956,7 @@ int fx_bloom_device(fx_sim* s, const fx_bloom_params* p, uint32_t w, uint32_t h,
 
 int fx_bloom(fx_sim* s, const fx_bloom_params* p, uint32_t w, uint32_t h, const uint16_t* host_src, uint8_t* host_dst) {
   if (!s || !p || !host_src || !host_dst) return fail(s, FX_ERR_INVALID_ARG, "null argument");
+  ONE_SCENE(s);
   if (w == 0 || h == 0 || uint64_t(w) * h > (1ull << 28)) return fail(s, FX_ERR_INVALID_ARG, "image size out of range");
   HIP_OK(s, hipSetDevice(s->device));
   hipStream_t rs = s->render_overlap ? s->rstream : s->stream;
@@ -1952,7 +1987,7 @@ int raymarch_host(fx_sim* s, const Grid& g, const fx_camera* c, const fx_render_
   int rc = raymarch_scene(s, g, c, p, w, h, dev);
   if (rc == FX_OK) {
     // the stream the raymarch was enqueued on (the simulation stream when rendering is not overlapped)
-    hipStream_t rs = s->render_overlap ? s->rstream : s->stream;
+    hipStream_t rs = s->last_render_stream;
     hipError_t e = hipMemcpyAsync(host, dev, size_t(w) * h * 8, hipMemcpyDeviceToHost, rs);
     if (e == hipSuccess) e = hipStreamSynchronize(rs);
     if (e != hipSuccess) rc = fail(s, FX_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
@@ -2044,6 +2079,7 @@ int fx_autotune_vcycle(fx_sim* s, uint32_t repeats, uint32_t* down_split, uint32
   HIP_OK(s, hipSetDevice(s->device));
   HIP_OK(s, hipStreamSynchronize(s->stream));
   HIP_OK(s, hipStreamSynchronize(s->rstream));
+  for (int r = 1; r < s->rlanes; ++r) HIP_OK(s, hipStreamSynchronize(s->rl_stream[r]));
   repeats = std::max(1u, repeats);
   hipEvent_t a = nullptr, b = nullptr;
   HIP_OK(s, hipEventCreate(&a));
@@ -2156,6 +2192,7 @@ int fx_set_stage_timing(fx_sim* s, int32_t enable) {
   HIP_OK(s, hipSetDevice(s->device));
   HIP_OK(s, hipStreamSynchronize(s->stream));
   HIP_OK(s, hipStreamSynchronize(s->rstream));
+  for (int r = 1; r < s->rlanes; ++r) HIP_OK(s, hipStreamSynchronize(s->rl_stream[r]));
   drain_timing(s);
   std::memset(s->acc_ms, 0, sizeof(s->acc_ms));
   std::memset(s->acc_calls, 0, sizeof(s->acc_calls));
@@ -2168,6 +2205,7 @@ int fx_read_stage_times(fx_sim* s, double* total_ms, uint32_t* calls, size_t n) 
   HIP_OK(s, hipSetDevice(s->device));
   HIP_OK(s, hipStreamSynchronize(s->stream));
   HIP_OK(s, hipStreamSynchronize(s->rstream));
+  for (int r = 1; r < s->rlanes; ++r) HIP_OK(s, hipStreamSynchronize(s->rl_stream[r]));
   drain_timing(s);
   for (size_t i = 0; i < std::min<size_t>(n, FX_TIME_COUNT); ++i) {
     if (total_ms) total_ms[i] = s->acc_ms[i];

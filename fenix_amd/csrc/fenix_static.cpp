@@ -56,6 +56,8 @@ using namespace fx;
 extern "C" int fx_bloom_device_noalloc(fx_sim* s, const fx_bloom_params* p, uint32_t w, uint32_t h, const void* src,
                                        void* dst) {
   if (!s || !p || !src || !dst) return sim_fail(s, FX_ERR_INVALID_ARG, "null argument");
+  if (fx_scene_count(s) > 1)  // scenes of a batch render on several streams
+    return sim_fail(s, FX_ERR_UNSUPPORTED, "bloom runs on a simulation of one scene");
   if (w == 0 || h == 0) return sim_fail(s, FX_ERR_INVALID_ARG, "image size out of range");
   const uint32_t hw = w / 2 > 0 ? w / 2 : 1, hh = h / 2 > 0 ? h / 2 : 1;  // createBloomPass.ts:84-89
   if (size_t(hw) * hh > kHalfPixels) {
