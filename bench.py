@@ -212,8 +212,13 @@ def run_batch(args):
         return (t1 - t0) * 1e3 / args.steps, 0.5 * (a0 + count())
 
     queue_plan = None
+    pinned = tuple(int(v, 0) for v in args.vcycle_plan.split(",")) if args.vcycle_plan else None
+    if pinned:  # a pinned plan replaces the autotuner (profiled runs)
+        args.autotune = False
     if queue:
         qb = CombustionVolumeQueueBatch(descs)
+        if pinned:
+            qb.set_vcycle_plan(*pinned)
         load(qb.scenes)
         qb.step(0.0, dt)
         render_all(qb.scenes, 0)
@@ -226,7 +231,23 @@ def run_batch(args):
 
         ms, active = timed(queue_step, qb.synchronize, qb.active_brick_count)
         qb.dispose()
+    if queue and args.no_compare:
+        out = {
+            "metric": "aggregate active voxel-updates/s per batch step (brick queue)",
+            "value": active * 512 / (ms * 1e-3), "unit": "active voxel-updates/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": label, "scenes": len(specs), "storage": args.precision, "active_bricks": active,
+                       "vcycle_plan": {"down_split": queue_plan[0], "up_split": queue_plan[1], "autotuned": args.autotune}},
+        }
+        for b in imgs:
+            b.free()
+        print(json.dumps(out), flush=True)
+        return
     batch = CombustionVolumeBatch(descs)
+    if pinned:
+        for sim in batch.scenes:
+            sim.set_vcycle_plan(*pinned)
     load(batch.scenes)
     del states
 
@@ -428,6 +449,8 @@ def main():
                     help="batches: one brick queue over all scenes (fx_create_scenes) or independent scenes on their "
                          "own streams (fx_batch_*); a queue run also times the streams batch ('streams'); both time "
                          "every scene alone ('plain')")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="brick-queue batches: skip the streams batch and the plain path (a run of one simulation)")
     ap.add_argument("--hw-queues", type=int, default=4, choices=range(1, 33), metavar="Q",
                     help="hardware queues per process for batched runs (GPU_MAX_HW_QUEUES, at most 32)")
     ap.add_argument("--bloom", action="store_true",
