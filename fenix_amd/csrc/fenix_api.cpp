@@ -531,9 +531,10 @@ int solve_vcycle(fx_sim* s) {
     launch_coarse_setup(s->stream, level_list(s, L - 1), level_count(s, L - 1), LC.g, bits_of(s, L - 1), LC.tab, LC.tab_n, LC.p[0],
                         LC.p[1], nsc);
   if (!s->sparse) s->dense_tables = true;
-  // n sweeps at level l; zero_start: the first sweep starts from p = 0 (a
-  // coarse level's correction), so neither a clear nor a read of p is needed.
-  auto smoothW = [&](int l, int n, bool zero_start, bool pair) {
+  // n sweeps at level l, `per` (1, 2 or 4) per launch; zero_start: the first sweep
+  // starts from p = 0 (a coarse level's correction), so neither a clear nor a read
+  // of p is needed.
+  auto smoothW = [&](int l, int n, bool zero_start, int per) {
     Level& V = s->lv[l];
     if (n == 0) {
       if (zero_start) { cur[l] = 0; zero_level_p(s, l, 0); }
@@ -548,7 +549,14 @@ int solve_vcycle(fx_sim* s) {
     }
     for (int k = 0; k < n; ++k) {
       const float* src = (zero_start && k == 0) ? nullptr : V.p[cur[l]];
-      if (pair && k + 1 < n) {
+      if (per >= 4 && k + 3 < n) {
+        launch_sweep_quad(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, bits_of(s, l), s->d_const, V.div,
+                          src, V.p[1 - cur[l]]);
+        cur[l] = 1 - cur[l];
+        k += 3;
+        continue;
+      }
+      if (per >= 2 && k + 1 < n) {
         launch_sweep_pair(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, bits_of(s, l), s->d_const, V.div,
                           src, V.p[1 - cur[l]]);
         cur[l] = 1 - cur[l];
@@ -579,7 +587,7 @@ int solve_vcycle(fx_sim* s) {
       const bool split = (s->split_down >> l) & 1u;
       if (split && ((s->split_down >> (kPlanFuseShift + l)) & 1u) && v.pre_sweeps >= 2) {
         // split leg whose last pre-sweep is fused with the residual and the restriction
-        smoothW(l, v.pre_sweeps - 1, zero, (s->split_down >> (kPlanPairShift + l)) & 1u);
+        smoothW(l, v.pre_sweeps - 1, zero, ((s->split_down >> (kPlanPairShift + l)) & 1u) ? 2 : 1);
         if (launch_vdown(s->stream, grid_for(s, V.nb), 1, level_list(s, l), level_count(s, l), V.g, bits_of(s, l), W.g,
                          mask_of(s, l + 1), s->d_const, V.div, V.p[cur[l]], V.p[1 - cur[l]], W.div, norm)) {
           cur[l] = 1 - cur[l];
@@ -593,14 +601,17 @@ int solve_vcycle(fx_sim* s) {
         continue;
       }
       // split leg: lean sweeps, residual into the other pressure buffer, restriction
-      smoothW(l, v.pre_sweeps, zero, (s->split_down >> (kPlanPairShift + l)) & 1u);
+      smoothW(l, v.pre_sweeps, zero, ((s->split_down >> (kPlanPairShift + l)) & 1u) ? 2 : 1);
       if (zero && v.pre_sweeps == 0) zero_level_p(s, l, cur[l]);
       launch_residual_lean(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, bits_of(s, l), V.div,
                            V.p[cur[l]], V.p[1 - cur[l]], norm);
       launch_restrict_lean(s->stream, grid_for(s, W.nb), level_list(s, l + 1), level_count(s, l + 1), V.g, W.g, bits_of(s, l + 1),
                            V.p[1 - cur[l]], W.div);
     }
-    smoothW(L - 1, v.coarse_sweeps, true, false);
+    // coarsest: the one-workgroup smoother, or (beyond its capacity) multi-workgroup
+    // launches of 1, 2 or 4 sweeps each
+    const uint32_t cper = (s->split_down >> kPlanCoarseShift) & 3u;
+    smoothW(L - 1, v.coarse_sweeps, true, cper == 2 ? 4 : cper == 1 ? 2 : 1);
     for (int l = L - 2; l >= 0; --l) {
       Level& V = s->lv[l];
       const uint32_t* lst = level_list(s, l);
@@ -611,13 +622,13 @@ int solve_vcycle(fx_sim* s) {
                      V.p[cur[l]], V.p[1 - cur[l]])) {
         // split leg whose prolongation is fused with the first post-sweep, then lean sweeps
         cur[l] = 1 - cur[l];
-        smoothW(l, v.post_sweeps - 1, false, (s->split_up >> (kPlanPairShift + l)) & 1u);
+        smoothW(l, v.post_sweeps - 1, false, ((s->split_up >> (kPlanPairShift + l)) & 1u) ? 2 : 1);
         continue;
       }
       if ((s->split_up >> l) & 1u) {
         // split leg: in-place prolongation, then lean sweeps
         launch_prolong_add(s->stream, grid_for(s, V.nb), lst, cnt, s->lv[l + 1].g, V.g, bits_of(s, l), e, V.p[cur[l]]);
-        smoothW(l, v.post_sweeps, false, (s->split_up >> (kPlanPairShift + l)) & 1u);
+        smoothW(l, v.post_sweeps, false, ((s->split_up >> (kPlanPairShift + l)) & 1u) ? 2 : 1);
         continue;
       }
       if (launch_vup(s->stream, grid_for(s, V.nb), v.post_sweeps, lst, cnt, s->lv[l + 1].g, V.g, bits_of(s, l), s->d_const, e,
@@ -626,7 +637,7 @@ int solve_vcycle(fx_sim* s) {
         continue;
       }
       launch_prolong(s->stream, grid_for(s, V.nb), lst, cnt, s->lv[l + 1].g, V.g, mask_of(s, l), e, V.p[cur[l]]);
-      smoothW(l, v.post_sweeps, false, false);
+      smoothW(l, v.post_sweeps, false, 1);
     }
   }
   return cur[0];
@@ -2085,6 +2096,7 @@ int fx_autotune_vcycle(fx_sim* s, uint32_t repeats, uint32_t* down_split, uint32
   HIP_OK(s, hipStreamSynchronize(s->rstream));
   for (int r = 1; r < s->rlanes; ++r) HIP_OK(s, hipStreamSynchronize(s->rl_stream[r]));
   repeats = std::max(1u, repeats);
+  update_coarse_plan(s);  // the last step's active count has arrived
   hipEvent_t a = nullptr, b = nullptr;
   HIP_OK(s, hipEventCreate(&a));
   HIP_OK(s, hipEventCreate(&b));
@@ -2148,6 +2160,19 @@ int fx_autotune_vcycle(fx_sim* s, uint32_t repeats, uint32_t* down_split, uint32
       if (t[c] < t[best_c]) best_c = c;
     set_leg(s->split_down, l, downs[best_c % downs.size()]);
     set_leg(s->split_up, l, ups[best_c / downs.size()]);
+  }
+  if (s->coarse_split && rc == FX_OK) {
+    // the coarsest level beyond the one-workgroup smoother: 1, 2 or 4 sweeps per launch
+    float t[3] = {0.f, 0.f, 0.f};
+    for (int round = 0; round < 2 && rc == FX_OK; ++round)
+      for (uint32_t opt = 0; opt < 3 && rc == FX_OK; ++opt) {
+        s->split_down = (s->split_down & ~(3u << kPlanCoarseShift)) | (opt << kPlanCoarseShift);
+        float ms = 0.f;
+        rc = time_plan(&ms);
+        if (rc == FX_OK && (round == 0 || ms < t[opt])) t[opt] = ms;
+      }
+    const uint32_t best = t[1] < t[0] ? (t[2] < t[1] ? 2u : 1u) : (t[2] < t[0] ? 2u : 0u);
+    s->split_down = (s->split_down & ~(3u << kPlanCoarseShift)) | (best << kPlanCoarseShift);
   }
   // leave the pressure buffers as the chosen plan computes them
   float ms = 0.f;

@@ -168,9 +168,16 @@ void launch_sweep(hipStream_t s, int blocks, const uint32_t* list, const uint32_
 // launch_sweep calls; src == nullptr starts from p = 0.
 void launch_sweep_pair(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
                        const float* rhs, const float* src, float* dst);
+// Four sweeps in one launch (LDS tiles of radius 4), bit-identical to four
+// launch_sweep calls.
+void launch_sweep_quad(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
+                       const float* rhs, const float* src, float* dst);
 // Bits 8.. of a V-cycle plan mask: the split leg of level l runs its sweeps two
 // per launch (launch_sweep_pair).
 constexpr int kPlanPairShift = 8;
+// Bits 24-25 of the down mask: sweeps per launch of the coarsest level when its
+// active voxels exceed the one-workgroup smoother (0: one, 1: two, 2: four).
+constexpr int kPlanCoarseShift = 24;
 // Bits 16.. of a V-cycle plan mask: the split leg of level l fuses its sweep next to
 // the level change with the restriction (down: k_vdown<1> after pre - 1 sweeps) or
 // with the prolongation (up: k_vup<1> before post - 1 sweeps).

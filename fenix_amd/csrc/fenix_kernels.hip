@@ -2138,19 +2138,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
   }
 }
 
-// Two weighted-Jacobi sweeps of a V-cycle level in one launch, temporally
-// blocked in LDS: tiles of radius 2 around the brick, the first sweep on the
-// brick +- 1, the second on the brick. The next brick's tile loads and
-// neighbourhood words are issued before the current brick's sweeps, so their
-// latency overlaps the LDS work. Every voxel sees the same arithmetic as in two
-// k_sweep launches (same neighbour order, inactive voxels 0), so results are
-// bit-identical. ZERO: the first sweep starts from p = 0.
-template <bool ZERO, bool MS>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_sweep2(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
+// Sweeps A -> B on brick +- R, then B -> A on brick +- (R - 1), ... down to the brick.
+template <int R0, int R>
+__device__ __forceinline__ void tile_sweeps(const uint64_t* nb, float* A, float* B, const float* rhs, float om) {
+  tile_sweep<R0, R>(nb, A, rhs, B, om);
+  __syncthreads();
+  if constexpr (R > 0) tile_sweeps<R0, R - 1>(nb, B, A, rhs, om);
+}
+// NS weighted-Jacobi sweeps of a V-cycle level in one launch, temporally blocked
+// in LDS: tiles of radius NS around the brick, sweep j on the brick +- (NS - j).
+// The next brick's tile loads and neighbourhood words are issued before the
+// current brick's sweeps, so their latency overlaps the LDS work. Every voxel sees
+// the same arithmetic as in NS k_sweep launches (same neighbour order, inactive
+// voxels 0), so results are bit-identical. ZERO: the first sweep starts from p = 0.
+template <int NS, bool ZERO, bool MS>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NS == 2 ? 8 : 4))) void k_sweepn(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
                                                 LvBits lb, const StepConst* __restrict__ C0, const float* __restrict__ rhs,
                                                 const float* __restrict__ src, float* __restrict__ dst) {
   FX_SIM_PRIORITY();
-  constexpr int R0 = 2, S = Tile<R0>::S;
+  constexpr int R0 = NS, S = Tile<R0>::S;
   __shared__ float ta[Tile<R0>::N], tb[Tile<R0>::N], tr[Tile<R0>::N];
   __shared__ uint64_t nb[216];
   const int tid = int(threadIdx.x);
@@ -2187,15 +2193,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     if (k + br.step < br.end) fetch(k + br.step);
     const Grid g = grid_at<MS>(G, cb);
     const float om = C0[g.scene].omega;
-    tile_sweep<R0, 1>(nb, ta, tr, tb, om);
-    __syncthreads();
-    tile_sweep<R0, 0>(nb, tb, tr, ta, om);
-    __syncthreads();
+    tile_sweeps<R0, R0 - 1>(nb, ta, tb, tr, om);
+    const float* res = (NS & 1) ? tb : ta;
     int bx, by, bz;
     brick_coords(g, cb, bx, by, bz);
     const int x = bx * 8 + (tid & 7), y = by * 8 + ((tid >> 3) & 7), z = bz * 8 + (tid >> 6);
     if (x < int(g.W) && y < int(g.H) && z < int(g.D))
-      dst[(cb << 9) | uint32_t(tid)] = ta[(R0 + (tid & 7)) + S * ((R0 + ((tid >> 3) & 7)) + S * (R0 + (tid >> 6)))];
+      dst[(cb << 9) | uint32_t(tid)] = res[(R0 + (tid & 7)) + S * ((R0 + ((tid >> 3) & 7)) + S * (R0 + (tid >> 6)))];
   }
 }
 
@@ -3345,8 +3349,13 @@ void launch_sweep(hipStream_t s, int blocks, const uint32_t* list, const uint32_
 }
 void launch_sweep_pair(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
                        const float* rhs, const float* src, float* dst) {
-  if (src) FX_LAUNCH_MS(g, (k_sweep2<false, true>), (k_sweep2<false, false>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
-  else FX_LAUNCH_MS(g, (k_sweep2<true, true>), (k_sweep2<true, false>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
+  if (src) FX_LAUNCH_MS(g, (k_sweepn<2, false, true>), (k_sweepn<2, false, false>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
+  else FX_LAUNCH_MS(g, (k_sweepn<2, true, true>), (k_sweepn<2, true, false>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
+}
+void launch_sweep_quad(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
+                       const float* rhs, const float* src, float* dst) {
+  if (src) FX_LAUNCH_MS(g, (k_sweepn<4, false, true>), (k_sweepn<4, false, false>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
+  else FX_LAUNCH_MS(g, (k_sweepn<4, true, true>), (k_sweepn<4, true, false>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
 }
 void launch_residual(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const float* rhs,
                      const float* p, float* r, double* norm2) {

@@ -473,12 +473,16 @@ int fx_read_stage_times(fx_sim* sim, double* total_ms, uint32_t* calls, size_t n
  * Bit 16 + l: that split leg fuses the sweep next to the level change into one
  * LDS kernel: the last pre-sweep with the residual and the restriction (down,
  * tiles of halo 3), or the prolongation with the first post-sweep (up, halo 1).
+ * Bits 24-25 of down_split: sweeps per launch of the coarsest level when its
+ * active voxels exceed the one-workgroup smoother's capacity (0: one, 1: two,
+ * 2: four, in LDS tiles of halo 2 or 4).
  * Every plan gives bit-identical results; only speed differs. */
 int fx_set_vcycle_plan(fx_sim* sim, uint32_t down_split, uint32_t up_split);
 int fx_get_vcycle_plan(fx_sim* sim, uint32_t* down_split, uint32_t* up_split);
 
 /* Autotuner: times each level's leg plans (fused, split, split with paired
- * sweeps, for the down and the up leg; greedily, finest level first; the
+ * sweeps, for the down and the up leg; greedily, finest level first; then the
+ * coarsest level's sweeps per launch when it runs multi-workgroup launches; the
  * bit 16 + l legs are not timed, only pinned by fx_set_vcycle_plan)
  * with `repeats` graph-captured pressure solves of the current state and keeps
  * the fastest; returns the chosen masks. Solves start from p = 0 and every plan

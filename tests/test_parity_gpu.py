@@ -551,3 +551,31 @@ def test_coarsest_level_beyond_the_lds_cap():
     assert f3[1:-1, 1:-1, 1:-1].sum() > 8192
     lst = sim.read_active_list()
     np.testing.assert_array_equal(lst, np.flatnonzero(orc.flags()).astype(np.uint32))
+
+
+def test_coarsest_sweeps_per_launch_bit_identical():
+    """Beyond the one-workgroup smoother the coarsest level's 16 sweeps per cycle run as 16, 8 or 4
+    multi-workgroup launches (plan bits 24-25: LDS tiles of halo 1, 2 or 4); the fields are
+    bit-identical after steps on the 70%-active 208^3 plume, and the autotuner keeps them so."""
+    n = 208
+    srcs = scenes.c2_sources(n, count=4, radius_vox=12.0)
+    st = scenes.plume_state_fast(n, srcs, target=0.7, seed=8)
+    out = []
+    for opt in (0, 1, 2):
+        sim = CombustionVolumeSimulation(scenes.c2_desc(n))
+        sim.updateSources(srcs)
+        for k, v in st.items():
+            sim.write_field(k, v)
+        sim.set_vcycle_plan(0x3 | (opt << 24), 0x7)
+        for k in range(3):
+            sim.step(None, k / 60.0, 1 / 60.0)
+        assert sim.vcycle_plan()[0] >> 24 == opt
+        out.append({f: sim.read_field(f) for f in ("pressure", "velocity_x", "velocity_y", "density")})
+        if opt == 2:
+            down, _ = sim.autotune_vcycle(2)
+            assert (down >> 24) in (0, 1, 2)
+            sim.step(None, 3 / 60.0, 1 / 60.0)
+        sim.dispose()
+    for opt in (1, 2):
+        for f in out[0]:
+            np.testing.assert_array_equal(out[opt][f], out[0][f], err_msg=f"plan option {opt}: {f}")

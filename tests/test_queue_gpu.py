@@ -93,6 +93,36 @@ def test_queue_batch_equals_scenes_alone(precision):
         sim.dispose()
 
 
+def test_queue_batch_with_an_empty_scene_and_a_batch_of_one():
+    """Edge cases of the queue: a scene with no emitters and no state (no active bricks at any
+    level, an empty run in every list) beside a busy one, and a batch of a single scene; both
+    equal the scenes simulated alone."""
+    (d0, s0, st0), (d1, _, _) = make_scenes(abi.FX_PRECISION_FP32)[:2]
+    for descs, busy in (([d0, d1], [True, False]), ([d0], [True])):
+        batch = CombustionVolumeQueueBatch(descs)
+        alone = [CombustionVolumeSimulation(d) for d in descs]
+        for sc, sim, b in zip(batch.scenes, alone, busy):
+            if b:
+                sc.updateSources(s0)
+                sim.updateSources(s0)
+                for f, v in st0.items():
+                    sc.write_field(f, v)
+                    sim.write_field(f, v)
+        for k in range(3):
+            batch.step(k * DT, DT)
+            for sim in alone:
+                sim.step(None, k * DT, DT)
+        for j, (sc, sim) in enumerate(zip(batch.scenes, alone)):
+            assert_same(sc, sim, f"{len(descs)} scenes, scene {j}")
+        assert batch.scenes[0].active_brick_count() > 0
+        if len(descs) == 2:
+            assert batch.scenes[1].active_brick_count() == 0
+            assert not batch.scenes[1].read_field("density").any()
+        batch.dispose()
+        for sim in alone:
+            sim.dispose()
+
+
 def test_queue_batch_scene_images_equal_scenes_alone():
     """Each scene of a batch renders from the batch's shared snapshot with its own grid, world
     bounds and brick flags (fx_scene_raymarch): the images equal those of the scenes alone, bit
