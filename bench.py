@@ -66,7 +66,8 @@ RAY_SCALAR_ACCESSES = 3
 # + WRITE_SIZE x 1.0 (the copy's writes read exactly). Summaries with the commands
 # are in profiles/.
 TRAFFIC_PER_VOXEL = {"jacobi_fine": 13.44, "advect_scalars": 63.5}
-PRECISIONS = {"fp32": (abi.FX_PRECISION_FP32, 4), "bf16": (abi.FX_PRECISION_BF16, 2), "fp16": (abi.FX_PRECISION_FP16, 2)}
+PRECISIONS = {"fp32": (abi.FX_PRECISION_FP32, 4), "bf16": (abi.FX_PRECISION_BF16, 2), "fp16": (abi.FX_PRECISION_FP16, 2),
+              "u16": (abi.FX_PRECISION_U16, 2), "uf16": (abi.FX_PRECISION_UF16, 2)}
 
 
 class HipBuffer:
@@ -314,8 +315,8 @@ def run_batch(args):
 def run_c4(args):
     """C4 precision at full size: sparse 512^3, 4 floor emitters (seed 4) under a
     seeded developed plume of ~8% of the bricks, 100 steps (or --steps) in config
-    mode, once per storage precision (FP32, BF16, FP16 for density, temperature and
-    fuel; FP32 arithmetic). Reports each run's step time and throughput, and the
+    mode, once per storage format of density, temperature and fuel (FP32, BF16, FP16,
+    U16 linear fixed point and UF16 unsigned 4/12-bit float; FP32 arithmetic). Reports each run's step time and throughput, and the
     16-bit runs' relative L2 and max error against the FP32 run after the last
     step. The FP64 oracle cannot run 512^3, so accuracy against FP64 is pinned at
     64^3 by the precision tests, where FP32 stays within ~4e-6 of FP64."""
@@ -328,7 +329,7 @@ def run_c4(args):
     fields = ("density", "temperature", "fuel", "reaction", "velocity_x", "velocity_y", "velocity_z")
     dt = 1.0 / 60.0
     runs, ref = {}, None
-    for name in ("fp32", "bf16", "fp16"):
+    for name in ("fp32", "bf16", "fp16", "u16", "uf16"):
         sim = CombustionVolumeSimulation(scenes.c2_desc(n, precision=PRECISIONS[name][0]))
         sim.updateSources(srcs)
         for k, v in state.items():

@@ -31,6 +31,10 @@ FX_PRECISION_BF16 = 1
 FX_PRECISION_FP16 = 2
 FX_PRECISION_BF16_RNE = 3
 FX_PRECISION_FP16_RNE = 4
+FX_PRECISION_U16 = 5
+FX_PRECISION_U16_RNE = 6
+FX_PRECISION_UF16 = 7
+FX_PRECISION_UF16_RNE = 8
 
 FIELDS = {
     "density": 0, "temperature": 1, "fuel": 2, "reaction": 3,

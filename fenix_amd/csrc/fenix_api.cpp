@@ -61,7 +61,7 @@ struct fx_sim {
   int cus = 1;             // compute units of the device
   bool blocks_env = false; // FENIX_SIM_BLOCKS_PER_CU set
   bool sparse = false;
-  bool stochastic = false; // 16-bit stores round stochastically (FX_PRECISION_BF16/FP16)
+  bool stochastic = false; // 16-bit stores round stochastically (FX_PRECISION_BF16/FP16/U16)
 
   void* S[2][4] = {};               // d, t, f in the storage precision; r FP32
   float* u[3] = {};
@@ -788,6 +788,36 @@ uint16_t f32_to_f16(float f) {
   return uint16_t(sign | half);
 }
 
+// 16-bit unsigned fixed point: value = u * 2^-16, clamped to [0, 1 - 2^-16].
+float u16_to_f32(uint16_t h) { return float(h) * (1.f / 65536.f); }
+uint16_t f32_to_u16(float f) {
+  const float c = f > 0.f ? (f < 1.f ? f : 1.f) : 0.f;  // NaN -> 0
+  return uint16_t(std::fmin(std::nearbyint(c * 65536.f), 65535.f));
+}
+// Unsigned float, 4 exponent / 12 mantissa bits: codes >= 4096 are the f32 bits
+// (c + 0x70000) << 11, smaller codes c * 2^-26; values clamped to [0, 1].
+float uf16_to_f32(uint16_t h) {
+  if (h < 4096u) return float(h) * (1.f / 67108864.f);
+  const uint32_t u = (uint32_t(h) + 0x70000u) << 11;
+  float f; std::memcpy(&f, &u, 4); return f;
+}
+uint16_t f32_to_uf16(float f) {
+  const float c = f > 0.f ? (f < 1.f ? f : 1.f) : 0.f;  // NaN -> 0
+  uint32_t b; std::memcpy(&b, &c, 4);
+  if (b < 0x38800000u) return uint16_t(std::nearbyint(c * 67108864.f));
+  b += 0x3ffu + ((b >> 11) & 1u);
+  return uint16_t((b >> 11) - 0x70000u);
+}
+// The 16-bit storage format of a simulation, decoded / encoded.
+float load16(int prec, uint16_t h) {
+  return prec == FX_PRECISION_BF16 ? bf16_to_f32(h) : prec == FX_PRECISION_U16 ? u16_to_f32(h)
+       : prec == FX_PRECISION_UF16 ? uf16_to_f32(h) : f16_to_f32(h);
+}
+uint16_t store16(int prec, float f) {
+  return prec == FX_PRECISION_BF16 ? f32_to_bf16(f) : prec == FX_PRECISION_U16 ? f32_to_u16(f)
+       : prec == FX_PRECISION_UF16 ? f32_to_uf16(f) : f32_to_f16(f);
+}
+
 int read_brickmajor_typed(fx_sim* s, const Grid& g, const void* dev, int esz, float* host, size_t bytes) {
   size_t nv = size_t(g.W) * g.H * g.D;
   if (bytes != nv * 4) return fail(s, FX_ERR_INVALID_ARG, "buffer size mismatch");
@@ -799,8 +829,8 @@ int read_brickmajor_typed(fx_sim* s, const Grid& g, const void* dev, int esz, fl
   } else {
     std::vector<uint16_t> h(n);
     HIP_OK(s, hipMemcpy(h.data(), dev, n * 2, hipMemcpyDeviceToHost));
-    const bool bf = s->desc.precision == FX_PRECISION_BF16;
-    for (size_t i = 0; i < n; ++i) tmp[i] = bf ? bf16_to_f32(h[i]) : f16_to_f32(h[i]);
+    const int prec = s->desc.precision;
+    for (size_t i = 0; i < n; ++i) tmp[i] = load16(prec, h[i]);
   }
   gather_rowmajor(g, tmp.data(), host);
   return FX_OK;
@@ -985,7 +1015,7 @@ int check_desc(const fx_create_desc* desc) {
     return fail(nullptr, FX_ERR_INVALID_ARG, "resolution must be at least 4 in every axis");
   if (uint64_t(desc->width) * desc->height * desc->depth > (1ull << 31))
     return fail(nullptr, FX_ERR_INVALID_ARG, "resolution exceeds 2^31 voxels");
-  if (desc->precision < FX_PRECISION_FP32 || desc->precision > FX_PRECISION_FP16_RNE)
+  if (desc->precision < FX_PRECISION_FP32 || desc->precision > FX_PRECISION_UF16_RNE)
     return fail(nullptr, FX_ERR_INVALID_ARG, "unknown precision");
   if (desc->solver == FX_SOLVER_VCYCLE && (desc->vcycle.levels < 2 || desc->vcycle.levels > FX_MAX_LEVELS_DEV))
     return fail(nullptr, FX_ERR_INVALID_ARG, "vcycle.levels must be in [2, 8]");
@@ -1030,9 +1060,12 @@ int create_impl(const fx_create_desc* descs, uint32_t nsc, bool multi, fx_sim** 
   s->multi = multi;
   if (multi) s->desc.debug_fields = 0;  // a batch records no debug pass
   // storage type in desc.precision; the rounding mode apart
-  s->stochastic = desc->precision == FX_PRECISION_BF16 || desc->precision == FX_PRECISION_FP16;
+  s->stochastic = desc->precision == FX_PRECISION_BF16 || desc->precision == FX_PRECISION_FP16 ||
+                  desc->precision == FX_PRECISION_U16 || desc->precision == FX_PRECISION_UF16;
   if (desc->precision == FX_PRECISION_BF16_RNE) s->desc.precision = FX_PRECISION_BF16;
   if (desc->precision == FX_PRECISION_FP16_RNE) s->desc.precision = FX_PRECISION_FP16;
+  if (desc->precision == FX_PRECISION_U16_RNE) s->desc.precision = FX_PRECISION_U16;
+  if (desc->precision == FX_PRECISION_UF16_RNE) s->desc.precision = FX_PRECISION_UF16;
   s->device = desc->device;
   s->sparse = desc->domain_mode == FX_DOMAIN_SPARSE;
   s->rt = desc->runtime;
@@ -1578,9 +1611,9 @@ int fx_write_field(fx_sim* s, int32_t id, const float* host, size_t bytes) {
   if (esz == 4) {
     HIP_OK(s, hipMemcpy(f, tmp.data(), tmp.size() * 4, hipMemcpyHostToDevice));
   } else {
-    const bool bf = s->desc.precision == FX_PRECISION_BF16;
+    const int prec = s->desc.precision;
     std::vector<uint16_t> h(tmp.size());
-    for (size_t i = 0; i < tmp.size(); ++i) h[i] = bf ? f32_to_bf16(tmp[i]) : f32_to_f16(tmp[i]);
+    for (size_t i = 0; i < tmp.size(); ++i) h[i] = store16(prec, tmp[i]);
     HIP_OK(s, hipMemcpy(f, h.data(), h.size() * 2, hipMemcpyHostToDevice));
   }
   if (s->sparse) {
@@ -1789,9 +1822,9 @@ int fx_scene_write_field(fx_sim* s, uint32_t scene, int32_t id, const float* hos
   if (esz == 4) {
     HIP_OK(s, hipMemcpy(dst, tmp.data(), tmp.size() * 4, hipMemcpyHostToDevice));
   } else {
-    const bool bf = s->desc.precision == FX_PRECISION_BF16;
+    const int prec = s->desc.precision;
     std::vector<uint16_t> h(tmp.size());
-    for (size_t i = 0; i < tmp.size(); ++i) h[i] = bf ? f32_to_bf16(tmp[i]) : f32_to_f16(tmp[i]);
+    for (size_t i = 0; i < tmp.size(); ++i) h[i] = store16(prec, tmp[i]);
     HIP_OK(s, hipMemcpy(dst, h.data(), h.size() * 2, hipMemcpyHostToDevice));
   }
   // bricks holding non-zero data join the next activation's candidates

@@ -98,7 +98,8 @@ struct LevelMaskArgs {
 struct Fields;  // host-side view, see fenix_api.cpp
 
 // One scalar set. Density, temperature and fuel use the simulation's storage
-// precision (FX_PRECISION_*: float, bf16 bits or half); reaction is FP32.
+// precision (FX_PRECISION_*: float, bf16 bits, half or 16-bit fixed point);
+// reaction is FP32.
 struct Scalars {
   void* d;
   void* t;

@@ -250,6 +250,83 @@ struct StF16 {
     *reinterpret_cast<uint2*>(p + i) = make_uint2(a | (b << 16), c | (d << 16));
   }
 };
+// 16-bit unsigned fixed point, value = u * 2^-16 (FX_PRECISION_U16): the three
+// scalars are clamped to [0, 1] by every pass, so a uniform step of 2^-16 keeps
+// 16 bits where a float format spends them on exponents. Encoding and decoding
+// are exact scalings by 2^16; stores clamp to [0, 1 - 2^-16] and round to
+// nearest even, or (sr) up with probability equal to the dropped fraction,
+// compared against 24 hash bits.
+__device__ __forceinline__ uint16_t u16_rne(float v) {
+  return uint16_t(fminf(rintf(clamp01(v) * 65536.f), 65535.f));
+}
+__device__ __forceinline__ uint16_t u16_sr(float v, uint32_t h) {
+  const float x = clamp01(v) * 65536.f, fl = floorf(x);
+  const float up = float(h >> 8) < (x - fl) * 16777216.f ? 1.f : 0.f;
+  return uint16_t(fminf(fl + up, 65535.f));
+}
+struct StU16 {
+  using T = uint16_t;
+  static __device__ __forceinline__ float dec(uint32_t u) { return float(u) * 1.52587890625e-05f; }  // 2^-16
+  static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return dec(p[i]); }
+  static __device__ __forceinline__ void st(T* p, uint32_t i, float v) { p[i] = u16_rne(v); }
+  static __device__ __forceinline__ void st_sr(T* p, uint32_t i, float v, uint32_t sr, uint32_t site, uint32_t hi) {
+    p[i] = sr ? u16_sr(v, sr_hash(hi, site)) : u16_rne(v);
+  }
+  static __device__ __forceinline__ float4 ld4(const T* p, uint32_t i) {
+    const uint2 w = *reinterpret_cast<const uint2*>(p + i);
+    return make_float4(dec(w.x & 0xffffu), dec(w.x >> 16), dec(w.y & 0xffffu), dec(w.y >> 16));
+  }
+  static __device__ __forceinline__ void st4(T* p, uint32_t i, float4 v) {
+    *reinterpret_cast<uint2*>(p + i) = make_uint2(uint32_t(u16_rne(v.x)) | (uint32_t(u16_rne(v.y)) << 16),
+                                                  uint32_t(u16_rne(v.z)) | (uint32_t(u16_rne(v.w)) << 16));
+  }
+};
+// Unsigned 16-bit float with 4 exponent and 12 mantissa bits (FX_PRECISION_UF16):
+// the three scalars lie in [0, 1], so the sign bit and the binades above 2 of a
+// half are spent on mantissa instead. Code c >= 4096 is the f32 with bits
+// (c + 0x70000) << 11, i.e. (1 + m/4096) * 2^(E - 15) for E = c >> 12 in 1..15;
+// c < 4096 is c * 2^-26 (steps of 2^-26 below 2^-14). Every value of [0, 1] is
+// held to within 2^-13 relative (a quarter of a half's step); 1.0 is exact.
+// Decoding is one shift and one exact multiply by 2^112: c << 11 is the f32 with
+// exponent field E and mantissa m (a subnormal f32 m * 2^-138 when E = 0), which
+// needs the kernels' IEEE f32 denormal mode (denormals neither flushed on input
+// nor on output, the HIP default).
+__device__ __forceinline__ float uf16_dec(uint32_t c) { return __uint_as_float(c << 11) * 0x1p112f; }
+__device__ __forceinline__ uint16_t uf16_rne(float v) {
+  const float a = clamp01(v);
+  uint32_t b = __float_as_uint(a);
+  if (b < 0x38800000u) return uint16_t(rintf(a * 67108864.f));  // below 2^-14: multiples of 2^-26
+  b += 0x3ffu + ((b >> 11) & 1u);
+  return uint16_t((b >> 11) - 0x70000u);
+}
+// Stochastic: up with probability equal to the dropped fraction, from the low 11
+// hash bits (normal codes) or the top 24 (small values).
+__device__ __forceinline__ uint16_t uf16_sr(float v, uint32_t h) {
+  const float a = clamp01(v);
+  uint32_t b = __float_as_uint(a);
+  if (b < 0x38800000u) {
+    const float x = a * 67108864.f, fl = floorf(x);
+    return uint16_t(uint32_t(fl) + (float(h >> 8) < (x - fl) * 16777216.f ? 1u : 0u));
+  }
+  b += (h & 0x7ffu) < (b & 0x7ffu) ? 0x800u : 0u;
+  return uint16_t((b >> 11) - 0x70000u);
+}
+struct StUF16 {
+  using T = uint16_t;
+  static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return uf16_dec(p[i]); }
+  static __device__ __forceinline__ void st(T* p, uint32_t i, float v) { p[i] = uf16_rne(v); }
+  static __device__ __forceinline__ void st_sr(T* p, uint32_t i, float v, uint32_t sr, uint32_t site, uint32_t hi) {
+    p[i] = sr ? uf16_sr(v, sr_hash(hi, site)) : uf16_rne(v);
+  }
+  static __device__ __forceinline__ float4 ld4(const T* p, uint32_t i) {
+    const uint2 w = *reinterpret_cast<const uint2*>(p + i);
+    return make_float4(uf16_dec(w.x & 0xffffu), uf16_dec(w.x >> 16), uf16_dec(w.y & 0xffffu), uf16_dec(w.y >> 16));
+  }
+  static __device__ __forceinline__ void st4(T* p, uint32_t i, float4 v) {
+    *reinterpret_cast<uint2*>(p + i) = make_uint2(uint32_t(uf16_rne(v.x)) | (uint32_t(uf16_rne(v.y)) << 16),
+                                                  uint32_t(uf16_rne(v.z)) | (uint32_t(uf16_rne(v.w)) << 16));
+  }
+};
 template <class ST>
 struct SS {
   typename ST::T* d;
@@ -3160,6 +3237,8 @@ void launch_inject(hipStream_t s, int blocks, const uint32_t* list, const uint32
   switch (set0.prec) {
     case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_inject<StBF16, true>), (k_inject<StBF16, false>), blocks, 512, s, list, count, g, c, src, typed<StBF16>(set0), u, touched, parts); break;
     case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_inject<StF16, true>), (k_inject<StF16, false>), blocks, 512, s, list, count, g, c, src, typed<StF16>(set0), u, touched, parts); break;
+    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_inject<StU16, true>), (k_inject<StU16, false>), blocks, 512, s, list, count, g, c, src, typed<StU16>(set0), u, touched, parts); break;
+    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_inject<StUF16, true>), (k_inject<StUF16, false>), blocks, 512, s, list, count, g, c, src, typed<StUF16>(set0), u, touched, parts); break;
     default: FX_LAUNCH_MS(g, (k_inject<StF32, true>), (k_inject<StF32, false>), blocks, 512, s, list, count, g, c, src, typed<StF32>(set0), u, touched, parts);
   }
 }
@@ -3185,12 +3264,16 @@ void launch_activate(hipStream_t s, int blocks, const uint32_t* listA, const uin
     switch (set.prec) {
       case FX_PRECISION_BF16: FX_LAUNCH(k_activate_reference<StBF16>, b, 256, s, listA, countA, listB, countB, g, typed<StBF16>(set), flags, lf); break;
       case FX_PRECISION_FP16: FX_LAUNCH(k_activate_reference<StF16>, b, 256, s, listA, countA, listB, countB, g, typed<StF16>(set), flags, lf); break;
+      case FX_PRECISION_U16: FX_LAUNCH(k_activate_reference<StU16>, b, 256, s, listA, countA, listB, countB, g, typed<StU16>(set), flags, lf); break;
+      case FX_PRECISION_UF16: FX_LAUNCH(k_activate_reference<StUF16>, b, 256, s, listA, countA, listB, countB, g, typed<StUF16>(set), flags, lf); break;
       default: FX_LAUNCH(k_activate_reference<StF32>, b, 256, s, listA, countA, listB, countB, g, typed<StF32>(set), flags, lf);
     }
   } else {
     switch (set.prec) {
       case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_activate_threshold<StBF16, true>), (k_activate_threshold<StBF16, false>), blocks, 512, s, listA, countA, listB, countB, g, c, typed<StBF16>(set).d, vel, flags, lf); break;
       case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_activate_threshold<StF16, true>), (k_activate_threshold<StF16, false>), blocks, 512, s, listA, countA, listB, countB, g, c, typed<StF16>(set).d, vel, flags, lf); break;
+      case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_activate_threshold<StU16, true>), (k_activate_threshold<StU16, false>), blocks, 512, s, listA, countA, listB, countB, g, c, typed<StU16>(set).d, vel, flags, lf); break;
+      case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_activate_threshold<StUF16, true>), (k_activate_threshold<StUF16, false>), blocks, 512, s, listA, countA, listB, countB, g, c, typed<StUF16>(set).d, vel, flags, lf); break;
       default: FX_LAUNCH_MS(g, (k_activate_threshold<StF32, true>), (k_activate_threshold<StF32, false>), blocks, 512, s, listA, countA, listB, countB, g, c, typed<StF32>(set).d, vel, flags, lf);
     }
   }
@@ -3208,6 +3291,8 @@ void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, con
   switch (set0.prec) {
     case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_advect_velocity<StBF16, true>), (k_advect_velocity<StBF16, false>), blocks, 512, s, list, count, g, c, u, us, U4, typed<StBF16>(set0), S4); break;
     case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_advect_velocity<StF16, true>), (k_advect_velocity<StF16, false>), blocks, 512, s, list, count, g, c, u, us, U4, typed<StF16>(set0), S4); break;
+    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_advect_velocity<StU16, true>), (k_advect_velocity<StU16, false>), blocks, 512, s, list, count, g, c, u, us, U4, typed<StU16>(set0), S4); break;
+    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_advect_velocity<StUF16, true>), (k_advect_velocity<StUF16, false>), blocks, 512, s, list, count, g, c, u, us, U4, typed<StUF16>(set0), S4); break;
     default: FX_LAUNCH_MS(g, (k_advect_velocity<StF32, true>), (k_advect_velocity<StF32, false>), blocks, 512, s, list, count, g, c, u, us, U4, typed<StF32>(set0), S4);
   }
 }
@@ -3218,6 +3303,8 @@ void launch_pack_gather(hipStream_t s, int blocks, const uint32_t* list, const u
   switch (set0.prec) {
     case FX_PRECISION_BF16: FX_LAUNCH(k_pack_gather<StBF16>, blocks, 512, s, list, count, typed<StBF16>(set0), us, rlist, rcount, S4, U4); break;
     case FX_PRECISION_FP16: FX_LAUNCH(k_pack_gather<StF16>, blocks, 512, s, list, count, typed<StF16>(set0), us, rlist, rcount, S4, U4); break;
+    case FX_PRECISION_U16: FX_LAUNCH(k_pack_gather<StU16>, blocks, 512, s, list, count, typed<StU16>(set0), us, rlist, rcount, S4, U4); break;
+    case FX_PRECISION_UF16: FX_LAUNCH(k_pack_gather<StUF16>, blocks, 512, s, list, count, typed<StUF16>(set0), us, rlist, rcount, S4, U4); break;
     default: FX_LAUNCH(k_pack_gather<StF32>, blocks, 512, s, list, count, typed<StF32>(set0), us, rlist, rcount, S4, U4);
   }
 }
@@ -3228,6 +3315,8 @@ void launch_advect_scalars(hipStream_t s, int blocks, const uint32_t* list, cons
   switch (dst.prec) {
     case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_advect_scalars<StBF16, true>), (k_advect_scalars<StBF16, false>), blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StBF16>(dst), flags); break;
     case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_advect_scalars<StF16, true>), (k_advect_scalars<StF16, false>), blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StF16>(dst), flags); break;
+    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_advect_scalars<StU16, true>), (k_advect_scalars<StU16, false>), blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StU16>(dst), flags); break;
+    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_advect_scalars<StUF16, true>), (k_advect_scalars<StUF16, false>), blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StUF16>(dst), flags); break;
     default: FX_LAUNCH_MS(g, (k_advect_scalars<StF32, true>), (k_advect_scalars<StF32, false>), blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StF32>(dst), flags);
   }
 }
@@ -3236,6 +3325,8 @@ void launch_combustion(hipStream_t s, int blocks, const uint32_t* list, const ui
   switch (src.prec) {
     case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_combustion<StBF16, true>), (k_combustion<StBF16, false>), blocks, 512, s, list, count, g, c, typed<StBF16>(src), typed<StBF16>(dst), flags); break;
     case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_combustion<StF16, true>), (k_combustion<StF16, false>), blocks, 512, s, list, count, g, c, typed<StF16>(src), typed<StF16>(dst), flags); break;
+    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_combustion<StU16, true>), (k_combustion<StU16, false>), blocks, 512, s, list, count, g, c, typed<StU16>(src), typed<StU16>(dst), flags); break;
+    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_combustion<StUF16, true>), (k_combustion<StUF16, false>), blocks, 512, s, list, count, g, c, typed<StUF16>(src), typed<StUF16>(dst), flags); break;
     default: FX_LAUNCH_MS(g, (k_combustion<StF32, true>), (k_combustion<StF32, false>), blocks, 512, s, list, count, g, c, typed<StF32>(src), typed<StF32>(dst), flags);
   }
 }
@@ -3244,6 +3335,8 @@ void launch_buoyancy(hipStream_t s, int blocks, const uint32_t* list, const uint
   switch (set0.prec) {
     case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_buoyancy<StBF16, true>), (k_buoyancy<StBF16, false>), blocks, 512, s, list, count, g, c, typed<StBF16>(set0), us, flags); break;
     case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_buoyancy<StF16, true>), (k_buoyancy<StF16, false>), blocks, 512, s, list, count, g, c, typed<StF16>(set0), us, flags); break;
+    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_buoyancy<StU16, true>), (k_buoyancy<StU16, false>), blocks, 512, s, list, count, g, c, typed<StU16>(set0), us, flags); break;
+    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_buoyancy<StUF16, true>), (k_buoyancy<StUF16, false>), blocks, 512, s, list, count, g, c, typed<StUF16>(set0), us, flags); break;
     default: FX_LAUNCH_MS(g, (k_buoyancy<StF32, true>), (k_buoyancy<StF32, false>), blocks, 512, s, list, count, g, c, typed<StF32>(set0), us, flags);
   }
 }
@@ -3255,6 +3348,8 @@ void launch_confinement(hipStream_t s, int blocks, const uint32_t* list, const u
   switch (set0.prec) {
     case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_confinement<StBF16, true>), (k_confinement<StBF16, false>), blocks, 512, s, list, count, g, c, w, wmag, us, cmag, typed<StBF16>(set0)); break;
     case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_confinement<StF16, true>), (k_confinement<StF16, false>), blocks, 512, s, list, count, g, c, w, wmag, us, cmag, typed<StF16>(set0)); break;
+    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_confinement<StU16, true>), (k_confinement<StU16, false>), blocks, 512, s, list, count, g, c, w, wmag, us, cmag, typed<StU16>(set0)); break;
+    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_confinement<StUF16, true>), (k_confinement<StUF16, false>), blocks, 512, s, list, count, g, c, w, wmag, us, cmag, typed<StUF16>(set0)); break;
     default: FX_LAUNCH_MS(g, (k_confinement<StF32, true>), (k_confinement<StF32, false>), blocks, 512, s, list, count, g, c, w, wmag, us, cmag, typed<StF32>(set0));
   }
 }
@@ -3382,6 +3477,8 @@ void launch_render_pack(hipStream_t s, int blocks, const uint32_t* list, const u
   switch (set0.prec) {
     case FX_PRECISION_BF16: FX_LAUNCH(k_pack_render<StBF16>, blocks, 512, s, list, count, typed<StBF16>(set0), rlist, rcount, RB); break;
     case FX_PRECISION_FP16: FX_LAUNCH(k_pack_render<StF16>, blocks, 512, s, list, count, typed<StF16>(set0), rlist, rcount, RB); break;
+    case FX_PRECISION_U16: FX_LAUNCH(k_pack_render<StU16>, blocks, 512, s, list, count, typed<StU16>(set0), rlist, rcount, RB); break;
+    case FX_PRECISION_UF16: FX_LAUNCH(k_pack_render<StUF16>, blocks, 512, s, list, count, typed<StUF16>(set0), rlist, rcount, RB); break;
     default: FX_LAUNCH(k_pack_render<StF32>, blocks, 512, s, list, count, typed<StF32>(set0), rlist, rcount, RB);
   }
   const uint32_t nwords = (nb + 31) / 32;

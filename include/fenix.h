@@ -68,6 +68,19 @@ extern "C" {
 #define FX_PRECISION_FP16 2
 #define FX_PRECISION_BF16_RNE 3
 #define FX_PRECISION_FP16_RNE 4
+/* 16-bit unsigned fixed point with 16 fraction bits (value = u / 65536, a uniform
+ * step of 2^-16 over [0, 1 - 2^-16]) for these three fields, which every pass
+ * clamps to [0, 1]: stored values are clamped to that range. U16 rounds
+ * stochastically (unbiased to within 2^-24 of a step), U16_RNE to nearest even. */
+#define FX_PRECISION_U16 5
+#define FX_PRECISION_U16_RNE 6
+/* Unsigned 16-bit float, 4 exponent and 12 mantissa bits, for values in [0, 1]
+ * (stored values are clamped to that range): normal codes (1 + m/4096) * 2^(E-15),
+ * E = 1..15, and steps of 2^-26 below 2^-14; every value is held to 2^-13 relative,
+ * a quarter of a half's step, and 1.0 exactly. UF16 rounds stochastically, UF16_RNE
+ * to nearest even. */
+#define FX_PRECISION_UF16 7
+#define FX_PRECISION_UF16_RNE 8
 
 /* Field ids for fx_read_field / fx_write_field / fx_field_device_ptr. */
 #define FX_FIELD_DENSITY 0          /* render set (getRenderBuffers) */

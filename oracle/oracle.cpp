@@ -142,6 +142,56 @@ inline float sr_f16(float f, uint32_t h) {
   return std::copysign(mag, f);
 }
 
+// 16-bit unsigned fixed point (FX_PRECISION_U16): value = u * 2^-16, u in
+// [0, 65535]; the value is clamped to [0, 1] first. Nearest even, or stochastic:
+// up when the top 24 hash bits, read as an integer, are below the dropped
+// fraction scaled by 2^24 (exact in float: the fraction has at most 24 bits).
+inline float round_u16(float f) {
+  const float x = clamp01(f) * 65536.f;
+  return std::fmin(std::nearbyint(x), 65535.f) / 65536.f;
+}
+inline float sr_u16(float f, uint32_t h) {
+  const float x = clamp01(f) * 65536.f, fl = std::floor(x);
+  const float up = float(h >> 8) < (x - fl) * 16777216.f ? 1.f : 0.f;
+  return std::fmin(fl + up, 65535.f) / 65536.f;
+}
+// Unsigned 16-bit float (FX_PRECISION_UF16): 4 exponent bits, 12 mantissa bits,
+// normal values (1 + m/4096) * 2^(E-15) for E = 1..15 and multiples of 2^-26 below
+// 2^-14; the value is clamped to [0, 1] first. Rounding by the dropped 11 low bits
+// of the f32 (normal values) or by the fraction of a 2^-26 step (small values).
+inline float round_uf16(float f) {
+  const float a = clamp01(f);
+  if (a < 6.103515625e-05f) return std::nearbyint(a * 67108864.f) / 67108864.f;
+  uint32_t b;
+  std::memcpy(&b, &a, 4);
+  b += 0x3ffu + ((b >> 11) & 1u);
+  b &= ~0x7ffu;
+  float r;
+  std::memcpy(&r, &b, 4);
+  return r;
+}
+inline float sr_uf16(float f, uint32_t h) {
+  const float a = clamp01(f);
+  if (a < 6.103515625e-05f) {
+    const float x = a * 67108864.f, fl = std::floor(x);
+    return (fl + (float(h >> 8) < (x - fl) * 16777216.f ? 1.f : 0.f)) / 67108864.f;
+  }
+  uint32_t b;
+  std::memcpy(&b, &a, 4);
+  b = (b & ~0x7ffu) + ((h & 0x7ffu) < (b & 0x7ffu) ? 0x800u : 0u);
+  float r;
+  std::memcpy(&r, &b, 4);
+  return r;
+}
+inline float round_storage(int precision, float f) {
+  return precision == FX_PRECISION_BF16 ? round_bf16(f) : precision == FX_PRECISION_U16 ? round_u16(f)
+       : precision == FX_PRECISION_UF16 ? round_uf16(f) : round_f16(f);
+}
+inline float sr_storage(int precision, float f, uint32_t h) {
+  return precision == FX_PRECISION_BF16 ? sr_bf16(f, h) : precision == FX_PRECISION_U16 ? sr_u16(f, h)
+       : precision == FX_PRECISION_UF16 ? sr_uf16(f, h) : sr_f16(f, h);
+}
+
 struct Dims {
   uint32_t W, H, D;
   size_t N() const { return size_t(W) * H * D; }
@@ -400,8 +450,7 @@ template <class T> struct Sim : Base {
   // build is the unrounded reference).
   void round_store(std::vector<T>& a) const {
     if (sizeof(T) != sizeof(float) || desc.precision == FX_PRECISION_FP32) return;
-    const bool bf = desc.precision == FX_PRECISION_BF16;
-    for (auto& v : a) v = T(bf ? round_bf16(float(v)) : round_f16(float(v)));
+    for (auto& v : a) v = T(round_storage(desc.precision, float(v)));
   }
   // The stores of one pass: stochastic rounding with that pass's sites, or nearest even.
   void round_set(int set, bool with_fuel, int pass) {
@@ -411,7 +460,7 @@ template <class T> struct Sim : Base {
       if (with_fuel) round_store(S[set][2]);
       return;
     }
-    const bool bf = desc.precision == FX_PRECISION_BF16;
+    const int prec = desc.precision;
     const uint32_t step = uint32_t(stepIndex);
     for (int f = 0; f < (with_fuel ? 3 : 2); ++f) {
       auto& a = S[set][f];
@@ -423,7 +472,7 @@ template <class T> struct Sim : Base {
             const size_t i = d.idx(x, y, uint32_t(z));
             const uint32_t bm = uint32_t(brickOf(x, y, uint32_t(z)) * 512u + (uint32_t(z) & 7u) * 64u + (y & 7u) * 8u + (x & 7u));
             const uint32_t h = sr_hash(bm, step, site);
-            a[i] = T(bf ? sr_bf16(float(a[i]), h) : sr_f16(float(a[i]), h));
+            a[i] = T(sr_storage(prec, float(a[i]), h));
           }
     }
   }
@@ -432,9 +481,12 @@ template <class T> struct Sim : Base {
 
   explicit Sim(const fx_create_desc& de) : desc(de) {
     // storage type in desc.precision, the rounding mode apart (as libfenix does)
-    stochastic = de.precision == FX_PRECISION_BF16 || de.precision == FX_PRECISION_FP16;
+    stochastic = de.precision == FX_PRECISION_BF16 || de.precision == FX_PRECISION_FP16 || de.precision == FX_PRECISION_U16 ||
+                 de.precision == FX_PRECISION_UF16;
     if (de.precision == FX_PRECISION_BF16_RNE) desc.precision = FX_PRECISION_BF16;
     if (de.precision == FX_PRECISION_FP16_RNE) desc.precision = FX_PRECISION_FP16;
+    if (de.precision == FX_PRECISION_U16_RNE) desc.precision = FX_PRECISION_U16;
+    if (de.precision == FX_PRECISION_UF16_RNE) desc.precision = FX_PRECISION_UF16;
     d = Dims{de.width, de.height, de.depth};
     BX = (d.W + 7) / 8; BY = (d.H + 7) / 8; BZ = (d.D + 7) / 8;  // createSparseBrickLayout.ts:11-26
     NB = size_t(BX) * BY * BZ;
@@ -1621,6 +1673,7 @@ template <class T> int Sim<T>::raymarch(const fx_camera* cam, const fx_render_pa
 //   fn 7 emitterNoise(in[0..2] position, in[3] scale, in[4] seed) -> out[0]
 //   fn 8 / 9 stochastic rounding to bf16 / f16 of in[0] with hash in[1] -> out[0]
 //   fn 10 the rounding hash of (in[0] voxel, in[1] step, in[2] site) -> out[0]
+//   fn 11 / 12 stochastic (hash in[1]) / nearest-even rounding of in[0] to 16-bit fixed point -> out[0]
 namespace orc {
 template <class T> int probe_t(int fn, const double* in, double* out) {
   using orc::V3;
@@ -1638,6 +1691,10 @@ template <class T> int probe_t(int fn, const double* in, double* out) {
     case 8: out[0] = double(orc::sr_bf16(float(in[0]), uint32_t(in[1]))); return 0;
     case 9: out[0] = double(orc::sr_f16(float(in[0]), uint32_t(in[1]))); return 0;
     case 10: out[0] = double(orc::sr_hash(uint32_t(in[0]), uint32_t(in[1]), uint32_t(in[2]))); return 0;
+    case 11: out[0] = double(orc::sr_u16(float(in[0]), uint32_t(in[1]))); return 0;
+    case 12: out[0] = double(orc::round_u16(float(in[0]))); return 0;
+    case 13: out[0] = double(orc::sr_uf16(float(in[0]), uint32_t(in[1]))); return 0;
+    case 14: out[0] = double(orc::round_uf16(float(in[0]))); return 0;
     default: return FX_ERR_INVALID_ARG;
   }
 }

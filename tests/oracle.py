@@ -156,7 +156,8 @@ class Oracle:
 
 # shading helpers of the raymarch (orc_probe)
 (PROBE_PHASE_HG, PROBE_FIRE_PALETTE, PROBE_SMOKE_ALBEDO, PROBE_LIGHT_DIR, PROBE_LIGHT_ATT, PROBE_LIGHT_POWER,
- PROBE_HASH31, PROBE_EMITTER_NOISE, PROBE_SR_BF16, PROBE_SR_F16, PROBE_SR_HASH) = range(11)
+ PROBE_HASH31, PROBE_EMITTER_NOISE, PROBE_SR_BF16, PROBE_SR_F16, PROBE_SR_HASH, PROBE_SR_U16,
+ PROBE_RNE_U16, PROBE_SR_UF16, PROBE_RNE_UF16) = range(15)
 
 
 def probe(fn, *args, bits=64):

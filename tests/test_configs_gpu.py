@@ -3,11 +3,12 @@
 C1: dense 64^3, reference semantics, 100 steps, 256 x 256 x 64 raymarch.
 C2: sparse 256^3 from the seeded rising plume (~15% of the bricks), 1024 x 1024 x 128 raymarch.
 C3: the 64-scene batch (edges 32-192, 5-40% active), every scene against its own oracle.
-C4: sparse 512^3, BF16 and FP16 storage of density/temperature/fuel, ~8% active.
+C4: sparse 512^3, BF16, FP16 and 16-bit fixed-point storage of density/temperature/fuel, ~8% active.
 
 Per-step parity: before each step both sides hold the oracle's state; after it the brick flags
 (and in sparse mode the active list and page table) are bit-exact and the FP32 fields within 1e-5
-relative L2 (16-bit storage: 2e-3 BF16, 3e-4 FP16, against the FP32 oracle that rounds on store).
+relative L2 (16-bit storage: 2e-3 BF16, 3e-4 FP16 and fixed point, against the FP32 oracle that rounds on
+store).
 Free-running drift against the FP64 oracle is written to $FENIX_REPORT_DIR (default reports/).
 """
 import json
@@ -26,7 +27,7 @@ from oracle import Oracle
 pytestmark = pytest.mark.gpu
 
 REL_L2 = 1e-5
-TOL16 = {abi.FX_PRECISION_BF16: 2e-3, abi.FX_PRECISION_FP16: 3e-4}
+TOL16 = {abi.FX_PRECISION_BF16: 2e-3, abi.FX_PRECISION_FP16: 3e-4, abi.FX_PRECISION_U16: 3e-4, abi.FX_PRECISION_UF16: 3e-4}
 PERSISTENT = ("density", "temperature", "fuel", "reaction", "velocity_x", "velocity_y", "velocity_z")
 DT = 1.0 / 60.0
 
@@ -222,7 +223,8 @@ def test_c3_batch_64_scenes_each_against_its_oracle(kind):
 
 
 # --------------------------------------------------------------------- C4 --
-@pytest.mark.parametrize("prec", [abi.FX_PRECISION_BF16, abi.FX_PRECISION_FP16])
+@pytest.mark.parametrize("prec", [abi.FX_PRECISION_BF16, abi.FX_PRECISION_FP16, abi.FX_PRECISION_U16,
+                                  abi.FX_PRECISION_UF16])
 def test_c4_sparse_512_16bit_storage(prec):
     """C4 at its own size: sparse 512^3, 4 floor emitters r = 20 (seed 4), a seeded plume of ~8%
     of the bricks (~21k), density/temperature/fuel stored in 16 bits. One step against the FP32

@@ -561,3 +561,82 @@ def test_stochastic_rounding_is_unbiased_and_brackets_the_value(fmt):
     for v in (0.5, 0.25 + 2.0 ** -9, 0.0):  # representable in both formats
         for h in (0, 1, 0xffffffff, 0x12345678):
             assert probe(fn, v, h, bits=32) == v
+
+
+def test_fixed_point_rounding_pins():
+    """FX_PRECISION_U16 stores u * 2^-16 with u in [0, 65535] after clamping to [0, 1]. The
+    stochastic store rounds up when the top 24 bits of the hash, as an integer, are below the
+    dropped fraction times 2^24: for a value whose fraction of a step is m / 2^k, the hashes
+    j << (32 - k) (j < 2^k) round up exactly for j < m, so their mean is the input exactly
+    (k is kept so that the input fits a float's 24 bits).
+    Nearest-even stores: ties go to the even step; clamping at both ends. Planted mistakes:
+    '<=' for '<' (mean biased by 1/4096 of a step), a dropped clamp, round-half-up ties."""
+    from oracle import PROBE_RNE_U16, PROBE_SR_U16
+    step = 2.0 ** -16
+    for n, m, k in ((0, 1, 12), (7, 3000, 12), (4095, 2048, 12), (12345, 128, 8), (65000, 255, 8), (65534, 17, 8)):
+        v = (n + m / 2 ** k) * step
+        assert float(np.float32(v)) == v
+        out = np.array([probe(PROBE_SR_U16, v, j << (32 - k), bits=32) for j in range(1 << k)])
+        assert set(np.unique(out)) == {n * step, (n + 1) * step}
+        assert int((out > v).sum()) == m
+        assert abs(out.mean() - v) <= 1e-15
+        half = 1 << (k - 1)
+        assert probe(PROBE_RNE_U16, v, bits=32) == (n + (m > half or (m == half and n % 2 == 1))) * step
+    for v in (0.0, 0.5, 12345 * step, 65535 * step):  # representable: unchanged for every hash
+        for h in (0, 1, 0xffffffff, 0x12345678):
+            assert probe(PROBE_SR_U16, v, h, bits=32) == v
+        assert probe(PROBE_RNE_U16, v, bits=32) == v
+    for v, want in ((-0.25, 0.0), (1.0, 65535 * step), (3.5, 65535 * step), (1.0 - step / 4, 65535 * step)):
+        assert probe(PROBE_RNE_U16, v, bits=32) == want
+        assert probe(PROBE_SR_U16, v, 0xffffffff, bits=32) == want
+    assert probe(PROBE_RNE_U16, 2.5 * step, bits=32) == 2 * step      # tie to even
+    assert probe(PROBE_RNE_U16, 3.5 * step, bits=32) == 4 * step
+
+
+def test_unsigned_float16_rounding_pins():
+    """FX_PRECISION_UF16 stores an unsigned float with 4 exponent and 12 mantissa bits: values
+    (1 + m/4096) * 2^(E-15) for E = 1..15, multiples of 2^-26 below 2^-14, clamped to [0, 1].
+    Pinned by construction from that definition (numpy's frexp/ldexp, not the oracle's bit
+    tricks): every input rounds to one of the two neighbours of its representable bracket; over
+    all 2048 values of the 11 low hash bits (normal values) or all top-k hash values (small
+    values, fraction m / 2^k) the mean is the input exactly; nearest even picks the closer
+    neighbour, ties to an even code; representable values, 0 and 1 are unchanged; clamping at
+    both ends. Planted mistakes: '<=' for '<', a mantissa of 11 or 13 bits, a wrong subnormal
+    step or threshold, an exponent off by one."""
+    from oracle import PROBE_RNE_UF16, PROBE_SR_UF16
+
+    def bracket(v):
+        if v < 2.0 ** -14:
+            step = 2.0 ** -26
+        else:
+            e = np.frexp(v)[1] - 1          # v in [2^e, 2^(e+1))
+            step = 2.0 ** (e - 12)
+        lo = np.floor(v / step) * step
+        return lo, lo + step, step
+
+    rng = np.random.default_rng(11)
+    vals = [float(np.float32(x)) for x in rng.uniform(0, 1, 6)] + [float(np.float32(x)) for x in 10 ** rng.uniform(-4.2, -1, 6)]
+    for v in vals:
+        lo, hi, step = bracket(v)
+        out = np.array([probe(PROBE_SR_UF16, v, h, bits=32) for h in range(2048)])
+        assert set(np.unique(out)) <= {lo, hi}, v
+        assert abs(out.mean() - v) <= 1e-12 * v, (v, out.mean())
+        r = probe(PROBE_RNE_UF16, v, bits=32)
+        code_lo = round(lo / step)
+        want = lo if (v - lo < hi - v or (v - lo == hi - v and code_lo % 2 == 0)) else hi
+        assert r == want, (v, r, want)
+    for n, m, k in ((0, 1, 12), (1000, 77, 12), (4095, 2048, 12)):  # below 2^-14: steps of 2^-26
+        v = (n + m / 2 ** k) * 2.0 ** -26
+        out = np.array([probe(PROBE_SR_UF16, v, j << (32 - k), bits=32) for j in range(1 << k)])
+        assert int((out > v).sum()) == m
+        assert abs(out.mean() - v) <= 1e-12 * v
+    for v in (0.0, 1.0, 0.5, 0.75 + 2.0 ** -13, 2.0 ** -14, 3 * 2.0 ** -26, (4096 + 4095) / 4096 * 2.0 ** -2):
+        for h in (0, 1, 0x7ff, 0xffffffff, 0x12345678):
+            assert probe(PROBE_SR_UF16, v, h, bits=32) == v
+        assert probe(PROBE_RNE_UF16, v, bits=32) == v
+    for v, want in ((-0.5, 0.0), (1.5, 1.0), (7.0, 1.0)):
+        assert probe(PROBE_RNE_UF16, v, bits=32) == want
+        assert probe(PROBE_SR_UF16, v, 0xffffffff, bits=32) == want
+    assert probe(PROBE_RNE_UF16, 1.0 - 2.0 ** -15, bits=32) == 1.0            # top binade: step 2^-13, tie -> even
+    assert probe(PROBE_RNE_UF16, 0.5 + 2.0 ** -14, bits=32) == 0.5            # tie between codes 0 and 1 of [0.5, 1)
+    assert probe(PROBE_RNE_UF16, 0.5 + 3 * 2.0 ** -14, bits=32) == 0.5 + 2.0 ** -12  # codes 1 and 2: to 2

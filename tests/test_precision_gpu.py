@@ -1,5 +1,5 @@
-"""Mixed precision: density, temperature and fuel stored in BF16 or FP16 (FP32
-arithmetic, FP32 reaction/velocity/pressure).
+"""Mixed precision: density, temperature and fuel stored in BF16, FP16 or 16-bit
+fixed point (FP32 arithmetic, FP32 reaction/velocity/pressure).
 
 Parity is checked against the FP32 oracle with the same round-to-nearest-even
 on store; accuracy is reported against the FP64 oracle (C4-style scene at a
@@ -21,9 +21,12 @@ pytestmark = pytest.mark.gpu
 
 PERSISTENT = ("density", "temperature", "fuel", "reaction", "velocity_x", "velocity_y", "velocity_z")
 # One 16-bit ulp flip in a handful of voxels moves rel L2 by ~1e-5 (bf16: 2^-8 relative).
+# Fixed point: one step is 2^-16 of the unit range, below a half ulp for values above 1/64.
 TOL = {abi.FX_PRECISION_BF16: 2e-3, abi.FX_PRECISION_FP16: 3e-4, abi.FX_PRECISION_BF16_RNE: 2e-3,
-       abi.FX_PRECISION_FP16_RNE: 3e-4}
-SIXTEEN = [abi.FX_PRECISION_BF16, abi.FX_PRECISION_FP16, abi.FX_PRECISION_BF16_RNE, abi.FX_PRECISION_FP16_RNE]
+       abi.FX_PRECISION_FP16_RNE: 3e-4, abi.FX_PRECISION_U16: 3e-4, abi.FX_PRECISION_U16_RNE: 3e-4,
+       abi.FX_PRECISION_UF16: 3e-4, abi.FX_PRECISION_UF16_RNE: 3e-4}
+SIXTEEN = [abi.FX_PRECISION_BF16, abi.FX_PRECISION_FP16, abi.FX_PRECISION_BF16_RNE, abi.FX_PRECISION_FP16_RNE,
+           abi.FX_PRECISION_U16, abi.FX_PRECISION_U16_RNE, abi.FX_PRECISION_UF16, abi.FX_PRECISION_UF16_RNE]
 
 
 def rel_l2(a, b):
@@ -94,9 +97,9 @@ def test_step_parity_16bit_sparse(prec):
 
 def test_precision_report_against_fp64():
     """C4-style sparse scene (4 seeded emitters, config mode) for 100 steps:
-    per-field relative L2 and max error of FP32 / BF16 / FP16 storage (stochastic
-    rounding, and round-to-nearest for comparison) against the FP64 oracle. FP32 must
-    stay within 1e-3 relative L2."""
+    per-field relative L2 and max error of FP32 / BF16 / FP16 / fixed-point storage (stochastic
+    rounding, and round-to-nearest for comparison) against the FP64 oracle. FP32 and UF16
+    storage must stay within 1e-3 relative L2."""
     n, steps = 64, 100
     srcs = c4_sources(n)
     ref = Oracle(scenes.c2_desc(n), 64)
@@ -107,7 +110,9 @@ def test_precision_report_against_fp64():
     report = {"scene": f"sparse {n}^3, 4 floor emitters (seed 4), config mode, {steps} steps",
               "active_bricks_fp64": ref.active_count(), "fields": {}}
     for label, prec in (("fp32", abi.FX_PRECISION_FP32), ("bf16", abi.FX_PRECISION_BF16), ("fp16", abi.FX_PRECISION_FP16),
-                        ("bf16_rne", abi.FX_PRECISION_BF16_RNE), ("fp16_rne", abi.FX_PRECISION_FP16_RNE)):
+                        ("bf16_rne", abi.FX_PRECISION_BF16_RNE), ("fp16_rne", abi.FX_PRECISION_FP16_RNE),
+                        ("u16", abi.FX_PRECISION_U16), ("u16_rne", abi.FX_PRECISION_U16_RNE),
+                        ("uf16", abi.FX_PRECISION_UF16), ("uf16_rne", abi.FX_PRECISION_UF16_RNE)):
         sim = CombustionVolumeSimulation(scenes.c2_desc(n, precision=prec))
         sim.updateSources(srcs)
         for k in range(steps):
@@ -127,7 +132,38 @@ def test_precision_report_against_fp64():
         json.dump(report, fh, indent=1)
     for f in ("density", "temperature", "fuel", "velocity_y"):
         assert report["fields"]["fp32"][f]["rel_l2"] <= 1e-3, (f, report["fields"]["fp32"][f])
-    for label in ("bf16", "fp16", "bf16_rne", "fp16_rne"):
+    # the unsigned 4/12-bit float with stochastic rounding meets the bound in every field
+    for f in PERSISTENT:
+        assert report["fields"]["uf16"][f]["rel_l2"] <= 1e-3, (f, report["fields"]["uf16"][f])
+    for label in ("bf16", "fp16", "bf16_rne", "fp16_rne", "u16", "u16_rne", "uf16", "uf16_rne"):
         for f in PERSISTENT:
             assert np.isfinite(report["fields"][label][f]["rel_l2"])
             assert report["fields"][label][f]["rel_l2"] < 0.2, (label, f, report["fields"][label][f])
+
+
+def test_uf16_small_codes_decode_exactly_on_device():
+    """UF16 codes below 2^-14 (multiples of 2^-26) decode on the device through an f32 subnormal
+    scaled by 2^112, so they are exact only under IEEE f32 denormals. The threshold rule compares
+    the decoded density with a = 2^-20 (code 64): a brick holding code 65 activates (with its 26
+    neighbours), a brick holding exactly a does not; flushed denormals would decode both as 0.
+    Flags bit-exact against the oracle; the written values read back unchanged."""
+    grid = (32, 32, 32)
+    a = 2.0 ** -20
+    desc = make_desc(grid, precision=abi.FX_PRECISION_UF16, activation_rule=abi.FX_ACTIVATION_THRESHOLD,
+                     activation_density=a, runtime={"worldSize": 32.0})
+    sim = CombustionVolumeSimulation(desc)
+    orc = Oracle(desc, 32)
+    dens = np.zeros(grid[::-1], np.float32)
+    dens[2, 3, 4] = a + 2.0 ** -26        # brick (0, 0, 0)
+    dens[20, 20, 20] = a                  # brick (2, 2, 2)
+    dens[28, 12, 4] = 3 * 2.0 ** -26      # brick (0, 1, 3)
+    for s in (sim, orc):
+        (s.write_field if s is sim else s.write)("density", dens)
+    np.testing.assert_array_equal(sim.read_field("density"), dens)
+    p = scenes.sim_params(time=0.0, dt=1.0 / 60.0, dx=1.0, world=32.0)
+    sim.run_pass("activate_render", p)
+    orc.run_pass("activate_render", p)
+    got = sim.read_flags(abi.FX_FLAGS_INJECTED)
+    np.testing.assert_array_equal(got, orc.flags(abi.FX_FLAGS_INJECTED))
+    on = got.reshape(4, 4, 4) != 0
+    assert on[0, 0, 0] and on[1, 1, 1] and not on[2, 2, 2] and not on[3, 1, 0]
