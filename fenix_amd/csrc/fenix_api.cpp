@@ -110,7 +110,7 @@ struct fx_sim {
   };
   std::vector<Scene> scenes;
   Grid* d_gtab = nullptr;             // [level][scene] grids
-  uint16_t* d_owner = nullptr;        // scene of every brick, level after level
+  uint32_t* d_owner = nullptr;        // scene of every brick, level after level
   uint32_t* d_scene_count = nullptr;  // active fine bricks per scene (end of the last step)
   uint32_t* h_scene_count = nullptr;  // pinned copy
   size_t coarse_nv_max = 0;           // largest coarsest-level storage of one scene
@@ -1156,13 +1156,13 @@ int create_impl(const fx_create_desc* descs, uint32_t nsc, bool multi, fx_sim** 
   if (multi) {
     // device grid tables and brick owners of every level
     std::vector<Grid> gt(size_t(nlev) * nsc);
-    std::vector<uint16_t> own;
+    std::vector<uint32_t> own;
     std::vector<size_t> own_at(size_t(nlev), 0);
     for (int l = 0; l < nlev; ++l) {
       own_at[size_t(l)] = own.size();
       for (uint32_t k = 0; k < nsc; ++k) {
         gt[size_t(l) * nsc + k] = s->scenes[k].g[l];
-        own.insert(own.end(), brick_count(s->scenes[k].g[l]), uint16_t(k));
+        own.insert(own.end(), brick_count(s->scenes[k].g[l]), k);
       }
     }
     rc = dalloc(s, &s->d_gtab, gt.size());
@@ -1171,7 +1171,7 @@ int create_impl(const fx_create_desc* descs, uint32_t nsc, bool multi, fx_sim** 
     if (rc == FX_OK && hipHostMalloc(reinterpret_cast<void**>(&s->h_scene_count), size_t(nsc) * 4, hipHostMallocDefault) != hipSuccess)
       rc = fail(s, FX_ERR_OUT_OF_MEMORY, "pinned count allocation failed");
     if (rc == FX_OK && (hipMemcpy(s->d_gtab, gt.data(), gt.size() * sizeof(Grid), hipMemcpyHostToDevice) != hipSuccess ||
-                        hipMemcpy(s->d_owner, own.data(), own.size() * 2, hipMemcpyHostToDevice) != hipSuccess))
+                        hipMemcpy(s->d_owner, own.data(), own.size() * 4, hipMemcpyHostToDevice) != hipSuccess))
       rc = fail(s, FX_ERR_DEVICE, "grid table upload failed");
     if (rc != FX_OK) return bail(rc);
     for (uint32_t k = 0; k < nsc; ++k) s->h_scene_count[k] = 0;

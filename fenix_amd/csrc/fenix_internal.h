@@ -31,7 +31,7 @@ struct Grid {
   uint32_t scene;         // scene index (its step constants and sources)
   uint32_t span;          // bricks in the level's storage
   const Grid* tab;        // selector: device grid of every scene of the level
-  const uint16_t* owner;  // selector: scene of every brick of the level
+  const uint32_t* owner;  // selector: scene of every brick of the level
 };
 
 inline Grid make_grid(uint32_t W, uint32_t H, uint32_t D) {

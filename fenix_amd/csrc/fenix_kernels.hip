@@ -72,13 +72,32 @@ __device__ __forceinline__ float fractf(float x) { return x - floorf(x); }
 __device__ __forceinline__ uint32_t decu(uint32_t v) { return v == 0u ? 0u : v - 1u; }
 __device__ __forceinline__ uint32_t incu(uint32_t v, uint32_t m) { return min(v + 1u, m); }
 
+// The grid and owner tables of a brick-queue batch, read through the constant
+// address space: the kernels never write them, and only loads the compiler knows
+// to be unclobbered become scalar loads of a wave-uniform address. Through a
+// generic pointer the table reads of a wave-uniform brick were vector loads, and
+// every grid field after them lived in vector registers.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* const_space(const T* p) {
+  return (const __attribute__((address_space(4))) T*)p;
+}
+__device__ __forceinline__ Grid table_grid(const Grid* tab, uint32_t k) {
+  const __attribute__((address_space(4))) Grid* t = const_space(tab) + k;
+  Grid g;
+  g.W = t->W; g.H = t->H; g.D = t->D;
+  g.BX = t->BX; g.BY = t->BY; g.BZ = t->BZ;
+  g.base = t->base; g.scene = t->scene; g.span = t->span;
+  g.tab = t->tab; g.owner = t->owner;
+  return g;
+}
+
 // Grid of brick b's scene. MS (several scenes share the level's storage): G is a
 // selector and the scene's grid comes from its table; otherwise G is the grid of
 // the only scene, whose base is 0.
 template <bool MS>
 __device__ __forceinline__ Grid grid_at(const Grid& G, uint32_t b) {
   if constexpr (MS) {
-    return G.tab[G.owner[b]];
+    return table_grid(G.tab, const_space(G.owner)[b]);
   } else {
     Grid g = G;
     g.base = 0u;
@@ -90,7 +109,7 @@ __device__ __forceinline__ Grid grid_at(const Grid& G, uint32_t b) {
 template <bool MS>
 __device__ __forceinline__ Grid grid_scene(const Grid& G, uint32_t s) {
   if constexpr (MS) {
-    return G.tab[s];
+    return table_grid(G.tab, s);
   } else {
     Grid g = G;
     g.base = 0u;
@@ -98,6 +117,34 @@ __device__ __forceinline__ Grid grid_scene(const Grid& G, uint32_t s) {
     return g;
   }
 }
+// grid_at for a loop over listed bricks. A brick-queue batch lays out the scenes'
+// bricks one scene after another and every brick list is ascending, so the
+// bricks one wave visits change scene rarely. The cache keeps the wave's last
+// scene index and that scene's brick range (three scalar registers): the grid
+// table entry of that scene is loaded together with the brick id (the table is
+// small and stays in the scalar cache), and the per-brick owner table is read only
+// for a brick outside the range. Without the cache every brick waited on the owner
+// load (an L2 access) and then on the table load before its own loads could
+// start. Single scenes fold the selector as grid_at does. `b` must be
+// wave-uniform.
+template <bool MS>
+struct GridCache {
+  uint32_t o = 0u, lo = 0u, n = 0u;
+  __device__ __forceinline__ Grid at(const Grid& G, uint32_t b) {
+    if constexpr (MS) {
+      Grid g = table_grid(G.tab, o);
+      if (b - lo >= n) {
+        o = const_space(G.owner)[b];
+        g = table_grid(G.tab, o);
+        lo = g.base;
+        n = g.BX * g.BY * g.BZ;
+      }
+      return g;
+    } else {
+      return grid_at<false>(G, b);
+    }
+  }
+};
 // A value that is the same in every lane of the wave (brick ids of kernels that
 // give each wave one brick), so that the table loads it feeds are scalar.
 __device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -555,9 +602,10 @@ template <class ST, bool MS>
 __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
                                                 const StepConst* __restrict__ C0, const float* __restrict__ srcs0,
                                                 SS<ST> S, Vec3 u, uint32_t* touched, int parts) {
+  GridCache<MS> gk;
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
-    const Grid g = grid_at<MS>(G, b);
+    const Grid g = gk.at(G, b);
     const StepConst* __restrict__ C = C0 + g.scene;
     const fx_sim_params& P = C->p;
     const uint32_t ns = C->nsrc;
@@ -734,9 +782,10 @@ __global__ __launch_bounds__(512) void k_activate_threshold(const uint32_t* __re
   // 27 neighbour marks run on 27 lanes at once.
   const uint32_t na = *countA, nb = listB ? *countB : 0u;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  GridCache<MS> gk;
   for (uint32_t k = blockIdx.x * 8 + wave; k < na + nb; k += gridDim.x * 8) {
     const uint32_t b = wave_uniform(k < na ? listA[k] : listB[k - na]);
-    const Grid g = grid_at<MS>(G, b);
+    const Grid g = gk.at(G, b);
     const StepConst* __restrict__ C = C0 + g.scene;
     const float dth = C->act_density, sth = C->act_speed;
     const uint32_t lb = b - g.base;
@@ -928,9 +977,10 @@ template <class ST, bool MS>
 __global__ __launch_bounds__(512) void k_advect_velocity(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
                                                          const StepConst* __restrict__ C0, Vec3 u, Vec3 us, float4* __restrict__ uq,
                                                          SS<ST> S, float4* __restrict__ sq) {
+  GridCache<MS> gk;
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
-    const Grid g = grid_at<MS>(G, b);
+    const Grid g = gk.at(G, b);
     const float dt = C0[g.scene].p.delta_time;
     Vox v = vox_of(g, b);
     if (sq) sq[v.i] = make_float4(ST::ld(S.d, v.i), ST::ld(S.t, v.i), ST::ld(S.f, v.i), S.r[v.i]);
@@ -1020,9 +1070,10 @@ template <class ST, bool MS>
 __global__ __launch_bounds__(512) void k_advect_scalars(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
                                                         const StepConst* __restrict__ C0, int mode, const float4* __restrict__ sq,
                                                         const float4* __restrict__ uq, SS<ST> D, const uint32_t* __restrict__ flags) {
+  GridCache<MS> gk;
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
-    const Grid g = grid_at<MS>(G, b);
+    const Grid g = gk.at(G, b);
     const StepConst* __restrict__ C = C0 + g.scene;
     const float dt = C->p.delta_time, time = C->p.time;
     const uint32_t sr = C->sr;
@@ -1075,9 +1126,10 @@ template <class ST, bool MS>
 __global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
                                                     const StepConst* __restrict__ C0, SS<ST> S, SS<ST> D,
                                                     const uint32_t* __restrict__ flags) {
+  GridCache<MS> gk;
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
-    const Grid g = grid_at<MS>(G, b);
+    const Grid g = gk.at(G, b);
     const StepConst* __restrict__ C = C0 + g.scene;
     const float dt = C->p.delta_time, time = C->p.time;
     const float ilo = C->ignition_lo, ihi = C->ignition_hi, rate = C->burn_rate;
@@ -1122,9 +1174,10 @@ __global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__
 template <class ST, bool MS>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_buoyancy(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
                                                   const StepConst* __restrict__ C0, SS<ST> S, Vec3 us, const uint32_t* __restrict__ flags) {
+  GridCache<MS> gk;
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
-    const Grid g = grid_at<MS>(G, b);
+    const Grid g = gk.at(G, b);
     const fx_sim_params& P = C0[g.scene].p;
     Vox v = vox_of(g, b);
     if (!v.in) continue;
@@ -1181,9 +1234,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
 template <bool MS>
 __global__ __launch_bounds__(512) void k_vorticity(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G, Vec3 us,
                                                    Vec3 w, float* wmag) {
+  GridCache<MS> gk;
   FX_ROW_LOOP(list, count) {
     const uint32_t b = wave_uniform(list[fx_k]);
-    const Grid g = grid_at<MS>(G, b);
+    const Grid g = gk.at(G, b);
     const RowCtx R = row_ctx(g, b, fx_t);
     float c[4], yxp[4], yxm[4], zxp[4], zxm[4], xyp[4], xym[4], zyp[4], zym[4], xzp[4], xzm[4], yzp[4], yzm[4];
     row_x<StF32>(g, R, us.y, c, yxp, yxm);
@@ -1211,9 +1265,10 @@ template <class ST, bool MS>
 __global__ __launch_bounds__(512) void k_confinement(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
                                                      const StepConst* __restrict__ C0, Vec3 w, const float* __restrict__ wmag, Vec3 us,
                                                      float* cmag, SS<ST> S) {
+  GridCache<MS> gk;
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
-    const Grid g = grid_at<MS>(G, b);
+    const Grid g = gk.at(G, b);
     const fx_sim_params& P = C0[g.scene].p;
     Vox v = vox_of(g, b);
     if (!v.in) continue;
@@ -1240,9 +1295,10 @@ __global__ __launch_bounds__(512) void k_confinement(const uint32_t* __restrict_
 template <bool MS>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_divergence(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G, Vec3 us,
                                                     float* div) {
+  GridCache<MS> gk;
   FX_ROW_LOOP(list, count) {
     const uint32_t b = wave_uniform(list[fx_k]);
-    const Grid g = grid_at<MS>(G, b);
+    const Grid g = gk.at(G, b);
     const RowCtx R = row_ctx(g, b, fx_t);
     float c[4], xp[4], xm[4], yp[4], ym[4], zp[4], zm[4], out[4];
     row_x<StF32>(g, R, us.x, c, xp, xm);
@@ -1741,9 +1797,10 @@ __global__ __launch_bounds__(512) void k_vdown(const uint32_t* __restrict__ list
   __shared__ uint64_t nb[216];
   __shared__ double part[8];
   double acc = 0.0;
+  GridCache<MS> gk;
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
-    const Grid f = grid_at<MS>(F, b);
+    const Grid f = gk.at(F, b);
     const Grid c = grid_scene<MS>(CG, f.scene);
     const float om = C0[f.scene].omega;
     int bx, by, bz;
@@ -1851,9 +1908,10 @@ __global__ __launch_bounds__(512) void k_vup(const uint32_t* __restrict__ list, 
   constexpr int CR = (POST + 3) / 2, CS = 4 + 2 * CR, CN = CS * CS * CS;
   __shared__ float ta[N], tb[N], tr[N], tc[CN];
   __shared__ uint64_t nb[216];
+  GridCache<MS> gk;
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
-    const Grid f = grid_at<MS>(F, b);
+    const Grid f = gk.at(F, b);
     const Grid c = grid_scene<MS>(CG, f.scene);
     const float om = C0[f.scene].omega;
     const float mcx = float(c.W - 1), mcy = float(c.H - 1), mcz = float(c.D - 1);
@@ -1942,11 +2000,12 @@ __global__ __launch_bounds__(512) void k_prolong_add(const uint32_t* __restrict_
   const uint32_t slot = threadIdx.x >> 7, t = threadIdx.x & 127u;
   const uint32_t h = t & 1u, ly = (t >> 1) & 7u, lz = t >> 4;
   const BrickRange br = brick_range_slots(*count, 4u, slot);
+  GridCache<MS> gk;
   for (uint32_t k = br.k; k < br.end; k += br.step) {
     const uint32_t b = wave_uniform(list[k]);
     const uint32_t act = uint32_t(lb.bits[size_t(k) * 8 + lz] >> ((ly << 3) | (h * 4u))) & 0xFu;
     if (!act) continue;
-    const Grid f = grid_at<MS>(F, b);
+    const Grid f = gk.at(F, b);
     const Grid c = grid_scene<MS>(CG, f.scene);
     const float mcx = float(c.W - 1), mcy = float(c.H - 1), mcz = float(c.D - 1);
     const uint32_t lbk = b - f.base;
@@ -2032,9 +2091,10 @@ __global__ __launch_bounds__(512) void k_residual_lean(const uint32_t* __restric
   const uint32_t rofs = (lz << 6) | (ly << 3) | (h * 4u);
   double acc = 0.0;
   const BrickRange br = brick_range_slots(*count, 4u, slot);
+  GridCache<MS> gk;
   for (uint32_t k = br.k; k < br.end; k += br.step) {
     const uint32_t b = wave_uniform(list[k]);
-    const Grid g = grid_at<MS>(G, b);
+    const Grid g = gk.at(G, b);
     const uint32_t BX = g.BX, BXY = g.BX * g.BY;
     const uint32_t i0 = (b << 9) | rofs;
     const uint32_t act = uint32_t(lb.bits[size_t(k) * 8 + lz] >> ((ly << 3) | (h * 4u))) & 0xFu;
@@ -2076,9 +2136,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   __shared__ float tf[N];
   const uint32_t t = threadIdx.x;
   const float wt[4] = {0.125f, 0.375f, 0.375f, 0.125f};
+  GridCache<MS> gk;
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
-    const Grid c = grid_at<MS>(CG, b);
+    const Grid c = gk.at(CG, b);
     const Grid f = grid_scene<MS>(F, c.scene);
     const uint32_t lb = b - c.base;
     const int cbx = int(lb % c.BX), cby = int((lb / c.BX) % c.BY), cbz = int(lb / (c.BX * c.BY));
@@ -2145,9 +2206,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
   const uint32_t last = G.span - 1u;  // brick ids of unused neighbour rows stay in the allocation
   const BrickRange br = brick_range_slots(*count, 4u, slot);
   const uint32_t rofs = (z0 << 6) | (ly << 3) | x0;
+  GridCache<MS> gk;
   for (uint32_t k = br.k; k < br.end; k += br.step) {
     const uint32_t b = wave_uniform(list[k]);
-    const Grid g = grid_at<MS>(G, b);
+    const Grid g = gk.at(G, b);
     const float om = C0[g.scene].omega;
     const uint32_t BX = g.BX, BXY = g.BX * g.BY;
     const uint32_t i0 = (b << 9) | rofs, i1 = i0 + 64u;
@@ -2241,9 +2303,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NS == 2 ? 8
   TileRegs<R0> rr, rp;
   uint64_t nw = 0;
   uint32_t b = 0;
+  GridCache<MS> gk;
+  Grid gn = G;
   auto fetch = [&](uint32_t k) {
     b = list[k];
-    const Grid g = grid_at<MS>(G, b);
+    gn = gk.at(G, b);
+    const Grid g = gn;
     int bx, by, bz;
     brick_coords(g, b, bx, by, bz);
     const int ox = bx * 8 - R0, oy = by * 8 - R0, oz = bz * 8 - R0;
@@ -2261,6 +2326,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NS == 2 ? 8
   if (br.k < br.end) fetch(br.k);
   for (uint32_t k = br.k; k < br.end; k += br.step) {
     const uint32_t cb = b;
+    const Grid g = gn;
     __syncthreads();  // the previous brick's tiles are no longer read
     if (tid < 216) nb[tid] = nw;
     __syncthreads();
@@ -2268,7 +2334,6 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NS == 2 ? 8
     if (!ZERO) tile_store<R0>(nb, rp, ta); else tile_zero<R0>(ta);
     __syncthreads();
     if (k + br.step < br.end) fetch(k + br.step);
-    const Grid g = grid_at<MS>(G, cb);
     const float om = C0[g.scene].omega;
     tile_sweeps<R0, R0 - 1>(nb, ta, tb, tr, om);
     const float* res = (NS & 1) ? tb : ta;
@@ -2284,9 +2349,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NS == 2 ? 8
 template <bool MS>
 __global__ __launch_bounds__(512) void k_projection(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G, Vec3 us,
                                                     const float* __restrict__ p, Vec3 u) {
+  GridCache<MS> gk;
   FX_ROW_LOOP(list, count) {
     const uint32_t b = wave_uniform(list[fx_k]);
-    const Grid g = grid_at<MS>(G, b);
+    const Grid g = gk.at(G, b);
     const RowCtx R = row_ctx(g, b, fx_t);
     float c[4], xp[4], xm[4], yp[4], ym[4], zp[4], zm[4], ux[4], uy[4], uz[4];
     row_x<StF32>(g, R, p, c, xp, xm);
