@@ -11,6 +11,12 @@ holding ~15% of the bricks active (`c2-cold` starts from an empty volume).
 (default 100) are timed: a "step" is fx_step + the raymarch, with every input
 already resident in HBM.
 
+After the timed window the same scene is stepped `--steps` more times without its
+raymarch (`sim_only`), and the default run then times the batched + mixed-precision
+path, the C3 batch of 64 scenes in one brick-queue simulation with UF16 storage
+(`batched_mixed`, with its ratio to this plain FP32 single-scene path with and
+without the raymarch; `--no-batch` skips it).
+
 Other workloads: `c1` (dense 64^3, reference semantics), `c3` and `--batch K`
 (batched scenes against one at a time), `c4` (sparse 512^3 precision runs),
 `c5` (loss + gradient of the differentiable step).
@@ -109,6 +115,41 @@ def workload(name, precision="fp32"):
         return dict(n=n, desc=scenes.c1_desc(n, precision=prec), sources=scenes.c1_sources(), render=(256, 256, 64), state=None,
                     label=f"dense 64^3 {tag} storage, reference semantics, 256x256x64 raymarch")
     raise SystemExit(f"unknown workload {name}")
+
+
+def batched_mixed(precision="uf16", warmup=10, steps=20):
+    """The batched + mixed-precision path on the C3 batch: its 64 sparse scenes
+    (edges 32-192, 5-40% active) in one brick-queue simulation with density,
+    temperature and fuel stored in `precision`, V-cycle plan autotuned after the
+    first step, simulation only. Returns the timed window's figures."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import scenes
+    specs = scenes.c3_scenes(64)
+    qb = CombustionVolumeQueueBatch([scenes.c2_desc(sp["n"], precision=PRECISIONS[precision][0]) for sp in specs])
+    for sim, sp in zip(qb.scenes, specs):
+        sim.updateSources(sp["sources"])
+        for name, values in scenes.plume_state_fast(sp["n"], sp["sources"], sp["target"], sp["seed"]).items():
+            sim.write_field(name, values)
+    dt = 1.0 / 60.0
+    qb.step(0.0, dt)
+    qb.synchronize()
+    plan = qb.autotune_vcycle(4)
+    for k in range(1, warmup):
+        qb.step(k * dt, dt)
+    qb.synchronize()
+    a0 = qb.active_brick_count()
+    t0 = time.perf_counter()
+    for k in range(warmup, warmup + steps):
+        qb.step(k * dt, dt)
+    qb.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    active = 0.5 * (a0 + qb.active_brick_count())
+    qb.dispose()
+    return {"workload": "C3 batch of 64 sparse scenes, edge 32-192, 5-40% target active, brick queue "
+                        "(one launch per pass over every scene's bricks), simulation only",
+            "storage": precision, "steps": steps, "warmup": warmup, "ms_per_batch_step": ms,
+            "active_bricks": active, "value": active * 512 / (ms * 1e-3),
+            "vcycle_plan": {"down_split": plan[0], "up_split": plan[1], "autotuned": True}}
 
 
 def cpu_baseline(wl, sim, elapsed0, steps):
@@ -459,6 +500,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of this many MiB (a known-size reference for the "
                          "HBM traffic counters)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="c2: skip the batched + mixed-precision comparison run (C3 brick queue, UF16 storage)")
     ap.add_argument("--precision", choices=sorted(PRECISIONS), default="fp32",
                     help="storage of density/temperature/fuel (arithmetic is FP32 throughout)")
     args = ap.parse_args()
@@ -586,12 +629,29 @@ def main():
         src.free()
         dst.free()
         out["copy_probe_bytes"] = nbytes
+    # the same scene stepped without its raymarch (the simulation alone)
+    t_sim = base + args.timing_steps
+    sim.synchronize()
+    a0 = sim.active_brick_count()
+    t0 = time.perf_counter()
+    for k in range(t_sim, t_sim + args.steps):
+        sim.step(None, k * dt, dt)
+    sim.synchronize()
+    ms_sim = (time.perf_counter() - t0) * 1e3 / args.steps
+    sim_value = 0.5 * (a0 + sim.active_brick_count()) * 512 / (ms_sim * 1e-3)
+    out["sim_only"] = {"ms_per_step": ms_sim, "value": sim_value}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(wl, sim, (base + args.timing_steps) * dt, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(wl, sim, (t_sim + args.steps) * dt, args.cpu_steps)
     img.free()
     if canvas:
         canvas.free()
     sim.dispose()
+    if args.workload == "c2" and not args.no_batch and args.precision == "fp32":
+        # batched + mixed precision against this plain FP32 single-scene path
+        bm = batched_mixed()
+        bm["vs_plain"] = bm["value"] / value
+        bm["vs_plain_sim_only"] = bm["value"] / sim_value
+        out["batched_mixed"] = bm
     print(json.dumps(out), flush=True)
 
 

@@ -144,6 +144,15 @@ struct GridCache {
       return grid_at<false>(G, b);
     }
   }
+  // Grid of the last brick's scene on another level (selector S): its table entry
+  // is addressed by the cached scene index, not by a loaded grid's.
+  __device__ __forceinline__ Grid scene(const Grid& S) const {
+    if constexpr (MS) {
+      return table_grid(S.tab, o);
+    } else {
+      return grid_scene<false>(S, 0u);
+    }
+  }
 };
 // A value that is the same in every lane of the wave (brick ids of kernels that
 // give each wave one brick), so that the table loads it feeds are scalar.
@@ -1801,8 +1810,8 @@ __global__ __launch_bounds__(512) void k_vdown(const uint32_t* __restrict__ list
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
     const Grid f = gk.at(F, b);
-    const Grid c = grid_scene<MS>(CG, f.scene);
-    const float om = C0[f.scene].omega;
+    const Grid c = gk.scene(CG);
+    const float om = C0->omega;  // the scenes of a batch share the V-cycle settings
     int bx, by, bz;
     brick_coords(f, b, bx, by, bz);
     const int ox = bx * 8 - R0, oy = by * 8 - R0, oz = bz * 8 - R0;
@@ -1812,7 +1821,7 @@ __global__ __launch_bounds__(512) void k_vdown(const uint32_t* __restrict__ list
       const int t = threadIdx.x;
       const uint32_t cx = uint32_t(bx * 4 + (t & 3)), cy = uint32_t(by * 4 + ((t >> 2) & 3)), cz = uint32_t(bz * 4 + (t >> 4));
       cact = cx < c.W && cy < c.H && cz < c.D && !on_boundary(c, cx, cy, cz) &&
-             lv_active(mc, grid_scene<MS>(mc.fine, f.scene), cx, cy, cz);
+             lv_active(mc, gk.scene(mc.fine), cx, cy, cz);
     }
     TileRegs<R0> rr, rp;
     tile_fetch<R0>(f, rhs, ox, oy, oz, rr);
@@ -1912,8 +1921,8 @@ __global__ __launch_bounds__(512) void k_vup(const uint32_t* __restrict__ list, 
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
     const Grid f = gk.at(F, b);
-    const Grid c = grid_scene<MS>(CG, f.scene);
-    const float om = C0[f.scene].omega;
+    const Grid c = gk.scene(CG);
+    const float om = C0->omega;  // the scenes of a batch share the V-cycle settings
     const float mcx = float(c.W - 1), mcy = float(c.H - 1), mcz = float(c.D - 1);
     int bx, by, bz;
     brick_coords(f, b, bx, by, bz);
@@ -2006,7 +2015,7 @@ __global__ __launch_bounds__(512) void k_prolong_add(const uint32_t* __restrict_
     const uint32_t act = uint32_t(lb.bits[size_t(k) * 8 + lz] >> ((ly << 3) | (h * 4u))) & 0xFu;
     if (!act) continue;
     const Grid f = gk.at(F, b);
-    const Grid c = grid_scene<MS>(CG, f.scene);
+    const Grid c = gk.scene(CG);
     const float mcx = float(c.W - 1), mcy = float(c.H - 1), mcz = float(c.D - 1);
     const uint32_t lbk = b - f.base;
     const uint32_t xb = (lbk % f.BX) * 8 + h * 4u, y = ((lbk / f.BX) % f.BY) * 8 + ly, z = (lbk / (f.BX * f.BY)) * 8 + lz;
@@ -2140,7 +2149,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
     const Grid c = gk.at(CG, b);
-    const Grid f = grid_scene<MS>(F, c.scene);
+    const Grid f = gk.scene(F);
     const uint32_t lb = b - c.base;
     const int cbx = int(lb % c.BX), cby = int((lb / c.BX) % c.BY), cbz = int(lb / (c.BX * c.BY));
     // fine origin of the region: 2 * (coarse brick origin) - 1
@@ -2210,7 +2219,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
   for (uint32_t k = br.k; k < br.end; k += br.step) {
     const uint32_t b = wave_uniform(list[k]);
     const Grid g = gk.at(G, b);
-    const float om = C0[g.scene].omega;
+    const float om = C0->omega;  // the scenes of a batch share the V-cycle settings
     const uint32_t BX = g.BX, BXY = g.BX * g.BY;
     const uint32_t i0 = (b << 9) | rofs, i1 = i0 + 64u;
     uint32_t act = 0;  // bits 0-3: layer z0, bits 4-7: layer z1
@@ -2334,7 +2343,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NS == 2 ? 8
     if (!ZERO) tile_store<R0>(nb, rp, ta); else tile_zero<R0>(ta);
     __syncthreads();
     if (k + br.step < br.end) fetch(k + br.step);
-    const float om = C0[g.scene].omega;
+    const float om = C0->omega;  // the scenes of a batch share the V-cycle settings
     tile_sweeps<R0, R0 - 1>(nb, ta, tb, tr, om);
     const float* res = (NS & 1) ? tb : ta;
     int bx, by, bz;

@@ -2,7 +2,7 @@
 runtime parameters share every level's storage, and each pass of a step is one launch over the
 active bricks of all scenes. Each scene must come out bit for bit as the same scene simulated
 alone (fields, brick flags, active list and page table), in FP32 and with stochastically
-rounded BF16 storage, through several steps, a V-cycle plan change and a reset; per-scene
+rounded BF16 and UF16 storage, through several steps, a V-cycle plan change and a reset; per-scene
 state written through the batch joins the next activation like a single scene's."""
 import ctypes as C
 
@@ -47,7 +47,7 @@ def assert_same(batch_scene, sim, label):
     assert batch_scene.active_brick_count() == sim.active_brick_count()
 
 
-@pytest.mark.parametrize("precision", [abi.FX_PRECISION_FP32, abi.FX_PRECISION_BF16])
+@pytest.mark.parametrize("precision", [abi.FX_PRECISION_FP32, abi.FX_PRECISION_BF16, abi.FX_PRECISION_UF16])
 def test_queue_batch_equals_scenes_alone(precision):
     spec = make_scenes(precision)
     batch = CombustionVolumeQueueBatch([d for d, _, _ in spec])
