@@ -103,6 +103,11 @@ export function createCombustionVolumeSimulation(device: number | unknown, optio
 
 export function packEmitterSources(sources: readonly EmitterSource[]): Float32Array
 
+/** Storage formats of density, temperature and fuel for `native.precision` (FX_PRECISION_*). */
+export const PRECISION: {
+  FP32: 0; BF16: 1; FP16: 2; BF16_RNE: 3; FP16_RNE: 4; U16: 5; U16_RNE: 6; UF16: 7; UF16_RNE: 8
+}
+
 export interface RenderLight { type: 0 | 1; direction: Vec3; position: Vec3; color: Vec3; intensity: number }
 export interface OrbitCameraSnapshot {
   position: Vec3; target: Vec3; forward: Vec3; right: Vec3; up: Vec3; aspect: number; tanHalfFovY: number

@@ -27,6 +27,9 @@ const FIELD = {
 }
 const FLAGS_RENDER = 0
 const DISPLAY_MODES = { temperature: 0, density: 1, fuel: 2 } // volumeDisplayMode.ts
+// Storage of density, temperature and fuel (FX_PRECISION_*), for options.native.precision;
+// UF16 is the 16-bit format that stays within 1e-3 of FP64.
+const PRECISION = { FP32: 0, BF16: 1, FP16: 2, BF16_RNE: 3, FP16_RNE: 4, U16: 5, U16_RNE: 6, UF16: 7, UF16_RNE: 8 }
 
 // Emitter packing, src/engine/simulation/emitters/packEmitterSources.ts:25-121:
 // 32 floats per source in the layout of the EmitterSource struct of the inject shaders.
@@ -165,4 +168,4 @@ function renderVolume (simulation, camera, params, displayMode, width, height, e
     { ...params, displayMode: DISPLAY_MODES[displayMode || 'temperature'], elapsedSeconds }, width, height)
 }
 
-module.exports = { createCombustionVolumeSimulation, packEmitterSources, renderVolume, FLOATS_PER_EMITTER, native }
+module.exports = { createCombustionVolumeSimulation, packEmitterSources, renderVolume, FLOATS_PER_EMITTER, PRECISION, native }
