@@ -12,6 +12,7 @@ import subprocess
 import numpy as np
 import pytest
 
+from fenix_amd import abi
 from fenix_amd.emitters import BurstEmitter, IgniterEmitter, ScalarEmitter, VelocityEmitter, pack_emitter_sources
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -63,7 +64,7 @@ def test_module_exports_the_reference_api_and_packs_like_the_python_host(tmp_pat
         "fs.writeFileSync(process.argv[3], Buffer.from(p.buffer));"
         "console.log(Object.keys(f).sort().join(','))", JS, str(src), str(tmp_path / "packed.bin"))
     assert out.strip().split(",") == sorted(["createCombustionVolumeSimulation", "packEmitterSources", "renderVolume",
-                                             "FLOATS_PER_EMITTER", "native"])
+                                             "FLOATS_PER_EMITTER", "PRECISION", "native"])
     packed = np.fromfile(tmp_path / "packed.bin", dtype=np.float32)
     np.testing.assert_array_equal(packed, pack_emitter_sources(SOURCES))
 
@@ -145,3 +146,11 @@ def test_node_host_matches_the_python_host(tmp_path):
     assert meta["mode"] == "maccormack" and abs(meta["runtime"]["buoyancy"] - 4.1) < 1e-6
     assert meta["quality"]["coarseIterations"] == 12 and meta["active"] == sim.active_brick_count()
     sim.dispose()
+
+
+def test_storage_format_constants_match_the_c_abi():
+    """The JS host's PRECISION table carries the FX_PRECISION_* values of fenix.h."""
+    out = run_node("console.log(JSON.stringify(require(process.argv[1]).PRECISION))", JS)
+    table = json.loads(out.strip().splitlines()[-1])
+    assert table == {name: getattr(abi, f"FX_PRECISION_{name}") for name in
+                     ("FP32", "BF16", "FP16", "BF16_RNE", "FP16_RNE", "U16", "U16_RNE", "UF16", "UF16_RNE")}
