@@ -38,6 +38,7 @@ struct Level {
   uint32_t* flags = nullptr;         // sparse: level activity flags
   BrickList L{};                     // sparse: active list; dense: identity
   uint64_t* bits = nullptr;          // voxel activity words, 8 per listed brick (V-cycle)
+  uint32_t* nbr = nullptr;           // fine level, sparse V-cycle: listed face neighbours per listed brick
   void* tab = nullptr;               // coarsest level: slot table of the block smoother
   uint32_t* tab_n = nullptr;
 };
@@ -269,7 +270,9 @@ LvMask mask_of(const fx_sim* s, int l) {
   if (!s->sparse) return LvMask{nullptr, s->g, 0u};
   return LvMask{s->flagsInj, s->g, uint32_t(l)};
 }
-LvBits bits_of(const fx_sim* s, int l) { return LvBits{s->lv[l].bits, s->sparse ? s->lv[l].L.page : nullptr}; }
+LvBits bits_of(const fx_sim* s, int l) {
+  return LvBits{s->lv[l].bits, s->sparse ? s->lv[l].L.page : nullptr, l == 0 ? s->lv[0].nbr : nullptr};
+}
 
 struct Buf { void* p; int esz; };
 std::vector<Buf> fine_buffers(fx_sim* s) {
@@ -515,6 +518,8 @@ int solve_vcycle(fx_sim* s) {
     const int ncyc = std::min(v.cycles, s->norm_cap);
     A.zero = ncyc > 0 ? reinterpret_cast<uint32_t*>(s->d_norms) : nullptr;
     A.zero_words = size_t(std::max(ncyc, 0)) * kNormLanes * 2;
+    A.nbr0 = s->lv[0].nbr;
+    A.page0 = s->sparse ? s->lv[0].L.page : nullptr;
     // A dense domain's levels list every brick and activity is geometric, so the
     // activity words and the coarsest slot table (whose inactive voxels the
     // sweeps never write) are built once; later solves only clear the norms.
@@ -1232,6 +1237,7 @@ int create_impl(const fx_create_desc* descs, uint32_t nsc, bool multi, fx_sim** 
     if (rc == FX_OK) rc = dalloc(s, &L.L.retire_count, 1);
     if (rc == FX_OK) rc = l == 0 ? FX_OK : dalloc(s, &L.flags, L.nb);
     if (rc == FX_OK && desc->solver == FX_SOLVER_VCYCLE) rc = dalloc(s, &L.bits, L.nb * 8);
+    if (rc == FX_OK && desc->solver == FX_SOLVER_VCYCLE && l == 0 && multi) rc = dalloc(s, &L.nbr, L.nb);  // batches only
     if (rc == FX_OK && desc->solver == FX_SOLVER_VCYCLE && l > 0) {
       // one slot table per scene (only the coarsest level's is used)
       uint8_t* t = nullptr;
@@ -1347,7 +1353,7 @@ void fx_destroy(fx_sim* s) {
   F(s->touched); F(s->d_bits); F(s->d_rqueue); F(s->d_render); F(s->d_sq); F(s->d_uq); F(s->d_bloom); F(s->d_bloom_io); F(s->identity); F(s->identity_count); F(s->d_const); F(s->d_src); F(s->d_norms);
   for (size_t l = 0; l < s->lv.size(); ++l) {
     Level& L = s->lv[l];
-    F(L.div); F(L.p[0]); F(L.p[1]); F(L.bits); F(L.tab); F(L.tab_n); F(L.L.list); F(L.L.count); F(L.L.page); F(L.L.retire); F(L.L.retire_count);
+    F(L.div); F(L.p[0]); F(L.p[1]); F(L.bits); F(L.nbr); F(L.tab); F(L.tab_n); F(L.L.list); F(L.L.count); F(L.L.page); F(L.L.retire); F(L.L.retire_count);
     if (l > 0) F(L.flags);
   }
   for (auto& slot : s->slots) {

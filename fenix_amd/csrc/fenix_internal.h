@@ -81,6 +81,11 @@ struct LvMask {
 struct LvBits {
   const uint64_t* bits;
   const int32_t* page;
+  // Fine level of a brick-queue batch: per list position, bit d set when the face
+  // neighbour brick d (+x, -x, +y, -y, +z, -z) of the brick is listed; an
+  // unlisted brick holds zeros, so the fine sweep reads its faces as 0 without
+  // loading them. nullptr: every neighbour is loaded.
+  const uint32_t* nbr;
 };
 struct LevelMaskArgs {
   int nlevels;
@@ -91,6 +96,8 @@ struct LevelMaskArgs {
   uint64_t* bits[FX_MAX_LEVELS_DEV];
   uint32_t* zero;  // zero_words words cleared by workgroup 0 (the solve's FP64 norm accumulators)
   size_t zero_words;
+  uint32_t* nbr0;        // LvBits::nbr of the fine level (nullptr: not built)
+  const int32_t* page0;  // the fine level's page table
 };
 
 // Launch helpers implemented in fenix_kernels.hip. All are asynchronous on `s`.
