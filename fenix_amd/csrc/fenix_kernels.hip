@@ -2069,21 +2069,31 @@ __global__ __launch_bounds__(512) void k_prolong_add(const uint32_t* __restrict_
 struct RowNbrs {
   float c[4], xp[4], xm[4], yp[4], ym[4], zp[4], zm[4];
 };
+// nb6: the listed face neighbours of the brick (LvBits::nbr bits); a row in an
+// unlisted neighbour reads the brick's own row (in cache) and is taken as 0.
 __device__ __forceinline__ void row_load(const float* __restrict__ f, uint32_t b, uint32_t h, uint32_t ly, uint32_t lz,
-                                         uint32_t BX, uint32_t BXY, uint32_t last, RowNbrs& R) {
+                                         uint32_t BX, uint32_t BXY, uint32_t last, RowNbrs& R, uint32_t nb6 = 0x3fu) {
   const uint32_t x0 = h * 4u, i0 = (b << 9) | (lz << 6) | (ly << 3) | x0;
-  const uint32_t iyp = ly < 7u ? i0 + 8u : ((min(b + BX, last) << 9) | (lz << 6) | x0);
-  const uint32_t iym = ly > 0u ? i0 - 8u : (((b >= BX ? b - BX : 0u) << 9) | (lz << 6) | (7u << 3) | x0);
-  const uint32_t izp = lz < 7u ? i0 + 64u : ((min(b + BXY, last) << 9) | (ly << 3) | x0);
-  const uint32_t izm = lz > 0u ? i0 - 64u : (((b >= BXY ? b - BXY : 0u) << 9) | (7u << 6) | (ly << 3) | x0);
-  const uint32_t iedge = h ? ((min(b + 1u, last) << 9) | (lz << 6) | (ly << 3))
-                           : (((b > 0u ? b - 1u : 0u) << 9) | (lz << 6) | (ly << 3) | 7u);
+  const bool typ = ly < 7u || (nb6 & 4u), tym = ly > 0u || (nb6 & 8u), tzp = lz < 7u || (nb6 & 16u),
+             tzm = lz > 0u || (nb6 & 32u), te = h ? (nb6 & 1u) : (nb6 & 2u);
+  const uint32_t iyp = !typ ? i0 : ly < 7u ? i0 + 8u : ((min(b + BX, last) << 9) | (lz << 6) | x0);
+  const uint32_t iym = !tym ? i0 : ly > 0u ? i0 - 8u : (((b >= BX ? b - BX : 0u) << 9) | (lz << 6) | (7u << 3) | x0);
+  const uint32_t izp = !tzp ? i0 : lz < 7u ? i0 + 64u : ((min(b + BXY, last) << 9) | (ly << 3) | x0);
+  const uint32_t izm = !tzm ? i0 : lz > 0u ? i0 - 64u : (((b >= BXY ? b - BXY : 0u) << 9) | (7u << 6) | (ly << 3) | x0);
+  const uint32_t iedge = !te ? i0 : h ? ((min(b + 1u, last) << 9) | (lz << 6) | (ly << 3))
+                                      : (((b > 0u ? b - 1u : 0u) << 9) | (lz << 6) | (ly << 3) | 7u);
+  const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
   const float4 c = *reinterpret_cast<const float4*>(f + i0);
-  const float4 yp = *reinterpret_cast<const float4*>(f + iyp);
-  const float4 ym = *reinterpret_cast<const float4*>(f + iym);
-  const float4 zp = *reinterpret_cast<const float4*>(f + izp);
-  const float4 zm = *reinterpret_cast<const float4*>(f + izm);
-  const float edge = f[iedge];
+  float4 yp = *reinterpret_cast<const float4*>(f + iyp);
+  float4 ym = *reinterpret_cast<const float4*>(f + iym);
+  float4 zp = *reinterpret_cast<const float4*>(f + izp);
+  float4 zm = *reinterpret_cast<const float4*>(f + izm);
+  float edge = f[iedge];
+  if (!typ) yp = z4;
+  if (!tym) ym = z4;
+  if (!tzp) zp = z4;
+  if (!tzm) zm = z4;
+  if (!te) edge = 0.f;
   const float other = __shfl_xor(h ? c.x : c.w, 1, 64);
   R.c[0] = c.x; R.c[1] = c.y; R.c[2] = c.z; R.c[3] = c.w;
   R.xp[0] = c.y; R.xp[1] = c.z; R.xp[2] = c.w; R.xp[3] = h ? edge : other;
@@ -2120,7 +2130,7 @@ __global__ __launch_bounds__(512) void k_residual_lean(const uint32_t* __restric
     const uint32_t act = uint32_t(lb.bits[size_t(k) * 8 + lz] >> ((ly << 3) | (h * 4u))) & 0xFu;
     const float4 rh = *reinterpret_cast<const float4*>(rhs + i0);
     RowNbrs R;
-    row_load(p, b, h, ly, lz, BX, BXY, last, R);
+    row_load(p, b, h, ly, lz, BX, BXY, last, R, (MS && lb.nbr) ? lb.nbr[k] : 0x3fu);
     const float rs[4] = {rh.x, rh.y, rh.z, rh.w};
     float out[4];
 #pragma unroll
