@@ -1237,7 +1237,7 @@ int create_impl(const fx_create_desc* descs, uint32_t nsc, bool multi, fx_sim** 
     if (rc == FX_OK) rc = dalloc(s, &L.L.retire_count, 1);
     if (rc == FX_OK) rc = l == 0 ? FX_OK : dalloc(s, &L.flags, L.nb);
     if (rc == FX_OK && desc->solver == FX_SOLVER_VCYCLE) rc = dalloc(s, &L.bits, L.nb * 8);
-    if (rc == FX_OK && desc->solver == FX_SOLVER_VCYCLE && l == 0 && multi) rc = dalloc(s, &L.nbr, L.nb);  // batches only
+    if (rc == FX_OK && desc->solver == FX_SOLVER_VCYCLE && l == 0 && s->sparse) rc = dalloc(s, &L.nbr, L.nb);
     if (rc == FX_OK && desc->solver == FX_SOLVER_VCYCLE && l > 0) {
       // one slot table per scene (only the coarsest level's is used)
       uint8_t* t = nullptr;

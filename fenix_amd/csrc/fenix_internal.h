@@ -81,7 +81,7 @@ struct LvMask {
 struct LvBits {
   const uint64_t* bits;
   const int32_t* page;
-  // Fine level of a brick-queue batch: per list position, bit d set when the face
+  // Fine level of a sparse V-cycle: per list position, bit d set when the face
   // neighbour brick d (+x, -x, +y, -y, +z, -z) of the brick is listed; an
   // unlisted brick holds zeros, so the fine sweep reads its faces as 0 without
   // loading them. nullptr: every neighbour is loaded.

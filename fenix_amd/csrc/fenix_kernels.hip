@@ -2130,7 +2130,7 @@ __global__ __launch_bounds__(512) void k_residual_lean(const uint32_t* __restric
     const uint32_t act = uint32_t(lb.bits[size_t(k) * 8 + lz] >> ((ly << 3) | (h * 4u))) & 0xFu;
     const float4 rh = *reinterpret_cast<const float4*>(rhs + i0);
     RowNbrs R;
-    row_load(p, b, h, ly, lz, BX, BXY, last, R, (MS && lb.nbr) ? lb.nbr[k] : 0x3fu);
+    row_load(p, b, h, ly, lz, BX, BXY, last, R, lb.nbr ? lb.nbr[k] : 0x3fu);
     const float rs[4] = {rh.x, rh.y, rh.z, rh.w};
     float out[4];
 #pragma unroll
@@ -2271,11 +2271,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
       // Rows in an unlisted face neighbour are 0 (the sparse zero invariant). Their
       // lanes load the brick's own row instead, which is in cache, and take 0: the
       // cold zero pages of inactive neighbours stay out of the traffic, and every
-      // load stays one unconditional instruction. Built for brick-queue batches,
-      // whose many small scenes put many listed bricks beside unlisted ones (C3
-      // 6.14 -> 6.03 ms per step); a single scene's sweep is as fast or faster
-      // without the extra word and selects (256^3 plume 8.6 -> 8.7 us).
-      const uint32_t nb6 = (FINE && MS && lb.nbr) ? lb.nbr[k] : 0x3fu;
+      // load stays one unconditional instruction (a branch around the loads made
+      // the sweep 50% slower).
+      const uint32_t nb6 = (FINE && lb.nbr) ? lb.nbr[k] : 0x3fu;
       const bool typ = ly < 7u || (nb6 & 4u), tym = ly > 0u || (nb6 & 8u), tzm = z0 > 0u || (nb6 & 32u),
                  tzp = z0 + 1u < 7u || (nb6 & 16u), te = h ? (nb6 & 1u) : (nb6 & 2u);
       const uint32_t jyp = typ ? iyp0 : i0, jym = tym ? iym0 : i0, jzm = tzm ? izm0 : i0, jzp = tzp ? izp1 : i1,
