@@ -169,11 +169,15 @@ struct fx_sim {
   bool rl_pending[kRenderLanes] = {};
   void* d_render = nullptr;      // float4 per voxel
   // Gather mirrors of the scalar advection (float4 per voxel): the scalars of
-  // set 0 and the advected velocity. Inside a step the mirror of the scalars is
-  // packed over the listed bricks and zeroed in the retired ones; outside a step
-  // (fx_run_pass) both mirrors are packed over every brick.
+  // set 0 and the advected velocity. The scalar mirror is written where set 0 is:
+  // by the combustion pass over the listed bricks (for the next step) and by the
+  // injection over its bricks; the retirement zeroes it with the fields. After set
+  // 0 was changed outside fx_step (sq_valid false) the next step repacks it over
+  // every brick first. A pass run alone (fx_run_pass) packs both mirrors over
+  // every brick.
   void* d_sq = nullptr;
   void* d_uq = nullptr;
+  bool sq_valid = false;
   bool dense_tables = false;  // dense V-cycle: level activity words and coarsest table built
   // Bloom: three half-resolution rgba16f images (bright, horizontal and vertical
   // blur) for the last size, and staging for the host-memory variant.
@@ -463,10 +467,10 @@ void pass_inject(fx_sim* s, int parts) {
   StageTimer timer(s, parts == 2 ? FX_PASS_INJECT_VELOCITY : parts == 4 ? FX_PASS_INJECT_IGNITER : FX_PASS_INJECT_SCALAR);
   if (s->sparse)
     launch_inject(s->stream, grid_for(s, s->nb), s->d_cand, &s->d_const->inj_count, s->g, s->d_const, s->d_src, set_of(s, 0),
-                  vec_of(s->u), s->touched, parts);
+                  vec_of(s->u), s->touched, parts, s->in_step ? s->d_sq : nullptr);
   else
     launch_inject(s->stream, grid_for(s, s->nb), s->identity, s->identity_count, s->g, s->d_const, s->d_src, set_of(s, 0),
-                  vec_of(s->u), nullptr, parts);
+                  vec_of(s->u), nullptr, parts, s->in_step ? s->d_sq : nullptr);
 }
 void zero_level_p(fx_sim* s, int l, int which) {
   float* b[1] = {s->lv[l].p[which]};
@@ -661,10 +665,7 @@ void run_stage(fx_sim* s, int pass) {
   const int blocks = grid_for(s, s->nb);
   switch (pass) {
     case FX_PASS_ADVECT_VELOCITY:
-      // inside a step the scalar gather mirror is packed here (retired bricks were
-      // zeroed by the activation pass); outside one, the scalar advection packs it
-      launch_advect_velocity(s->stream, blocks, list, count, s->g, s->d_const, vec_of(s->u), vec_of(s->us), s->d_uq, set_of(s, 0),
-                             s->in_step ? s->d_sq : nullptr);
+      launch_advect_velocity(s->stream, blocks, list, count, s->g, s->d_const, vec_of(s->u), vec_of(s->us), s->d_uq);
       break;
     case FX_PASS_ADVECT_SCALARS: {
       if (!s->in_step)  // a pass run alone: both mirrors over every brick
@@ -674,7 +675,9 @@ void run_stage(fx_sim* s, int pass) {
       break;
     }
     case FX_PASS_COMBUSTION:
-      launch_combustion(s->stream, blocks, list, count, s->g, s->d_const, set_of(s, 1), set_of(s, 0), s->flagsRender);
+      // inside a step it also writes set 0's scalar gather mirror for the next step
+      launch_combustion(s->stream, blocks, list, count, s->g, s->d_const, set_of(s, 1), set_of(s, 0), s->flagsRender,
+                        s->in_step ? s->d_sq : nullptr);
       break;
     case FX_PASS_BUOYANCY:
       launch_buoyancy(s->stream, blocks, list, count, s->g, s->d_const, set_of(s, 0), vec_of(s->us), s->flagsRender);
@@ -1385,6 +1388,7 @@ int fx_reset(fx_sim* s) {  // ORCH:206-226
   if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
   HIP_OK(s, hipSetDevice(s->device));
   snapshot_stale(s);
+  s->sq_valid = false;
   s->curSet = 0; s->stepIndex = 0; s->lastPressure = 0; s->initialized = false;
   s->startS = s->lastS;
   s->pending.clear();
@@ -1438,6 +1442,13 @@ int fx_step(fx_sim* s, double elapsed_s, double step_s) {  // ORCH:230-305
   // workgroup; while the snapshots go unrendered no raymarch shares the CUs, and 4
   // per CU are faster (C3 batch 6.61 -> 6.46 ms per step). FENIX_SIM_BLOCKS_PER_CU fixes it.
   if (!s->blocks_env) s->blocks = s->cus * (s->render_wanted ? 3 : 4);
+  if (!s->sq_valid) {
+    // set 0 changed outside fx_step: the scalar gather mirror is repacked over
+    // every brick before the step (its passes keep it from then on)
+    const Vec3 none{nullptr, nullptr, nullptr};
+    launch_pack_gather(s->stream, grid_for(s, s->nb), s->identity, s->identity_count, set_of(s, 0), none, nullptr, nullptr, s->d_sq,
+                       s->d_uq);
+  }
   if (s->use_graph && !s->timing) {
     std::vector<int64_t> key = {vort, pres, dbg, s->mode, pres ? -1 : s->lastPressure, s->q.fine_pre_iterations,
                                 s->q.fine_post_iterations, s->q.mid_pre_iterations, s->q.mid_post_iterations,
@@ -1466,6 +1477,7 @@ int fx_step(fx_sim* s, double elapsed_s, double step_s) {  // ORCH:230-305
   s->curSet = s->curSet == 0 ? 1 : 0;
   s->lastS = elapsed_s;
   s->stepIndex += 1;
+  s->sq_valid = true;
   HIP_OK(s, hipGetLastError());
   return FX_OK;
 }
@@ -1479,6 +1491,7 @@ int fx_run_pass(fx_sim* s, int32_t pass, const fx_sim_params* params) {
   const std::vector<fx_sim_params> pv(1, *params);
   if (s->sparse) build_candidates(s, pv, cand, inj);
   snapshot_stale(s);
+  s->sq_valid = false;
   int rc = upload_step(s, pv, s->sparse ? &cand : nullptr, inj);
   if (rc != FX_OK) return rc;
   update_coarse_plan(s);
@@ -1611,6 +1624,7 @@ int fx_write_field(fx_sim* s, int32_t id, const float* host, size_t bytes) {
   if (bytes != s->nv * 4) return fail(s, FX_ERR_INVALID_ARG, "buffer size mismatch");
   HIP_OK(s, hipSetDevice(s->device));
   snapshot_stale(s);
+  s->sq_valid = false;
   std::vector<float> tmp(storage_count(s->g), 0.f);
   scatter_brickmajor(s->g, host, tmp.data());
   HIP_OK(s, hipStreamSynchronize(s->stream));
@@ -1691,7 +1705,12 @@ int fx_read_page_table(fx_sim* s, int32_t* host, size_t count) {
   return FX_OK;
 }
 
-void* fx_field_device_ptr(fx_sim* s, int32_t id) { return s && !s->multi ? field_ptr(s, id) : nullptr; }
+void* fx_field_device_ptr(fx_sim* s, int32_t id) {
+  if (!s || s->multi) return nullptr;
+  // the caller may write set 0's scalars through the pointer before the next step
+  if (id >= FX_FIELD_DENSITY && id <= FX_FIELD_REACTION) s->sq_valid = false;
+  return field_ptr(s, id);
+}
 
 int fx_read_level(fx_sim* s, int32_t level, int32_t which, float* host, size_t bytes) {
   if (!s || !host) return fail(s, FX_ERR_INVALID_ARG, "null argument");
@@ -1820,6 +1839,8 @@ int fx_scene_write_field(fx_sim* s, uint32_t scene, int32_t id, const float* hos
   if (!f) return fail(s, FX_ERR_INVALID_ARG, "unknown field id");
   if (bytes != size_t(g.W) * g.H * g.D * 4) return fail(s, FX_ERR_INVALID_ARG, "buffer size mismatch");
   HIP_OK(s, hipSetDevice(s->device));
+  snapshot_stale(s);
+  s->sq_valid = false;
   const size_t nb = brick_count(g);
   std::vector<float> tmp(nb * 512, 0.f);
   scatter_brickmajor(g, host, tmp.data());

@@ -164,6 +164,25 @@ __device__ __forceinline__ uint32_t vidx(const Grid& g, uint32_t x, uint32_t y, 
 __device__ __forceinline__ uint32_t cidx(const Grid& g, uint32_t x, uint32_t y, uint32_t z) {
   return vidx(g, min(x, g.W - 1), min(y, g.H - 1), min(z, g.D - 1));
 }
+// Brick-major indices of the six face neighbours of the in-domain voxel (x, y, z)
+// at index i, clamped to the domain like cidx(incu(x, W - 1), y, z) and
+// cidx(decu(x), y, z) (a neighbour past a face is the voxel itself): a step of 1,
+// 8 or 64 inside the brick, or into the next brick when the local coordinate is 7
+// (0 going down). Per-axis steps instead of a full cidx per neighbour.
+struct Nbr6 {
+  uint32_t xp, xm, yp, ym, zp, zm;
+};
+__device__ __forceinline__ Nbr6 nbr6(const Grid& g, uint32_t x, uint32_t y, uint32_t z, uint32_t i) {
+  const uint32_t sy = g.BX * 512u - 56u, sz = g.BX * g.BY * 512u - 448u;
+  Nbr6 n;
+  n.xp = x + 1u < g.W ? i + ((x & 7u) < 7u ? 1u : 505u) : i;
+  n.xm = x > 0u ? i - ((x & 7u) > 0u ? 1u : 505u) : i;
+  n.yp = y + 1u < g.H ? i + ((y & 7u) < 7u ? 8u : sy) : i;
+  n.ym = y > 0u ? i - ((y & 7u) > 0u ? 8u : sy) : i;
+  n.zp = z + 1u < g.D ? i + ((z & 7u) < 7u ? 64u : sz) : i;
+  n.zm = z > 0u ? i - ((z & 7u) > 0u ? 64u : sz) : i;
+  return n;
+}
 __device__ __forceinline__ bool on_boundary(const Grid& g, uint32_t x, uint32_t y, uint32_t z) {
   return x == 0 || y == 0 || z == 0 || x == g.W - 1 || y == g.H - 1 || z == g.D - 1;
 }
@@ -258,6 +277,8 @@ __device__ __forceinline__ uint16_t f16_sr(float v, uint32_t h) {
 
 struct StF32 {
   using T = float;
+  static __device__ __forceinline__ T enc_sr(float v, uint32_t, uint32_t, uint32_t) { return v; }
+  static __device__ __forceinline__ float dec(T v) { return v; }
   static __device__ __forceinline__ void st_sr(T* p, uint32_t i, float v, uint32_t, uint32_t, uint32_t) { p[i] = v; }
   static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return p[i]; }
   static __device__ __forceinline__ void st(T* p, uint32_t i, float v) { p[i] = v; }
@@ -271,12 +292,16 @@ struct StBF16 {
     u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite values)
     return uint16_t(u >> 16);
   }
-  static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return __uint_as_float(uint32_t(p[i]) << 16); }
+  static __device__ __forceinline__ float dec(T v) { return __uint_as_float(uint32_t(v) << 16); }
+  static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return dec(p[i]); }
   static __device__ __forceinline__ void st(T* p, uint32_t i, float v) { p[i] = enc(v); }
   // sr != 0: stochastic rounding with the hash term of the store site; hi is the
   // voxel's brick-major index within its scene (i less the scene's first element)
+  static __device__ __forceinline__ T enc_sr(float v, uint32_t sr, uint32_t site, uint32_t hi) {
+    return sr ? bf16_sr(v, sr_hash(hi, site)) : enc(v);
+  }
   static __device__ __forceinline__ void st_sr(T* p, uint32_t i, float v, uint32_t sr, uint32_t site, uint32_t hi) {
-    p[i] = sr ? bf16_sr(v, sr_hash(hi, site)) : enc(v);
+    p[i] = enc_sr(v, sr, site, hi);
   }
   static __device__ __forceinline__ float4 ld4(const T* p, uint32_t i) {
     const uint2 w = *reinterpret_cast<const uint2*>(p + i);
@@ -290,10 +315,14 @@ struct StBF16 {
 };
 struct StF16 {
   using T = __half;
-  static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return __half2float(p[i]); }
+  static __device__ __forceinline__ float dec(T v) { return __half2float(v); }
+  static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return dec(p[i]); }
   static __device__ __forceinline__ void st(T* p, uint32_t i, float v) { p[i] = __float2half(v); }
+  static __device__ __forceinline__ T enc_sr(float v, uint32_t sr, uint32_t site, uint32_t hi) {
+    return sr ? __ushort_as_half(f16_sr(v, sr_hash(hi, site))) : __float2half(v);
+  }
   static __device__ __forceinline__ void st_sr(T* p, uint32_t i, float v, uint32_t sr, uint32_t site, uint32_t hi) {
-    p[i] = sr ? __ushort_as_half(f16_sr(v, sr_hash(hi, site))) : __float2half(v);
+    p[i] = enc_sr(v, sr, site, hi);
   }
   static __device__ __forceinline__ float4 ld4(const T* p, uint32_t i) {
     const uint2 w = *reinterpret_cast<const uint2*>(p + i);
@@ -325,8 +354,11 @@ struct StU16 {
   static __device__ __forceinline__ float dec(uint32_t u) { return float(u) * 1.52587890625e-05f; }  // 2^-16
   static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return dec(p[i]); }
   static __device__ __forceinline__ void st(T* p, uint32_t i, float v) { p[i] = u16_rne(v); }
+  static __device__ __forceinline__ T enc_sr(float v, uint32_t sr, uint32_t site, uint32_t hi) {
+    return sr ? u16_sr(v, sr_hash(hi, site)) : u16_rne(v);
+  }
   static __device__ __forceinline__ void st_sr(T* p, uint32_t i, float v, uint32_t sr, uint32_t site, uint32_t hi) {
-    p[i] = sr ? u16_sr(v, sr_hash(hi, site)) : u16_rne(v);
+    p[i] = enc_sr(v, sr, site, hi);
   }
   static __device__ __forceinline__ float4 ld4(const T* p, uint32_t i) {
     const uint2 w = *reinterpret_cast<const uint2*>(p + i);
@@ -369,10 +401,14 @@ __device__ __forceinline__ uint16_t uf16_sr(float v, uint32_t h) {
 }
 struct StUF16 {
   using T = uint16_t;
-  static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return uf16_dec(p[i]); }
+  static __device__ __forceinline__ float dec(T v) { return uf16_dec(v); }
+  static __device__ __forceinline__ float ld(const T* p, uint32_t i) { return dec(p[i]); }
   static __device__ __forceinline__ void st(T* p, uint32_t i, float v) { p[i] = uf16_rne(v); }
+  static __device__ __forceinline__ T enc_sr(float v, uint32_t sr, uint32_t site, uint32_t hi) {
+    return sr ? uf16_sr(v, sr_hash(hi, site)) : uf16_rne(v);
+  }
   static __device__ __forceinline__ void st_sr(T* p, uint32_t i, float v, uint32_t sr, uint32_t site, uint32_t hi) {
-    p[i] = sr ? uf16_sr(v, sr_hash(hi, site)) : uf16_rne(v);
+    p[i] = enc_sr(v, sr, site, hi);
   }
   static __device__ __forceinline__ float4 ld4(const T* p, uint32_t i) {
     const uint2 w = *reinterpret_cast<const uint2*>(p + i);
@@ -610,7 +646,8 @@ __device__ __noinline__ F3 turbulent_dir(F3 pos, float seed) {  // SH/inject-vel
 template <class ST, bool MS>
 __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
                                                 const StepConst* __restrict__ C0, const float* __restrict__ srcs0,
-                                                SS<ST> S, Vec3 u, uint32_t* touched, int parts) {
+                                                SS<ST> S, Vec3 u, uint32_t* touched, int parts,
+                                                float4* __restrict__ sq) {
   GridCache<MS> gk;
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
@@ -626,6 +663,9 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
     if (!v.in) continue;
     const uint32_t i = v.i, hi = i - (g.base << 9);
     F3 pos = f3((float(v.x) + 0.5f) / float(g.W), (float(v.y) + 0.5f) / float(g.H), (float(v.z) + 0.5f) / float(g.D));
+    // the stored (decoded) density, temperature, fuel and reaction, for the
+    // scalar gather mirror
+    float dd = 0.f, tt = 0.f, ff = 0.f, rr = 0.f;
     if (parts & 1) {
       float den = ST::ld(S.d, i), tem = ST::ld(S.t, i), fu = ST::ld(S.f, i);
       for (uint32_t k = 0; k < ns; ++k) {
@@ -646,7 +686,9 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
         tem = clamp01(tem + s.yt() * contribution);
         fu = clamp01(fu + s.yf() * contribution);
       }
-      ST::st_sr(S.d, i, den, sr, s0, hi); ST::st_sr(S.t, i, tem, sr, s1, hi); ST::st_sr(S.f, i, fu, sr, s2, hi);
+      const typename ST::T ed = ST::enc_sr(den, sr, s0, hi), et = ST::enc_sr(tem, sr, s1, hi), ef = ST::enc_sr(fu, sr, s2, hi);
+      S.d[i] = ed; S.t[i] = et; S.f[i] = ef;
+      dd = ST::dec(ed); tt = ST::dec(et); ff = ST::dec(ef);
     }
     if (parts & 2) {
       F3 vel = f3(u.x[i], u.y[i], u.z[i]);
@@ -687,8 +729,9 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
       u.x[i] = clampf(vel.x, -lim, lim); u.y[i] = clampf(vel.y, -lim, lim); u.z[i] = clampf(vel.z, -lim, lim);
     }
     if (parts & 4) {
-      float rea = S.r[i], tem = ST::ld(S.t, i);
-      const float fuel = ST::ld(S.f, i);
+      // temperature and fuel as the scalar part stored them (the values a reload returns)
+      float rea = S.r[i], tem = (parts & 1) ? tt : ST::ld(S.t, i);
+      const float fuel = (parts & 1) ? ff : ST::ld(S.f, i);
       for (uint32_t k = 0; k < ns; ++k) {
         Src s{srcs + size_t(k) * FX_FLOATS_PER_EMITTER};
         bool burst = s.kind() == 3.f;
@@ -705,8 +748,14 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
         rea = clamp01(rea + seedv);
         tem = clamp01(tem + intensity * weight * P.delta_time * 4.f);
       }
-      S.r[i] = rea; ST::st_sr(S.t, i, tem, sr, s5, hi);
+      const typename ST::T et = ST::enc_sr(tem, sr, s5, hi);
+      S.r[i] = rea; S.t[i] = et;
+      rr = rea; tt = ST::dec(et);
     }
+    // inside a step the injected bricks' scalar gather mirror follows set 0 (the
+    // combustion pass of the previous step wrote it for the listed bricks)
+    if (sq && (parts & 5))
+      sq[i] = make_float4((parts & 1) ? dd : ST::ld(S.d, i), tt, (parts & 1) ? ff : ST::ld(S.f, i), (parts & 4) ? rr : S.r[i]);
   }
 }
 
@@ -982,32 +1031,65 @@ __global__ __launch_bounds__(512) void k_zero_bricks(const uint32_t* __restrict_
 // SH/advect-velocity-semilagrangian.wgsl.ts:23-41
 // sq (optional): also packs the scalar gather mirror (d, T, f, r) of set S for
 // every voxel of the listed bricks, which the scalar advection reads next.
-template <class ST, bool MS>
-__global__ __launch_bounds__(512) void k_advect_velocity(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
-                                                         const StepConst* __restrict__ C0, Vec3 u, Vec3 us, float4* __restrict__ uq,
-                                                         SS<ST> S, float4* __restrict__ sq) {
-  GridCache<MS> gk;
-  FX_BRICK_LOOP(list, count) {
-    const uint32_t b = list[fx_k];
-    const Grid g = gk.at(G, b);
-    const float dt = C0[g.scene].p.delta_time;
-    Vox v = vox_of(g, b);
-    if (sq) sq[v.i] = make_float4(ST::ld(S.d, v.i), ST::ld(S.t, v.i), ST::ld(S.f, v.i), S.r[v.i]);
-    if (!v.in) continue;
+// One brick of the velocity advection, given the voxel's own velocity (loaded
+// one brick ahead by the caller). uq is the float4 gather mirror of the
+// advected velocity that the scalar advection samples.
+__device__ __forceinline__ void advect_u_brick(const Grid& g, uint32_t b, float ux, float uy, float uz, float dt, const Vec3& u,
+                                               const Vec3& us, float4* __restrict__ uq) {
+  const Vox v = vox_of(g, b);
+  if (v.in) {
     if (on_boundary(g, v.x, v.y, v.z)) {
       us.x[v.i] = 0.f; us.y[v.i] = 0.f; us.z[v.i] = 0.f;
       uq[v.i] = make_float4(0.f, 0.f, 0.f, 0.f);
-      continue;
+    } else {
+      F3 coord = f3(float(v.x) + 0.5f, float(v.y) + 0.5f, float(v.z) + 0.5f);
+      F3 half = f3(0.5f, 0.5f, 0.5f);
+      // sampleVelocity(coord - 0.5) lands on the voxel with all fractions 0; the
+      // trilinear blend a*(1-0) + b*0 returns a exactly for finite fields, so the
+      // voxel's own value is read instead of 8 corners per component.
+      F3 back = coord - f3(ux, uy, uz) * dt;
+      F3 adv = sample_u(g, u, back - half) * (1.f - dt * 0.025f);
+      us.x[v.i] = adv.x; us.y[v.i] = adv.y; us.z[v.i] = adv.z;
+      uq[v.i] = make_float4(adv.x, adv.y, adv.z, 0.f);
     }
-    F3 coord = f3(float(v.x) + 0.5f, float(v.y) + 0.5f, float(v.z) + 0.5f);
-    F3 half = f3(0.5f, 0.5f, 0.5f);
-    // sampleVelocity(coord - 0.5) lands on the voxel with all fractions 0; the
-    // trilinear blend a*(1-0) + b*0 returns a exactly for finite fields, so the
-    // voxel's own value is read instead of 8 corners per component.
-    F3 back = coord - f3(u.x[v.i], u.y[v.i], u.z[v.i]) * dt;
-    F3 adv = sample_u(g, u, back - half) * (1.f - dt * 0.025f);
-    us.x[v.i] = adv.x; us.y[v.i] = adv.y; us.z[v.i] = adv.z;
-    uq[v.i] = make_float4(adv.x, adv.y, adv.z, 0.f);
+  }
+}
+template <bool MS>
+__global__ __launch_bounds__(512) void k_advect_velocity(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
+                                                         const StepConst* __restrict__ C0, Vec3 u, Vec3 us, float4* __restrict__ uq) {
+  FX_SIM_PRIORITY();
+  GridCache<MS> gk;
+  const BrickRange br = brick_range(*count);
+  if (br.k >= br.end) return;
+  // The voxel's own velocity is loaded one brick ahead: the backtrace's gathers
+  // depend on it, so loading it beside the previous brick's gathers leaves one
+  // memory round trip per brick instead of two. Two register sets take turns
+  // (the loop is unrolled twice), so the loaded values are never copied, which
+  // would wait for the loads. Every element of a listed brick lies in the
+  // allocation, also outside the domain.
+  const uint32_t t = threadIdx.x, last = br.end - 1u;
+  uint32_t k = br.k, b0 = list[k];
+  float x0 = u.x[(b0 << 9) | t], y0 = u.y[(b0 << 9) | t], z0 = u.z[(b0 << 9) | t];
+  for (;;) {
+    // the next brick (the current one again at the end of the walk: a reread
+    // that hits the cache, with no branch around the loads)
+    const uint32_t k1 = k + br.step;
+    const uint32_t b1 = list[min(k1, last)];
+    const float x1 = u.x[(b1 << 9) | t], y1 = u.y[(b1 << 9) | t], z1 = u.z[(b1 << 9) | t];
+    {
+      const Grid g = gk.at(G, b0);
+      advect_u_brick(g, b0, x0, y0, z0, C0[g.scene].p.delta_time, u, us, uq);
+    }
+    if (k1 >= br.end) break;
+    const uint32_t k2 = k1 + br.step;
+    b0 = list[min(k2, last)];
+    x0 = u.x[(b0 << 9) | t]; y0 = u.y[(b0 << 9) | t]; z0 = u.z[(b0 << 9) | t];
+    {
+      const Grid g = gk.at(G, b1);
+      advect_u_brick(g, b1, x1, y1, z1, C0[g.scene].p.delta_time, u, us, uq);
+    }
+    if (k2 >= br.end) break;
+    k = k2;
   }
 }
 
@@ -1074,50 +1156,78 @@ __global__ __launch_bounds__(512) void k_pack_gather(const uint32_t* __restrict_
 }
 // SH/advect-scalar-maccormack.wgsl.ts:33-101 / SH/advect-scalar-semilagrangian.wgsl.ts:26-44.
 // The four advect<Name> calls share every position; sampleVelocity(coord-0.5)
-// and the `original` sample hit the voxel exactly (fraction 0).
+// and the `original` sample hit the voxel exactly (fraction 0). One brick, given
+// the voxel's own advected velocity uv (loaded one brick ahead by the caller).
+template <class ST>
+__device__ __forceinline__ void advect_s_brick(const Grid& g, uint32_t b, float4 uv, const StepConst* __restrict__ C, int mode,
+                                               const float4* __restrict__ sq, const float4* __restrict__ uq, const SS<ST>& D,
+                                               const uint32_t* __restrict__ flags) {
+  const float dt = C->p.delta_time, time = C->p.time;
+  const uint32_t sr = C->sr;
+  const uint32_t site[3] = {sr_site(C, 8), sr_site(C, 9), sr_site(C, 10)};
+  const Vox v = vox_of(g, b);
+  if (!v.in) return;
+  const uint32_t i = v.i;
+  if (!flag_on(g, flags, v.x, v.y, v.z)) { ST::st(D.d, i, 0.f); ST::st(D.t, i, 0.f); ST::st(D.f, i, 0.f); D.r[i] = 0.f; return; }
+  F3 coord = f3(float(v.x) + 0.5f, float(v.y) + 0.5f, float(v.z) + 0.5f);
+  F3 half = f3(0.5f, 0.5f, 0.5f);
+  F3 velocity = f3(uv.x, uv.y, uv.z);
+  F3 back = coord - velocity * dt;
+  F3 sp = back - half;
+  Tri tp = tri(g, sp);
+  if (mode == FX_ADVECT_SEMI_LAGRANGIAN) {
+    const float4 s4 = tri_eval4(tp, sq);
+    decay_write<ST>(g, i, v.y, s4.x, s4.y, s4.z, s4.w, thermal_noise(sp, time), dt, D, sr, site);
+    return;
+  }
+  const float4 f4 = tri_eval4(tp, uq);
+  F3 fwd = f3(f4.x, f4.y, f4.z);
+  F3 rec = back + fwd * dt;
+  Tri tr = tri(g, rec - half);
+  auto mac = [&](float predicted, float recovered, float original) {
+    float corrected = predicted + 0.5f * (original - recovered);
+    return clampf(corrected, fminf(original, predicted), fmaxf(original, predicted));
+  };
+  const float4 pr = tri_eval4(tp, sq), rc = tri_eval4(tr, sq), og = sq[i];
+  float out[4];
+  out[0] = mac(pr.x, rc.x, og.x);
+  out[1] = mac(pr.y, rc.y, og.y);
+  out[2] = mac(pr.z, rc.z, og.z);
+  out[3] = mac(pr.w, rc.w, og.w);
+  decay_write<ST>(g, i, v.y, out[0], out[1], out[2], out[3], thermal_noise(coord, time), dt, D, sr, site);
+}
 template <class ST, bool MS>
 __global__ __launch_bounds__(512) void k_advect_scalars(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
                                                         const StepConst* __restrict__ C0, int mode, const float4* __restrict__ sq,
                                                         const float4* __restrict__ uq, SS<ST> D, const uint32_t* __restrict__ flags) {
+  FX_SIM_PRIORITY();
   GridCache<MS> gk;
-  FX_BRICK_LOOP(list, count) {
-    const uint32_t b = list[fx_k];
-    const Grid g = gk.at(G, b);
-    const StepConst* __restrict__ C = C0 + g.scene;
-    const float dt = C->p.delta_time, time = C->p.time;
-    const uint32_t sr = C->sr;
-    const uint32_t site[3] = {sr_site(C, 8), sr_site(C, 9), sr_site(C, 10)};
-    Vox v = vox_of(g, b);
-    if (!v.in) continue;
-    const uint32_t i = v.i;
-    if (!flag_on(g, flags, v.x, v.y, v.z)) { ST::st(D.d, i, 0.f); ST::st(D.t, i, 0.f); ST::st(D.f, i, 0.f); D.r[i] = 0.f; continue; }
-    F3 coord = f3(float(v.x) + 0.5f, float(v.y) + 0.5f, float(v.z) + 0.5f);
-    F3 half = f3(0.5f, 0.5f, 0.5f);
-    const float4 uv = uq[i];
-    F3 velocity = f3(uv.x, uv.y, uv.z);
-    F3 back = coord - velocity * dt;
-    F3 sp = back - half;
-    Tri tp = tri(g, sp);
-    if (mode == FX_ADVECT_SEMI_LAGRANGIAN) {
-      const float4 s4 = tri_eval4(tp, sq);
-      decay_write<ST>(g, i, v.y, s4.x, s4.y, s4.z, s4.w, thermal_noise(sp, time), dt, D, sr, site);
-      continue;
+  const BrickRange br = brick_range(*count);
+  if (br.k >= br.end) return;
+  // The voxel's own advected velocity is loaded one brick ahead (the first
+  // gathers depend on it), in two register sets that take turns as in
+  // k_advect_velocity: two memory round trips per brick instead of three.
+  const uint32_t t = threadIdx.x, last = br.end - 1u;
+  uint32_t k = br.k, b0 = list[k];
+  float4 v0 = uq[(b0 << 9) | t];
+  for (;;) {
+    const uint32_t k1 = k + br.step;
+    const uint32_t b1 = list[min(k1, last)];
+    const float4 v1 = uq[(b1 << 9) | t];
+    {
+      const Grid g = gk.at(G, b0);
+      advect_s_brick<ST>(g, b0, v0, C0 + g.scene, mode, sq, uq, D, flags);
     }
-    const float4 f4 = tri_eval4(tp, uq);
-    F3 fwd = f3(f4.x, f4.y, f4.z);
-    F3 rec = back + fwd * dt;
-    Tri tr = tri(g, rec - half);
-    auto mac = [&](float predicted, float recovered, float original) {
-      float corrected = predicted + 0.5f * (original - recovered);
-      return clampf(corrected, fminf(original, predicted), fmaxf(original, predicted));
-    };
-    const float4 pr = tri_eval4(tp, sq), rc = tri_eval4(tr, sq), og = sq[i];
-    float out[4];
-    out[0] = mac(pr.x, rc.x, og.x);
-    out[1] = mac(pr.y, rc.y, og.y);
-    out[2] = mac(pr.z, rc.z, og.z);
-    out[3] = mac(pr.w, rc.w, og.w);
-    decay_write<ST>(g, i, v.y, out[0], out[1], out[2], out[3], thermal_noise(coord, time), dt, D, sr, site);
+    if (k1 >= br.end) break;
+    const uint32_t k2 = k1 + br.step;
+    b0 = list[min(k2, last)];
+    v0 = uq[(b0 << 9) | t];
+    {
+      const Grid g = gk.at(G, b1);
+      advect_s_brick<ST>(g, b1, v1, C0 + g.scene, mode, sq, uq, D, flags);
+    }
+    if (k2 >= br.end) break;
+    k = k2;
   }
 }
 
@@ -1134,7 +1244,7 @@ __device__ __forceinline__ float thermal_cluster(F3 c, float time) {
 template <class ST, bool MS>
 __global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
                                                     const StepConst* __restrict__ C0, SS<ST> S, SS<ST> D,
-                                                    const uint32_t* __restrict__ flags) {
+                                                    const uint32_t* __restrict__ flags, float4* __restrict__ sq) {
   GridCache<MS> gk;
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
@@ -1146,16 +1256,21 @@ __global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__
     Vox v = vox_of(g, b);
     if (!v.in) continue;
     const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
-    if (!flag_on(g, flags, x, y, z)) { ST::st(D.d, i, ST::ld(S.d, i)); ST::st(D.t, i, ST::ld(S.t, i)); ST::st(D.f, i, ST::ld(S.f, i)); D.r[i] = S.r[i]; continue; }
-    auto hotgas = [&](uint32_t a, uint32_t b2, uint32_t c2) {
-      uint32_t k = cidx(g, a, b2, c2);
+    if (!flag_on(g, flags, x, y, z)) {
+      // copied: the stored values are representable, so storing them is exact
+      const float cd = ST::ld(S.d, i), ct = ST::ld(S.t, i), cf = ST::ld(S.f, i), cr = S.r[i];
+      ST::st(D.d, i, cd); ST::st(D.t, i, ct); ST::st(D.f, i, cf); D.r[i] = cr;
+      if (sq) sq[i] = make_float4(cd, ct, cf, cr);
+      continue;
+    }
+    auto hotgas = [&](uint32_t k) {
       return clampf(ST::ld(S.t, k) * 1.55f + ST::ld(S.f, k) * 0.26f + S.r[k] * 0.44f, 0.f, 1.f);
     };
+    const Nbr6 nb = nbr6(g, x, y, z, i);
     float ny = height01(y, g.H);
     float cluster = thermal_cluster(f3(float(x), float(y), float(z)), time);
-    float nh = (hotgas(incu(x, g.W - 1), y, z) + hotgas(decu(x), y, z) + hotgas(x, incu(y, g.H - 1), z) + hotgas(x, decu(y), z) +
-                hotgas(x, y, incu(z, g.D - 1)) + hotgas(x, y, decu(z))) * 0.16666667f;
-    float localBefore = hotgas(x, y, z);
+    float nh = (hotgas(nb.xp) + hotgas(nb.xm) + hotgas(nb.yp) + hotgas(nb.ym) + hotgas(nb.zp) + hotgas(nb.zm)) * 0.16666667f;
+    float localBefore = hotgas(i);
     float den = ST::ld(S.d, i), tem = ST::ld(S.t, i), fu = ST::ld(S.f, i), rea = S.r[i];
     float radiative = fmaxf(nh - localBefore * 0.55f, 0.f) * smoothk(0.04f, 0.76f, den) * (0.45f + (1.f - cluster) * 0.55f) * dt * 2.6f;
     float hotPocket = smoothk(0.58f, 0.92f, cluster + rea * 0.22f);
@@ -1171,10 +1286,13 @@ __global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__
     float coolingSmoke = den * tem * (coolingPocket + hotSoot * 0.32f) * dt * 0.16f;
     float reactionDecay = rea * (1.f - dt * mixf(0.68f, 0.22f, hotPocket));
     const uint32_t hi = i - (g.base << 9);
-    ST::st_sr(D.f, i, clamp01(fu - burn), sr, sf, hi);
-    ST::st_sr(D.t, i, clamp01(tem + burn * mixf(2.1f, 3.25f, hotPocket) + radiative - heatLoss), sr, st, hi);
-    ST::st_sr(D.d, i, clamp01(fmaxf(0.f, den + smokeYield + coolingSmoke - densityLoss - clearing * 0.72f)), sr, sd, hi);
-    D.r[i] = clamp01(fmaxf(reactionDecay, burn * mixf(4.f, 7.5f, hotPocket) + radiative * 1.8f));
+    const typename ST::T ef = ST::enc_sr(clamp01(fu - burn), sr, sf, hi);
+    const typename ST::T et = ST::enc_sr(clamp01(tem + burn * mixf(2.1f, 3.25f, hotPocket) + radiative - heatLoss), sr, st, hi);
+    const typename ST::T ed = ST::enc_sr(clamp01(fmaxf(0.f, den + smokeYield + coolingSmoke - densityLoss - clearing * 0.72f)), sr, sd, hi);
+    const float er = clamp01(fmaxf(reactionDecay, burn * mixf(4.f, 7.5f, hotPocket) + radiative * 1.8f));
+    D.f[i] = ef; D.t[i] = et; D.d[i] = ed; D.r[i] = er;
+    // set 0's scalar gather mirror for the next step's scalar advection
+    if (sq) sq[i] = make_float4(ST::dec(ed), ST::dec(et), ST::dec(ef), er);
   }
 }
 
@@ -1192,15 +1310,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     if (!v.in) continue;
     const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
     if (!flag_on(g, flags, x, y, z)) continue;
-    auto hot = [&](uint32_t a, uint32_t b2, uint32_t c2) {
-      uint32_t k = cidx(g, a, b2, c2);
-      return clampf(ST::ld(S.t, k) * 1.f + S.r[k] * 0.85f, 0.f, 1.f);
-    };
-    auto rd = [&](uint32_t a, uint32_t b2, uint32_t c2) { return ST::ld(S.d, cidx(g, a, b2, c2)); };
+    auto hot = [&](uint32_t k) { return clampf(ST::ld(S.t, k) * 1.f + S.r[k] * 0.85f, 0.f, 1.f); };
+    auto rd = [&](uint32_t k) { return ST::ld(S.d, k); };
+    const Nbr6 nb = nbr6(g, x, y, z, i);
     const float dt = P.delta_time;
     const float den = ST::ld(S.d, i);
     float response = P.buoyancy * (1.f + smoothf(6.f, 24.f, fabsf(P.buoyancy)) * 0.38f);
-    float hotGas = hot(x, y, z);
+    float hotGas = hot(i);
     float lift = hotGas * response;
     F3 gdir = f3(P.gravity[0], P.gravity[1], P.gravity[2]);
     F3 gravityForce = gdir * den * P.gravity[3];
@@ -1210,10 +1326,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     float windAmount = P.wind[3] * (0.18f + den * 0.55f + ny * 0.45f);
     F3 wdir = f3(P.wind[0], P.wind[1], P.wind[2]);
     F3 wind = wdir * windAmount + f3(-P.wind[2], 0.f, P.wind[0]) * windAmount * shear;
-    F3 tg = f3(hot(incu(x, g.W - 1), y, z) - hot(decu(x), y, z), hot(x, incu(y, g.H - 1), z) - hot(x, decu(y), z),
-               hot(x, y, incu(z, g.D - 1)) - hot(x, y, decu(z))) * 0.5f;
-    F3 dg = f3(rd(incu(x, g.W - 1), y, z) - rd(decu(x), y, z), rd(x, incu(y, g.H - 1), z) - rd(x, decu(y), z),
-               rd(x, y, incu(z, g.D - 1)) - rd(x, y, decu(z))) * 0.5f;
+    F3 tg = f3(hot(nb.xp) - hot(nb.xm), hot(nb.yp) - hot(nb.ym), hot(nb.zp) - hot(nb.zm)) * 0.5f;
+    F3 dg = f3(rd(nb.xp) - rd(nb.xm), rd(nb.yp) - rd(nb.ym), rd(nb.zp) - rd(nb.zm)) * 0.5f;
     float dgl = fmaxf(len3(dg), 0.0001f);
     float coldSmoke = den * (1.f - smoothk(0.12f, 0.72f, hotGas));
     float surfaceBreakup = smoothk(0.008f, 0.06f, dgl) * coldSmoke;
@@ -1283,9 +1397,8 @@ __global__ __launch_bounds__(512) void k_confinement(const uint32_t* __restrict_
     if (!v.in) continue;
     const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
     if (on_boundary(g, x, y, z)) { cmag[i] = 0.f; continue; }
-    auto m = [&](uint32_t a, uint32_t b2, uint32_t c2) { return wmag[cidx(g, a, b2, c2)]; };
-    F3 grad = f3(m(incu(x, g.W - 1), y, z) - m(decu(x), y, z), m(x, incu(y, g.H - 1), z) - m(x, decu(y), z),
-                 m(x, y, incu(z, g.D - 1)) - m(x, y, decu(z))) * 0.5f;
+    const Nbr6 nb = nbr6(g, x, y, z, i);
+    F3 grad = f3(wmag[nb.xp] - wmag[nb.xm], wmag[nb.yp] - wmag[nb.ym], wmag[nb.zp] - wmag[nb.zm]) * 0.5f;
     F3 dir = grad / fmaxf(len3(grad), 0.0001f);
     float hotGas = clampf(ST::ld(S.t, i) + S.r[i] * 0.85f, 0.f, 1.f);
     float mask = clampf(P.vorticity_mask[0] + smoothk(0.02f, 0.38f, wmag[i]) * P.vorticity_mask[1] +
@@ -1661,49 +1774,39 @@ __global__ __launch_bounds__(512) void k_level_masks(LevelMaskArgs a) {
   start[0] = 0;
   for (int l = 0; l < a.nlevels; ++l) start[l + 1] = start[l] + *a.count[l];
   const uint32_t total = start[a.nlevels];
-  // one grid-stride loop over the bricks of all levels, 4 bricks per block and
-  // round so that their list and flag loads overlap
-  constexpr int U = 4;
-  for (uint32_t e0 = blockIdx.x; e0 < total; e0 += gridDim.x * U) {
-    int lv[U];
-    uint32_t kk[U], bb[U];
+  // one wave per brick, over the bricks of all levels: lane (x, y) = (lane & 7,
+  // lane >> 3) evaluates its column of the brick, one ballot per z-slice, and
+  // lane z stores word z (8 bricks per workgroup and round, so that few rounds of
+  // list and flag loads are in flight one after another)
+  const uint32_t waves = gridDim.x * 8u;
+  for (uint32_t e = blockIdx.x * 8u + wave; e < total; e += waves) {
+    int l = 0;
+    while (e >= start[l + 1]) ++l;
+    const uint32_t k = e - start[l];
+    const uint32_t b = wave_uniform(a.list[l][k]);
+    const Grid g = grid_at<MS>(a.g[l], b);
+    const Grid fg = grid_scene<MS>(a.m[l].fine, g.scene);
+    const uint32_t lb = b - g.base;
+    const uint32_t bx = lb % g.BX, by = (lb / g.BX) % g.BY, bz = lb / (g.BX * g.BY);
+    const uint32_t x = bx * 8 + (lane & 7u), y = by * 8 + (lane >> 3);
+    uint64_t mine = 0;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t e = e0 + uint32_t(u) * gridDim.x;
-      lv[u] = -1;
-      kk[u] = 0;
-      bb[u] = 0;
-      if (e < total) {
-        int l = 0;
-        while (e >= start[l + 1]) ++l;
-        lv[u] = l;
-        kk[u] = e - start[l];
-        bb[u] = a.list[l][kk[u]];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (lv[u] < 0) continue;
-      const uint32_t b = bb[u];
-      const Grid g = grid_at<MS>(a.g[lv[u]], b);
-      const uint32_t lb = b - g.base;
-      const uint32_t x = (lb % g.BX) * 8 + (lane & 7u), y = ((lb / g.BX) % g.BY) * 8 + (lane >> 3),
-                     z = (lb / (g.BX * g.BY)) * 8 + wave;
-      const bool act = x < g.W && y < g.H && z < g.D && !on_boundary(g, x, y, z) &&
-                       lv_active(a.m[lv[u]], grid_scene<MS>(a.m[lv[u]].fine, g.scene), x, y, z);
+    for (uint32_t zz = 0; zz < 8; ++zz) {
+      const uint32_t z = bz * 8 + zz;
+      const bool act = x < g.W && y < g.H && z < g.D && !on_boundary(g, x, y, z) && lv_active(a.m[l], fg, x, y, z);
       const uint64_t w = __ballot(act);
-      if (lane == 0) a.bits[lv[u]][size_t(kk[u]) * 8 + wave] = w;
-      if (lv[u] == 0 && a.nbr0 && wave == 0) {
-        // lane d < 6: is face neighbour d (+x, -x, +y, -y, +z, -z) of the brick listed?
-        const uint32_t bx = lb % g.BX, by = (lb / g.BX) % g.BY, bz = lb / (g.BX * g.BY);
-        const int d = int(lane);
-        const int nx = int(bx) + (d == 0) - (d == 1), ny = int(by) + (d == 2) - (d == 3), nz = int(bz) + (d == 4) - (d == 5);
-        bool on = false;
-        if (d < 6 && nx >= 0 && ny >= 0 && nz >= 0 && nx < int(g.BX) && ny < int(g.BY) && nz < int(g.BZ))
-          on = a.page0[g.base + uint32_t(nx) + g.BX * (uint32_t(ny) + g.BY * uint32_t(nz))] >= 0;
-        const uint64_t nw = __ballot(on);
-        if (lane == 0) a.nbr0[kk[u]] = uint32_t(nw) & 0x3fu;
-      }
+      mine = lane == zz ? w : mine;
+    }
+    if (lane < 8u) a.bits[l][size_t(k) * 8 + lane] = mine;
+    if (l == 0 && a.nbr0) {
+      // lane d < 6: is face neighbour d (+x, -x, +y, -y, +z, -z) of the brick listed?
+      const int d = int(lane);
+      const int nx = int(bx) + (d == 0) - (d == 1), ny = int(by) + (d == 2) - (d == 3), nz = int(bz) + (d == 4) - (d == 5);
+      bool on = false;
+      if (d < 6 && nx >= 0 && ny >= 0 && nz >= 0 && nx < int(g.BX) && ny < int(g.BY) && nz < int(g.BZ))
+        on = a.page0[g.base + uint32_t(nx) + g.BX * (uint32_t(ny) + g.BY * uint32_t(nz))] >= 0;
+      const uint64_t nw = __ballot(on);
+      if (lane == 0) a.nbr0[k] = uint32_t(nw) & 0x3fu;
     }
   }
 }
@@ -3347,13 +3450,14 @@ __global__ void k_scene_counts(const uint32_t* __restrict__ list, const uint32_t
     else FX_LAUNCH(KF, blocks, threads, stream, __VA_ARGS__);   \
   } while (0)
 void launch_inject(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
-                   const float* src, Scalars set0, Vec3 u, uint32_t* touched, int parts) {
+                   const float* src, Scalars set0, Vec3 u, uint32_t* touched, int parts, void* sq) {
+  float4* S4 = static_cast<float4*>(sq);
   switch (set0.prec) {
-    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_inject<StBF16, true>), (k_inject<StBF16, false>), blocks, 512, s, list, count, g, c, src, typed<StBF16>(set0), u, touched, parts); break;
-    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_inject<StF16, true>), (k_inject<StF16, false>), blocks, 512, s, list, count, g, c, src, typed<StF16>(set0), u, touched, parts); break;
-    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_inject<StU16, true>), (k_inject<StU16, false>), blocks, 512, s, list, count, g, c, src, typed<StU16>(set0), u, touched, parts); break;
-    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_inject<StUF16, true>), (k_inject<StUF16, false>), blocks, 512, s, list, count, g, c, src, typed<StUF16>(set0), u, touched, parts); break;
-    default: FX_LAUNCH_MS(g, (k_inject<StF32, true>), (k_inject<StF32, false>), blocks, 512, s, list, count, g, c, src, typed<StF32>(set0), u, touched, parts);
+    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_inject<StBF16, true>), (k_inject<StBF16, false>), blocks, 512, s, list, count, g, c, src, typed<StBF16>(set0), u, touched, parts, S4); break;
+    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_inject<StF16, true>), (k_inject<StF16, false>), blocks, 512, s, list, count, g, c, src, typed<StF16>(set0), u, touched, parts, S4); break;
+    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_inject<StU16, true>), (k_inject<StU16, false>), blocks, 512, s, list, count, g, c, src, typed<StU16>(set0), u, touched, parts, S4); break;
+    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_inject<StUF16, true>), (k_inject<StUF16, false>), blocks, 512, s, list, count, g, c, src, typed<StUF16>(set0), u, touched, parts, S4); break;
+    default: FX_LAUNCH_MS(g, (k_inject<StF32, true>), (k_inject<StF32, false>), blocks, 512, s, list, count, g, c, src, typed<StF32>(set0), u, touched, parts, S4);
   }
 }
 void launch_clear_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t value) {
@@ -3399,16 +3503,8 @@ void launch_compact(hipStream_t s, const CompactArgs& A) {
   FX_LAUNCH(k_compact_write, blocks, 256, s, A);
 }
 void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
-                            Vec3 u, Vec3 us, void* uq, Scalars set0, void* sq) {
-  float4* U4 = static_cast<float4*>(uq);
-  float4* S4 = static_cast<float4*>(sq);
-  switch (set0.prec) {
-    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_advect_velocity<StBF16, true>), (k_advect_velocity<StBF16, false>), blocks, 512, s, list, count, g, c, u, us, U4, typed<StBF16>(set0), S4); break;
-    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_advect_velocity<StF16, true>), (k_advect_velocity<StF16, false>), blocks, 512, s, list, count, g, c, u, us, U4, typed<StF16>(set0), S4); break;
-    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_advect_velocity<StU16, true>), (k_advect_velocity<StU16, false>), blocks, 512, s, list, count, g, c, u, us, U4, typed<StU16>(set0), S4); break;
-    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_advect_velocity<StUF16, true>), (k_advect_velocity<StUF16, false>), blocks, 512, s, list, count, g, c, u, us, U4, typed<StUF16>(set0), S4); break;
-    default: FX_LAUNCH_MS(g, (k_advect_velocity<StF32, true>), (k_advect_velocity<StF32, false>), blocks, 512, s, list, count, g, c, u, us, U4, typed<StF32>(set0), S4);
-  }
+                            Vec3 u, Vec3 us, void* uq) {
+  FX_LAUNCH_MS(g, (k_advect_velocity<true>), (k_advect_velocity<false>), blocks, 512, s, list, count, g, c, u, us, static_cast<float4*>(uq));
 }
 void launch_pack_gather(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Scalars set0, Vec3 us,
                         const uint32_t* rlist, const uint32_t* rcount, void* sq, void* uq) {
@@ -3435,13 +3531,14 @@ void launch_advect_scalars(hipStream_t s, int blocks, const uint32_t* list, cons
   }
 }
 void launch_combustion(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
-                       Scalars src, Scalars dst, const uint32_t* flags) {
+                       Scalars src, Scalars dst, const uint32_t* flags, void* sq) {
+  float4* S4 = static_cast<float4*>(sq);
   switch (src.prec) {
-    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_combustion<StBF16, true>), (k_combustion<StBF16, false>), blocks, 512, s, list, count, g, c, typed<StBF16>(src), typed<StBF16>(dst), flags); break;
-    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_combustion<StF16, true>), (k_combustion<StF16, false>), blocks, 512, s, list, count, g, c, typed<StF16>(src), typed<StF16>(dst), flags); break;
-    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_combustion<StU16, true>), (k_combustion<StU16, false>), blocks, 512, s, list, count, g, c, typed<StU16>(src), typed<StU16>(dst), flags); break;
-    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_combustion<StUF16, true>), (k_combustion<StUF16, false>), blocks, 512, s, list, count, g, c, typed<StUF16>(src), typed<StUF16>(dst), flags); break;
-    default: FX_LAUNCH_MS(g, (k_combustion<StF32, true>), (k_combustion<StF32, false>), blocks, 512, s, list, count, g, c, typed<StF32>(src), typed<StF32>(dst), flags);
+    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_combustion<StBF16, true>), (k_combustion<StBF16, false>), blocks, 512, s, list, count, g, c, typed<StBF16>(src), typed<StBF16>(dst), flags, S4); break;
+    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_combustion<StF16, true>), (k_combustion<StF16, false>), blocks, 512, s, list, count, g, c, typed<StF16>(src), typed<StF16>(dst), flags, S4); break;
+    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_combustion<StU16, true>), (k_combustion<StU16, false>), blocks, 512, s, list, count, g, c, typed<StU16>(src), typed<StU16>(dst), flags, S4); break;
+    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_combustion<StUF16, true>), (k_combustion<StUF16, false>), blocks, 512, s, list, count, g, c, typed<StUF16>(src), typed<StUF16>(dst), flags, S4); break;
+    default: FX_LAUNCH_MS(g, (k_combustion<StF32, true>), (k_combustion<StF32, false>), blocks, 512, s, list, count, g, c, typed<StF32>(src), typed<StF32>(dst), flags, S4);
   }
 }
 void launch_buoyancy(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,

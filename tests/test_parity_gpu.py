@@ -525,6 +525,55 @@ def test_velocity_written_through_device_pointer_reaches_the_advection():
         s.dispose()
 
 
+@pytest.mark.gpu
+def test_scalars_written_through_device_pointer_reach_the_advection():
+    """The scalar advection gathers a float4 mirror of density, temperature, fuel
+    and reaction that the injection and combustion passes keep. Scalars written
+    through fx_field_device_ptr (a borrowed pointer) between steps must reach the
+    next step exactly like ones written with fx_write_field. Dense domain, as in
+    the velocity case above."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = 40
+    desc = scenes.c2_desc(n, domain_mode=abi.FX_DOMAIN_DENSE)
+    srcs = scenes.c2_sources(n, radius_vox=3.0)
+    st, _ = scenes.c2_plume_state(n, srcs, target=0.2)
+    other = {k: np.clip(v, 0.0, 1.0) for k, v in scenes.random_state((n, n, n), 12).items()
+             if k in ("density", "temperature", "fuel", "reaction")}
+    sims = []
+    for _ in range(3):
+        sim = CombustionVolumeSimulation(desc)
+        sim.updateSources(srcs)
+        for k, v in st.items():
+            sim.write_field(k, v)
+        sims.append(sim)
+    a, b, c = sims
+    for k, v in other.items():
+        c.write_field(k, v)
+    for k in range(2):
+        a.step(None, k / 60.0, 1 / 60.0)
+        b.step(None, k / 60.0, 1 / 60.0)
+    for s in sims:
+        s.synchronize()
+    nbytes = ((n + 7) // 8) ** 3 * 512 * 4
+    for name in ("density", "temperature", "fuel", "reaction"):
+        fid = abi.FIELDS[name]
+        src = c._lib.fx_field_device_ptr(c._h, fid)
+        dst = b._lib.fx_field_device_ptr(b._h, fid)
+        assert src and dst
+        assert hip.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes), 3) == 0  # device to device
+        a.write_field(name, c.read_field(name))
+    assert hip.hipDeviceSynchronize() == 0
+    for k in range(2, 4):
+        a.step(None, k / 60.0, 1 / 60.0)
+        b.step(None, k / 60.0, 1 / 60.0)
+    for name in PERSISTENT:
+        np.testing.assert_array_equal(b.read_field(name), a.read_field(name), err_msg=name)
+    assert float(np.abs(a.read_field("density")).max()) > 0.1
+    for s in sims:
+        s.dispose()
+
+
 def test_coarsest_level_beyond_the_lds_cap():
     """Sparse 208^3 plume with ~70% of the bricks active: the coarsest V-cycle level (26^3) holds
     more active voxels than the one-workgroup LDS smoother (8192). The first step, before the
