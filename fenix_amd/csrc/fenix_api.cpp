@@ -429,8 +429,9 @@ void drain_timing(fx_sim* s) {
 void pass_activate_render(fx_sim* s) {
   StageTimer timer(s, FX_PASS_ACTIVATE_RENDER);
   if (s->sparse) {
-    // Candidates: last step's active list + injection/host-written bricks.
-    launch_clear_u32(s->stream, s->flagsInj, s->nb, 0u);
+    // Candidates: last step's active list + injection/host-written bricks. Inside a
+    // step the injection launch cleared the flags.
+    if (!s->in_step) launch_clear_u32(s->stream, s->flagsInj, s->nb, 0u);
     launch_activate(s->stream, grid_for(s, s->nb), s->lv[0].L.list, s->lv[0].L.count, s->d_cand, &s->d_const->cand_count,
                     s->g, s->d_const, s->desc.activation_rule, set_of(s, 0), vec_of(s->u), s->flagsInj, s->lflags);
     launch_compact(s->stream, s->cargs);
@@ -452,7 +453,7 @@ void pass_activate_render(fx_sim* s) {
     R.first[R.nlevels] = nb;
     launch_retire(s->stream, grid_for(s, s->nb), R);
   } else {
-    launch_clear_u32(s->stream, s->flagsInj, s->nb, 0u);  // ORCH:271
+    if (!s->in_step) launch_clear_u32(s->stream, s->flagsInj, s->nb, 0u);  // ORCH:271 (in a step: by the injection launch)
     launch_activate(s->stream, grid_for(s, s->nb), s->identity, s->identity_count, nullptr, nullptr, s->g, s->d_const,
                     s->desc.activation_rule, set_of(s, 0), vec_of(s->u), s->flagsInj, s->lflags);
   }
@@ -467,10 +468,12 @@ void pass_inject(fx_sim* s, int parts) {
   StageTimer timer(s, parts == 2 ? FX_PASS_INJECT_VELOCITY : parts == 4 ? FX_PASS_INJECT_IGNITER : FX_PASS_INJECT_SCALAR);
   if (s->sparse)
     launch_inject(s->stream, grid_for(s, s->nb), s->d_cand, &s->d_const->inj_count, s->g, s->d_const, s->d_src, set_of(s, 0),
-                  vec_of(s->u), s->touched, parts, s->in_step ? s->d_sq : nullptr);
+                  vec_of(s->u), s->touched, parts, s->in_step ? s->d_sq : nullptr, s->in_step ? s->flagsInj : nullptr,
+                  uint32_t(s->nb));
   else
     launch_inject(s->stream, grid_for(s, s->nb), s->identity, s->identity_count, s->g, s->d_const, s->d_src, set_of(s, 0),
-                  vec_of(s->u), nullptr, parts, s->in_step ? s->d_sq : nullptr);
+                  vec_of(s->u), nullptr, parts, s->in_step ? s->d_sq : nullptr, s->in_step ? s->flagsInj : nullptr,
+                  uint32_t(s->nb));
 }
 void zero_level_p(fx_sim* s, int l, int which) {
   float* b[1] = {s->lv[l].p[which]};

@@ -122,9 +122,10 @@ struct Vec3 {
 };
 
 // sq (optional, parts 1 or 4): also writes the scalar gather mirror of set0 at
-// every injected voxel.
+// every injected voxel. clear (optional): clear_n words zeroed by the same launch.
 void launch_inject(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
-                   const StepConst* c, const float* src, Scalars set0, Vec3 u, uint32_t* touched, int parts, void* sq);
+                   const StepConst* c, const float* src, Scalars set0, Vec3 u, uint32_t* touched, int parts, void* sq,
+                   uint32_t* clear = nullptr, uint32_t clear_n = 0);
 void launch_clear_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t value);
 void launch_mark(hipStream_t s, const uint32_t* list, uint32_t n, uint32_t* touched);
 void launch_clear_f32(hipStream_t s, float* p, size_t n);

@@ -647,7 +647,11 @@ template <class ST, bool MS>
 __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
                                                 const StepConst* __restrict__ C0, const float* __restrict__ srcs0,
                                                 SS<ST> S, Vec3 u, uint32_t* touched, int parts,
-                                                float4* __restrict__ sq) {
+                                                float4* __restrict__ sq, uint32_t* __restrict__ clear, uint32_t clear_n) {
+  // the step's clear of the injected-activation flags (ORCH:271), which the next
+  // launch (the activation) fills: one launch fewer than a separate fill
+  if (clear)
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < clear_n; q += gridDim.x * blockDim.x) clear[q] = 0u;
   GridCache<MS> gk;
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
@@ -2560,15 +2564,6 @@ __device__ __forceinline__ float phase_hg(float c, float gg) {
 }
 
 // Brick flags packed 32 per word (the raymarch keeps them in LDS).
-__global__ void k_pack_flags(const uint32_t* __restrict__ flags, uint32_t nb, uint32_t* __restrict__ bits) {
-  FX_SIM_PRIORITY();
-  uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w * 32 >= nb) return;
-  uint32_t v = 0;
-  for (uint32_t k = 0; k < 32 && w * 32 + k < nb; ++k) v |= (flags[w * 32 + k] != 0u ? 1u : 0u) << k;
-  bits[w] = v;
-}
-
 // One ray (createVolumeRaymarchPass.ts fsMain) for pixel (px, py), reading the
 // render snapshot RB (density, temperature, fuel, reaction per voxel) and the
 // packed brick flags (sbits in LDS, else gbits in memory). g is one scene's grid;
@@ -2771,7 +2766,14 @@ __device__ __forceinline__ void march_pixel(const Grid& g, const RayConst& R, co
 template <class ST>
 __global__ __launch_bounds__(512) void k_pack_render(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, SS<ST> S,
                                                      const uint32_t* __restrict__ rlist, const uint32_t* __restrict__ rcount,
-                                                     float4* __restrict__ RB) {
+                                                     float4* __restrict__ RB, const uint32_t* __restrict__ flags, uint32_t nb,
+                                                     uint32_t* __restrict__ bits) {
+  // the brick flags packed 32 per word for the raymarch's empty-brick test
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w * 32u < nb; w += gridDim.x * blockDim.x) {
+    uint32_t v = 0;
+    for (uint32_t k = 0; k < 32 && w * 32 + k < nb; ++k) v |= (flags[w * 32 + k] != 0u ? 1u : 0u) << k;
+    bits[w] = v;
+  }
   FX_BRICK_LOOP(list, count) {
     const uint32_t i = (list[fx_k] << 9) | threadIdx.x;
     RB[i] = make_float4(ST::ld(S.d, i), ST::ld(S.t, i), ST::ld(S.f, i), S.r[i]);
@@ -3450,14 +3452,15 @@ __global__ void k_scene_counts(const uint32_t* __restrict__ list, const uint32_t
     else FX_LAUNCH(KF, blocks, threads, stream, __VA_ARGS__);   \
   } while (0)
 void launch_inject(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
-                   const float* src, Scalars set0, Vec3 u, uint32_t* touched, int parts, void* sq) {
+                   const float* src, Scalars set0, Vec3 u, uint32_t* touched, int parts, void* sq, uint32_t* clear,
+                   uint32_t clear_n) {
   float4* S4 = static_cast<float4*>(sq);
   switch (set0.prec) {
-    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_inject<StBF16, true>), (k_inject<StBF16, false>), blocks, 512, s, list, count, g, c, src, typed<StBF16>(set0), u, touched, parts, S4); break;
-    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_inject<StF16, true>), (k_inject<StF16, false>), blocks, 512, s, list, count, g, c, src, typed<StF16>(set0), u, touched, parts, S4); break;
-    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_inject<StU16, true>), (k_inject<StU16, false>), blocks, 512, s, list, count, g, c, src, typed<StU16>(set0), u, touched, parts, S4); break;
-    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_inject<StUF16, true>), (k_inject<StUF16, false>), blocks, 512, s, list, count, g, c, src, typed<StUF16>(set0), u, touched, parts, S4); break;
-    default: FX_LAUNCH_MS(g, (k_inject<StF32, true>), (k_inject<StF32, false>), blocks, 512, s, list, count, g, c, src, typed<StF32>(set0), u, touched, parts, S4);
+    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_inject<StBF16, true>), (k_inject<StBF16, false>), blocks, 512, s, list, count, g, c, src, typed<StBF16>(set0), u, touched, parts, S4, clear, clear_n); break;
+    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_inject<StF16, true>), (k_inject<StF16, false>), blocks, 512, s, list, count, g, c, src, typed<StF16>(set0), u, touched, parts, S4, clear, clear_n); break;
+    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_inject<StU16, true>), (k_inject<StU16, false>), blocks, 512, s, list, count, g, c, src, typed<StU16>(set0), u, touched, parts, S4, clear, clear_n); break;
+    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_inject<StUF16, true>), (k_inject<StUF16, false>), blocks, 512, s, list, count, g, c, src, typed<StUF16>(set0), u, touched, parts, S4, clear, clear_n); break;
+    default: FX_LAUNCH_MS(g, (k_inject<StF32, true>), (k_inject<StF32, false>), blocks, 512, s, list, count, g, c, src, typed<StF32>(set0), u, touched, parts, S4, clear, clear_n);
   }
 }
 void launch_clear_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t value) {
@@ -3686,14 +3689,12 @@ void launch_render_pack(hipStream_t s, int blocks, const uint32_t* list, const u
                         const uint32_t* rcount, void* rb, const uint32_t* flags, uint32_t nb, uint32_t* bits) {
   float4* RB = static_cast<float4*>(rb);
   switch (set0.prec) {
-    case FX_PRECISION_BF16: FX_LAUNCH(k_pack_render<StBF16>, blocks, 512, s, list, count, typed<StBF16>(set0), rlist, rcount, RB); break;
-    case FX_PRECISION_FP16: FX_LAUNCH(k_pack_render<StF16>, blocks, 512, s, list, count, typed<StF16>(set0), rlist, rcount, RB); break;
-    case FX_PRECISION_U16: FX_LAUNCH(k_pack_render<StU16>, blocks, 512, s, list, count, typed<StU16>(set0), rlist, rcount, RB); break;
-    case FX_PRECISION_UF16: FX_LAUNCH(k_pack_render<StUF16>, blocks, 512, s, list, count, typed<StUF16>(set0), rlist, rcount, RB); break;
-    default: FX_LAUNCH(k_pack_render<StF32>, blocks, 512, s, list, count, typed<StF32>(set0), rlist, rcount, RB);
+    case FX_PRECISION_BF16: FX_LAUNCH(k_pack_render<StBF16>, blocks, 512, s, list, count, typed<StBF16>(set0), rlist, rcount, RB, flags, nb, bits); break;
+    case FX_PRECISION_FP16: FX_LAUNCH(k_pack_render<StF16>, blocks, 512, s, list, count, typed<StF16>(set0), rlist, rcount, RB, flags, nb, bits); break;
+    case FX_PRECISION_U16: FX_LAUNCH(k_pack_render<StU16>, blocks, 512, s, list, count, typed<StU16>(set0), rlist, rcount, RB, flags, nb, bits); break;
+    case FX_PRECISION_UF16: FX_LAUNCH(k_pack_render<StUF16>, blocks, 512, s, list, count, typed<StUF16>(set0), rlist, rcount, RB, flags, nb, bits); break;
+    default: FX_LAUNCH(k_pack_render<StF32>, blocks, 512, s, list, count, typed<StF32>(set0), rlist, rcount, RB, flags, nb, bits);
   }
-  const uint32_t nwords = (nb + 31) / 32;
-  FX_LAUNCH(k_pack_flags, (nwords + 255) / 256, 256, s, flags, nb, bits);
 }
 void launch_raymarch(hipStream_t s, Grid g, const RayConst& rc, const void* rb, const uint32_t* bits, uint32_t* queue, int blocks,
                      uint16_t* out) {
