@@ -559,8 +559,21 @@ int solve_vcycle(fx_sim* s) {
       if (n & 1) cur[l] = 1 - cur[l];
       return;
     }
+    // compute tile of the level's split sweeps (plan bits kPlanTileShift + 2l of the up mask)
+    const uint32_t tile = l < L - 1 ? (s->split_up >> (kPlanTileShift + 2 * l)) & 3u : 0u;
     for (int k = 0; k < n; ++k) {
       const float* src = (zero_start && k == 0) ? nullptr : V.p[cur[l]];
+      if (tile == 1u || tile == 2u) {
+        const int ns = (per >= 2 && k + 1 < n) ? 2 : 1;
+        {
+          StageTimer timer(s, l == 0 ? FX_TIME_JACOBI_FINE : -1);
+          launch_sweep_tile(s->stream, grid_for(s, V.nb), tile == 1u ? 4 : 16, ns, level_list(s, l), level_count(s, l), V.g,
+                            bits_of(s, l), s->d_const, V.div, src, V.p[1 - cur[l]]);
+        }
+        cur[l] = 1 - cur[l];
+        k += ns - 1;
+        continue;
+      }
       if (per >= 4 && k + 3 < n) {
         launch_sweep_quad(s->stream, grid_for(s, V.nb), level_list(s, l), level_count(s, l), V.g, bits_of(s, l), s->d_const, V.div,
                           src, V.p[1 - cur[l]]);
@@ -2208,6 +2221,7 @@ int fx_autotune_vcycle(fx_sim* s, uint32_t repeats, uint32_t* down_split, uint32
   if (vc.post_sweeps >= 2) ups.push_back(2u);
   const uint32_t np = uint32_t(downs.size() * ups.size());
   for (int l = 0; l + 1 < L && rc == FX_OK; ++l) {
+    s->split_up &= ~(3u << (kPlanTileShift + 2 * l));  // legs are compared on 8^3 bricks
     // each plan's best of two interleaved rounds
     std::vector<float> t(np, 0.f);
     for (int round = 0; round < 2 && rc == FX_OK; ++round)
@@ -2223,6 +2237,21 @@ int fx_autotune_vcycle(fx_sim* s, uint32_t repeats, uint32_t* down_split, uint32
       if (t[c] < t[best_c]) best_c = c;
     set_leg(s->split_down, l, downs[best_c % downs.size()]);
     set_leg(s->split_up, l, ups[best_c / downs.size()]);
+    // a split leg at this level: its sweeps' compute tile (8^3 bricks, 4^3 octants
+    // or 16^3 brick groups; the pair bits chosen above give the halo)
+    if ((((s->split_down | s->split_up) >> l) & 1u) && rc == FX_OK) {
+      const uint32_t tmask = 3u << (kPlanTileShift + 2 * l);
+      float tt[3] = {t[best_c], 0.f, 0.f};
+      for (int round = 0; round < 2 && rc == FX_OK; ++round)
+        for (uint32_t tc = 1; tc < 3 && rc == FX_OK; ++tc) {
+          s->split_up = (s->split_up & ~tmask) | (tc << (kPlanTileShift + 2 * l));
+          float ms = 0.f;
+          rc = time_plan(&ms);
+          if (rc == FX_OK && (round == 0 || ms < tt[tc])) tt[tc] = ms;
+        }
+      const uint32_t best_t = tt[1] < tt[0] ? (tt[2] < tt[1] ? 2u : 1u) : (tt[2] < tt[0] ? 2u : 0u);
+      s->split_up = (s->split_up & ~tmask) | (best_t << (kPlanTileShift + 2 * l));
+    }
   }
   if (s->coarse_split && rc == FX_OK) {
     // the coarsest level beyond the one-workgroup smoother: 1, 2 or 4 sweeps per launch

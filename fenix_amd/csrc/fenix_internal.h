@@ -183,9 +183,18 @@ void launch_sweep_pair(hipStream_t s, int blocks, const uint32_t* list, const ui
 // launch_sweep calls.
 void launch_sweep_quad(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvBits b, const StepConst* c,
                        const float* rhs, const float* src, float* dst);
+// ns (1 or 2) sweeps in one launch on compute tiles of 4^3 (a brick's octants,
+// one wave each) or 16^3 (2x2x2 brick groups) with halo ns, bit-identical to ns
+// launch_sweep calls.
+void launch_sweep_tile(hipStream_t s, int blocks, int tile, int ns, const uint32_t* list, const uint32_t* count, Grid g, LvBits b,
+                       const StepConst* c, const float* rhs, const float* src, float* dst);
 // Bits 8.. of a V-cycle plan mask: the split leg of level l runs its sweeps two
 // per launch (launch_sweep_pair).
 constexpr int kPlanPairShift = 8;
+// Bits 20 + 2l .. 21 + 2l of the up mask: the compute tile of level l's split-leg
+// sweeps (0: the 8^3 brick, 1: 4^3 octants, 2: 16^3 brick groups; launch_sweep_tile);
+// the pair bit gives the halo (sweeps per launch).
+constexpr int kPlanTileShift = 20;
 // Bits 24-25 of the down mask: sweeps per launch of the coarsest level when its
 // active voxels exceed the one-workgroup smoother (0: one, 1: two, 2: four).
 constexpr int kPlanCoarseShift = 24;

@@ -2500,6 +2500,203 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NS == 2 ? 8
   }
 }
 
+// ------------------------------------------------ sweep compute tiles --
+// Split-leg sweeps on compute tiles other than the 8^3 brick, the tile
+// dimension of the V-cycle autotuner: NS (1 or 2) weighted-Jacobi sweeps per
+// launch on a cube of E voxels per axis, staged in LDS with a halo of NS.
+// - E = 4: the eight 4^3 octants of a brick, one wave each with its own tiles:
+//   eight times the waves of a brick-per-wave sweep on levels with few bricks;
+// - E = 16: a 2x2x2 group of bricks per workgroup, taken by the group's first
+//   listed brick (lowest index): one halo for eight bricks.
+// Every voxel's arithmetic and neighbour order are those of k_sweep (xp, xm, yp,
+// ym, zp, zm; inactive voxels 0; ZERO starts from p = 0), so results are
+// bit-identical to it and to every other plan.
+
+// Activity of the voxel at coordinates (lx, ly, lz) in [0, 8K)^3 of a K^3 brick
+// neighbourhood whose words nb hold brick (i, j, k) at ((i + K (j + K k)) * 8 + z).
+template <int K>
+__device__ __forceinline__ bool nb_on(const uint64_t* nb, int lx, int ly, int lz) {
+  return (nb[((lx >> 3) + K * ((ly >> 3) + K * (lz >> 3))) * 8 + (lz & 7)] >> ((lx & 7) | ((ly & 7) << 3))) & 1ull;
+}
+// Word q of the K^3 brick neighbourhood whose first brick is (bx0, by0, bz0);
+// bricks outside the level or unlisted give 0.
+template <int K>
+__device__ __forceinline__ uint64_t nbk_word(const Grid& g, const LvBits& lb, int bx0, int by0, int bz0, int q) {
+  const int c = q >> 3, w = q & 7;
+  const int nx = bx0 + c % K, ny = by0 + (c / K) % K, nz = bz0 + c / (K * K);
+  uint64_t v = 0;
+  if (q < K * K * K * 8 && nx >= 0 && ny >= 0 && nz >= 0 && nx < int(g.BX) && ny < int(g.BY) && nz < int(g.BZ)) {
+    const uint32_t nbk = g.base + uint32_t(nx) + g.BX * (uint32_t(ny) + g.BY * uint32_t(nz));
+    const int pos = lb.page ? lb.page[nbk] : int(nbk);
+    if (pos >= 0) v = lb.bits[size_t(pos) * 8 + w];
+  }
+  return v;
+}
+// One sweep A -> B over the cube [lo, lo + side)^3 of an S^3 tile whose voxel
+// (0, 0, 0) sits at neighbourhood coordinates (ox, oy, oz); `lane`/`nt` split the
+// cube over the calling threads.
+template <int K, int S>
+__device__ __forceinline__ void tile_sweep_k(const uint64_t* nb, int ox, int oy, int oz, const float* A, const float* R, float* B,
+                                             int lo, int side, int lane, int nt, float om) {
+  const int n = side * side * side;
+  for (int q = lane; q < n; q += nt) {
+    const int tx = lo + q % side, ty = lo + (q / side) % side, tz = lo + q / (side * side);
+    const int i = tx + S * (ty + S * tz);
+    if (!nb_on<K>(nb, ox + tx, oy + ty, oz + tz)) { B[i] = 0.f; continue; }
+    const float sum = A[i + 1] + A[i - 1] + A[i + S] + A[i - S] + A[i + S * S] + A[i - S * S];
+    const float pj = (sum - R[i]) / 6.f;
+    B[i] = mixf(A[i], pj, om);
+  }
+}
+template <int NS, bool ZERO, bool MS>
+__global__ __launch_bounds__(512) void k_sweep_t4(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
+                                                  LvBits lb, const StepConst* __restrict__ C0, const float* __restrict__ rhs,
+                                                  const float* __restrict__ src, float* __restrict__ dst) {
+  FX_SIM_PRIORITY();
+  constexpr int S = 4 + 2 * NS, N = S * S * S, IT = (N + 63) / 64;
+  __shared__ float tr[8][N], ta[8][N], tb[8][NS > 1 ? N : 1];
+  __shared__ uint64_t nb[216];
+  const int lane = int(threadIdx.x & 63u), w = int(threadIdx.x >> 6);
+  // the wave's octant: origin (sx, sy, sz) in the brick; its tile starts NS before
+  const int sx = (w & 1) * 4, sy = ((w >> 1) & 1) * 4, sz = (w >> 2) * 4;
+  const int off_x = sx - NS + 8, off_y = sy - NS + 8, off_z = sz - NS + 8;  // in the 3^3 neighbourhood
+  GridCache<MS> gk;
+  FX_BRICK_LOOP(list, count) {
+    const uint32_t b = list[fx_k];
+    const Grid g = gk.at(G, b);
+    int bx, by, bz;
+    brick_coords(g, b, bx, by, bz);
+    const int ox = bx * 8 + sx - NS, oy = by * 8 + sy - NS, oz = bz * 8 + sz - NS;
+    float vr[IT], vp[IT];
+#pragma unroll
+    for (int j = 0; j < IT; ++j) {
+      const int q = lane + j * 64;
+      const int x = ox + q % S, y = oy + (q / S) % S, z = oz + q / (S * S);
+      const bool in = q < N && x >= 0 && y >= 0 && z >= 0 && x < int(g.W) && y < int(g.H) && z < int(g.D);
+      const uint32_t id = in ? vidx(g, uint32_t(x), uint32_t(y), uint32_t(z)) : 0u;
+      vr[j] = in ? rhs[id] : 0.f;
+      vp[j] = (!ZERO && in) ? src[id] : 0.f;
+    }
+    const uint64_t nw = nbhd_fetch(g, lb, bx, by, bz);
+    __syncthreads();  // the previous brick's words and tiles are no longer read
+    nbhd_store(nw, nb);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < IT; ++j) {
+      const int q = lane + j * 64;
+      if (q < N) {
+        const bool on = nb_on<3>(nb, off_x + q % S, off_y + (q / S) % S, off_z + q / (S * S));
+        tr[w][q] = on ? vr[j] : 0.f;
+        ta[w][q] = on ? vp[j] : 0.f;
+      }
+    }
+    __syncthreads();
+    const float om = C0->omega;  // the scenes of a batch share the V-cycle settings
+    const float* A = ta[w];
+    if constexpr (NS == 2) {
+      tile_sweep_k<3, S>(nb, off_x, off_y, off_z, ta[w], tr[w], tb[w], 1, 6, lane, 64, om);  // the octant +- 1
+      __syncthreads();
+      A = tb[w];
+    }
+    // the octant itself, one voxel per lane
+    const int tx = NS + (lane & 3), ty = NS + ((lane >> 2) & 3), tz = NS + (lane >> 4);
+    const int i = tx + S * (ty + S * tz);
+    float out = 0.f;
+    if (nb_on<3>(nb, off_x + tx, off_y + ty, off_z + tz)) {
+      const float sum = A[i + 1] + A[i - 1] + A[i + S] + A[i - S] + A[i + S * S] + A[i - S * S];
+      const float pj = (sum - tr[w][i]) / 6.f;
+      out = mixf(A[i], pj, om);
+    }
+    const int lx = sx + (lane & 3), ly = sy + ((lane >> 2) & 3), lz = sz + (lane >> 4);
+    if (bx * 8 + lx < int(g.W) && by * 8 + ly < int(g.H) && bz * 8 + lz < int(g.D))
+      dst[(b << 9) | uint32_t((lz << 6) | (ly << 3) | lx)] = out;
+  }
+}
+template <int NS, bool ZERO, bool MS>
+__global__ __launch_bounds__(512) void k_sweep_t16(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
+                                                   LvBits lb, const StepConst* __restrict__ C0, const float* __restrict__ rhs,
+                                                   const float* __restrict__ src, float* __restrict__ dst) {
+  FX_SIM_PRIORITY();
+  constexpr int S = 16 + 2 * NS, N = S * S * S, IT = (N + 511) / 512;
+  __shared__ float tr[N], ta[N], tb[NS > 1 ? N : 1];
+  __shared__ uint64_t nb[512];
+  const int tid = int(threadIdx.x);
+  constexpr int off = 8 - NS;  // tile voxel 0 in the 4^3 neighbourhood that starts one brick before the group
+  GridCache<MS> gk;
+  FX_BRICK_LOOP(list, count) {
+    const uint32_t b = list[fx_k];
+    const Grid g = gk.at(G, b);
+    int bx, by, bz;
+    brick_coords(g, b, bx, by, bz);
+    const int ax = bx & ~1, ay = by & ~1, az = bz & ~1;
+    // the group's listed bricks (bit m: brick (ax + (m & 1), ay + (m >> 1 & 1), az + (m >> 2)));
+    // the lowest-index one takes the group
+    uint32_t listed = 0;
+    bool lead = true;
+    for (int m = 0; m < 8; ++m) {
+      const int nx = ax + (m & 1), ny = ay + ((m >> 1) & 1), nz = az + (m >> 2);
+      if (nx >= int(g.BX) || ny >= int(g.BY) || nz >= int(g.BZ)) continue;
+      const uint32_t nbk = g.base + uint32_t(nx) + g.BX * (uint32_t(ny) + g.BY * uint32_t(nz));
+      if (lb.page && lb.page[nbk] < 0) continue;
+      listed |= 1u << m;
+      if (nbk < b) lead = false;
+    }
+    if (!lead) continue;
+    const int ox = ax * 8 - NS, oy = ay * 8 - NS, oz = az * 8 - NS;
+    float vr[IT], vp[IT];
+#pragma unroll
+    for (int j = 0; j < IT; ++j) {
+      const int q = tid + j * 512;
+      const int x = ox + q % S, y = oy + (q / S) % S, z = oz + q / (S * S);
+      const bool in = q < N && x >= 0 && y >= 0 && z >= 0 && x < int(g.W) && y < int(g.H) && z < int(g.D);
+      const uint32_t id = in ? vidx(g, uint32_t(x), uint32_t(y), uint32_t(z)) : 0u;
+      vr[j] = in ? rhs[id] : 0.f;
+      vp[j] = (!ZERO && in) ? src[id] : 0.f;
+    }
+    const uint64_t nw = nbk_word<4>(g, lb, ax - 1, ay - 1, az - 1, tid);
+    __syncthreads();  // the previous group's words and tiles are no longer read
+    nb[tid] = nw;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < IT; ++j) {
+      const int q = tid + j * 512;
+      if (q < N) {
+        const bool on = nb_on<4>(nb, off + q % S, off + (q / S) % S, off + q / (S * S));
+        tr[q] = on ? vr[j] : 0.f;
+        ta[q] = on ? vp[j] : 0.f;
+      }
+    }
+    __syncthreads();
+    const float om = C0->omega;  // the scenes of a batch share the V-cycle settings
+    const float* A = ta;
+    if constexpr (NS == 2) {
+      tile_sweep_k<4, S>(nb, off, off, off, ta, tr, tb, 1, 18, tid, 512, om);  // the group +- 1
+      __syncthreads();
+      A = tb;
+    }
+    // the group's listed bricks, 8 voxels per thread
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int q = tid + j * 512;
+      const int cx = q & 15, cy = (q >> 4) & 15, cz = q >> 8;
+      const int m = (cx >> 3) | ((cy >> 3) << 1) | ((cz >> 3) << 2);
+      if (!((listed >> m) & 1u)) continue;
+      const int tx = NS + cx, ty = NS + cy, tz = NS + cz;
+      const int i = tx + S * (ty + S * tz);
+      float out = 0.f;
+      if (nb_on<4>(nb, off + tx, off + ty, off + tz)) {
+        const float sum = A[i + 1] + A[i - 1] + A[i + S] + A[i - S] + A[i + S * S] + A[i - S * S];
+        const float pj = (sum - tr[i]) / 6.f;
+        out = mixf(A[i], pj, om);
+      }
+      if (ax * 8 + cx < int(g.W) && ay * 8 + cy < int(g.H) && az * 8 + cz < int(g.D)) {
+        const uint32_t gb = g.base + uint32_t(ax + (cx >> 3)) + g.BX * (uint32_t(ay + (cy >> 3)) + g.BY * uint32_t(az + (cz >> 3)));
+        dst[(gb << 9) | uint32_t(((cz & 7) << 6) | ((cy & 7) << 3) | (cx & 7))] = out;
+      }
+    }
+  }
+}
+
 // SH/project-velocity.wgsl.ts:21-43, on float4 brick rows.
 template <bool MS>
 __global__ __launch_bounds__(512) void k_projection(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G, Vec3 us,
@@ -3665,6 +3862,20 @@ void launch_sweep_quad(hipStream_t s, int blocks, const uint32_t* list, const ui
                        const float* rhs, const float* src, float* dst) {
   if (src) FX_LAUNCH_MS(g, (k_sweepn<4, false, true>), (k_sweepn<4, false, false>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
   else FX_LAUNCH_MS(g, (k_sweepn<4, true, true>), (k_sweepn<4, true, false>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);
+}
+void launch_sweep_tile(hipStream_t s, int blocks, int tile, int ns, const uint32_t* list, const uint32_t* count, Grid g, LvBits b,
+                       const StepConst* c, const float* rhs, const float* src, float* dst) {
+#define FX_TILE_LAUNCH(K, NS)                                                                                                   \
+  do {                                                                                                                          \
+    if (src) FX_LAUNCH_MS(g, (K<NS, false, true>), (K<NS, false, false>), blocks, 512, s, list, count, g, b, c, rhs, src, dst); \
+    else FX_LAUNCH_MS(g, (K<NS, true, true>), (K<NS, true, false>), blocks, 512, s, list, count, g, b, c, rhs, src, dst);      \
+  } while (0)
+  if (tile == 4) {
+    if (ns == 2) FX_TILE_LAUNCH(k_sweep_t4, 2); else FX_TILE_LAUNCH(k_sweep_t4, 1);
+  } else {
+    if (ns == 2) FX_TILE_LAUNCH(k_sweep_t16, 2); else FX_TILE_LAUNCH(k_sweep_t16, 1);
+  }
+#undef FX_TILE_LAUNCH
 }
 void launch_residual(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const float* rhs,
                      const float* p, float* r, double* norm2) {

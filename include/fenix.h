@@ -489,14 +489,20 @@ int fx_read_stage_times(fx_sim* sim, double* total_ms, uint32_t* calls, size_t n
  * Bits 24-25 of down_split: sweeps per launch of the coarsest level when its
  * active voxels exceed the one-workgroup smoother's capacity (0: one, 1: two,
  * 2: four, in LDS tiles of halo 2 or 4).
+ * Bits 20 + 2l .. 21 + 2l of up_split: the compute tile of the split sweeps of
+ * level l (both legs): 0 the 8^3 brick, 1 its 4^3 octants (one wave each), 2 a
+ * 2x2x2 group of bricks (16^3, one workgroup); the pair bit of the leg gives the
+ * halo (1 or 2 sweeps per launch, staged in LDS).
  * Every plan gives bit-identical results; only speed differs. */
 int fx_set_vcycle_plan(fx_sim* sim, uint32_t down_split, uint32_t up_split);
 int fx_get_vcycle_plan(fx_sim* sim, uint32_t* down_split, uint32_t* up_split);
 
 /* Autotuner: times each level's leg plans (fused, split, split with paired
- * sweeps, for the down and the up leg; greedily, finest level first; then the
- * coarsest level's sweeps per launch when it runs multi-workgroup launches; the
- * bit 16 + l legs are not timed, only pinned by fx_set_vcycle_plan)
+ * sweeps, for the down and the up leg; greedily, finest level first), then, where
+ * a leg of the level is split, its sweeps' compute tile (4^3, 8^3 or 16^3, with
+ * the chosen halo of 1 or 2); then the coarsest level's sweeps per launch when it
+ * runs multi-workgroup launches; the bit 16 + l legs are not timed, only pinned by
+ * fx_set_vcycle_plan)
  * with `repeats` graph-captured pressure solves of the current state and keeps
  * the fastest; returns the chosen masks. Solves start from p = 0 and every plan
  * is bit-identical, so the simulation's results do not depend on the outcome.

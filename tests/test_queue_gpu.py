@@ -61,8 +61,9 @@ def test_queue_batch_equals_scenes_alone(precision):
         assert sc.getRuntimeParams()["buoyancy"] == pytest.approx(d.runtime.buoyancy)
     total = 0
     for k in range(6):
-        if k == 3:  # a plan change mid-run (results never depend on the plan)
-            batch.set_vcycle_plan(0x3, 0x7)
+        if k == 3:  # a plan change mid-run (results never depend on the plan): the batch's split
+            # sweeps on 16^3 brick groups two per launch (level 0 down) and 4^3 octants (level 1)
+            batch.set_vcycle_plan(0x103, 0x7 | (2 << 20) | (1 << 22))
             for sim in alone:
                 sim.set_vcycle_plan(0x3, 0x7)
         batch.step(k * DT, DT)
