@@ -2965,11 +2965,15 @@ __global__ __launch_bounds__(512) void k_pack_render(const uint32_t* __restrict_
                                                      const uint32_t* __restrict__ rlist, const uint32_t* __restrict__ rcount,
                                                      float4* __restrict__ RB, const uint32_t* __restrict__ flags, uint32_t nb,
                                                      uint32_t* __restrict__ bits) {
-  // the brick flags packed 32 per word for the raymarch's empty-brick test
-  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w * 32u < nb; w += gridDim.x * blockDim.x) {
-    uint32_t v = 0;
-    for (uint32_t k = 0; k < 32 && w * 32 + k < nb; ++k) v |= (flags[w * 32 + k] != 0u ? 1u : 0u) << k;
-    bits[w] = v;
+  // the brick flags packed 32 per word for the raymarch's empty-brick test: each
+  // wave packs 64 flags (two words) with one coalesced load and one ballot
+  {
+    const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6, nw = (nb + 31u) / 32u;
+    for (uint32_t v = blockIdx.x * wpb + (threadIdx.x >> 6); v * 64u < nb; v += gridDim.x * wpb) {
+      const uint32_t f = v * 64u + lane;
+      const uint64_t m = __ballot(f < nb && flags[f] != 0u);
+      if (lane < 2u && v * 2u + lane < nw) bits[v * 2u + lane] = uint32_t(m >> (32u * lane));
+    }
   }
   FX_BRICK_LOOP(list, count) {
     const uint32_t i = (list[fx_k] << 9) | threadIdx.x;
