@@ -186,11 +186,10 @@ __device__ __forceinline__ Nbr6 nbr6(const Grid& g, uint32_t x, uint32_t y, uint
 __device__ __forceinline__ bool on_boundary(const Grid& g, uint32_t x, uint32_t y, uint32_t z) {
   return x == 0 || y == 0 || z == 0 || x == g.W - 1 || y == g.H - 1 || z == g.D - 1;
 }
-// isActiveCoord, src/engine/simulation/shaders/common/active-brick-mask.wgsl.ts:13-18
-__device__ __forceinline__ bool flag_on(const Grid& g, const uint32_t* flags, uint32_t x, uint32_t y, uint32_t z) {
-  uint32_t bx = min(x >> 3, g.BX - 1), by = min(y >> 3, g.BY - 1), bz = min(z >> 3, g.BZ - 1);
-  return flags[g.base + bx + g.BX * (by + g.BY * bz)] != 0u;
-}
+// isActiveCoord (src/engine/simulation/shaders/common/active-brick-mask.wgsl.ts:13-18)
+// clamps the voxel's brick coordinates to the grid; for an in-domain voxel of
+// brick b that brick is b itself, so the kernels read flags[b], one wave-uniform
+// (scalar) load per brick.
 
 // Voxel owned by this thread inside brick b.
 struct Vox {
@@ -1172,7 +1171,10 @@ __device__ __forceinline__ void advect_s_brick(const Grid& g, uint32_t b, float4
   const Vox v = vox_of(g, b);
   if (!v.in) return;
   const uint32_t i = v.i;
-  if (!flag_on(g, flags, v.x, v.y, v.z)) { ST::st(D.d, i, 0.f); ST::st(D.t, i, 0.f); ST::st(D.f, i, 0.f); D.r[i] = 0.f; return; }
+  // isActiveCoord of an in-domain voxel of brick b is brick b's flag: one scalar
+  // load for the wave (a per-lane load's wait would also wait for the next
+  // brick's velocity, loaded ahead)
+  if (flags[b] == 0u) { ST::st(D.d, i, 0.f); ST::st(D.t, i, 0.f); ST::st(D.f, i, 0.f); D.r[i] = 0.f; return; }
   F3 coord = f3(float(v.x) + 0.5f, float(v.y) + 0.5f, float(v.z) + 0.5f);
   F3 half = f3(0.5f, 0.5f, 0.5f);
   F3 velocity = f3(uv.x, uv.y, uv.z);
@@ -1260,7 +1262,7 @@ __global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__
     Vox v = vox_of(g, b);
     if (!v.in) continue;
     const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
-    if (!flag_on(g, flags, x, y, z)) {
+    if (flags[b] == 0u) {  // isActiveCoord: brick b's flag for its in-domain voxels
       // copied: the stored values are representable, so storing them is exact
       const float cd = ST::ld(S.d, i), ct = ST::ld(S.t, i), cf = ST::ld(S.f, i), cr = S.r[i];
       ST::st(D.d, i, cd); ST::st(D.t, i, ct); ST::st(D.f, i, cf); D.r[i] = cr;
@@ -1313,7 +1315,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     Vox v = vox_of(g, b);
     if (!v.in) continue;
     const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
-    if (!flag_on(g, flags, x, y, z)) continue;
+    if (flags[b] == 0u) continue;  // isActiveCoord: brick b's flag for its in-domain voxels
     auto hot = [&](uint32_t k) { return clampf(ST::ld(S.t, k) * 1.f + S.r[k] * 0.85f, 0.f, 1.f); };
     auto rd = [&](uint32_t k) { return ST::ld(S.d, k); };
     const Nbr6 nb = nbr6(g, x, y, z, i);
