@@ -209,6 +209,11 @@ __device__ __forceinline__ Vox vox_of(const Grid& g, uint32_t b) {
   return v;
 }
 
+// Scene-local brick coordinates of brick b.
+__device__ __forceinline__ void brick_coords(const Grid& g, uint32_t b, int& bx, int& by, int& bz) {
+  const uint32_t lb = b - g.base;
+  bx = int(lb % g.BX); by = int((lb / g.BX) % g.BY); bz = int(lb / (g.BX * g.BY));
+}
 // Trilinear sampler, src/engine/simulation/shaders/common/sampling-scalar.wgsl.ts:11-34.
 struct Tri {
   uint32_t i[8];
@@ -1034,65 +1039,99 @@ __global__ __launch_bounds__(512) void k_zero_bricks(const uint32_t* __restrict_
 // SH/advect-velocity-semilagrangian.wgsl.ts:23-41
 // sq (optional): also packs the scalar gather mirror (d, T, f, r) of set S for
 // every voxel of the listed bricks, which the scalar advection reads next.
-// One brick of the velocity advection, given the voxel's own velocity (loaded
-// one brick ahead by the caller). uq is the float4 gather mirror of the
-// advected velocity that the scalar advection samples.
-__device__ __forceinline__ void advect_u_brick(const Grid& g, uint32_t b, float ux, float uy, float uz, float dt, const Vec3& u,
-                                               const Vec3& us, float4* __restrict__ uq) {
-  const Vox v = vox_of(g, b);
-  if (v.in) {
+// SH/advect-velocity-semilagrangian.wgsl.ts. The velocity of the brick +- 2 voxels
+// is staged in LDS (one 12^3 tile per component, loaded a brick ahead), and the
+// eight trilinear corners of each backtrace are read there: a 12^3 tile is 3.4
+// bricks of coalesced row loads, where 24 scattered gathers per voxel pulled far
+// more cache lines from L2. A backtrace whose corners leave the tile (|u dt| of 2
+// voxels or more along an axis) gathers from memory instead. Corner coordinates,
+// fractions and blend order are those of tri / tri_eval, so results are unchanged.
+constexpr int kAdvH = 2, kAdvS = 8 + 2 * kAdvH, kAdvN = kAdvS * kAdvS * kAdvS, kAdvIT = (kAdvN + 511) / 512;
+// At most 64 VGPRs, so that three workgroups fit on a CU beside a raymarch workgroup.
+template <bool MS>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_advect_velocity(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
+                                                         const StepConst* __restrict__ C0, Vec3 u, Vec3 us, float4* __restrict__ uq) {
+  FX_SIM_PRIORITY();
+  __shared__ float tux[kAdvN], tuy[kAdvN], tuz[kAdvN];
+  GridCache<MS> gk;
+  const BrickRange br = brick_range(*count);
+  const int tid = int(threadIdx.x);
+  float rx[kAdvIT], ry[kAdvIT], rz[kAdvIT];
+  uint32_t bn = 0;
+  Grid gn = G;
+  // the tile loads of brick list[k] into registers (0 outside the domain)
+  auto fetch = [&](uint32_t k) {
+    bn = list[k];
+    gn = gk.at(G, bn);
+    int bx, by, bz;
+    brick_coords(gn, bn, bx, by, bz);
+    const int ox = bx * 8 - kAdvH, oy = by * 8 - kAdvH, oz = bz * 8 - kAdvH;
+#pragma unroll
+    for (int j = 0; j < kAdvIT; ++j) {
+      const int q = tid + j * 512;
+      const int x = ox + q % kAdvS, y = oy + (q / kAdvS) % kAdvS, z = oz + q / (kAdvS * kAdvS);
+      const bool in = q < kAdvN && x >= 0 && y >= 0 && z >= 0 && x < int(gn.W) && y < int(gn.H) && z < int(gn.D);
+      const uint32_t id = in ? vidx(gn, uint32_t(x), uint32_t(y), uint32_t(z)) : 0u;
+      rx[j] = in ? u.x[id] : 0.f;
+      ry[j] = in ? u.y[id] : 0.f;
+      rz[j] = in ? u.z[id] : 0.f;
+    }
+  };
+  if (br.k < br.end) fetch(br.k);
+  for (uint32_t k = br.k; k < br.end; k += br.step) {
+    const uint32_t b = bn;
+    const Grid g = gn;
+    __syncthreads();  // the previous brick's tiles are no longer read
+#pragma unroll
+    for (int j = 0; j < kAdvIT; ++j) {
+      const int q = tid + j * 512;
+      if (q < kAdvN) { tux[q] = rx[j]; tuy[q] = ry[j]; tuz[q] = rz[j]; }
+    }
+    __syncthreads();
+    if (k + br.step < br.end) fetch(k + br.step);
+    const float dt = C0[g.scene].p.delta_time;
+    const Vox v = vox_of(g, b);
+    if (!v.in) continue;
     if (on_boundary(g, v.x, v.y, v.z)) {
       us.x[v.i] = 0.f; us.y[v.i] = 0.f; us.z[v.i] = 0.f;
       uq[v.i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      continue;
+    }
+    int bx, by, bz;
+    brick_coords(g, b, bx, by, bz);
+    const int ox = bx * 8 - kAdvH, oy = by * 8 - kAdvH, oz = bz * 8 - kAdvH;
+    const int self = (int(v.x) - ox) + kAdvS * ((int(v.y) - oy) + kAdvS * (int(v.z) - oz));
+    F3 coord = f3(float(v.x) + 0.5f, float(v.y) + 0.5f, float(v.z) + 0.5f);
+    F3 half = f3(0.5f, 0.5f, 0.5f);
+    // sampleVelocity(coord - 0.5) lands on the voxel with all fractions 0; the
+    // trilinear blend a*(1-0) + b*0 returns a exactly for finite fields, so the
+    // voxel's own value is read instead of 8 corners per component.
+    F3 back = coord - f3(tux[self], tuy[self], tuz[self]) * dt;
+    const F3 p = back - half;
+    // tri's corners: the clamped sample's floor and the next voxel (itself at the last one)
+    const float cx = clampf(p.x, 0.f, float(g.W - 1)), cy = clampf(p.y, 0.f, float(g.H - 1)), cz = clampf(p.z, 0.f, float(g.D - 1));
+    const int x0 = int(floorf(cx)), y0 = int(floorf(cy)), z0 = int(floorf(cz));
+    const int x1 = x0 + 1 < int(g.W) ? x0 + 1 : x0, y1 = y0 + 1 < int(g.H) ? y0 + 1 : y0, z1 = z0 + 1 < int(g.D) ? z0 + 1 : z0;
+    const int ax = x0 - ox, ay = y0 - oy, az = z0 - oz, bxx = x1 - ox, byy = y1 - oy, bzz = z1 - oz;
+    F3 smp;
+    if (ax >= 0 && ay >= 0 && az >= 0 && bxx < kAdvS && byy < kAdvS && bzz < kAdvS) {
+      const float fx = fractf(cx), fy = fractf(cy), fz = fractf(cz);
+      const int i0 = ax + kAdvS * (ay + kAdvS * az), i1 = bxx + kAdvS * (ay + kAdvS * az);
+      const int i2 = ax + kAdvS * (byy + kAdvS * az), i3 = bxx + kAdvS * (byy + kAdvS * az);
+      const int i4 = ax + kAdvS * (ay + kAdvS * bzz), i5 = bxx + kAdvS * (ay + kAdvS * bzz);
+      const int i6 = ax + kAdvS * (byy + kAdvS * bzz), i7 = bxx + kAdvS * (byy + kAdvS * bzz);
+      auto blend = [&](const float* f) {
+        const float x00 = mixf(f[i0], f[i1], fx), x10 = mixf(f[i2], f[i3], fx);
+        const float x01 = mixf(f[i4], f[i5], fx), x11 = mixf(f[i6], f[i7], fx);
+        return mixf(mixf(x00, x10, fy), mixf(x01, x11, fy), fz);
+      };
+      smp = F3{blend(tux), blend(tuy), blend(tuz)};
     } else {
-      F3 coord = f3(float(v.x) + 0.5f, float(v.y) + 0.5f, float(v.z) + 0.5f);
-      F3 half = f3(0.5f, 0.5f, 0.5f);
-      // sampleVelocity(coord - 0.5) lands on the voxel with all fractions 0; the
-      // trilinear blend a*(1-0) + b*0 returns a exactly for finite fields, so the
-      // voxel's own value is read instead of 8 corners per component.
-      F3 back = coord - f3(ux, uy, uz) * dt;
-      F3 adv = sample_u(g, u, back - half) * (1.f - dt * 0.025f);
-      us.x[v.i] = adv.x; us.y[v.i] = adv.y; us.z[v.i] = adv.z;
-      uq[v.i] = make_float4(adv.x, adv.y, adv.z, 0.f);
+      smp = sample_u(g, u, p);
     }
-  }
-}
-template <bool MS>
-__global__ __launch_bounds__(512) void k_advect_velocity(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
-                                                         const StepConst* __restrict__ C0, Vec3 u, Vec3 us, float4* __restrict__ uq) {
-  FX_SIM_PRIORITY();
-  GridCache<MS> gk;
-  const BrickRange br = brick_range(*count);
-  if (br.k >= br.end) return;
-  // The voxel's own velocity is loaded one brick ahead: the backtrace's gathers
-  // depend on it, so loading it beside the previous brick's gathers leaves one
-  // memory round trip per brick instead of two. Two register sets take turns
-  // (the loop is unrolled twice), so the loaded values are never copied, which
-  // would wait for the loads. Every element of a listed brick lies in the
-  // allocation, also outside the domain.
-  const uint32_t t = threadIdx.x, last = br.end - 1u;
-  uint32_t k = br.k, b0 = list[k];
-  float x0 = u.x[(b0 << 9) | t], y0 = u.y[(b0 << 9) | t], z0 = u.z[(b0 << 9) | t];
-  for (;;) {
-    // the next brick (the current one again at the end of the walk: a reread
-    // that hits the cache, with no branch around the loads)
-    const uint32_t k1 = k + br.step;
-    const uint32_t b1 = list[min(k1, last)];
-    const float x1 = u.x[(b1 << 9) | t], y1 = u.y[(b1 << 9) | t], z1 = u.z[(b1 << 9) | t];
-    {
-      const Grid g = gk.at(G, b0);
-      advect_u_brick(g, b0, x0, y0, z0, C0[g.scene].p.delta_time, u, us, uq);
-    }
-    if (k1 >= br.end) break;
-    const uint32_t k2 = k1 + br.step;
-    b0 = list[min(k2, last)];
-    x0 = u.x[(b0 << 9) | t]; y0 = u.y[(b0 << 9) | t]; z0 = u.z[(b0 << 9) | t];
-    {
-      const Grid g = gk.at(G, b1);
-      advect_u_brick(g, b1, x1, y1, z1, C0[g.scene].p.delta_time, u, us, uq);
-    }
-    if (k2 >= br.end) break;
-    k = k2;
+    F3 adv = smp * (1.f - dt * 0.025f);
+    us.x[v.i] = adv.x; us.y[v.i] = adv.y; us.z[v.i] = adv.z;
+    uq[v.i] = make_float4(adv.x, adv.y, adv.z, 0.f);
   }
 }
 
@@ -1906,10 +1945,6 @@ __device__ __forceinline__ void tile_sweep(const uint64_t* nb, const float* A, c
     const float pj = (sum - rhs[i]) / 6.f;
     B[i] = mixf(A[i], pj, om);
   }
-}
-__device__ __forceinline__ void brick_coords(const Grid& g, uint32_t b, int& bx, int& by, int& bz) {
-  const uint32_t lb = b - g.base;
-  bx = int(lb % g.BX); by = int((lb / g.BX) % g.BY); bz = int(lb / (g.BX * g.BY));
 }
 
 // Pre-smoothing (PRE sweeps), residual, FP64 norm and (1,3,3,1)/8 x 4 restriction.
