@@ -168,16 +168,6 @@ struct fx_sim {
   hipEvent_t rl_ev[kRenderLanes] = {};
   bool rl_pending[kRenderLanes] = {};
   void* d_render = nullptr;      // float4 per voxel
-  // Gather mirrors of the scalar advection (float4 per voxel): the scalars of
-  // set 0 and the advected velocity. The scalar mirror is written where set 0 is:
-  // by the combustion pass over the listed bricks (for the next step) and by the
-  // injection over its bricks; the retirement zeroes it with the fields. After set
-  // 0 was changed outside fx_step (sq_valid false) the next step repacks it over
-  // every brick first. A pass run alone (fx_run_pass) packs both mirrors over
-  // every brick.
-  void* d_sq = nullptr;
-  void* d_uq = nullptr;
-  bool sq_valid = false;
   bool dense_tables = false;  // dense V-cycle: level activity words and coarsest table built
   // Bloom: three half-resolution rgba16f images (bright, horizontal and vertical
   // blur) for the last size, and staging for the host-memory variant.
@@ -444,8 +434,6 @@ void pass_activate_render(fx_sim* s) {
       R.first[l] = nb;
       if (l == 0) {
         for (const Buf& b : fine_buffers(s)) { R.esz[nb] = uint8_t(b.esz); R.b[nb++] = b.p; }
-        // the float4 gather mirrors, which the step packs for listed bricks only
-        for (void* m : {s->d_sq, s->d_uq}) { R.esz[nb] = 16; R.b[nb++] = m; }
       } else {
         for (float* p : {s->lv[l].div, s->lv[l].p[0], s->lv[l].p[1]}) { R.esz[nb] = 4; R.b[nb++] = p; }
       }
@@ -468,11 +456,11 @@ void pass_inject(fx_sim* s, int parts) {
   StageTimer timer(s, parts == 2 ? FX_PASS_INJECT_VELOCITY : parts == 4 ? FX_PASS_INJECT_IGNITER : FX_PASS_INJECT_SCALAR);
   if (s->sparse)
     launch_inject(s->stream, grid_for(s, s->nb), s->d_cand, &s->d_const->inj_count, s->g, s->d_const, s->d_src, set_of(s, 0),
-                  vec_of(s->u), s->touched, parts, s->in_step ? s->d_sq : nullptr, s->in_step ? s->flagsInj : nullptr,
+                  vec_of(s->u), s->touched, parts, s->in_step ? s->flagsInj : nullptr,
                   uint32_t(s->nb));
   else
     launch_inject(s->stream, grid_for(s, s->nb), s->identity, s->identity_count, s->g, s->d_const, s->d_src, set_of(s, 0),
-                  vec_of(s->u), nullptr, parts, s->in_step ? s->d_sq : nullptr, s->in_step ? s->flagsInj : nullptr,
+                  vec_of(s->u), nullptr, parts, s->in_step ? s->flagsInj : nullptr,
                   uint32_t(s->nb));
 }
 void zero_level_p(fx_sim* s, int l, int which) {
@@ -681,19 +669,14 @@ void run_stage(fx_sim* s, int pass) {
   const int blocks = grid_for(s, s->nb);
   switch (pass) {
     case FX_PASS_ADVECT_VELOCITY:
-      launch_advect_velocity(s->stream, blocks, list, count, s->g, s->d_const, vec_of(s->u), vec_of(s->us), s->d_uq);
+      launch_advect_velocity(s->stream, blocks, list, count, s->g, s->d_const, vec_of(s->u), vec_of(s->us));
       break;
-    case FX_PASS_ADVECT_SCALARS: {
-      if (!s->in_step)  // a pass run alone: both mirrors over every brick
-        launch_pack_gather(s->stream, grid_for(s, s->nb), s->identity, s->identity_count, set_of(s, 0), vec_of(s->us), nullptr,
-                           nullptr, s->d_sq, s->d_uq);
-      launch_advect_scalars(s->stream, blocks, list, count, s->g, s->d_const, s->mode, s->d_sq, s->d_uq, set_of(s, 1), s->flagsInj);
+    case FX_PASS_ADVECT_SCALARS:
+      launch_advect_scalars(s->stream, blocks, list, count, s->g, s->d_const, s->mode, set_of(s, 0), vec_of(s->us), set_of(s, 1),
+                            s->flagsInj);
       break;
-    }
     case FX_PASS_COMBUSTION:
-      // inside a step it also writes set 0's scalar gather mirror for the next step
-      launch_combustion(s->stream, blocks, list, count, s->g, s->d_const, set_of(s, 1), set_of(s, 0), s->flagsRender,
-                        s->in_step ? s->d_sq : nullptr);
+      launch_combustion(s->stream, blocks, list, count, s->g, s->d_const, set_of(s, 1), set_of(s, 0), s->flagsRender);
       break;
     case FX_PASS_BUOYANCY:
       launch_buoyancy(s->stream, blocks, list, count, s->g, s->d_const, set_of(s, 0), vec_of(s->us), s->flagsRender);
@@ -1305,13 +1288,11 @@ int create_impl(const fx_create_desc* descs, uint32_t nsc, bool multi, fx_sim** 
           hipEventCreateWithFlags(&s->rl_ev[r], hipEventDisableTiming) != hipSuccess)
         return bail(fail(s, FX_ERR_DEVICE, "stream creation failed"));
   }
-  for (void** mirror : {&s->d_render, &s->d_sq, &s->d_uq}) {
+  {
     float* q = nullptr;
     if (dalloc(s, &q, nsv * 4) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
-    *mirror = q;
+    s->d_render = q;
   }
-  launch_clear_f32(s->stream, static_cast<float*>(s->d_sq), nsv * 4);
-  launch_clear_f32(s->stream, static_cast<float*>(s->d_uq), nsv * 4);
   s->src_cap = size_t(FX_FLOATS_PER_EMITTER) * nsc;
   if (dalloc(s, &s->d_src, s->src_cap) != FX_OK) return bail(FX_ERR_OUT_OF_MEMORY);
   for (auto& slot : s->slots) {
@@ -1369,7 +1350,7 @@ void fx_destroy(fx_sim* s) {
   F(s->wmag); F(s->cmag); F(s->umag);
   F(s->flagsInj);
   if (!s->sparse) F(s->flagsRender);
-  F(s->touched); F(s->d_bits); F(s->d_rqueue); F(s->d_render); F(s->d_sq); F(s->d_uq); F(s->d_bloom); F(s->d_bloom_io); F(s->identity); F(s->identity_count); F(s->d_const); F(s->d_src); F(s->d_norms);
+  F(s->touched); F(s->d_bits); F(s->d_rqueue); F(s->d_render); F(s->d_bloom); F(s->d_bloom_io); F(s->identity); F(s->identity_count); F(s->d_const); F(s->d_src); F(s->d_norms);
   for (size_t l = 0; l < s->lv.size(); ++l) {
     Level& L = s->lv[l];
     F(L.div); F(L.p[0]); F(L.p[1]); F(L.bits); F(L.nbr); F(L.tab); F(L.tab_n); F(L.L.list); F(L.L.count); F(L.L.page); F(L.L.retire); F(L.L.retire_count);
@@ -1404,14 +1385,11 @@ int fx_reset(fx_sim* s) {  // ORCH:206-226
   if (!s) return fail(nullptr, FX_ERR_INVALID_ARG, "null simulation");
   HIP_OK(s, hipSetDevice(s->device));
   snapshot_stale(s);
-  s->sq_valid = false;
   s->curSet = 0; s->stepIndex = 0; s->lastPressure = 0; s->initialized = false;
   s->startS = s->lastS;
   s->pending.clear();
   const size_t nsv = s->nb * 512;
   for (const Buf& b : fine_buffers(s)) launch_clear_u32(s->stream, static_cast<uint32_t*>(b.p), nsv * size_t(b.esz) / 4, 0u);
-  launch_clear_f32(s->stream, static_cast<float*>(s->d_sq), nsv * 4);
-  launch_clear_f32(s->stream, static_cast<float*>(s->d_uq), nsv * 4);
   launch_clear_u32(s->stream, s->flagsInj, s->nb, 0u);
   if (!s->sparse) launch_clear_u32(s->stream, s->flagsRender, s->nb, 0u);
   launch_clear_u32(s->stream, s->touched, s->nb, 0u);
@@ -1458,13 +1436,6 @@ int fx_step(fx_sim* s, double elapsed_s, double step_s) {  // ORCH:230-305
   // workgroup; while the snapshots go unrendered no raymarch shares the CUs, and 4
   // per CU are faster (C3 batch 6.61 -> 6.46 ms per step). FENIX_SIM_BLOCKS_PER_CU fixes it.
   if (!s->blocks_env) s->blocks = s->cus * (s->render_wanted ? 3 : 4);
-  if (!s->sq_valid) {
-    // set 0 changed outside fx_step: the scalar gather mirror is repacked over
-    // every brick before the step (its passes keep it from then on)
-    const Vec3 none{nullptr, nullptr, nullptr};
-    launch_pack_gather(s->stream, grid_for(s, s->nb), s->identity, s->identity_count, set_of(s, 0), none, nullptr, nullptr, s->d_sq,
-                       s->d_uq);
-  }
   if (s->use_graph && !s->timing) {
     std::vector<int64_t> key = {vort, pres, dbg, s->mode, pres ? -1 : s->lastPressure, s->q.fine_pre_iterations,
                                 s->q.fine_post_iterations, s->q.mid_pre_iterations, s->q.mid_post_iterations,
@@ -1493,7 +1464,6 @@ int fx_step(fx_sim* s, double elapsed_s, double step_s) {  // ORCH:230-305
   s->curSet = s->curSet == 0 ? 1 : 0;
   s->lastS = elapsed_s;
   s->stepIndex += 1;
-  s->sq_valid = true;
   HIP_OK(s, hipGetLastError());
   return FX_OK;
 }
@@ -1507,7 +1477,6 @@ int fx_run_pass(fx_sim* s, int32_t pass, const fx_sim_params* params) {
   const std::vector<fx_sim_params> pv(1, *params);
   if (s->sparse) build_candidates(s, pv, cand, inj);
   snapshot_stale(s);
-  s->sq_valid = false;
   int rc = upload_step(s, pv, s->sparse ? &cand : nullptr, inj);
   if (rc != FX_OK) return rc;
   update_coarse_plan(s);
@@ -1640,7 +1609,6 @@ int fx_write_field(fx_sim* s, int32_t id, const float* host, size_t bytes) {
   if (bytes != s->nv * 4) return fail(s, FX_ERR_INVALID_ARG, "buffer size mismatch");
   HIP_OK(s, hipSetDevice(s->device));
   snapshot_stale(s);
-  s->sq_valid = false;
   std::vector<float> tmp(storage_count(s->g), 0.f);
   scatter_brickmajor(s->g, host, tmp.data());
   HIP_OK(s, hipStreamSynchronize(s->stream));
@@ -1723,8 +1691,6 @@ int fx_read_page_table(fx_sim* s, int32_t* host, size_t count) {
 
 void* fx_field_device_ptr(fx_sim* s, int32_t id) {
   if (!s || s->multi) return nullptr;
-  // the caller may write set 0's scalars through the pointer before the next step
-  if (id >= FX_FIELD_DENSITY && id <= FX_FIELD_REACTION) s->sq_valid = false;
   return field_ptr(s, id);
 }
 
@@ -1856,7 +1822,6 @@ int fx_scene_write_field(fx_sim* s, uint32_t scene, int32_t id, const float* hos
   if (bytes != size_t(g.W) * g.H * g.D * 4) return fail(s, FX_ERR_INVALID_ARG, "buffer size mismatch");
   HIP_OK(s, hipSetDevice(s->device));
   snapshot_stale(s);
-  s->sq_valid = false;
   const size_t nb = brick_count(g);
   std::vector<float> tmp(nb * 512, 0.f);
   scatter_brickmajor(g, host, tmp.data());

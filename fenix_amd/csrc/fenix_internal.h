@@ -121,10 +121,9 @@ struct Vec3 {
   float* z;
 };
 
-// sq (optional, parts 1 or 4): also writes the scalar gather mirror of set0 at
-// every injected voxel. clear (optional): clear_n words zeroed by the same launch.
+// clear (optional): clear_n words zeroed by the same launch.
 void launch_inject(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
-                   const StepConst* c, const float* src, Scalars set0, Vec3 u, uint32_t* touched, int parts, void* sq,
+                   const StepConst* c, const float* src, Scalars set0, Vec3 u, uint32_t* touched, int parts,
                    uint32_t* clear = nullptr, uint32_t clear_n = 0);
 void launch_clear_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t value);
 void launch_mark(hipStream_t s, const uint32_t* list, uint32_t n, uint32_t* touched);
@@ -213,18 +212,12 @@ void launch_coarse_setup(hipStream_t s, const uint32_t* list, const uint32_t* co
 void launch_smooth_block(hipStream_t s, const uint32_t* list, const uint32_t* count, Grid g, LvMask m, const StepConst* c,
                          const float* rhs, float* p0, float* p1, int sweeps, int weighted, int zero_start, const void* tab,
                          const uint32_t* tab_n, uint32_t scenes = 1);
-// uq: float4 mirror of us (written with it)
 void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
-                            const StepConst* c, Vec3 u, Vec3 us, void* uq);
-// Gather mirrors for the scalar advection: sq = float4 (d, T, f, r) of set0 over the
-// list, uq = float4 of us when us.x is non-null; both zeroed in the retired bricks.
-void launch_pack_gather(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Scalars set0, Vec3 us,
-                        const uint32_t* rlist, const uint32_t* rcount, void* sq, void* uq);
+                            const StepConst* c, Vec3 u, Vec3 us);
 void launch_advect_scalars(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
-                           const StepConst* c, int mode, const void* sq, const void* uq, Scalars dst, const uint32_t* flags);
-// sq (optional): also writes the scalar gather mirror of dst (the next step's set 0).
+                           const StepConst* c, int mode, Scalars set0, Vec3 us, Scalars dst, const uint32_t* flags);
 void launch_combustion(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
-                       const StepConst* c, Scalars src, Scalars dst, const uint32_t* flags, void* sq);
+                       const StepConst* c, Scalars src, Scalars dst, const uint32_t* flags);
 void launch_buoyancy(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,
                      const StepConst* c, Scalars set0, Vec3 us, const uint32_t* flags);
 void launch_vorticity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g,

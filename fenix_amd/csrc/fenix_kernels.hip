@@ -651,7 +651,7 @@ template <class ST, bool MS>
 __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
                                                 const StepConst* __restrict__ C0, const float* __restrict__ srcs0,
                                                 SS<ST> S, Vec3 u, uint32_t* touched, int parts,
-                                                float4* __restrict__ sq, uint32_t* __restrict__ clear, uint32_t clear_n) {
+                                                uint32_t* __restrict__ clear, uint32_t clear_n) {
   // the step's clear of the injected-activation flags (ORCH:271), which the next
   // launch (the activation) fills: one launch fewer than a separate fill
   if (clear)
@@ -671,9 +671,9 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
     if (!v.in) continue;
     const uint32_t i = v.i, hi = i - (g.base << 9);
     F3 pos = f3((float(v.x) + 0.5f) / float(g.W), (float(v.y) + 0.5f) / float(g.H), (float(v.z) + 0.5f) / float(g.D));
-    // the stored (decoded) density, temperature, fuel and reaction, for the
-    // scalar gather mirror
-    float dd = 0.f, tt = 0.f, ff = 0.f, rr = 0.f;
+    // the stored (decoded) temperature and fuel of the scalar part, which the
+    // igniter part reads (the values a reload returns)
+    float tt = 0.f, ff = 0.f;
     if (parts & 1) {
       float den = ST::ld(S.d, i), tem = ST::ld(S.t, i), fu = ST::ld(S.f, i);
       for (uint32_t k = 0; k < ns; ++k) {
@@ -694,9 +694,9 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
         tem = clamp01(tem + s.yt() * contribution);
         fu = clamp01(fu + s.yf() * contribution);
       }
-      const typename ST::T ed = ST::enc_sr(den, sr, s0, hi), et = ST::enc_sr(tem, sr, s1, hi), ef = ST::enc_sr(fu, sr, s2, hi);
-      S.d[i] = ed; S.t[i] = et; S.f[i] = ef;
-      dd = ST::dec(ed); tt = ST::dec(et); ff = ST::dec(ef);
+      const typename ST::T et = ST::enc_sr(tem, sr, s1, hi), ef = ST::enc_sr(fu, sr, s2, hi);
+      ST::st_sr(S.d, i, den, sr, s0, hi); S.t[i] = et; S.f[i] = ef;
+      tt = ST::dec(et); ff = ST::dec(ef);
     }
     if (parts & 2) {
       F3 vel = f3(u.x[i], u.y[i], u.z[i]);
@@ -737,7 +737,6 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
       u.x[i] = clampf(vel.x, -lim, lim); u.y[i] = clampf(vel.y, -lim, lim); u.z[i] = clampf(vel.z, -lim, lim);
     }
     if (parts & 4) {
-      // temperature and fuel as the scalar part stored them (the values a reload returns)
       float rea = S.r[i], tem = (parts & 1) ? tt : ST::ld(S.t, i);
       const float fuel = (parts & 1) ? ff : ST::ld(S.f, i);
       for (uint32_t k = 0; k < ns; ++k) {
@@ -756,14 +755,8 @@ __global__ __launch_bounds__(512) void k_inject(const uint32_t* __restrict__ lis
         rea = clamp01(rea + seedv);
         tem = clamp01(tem + intensity * weight * P.delta_time * 4.f);
       }
-      const typename ST::T et = ST::enc_sr(tem, sr, s5, hi);
-      S.r[i] = rea; S.t[i] = et;
-      rr = rea; tt = ST::dec(et);
+      S.r[i] = rea; ST::st_sr(S.t, i, tem, sr, s5, hi);
     }
-    // inside a step the injected bricks' scalar gather mirror follows set 0 (the
-    // combustion pass of the previous step wrote it for the listed bricks)
-    if (sq && (parts & 5))
-      sq[i] = make_float4((parts & 1) ? dd : ST::ld(S.d, i), tt, (parts & 1) ? ff : ST::ld(S.f, i), (parts & 4) ? rr : S.r[i]);
   }
 }
 
@@ -1036,21 +1029,20 @@ __global__ __launch_bounds__(512) void k_zero_bricks(const uint32_t* __restrict_
 }
 
 // ----------------------------------------------------------- advection --
-// SH/advect-velocity-semilagrangian.wgsl.ts:23-41
-// sq (optional): also packs the scalar gather mirror (d, T, f, r) of set S for
-// every voxel of the listed bricks, which the scalar advection reads next.
-// SH/advect-velocity-semilagrangian.wgsl.ts. The velocity of the brick +- 2 voxels
-// is staged in LDS (one 12^3 tile per component, loaded a brick ahead), and the
-// eight trilinear corners of each backtrace are read there: a 12^3 tile is 3.4
+// SH/advect-velocity-semilagrangian.wgsl.ts. The velocity of the brick +- 1 voxel
+// is staged in LDS (one 10^3 tile per component, loaded a brick ahead), and the
+// eight trilinear corners of each backtrace are read there: a 10^3 tile is 2
 // bricks of coalesced row loads, where 24 scattered gathers per voxel pulled far
-// more cache lines from L2. A backtrace whose corners leave the tile (|u dt| of 2
-// voxels or more along an axis) gathers from memory instead. Corner coordinates,
-// fractions and blend order are those of tri / tri_eval, so results are unchanged.
-constexpr int kAdvH = 2, kAdvS = 8 + 2 * kAdvH, kAdvN = kAdvS * kAdvS * kAdvS, kAdvIT = (kAdvN + 511) / 512;
+// more cache lines from L2. A backtrace whose corners leave the tile (|u dt| of a
+// voxel or more along an axis; the plume's |u dt| stays below 0.2) gathers from
+// memory instead. Corner coordinates, fractions and blend order are those of
+// tri / tri_eval, so results are unchanged. A 12^3 tile (halo 2) measured 41.9 us
+// on the 256^3 plume, the 10^3 tile 27.5 us.
+constexpr int kAdvH = 1, kAdvS = 8 + 2 * kAdvH, kAdvN = kAdvS * kAdvS * kAdvS, kAdvIT = (kAdvN + 511) / 512;
 // At most 64 VGPRs, so that three workgroups fit on a CU beside a raymarch workgroup.
 template <bool MS>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_advect_velocity(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
-                                                         const StepConst* __restrict__ C0, Vec3 u, Vec3 us, float4* __restrict__ uq) {
+                                                         const StepConst* __restrict__ C0, Vec3 u, Vec3 us) {
   FX_SIM_PRIORITY();
   __shared__ float tux[kAdvN], tuy[kAdvN], tuz[kAdvN];
   GridCache<MS> gk;
@@ -1094,7 +1086,6 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     if (!v.in) continue;
     if (on_boundary(g, v.x, v.y, v.z)) {
       us.x[v.i] = 0.f; us.y[v.i] = 0.f; us.z[v.i] = 0.f;
-      uq[v.i] = make_float4(0.f, 0.f, 0.f, 0.f);
       continue;
     }
     int bx, by, bz;
@@ -1131,7 +1122,6 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     }
     F3 adv = smp * (1.f - dt * 0.025f);
     us.x[v.i] = adv.x; us.y[v.i] = adv.y; us.z[v.i] = adv.z;
-    uq[v.i] = make_float4(adv.x, adv.y, adv.z, 0.f);
   }
 }
 
@@ -1161,12 +1151,38 @@ __device__ __forceinline__ void decay_write(const Grid& g, uint32_t i, uint32_t 
   ST::st_sr(D.f, i, clamp01(fmaxf(0.f, fu - dt * mixf(0.012f, 0.034f, tn))), sr, site[2], hi);
   D.r[i] = clamp01(fmaxf(0.f, rea - dt * mixf(0.46f, 0.82f, tn)));
 }
-// Gather mirrors for the scalar advection: the four scalars (d, T, f, r) and the
-// advected velocity as one float4 per voxel, so that each trilinear corner is one
-// 16-byte load instead of four (or three) 4-byte ones; the advection is bound by
-// the number of gather instructions, not by bytes. tri_eval4 blends each
-// component in tri_eval's order, so results are unchanged.
-__device__ __forceinline__ float4 tri_eval4(const Tri& t, const float4* __restrict__ f) {
+// SH/advect-scalar-maccormack.wgsl.ts:33-101 / SH/advect-scalar-semilagrangian.wgsl.ts:26-44.
+// The four advect<Name> calls share every position; sampleVelocity(coord-0.5)
+// and the `original` sample hit the voxel exactly (fraction 0). One workgroup per
+// brick stages the brick +- 1 voxel in LDS, loaded into registers a brick ahead:
+// set 0's scalars as stored (decoded) and the advected velocity, each as one
+// float4 per voxel, so that every trilinear corner is one 16-byte LDS read. A
+// sample whose corners leave the tile (a backtrace of a voxel or more along an
+// axis) gathers from memory instead. Corner coordinates, fractions and blend
+// order are those of tri / tri_eval.
+constexpr int kScH = 1, kScS = 8 + 2 * kScH, kScN = kScS * kScS * kScS, kScIT = (kScN + 511) / 512;
+struct TileCorners {
+  int i[8];
+  float fx, fy, fz;
+  bool in;  // every corner lies in the tile
+};
+// tri(g, p)'s corners as indices of the tile whose voxel 0 sits at (ox, oy, oz).
+__device__ __forceinline__ TileCorners tile_corners(const Grid& g, F3 p, int ox, int oy, int oz) {
+  TileCorners c;
+  const float cx = clampf(p.x, 0.f, float(g.W - 1)), cy = clampf(p.y, 0.f, float(g.H - 1)), cz = clampf(p.z, 0.f, float(g.D - 1));
+  const int x0 = int(floorf(cx)), y0 = int(floorf(cy)), z0 = int(floorf(cz));
+  const int x1 = x0 + 1 < int(g.W) ? x0 + 1 : x0, y1 = y0 + 1 < int(g.H) ? y0 + 1 : y0, z1 = z0 + 1 < int(g.D) ? z0 + 1 : z0;
+  const int ax = x0 - ox, ay = y0 - oy, az = z0 - oz, bx = x1 - ox, by = y1 - oy, bz = z1 - oz;
+  c.in = ax >= 0 && ay >= 0 && az >= 0 && bx < kScS && by < kScS && bz < kScS;
+  c.fx = fractf(cx); c.fy = fractf(cy); c.fz = fractf(cz);
+  c.i[0] = ax + kScS * (ay + kScS * az); c.i[1] = bx + kScS * (ay + kScS * az);
+  c.i[2] = ax + kScS * (by + kScS * az); c.i[3] = bx + kScS * (by + kScS * az);
+  c.i[4] = ax + kScS * (ay + kScS * bz); c.i[5] = bx + kScS * (ay + kScS * bz);
+  c.i[6] = ax + kScS * (by + kScS * bz); c.i[7] = bx + kScS * (by + kScS * bz);
+  return c;
+}
+// Each component blended in tri_eval's order.
+__device__ __forceinline__ float4 blend4(const TileCorners& t, const float4* f) {
   const float4 c0 = f[t.i[0]], c1 = f[t.i[1]], c2 = f[t.i[2]], c3 = f[t.i[3]];
   const float4 c4 = f[t.i[4]], c5 = f[t.i[5]], c6 = f[t.i[6]], c7 = f[t.i[7]];
   auto blend = [&](float a0, float a1, float a2, float a3, float a4, float a5, float a6, float a7) {
@@ -1176,103 +1192,107 @@ __device__ __forceinline__ float4 tri_eval4(const Tri& t, const float4* __restri
   return make_float4(blend(c0.x, c1.x, c2.x, c3.x, c4.x, c5.x, c6.x, c7.x), blend(c0.y, c1.y, c2.y, c3.y, c4.y, c5.y, c6.y, c7.y),
                      blend(c0.z, c1.z, c2.z, c3.z, c4.z, c5.z, c6.z, c7.z), blend(c0.w, c1.w, c2.w, c3.w, c4.w, c5.w, c6.w, c7.w));
 }
-// Packs the scalar mirror of the listed bricks (and the velocity mirror from us
-// when us.x is set), and zeroes both mirrors in the retired bricks, so that every
-// unlisted brick reads 0 there as it does in the fields (sparse zero invariant).
+// The scalars of set 0 at p from memory (corners outside the tile).
 template <class ST>
-__global__ __launch_bounds__(512) void k_pack_gather(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, SS<ST> S,
-                                                     Vec3 us, const uint32_t* __restrict__ rlist, const uint32_t* __restrict__ rcount,
-                                                     float4* __restrict__ sq, float4* __restrict__ uq) {
-  FX_BRICK_LOOP(list, count) {
-    const uint32_t i = (list[fx_k] << 9) | threadIdx.x;
-    sq[i] = make_float4(ST::ld(S.d, i), ST::ld(S.t, i), ST::ld(S.f, i), S.r[i]);
-    if (us.x) uq[i] = make_float4(us.x[i], us.y[i], us.z[i], 0.f);
-  }
-  if (!rlist) return;
-  const uint32_t nr = *rcount;
-  for (uint32_t k = blockIdx.x; k < nr; k += gridDim.x) {
-    const uint32_t i = (rlist[k] << 9) | threadIdx.x;
-    sq[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    uq[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  }
+__device__ __forceinline__ float4 sample_scalars(const Grid& g, const SS<ST>& S, F3 p) {
+  const Tri t = tri(g, p);
+  return make_float4(tri_evalP<ST>(t, S.d), tri_evalP<ST>(t, S.t), tri_evalP<ST>(t, S.f), tri_eval(t, S.r));
 }
-// SH/advect-scalar-maccormack.wgsl.ts:33-101 / SH/advect-scalar-semilagrangian.wgsl.ts:26-44.
-// The four advect<Name> calls share every position; sampleVelocity(coord-0.5)
-// and the `original` sample hit the voxel exactly (fraction 0). One brick, given
-// the voxel's own advected velocity uv (loaded one brick ahead by the caller).
-template <class ST>
-__device__ __forceinline__ void advect_s_brick(const Grid& g, uint32_t b, float4 uv, const StepConst* __restrict__ C, int mode,
-                                               const float4* __restrict__ sq, const float4* __restrict__ uq, const SS<ST>& D,
-                                               const uint32_t* __restrict__ flags) {
-  const float dt = C->p.delta_time, time = C->p.time;
-  const uint32_t sr = C->sr;
-  const uint32_t site[3] = {sr_site(C, 8), sr_site(C, 9), sr_site(C, 10)};
-  const Vox v = vox_of(g, b);
-  if (!v.in) return;
-  const uint32_t i = v.i;
-  // isActiveCoord of an in-domain voxel of brick b is brick b's flag: one scalar
-  // load for the wave (a per-lane load's wait would also wait for the next
-  // brick's velocity, loaded ahead)
-  if (flags[b] == 0u) { ST::st(D.d, i, 0.f); ST::st(D.t, i, 0.f); ST::st(D.f, i, 0.f); D.r[i] = 0.f; return; }
-  F3 coord = f3(float(v.x) + 0.5f, float(v.y) + 0.5f, float(v.z) + 0.5f);
-  F3 half = f3(0.5f, 0.5f, 0.5f);
-  F3 velocity = f3(uv.x, uv.y, uv.z);
-  F3 back = coord - velocity * dt;
-  F3 sp = back - half;
-  Tri tp = tri(g, sp);
-  if (mode == FX_ADVECT_SEMI_LAGRANGIAN) {
-    const float4 s4 = tri_eval4(tp, sq);
-    decay_write<ST>(g, i, v.y, s4.x, s4.y, s4.z, s4.w, thermal_noise(sp, time), dt, D, sr, site);
-    return;
-  }
-  const float4 f4 = tri_eval4(tp, uq);
-  F3 fwd = f3(f4.x, f4.y, f4.z);
-  F3 rec = back + fwd * dt;
-  Tri tr = tri(g, rec - half);
-  auto mac = [&](float predicted, float recovered, float original) {
-    float corrected = predicted + 0.5f * (original - recovered);
-    return clampf(corrected, fminf(original, predicted), fmaxf(original, predicted));
-  };
-  const float4 pr = tri_eval4(tp, sq), rc = tri_eval4(tr, sq), og = sq[i];
-  float out[4];
-  out[0] = mac(pr.x, rc.x, og.x);
-  out[1] = mac(pr.y, rc.y, og.y);
-  out[2] = mac(pr.z, rc.z, og.z);
-  out[3] = mac(pr.w, rc.w, og.w);
-  decay_write<ST>(g, i, v.y, out[0], out[1], out[2], out[3], thermal_noise(coord, time), dt, D, sr, site);
-}
+// At most 128 VGPRs: two workgroups per CU.
 template <class ST, bool MS>
-__global__ __launch_bounds__(512) void k_advect_scalars(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
-                                                        const StepConst* __restrict__ C0, int mode, const float4* __restrict__ sq,
-                                                        const float4* __restrict__ uq, SS<ST> D, const uint32_t* __restrict__ flags) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_advect_scalars(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
+                                                        const StepConst* __restrict__ C0, int mode, SS<ST> S, Vec3 us, SS<ST> D,
+                                                        const uint32_t* __restrict__ flags) {
   FX_SIM_PRIORITY();
+  __shared__ float4 ts[kScN], tv[kScN];
+  const int tid = int(threadIdx.x);
   GridCache<MS> gk;
   const BrickRange br = brick_range(*count);
-  if (br.k >= br.end) return;
-  // The voxel's own advected velocity is loaded one brick ahead (the first
-  // gathers depend on it), in two register sets that take turns as in
-  // k_advect_velocity: two memory round trips per brick instead of three.
-  const uint32_t t = threadIdx.x, last = br.end - 1u;
-  uint32_t k = br.k, b0 = list[k];
-  float4 v0 = uq[(b0 << 9) | t];
-  for (;;) {
-    const uint32_t k1 = k + br.step;
-    const uint32_t b1 = list[min(k1, last)];
-    const float4 v1 = uq[(b1 << 9) | t];
-    {
-      const Grid g = gk.at(G, b0);
-      advect_s_brick<ST>(g, b0, v0, C0 + g.scene, mode, sq, uq, D, flags);
+  float4 rs[kScIT];
+  float rvx[kScIT], rvy[kScIT], rvz[kScIT];
+  uint32_t bn = 0;
+  Grid gn = G;
+  // the tile loads of brick list[k] into registers (0 outside the domain)
+  auto fetch = [&](uint32_t k) {
+    bn = list[k];
+    gn = gk.at(G, bn);
+    int bx, by, bz;
+    brick_coords(gn, bn, bx, by, bz);
+    const int ox = bx * 8 - kScH, oy = by * 8 - kScH, oz = bz * 8 - kScH;
+#pragma unroll
+    for (int j = 0; j < kScIT; ++j) {
+      const int q = tid + j * 512;
+      const int x = ox + q % kScS, y = oy + (q / kScS) % kScS, z = oz + q / (kScS * kScS);
+      const bool in = q < kScN && x >= 0 && y >= 0 && z >= 0 && x < int(gn.W) && y < int(gn.H) && z < int(gn.D);
+      const uint32_t id = in ? vidx(gn, uint32_t(x), uint32_t(y), uint32_t(z)) : 0u;
+      rs[j] = in ? make_float4(ST::ld(S.d, id), ST::ld(S.t, id), ST::ld(S.f, id), S.r[id]) : make_float4(0.f, 0.f, 0.f, 0.f);
+      rvx[j] = in ? us.x[id] : 0.f;
+      rvy[j] = in ? us.y[id] : 0.f;
+      rvz[j] = in ? us.z[id] : 0.f;
     }
-    if (k1 >= br.end) break;
-    const uint32_t k2 = k1 + br.step;
-    b0 = list[min(k2, last)];
-    v0 = uq[(b0 << 9) | t];
-    {
-      const Grid g = gk.at(G, b1);
-      advect_s_brick<ST>(g, b1, v1, C0 + g.scene, mode, sq, uq, D, flags);
+  };
+  if (br.k < br.end) fetch(br.k);
+  for (uint32_t k = br.k; k < br.end; k += br.step) {
+    const uint32_t b = bn;
+    const Grid g = gn;
+    __syncthreads();  // the previous brick's tiles are no longer read
+#pragma unroll
+    for (int j = 0; j < kScIT; ++j) {
+      const int q = tid + j * 512;
+      if (q < kScN) { ts[q] = rs[j]; tv[q] = make_float4(rvx[j], rvy[j], rvz[j], 0.f); }
     }
-    if (k2 >= br.end) break;
-    k = k2;
+    __syncthreads();
+    if (k + br.step < br.end) fetch(k + br.step);
+    int bx, by, bz;
+    brick_coords(g, b, bx, by, bz);
+    const int ox = bx * 8 - kScH, oy = by * 8 - kScH, oz = bz * 8 - kScH;
+    const StepConst* __restrict__ C = C0 + g.scene;
+    const float dt = C->p.delta_time, time = C->p.time;
+    const uint32_t sr = C->sr;
+    const uint32_t site[3] = {sr_site(C, 8), sr_site(C, 9), sr_site(C, 10)};
+    const Vox v = vox_of(g, b);
+    if (!v.in) continue;
+    const uint32_t i = v.i;
+    // isActiveCoord of an in-domain voxel of brick b is brick b's flag
+    if (flags[b] == 0u) { ST::st(D.d, i, 0.f); ST::st(D.t, i, 0.f); ST::st(D.f, i, 0.f); D.r[i] = 0.f; continue; }
+    const int self = (int(v.x) - ox) + kScS * ((int(v.y) - oy) + kScS * (int(v.z) - oz));
+    F3 coord = f3(float(v.x) + 0.5f, float(v.y) + 0.5f, float(v.z) + 0.5f);
+    F3 half = f3(0.5f, 0.5f, 0.5f);
+    const float4 uv = tv[self];
+    F3 velocity = f3(uv.x, uv.y, uv.z);
+    F3 back = coord - velocity * dt;
+    F3 sp = back - half;
+    const TileCorners tp = tile_corners(g, sp, ox, oy, oz);
+    if (mode == FX_ADVECT_SEMI_LAGRANGIAN) {
+      const float4 s4 = tp.in ? blend4(tp, ts) : sample_scalars<ST>(g, S, sp);
+      decay_write<ST>(g, i, v.y, s4.x, s4.y, s4.z, s4.w, thermal_noise(sp, time), dt, D, sr, site);
+      continue;
+    }
+    F3 fwd;
+    float4 pr;
+    if (tp.in) {
+      const float4 f4 = blend4(tp, tv);
+      fwd = f3(f4.x, f4.y, f4.z);
+      pr = blend4(tp, ts);
+    } else {
+      fwd = sample_u(g, us, sp);
+      pr = sample_scalars<ST>(g, S, sp);
+    }
+    F3 rec = back + fwd * dt;
+    const F3 rp = rec - half;
+    const TileCorners tr = tile_corners(g, rp, ox, oy, oz);
+    const float4 rc = tr.in ? blend4(tr, ts) : sample_scalars<ST>(g, S, rp);
+    const float4 og = ts[self];
+    auto mac = [&](float predicted, float recovered, float original) {
+      float corrected = predicted + 0.5f * (original - recovered);
+      return clampf(corrected, fminf(original, predicted), fmaxf(original, predicted));
+    };
+    float out[4];
+    out[0] = mac(pr.x, rc.x, og.x);
+    out[1] = mac(pr.y, rc.y, og.y);
+    out[2] = mac(pr.z, rc.z, og.z);
+    out[3] = mac(pr.w, rc.w, og.w);
+    decay_write<ST>(g, i, v.y, out[0], out[1], out[2], out[3], thermal_noise(coord, time), dt, D, sr, site);
   }
 }
 
@@ -1289,7 +1309,7 @@ __device__ __forceinline__ float thermal_cluster(F3 c, float time) {
 template <class ST, bool MS>
 __global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, Grid G,
                                                     const StepConst* __restrict__ C0, SS<ST> S, SS<ST> D,
-                                                    const uint32_t* __restrict__ flags, float4* __restrict__ sq) {
+                                                    const uint32_t* __restrict__ flags) {
   GridCache<MS> gk;
   FX_BRICK_LOOP(list, count) {
     const uint32_t b = list[fx_k];
@@ -1303,9 +1323,7 @@ __global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__
     const uint32_t i = v.i, x = v.x, y = v.y, z = v.z;
     if (flags[b] == 0u) {  // isActiveCoord: brick b's flag for its in-domain voxels
       // copied: the stored values are representable, so storing them is exact
-      const float cd = ST::ld(S.d, i), ct = ST::ld(S.t, i), cf = ST::ld(S.f, i), cr = S.r[i];
-      ST::st(D.d, i, cd); ST::st(D.t, i, ct); ST::st(D.f, i, cf); D.r[i] = cr;
-      if (sq) sq[i] = make_float4(cd, ct, cf, cr);
+      ST::st(D.d, i, ST::ld(S.d, i)); ST::st(D.t, i, ST::ld(S.t, i)); ST::st(D.f, i, ST::ld(S.f, i)); D.r[i] = S.r[i];
       continue;
     }
     auto hotgas = [&](uint32_t k) {
@@ -1331,13 +1349,10 @@ __global__ __launch_bounds__(512) void k_combustion(const uint32_t* __restrict__
     float coolingSmoke = den * tem * (coolingPocket + hotSoot * 0.32f) * dt * 0.16f;
     float reactionDecay = rea * (1.f - dt * mixf(0.68f, 0.22f, hotPocket));
     const uint32_t hi = i - (g.base << 9);
-    const typename ST::T ef = ST::enc_sr(clamp01(fu - burn), sr, sf, hi);
-    const typename ST::T et = ST::enc_sr(clamp01(tem + burn * mixf(2.1f, 3.25f, hotPocket) + radiative - heatLoss), sr, st, hi);
-    const typename ST::T ed = ST::enc_sr(clamp01(fmaxf(0.f, den + smokeYield + coolingSmoke - densityLoss - clearing * 0.72f)), sr, sd, hi);
-    const float er = clamp01(fmaxf(reactionDecay, burn * mixf(4.f, 7.5f, hotPocket) + radiative * 1.8f));
-    D.f[i] = ef; D.t[i] = et; D.d[i] = ed; D.r[i] = er;
-    // set 0's scalar gather mirror for the next step's scalar advection
-    if (sq) sq[i] = make_float4(ST::dec(ed), ST::dec(et), ST::dec(ef), er);
+    ST::st_sr(D.f, i, clamp01(fu - burn), sr, sf, hi);
+    ST::st_sr(D.t, i, clamp01(tem + burn * mixf(2.1f, 3.25f, hotPocket) + radiative - heatLoss), sr, st, hi);
+    ST::st_sr(D.d, i, clamp01(fmaxf(0.f, den + smokeYield + coolingSmoke - densityLoss - clearing * 0.72f)), sr, sd, hi);
+    D.r[i] = clamp01(fmaxf(reactionDecay, burn * mixf(4.f, 7.5f, hotPocket) + radiative * 1.8f));
   }
 }
 
@@ -3690,15 +3705,14 @@ __global__ void k_scene_counts(const uint32_t* __restrict__ list, const uint32_t
     else FX_LAUNCH(KF, blocks, threads, stream, __VA_ARGS__);   \
   } while (0)
 void launch_inject(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
-                   const float* src, Scalars set0, Vec3 u, uint32_t* touched, int parts, void* sq, uint32_t* clear,
+                   const float* src, Scalars set0, Vec3 u, uint32_t* touched, int parts, uint32_t* clear,
                    uint32_t clear_n) {
-  float4* S4 = static_cast<float4*>(sq);
   switch (set0.prec) {
-    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_inject<StBF16, true>), (k_inject<StBF16, false>), blocks, 512, s, list, count, g, c, src, typed<StBF16>(set0), u, touched, parts, S4, clear, clear_n); break;
-    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_inject<StF16, true>), (k_inject<StF16, false>), blocks, 512, s, list, count, g, c, src, typed<StF16>(set0), u, touched, parts, S4, clear, clear_n); break;
-    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_inject<StU16, true>), (k_inject<StU16, false>), blocks, 512, s, list, count, g, c, src, typed<StU16>(set0), u, touched, parts, S4, clear, clear_n); break;
-    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_inject<StUF16, true>), (k_inject<StUF16, false>), blocks, 512, s, list, count, g, c, src, typed<StUF16>(set0), u, touched, parts, S4, clear, clear_n); break;
-    default: FX_LAUNCH_MS(g, (k_inject<StF32, true>), (k_inject<StF32, false>), blocks, 512, s, list, count, g, c, src, typed<StF32>(set0), u, touched, parts, S4, clear, clear_n);
+    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_inject<StBF16, true>), (k_inject<StBF16, false>), blocks, 512, s, list, count, g, c, src, typed<StBF16>(set0), u, touched, parts, clear, clear_n); break;
+    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_inject<StF16, true>), (k_inject<StF16, false>), blocks, 512, s, list, count, g, c, src, typed<StF16>(set0), u, touched, parts, clear, clear_n); break;
+    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_inject<StU16, true>), (k_inject<StU16, false>), blocks, 512, s, list, count, g, c, src, typed<StU16>(set0), u, touched, parts, clear, clear_n); break;
+    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_inject<StUF16, true>), (k_inject<StUF16, false>), blocks, 512, s, list, count, g, c, src, typed<StUF16>(set0), u, touched, parts, clear, clear_n); break;
+    default: FX_LAUNCH_MS(g, (k_inject<StF32, true>), (k_inject<StF32, false>), blocks, 512, s, list, count, g, c, src, typed<StF32>(set0), u, touched, parts, clear, clear_n);
   }
 }
 void launch_clear_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t value) {
@@ -3744,42 +3758,27 @@ void launch_compact(hipStream_t s, const CompactArgs& A) {
   FX_LAUNCH(k_compact_write, blocks, 256, s, A);
 }
 void launch_advect_velocity(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
-                            Vec3 u, Vec3 us, void* uq) {
-  FX_LAUNCH_MS(g, (k_advect_velocity<true>), (k_advect_velocity<false>), blocks, 512, s, list, count, g, c, u, us, static_cast<float4*>(uq));
-}
-void launch_pack_gather(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Scalars set0, Vec3 us,
-                        const uint32_t* rlist, const uint32_t* rcount, void* sq, void* uq) {
-  float4* S4 = static_cast<float4*>(sq);
-  float4* U4 = static_cast<float4*>(uq);
-  switch (set0.prec) {
-    case FX_PRECISION_BF16: FX_LAUNCH(k_pack_gather<StBF16>, blocks, 512, s, list, count, typed<StBF16>(set0), us, rlist, rcount, S4, U4); break;
-    case FX_PRECISION_FP16: FX_LAUNCH(k_pack_gather<StF16>, blocks, 512, s, list, count, typed<StF16>(set0), us, rlist, rcount, S4, U4); break;
-    case FX_PRECISION_U16: FX_LAUNCH(k_pack_gather<StU16>, blocks, 512, s, list, count, typed<StU16>(set0), us, rlist, rcount, S4, U4); break;
-    case FX_PRECISION_UF16: FX_LAUNCH(k_pack_gather<StUF16>, blocks, 512, s, list, count, typed<StUF16>(set0), us, rlist, rcount, S4, U4); break;
-    default: FX_LAUNCH(k_pack_gather<StF32>, blocks, 512, s, list, count, typed<StF32>(set0), us, rlist, rcount, S4, U4);
-  }
+                            Vec3 u, Vec3 us) {
+  FX_LAUNCH_MS(g, (k_advect_velocity<true>), (k_advect_velocity<false>), blocks, 512, s, list, count, g, c, u, us);
 }
 void launch_advect_scalars(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
-                           int mode, const void* sq, const void* uq, Scalars dst, const uint32_t* flags) {
-  const float4* S4 = static_cast<const float4*>(sq);
-  const float4* U4 = static_cast<const float4*>(uq);
+                           int mode, Scalars set0, Vec3 us, Scalars dst, const uint32_t* flags) {
   switch (dst.prec) {
-    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_advect_scalars<StBF16, true>), (k_advect_scalars<StBF16, false>), blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StBF16>(dst), flags); break;
-    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_advect_scalars<StF16, true>), (k_advect_scalars<StF16, false>), blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StF16>(dst), flags); break;
-    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_advect_scalars<StU16, true>), (k_advect_scalars<StU16, false>), blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StU16>(dst), flags); break;
-    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_advect_scalars<StUF16, true>), (k_advect_scalars<StUF16, false>), blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StUF16>(dst), flags); break;
-    default: FX_LAUNCH_MS(g, (k_advect_scalars<StF32, true>), (k_advect_scalars<StF32, false>), blocks, 512, s, list, count, g, c, mode, S4, U4, typed<StF32>(dst), flags);
+    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_advect_scalars<StBF16, true>), (k_advect_scalars<StBF16, false>), blocks, 512, s, list, count, g, c, mode, typed<StBF16>(set0), us, typed<StBF16>(dst), flags); break;
+    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_advect_scalars<StF16, true>), (k_advect_scalars<StF16, false>), blocks, 512, s, list, count, g, c, mode, typed<StF16>(set0), us, typed<StF16>(dst), flags); break;
+    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_advect_scalars<StU16, true>), (k_advect_scalars<StU16, false>), blocks, 512, s, list, count, g, c, mode, typed<StU16>(set0), us, typed<StU16>(dst), flags); break;
+    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_advect_scalars<StUF16, true>), (k_advect_scalars<StUF16, false>), blocks, 512, s, list, count, g, c, mode, typed<StUF16>(set0), us, typed<StUF16>(dst), flags); break;
+    default: FX_LAUNCH_MS(g, (k_advect_scalars<StF32, true>), (k_advect_scalars<StF32, false>), blocks, 512, s, list, count, g, c, mode, typed<StF32>(set0), us, typed<StF32>(dst), flags);
   }
 }
 void launch_combustion(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
-                       Scalars src, Scalars dst, const uint32_t* flags, void* sq) {
-  float4* S4 = static_cast<float4*>(sq);
+                       Scalars src, Scalars dst, const uint32_t* flags) {
   switch (src.prec) {
-    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_combustion<StBF16, true>), (k_combustion<StBF16, false>), blocks, 512, s, list, count, g, c, typed<StBF16>(src), typed<StBF16>(dst), flags, S4); break;
-    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_combustion<StF16, true>), (k_combustion<StF16, false>), blocks, 512, s, list, count, g, c, typed<StF16>(src), typed<StF16>(dst), flags, S4); break;
-    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_combustion<StU16, true>), (k_combustion<StU16, false>), blocks, 512, s, list, count, g, c, typed<StU16>(src), typed<StU16>(dst), flags, S4); break;
-    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_combustion<StUF16, true>), (k_combustion<StUF16, false>), blocks, 512, s, list, count, g, c, typed<StUF16>(src), typed<StUF16>(dst), flags, S4); break;
-    default: FX_LAUNCH_MS(g, (k_combustion<StF32, true>), (k_combustion<StF32, false>), blocks, 512, s, list, count, g, c, typed<StF32>(src), typed<StF32>(dst), flags, S4);
+    case FX_PRECISION_BF16: FX_LAUNCH_MS(g, (k_combustion<StBF16, true>), (k_combustion<StBF16, false>), blocks, 512, s, list, count, g, c, typed<StBF16>(src), typed<StBF16>(dst), flags); break;
+    case FX_PRECISION_FP16: FX_LAUNCH_MS(g, (k_combustion<StF16, true>), (k_combustion<StF16, false>), blocks, 512, s, list, count, g, c, typed<StF16>(src), typed<StF16>(dst), flags); break;
+    case FX_PRECISION_U16: FX_LAUNCH_MS(g, (k_combustion<StU16, true>), (k_combustion<StU16, false>), blocks, 512, s, list, count, g, c, typed<StU16>(src), typed<StU16>(dst), flags); break;
+    case FX_PRECISION_UF16: FX_LAUNCH_MS(g, (k_combustion<StUF16, true>), (k_combustion<StUF16, false>), blocks, 512, s, list, count, g, c, typed<StUF16>(src), typed<StUF16>(dst), flags); break;
+    default: FX_LAUNCH_MS(g, (k_combustion<StF32, true>), (k_combustion<StF32, false>), blocks, 512, s, list, count, g, c, typed<StF32>(src), typed<StF32>(dst), flags);
   }
 }
 void launch_buoyancy(hipStream_t s, int blocks, const uint32_t* list, const uint32_t* count, Grid g, const StepConst* c,
