@@ -101,6 +101,30 @@ def test_semi_lagrangian_parity():
     compare_all(sim, orc, "semi-lagrangian")
 
 
+@pytest.mark.parametrize("mode", [abi.FX_ADVECT_MACCORMACK, abi.FX_ADVECT_SEMI_LAGRANGIAN])
+@pytest.mark.parametrize("dt", [0.5, 1.0])
+def test_advection_parity_with_long_backtraces(mode, dt):
+    """The advection kernels read trilinear corners from LDS tiles of the brick +- 1 voxel and
+    gather from the fields in memory where a sample's corners leave the tile. The plume's
+    backtraces stay below 0.2 voxels, so the other parity tests never take that path: here the
+    velocities (+-3 voxels per unit time) with dt 0.5 and 1 give backtraces of up to 1.5 and 3
+    voxels per axis, a mix of tile and memory samples (and clamped ones at the domain faces),
+    both advection passes against the oracle."""
+    desc = make_desc(GRID, advection_mode=mode, runtime={"worldSize": 24.0})
+    sim, orc = make_pair(desc, scenes.mixed_sources())
+    st = scenes.random_state(GRID[::-1], seed=13)
+    load_state(sim, orc, st)
+    # the share of voxels whose velocity backtrace leaves the tile along some axis
+    u = np.stack([st[f"velocity_{a}"] for a in "xyz"])
+    assert float(np.mean(np.any(np.abs(u) * dt >= 1.0, axis=0))) > 0.5
+    p = scenes.sim_params(time=0.4, dt=dt, dx=1.0, world=24.0)
+    _prepare_flags(sim, orc, p)
+    for name in ("advect_velocity", "advect_scalars"):
+        sim.run_pass(name, p)
+        orc.run_pass(name, p)
+        compare_all(sim, orc, f"{name}, dt {dt}")
+
+
 @pytest.mark.parametrize("solver,quality,grid,nlev", [
     (abi.FX_SOLVER_REFERENCE, {}, (32, 24, 16), 3),
     (abi.FX_SOLVER_REFERENCE, {"finePreIterations": 3, "midPreIterations": 1, "coarseIterations": 7, "finePostIterations": 0},
