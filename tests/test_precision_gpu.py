@@ -75,6 +75,28 @@ def test_pass_parity_16bit(prec, pass_name):
 
 
 @pytest.mark.parametrize("prec", SIXTEEN)
+def test_advect_scalars_16bit_long_backtraces(prec):
+    """Backtraces of up to 3 voxels (velocity +-3, dt 1): the scalar advection's samples mostly
+    leave its LDS tile (decoded 16-bit values) and decode their corners from memory instead;
+    both against the oracle with the same storage."""
+    grid = (24, 16, 20)
+    desc = make_desc(grid, precision=prec, runtime={"worldSize": 24.0})
+    sim = CombustionVolumeSimulation(desc)
+    orc = Oracle(desc, 32)
+    st = scenes.random_state(grid[::-1], seed=22)
+    for k, v in st.items():
+        sim.write_field(k, v)
+        orc.write(k, v)
+    p = scenes.sim_params(time=0.4, dt=1.0, dx=1.0, world=24.0)
+    for name in ("activate_render", "activate_advected", "advect_scalars"):
+        sim.run_pass(name, p)
+        orc.run_pass(name, p)
+    for name in abi.FIELDS:
+        e = rel_l2(sim.read_field(name), orc.read(name))
+        assert e <= TOL[prec], f"advect_scalars/{prec}: {name} rel L2 {e:.3e}"
+
+
+@pytest.mark.parametrize("prec", SIXTEEN)
 def test_step_parity_16bit_sparse(prec):
     n = 48
     desc = scenes.c2_desc(n, precision=prec)
