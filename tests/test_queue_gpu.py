@@ -94,6 +94,34 @@ def test_queue_batch_equals_scenes_alone(precision):
         sim.dispose()
 
 
+def test_queue_batch_long_steps_equal_scenes_alone():
+    """Steps of dt = 0.5 and 1 carry the plumes' backtraces past the advection kernels' LDS tiles
+    (the brick +- 1 voxel), so their samples gather from each scene's fields in memory through the
+    batch's per-scene grids: still bit for bit the scenes alone."""
+    spec = make_scenes(abi.FX_PRECISION_FP32)[:3]
+    batch = CombustionVolumeQueueBatch([d for d, _, _ in spec])
+    alone = [CombustionVolumeSimulation(d) for d, _, _ in spec]
+    vmax = 0.0
+    for sc, sim, (d, srcs, st) in zip(batch.scenes, alone, spec):
+        sc.updateSources(srcs)
+        sim.updateSources(srcs)
+        for f, v in st.items():
+            sc.write_field(f, v)
+            sim.write_field(f, v)
+        vmax = max(vmax, max(float(np.abs(st[f"velocity_{a}"]).max()) for a in "xyz"))
+    assert vmax * 0.5 >= 1.0, vmax  # the longest backtraces leave the tile
+    for k, dt in enumerate((0.5, 1.0)):
+        batch.step(k * 1.0, dt)
+        for sim in alone:
+            sim.step(None, k * 1.0, dt)
+        batch.synchronize()
+        for j, (sc, sim) in enumerate(zip(batch.scenes, alone)):
+            assert_same(sc, sim, f"scene {j} step {k} dt {dt}")
+    batch.dispose()
+    for sim in alone:
+        sim.dispose()
+
+
 def test_queue_batch_with_an_empty_scene_and_a_batch_of_one():
     """Edge cases of the queue: a scene with no emitters and no state (no active bricks at any
     level, an empty run in every list) beside a busy one, and a batch of a single scene; both
