@@ -259,3 +259,33 @@ def test_c4_sparse_512_16bit_storage(prec):
         blocks = np.abs(f).reshape(nb, 8, nb, 8, nb, 8).max(axis=(1, 3, 5)).reshape(-1)
         assert np.all(blocks[~flags] == 0), f"{name} is non-zero outside the active bricks"
     sim.dispose()
+
+
+def test_sparse_1024_domain_fits_and_steps():
+    """Capacity of the home-page layout: a sparse 1024^3 domain (2.1 M bricks) allocates
+    every brick's storage on one GPU (measured 110 GiB of 288) and steps from a cold start
+    with 4 floor emitters r = 20; the few active bricks keep the step short. Checks the
+    allocation against the layout's bytes per voxel, the active set and a bounded step time."""
+    import torch
+    n = 1024
+    torch.cuda.init()
+    free0, _ = torch.cuda.mem_get_info()
+    sim = CombustionVolumeSimulation(scenes.c2_desc(n))
+    free1, _ = torch.cuda.mem_get_info()
+    gib = (free0 - free1) / 2 ** 30
+    # ~108 B per fine voxel in FP32 storage plus the coarse levels: 108 GiB at 1024^3
+    assert 100.0 <= gib <= 125.0, gib
+    sim.updateSources(scenes.c2_sources(n, seed=4, count=4, radius_vox=20.0))
+    for k in range(20):
+        sim.step(None, k * DT, DT)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for k in range(20, 40):
+        sim.step(None, k * DT, DT)
+    sim.synchronize()
+    ms = (time.perf_counter() - t0) / 20 * 1e3
+    active = sim.active_brick_count()
+    print(f"1024^3: {gib:.1f} GiB, {active} active bricks after 40 steps, {ms:.2f} ms per step", flush=True)
+    assert 0 < active < (n // 8) ** 3 // 100
+    assert ms < 20.0, ms
+    sim.dispose()
